@@ -1,0 +1,210 @@
+/*
+ * khs.h — C ABI of the MI355X (gfx950) Kelvin–Helmholtz time-step library.
+ *
+ * The library advances the ymgpu2d two-fluid SU(2) Yang–Mills plasma (and the
+ * emhdgpu2d Abelian EMHD variant) on one GPU.  Everything behind this header is
+ * owned by an opaque context; no device pointer crosses the ABI.  Host arrays
+ * use the reference layout: index = x + z*nx, x contiguous (FCT_DeviceMath.cuh:12).
+ *
+ * Every entry point returns an int status (KHS_OK = 0, negative on error) and
+ * never exits the process; the reference's CUDA_CHECK -> exit(1) convention
+ * (ymgpu2d/YM_Common.cuh:8-11) and config exit(1) paths (YM_Config.cu:17-20,
+ * 120-127) become error codes with a message in khs_last_error().
+ * One context must be driven by one host thread at a time; separate contexts
+ * are independent.
+ */
+#ifndef KHS_H
+#define KHS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KHS_OK            0
+#define KHS_ERR_ARG      -1   /* bad argument (null pointer, size mismatch, unknown id) */
+#define KHS_ERR_CONFIG   -2   /* config file missing / unparsable / violates a constraint */
+#define KHS_ERR_IO       -3   /* file read or write failed */
+#define KHS_ERR_DEVICE   -4   /* HIP runtime error, or no GPU */
+#define KHS_ERR_STATE    -5   /* call out of order (e.g. z-sweep before x-sweep) */
+
+#define KHS_FP64 0            /* plain FP64 drop-in form */
+#define KHS_FP32 1            /* precision form: FP32 bulk, FP64 reductions */
+
+#define KHS_SPECIES_A 0       /* beam +z, Q = (+1,0,0) at init */
+#define KHS_SPECIES_B 1       /* beam -z, Q = (-1,0,0) at init */
+
+/* State array ids (khs_get_array / khs_set_array).  Names accepted by
+ * khs_array_id() are the CSV column names of main_ym.cu:291-292 plus the
+ * lower-case px/pz spellings. */
+enum khs_array {
+    KHS_EX1 = 0, KHS_EX2, KHS_EX3,
+    KHS_EZ1, KHS_EZ2, KHS_EZ3,
+    KHS_BY1, KHS_BY2, KHS_BY3,
+    KHS_AZ1, KHS_AZ2, KHS_AZ3,
+    KHS_NA, KHS_PXA, KHS_PZA, KHS_Q1A, KHS_Q2A, KHS_Q3A,
+    KHS_NB, KHS_PXB, KHS_PZB, KHS_Q1B, KHS_Q2B, KHS_Q3B,
+    KHS_N_STATE,                       /* = 24 evolved arrays */
+    /* derived, read-only: evaluated from the current state on request */
+    KHS_JX1 = 32, KHS_JX2, KHS_JX3,    /* kernel_ym_currents (YM_Physics.cu:9-60) */
+    KHS_JZ1, KHS_JZ2, KHS_JZ3,
+    KHS_UXA, KHS_UZA, KHS_UXB, KHS_UZB,                 /* kernel_update_velocities */
+    KHS_SRCPXA, KHS_SRCPZA, KHS_SRCPXB, KHS_SRCPZB,     /* kernel_ym_lorentz */
+    KHS_SQ1A, KHS_SQ2A, KHS_SQ3A, KHS_SQ1B, KHS_SQ2B, KHS_SQ3B, /* kernel_ym_precession */
+    KHS_ENERGY_DENSITY,                                 /* kernel_ym_energy */
+    KHS_N_ARRAY_IDS
+};
+
+/* Run configuration.  The first block holds the reference's RunConfig keys
+ * (ymgpu2d/YM_Config.cuh:6-50) with the same names, defaults and float storage
+ * (std::stof semantics, YM_Config.cu:99-123).  The second block holds keys the
+ * reference cannot express.  The third block is derived by khs_derive()
+ * exactly as main_ym.cu:51-83 and :244-249 derive it. */
+typedef struct khs_config {
+    /* --- reference keys (example.ini) --- */
+    int   k_mode;
+    float alpha_YM;
+    float perturb_amp;
+    int   run_mode;
+    float V0;
+    float xi_sponge;
+    float sigma_sponge;
+    float xi_cut;
+    float warm_T;
+    int   freeze_override;
+    int   suppress_kz0;
+    float hyp_diff;
+    int   kz_suppress_max;
+    float eps_override;
+    int   kz_suppress_hi;
+    float vz_edge_taper;
+    int   init_by1_eq;
+    int   nz_override;
+    int   nx_override;
+    float courant_override;
+    float lz_override;
+    float lx_override;
+    float target_tu;
+    char  run_tag[128];
+    char  seed_profile_file[512];
+    /* --- extension keys --- */
+    int      n_steps;          /* >0: exact step count, overrides target_tu */
+    double   noise_amp;        /* white noise U[-a,a] added to colour-2/3 E and By */
+    uint64_t noise_seed;       /* counter-based noise key */
+    int      kz_keep_max;      /* >0: keep only |kz| <= K in the 21 filtered arrays */
+    int      precision;        /* KHS_FP64 or KHS_FP32 */
+    int      energy_stride_override;
+    int      export_stride_override;  /* <0 disables snapshots */
+    /* --- derived (khs_derive) --- */
+    int    nx, nz;
+    int    periodic_x;         /* run_mode >= 1 */
+    int    freeze_az1;
+    double lx, lz, dx, dz, courant, dt;
+    double alpha, v0, eps;     /* double copies of the float keys */
+    double c, eps_0;           /* 1, 1 (main_ym.cu:69) */
+    int    total_steps, export_stride, energy_stride;
+    double energy_thresh;
+} khs_config;
+
+typedef struct khs_ctx khs_ctx;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* khs_last_error(void);
+/* Library version string. */
+const char* khs_version(void);
+/* sizeof(khs_config) as compiled into the library (binding layout check). */
+size_t khs_config_sizeof(void);
+
+/* ---------------- configuration (replaces YM_Config.cuh:55 load_config) ---------------- */
+int khs_config_defaults(khs_config* cfg);
+/* Parse one key/value pair with the reference's parser semantics. Unknown keys
+ * return KHS_ERR_CONFIG (the reference only warns; the ini loader below warns). */
+int khs_config_set(khs_config* cfg, const char* key, const char* value);
+/* Parse an .ini file (YM_Config.cu:15-73), then khs_derive(). */
+int khs_load_ini(const char* path, khs_config* cfg);
+/* Fill the derived block and run the constraint checks of YM_Config.cu:119-133. */
+int khs_derive(khs_config* cfg);
+/* Output directory name built from the parameters (main_ym.cu:143-170). */
+int khs_output_dir(const khs_config* cfg, char* buf, size_t buflen);
+
+/* ---------------- context ---------------- */
+/* Allocate device state for cfg (khs_derive must have run).  Replaces the
+ * allocation block of main_ym.cu:86-121. */
+int khs_create(const khs_config* cfg, khs_ctx** out);
+int khs_destroy(khs_ctx* ctx);
+/* Initial state (kernel_ym_init, YM_Init.cu:32-257, evaluated on the host in
+ * the same float arithmetic) plus the optional eigenmode seed file
+ * (main_ym.cu:178-217).  seed_path NULL uses cfg->seed_profile_file. */
+int khs_init(khs_ctx* ctx, const char* seed_path);
+
+/* ---------------- state I/O ---------------- */
+int khs_array_id(const char* name);
+int khs_get_array(khs_ctx* ctx, int id, double* host, size_t n);
+int khs_set_array(khs_ctx* ctx, int id, const double* host, size_t n);
+/* Read a state array from one of the step buffers: the committed state, the
+ * x-swept stage (fluid ids 12..23 only) or the next state being built by the
+ * staged operators below.  Lets a caller inspect the result of each operator,
+ * as the reference's device arrays can be read between kernel launches. */
+#define KHS_BUF_CURRENT 0
+#define KHS_BUF_STAGE   1
+#define KHS_BUF_NEXT    2
+int khs_get_array_buf(khs_ctx* ctx, int buf, int id, double* host, size_t n);
+
+/* ---------------- operators (reference call boundaries) ----------------
+ * A step is staged: x-sweeps read the committed state and write a stage
+ * buffer; z-sweeps read the stage and write the next state; khs_maxwell reads
+ * the next fluid and committed fields and writes the next fields; khs_filters
+ * acts on the next state; khs_commit makes it current.  Calling them in the
+ * order of main_ym.cu:322-455 reproduces khs_step.  Velocities and Lorentz /
+ * precession sources are evaluated on the fly from the committed state, which
+ * is the value the reference holds in its d_src / d_sQ arrays at that point
+ * (they are computed from the end-of-step state, main_ym.cu:450-455). */
+int khs_fct_x_sweep(khs_ctx* ctx, int species);   /* ym_fct_x_sweep[_periodic], YM_FCT.cu:30-91 */
+int khs_fct_z_sweep(khs_ctx* ctx, int species);   /* ym_fct_z_sweep, YM_FCT.cu:93-135; + enforce_boundaries on the fluid */
+int khs_maxwell(khs_ctx* ctx);                    /* enforce_boundaries(Ez) + currents + Ampere + potential + Faraday */
+int khs_filters(khs_ctx* ctx);                    /* sponge, xi_cut, z-mean, kz bands, hyperdiffusion, colour-1 zeroing */
+int khs_commit(khs_ctx* ctx);
+
+/* Fused step: nsteps full steps.  Energy is checked every energy_stride steps
+ * (counted from the context's step counter) with the reference's halt rule
+ * (main_ym.cu:457-475); *halted (may be NULL) is set to 1 and stepping stops
+ * on NaN/Inf or E/E0 > energy_thresh. */
+int khs_step(khs_ctx* ctx, int nsteps, int* halted);
+/* Current step counter and simulation time. */
+int khs_get_step(khs_ctx* ctx, int* step, double* time);
+
+/* ---------------- diagnostics ---------------- */
+/* Total energy (YM_Energy.cu:3-32) as a compensated FP64 device reduction. */
+int khs_energy(khs_ctx* ctx, double* E);
+/* Initial energy E0 recorded by khs_init. */
+int khs_energy0(khs_ctx* ctx, double* E0);
+/* Gauss-law residual (analysis/gauss_check.py:40-115 formula, computed on the
+ * device in FP64).  out[0..2] = rms G^a with rho = +nQ (reference sign),
+ * out[3..5] = rms G^a with rho = -nQ (continuity-consistent sign),
+ * out[6] = Etot_scale = rms(sum|E|)/dx. */
+int khs_gauss(khs_ctx* ctx, double out[7]);
+
+/* ---------------- output files (byte-compatible with main_ym.cu) ---------------- */
+int khs_write_snapshot(khs_ctx* ctx, const char* dir, int step);      /* ym_%06d.csv, :287-306 */
+int khs_energy_csv_open(khs_ctx* ctx, const char* path);              /* header + "0,0,E0,1.0", :252-255 */
+int khs_energy_csv_row(khs_ctx* ctx, int step, double E);            /* step,time,E,E/E0, :462 */
+int khs_energy_csv_close(khs_ctx* ctx);
+
+/* ---------------- timing ---------------- */
+int khs_sync(khs_ctx* ctx);
+/* Enable per-kernel HIP-event timing on the context's stream.  While enabled,
+ * every launch of the step kernels is bracketed by events; khs_kernel_times
+ * returns, per kernel slot, the summed milliseconds and the launch count since
+ * the last reset.  Slot names come from khs_kernel_name(). */
+int khs_timing_enable(khs_ctx* ctx, int on);
+int khs_timing_reset(khs_ctx* ctx);
+int khs_kernel_times(khs_ctx* ctx, double* ms, int* launches, int nslots);
+const char* khs_kernel_name(int slot);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KHS_H */

@@ -1,0 +1,388 @@
+// khs_aux_kernels.hip — filters, diagnostics and derived fields on gfx950.
+//
+//   k_cell_filters  sponge damping / xi_cut zeroing of the colour-2/3 set
+//                   (YM_Physics.cu:161-210), one pass, per-column action table.
+//   k_col_coef      DFT coefficients of whole z columns: one lane per x column
+//   k_col_apply     (coalesced), modes in registers; then subtraction (reference
+//                   band filter, YM_Physics.cu:349-584; z-mean removal :272-327)
+//                   or projection (keep |kz| <= K extension).
+//   k_hyperdiff     4th-order z hyperdiffusion (YM_Physics.cu:228-259) as an
+//                   in-place column march with a register window of the old
+//                   values, i.e. Jacobi semantics without a scratch copy.
+//   k_energy_*      FP64 compensated (Neumaier) energy reduction: per-lane sums,
+//   k_gauss_*       LDS tree of (sum, error) pairs merged with TwoSum, fixed
+//                   launch shape so the result is deterministic.
+//   k_derived       velocities / sources / currents / energy density of a state.
+#include "khs_device.h"
+#include "khs_launch.h"
+
+namespace khs {
+
+// --------------------------------------------------------------------------
+// sponge / xi_cut
+// --------------------------------------------------------------------------
+template <typename T>
+__global__ void k_cell_filters(ArrayList<T> arrs, const double* decay, const int* action,
+                               KParams p) {
+    const size_t N = (size_t)p.nx * p.nz;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int x = (int)(i % p.nx);
+        const int act = action[x];
+        if (act == 0) continue;
+        if (act == 1) {
+            const T d = T(decay[x]);
+            for (int k = 0; k < arrs.n; ++k) arrs.a[k][i] *= d;
+        } else {
+            for (int k = 0; k < arrs.n; ++k) arrs.a[k][i] = T(0.0);
+        }
+    }
+}
+
+template <typename T>
+void launch_cell_filters(const ArrayList<T>& arrs, const double* decay, const int* action,
+                         const KParams& p, hipStream_t st) {
+    size_t N = (size_t)p.nx * p.nz;
+    int blocks = (int)std::min<size_t>((N + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_cell_filters<T>, dim3(blocks), dim3(256), 0, st, arrs, decay, action, p);
+}
+
+// --------------------------------------------------------------------------
+// column DFT filter
+// --------------------------------------------------------------------------
+constexpr int CM = 16;   // modes per coefficient pass
+constexpr int CZ = 64;   // z rows per apply block
+
+template <typename T, typename TW>
+__global__ __launch_bounds__(64) void k_col_coef(ArrayList<T> arrs, const TW* ctab, const TW* stab,
+                                                 const TW* weight, int nmodes, double* coef, KParams p) {
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int k = blockIdx.y;
+    const int m0 = blockIdx.z * CM;
+    if (x >= p.nx) return;
+    const int mc = min(CM, nmodes - m0);
+    const T* v = arrs.a[k];
+    double sc[CM], ss[CM];
+#pragma unroll
+    for (int j = 0; j < CM; ++j) { sc[j] = 0.0; ss[j] = 0.0; }
+    for (int z = 0; z < p.nz; ++z) {
+        const double val = (double)v[(size_t)x + (size_t)z * p.nx];
+#pragma unroll
+        for (int j = 0; j < CM; ++j) {
+            if (j < mc) {
+                sc[j] += val * (double)ctab[(size_t)(m0 + j) * p.nz + z];
+                ss[j] += val * (double)stab[(size_t)(m0 + j) * p.nz + z];
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < CM; ++j) {
+        if (j < mc) {
+            const size_t base = ((size_t)k * nmodes + (m0 + j)) * 2;
+            coef[(base + 0) * p.nx + x] = sc[j] * (double)weight[m0 + j];
+            coef[(base + 1) * p.nx + x] = ss[j] * (double)weight[m0 + j];
+        }
+    }
+}
+
+template <typename T, typename TW>
+__global__ __launch_bounds__(64) void k_col_apply(ArrayList<T> arrs, const TW* ctab, const TW* stab,
+                                                  int nmodes, int project, const double* coef, KParams p) {
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int k = blockIdx.y;
+    const int zb = blockIdx.z * CZ;
+    if (x >= p.nx) return;
+    T* v = arrs.a[k];
+    const int ze = min(zb + CZ, p.nz);
+    for (int z = zb; z < ze; ++z) {
+        double delta = 0.0;
+        for (int m = 0; m < nmodes; ++m) {
+            const size_t base = ((size_t)k * nmodes + m) * 2;
+            const double A = coef[(base + 0) * p.nx + x];
+            const double B = coef[(base + 1) * p.nx + x];
+            delta += A * (double)ctab[(size_t)m * p.nz + z] + B * (double)stab[(size_t)m * p.nz + z];
+        }
+        const size_t i = (size_t)x + (size_t)z * p.nx;
+        if (project) v[i] = (T)delta;
+        else v[i] = (T)((double)v[i] - delta);
+    }
+}
+
+template <typename T, typename TW>
+void launch_column_modes(const ArrayList<T>& arrs, const TW* ctab, const TW* stab,
+                         const TW* weight, int nmodes, int project, double* coef,
+                         const KParams& p, hipStream_t st) {
+    if (arrs.n == 0 || nmodes == 0) return;
+    dim3 g1((p.nx + 63) / 64, arrs.n, (nmodes + CM - 1) / CM);
+    hipLaunchKernelGGL((k_col_coef<T, TW>), g1, dim3(64), 0, st, arrs, ctab, stab, weight, nmodes, coef, p);
+    dim3 g2((p.nx + 63) / 64, arrs.n, (p.nz + CZ - 1) / CZ);
+    hipLaunchKernelGGL((k_col_apply<T, TW>), g2, dim3(64), 0, st, arrs, ctab, stab, nmodes, project, coef, p);
+}
+
+// --------------------------------------------------------------------------
+// hyperdiffusion (requires nz >= 5)
+// --------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(64) void k_hyperdiff(ArrayList<T> arrs, double mu_d, KParams p) {
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int k = blockIdx.y;
+    if (x >= p.nx) return;
+    T* v = arrs.a[k];
+    const int nz = p.nz;
+    const size_t nx = p.nx;
+    const T mu = T(mu_d);
+    auto at = [&](int z) { return v[(size_t)x + (size_t)z * nx]; };
+    const T o0 = at(0), o1 = at(1), o2 = at(2);
+    T zm2 = at(nz - 2), zm1 = at(nz - 1), c = o0, zp1 = o1, zp2 = o2;
+    for (int z = 0; z < nz; ++z) {
+        T d4 = zm2 - T(4.0) * zm1 + T(6.0) * c - T(4.0) * zp1 + zp2;
+        T nw = c - mu * d4;
+        // next old value (z+3), read before any write reaches it
+        const int z3 = z + 3;
+        T nxt = (z3 < nz) ? at(z3) : (z3 - nz == 0 ? o0 : (z3 - nz == 1 ? o1 : o2));
+        v[(size_t)x + (size_t)z * nx] = nw;
+        zm2 = zm1; zm1 = c; c = zp1; zp1 = zp2; zp2 = nxt;
+    }
+}
+
+template <typename T>
+void launch_hyperdiffuse(const ArrayList<T>& arrs, double mu, const KParams& p, hipStream_t st) {
+    if (arrs.n == 0) return;
+    dim3 g((p.nx + 63) / 64, arrs.n);
+    hipLaunchKernelGGL(k_hyperdiff<T>, g, dim3(64), 0, st, arrs, mu, p);
+}
+
+template <typename T>
+__global__ void k_fill_zero(ArrayList<T> arrs, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        for (int k = 0; k < arrs.n; ++k) arrs.a[k][i] = T(0.0);
+}
+
+template <typename T>
+void launch_fill_zero(const ArrayList<T>& arrs, size_t n, hipStream_t st) {
+    int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill_zero<T>, dim3(blocks), dim3(256), 0, st, arrs, n);
+}
+
+// --------------------------------------------------------------------------
+// compensated reductions
+// --------------------------------------------------------------------------
+struct CSum {
+    double s, c;
+};
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+    s = a + b;
+    double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+__device__ __forceinline__ void cadd(CSum& acc, double v) {   // Neumaier
+    double s, e;
+    two_sum(acc.s, v, s, e);
+    acc.s = s;
+    acc.c += e;
+}
+__device__ __forceinline__ CSum cmerge(CSum a, CSum b) {
+    double s, e;
+    two_sum(a.s, b.s, s, e);
+    return CSum{s, a.c + b.c + e};
+}
+
+template <int K>
+__device__ void block_reduce_store(CSum (&acc)[K], double* partials) {
+    __shared__ double ss[K][kRedThreads], sc[K][kRedThreads];
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < K; ++k) { ss[k][t] = acc[k].s; sc[k][t] = acc[k].c; }
+    __syncthreads();
+    for (int off = kRedThreads / 2; off > 0; off >>= 1) {
+        if (t < off) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                CSum r = cmerge(CSum{ss[k][t], sc[k][t]}, CSum{ss[k][t + off], sc[k][t + off]});
+                ss[k][t] = r.s; sc[k][t] = r.c;
+            }
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            partials[((size_t)blockIdx.x * K + k) * 2 + 0] = ss[k][0];
+            partials[((size_t)blockIdx.x * K + k) * 2 + 1] = sc[k][0];
+        }
+    }
+}
+
+template <int K>
+__global__ __launch_bounds__(kRedThreads) void k_reduce_final(const double* partials, int nblocks, double* out) {
+    CSum acc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] = CSum{0.0, 0.0};
+    for (int b = threadIdx.x; b < nblocks; b += kRedThreads)
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            acc[k] = cmerge(acc[k], CSum{partials[((size_t)b * K + k) * 2], partials[((size_t)b * K + k) * 2 + 1]});
+    __shared__ double ss[K][kRedThreads], sc[K][kRedThreads];
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < K; ++k) { ss[k][t] = acc[k].s; sc[k][t] = acc[k].c; }
+    __syncthreads();
+    for (int off = kRedThreads / 2; off > 0; off >>= 1) {
+        if (t < off) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                CSum r = cmerge(CSum{ss[k][t], sc[k][t]}, CSum{ss[k][t + off], sc[k][t + off]});
+                ss[k][t] = r.s; sc[k][t] = r.c;
+            }
+        }
+        __syncthreads();
+    }
+    if (t == 0)
+#pragma unroll
+        for (int k = 0; k < K; ++k) out[k] = ss[k][0] + sc[k][0];
+}
+
+// per-cell energy density, FP64 (YM_Energy.cu:3-21)
+template <typename T>
+__device__ __forceinline__ double energy_cell(const State<T>& S, size_t i) {
+    double nA = (double)S.a[S_NA][i], nB = (double)S.a[S_NB][i];
+    nA = kmax(nA, 1e-10); nB = kmax(nB, 1e-10);
+    double pxA = S.a[S_PXA][i], pzA = S.a[S_PZA][i], pxB = S.a[S_PXB][i], pzB = S.a[S_PZB][i];
+    double keA = 0.5 * (pxA * pxA + pzA * pzA) / nA;
+    double keB = 0.5 * (pxB * pxB + pzB * pzB) / nB;
+    const int ids[9] = {S_EX1, S_EX2, S_EX3, S_EZ1, S_EZ2, S_EZ3, S_BY1, S_BY2, S_BY3};
+    double e0 = S.a[ids[0]][i];
+    double acc = e0 * e0;
+#pragma unroll
+    for (int k = 1; k < 9; ++k) { double f = S.a[ids[k]][i]; acc = acc + f * f; }
+    double em = 0.5 * acc;
+    return keA + keB + em;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void k_energy(State<T> S, KParams p, double* partials) {
+    const size_t N = (size_t)p.nx * p.nz;
+    CSum acc[1] = {CSum{0.0, 0.0}};
+    for (size_t i = blockIdx.x * (size_t)kRedThreads + threadIdx.x; i < N; i += (size_t)gridDim.x * kRedThreads)
+        cadd(acc[0], energy_cell(S, i));
+    block_reduce_store<1>(acc, partials);
+}
+
+template <typename T>
+void launch_energy(const State<T>& S, const KParams& p, double* partials, double* out, hipStream_t st) {
+    size_t N = (size_t)p.nx * p.nz;
+    int blocks = (int)std::min<size_t>((N + kRedThreads - 1) / kRedThreads, kRedBlocks);
+    hipLaunchKernelGGL(k_energy<T>, dim3(blocks), dim3(kRedThreads), 0, st, S, p, partials);
+    hipLaunchKernelGGL(k_reduce_final<1>, dim3(1), dim3(kRedThreads), 0, st, partials, blocks, out);
+}
+
+// Gauss residual, analysis/gauss_check.py:40-115, periodic differences in x and z.
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void k_gauss(State<T> S, KParams p, double* partials) {
+    const int nx = p.nx, nz = p.nz;
+    const size_t N = (size_t)nx * nz;
+    CSum acc[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) acc[k] = CSum{0.0, 0.0};
+    const double DX = p.dx, DZ = p.dz, al = p.alpha;
+    for (size_t i = blockIdx.x * (size_t)kRedThreads + threadIdx.x; i < N; i += (size_t)gridDim.x * kRedThreads) {
+        const int x = (int)(i % nx), z = (int)(i / nx);
+        const size_t ixm = (size_t)((x + nx - 1) % nx) + (size_t)z * nx;
+        const size_t izm = (size_t)x + (size_t)((z + nz - 1) % nz) * nx;
+        double dE[3], Ezc[3], Az[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const T* Ex = S.a[S_EX1 + c];
+            const T* Ez = S.a[S_EZ1 + c];
+            dE[c] = ((double)Ex[i] - (double)Ex[ixm]) / DX + ((double)Ez[i] - (double)Ez[izm]) / DZ;
+            Ezc[c] = 0.5 * ((double)Ez[i] + (double)Ez[izm]);
+            Az[c] = S.a[S_AZ1 + c][i];
+        }
+        double nab[3] = {al * (Az[1] * Ezc[2] - Az[2] * Ezc[1]),
+                         al * (Az[2] * Ezc[0] - Az[0] * Ezc[2]),
+                         al * (Az[0] * Ezc[1] - Az[1] * Ezc[0])};
+        double nA = S.a[S_NA][i], nB = S.a[S_NB][i];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double rho = (double)S.a[S_Q1A + c][i] * nA + (double)S.a[S_Q1B + c][i] * nB;
+            double Gp = dE[c] + nab[c] - rho;
+            double Gm = dE[c] + nab[c] + rho;
+            cadd(acc[c], Gp * Gp);
+            cadd(acc[3 + c], Gm * Gm);
+        }
+        double ae = fabs((double)S.a[S_EX1][i]);
+        ae = ae + fabs((double)S.a[S_EX2][i]);
+        ae = ae + fabs((double)S.a[S_EX3][i]);
+        ae = ae + fabs((double)S.a[S_EZ1][i]);
+        ae = ae + fabs((double)S.a[S_EZ2][i]);
+        ae = ae + fabs((double)S.a[S_EZ3][i]);
+        cadd(acc[6], ae * ae);
+    }
+    block_reduce_store<7>(acc, partials);
+}
+
+template <typename T>
+void launch_gauss(const State<T>& S, const KParams& p, double* partials, double* out, hipStream_t st) {
+    size_t N = (size_t)p.nx * p.nz;
+    int blocks = (int)std::min<size_t>((N + kRedThreads - 1) / kRedThreads, kRedBlocks);
+    hipLaunchKernelGGL(k_gauss<T>, dim3(blocks), dim3(kRedThreads), 0, st, S, p, partials);
+    hipLaunchKernelGGL(k_reduce_final<7>, dim3(1), dim3(kRedThreads), 0, st, partials, blocks, out);
+}
+
+// --------------------------------------------------------------------------
+// derived arrays
+// --------------------------------------------------------------------------
+template <typename T>
+__global__ void k_derived(State<T> S, int id, KParams p, double* out) {
+    const size_t N = (size_t)p.nx * p.nz;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int x = (int)(i % p.nx), z = (int)(i / p.nx);
+        double r = 0.0;
+        if (id >= 32 && id <= 37) {                       // Jx1..3, Jz1..3
+            T Jx[3], Jz[3];
+            currents_at(S, x, z, p, Jx, Jz);
+            r = (id < 35) ? (double)Jx[id - 32] : (double)Jz[id - 35];
+        } else if (id >= 38 && id <= 41) {                // uxA, uzA, uxB, uzB
+            const int sp = (id - 38) / 2, comp = (id - 38) % 2;
+            const int b = S_NA + sp * F_COUNT;
+            T n_safe = kmax(S.a[b + F_N][i], T(0.05));
+            r = (double)(S.a[b + (comp ? F_PZ : F_PX)][i] / n_safe);
+        } else if (id >= 42 && id <= 45) {                // src px/pz of A, B
+            const int sp = (id - 42) / 2, comp = (id - 42) % 2;
+            r = comp ? (double)lorentz_pz(S, sp, x, z, p) : (double)lorentz_px(S, sp, x, z, p);
+        } else if (id >= 46 && id <= 51) {                // sQ1..3 of A, B
+            const int sp = (id - 46) / 3, c = (id - 46) % 3;
+            T q[3];
+            precession_at(S, sp, x, z, p, q);
+            r = (double)q[c];
+        } else if (id == 52) {
+            r = energy_cell(S, i);
+        }
+        out[i] = r;
+    }
+}
+
+template <typename T>
+void launch_derived(const State<T>& S, int id, const KParams& p, double* out, hipStream_t st) {
+    size_t N = (size_t)p.nx * p.nz;
+    int blocks = (int)std::min<size_t>((N + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_derived<T>, dim3(blocks), dim3(256), 0, st, S, id, p, out);
+}
+
+// explicit instantiations
+#define KHS_INST(T)                                                                                   \
+    template void launch_cell_filters<T>(const ArrayList<T>&, const double*, const int*, const KParams&, hipStream_t); \
+    template void launch_column_modes<T, float>(const ArrayList<T>&, const float*, const float*, const float*, int, int, double*, const KParams&, hipStream_t); \
+    template void launch_column_modes<T, double>(const ArrayList<T>&, const double*, const double*, const double*, int, int, double*, const KParams&, hipStream_t); \
+    template void launch_hyperdiffuse<T>(const ArrayList<T>&, double, const KParams&, hipStream_t); \
+    template void launch_fill_zero<T>(const ArrayList<T>&, size_t, hipStream_t);                    \
+    template void launch_energy<T>(const State<T>&, const KParams&, double*, double*, hipStream_t);  \
+    template void launch_gauss<T>(const State<T>&, const KParams&, double*, double*, hipStream_t);   \
+    template void launch_derived<T>(const State<T>&, int, const KParams&, double*, hipStream_t);
+KHS_INST(double)
+KHS_INST(float)
+#undef KHS_INST
+
+}  // namespace khs

@@ -1,0 +1,226 @@
+// khs_config.cpp — .ini parser and derived run parameters.
+//
+// Keys, defaults and parse semantics follow ymgpu2d/YM_Config.cuh:6-50 and
+// YM_Config.cu:15-136 (numbers through std::stoi / std::stof into int / float
+// fields, inline '#' comments stripped, unknown keys warned).  The derived grid
+// and cadence follow main_ym.cu:51-83 and :244-249, evaluated in double from
+// the float fields exactly as the reference does (so DT at courant 0.4,
+// dx 0.1 is 0.0400000011920929 and target_tu = 80 gives 1999 steps).
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <sstream>
+
+#include "khs_host.h"
+
+namespace khs {
+
+static std::string trim(const std::string& s) {
+    size_t a = s.find_first_not_of(" \t\r\n");
+    size_t b = s.find_last_not_of(" \t\r\n");
+    return (a == std::string::npos) ? "" : s.substr(a, b - a + 1);
+}
+
+void config_defaults(khs_config& c) {
+    std::memset(&c, 0, sizeof(c));
+    c.k_mode = 1;
+    c.alpha_YM = 0.5f;
+    c.perturb_amp = 0.001f;
+    c.run_mode = 6;
+    c.V0 = 0.1f;
+    c.xi_sponge = 0.0f;
+    c.sigma_sponge = 5.0f;
+    c.xi_cut = 0.0f;
+    c.warm_T = 0.0f;
+    c.freeze_override = -1;
+    c.suppress_kz0 = 0;
+    c.hyp_diff = 0.0f;
+    c.kz_suppress_max = 0;
+    c.eps_override = -1.0f;
+    c.kz_suppress_hi = 0;
+    c.vz_edge_taper = 0.0f;
+    c.init_by1_eq = 0;
+    c.nz_override = -1;
+    c.nx_override = -1;
+    c.courant_override = -1.0f;
+    c.lz_override = -1.0f;
+    c.lx_override = -1.0f;
+    c.target_tu = -1.0f;
+    c.run_tag[0] = '\0';
+    c.seed_profile_file[0] = '\0';
+    c.n_steps = 0;
+    c.noise_amp = 0.0;
+    c.noise_seed = 0;
+    c.kz_keep_max = 0;
+    c.precision = KHS_FP64;
+    c.energy_stride_override = 0;
+    c.export_stride_override = 0;
+}
+
+static int parse_int(const std::string& k, const std::string& v) {
+    try { return std::stoi(v); } catch (...) {
+        throw Error(KHS_ERR_CONFIG, "bad integer for key '" + k + "': '" + v + "'");
+    }
+}
+static float parse_float(const std::string& k, const std::string& v) {
+    try { return std::stof(v); } catch (...) {
+        throw Error(KHS_ERR_CONFIG, "bad number for key '" + k + "': '" + v + "'");
+    }
+}
+static double parse_double(const std::string& k, const std::string& v) {
+    try { return std::stod(v); } catch (...) {
+        throw Error(KHS_ERR_CONFIG, "bad number for key '" + k + "': '" + v + "'");
+    }
+}
+static void set_str(char* dst, size_t cap, const std::string& k, const std::string& v) {
+    if (v.size() + 1 > cap) throw Error(KHS_ERR_CONFIG, "value too long for key '" + k + "'");
+    std::memcpy(dst, v.c_str(), v.size() + 1);
+}
+
+bool config_set(khs_config& c, const std::string& key, const std::string& val) {
+    // reference keys (YM_Config.cu:99-123)
+    if      (key == "k_mode")            c.k_mode = parse_int(key, val);
+    else if (key == "alpha_YM")          c.alpha_YM = parse_float(key, val);
+    else if (key == "perturb_amp")       c.perturb_amp = parse_float(key, val);
+    else if (key == "run_mode")          c.run_mode = parse_int(key, val);
+    else if (key == "V0")                c.V0 = parse_float(key, val);
+    else if (key == "xi_sponge")         c.xi_sponge = parse_float(key, val);
+    else if (key == "sigma_sponge")      c.sigma_sponge = parse_float(key, val);
+    else if (key == "xi_cut")            c.xi_cut = parse_float(key, val);
+    else if (key == "warm_T")            c.warm_T = parse_float(key, val);
+    else if (key == "freeze_override")   c.freeze_override = parse_int(key, val);
+    else if (key == "suppress_kz0")      c.suppress_kz0 = parse_int(key, val);
+    else if (key == "hyp_diff")          c.hyp_diff = parse_float(key, val);
+    else if (key == "kz_suppress_max")   c.kz_suppress_max = parse_int(key, val);
+    else if (key == "eps_override")      c.eps_override = parse_float(key, val);
+    else if (key == "kz_suppress_hi")    c.kz_suppress_hi = parse_int(key, val);
+    else if (key == "init_by1_eq")       c.init_by1_eq = parse_int(key, val);
+    else if (key == "vz_edge_taper")     c.vz_edge_taper = parse_float(key, val);
+    else if (key == "nz_override")       c.nz_override = parse_int(key, val);
+    else if (key == "nx_override")       c.nx_override = parse_int(key, val);
+    else if (key == "courant_override")  c.courant_override = parse_float(key, val);
+    else if (key == "lz_override")       c.lz_override = parse_float(key, val);
+    else if (key == "lx_override")       c.lx_override = parse_float(key, val);
+    else if (key == "target_tu")         c.target_tu = parse_float(key, val);
+    else if (key == "run_tag")           set_str(c.run_tag, sizeof(c.run_tag), key, val);
+    else if (key == "seed_profile_file") set_str(c.seed_profile_file, sizeof(c.seed_profile_file), key, val);
+    // extension keys
+    else if (key == "n_steps")           c.n_steps = parse_int(key, val);
+    else if (key == "noise_amp")         c.noise_amp = parse_double(key, val);
+    else if (key == "noise_seed")        c.noise_seed = (uint64_t)std::stoull(val);
+    else if (key == "kz_keep_max")       c.kz_keep_max = parse_int(key, val);
+    else if (key == "precision") {
+        if (val == "fp64") c.precision = KHS_FP64;
+        else if (val == "fp32") c.precision = KHS_FP32;
+        else throw Error(KHS_ERR_CONFIG, "precision must be fp64 or fp32, got '" + val + "'");
+    }
+    else if (key == "energy_stride")     c.energy_stride_override = parse_int(key, val);
+    else if (key == "export_stride")     c.export_stride_override = parse_int(key, val);
+    else return false;
+    return true;
+}
+
+void config_load(const std::string& path, khs_config& c, std::string* warnings) {
+    std::ifstream f(path);
+    if (!f.is_open()) throw Error(KHS_ERR_CONFIG, "cannot open config file '" + path + "'");
+    std::string line;
+    int lineno = 0;
+    std::ostringstream warn;
+    while (std::getline(f, line)) {
+        ++lineno;
+        line = trim(line);
+        if (line.empty() || line[0] == '#') continue;
+        auto eq = line.find('=');
+        if (eq == std::string::npos) {
+            warn << "WARNING: " << path << ":" << lineno << ": no '=' found, skipping: " << line << "\n";
+            continue;
+        }
+        std::string key = trim(line.substr(0, eq));
+        std::string val = trim(line.substr(eq + 1));
+        auto hash = val.find('#');
+        if (hash != std::string::npos) val = trim(val.substr(0, hash));
+        if (!config_set(c, key, val))
+            warn << "WARNING: " << path << ":" << lineno << ": unknown key '" << key << "'\n";
+    }
+    if (warnings) *warnings = warn.str();
+    config_derive(c);
+}
+
+void config_derive(khs_config& c) {
+    c.nz = (c.nz_override > 0) ? c.nz_override : 64;
+    c.nx = (c.nx_override > 0) ? c.nx_override : 768;
+    c.lx = (c.lx_override > 0.f) ? (double)c.lx_override : (double)(6.0 * M_PI);
+    c.lz = (c.lz_override > 0.f) ? (double)c.lz_override : (double)(2.0 * M_PI);
+    c.dx = c.lx / c.nx;
+    c.dz = c.lz / c.nz;
+    c.courant = (c.courant_override > 0.f) ? (double)c.courant_override : (double)0.1f;
+    c.dt = c.courant * c.dx;
+    c.v0 = (double)c.V0;
+    c.alpha = (double)c.alpha_YM;
+    c.eps = (c.eps_override > 0.f) ? (double)c.eps_override : c.lx / 6.0;
+    c.c = 1.0;
+    c.eps_0 = 1.0;
+    const int m = c.run_mode;
+    c.freeze_az1 = (m == 1 || m == 3 || m == 4 || m == 5 || m == 6) ? 1 : 0;
+    if (c.freeze_override >= 0) c.freeze_az1 = c.freeze_override;
+    c.periodic_x = (m >= 1) ? 1 : 0;
+    // main_ym.cu:244-249
+    c.total_steps = (c.target_tu > 0.0f) ? (int)((double)c.target_tu / c.dt) : 2000000;
+    c.export_stride = (c.target_tu > 0.0f) ? std::max(1, (int)(1.0 / c.dt)) : 20000;
+    c.energy_stride = (c.target_tu > 0.0f) ? std::max(1, (int)(0.5 / c.dt)) : 10000;
+    c.energy_thresh = (m >= 3 || c.xi_sponge > 0.0) ? 100.0 : 5.0;
+    if (c.n_steps > 0) c.total_steps = c.n_steps;
+    if (c.energy_stride_override > 0) c.energy_stride = c.energy_stride_override;
+    if (c.export_stride_override > 0) c.export_stride = c.export_stride_override;
+    if (c.export_stride_override < 0) c.export_stride = 0;
+
+    // constraint checks (YM_Config.cu:119-133) plus the build's own limits
+    const int NZ = c.nz, NX = c.nx;
+    if (NX < 4 || NZ < 4)
+        throw Error(KHS_ERR_CONFIG, "grid too small (NX=" + std::to_string(NX) + ", NZ=" + std::to_string(NZ) + ")");
+    if (c.suppress_kz0 && (NZ & (NZ - 1)) != 0)
+        throw Error(KHS_ERR_CONFIG, "suppress_kz0=1 requires NZ to be a power of 2 (got NZ=" + std::to_string(NZ) + ")");
+    if ((c.kz_suppress_max >= 1 || c.kz_suppress_hi > 0) && (NZ % 32 != 0))
+        throw Error(KHS_ERR_CONFIG, "kz_suppress_max/kz_suppress_hi require NZ multiple of 32 (got NZ=" + std::to_string(NZ) + ")");
+    if (c.kz_keep_max < 0 || (c.kz_keep_max > 0 && 2 * c.kz_keep_max >= NZ))
+        throw Error(KHS_ERR_CONFIG, "kz_keep_max must satisfy 0 <= K < NZ/2");
+    if (c.hyp_diff > 0.0f && NZ < 5)
+        throw Error(KHS_ERR_CONFIG, "hyp_diff requires NZ >= 5");
+    if (c.kz_suppress_hi > NZ / 2 || c.kz_suppress_max > NZ / 2)
+        throw Error(KHS_ERR_CONFIG, "kz band edges must not exceed NZ/2");
+    if (c.run_mode < 0 || c.run_mode > 6)
+        throw Error(KHS_ERR_CONFIG, "run_mode must be 0..6");
+}
+
+// Output directory name (main_ym.cu:44-49, 143-170).
+std::string output_dir_name(const khs_config& c) {
+    const int run_mode = c.run_mode;
+    const char* mode_tag = (run_mode == 1) ? "_circ"
+                         : (run_mode == 2) ? "_emhd"
+                         : (run_mode == 3) ? "_dtanh"
+                         : (run_mode == 4) ? "_step"
+                         : (run_mode == 5) ? "_tanh"
+                         : (run_mode == 6) ? "_circ_az2seed" : "";
+    const double aYM = c.alpha_YM, p_amp = c.perturb_amp, V0 = c.V0;
+    const double xi_sponge = c.xi_sponge, xi_cut = c.xi_cut, hyp_diff = c.hyp_diff;
+    std::ostringstream d;
+    d << "ym_k" << c.k_mode << "_a" << std::fixed << std::setprecision(3) << aYM;
+    if (p_amp == 0.0) d << "_noperturb";
+    d << mode_tag;
+    if (V0 != 0.1) d << "_v" << std::setprecision(4) << V0;
+    if (xi_sponge > 0.0) d << "_sp" << std::setprecision(1) << xi_sponge;
+    if (xi_cut > 0.0) d << "_xc" << std::setprecision(1) << xi_cut;
+    if (c.freeze_az1 && run_mode == 0) d << "_frz";
+    if (c.eps_override > 0.0f) d << "_eps" << std::fixed << std::setprecision(2) << c.eps_override;
+    if (c.nx_override > 0) d << "_nx" << c.nx_override;
+    if (c.lx_override > 0.f) d << "_lx" << std::fixed << std::setprecision(2) << c.lx_override;
+    if (c.suppress_kz0) d << "_nkz0";
+    if (c.kz_suppress_max >= 1) d << "_nkz1to" << c.kz_suppress_max;
+    if (c.kz_suppress_hi > 0) d << "_bp" << c.kz_suppress_hi;
+    if (hyp_diff > 0.0) d << "_hd" << std::scientific << std::setprecision(0) << hyp_diff;
+    if (c.run_tag[0]) d << "_" << c.run_tag;
+    return d.str();
+}
+
+}  // namespace khs

@@ -1,0 +1,796 @@
+// khs_context.cpp — device context, step orchestration and the C ABI of libkhs.
+//
+// Memory layout in HBM: one allocation holding two full state sets (24 arrays
+// each; a frozen Az1 is shared by both) and a 12-array stage buffer for the
+// x-swept fluid.  Each array is nx*nz elements of the run's precision in the
+// reference layout (x contiguous), padded to 256 bytes.  A step ping-pongs
+// between the state sets so that no kernel updates a stencil input in place:
+//
+//   k_fct_x   : state[cur] -> stage                (both species, one launch)
+//   k_fct_z   : stage + state[cur] -> state[nxt].fluid
+//   k_maxwell : state[nxt].fluid + state[cur].fields -> state[nxt].fields
+//   filters   : state[nxt] in place (column-local / cell-local only)
+//   commit    : cur <-> nxt
+//
+// This replaces main_ym.cu:322-455 (about 116 launches, 44 device syncs and 32
+// device copies per step) with three launches and no host synchronisation;
+// the host only synchronises for the energy check every energy_stride steps.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "khs_host.h"
+#include "khs_launch.h"
+
+namespace khs {
+
+#define HIPCHK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess)                                                              \
+            throw Error(KHS_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+enum Slot { K_FCT_X = 0, K_FCT_Z, K_MAXWELL, K_FILTERS, K_ENERGY, K_NSLOTS };
+static const char* kSlotNames[K_NSLOTS] = {"k_fct_x", "k_fct_z", "k_maxwell", "filters", "energy"};
+
+static const int kFilt15[15] = {S_BY1, S_EX1, S_EZ1, S_BY2, S_BY3, S_EX2, S_EX3, S_EZ2, S_EZ3,
+                                S_AZ2, S_AZ3, S_Q2A, S_Q3A, S_Q2B, S_Q3B};
+static const int kFluid6[6] = {S_PZA, S_PZB, S_PXA, S_PXB, S_NA, S_NB};
+static const int kCol23[12] = {S_BY2, S_BY3, S_EX2, S_EX3, S_EZ2, S_EZ3, S_AZ2, S_AZ3,
+                               S_Q2A, S_Q3A, S_Q2B, S_Q3B};
+
+struct ModeTable {
+    void* c = nullptr;   // [nmodes][nz] of float or double
+    void* s = nullptr;
+    void* w = nullptr;   // [nmodes]
+    int nmodes = 0;
+    bool is_float = false;
+};
+
+struct Ctx {
+    khs_config cfg{};
+    KParams kp{};
+    int nx = 0, nz = 0;
+    size_t N = 0, stride = 0, esz = 8;
+    bool fp32 = false;
+
+    char* dmem = nullptr;
+    void* st[2][S_COUNT] = {};
+    void* stage[2 * F_COUNT] = {};
+    int cur = 0;
+
+    // filters
+    double* d_decay = nullptr;
+    int* d_action = nullptr;
+    bool cell_filters = false;
+    ModeTable t_zmean, t_lo, t_hi, t_keep;
+    double* d_coef = nullptr;
+
+    // reductions / derived
+    double* d_partials = nullptr;
+    double* d_red = nullptr;
+    double* h_red = nullptr;   // pinned
+    double* d_derived = nullptr;
+
+    hipStream_t stream = nullptr;
+
+    // timing
+    bool timing = false;
+    struct Pending { int slot; hipEvent_t a, b; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> evpool;
+    double ms[K_NSLOTS] = {};
+    int cnt[K_NSLOTS] = {};
+
+    // step state
+    long step = 0;
+    double E0 = 0.0;
+    int staged_x = 0, staged_z = 0;
+    bool maxwell_done = false, filters_done = false;
+    std::ofstream ecsv;
+
+    ~Ctx();
+    template <typename T> State<T> state(int k) const {
+        State<T> s;
+        for (int i = 0; i < S_COUNT; ++i) s.a[i] = static_cast<T*>(st[k][i]);
+        return s;
+    }
+    template <typename T> Stage<T> stg() const {
+        Stage<T> s;
+        for (int i = 0; i < 2 * F_COUNT; ++i) s.a[i] = static_cast<T*>(stage[i]);
+        return s;
+    }
+    template <typename T> ArrayList<T> list(int k, const int* ids, int n, bool skip_c1) const {
+        ArrayList<T> l{};
+        l.n = 0;
+        for (int i = 0; i < n; ++i) {
+            int id = ids[i];
+            if (skip_c1 && (id == S_BY1 || id == S_EX1 || id == S_EZ1)) continue;
+            l.a[l.n++] = static_cast<T*>(st[k][id]);
+        }
+        return l;
+    }
+
+    hipEvent_t get_event() {
+        if (!evpool.empty()) { hipEvent_t e = evpool.back(); evpool.pop_back(); return e; }
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        return e;
+    }
+    template <typename F> void timed(int slot, F&& f) {
+        if (!timing) { f(); return; }
+        hipEvent_t a = get_event(), b = get_event();
+        HIPCHK(hipEventRecord(a, stream));
+        f();
+        HIPCHK(hipEventRecord(b, stream));
+        pending.push_back({slot, a, b});
+    }
+    void harvest() {
+        if (pending.empty()) return;
+        HIPCHK(hipStreamSynchronize(stream));
+        for (auto& p : pending) {
+            float t = 0.f;
+            HIPCHK(hipEventElapsedTime(&t, p.a, p.b));
+            ms[p.slot] += t;
+            cnt[p.slot] += 1;
+            evpool.push_back(p.a);
+            evpool.push_back(p.b);
+        }
+        pending.clear();
+    }
+    void check_launch() {
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) throw Error(KHS_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+    }
+
+    // ---- stages ----
+    template <typename T> void do_x(int sp0, int nsp) {
+        timed(K_FCT_X, [&] { launch_fct_x<T>(state<T>(cur), stg<T>(), kp, sp0, nsp, stream); });
+        check_launch();
+    }
+    template <typename T> void do_z(int sp0, int nsp) {
+        timed(K_FCT_Z, [&] { launch_fct_z<T>(state<T>(cur), stg<T>(), state<T>(1 - cur), kp, sp0, nsp, stream); });
+        check_launch();
+    }
+    template <typename T> void do_maxwell() {
+        timed(K_MAXWELL, [&] { launch_maxwell<T>(state<T>(cur), state<T>(1 - cur), kp, stream); });
+        check_launch();
+    }
+    template <typename T> void do_modes(const ModeTable& t, const ArrayList<T>& l, int project) {
+        if (t.nmodes == 0 || l.n == 0) return;
+        if (t.is_float)
+            launch_column_modes<T, float>(l, (const float*)t.c, (const float*)t.s, (const float*)t.w,
+                                          t.nmodes, project, d_coef, kp, stream);
+        else
+            launch_column_modes<T, double>(l, (const double*)t.c, (const double*)t.s, (const double*)t.w,
+                                           t.nmodes, project, d_coef, kp, stream);
+    }
+    template <typename T> void do_filters() {
+        const int k = 1 - cur;
+        const khs_config& c = cfg;
+        const bool skip_c1 = c.suppress_kz0 != 0;   // colour-1 EM is zeroed at the end anyway
+        bool any = cell_filters || c.suppress_kz0 || c.kz_suppress_max >= 1 ||
+                   c.kz_suppress_hi > c.kz_suppress_max + 1 || c.kz_keep_max > 0 || c.hyp_diff > 0.0f;
+        if (!any) return;
+        timed(K_FILTERS, [&] {
+            if (cell_filters) launch_cell_filters<T>(list<T>(k, kCol23, 12, false), d_decay, d_action, kp, stream);
+            if (c.suppress_kz0) do_modes<T>(t_zmean, list<T>(k, kFilt15, 15, skip_c1), 0);
+            // reference band filter: 15 field/charge arrays + (pz, px, n) x 2 species
+            int ids21[21];
+            std::memcpy(ids21, kFilt15, sizeof(kFilt15));
+            std::memcpy(ids21 + 15, kFluid6, sizeof(kFluid6));
+            if (c.kz_suppress_max >= 1) do_modes<T>(t_lo, list<T>(k, ids21, 21, skip_c1), 0);
+            if (c.kz_suppress_hi > c.kz_suppress_max + 1) do_modes<T>(t_hi, list<T>(k, ids21, 21, skip_c1), 0);
+            if (c.kz_keep_max > 0) do_modes<T>(t_keep, list<T>(k, ids21, 21, skip_c1), 1);
+            if (c.hyp_diff > 0.0f) launch_hyperdiffuse<T>(list<T>(k, kCol23, 12, false), (double)c.hyp_diff, kp, stream);
+            if (c.suppress_kz0) {
+                const int c1[3] = {S_BY1, S_EX1, S_EZ1};
+                launch_fill_zero<T>(list<T>(k, c1, 3, false), N, stream);
+            }
+        });
+        check_launch();
+    }
+    void x_stage(int sp0, int nsp) { fp32 ? do_x<float>(sp0, nsp) : do_x<double>(sp0, nsp); }
+    void z_stage(int sp0, int nsp) { fp32 ? do_z<float>(sp0, nsp) : do_z<double>(sp0, nsp); }
+    void maxwell_stage() { fp32 ? do_maxwell<float>() : do_maxwell<double>(); }
+    void filter_stage() { fp32 ? do_filters<float>() : do_filters<double>(); }
+
+    double energy() {
+        timed(K_ENERGY, [&] {
+            if (fp32) launch_energy<float>(state<float>(cur), kp, d_partials, d_red, stream);
+            else launch_energy<double>(state<double>(cur), kp, d_partials, d_red, stream);
+        });
+        check_launch();
+        HIPCHK(hipMemcpyAsync(h_red, d_red, sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        return h_red[0];
+    }
+};
+
+Ctx::~Ctx() {
+    if (stream) hipStreamSynchronize(stream);
+    for (auto& p : pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
+    for (auto e : evpool) hipEventDestroy(e);
+    if (dmem) hipFree(dmem);
+    if (h_red) hipHostFree(h_red);
+    if (stream) hipStreamDestroy(stream);
+}
+
+// --------------------------------------------------------------------------
+static size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+static float band_phase(int m, int z, int nz) {     // YM_Physics.cu:387
+    float ph = 2.0f * (float)M_PI;
+    ph = ph * (float)m;
+    ph = ph * (float)z;
+    ph = ph / (float)nz;
+    return ph;
+}
+
+static std::unique_ptr<Ctx> make_ctx(const khs_config& in) {
+    auto ctx = std::make_unique<Ctx>();
+    ctx->cfg = in;
+    config_derive(ctx->cfg);          // validates; idempotent on a derived config
+    const khs_config& c = ctx->cfg;
+    ctx->nx = c.nx; ctx->nz = c.nz;
+    ctx->N = (size_t)c.nx * c.nz;
+    ctx->fp32 = (c.precision == KHS_FP32);
+    ctx->esz = ctx->fp32 ? 4 : 8;
+    ctx->stride = round_up(ctx->N * ctx->esz, 256);
+
+    KParams& kp = ctx->kp;
+    kp.nx = c.nx; kp.nz = c.nz;
+    kp.periodic_x = c.periodic_x;
+    kp.freeze_az1 = c.freeze_az1;
+    kp.dt = c.dt; kp.dx = c.dx; kp.dz = c.dz;
+    kp.dtdx = c.dt / c.dx;
+    kp.dtdz = c.dt / c.dz;
+    kp.c2 = c.c * c.c;
+    kp.eps0 = c.eps_0;
+    kp.alpha = c.alpha;
+    kp.warm_T = (double)c.warm_T;
+
+    int count;
+    if (hipGetDeviceCount(&count) != hipSuccess || count < 1)
+        throw Error(KHS_ERR_DEVICE, "no HIP device available");
+    HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+
+    // ---- mode tables (host) ----
+    const int nz = c.nz;
+    auto float_band = [&](int lo, int hi, std::vector<float>& hc, std::vector<float>& hs, std::vector<float>& hw) {
+        int M = std::max(0, hi - lo + 1);
+        hc.resize((size_t)M * nz); hs.resize((size_t)M * nz); hw.assign(M, 2.0f / (float)nz);
+        for (int j = 0; j < M; ++j)
+            for (int z = 0; z < nz; ++z) {
+                float ph = band_phase(lo + j, z, nz);
+                hc[(size_t)j * nz + z] = (float)std::cos((double)ph);
+                hs[(size_t)j * nz + z] = (float)std::sin((double)ph);
+            }
+        return M;
+    };
+    std::vector<float> lo_c, lo_s, lo_w, hi_c, hi_s, hi_w;
+    int nlo = (c.kz_suppress_max >= 1) ? float_band(1, c.kz_suppress_max, lo_c, lo_s, lo_w) : 0;
+    int nhi = (c.kz_suppress_hi > c.kz_suppress_max + 1)
+                  ? float_band(c.kz_suppress_max + 2, c.kz_suppress_hi, hi_c, hi_s, hi_w) : 0;
+    std::vector<double> kp_c, kp_s, kp_w;
+    int nkeep = 0;
+    if (c.kz_keep_max > 0) {
+        nkeep = c.kz_keep_max + 1;
+        kp_c.resize((size_t)nkeep * nz); kp_s.resize((size_t)nkeep * nz); kp_w.resize(nkeep);
+        for (int m = 0; m < nkeep; ++m) {
+            kp_w[m] = (m == 0) ? 1.0 / nz : 2.0 / nz;
+            for (int z = 0; z < nz; ++z) {
+                double ph = 2.0 * M_PI * (double)((long)m * z % nz) / (double)nz;
+                kp_c[(size_t)m * nz + z] = std::cos(ph);
+                kp_s[(size_t)m * nz + z] = std::sin(ph);
+            }
+        }
+    }
+    std::vector<double> zm_c(nz, 1.0), zm_s(nz, 0.0), zm_w(1, 1.0 / (double)nz);
+
+    // sponge / xi_cut per-column action (YM_Physics.cu:161-210)
+    std::vector<double> decay(c.nx, 1.0);
+    std::vector<int> action(c.nx, 0);
+    for (int x = 0; x < c.nx; ++x) {
+        double x_c = (c.nx * c.dx) * 0.5;
+        double xi = std::fabs((x * c.dx - x_c) / c.eps);
+        if (c.xi_sponge > 0.0f && xi > (double)c.xi_sponge) {
+            double sigma = (double)c.sigma_sponge * (xi - (double)c.xi_sponge);
+            decay[x] = std::exp(-sigma * c.dt);
+            action[x] = 1;
+        }
+        if (c.xi_cut > 0.0f && xi > (double)c.xi_cut) action[x] = 2;
+    }
+    ctx->cell_filters = (c.xi_sponge > 0.0f) || (c.xi_cut > 0.0f);
+
+    // ---- one device allocation ----
+    const bool share_az1 = c.freeze_az1 != 0;
+    size_t off = 0;
+    std::vector<std::pair<void**, size_t>> plan;
+    auto reserve = [&](void** p, size_t bytes) { plan.push_back({p, off}); off += round_up(bytes, 256); };
+    for (int k = 0; k < 2; ++k)
+        for (int i = 0; i < S_COUNT; ++i) {
+            if (k == 1 && i == S_AZ1 && share_az1) continue;
+            reserve(&ctx->st[k][i], ctx->stride);
+        }
+    for (int i = 0; i < 2 * F_COUNT; ++i) reserve(&ctx->stage[i], ctx->stride);
+    reserve((void**)&ctx->d_decay, c.nx * sizeof(double));
+    reserve((void**)&ctx->d_action, c.nx * sizeof(int));
+    auto reserve_tab = [&](ModeTable& t, int M, bool isf) {
+        t.nmodes = M; t.is_float = isf;
+        if (!M) return;
+        size_t e = isf ? 4 : 8;
+        reserve(&t.c, (size_t)M * nz * e);
+        reserve(&t.s, (size_t)M * nz * e);
+        reserve(&t.w, (size_t)M * e);
+    };
+    reserve_tab(ctx->t_lo, nlo, true);
+    reserve_tab(ctx->t_hi, nhi, true);
+    reserve_tab(ctx->t_keep, nkeep, false);
+    reserve_tab(ctx->t_zmean, c.suppress_kz0 ? 1 : 0, false);
+    int maxm = std::max(std::max(nlo, nhi), std::max(nkeep, c.suppress_kz0 ? 1 : 0));
+    if (maxm) reserve((void**)&ctx->d_coef, (size_t)21 * 2 * maxm * c.nx * sizeof(double));
+    reserve((void**)&ctx->d_partials, (size_t)kRedBlocks * 8 * 2 * sizeof(double));
+    reserve((void**)&ctx->d_red, 16 * sizeof(double));
+    reserve((void**)&ctx->d_derived, ctx->N * sizeof(double));
+
+    HIPCHK(hipMalloc((void**)&ctx->dmem, off));
+    for (auto& pr : plan) *pr.first = ctx->dmem + pr.second;
+    if (share_az1) ctx->st[1][S_AZ1] = ctx->st[0][S_AZ1];
+    HIPCHK(hipMemsetAsync(ctx->dmem, 0, off, ctx->stream));
+    HIPCHK(hipHostMalloc((void**)&ctx->h_red, 16 * sizeof(double), hipHostMallocDefault));
+
+    auto up = [&](void* dst, const void* src, size_t bytes) {
+        if (bytes) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    };
+    up(ctx->d_decay, decay.data(), decay.size() * sizeof(double));
+    up(ctx->d_action, action.data(), action.size() * sizeof(int));
+    if (nlo) { up(ctx->t_lo.c, lo_c.data(), lo_c.size() * 4); up(ctx->t_lo.s, lo_s.data(), lo_s.size() * 4); up(ctx->t_lo.w, lo_w.data(), lo_w.size() * 4); }
+    if (nhi) { up(ctx->t_hi.c, hi_c.data(), hi_c.size() * 4); up(ctx->t_hi.s, hi_s.data(), hi_s.size() * 4); up(ctx->t_hi.w, hi_w.data(), hi_w.size() * 4); }
+    if (nkeep) { up(ctx->t_keep.c, kp_c.data(), kp_c.size() * 8); up(ctx->t_keep.s, kp_s.data(), kp_s.size() * 8); up(ctx->t_keep.w, kp_w.data(), kp_w.size() * 8); }
+    if (c.suppress_kz0) { up(ctx->t_zmean.c, zm_c.data(), nz * 8); up(ctx->t_zmean.s, zm_s.data(), nz * 8); up(ctx->t_zmean.w, zm_w.data(), 8); }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return ctx;
+}
+
+static void upload_state(Ctx& ctx, int k, int id, const double* h) {
+    void* dst = ctx.st[k][id];
+    if (!ctx.fp32) {
+        HIPCHK(hipMemcpyAsync(dst, h, ctx.N * 8, hipMemcpyHostToDevice, ctx.stream));
+        HIPCHK(hipStreamSynchronize(ctx.stream));
+    } else {
+        std::vector<float> tmp(ctx.N);
+        for (size_t i = 0; i < ctx.N; ++i) tmp[i] = (float)h[i];
+        HIPCHK(hipMemcpyAsync(dst, tmp.data(), ctx.N * 4, hipMemcpyHostToDevice, ctx.stream));
+        HIPCHK(hipStreamSynchronize(ctx.stream));
+    }
+}
+
+static void download(Ctx& ctx, const void* src, bool isf, double* h) {
+    if (!isf) {
+        HIPCHK(hipMemcpyAsync(h, src, ctx.N * 8, hipMemcpyDeviceToHost, ctx.stream));
+        HIPCHK(hipStreamSynchronize(ctx.stream));
+    } else {
+        std::vector<float> tmp(ctx.N);
+        HIPCHK(hipMemcpyAsync(tmp.data(), src, ctx.N * 4, hipMemcpyDeviceToHost, ctx.stream));
+        HIPCHK(hipStreamSynchronize(ctx.stream));
+        for (size_t i = 0; i < ctx.N; ++i) h[i] = (double)tmp[i];
+    }
+}
+
+static void require_idle(const Ctx& c, const char* what) {
+    if (c.staged_x || c.staged_z || c.maxwell_done || c.filters_done)
+        throw Error(KHS_ERR_STATE, std::string(what) + " called in the middle of a staged step (call khs_commit first)");
+}
+
+static void commit(Ctx& c) {
+    c.cur = 1 - c.cur;
+    c.staged_x = c.staged_z = 0;
+    c.maxwell_done = c.filters_done = false;
+    c.step += 1;
+}
+
+static void full_step(Ctx& c) {
+    c.x_stage(0, 2);
+    c.z_stage(0, 2);
+    c.maxwell_stage();
+    c.filter_stage();
+    commit(c);
+}
+
+static void write_energy_row(Ctx& c, long step, double E) {
+    if (!c.ecsv.is_open()) return;
+    double ratio = E / c.E0;
+    double t_now = step * c.cfg.dt;
+    c.ecsv << step << ',' << t_now << ',' << E << ',' << ratio << '\n';
+    c.ecsv.flush();
+}
+
+}  // namespace khs
+
+// ==========================================================================
+// C ABI
+// ==========================================================================
+using khs::Ctx;
+using khs::Error;
+
+struct khs_ctx {
+    std::unique_ptr<Ctx> impl;
+};
+
+static thread_local std::string g_last_error;
+
+template <typename F>
+static int guard(F&& f) {
+    try {
+        f();
+        g_last_error.clear();
+        return KHS_OK;
+    } catch (const Error& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return KHS_ERR_ARG;
+    }
+}
+
+#define CTX_OR_FAIL(ctx)                                                 \
+    if (!(ctx) || !(ctx)->impl) { g_last_error = "null context"; return KHS_ERR_ARG; }
+
+extern "C" {
+
+const char* khs_last_error(void) { return g_last_error.c_str(); }
+const char* khs_version(void) { return "khs 0.1 (gfx950)"; }
+size_t khs_config_sizeof(void) { return sizeof(khs_config); }
+
+int khs_config_defaults(khs_config* cfg) {
+    if (!cfg) { g_last_error = "null config"; return KHS_ERR_ARG; }
+    return guard([&] { khs::config_defaults(*cfg); khs::config_derive(*cfg); });
+}
+
+int khs_config_set(khs_config* cfg, const char* key, const char* value) {
+    if (!cfg || !key || !value) { g_last_error = "null argument"; return KHS_ERR_ARG; }
+    return guard([&] {
+        if (!khs::config_set(*cfg, key, value))
+            throw Error(KHS_ERR_CONFIG, std::string("unknown key '") + key + "'");
+    });
+}
+
+int khs_load_ini(const char* path, khs_config* cfg) {
+    if (!path || !cfg) { g_last_error = "null argument"; return KHS_ERR_ARG; }
+    return guard([&] {
+        khs::config_defaults(*cfg);
+        std::string warn;
+        khs::config_load(path, *cfg, &warn);
+        if (!warn.empty()) fputs(warn.c_str(), stderr);
+    });
+}
+
+int khs_derive(khs_config* cfg) {
+    if (!cfg) { g_last_error = "null config"; return KHS_ERR_ARG; }
+    return guard([&] { khs::config_derive(*cfg); });
+}
+
+int khs_output_dir(const khs_config* cfg, char* buf, size_t buflen) {
+    if (!cfg || !buf) { g_last_error = "null argument"; return KHS_ERR_ARG; }
+    return guard([&] {
+        std::string s = khs::output_dir_name(*cfg);
+        if (s.size() + 1 > buflen) throw Error(KHS_ERR_ARG, "buffer too small");
+        std::memcpy(buf, s.c_str(), s.size() + 1);
+    });
+}
+
+int khs_create(const khs_config* cfg, khs_ctx** out) {
+    if (!cfg || !out) { g_last_error = "null argument"; return KHS_ERR_ARG; }
+    *out = nullptr;
+    return guard([&] {
+        auto h = new khs_ctx;
+        try { h->impl = khs::make_ctx(*cfg); } catch (...) { delete h; throw; }
+        *out = h;
+    });
+}
+
+int khs_destroy(khs_ctx* ctx) {
+    if (!ctx) return KHS_OK;
+    int rc = guard([&] { ctx->impl.reset(); });
+    delete ctx;
+    return rc;
+}
+
+int khs_init(khs_ctx* ctx, const char* seed_path) {
+    CTX_OR_FAIL(ctx);
+    return guard([&] {
+        Ctx& c = *ctx->impl;
+        std::string sp = seed_path ? std::string(seed_path) : std::string(c.cfg.seed_profile_file);
+        khs::SeedProfiles seed;
+        if (!sp.empty()) seed = khs::read_seed_file(sp, c.nx);
+        std::vector<std::vector<double>> host;
+        khs::host_initial_state(c.cfg, seed, host);
+        c.cur = 0;
+        c.staged_x = c.staged_z = 0;
+        c.maxwell_done = c.filters_done = false;
+        for (int i = 0; i < khs::S_COUNT; ++i) khs::upload_state(c, 0, i, host[i].data());
+        c.step = 0;
+        c.E0 = c.energy();
+    });
+}
+
+int khs_array_id(const char* name) {
+    if (!name) return -1;
+    static const char* names[] = {"Ex1", "Ex2", "Ex3", "Ez1", "Ez2", "Ez3", "By1", "By2", "By3",
+                                  "Az1", "Az2", "Az3", "nA", "PxA", "PzA", "Q1A", "Q2A", "Q3A",
+                                  "nB", "PxB", "PzB", "Q1B", "Q2B", "Q3B"};
+    static const char* alt[] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                nullptr, nullptr, nullptr, nullptr, "pxA", "pzA", nullptr, nullptr, nullptr,
+                                nullptr, "pxB", "pzB", nullptr, nullptr, nullptr};
+    for (int i = 0; i < KHS_N_STATE; ++i) {
+        if (std::strcmp(name, names[i]) == 0) return i;
+        if (alt[i] && std::strcmp(name, alt[i]) == 0) return i;
+    }
+    static const char* dnames[] = {"Jx1", "Jx2", "Jx3", "Jz1", "Jz2", "Jz3", "uxA", "uzA", "uxB", "uzB",
+                                   "srcpxA", "srcpzA", "srcpxB", "srcpzB", "sQ1A", "sQ2A", "sQ3A",
+                                   "sQ1B", "sQ2B", "sQ3B", "energy_density"};
+    for (int i = 0; i < (int)(sizeof(dnames) / sizeof(dnames[0])); ++i)
+        if (std::strcmp(name, dnames[i]) == 0) return KHS_JX1 + i;
+    return -1;
+}
+
+int khs_get_array(khs_ctx* ctx, int id, double* host, size_t n) {
+    CTX_OR_FAIL(ctx);
+    return guard([&] {
+        Ctx& c = *ctx->impl;
+        if (!host || n != c.N) throw Error(KHS_ERR_ARG, "host buffer must hold nx*nz doubles");
+        if (id >= 0 && id < KHS_N_STATE) {
+            khs::download(c, c.st[c.cur][id], c.fp32, host);
+        } else if (id >= KHS_JX1 && id < KHS_N_ARRAY_IDS) {
+            if (c.fp32) khs::launch_derived<float>(c.state<float>(c.cur), id, c.kp, c.d_derived, c.stream);
+            else khs::launch_derived<double>(c.state<double>(c.cur), id, c.kp, c.d_derived, c.stream);
+            c.check_launch();
+            khs::download(c, c.d_derived, false, host);
+        } else {
+            throw Error(KHS_ERR_ARG, "unknown array id " + std::to_string(id));
+        }
+    });
+}
+
+int khs_get_array_buf(khs_ctx* ctx, int buf, int id, double* host, size_t n) {
+    CTX_OR_FAIL(ctx);
+    return guard([&] {
+        Ctx& c = *ctx->impl;
+        if (!host || n != c.N) throw Error(KHS_ERR_ARG, "host buffer must hold nx*nz doubles");
+        if (id < 0 || id >= KHS_N_STATE) throw Error(KHS_ERR_ARG, "state ids only (0..23)");
+        const void* src = nullptr;
+        if (buf == KHS_BUF_CURRENT) src = c.st[c.cur][id];
+        else if (buf == KHS_BUF_NEXT) src = c.st[1 - c.cur][id];
+        else if (buf == KHS_BUF_STAGE) {
+            if (id < KHS_NA) throw Error(KHS_ERR_ARG, "the stage holds fluid arrays only");
+            src = c.stage[id - KHS_NA];
+        } else throw Error(KHS_ERR_ARG, "unknown buffer");
+        khs::download(c, src, c.fp32, host);
+    });
+}
+
+int khs_set_array(khs_ctx* ctx, int id, const double* host, size_t n) {
+    CTX_OR_FAIL(ctx);
+    return guard([&] {
+        Ctx& c = *ctx->impl;
+        if (!host || n != c.N) throw Error(KHS_ERR_ARG, "host buffer must hold nx*nz doubles");
+        if (id < 0 || id >= KHS_N_STATE) throw Error(KHS_ERR_ARG, "only state arrays (0..23) can be set");
+        khs::require_idle(c, "khs_set_array");
+        khs::upload_state(c, c.cur, id, host);
+    });
+}
+
+int khs_fct_x_sweep(khs_ctx* ctx, int species) {
+    CTX_OR_FAIL(ctx);
+    return guard([&] {
+        Ctx& c = *ctx->impl;
+        if (species != 0 && species != 1) throw Error(KHS_ERR_ARG, "species must be 0 or 1");
+        if (c.staged_z || c.maxwell_done) throw Error(KHS_ERR_STATE, "x-sweep after z-sweep: commit the step first");
+        c.x_stage(species, 1);
+        c.staged_x |= (1 << species);
+    });
+}
+
+int khs_fct_z_sweep(khs_ctx* ctx, int species) {
+    CTX_OR_FAIL(ctx);
+    return guard([&] {
+        Ctx& c = *ctx->impl;
+        if (species != 0 && species != 1) throw Error(KHS_ERR_ARG, "species must be 0 or 1");
+        if (!(c.staged_x & (1 << species))) throw Error(KHS_ERR_STATE, "z-sweep before x-sweep of this species");
+        c.z_stage(species, 1);
+        c.staged_z |= (1 << species);
+    });
+}
+
+int khs_maxwell(khs_ctx* ctx) {
+    CTX_OR_FAIL(ctx);
+    return guard([&] {
+        Ctx& c = *ctx->impl;
+        if (c.staged_z != 3) throw Error(KHS_ERR_STATE, "khs_maxwell needs both species z-swept");
+        c.maxwell_stage();
+        c.maxwell_done = true;
+    });
+}
+
+int khs_filters(khs_ctx* ctx) {
+    CTX_OR_FAIL(ctx);
+    return guard([&] {
+        Ctx& c = *ctx->impl;
+        if (!c.maxwell_done) throw Error(KHS_ERR_STATE, "khs_filters needs khs_maxwell first");
+        c.filter_stage();
+        c.filters_done = true;
+    });
+}
+
+int khs_commit(khs_ctx* ctx) {
+    CTX_OR_FAIL(ctx);
+    return guard([&] {
+        Ctx& c = *ctx->impl;
+        if (!c.maxwell_done) throw Error(KHS_ERR_STATE, "khs_commit needs khs_maxwell first");
+        if (!c.filters_done) c.filter_stage();
+        khs::commit(c);
+    });
+}
+
+int khs_step(khs_ctx* ctx, int nsteps, int* halted) {
+    CTX_OR_FAIL(ctx);
+    if (halted) *halted = 0;
+    return guard([&] {
+        Ctx& c = *ctx->impl;
+        khs::require_idle(c, "khs_step");
+        const int stride = c.cfg.energy_stride;
+        for (int s = 0; s < nsteps; ++s) {
+            khs::full_step(c);
+            if (stride > 0 && c.step % stride == 0) {
+                double E = c.energy();
+                khs::write_energy_row(c, c.step, E);
+                double ratio = E / c.E0;
+                if (std::isnan(E) || std::isinf(E) || ratio > c.cfg.energy_thresh) {
+                    if (halted) *halted = 1;
+                    break;
+                }
+            }
+        }
+    });
+}
+
+int khs_get_step(khs_ctx* ctx, int* step, double* time) {
+    CTX_OR_FAIL(ctx);
+    Ctx& c = *ctx->impl;
+    if (step) *step = (int)c.step;
+    if (time) *time = c.step * c.cfg.dt;
+    return KHS_OK;
+}
+
+int khs_energy(khs_ctx* ctx, double* E) {
+    CTX_OR_FAIL(ctx);
+    if (!E) { g_last_error = "null argument"; return KHS_ERR_ARG; }
+    return guard([&] { *E = ctx->impl->energy(); });
+}
+
+int khs_energy0(khs_ctx* ctx, double* E0) {
+    CTX_OR_FAIL(ctx);
+    if (!E0) { g_last_error = "null argument"; return KHS_ERR_ARG; }
+    *E0 = ctx->impl->E0;
+    return KHS_OK;
+}
+
+int khs_gauss(khs_ctx* ctx, double out[7]) {
+    CTX_OR_FAIL(ctx);
+    if (!out) { g_last_error = "null argument"; return KHS_ERR_ARG; }
+    return guard([&] {
+        Ctx& c = *ctx->impl;
+        if (c.fp32) khs::launch_gauss<float>(c.state<float>(c.cur), c.kp, c.d_partials, c.d_red, c.stream);
+        else khs::launch_gauss<double>(c.state<double>(c.cur), c.kp, c.d_partials, c.d_red, c.stream);
+        c.check_launch();
+        HIPCHK(hipMemcpyAsync(c.h_red, c.d_red, 7 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        HIPCHK(hipStreamSynchronize(c.stream));
+        const double Nd = (double)c.N;
+        for (int k = 0; k < 6; ++k) out[k] = std::sqrt(c.h_red[k] / Nd);
+        out[6] = std::sqrt(c.h_red[6] / Nd) / c.cfg.dx + 1e-30;
+    });
+}
+
+int khs_write_snapshot(khs_ctx* ctx, const char* dir, int step) {
+    CTX_OR_FAIL(ctx);
+    if (!dir) { g_last_error = "null argument"; return KHS_ERR_ARG; }
+    return guard([&] {
+        Ctx& c = *ctx->impl;
+        // column order of main_ym.cu:291-292
+        const int cols[24] = {KHS_BY1, KHS_BY2, KHS_BY3, KHS_AZ2, KHS_AZ3, KHS_PZA, KHS_PXA, KHS_Q1A,
+                              KHS_PZB, KHS_PXB, KHS_Q1B, KHS_EX1, KHS_EX2, KHS_EX3, KHS_EZ1, KHS_EZ2,
+                              KHS_EZ3, KHS_NA, KHS_NB, KHS_Q2A, KHS_Q3A, KHS_Q2B, KHS_Q3B, KHS_AZ1};
+        std::vector<std::vector<double>> h(24, std::vector<double>(c.N));
+        for (int k = 0; k < 24; ++k) khs::download(c, c.st[c.cur][cols[k]], c.fp32, h[k].data());
+        std::ostringstream fn;
+        fn << dir << "/ym_" << std::setfill('0') << std::setw(6) << step << ".csv";
+        std::ofstream out(fn.str());
+        if (!out) throw Error(KHS_ERR_IO, "cannot write " + fn.str());
+        out << "X,Z,By1,By2,By3,Az2,Az3,PzA,PxA,Q1A,PzB,PxB,Q1B,"
+               "Ex1,Ex2,Ex3,Ez1,Ez2,Ez3,nA,nB,Q2A,Q3A,Q2B,Q3B,Az1\n";
+        const double DX = c.cfg.dx, DZ = c.cfg.dz;
+        for (int z = 0; z < c.nz; ++z)
+            for (int x = 0; x < c.nx; ++x) {
+                size_t i = (size_t)x + (size_t)z * c.nx;
+                out << x * DX << ',' << z * DZ;
+                for (int k = 0; k < 24; ++k) out << ',' << h[k][i];
+                out << '\n';
+            }
+        if (!out) throw Error(KHS_ERR_IO, "write failed: " + fn.str());
+    });
+}
+
+int khs_energy_csv_open(khs_ctx* ctx, const char* path) {
+    CTX_OR_FAIL(ctx);
+    if (!path) { g_last_error = "null argument"; return KHS_ERR_ARG; }
+    return guard([&] {
+        Ctx& c = *ctx->impl;
+        if (c.ecsv.is_open()) c.ecsv.close();
+        c.ecsv.open(path);
+        if (!c.ecsv) throw Error(KHS_ERR_IO, std::string("cannot open ") + path);
+        c.ecsv << std::scientific << std::setprecision(8);
+        c.ecsv << "step,time,E_total,E_ratio\n";
+        c.ecsv << "0,0," << c.E0 << ",1.0\n";
+        c.ecsv.flush();
+    });
+}
+
+int khs_energy_csv_row(khs_ctx* ctx, int step, double E) {
+    CTX_OR_FAIL(ctx);
+    return guard([&] { khs::write_energy_row(*ctx->impl, step, E); });
+}
+
+int khs_energy_csv_close(khs_ctx* ctx) {
+    CTX_OR_FAIL(ctx);
+    return guard([&] { if (ctx->impl->ecsv.is_open()) ctx->impl->ecsv.close(); });
+}
+
+int khs_sync(khs_ctx* ctx) {
+    CTX_OR_FAIL(ctx);
+    return guard([&] { HIPCHK(hipStreamSynchronize(ctx->impl->stream)); });
+}
+
+int khs_timing_enable(khs_ctx* ctx, int on) {
+    CTX_OR_FAIL(ctx);
+    return guard([&] { ctx->impl->harvest(); ctx->impl->timing = (on != 0); });
+}
+
+int khs_timing_reset(khs_ctx* ctx) {
+    CTX_OR_FAIL(ctx);
+    return guard([&] {
+        Ctx& c = *ctx->impl;
+        c.harvest();
+        for (int i = 0; i < khs::K_NSLOTS; ++i) { c.ms[i] = 0.0; c.cnt[i] = 0; }
+    });
+}
+
+int khs_kernel_times(khs_ctx* ctx, double* ms, int* launches, int nslots) {
+    CTX_OR_FAIL(ctx);
+    return guard([&] {
+        Ctx& c = *ctx->impl;
+        c.harvest();
+        for (int i = 0; i < nslots && i < khs::K_NSLOTS; ++i) {
+            if (ms) ms[i] = c.ms[i];
+            if (launches) launches[i] = c.cnt[i];
+        }
+    });
+}
+
+const char* khs_kernel_name(int slot) {
+    if (slot < 0 || slot >= khs::K_NSLOTS) return nullptr;
+    return khs::kSlotNames[slot];
+}
+
+}  // extern "C"

@@ -1,0 +1,204 @@
+// khs_device.h — device-side types and inline physics shared by the gfx950 kernels.
+//
+// All kernels are templated on the storage/arithmetic type T (double for the
+// FP64 drop-in, float for the precision form).  The FP64 expressions keep the
+// reference's evaluation order (citations inline) and the library is built with
+// -ffp-contract=off, so FP64 stencil results are reproducible bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "khs_layout.h"
+
+namespace khs {
+
+template <typename T>
+struct State {
+    T* a[S_COUNT];
+};
+
+template <typename T>
+struct Stage {            // x-swept fluid of both species, 12 arrays
+    T* a[2 * F_COUNT];
+};
+
+// Scalar parameters of one run, passed by value to every kernel.
+struct KParams {
+    int nx, nz;
+    int periodic_x;
+    int freeze_az1;
+    double dt, dx, dz;
+    double dtdx, dtdz;        // dt/dx, dt/dz (the reference evaluates dt_step/dx per call; same value)
+    double c2, eps0, alpha, warm_T;
+};
+
+// --------------------------------------------------------------------------
+// index helpers
+// --------------------------------------------------------------------------
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+__device__ __forceinline__ int wrapi(int v, int n) { int r = v % n; return r < 0 ? r + n : r; }
+// Map an extended x index onto a real column: periodic wrap (run_mode >= 1) or
+// clamped replication (walls), which is what the reference's ghost padding
+// (YM_FCT.cu:7-28) and clamped block-halo loads (FCT_Sweeps.cu:52,73,78) produce.
+__device__ __forceinline__ int map_x(int x, const KParams& p) {
+    return p.periodic_x ? wrapi(x, p.nx) : clampi(x, 0, p.nx - 1);
+}
+// neighbour indices of the EM / source stencils
+__device__ __forceinline__ int xm_src(int x, const KParams& p) {   // YM_Fluid.cu:38
+    return p.periodic_x ? (x + p.nx - 1) % p.nx : (x > 0 ? x - 1 : 0);
+}
+__device__ __forceinline__ int xp_src(int x, const KParams& p) {   // YM_Physics.cu:148
+    return p.periodic_x ? (x + 1) % p.nx : (x + 1 < p.nx ? x + 1 : p.nx - 1);
+}
+__device__ __forceinline__ int zm_per(int z, int nz) { return z == 0 ? nz - 1 : z - 1; }
+__device__ __forceinline__ int zp_per(int z, int nz) { return z == nz - 1 ? 0 : z + 1; }
+
+// Explicit selects: deterministic sign of zero (NaN propagates through the
+// first operand of kmax(a, floor)).
+template <typename T> __device__ __forceinline__ T kmin(T a, T b) { return (b < a) ? b : a; }
+template <typename T> __device__ __forceinline__ T kmax(T a, T b) { return (a < b) ? b : a; }
+
+// --------------------------------------------------------------------------
+// FCT coefficients and limiter (FCT_DeviceMath.cuh:38-63, 108-125)
+// --------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void fct_nu_mu(T eps, T& nu, T& mu) {
+    T scrh = kmin(T(1.0 / 6.0), fabs(eps));
+    scrh = T(1.0 / 3.0) * scrh * scrh;
+    nu = T(1.0 / 6.0) + T(1.0 / 3.0) * (eps * eps - scrh * scrh);
+    mu = T(0.25) - T(0.5) * nu;
+}
+
+template <typename T>
+__device__ __forceinline__ T fct_limit(T flx, T dm1, T d0, T dp1) {
+    T fsgn = copysign(T(1.0), d0);
+    T term = fsgn * dm1;
+    T terp = fsgn * dp1;
+    T fa = fabs(flx);
+    T lim = kmax(T(0.0), kmin(term, kmin(fa, terp)));
+    return fsgn * lim;
+}
+
+// --------------------------------------------------------------------------
+// Sources evaluated from a state (the reference stores them between steps;
+// they are pure functions of the end-of-step state, main_ym.cu:450-455).
+// --------------------------------------------------------------------------
+// kernel_ym_lorentz (YM_Fluid.cu:32-84): x-component at real cell (x, z).
+template <typename T>
+__device__ __forceinline__ T lorentz_px(const State<T>& S, int sp, int x, int z, const KParams& p) {
+    const int nx = p.nx;
+    const int b = S_NA + sp * F_COUNT;
+    const int xm = xm_src(x, p), zm = zm_per(z, p.nz);
+    const size_t i = (size_t)x + (size_t)z * nx;
+    const size_t ixm = (size_t)xm + (size_t)z * nx;
+    const size_t izm = (size_t)x + (size_t)zm * nx;
+    const size_t ixmzm = (size_t)xm + (size_t)zm * nx;
+    T n_safe = kmax(S.a[b + F_N][i], T(0.05));
+    T vz = S.a[b + F_PZ][i] / n_safe;
+    T acc[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const T* Ex = S.a[S_EX1 + c];
+        const T* By = S.a[S_BY1 + c];
+        T Exn = T(0.5) * (Ex[i] + Ex[ixm]);
+        T Byn = T(0.25) * (By[i] + By[ixm] + By[izm] + By[ixmzm]);
+        acc[c] = S.a[b + F_Q1 + c][i] * (Exn + vz * Byn);
+    }
+    T s = -(acc[0] + acc[1] + acc[2]);
+    if (p.warm_T > 0.0) {
+        const T* n = S.a[b + F_N];
+        const int xp = xp_src(x, p);
+        T dndx = (n[(size_t)xp + (size_t)z * nx] - n[ixm]) / (T(2.0) * T(p.dx));
+        s -= T(p.warm_T) * dndx;
+    }
+    return s;
+}
+
+// z-component of kernel_ym_lorentz at real cell (x, z).
+template <typename T>
+__device__ __forceinline__ T lorentz_pz(const State<T>& S, int sp, int x, int z, const KParams& p) {
+    const int nx = p.nx;
+    const int b = S_NA + sp * F_COUNT;
+    const int xm = xm_src(x, p), zm = zm_per(z, p.nz);
+    const size_t i = (size_t)x + (size_t)z * nx;
+    const size_t ixm = (size_t)xm + (size_t)z * nx;
+    const size_t izm = (size_t)x + (size_t)zm * nx;
+    const size_t ixmzm = (size_t)xm + (size_t)zm * nx;
+    T n_safe = kmax(S.a[b + F_N][i], T(0.05));
+    T vx = S.a[b + F_PX][i] / n_safe;
+    T acc[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const T* Ez = S.a[S_EZ1 + c];
+        const T* By = S.a[S_BY1 + c];
+        T Ezn = T(0.5) * (Ez[i] + Ez[izm]);
+        T Byn = T(0.25) * (By[i] + By[ixm] + By[izm] + By[ixmzm]);
+        acc[c] = S.a[b + F_Q1 + c][i] * (Ezn - vx * Byn);
+    }
+    T s = -(acc[0] + acc[1] + acc[2]);
+    if (p.warm_T > 0.0) {
+        const T* n = S.a[b + F_N];
+        const int zp = zp_per(z, p.nz);
+        T dndz = (n[(size_t)x + (size_t)zp * nx] - n[izm]) / (T(2.0) * T(p.dz));
+        s -= T(p.warm_T) * dndz;
+    }
+    return s;
+}
+
+// kernel_ym_precession (YM_Fluid.cu:86-108) at real cell (x, z); out[0..2] = sQ^a.
+template <typename T>
+__device__ __forceinline__ void precession_at(const State<T>& S, int sp, int x, int z,
+                                              const KParams& p, T out[3]) {
+    const int nx = p.nx;
+    const int b = S_NA + sp * F_COUNT;
+    const int zm = zm_per(z, p.nz);
+    const size_t i = (size_t)x + (size_t)z * nx;
+    const size_t izm = (size_t)x + (size_t)zm * nx;
+    T n_safe = kmax(S.a[b + F_N][i], T(0.05));
+    T vz = S.a[b + F_PZ][i] / n_safe;
+    T a1 = T(0.5) * (S.a[S_AZ1][i] + S.a[S_AZ1][izm]);
+    T a2 = T(0.5) * (S.a[S_AZ2][i] + S.a[S_AZ2][izm]);
+    T a3 = T(0.5) * (S.a[S_AZ3][i] + S.a[S_AZ3][izm]);
+    T q1 = S.a[b + F_Q1][i], q2 = S.a[b + F_Q2][i], q3 = S.a[b + F_Q3][i];
+    T al = T(p.alpha);
+    out[0] = al * vz * (q2 * a3 - q3 * a2);
+    out[1] = al * vz * (q3 * a1 - q1 * a3);
+    out[2] = al * vz * (q1 * a2 - q2 * a1);
+}
+
+// kernel_ym_currents (YM_Physics.cu:9-60) at real cell (x, z) from fluid F.
+template <typename T>
+__device__ __forceinline__ void currents_at(const State<T>& F, int x, int z, const KParams& p,
+                                            T Jx[3], T Jz[3]) {
+    const int nx = p.nx;
+    const int xp = xp_src(x, p), zp = zp_per(z, p.nz);
+    const size_t i = (size_t)x + (size_t)z * nx;
+    const size_t ixp = (size_t)xp + (size_t)z * nx;
+    const size_t izp = (size_t)x + (size_t)zp * nx;
+    T vxr[2], vzt[2], nQx[2][3], nQz[2][3];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const int b = S_NA + s * F_COUNT;
+        const T* n = F.a[b + F_N];
+        const T* px = F.a[b + F_PX];
+        const T* pz = F.a[b + F_PZ];
+        T n0 = n[i], np = n[ixp], nzp = n[izp];
+        vxr[s] = T(0.5) * (px[i] / kmax(n0, T(1e-10)) + px[ixp] / kmax(np, T(1e-10)));
+        vzt[s] = T(0.5) * (pz[i] / kmax(n0, T(1e-10)) + pz[izp] / kmax(nzp, T(1e-10)));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const T* Q = F.a[b + F_Q1 + c];
+            T q0 = Q[i];
+            nQx[s][c] = T(0.5) * (n0 * q0 + np * Q[ixp]);
+            nQz[s][c] = T(0.5) * (n0 * q0 + nzp * Q[izp]);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        Jx[c] = -(nQx[0][c] * vxr[0] + nQx[1][c] * vxr[1]);
+        Jz[c] = -(nQz[0][c] * vzt[0] + nQz[1][c] * vzt[1]);
+    }
+}
+
+}  // namespace khs

@@ -1,0 +1,58 @@
+// khs_launch.h — host-callable launchers of the gfx950 kernels.
+#pragma once
+
+#include "khs_device.h"
+
+namespace khs {
+
+// step kernels (khs_step_kernels.hip)
+template <typename T>
+void launch_fct_x(const State<T>& S, const Stage<T>& out, const KParams& p, int sp0, int nsp,
+                  hipStream_t st);
+template <typename T>
+void launch_fct_z(const State<T>& S, const Stage<T>& X, const State<T>& O, const KParams& p,
+                  int sp0, int nsp, hipStream_t st);
+template <typename T>
+void launch_maxwell(const State<T>& S, const State<T>& O, const KParams& p, hipStream_t st);
+
+// filters (khs_aux_kernels.hip)
+constexpr int kMaxFilterArrays = 24;
+template <typename T>
+struct ArrayList {
+    T* a[kMaxFilterArrays];
+    int n;
+};
+// Per-column action of the sponge / xi_cut kernels: 0 none, 1 multiply by
+// decay[x], 2 zero.
+template <typename T>
+void launch_cell_filters(const ArrayList<T>& arrs, const double* decay, const int* action,
+                         const KParams& p, hipStream_t st);
+// Column DFT filter.  coef scratch: arrays * 2 * nmodes * nx doubles.
+//   project == 0: v -= sum_m (a_m c_m(z) + b_m s_m(z))   (reference band subtraction)
+//   project == 1: v  = sum_m (a_m c_m(z) + b_m s_m(z))   (keep-modes projection)
+// a_m = weight[m] * sum_z v c_m(z), b_m likewise with s_m; tables are [m][nz].
+template <typename T, typename TW>
+void launch_column_modes(const ArrayList<T>& arrs, const TW* ctab, const TW* stab,
+                         const TW* weight, int nmodes, int project, double* coef,
+                         const KParams& p, hipStream_t st);
+template <typename T>
+void launch_hyperdiffuse(const ArrayList<T>& arrs, double mu, const KParams& p, hipStream_t st);
+template <typename T>
+void launch_fill_zero(const ArrayList<T>& arrs, size_t n, hipStream_t st);
+
+// diagnostics
+constexpr int kRedBlocks = 1024;
+constexpr int kRedThreads = 256;
+// Energy: out[0] = sum, out[1] = nonfinite flag (>0 if NaN/Inf seen).
+template <typename T>
+void launch_energy(const State<T>& S, const KParams& p, double* partials, double* out,
+                   hipStream_t st);
+// Gauss residual sums: out[0..5] = sum G^2 (two signs), out[6] = sum (sum|E|)^2.
+template <typename T>
+void launch_gauss(const State<T>& S, const KParams& p, double* partials, double* out,
+                  hipStream_t st);
+// Derived per-cell array (ids of khs.h >= 32) evaluated from S into out (double).
+template <typename T>
+void launch_derived(const State<T>& S, int id, const KParams& p, double* out, hipStream_t st);
+
+}  // namespace khs

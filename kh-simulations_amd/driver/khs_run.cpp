@@ -1,0 +1,94 @@
+// khs_run — command-line driver with the reference's CLI: `khs_run <config.ini>`
+// (ymgpu2d/main_ym.cu:18-24).  It runs the whole simulation through the C ABI:
+// config parse, init (with optional eigenmode seed), output directory, energy
+// checks and halting every energy_stride steps, CSV snapshots every
+// export_stride steps (main_ym.cu:143-176, 244-491).
+#include <sys/stat.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <filesystem>
+#include <iomanip>
+#include <iostream>
+#include <string>
+
+#include "../../include/khs.h"
+
+namespace fs = std::filesystem;
+
+static int fail(const char* what) {
+    std::cerr << "ERROR: " << what << ": " << khs_last_error() << "\n";
+    return 1;
+}
+
+int main(int argc, char** argv) {
+    if (argc < 2) {
+        std::cerr << "Usage: khs_run <config.ini>\n"
+                  << "  Config is a key=value file with the ymgpu2d example.ini keys.\n";
+        return 1;
+    }
+    khs_config cfg;
+    if (khs_load_ini(argv[1], &cfg) != KHS_OK) return fail("config");
+
+    static const char* mode_names[] = {"NAB_LINEAR", "NAB_CIRC", "EMHD_KH", "NAB_DTANH",
+                                       "NAB_STEP", "NAB_TANH_COSAZ", "NAB_CIRC_AZ2"};
+    std::cout << "================================================================\n"
+              << " MI355X SU(2) Yang-Mills KH solver — config: " << argv[1] << "\n"
+              << " k_mode=" << cfg.k_mode << "  alpha_YM=" << cfg.alpha_YM
+              << "  perturb_amp=" << cfg.perturb_amp << "  V0=" << cfg.V0
+              << "  mode=" << mode_names[cfg.run_mode] << "\n"
+              << " NX=" << cfg.nx << "  NZ=" << cfg.nz << "  DX=" << cfg.dx << "  DZ=" << cfg.dz
+              << "  DT=" << cfg.dt << "  steps=" << cfg.total_steps
+              << "  precision=" << (cfg.precision == KHS_FP32 ? "fp32" : "fp64") << "\n"
+              << "================================================================\n";
+
+    khs_ctx* ctx = nullptr;
+    if (khs_create(&cfg, &ctx) != KHS_OK) return fail("create");
+
+    char dirbuf[1024];
+    if (khs_output_dir(&cfg, dirbuf, sizeof(dirbuf)) != KHS_OK) return fail("output dir");
+    std::string out_dir = dirbuf;
+    fs::create_directories(out_dir);
+    for (auto& e : fs::directory_iterator(out_dir)) {
+        auto nm = e.path().filename().string();
+        if (nm.rfind("ym_", 0) == 0 && e.path().extension() == ".csv") fs::remove(e.path());
+    }
+
+    if (khs_init(ctx, nullptr) != KHS_OK) return fail("init");
+    double E0 = 0.0;
+    khs_energy0(ctx, &E0);
+    if (khs_energy_csv_open(ctx, (out_dir + "/energy.csv").c_str()) != KHS_OK) return fail("energy.csv");
+    std::cout << " Initial energy: " << E0 << "\n";
+    if (cfg.export_stride > 0 && khs_write_snapshot(ctx, out_dir.c_str(), 0) != KHS_OK)
+        return fail("snapshot");
+
+    const int total = cfg.total_steps;
+    const int es = cfg.energy_stride, xs = cfg.export_stride;
+    int step = 0, halted = 0;
+    while (step < total && !halted) {
+        int next = total;
+        if (es > 0) next = std::min(next, (step / es + 1) * es);
+        if (xs > 0) next = std::min(next, (step / xs + 1) * xs);
+        if (khs_step(ctx, next - step, &halted) != KHS_OK) return fail("step");
+        khs_get_step(ctx, &step, nullptr);
+        if (es > 0 && step % es == 0) {
+            double E = 0.0;
+            khs_energy(ctx, &E);
+            double t = step * cfg.dt;
+            std::cout << " Step " << std::setw(7) << step << " | t=" << std::fixed << std::setprecision(1)
+                      << t << " | E/E0=" << std::setprecision(4) << E / E0 << "     \r" << std::flush;
+            if (halted) {
+                if (std::isnan(E) || std::isinf(E)) std::cout << "\n[HALT] NaN/Inf at t=" << t << "\n";
+                else std::cout << "\n[HALT] E/E0=" << E / E0 << " > " << cfg.energy_thresh << " at t=" << t << "\n";
+            }
+        }
+        if (!halted && xs > 0 && step % xs == 0 && khs_write_snapshot(ctx, out_dir.c_str(), step) != KHS_OK)
+            return fail("snapshot");
+    }
+    if (!halted) std::cout << "\n Run completed normally.\n";
+    khs_energy_csv_close(ctx);
+    khs_destroy(ctx);
+    return 0;
+}

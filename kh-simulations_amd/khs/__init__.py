@@ -1,0 +1,321 @@
+"""ctypes binding of libkhs, the MI355X (gfx950) Kelvin-Helmholtz time-step library.
+
+This is the Python surface the ymgpu2d analysis scripts use instead of parsing
+CSV snapshots: the same config keys (ymgpu2d/example.ini), the same operator
+boundaries (YM_FCT / YM_Physics / YM_Fluid / YM_Energy) and the same output
+files, driven through the C ABI declared in include/khs.h.
+
+    import khs
+    cfg = khs.Config.from_ini("run.ini")       # or khs.Config(nx_override=256, ...)
+    sim = khs.Simulation(cfg)
+    sim.init()
+    halted = sim.step(100)
+    By2 = sim.get("By2")                         # numpy (nz, nx), reference layout
+
+The library is loaded from this directory (built by kh-simulations_amd/build.py).
+There is no CPU fallback: without libkhs.so or without a GPU every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libkhs.so")
+
+KHS_OK, KHS_ERR_ARG, KHS_ERR_CONFIG, KHS_ERR_IO, KHS_ERR_DEVICE, KHS_ERR_STATE = 0, -1, -2, -3, -4, -5
+FP64, FP32 = 0, 1
+SPECIES_A, SPECIES_B = 0, 1
+BUF_CURRENT, BUF_STAGE, BUF_NEXT = 0, 1, 2
+
+STATE_NAMES = ["Ex1", "Ex2", "Ex3", "Ez1", "Ez2", "Ez3", "By1", "By2", "By3",
+               "Az1", "Az2", "Az3", "nA", "PxA", "PzA", "Q1A", "Q2A", "Q3A",
+               "nB", "PxB", "PzB", "Q1B", "Q2B", "Q3B"]
+DERIVED_NAMES = ["Jx1", "Jx2", "Jx3", "Jz1", "Jz2", "Jz3", "uxA", "uzA", "uxB", "uzB",
+                 "srcpxA", "srcpzA", "srcpxB", "srcpzB", "sQ1A", "sQ2A", "sQ3A",
+                 "sQ1B", "sQ2B", "sQ3B", "energy_density"]
+ARRAY_IDS = {n: i for i, n in enumerate(STATE_NAMES)}
+ARRAY_IDS.update({n: 32 + i for i, n in enumerate(DERIVED_NAMES)})
+ARRAY_IDS.update({"pxA": 13, "pzA": 14, "pxB": 19, "pzB": 20})
+
+
+class KhsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"khs error {code}: {msg}")
+        self.code = code
+
+
+class khs_config(ctypes.Structure):
+    """Mirror of struct khs_config (include/khs.h)."""
+    _fields_ = [
+        ("k_mode", ctypes.c_int), ("alpha_YM", ctypes.c_float), ("perturb_amp", ctypes.c_float),
+        ("run_mode", ctypes.c_int), ("V0", ctypes.c_float), ("xi_sponge", ctypes.c_float),
+        ("sigma_sponge", ctypes.c_float), ("xi_cut", ctypes.c_float), ("warm_T", ctypes.c_float),
+        ("freeze_override", ctypes.c_int), ("suppress_kz0", ctypes.c_int), ("hyp_diff", ctypes.c_float),
+        ("kz_suppress_max", ctypes.c_int), ("eps_override", ctypes.c_float), ("kz_suppress_hi", ctypes.c_int),
+        ("vz_edge_taper", ctypes.c_float), ("init_by1_eq", ctypes.c_int), ("nz_override", ctypes.c_int),
+        ("nx_override", ctypes.c_int), ("courant_override", ctypes.c_float), ("lz_override", ctypes.c_float),
+        ("lx_override", ctypes.c_float), ("target_tu", ctypes.c_float),
+        ("run_tag", ctypes.c_char * 128), ("seed_profile_file", ctypes.c_char * 512),
+        ("n_steps", ctypes.c_int), ("noise_amp", ctypes.c_double), ("noise_seed", ctypes.c_uint64),
+        ("kz_keep_max", ctypes.c_int), ("precision", ctypes.c_int),
+        ("energy_stride_override", ctypes.c_int), ("export_stride_override", ctypes.c_int),
+        ("nx", ctypes.c_int), ("nz", ctypes.c_int), ("periodic_x", ctypes.c_int), ("freeze_az1", ctypes.c_int),
+        ("lx", ctypes.c_double), ("lz", ctypes.c_double), ("dx", ctypes.c_double), ("dz", ctypes.c_double),
+        ("courant", ctypes.c_double), ("dt", ctypes.c_double),
+        ("alpha", ctypes.c_double), ("v0", ctypes.c_double), ("eps", ctypes.c_double),
+        ("c", ctypes.c_double), ("eps_0", ctypes.c_double),
+        ("total_steps", ctypes.c_int), ("export_stride", ctypes.c_int), ("energy_stride", ctypes.c_int),
+        ("energy_thresh", ctypes.c_double),
+    ]
+
+
+_lib = None
+
+
+def load_library(path=None):
+    """Load libkhs.so (raises ImportError with a build hint if absent)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise ImportError(f"{p} not found: build it with `python kh-simulations_amd/build.py`")
+    lib = ctypes.CDLL(p)
+    c_int, c_double, c_char_p, c_size_t, c_void_p = (ctypes.c_int, ctypes.c_double, ctypes.c_char_p,
+                                                      ctypes.c_size_t, ctypes.c_void_p)
+    P = ctypes.POINTER
+    cfgp = P(khs_config)
+    dp = P(c_double)
+    sigs = {
+        "khs_last_error": (c_char_p, []), "khs_version": (c_char_p, []), "khs_config_sizeof": (c_size_t, []),
+        "khs_config_defaults": (c_int, [cfgp]), "khs_config_set": (c_int, [cfgp, c_char_p, c_char_p]),
+        "khs_load_ini": (c_int, [c_char_p, cfgp]), "khs_derive": (c_int, [cfgp]),
+        "khs_output_dir": (c_int, [cfgp, c_char_p, c_size_t]),
+        "khs_create": (c_int, [cfgp, P(c_void_p)]), "khs_destroy": (c_int, [c_void_p]),
+        "khs_init": (c_int, [c_void_p, c_char_p]), "khs_array_id": (c_int, [c_char_p]),
+        "khs_get_array": (c_int, [c_void_p, c_int, dp, c_size_t]),
+        "khs_set_array": (c_int, [c_void_p, c_int, dp, c_size_t]),
+        "khs_get_array_buf": (c_int, [c_void_p, c_int, c_int, dp, c_size_t]),
+        "khs_fct_x_sweep": (c_int, [c_void_p, c_int]), "khs_fct_z_sweep": (c_int, [c_void_p, c_int]),
+        "khs_maxwell": (c_int, [c_void_p]), "khs_filters": (c_int, [c_void_p]), "khs_commit": (c_int, [c_void_p]),
+        "khs_step": (c_int, [c_void_p, c_int, P(c_int)]),
+        "khs_get_step": (c_int, [c_void_p, P(c_int), dp]),
+        "khs_energy": (c_int, [c_void_p, dp]), "khs_energy0": (c_int, [c_void_p, dp]),
+        "khs_gauss": (c_int, [c_void_p, dp]),
+        "khs_write_snapshot": (c_int, [c_void_p, c_char_p, c_int]),
+        "khs_energy_csv_open": (c_int, [c_void_p, c_char_p]),
+        "khs_energy_csv_row": (c_int, [c_void_p, c_int, c_double]),
+        "khs_energy_csv_close": (c_int, [c_void_p]),
+        "khs_sync": (c_int, [c_void_p]), "khs_timing_enable": (c_int, [c_void_p, c_int]),
+        "khs_timing_reset": (c_int, [c_void_p]),
+        "khs_kernel_times": (c_int, [c_void_p, dp, P(c_int), c_int]),
+        "khs_kernel_name": (c_char_p, [c_int]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.khs_config_sizeof() != ctypes.sizeof(khs_config):
+        raise ImportError("khs_config layout mismatch between binding and library")
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def exported_symbols():
+    """Names this binding binds (all entry points of include/khs.h)."""
+    return ["khs_last_error", "khs_version", "khs_config_sizeof", "khs_config_defaults", "khs_config_set",
+            "khs_load_ini", "khs_derive", "khs_output_dir", "khs_create", "khs_destroy", "khs_init",
+            "khs_array_id", "khs_get_array", "khs_set_array", "khs_get_array_buf", "khs_fct_x_sweep", "khs_fct_z_sweep",
+            "khs_maxwell", "khs_filters", "khs_commit", "khs_step", "khs_get_step", "khs_energy",
+            "khs_energy0", "khs_gauss", "khs_write_snapshot", "khs_energy_csv_open", "khs_energy_csv_row",
+            "khs_energy_csv_close", "khs_sync", "khs_timing_enable", "khs_timing_reset", "khs_kernel_times",
+            "khs_kernel_name"]
+
+
+def _check(rc):
+    if rc != KHS_OK:
+        raise KhsError(rc, _lib.khs_last_error().decode(errors="replace"))
+
+
+class Config:
+    """A run configuration: reference .ini keys plus extension keys."""
+
+    def __init__(self, **keys):
+        load_library()
+        self._c = khs_config()
+        _check(_lib.khs_config_defaults(ctypes.byref(self._c)))
+        for k, v in keys.items():
+            self.set(k, v)
+        if keys:
+            self.derive()
+
+    @classmethod
+    def from_ini(cls, path):
+        self = cls()
+        _check(_lib.khs_load_ini(os.fsencode(path), ctypes.byref(self._c)))
+        return self
+
+    def set(self, key, value):
+        _check(_lib.khs_config_set(ctypes.byref(self._c), key.encode(), str(value).encode()))
+        return self
+
+    def derive(self):
+        _check(_lib.khs_derive(ctypes.byref(self._c)))
+        return self
+
+    def output_dir(self):
+        buf = ctypes.create_string_buffer(1024)
+        _check(_lib.khs_output_dir(ctypes.byref(self._c), buf, 1024))
+        return buf.value.decode()
+
+    @property
+    def struct(self):
+        return self._c
+
+    def __getattr__(self, name):
+        if name.startswith("_"):
+            raise AttributeError(name)
+        v = getattr(self._c, name)
+        return v.decode() if isinstance(v, bytes) else v
+
+
+class Simulation:
+    """One device context (single GPU, single host thread)."""
+
+    def __init__(self, config):
+        load_library()
+        self.config = config if isinstance(config, Config) else Config(**config)
+        h = ctypes.c_void_p()
+        _check(_lib.khs_create(ctypes.byref(self.config.struct), ctypes.byref(h)))
+        self._h = h
+        self.nx, self.nz = self.config.nx, self.config.nz
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.khs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- state ----
+    def init(self, seed_path=None):
+        _check(_lib.khs_init(self._h, os.fsencode(seed_path) if seed_path else None))
+
+    @staticmethod
+    def array_id(name):
+        if isinstance(name, int):
+            return name
+        if name not in ARRAY_IDS:
+            raise KeyError(name)
+        return ARRAY_IDS[name]
+
+    def get(self, name):
+        out = np.empty((self.nz, self.nx), dtype=np.float64)
+        _check(_lib.khs_get_array(self._h, self.array_id(name),
+                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), out.size))
+        return out
+
+    def get_buf(self, buf, name):
+        """Read a state array from BUF_CURRENT / BUF_STAGE / BUF_NEXT."""
+        out = np.empty((self.nz, self.nx), dtype=np.float64)
+        _check(_lib.khs_get_array_buf(self._h, int(buf), self.array_id(name),
+                                      out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), out.size))
+        return out
+
+    def set(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.nz, self.nx)
+        _check(_lib.khs_set_array(self._h, self.array_id(name),
+                                  a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), a.size))
+
+    def state(self):
+        return {n: self.get(n) for n in STATE_NAMES}
+
+    # ---- operators ----
+    def fct_x_sweep(self, species):
+        _check(_lib.khs_fct_x_sweep(self._h, species))
+
+    def fct_z_sweep(self, species):
+        _check(_lib.khs_fct_z_sweep(self._h, species))
+
+    def maxwell(self):
+        _check(_lib.khs_maxwell(self._h))
+
+    def filters(self):
+        _check(_lib.khs_filters(self._h))
+
+    def commit(self):
+        _check(_lib.khs_commit(self._h))
+
+    def step(self, n=1):
+        halted = ctypes.c_int(0)
+        _check(_lib.khs_step(self._h, int(n), ctypes.byref(halted)))
+        return bool(halted.value)
+
+    @property
+    def step_count(self):
+        s = ctypes.c_int()
+        _check(_lib.khs_get_step(self._h, ctypes.byref(s), None))
+        return s.value
+
+    # ---- diagnostics ----
+    def energy(self):
+        e = ctypes.c_double()
+        _check(_lib.khs_energy(self._h, ctypes.byref(e)))
+        return e.value
+
+    @property
+    def energy0(self):
+        e = ctypes.c_double()
+        _check(_lib.khs_energy0(self._h, ctypes.byref(e)))
+        return e.value
+
+    def gauss(self):
+        out = (ctypes.c_double * 7)()
+        _check(_lib.khs_gauss(self._h, out))
+        v = list(out)
+        return {"rms_ref": v[0:3], "rms_cont": v[3:6], "Etot_scale": v[6]}
+
+    # ---- files ----
+    def write_snapshot(self, directory, step):
+        _check(_lib.khs_write_snapshot(self._h, os.fsencode(directory), int(step)))
+
+    def energy_csv_open(self, path):
+        _check(_lib.khs_energy_csv_open(self._h, os.fsencode(path)))
+
+    def energy_csv_close(self):
+        _check(_lib.khs_energy_csv_close(self._h))
+
+    # ---- timing ----
+    def sync(self):
+        _check(_lib.khs_sync(self._h))
+
+    def timing(self, on=True):
+        _check(_lib.khs_timing_enable(self._h, 1 if on else 0))
+
+    def timing_reset(self):
+        _check(_lib.khs_timing_reset(self._h))
+
+    def kernel_times(self):
+        n = 8
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_int * n)()
+        _check(_lib.khs_kernel_times(self._h, ms, cnt, n))
+        out = {}
+        for i in range(n):
+            nm = _lib.khs_kernel_name(i)
+            if nm is None:
+                break
+            out[nm.decode()] = (ms[i], cnt[i])
+        return out

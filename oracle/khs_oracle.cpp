@@ -1,0 +1,902 @@
+// khs_oracle.cpp — TEST INFRASTRUCTURE, NOT PRODUCT CODE.
+//
+// FP64 CPU (OpenMP) restatement of the ymgpu2d / emhdgpu2d time step of
+// Obasho10/kh-simulations.  It is the parity oracle for the HIP library and the
+// timed CPU baseline ("port").  Only tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py may load it.  It shares no code with the product.
+//
+// It follows the reference structure operator by operator, with the reference's
+// expression order, so that the GPU's FP64 results can be compared bit for bit
+// where the GPU keeps the same order (stencils), and to a stated tolerance where
+// it sums in another order (reductions).  Citations are path:line in the
+// reference.  Deviations that are deliberate:
+//   * hyperdiffusion is evaluated out of place (Jacobi); the reference's
+//     in-place kernel is racy (ymgpu2d/YM_Physics.cu:241-242);
+//   * float twiddles of the DFT band filter are (float)cos((double)phase),
+//     i.e. a correctly rounded float cosine, where CUDA's cosf is within 2 ulp;
+//   * min/max are explicit selects so that the sign of zero is defined;
+//   * the z-mean removal is evaluated at any NZ (the reference's launch silently
+//     fails for NZ >= 512 in FP64, main_ym.cu:382-383).
+// Compiled with -ffp-contract=off so that no multiply-add is fused.
+//
+// Parity pinning of each function (which test pins it) is listed in the
+// header comment of tests/oracle_ref.py.
+
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+
+extern "C" {
+
+struct OrcParams {
+    int nx, nz;
+    int periodic_x, freeze_az1;
+    int suppress_kz0, kz_suppress_max, kz_suppress_hi, kz_keep_max;
+    double dx, dz, dt, c, eps0, alpha, V0, eps;
+    double xi_sponge, sigma_sponge, xi_cut, warm_T, hyp_diff;
+};
+
+}  // extern "C"
+
+namespace {
+
+enum {
+    EX1 = 0, EX2, EX3, EZ1, EZ2, EZ3, BY1, BY2, BY3, AZ1, AZ2, AZ3,
+    NA, PXA, PZA, Q1A, Q2A, Q3A, NB, PXB, PZB, Q1B, Q2B, Q3B,
+    NSTATE,
+    JX1 = 32, JX2, JX3, JZ1, JZ2, JZ3,
+    UXA, UZA, UXB, UZB,
+    SRCPXA, SRCPZA, SRCPXB, SRCPZB,
+    SQ1A, SQ2A, SQ3A, SQ1B, SQ2B, SQ3B,
+    EDENS,
+    NIDS
+};
+
+inline double dmin(double a, double b) { return (b < a) ? b : a; }
+inline double dmax(double a, double b) { return (a < b) ? b : a; }
+
+struct Oracle {
+    OrcParams p;
+    size_t N;
+    std::vector<double> a[NIDS];
+    double* A(int id) { return a[id].data(); }
+
+    explicit Oracle(const OrcParams& pp) : p(pp) {
+        N = (size_t)p.nx * p.nz;
+        for (int i = 0; i < NIDS; ++i)
+            if (i < NSTATE || i >= JX1) a[i].assign(N, 0.0);
+    }
+    inline size_t I(int x, int z) const { return (size_t)x + (size_t)z * p.nx; }
+};
+
+// ---------------------------------------------------------------------------
+// FCT 1-D sweep over a contiguous row of length n with clamped (replicated)
+// ends: the semantics of fct_sweep_kernel (ymgpu2d/FCT_Sweeps.cu:39-177) with
+// its block-halo loads (:68-82) resolved to "every index is clamped to
+// [0, n-1]".  Coefficients: compute_eps / compute_nu_mu
+// (FCT_DeviceMath.cuh:38-63); limiter apply_fct_limiter (:108-125).
+// ---------------------------------------------------------------------------
+inline void nu_mu(double eps, double& nu, double& mu) {
+    double scrh = dmin(1.0 / 6.0, std::fabs(eps));
+    scrh = (1.0 / 3.0) * scrh * scrh;
+    nu = (1.0 / 6.0) + (1.0 / 3.0) * (eps * eps - scrh * scrh);
+    mu = 0.25 - 0.5 * nu;
+}
+
+inline double limiter(double flx, double dm1, double d0, double dp1) {
+    double fsgn = std::copysign(1.0, d0);
+    double term = fsgn * dm1;
+    double terp = fsgn * dp1;
+    double fa = std::fabs(flx);
+    double lim = dmax(0.0, dmin(term, dmin(fa, terp)));
+    return fsgn * lim;
+}
+
+struct FctScratch {
+    std::vector<double> U, V, S, LOR, LNR, FLX;
+    void resize(int n) {
+        U.resize(n + 6); V.resize(n + 6); S.resize(n + 6);
+        LOR.resize(n + 6); LNR.resize(n + 6); FLX.resize(n + 6);
+    }
+};
+
+void fct_row(int n, const double* u, const double* v, const double* s,
+             double* out, double dx, double dt, FctScratch& w) {
+    w.resize(n);
+    double* U = w.U.data(); double* V = w.V.data(); double* S = w.S.data();
+    double* LOR = w.LOR.data(); double* LNR = w.LNR.data(); double* FLX = w.FLX.data();
+    for (int e = 0; e < n + 6; ++e) {
+        int j = std::min(std::max(e - 3, 0), n - 1);
+        U[e] = u[j]; V[e] = v[j]; S[e] = s ? s[j] : 0.0;
+    }
+    const double dtdx = dt / dx;
+    auto epsf = [&](int l, int r) { double vi = 0.5 * (U[l] + U[r]); return vi * dtdx; };
+    // LORHOT / LNRHOT at ext positions 1..n+4 (FCT_Sweeps.cu:88-103)
+    for (int e = 1; e <= n + 4; ++e) {
+        double eL = epsf(e - 1, e), eR = epsf(e, e + 1);
+        double nuL, muL, nuR, muR;
+        nu_mu(eL, nuL, muL); nu_mu(eR, nuR, muR);
+        double fL = 0.5 * eL * (V[e - 1] + V[e]);
+        double fR = 0.5 * eR * (V[e] + V[e + 1]);
+        double dL = nuL * (V[e] - V[e - 1]);
+        double dR = nuR * (V[e + 1] - V[e]);
+        LOR[e] = V[e] + dt * S[e] - (fR - fL);
+        LNR[e] = LOR[e] + (dR - dL);
+    }
+    // antidiffusive flux at interface e+1/2 uses transported LORHOT (:139)
+    for (int e = 1; e <= n + 3; ++e) {
+        double eR = epsf(e, e + 1);
+        double nuR, muR;
+        nu_mu(eR, nuR, muR);
+        FLX[e] = muR * (LOR[e + 1] - LOR[e]);
+    }
+    // limiter with differences of LNRHOT (:155-162)
+    for (int i = 0; i < n; ++i) {
+        int e = i + 3;
+        double dm1 = LNR[e] - LNR[e - 1];
+        double d0 = LNR[e + 1] - LNR[e];
+        double dp1 = LNR[e + 2] - LNR[e + 1];
+        double fl = limiter(FLX[e - 1], LNR[e - 1] - LNR[e - 2], dm1, d0);
+        double fr = limiter(FLX[e], dm1, d0, dp1);
+        out[i] = LNR[e] - (fr - fl);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fluid kernels (ymgpu2d/YM_Fluid.cu)
+// ---------------------------------------------------------------------------
+void update_velocities(Oracle& o, int s) {        // YM_Fluid.cu:3-11
+    const double* n = o.A(s ? NB : NA);
+    const double* px = o.A(s ? PXB : PXA);
+    const double* pz = o.A(s ? PZB : PZA);
+    double* ux = o.A(s ? UXB : UXA);
+    double* uz = o.A(s ? UZB : UZA);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)o.N; ++i) {
+        double safe_n = dmax(n[i], 0.05);
+        ux[i] = px[i] / safe_n;
+        uz[i] = pz[i] / safe_n;
+    }
+}
+
+// x-sweep of one species.  Walled: clamped ends (YM_FCT.cu:69-84).  Periodic:
+// 3-cell ghost padding then a clamped sweep over the padded row
+// (YM_FCT.cu:7-28, 86-130).  px gets +dt*src.px; n, pz, Q1-3 are unsourced.
+void x_sweep(Oracle& o, int s) {
+    const int nx = o.p.nx, nz = o.p.nz;
+    const int base = s ? NB : NA;
+    const double* ux = o.A(s ? UXB : UXA);
+    const double* spx = o.A(s ? SRCPXB : SRCPXA);
+    std::vector<double> old[6];
+    for (int v = 0; v < 6; ++v) old[v] = o.a[base + v];   // copy_ym_fluid (YM_Fluid.cu:134-143)
+    const int H = 3;
+#pragma omp parallel
+    {
+        FctScratch w;
+        std::vector<double> pu(nx + 2 * H), pv(nx + 2 * H), ps(nx + 2 * H), pn(nx + 2 * H);
+#pragma omp for schedule(static)
+        for (int z = 0; z < nz; ++z) {
+            size_t r = (size_t)z * nx;
+            for (int v = 0; v < 6; ++v) {
+                const double* src = (v == 1) ? spx + r : nullptr;
+                double* dst = o.A(base + v) + r;
+                if (!o.p.periodic_x) {
+                    fct_row(nx, ux + r, old[v].data() + r, src, dst, o.p.dx, o.p.dt, w);
+                } else {
+                    const int np = nx + 2 * H;
+                    auto pack = [&](const double* in, double* outp) {
+                        for (int x = 0; x < nx; ++x) outp[x + H] = in[x];
+                        for (int x = 0; x < H; ++x) {
+                            outp[x] = in[nx - H + x];
+                            outp[nx + H + x] = in[x];
+                        }
+                    };
+                    pack(ux + r, pu.data());
+                    pack(old[v].data() + r, pv.data());
+                    if (src) pack(src, ps.data());
+                    fct_row(np, pu.data(), pv.data(), src ? ps.data() : nullptr, pn.data(),
+                            o.p.dx, o.p.dt, w);
+                    for (int x = 0; x < nx; ++x) dst[x] = pn[x + H];
+                }
+            }
+        }
+    }
+}
+
+// z-sweep of one species (YM_FCT.cu:93-135): each x column is gathered into
+// a contiguous row (the reference's transpose) and swept with clamped ends
+// using the stale z-velocity from before the x-sweep.  pz gets +dt*src.pz,
+// Q^a gets +dt*sQ^a; n and px are unsourced.
+void z_sweep(Oracle& o, int s) {
+    const int nx = o.p.nx, nz = o.p.nz;
+    const int base = s ? NB : NA;
+    const double* uz = o.A(s ? UZB : UZA);
+    const double* srcs[6] = {nullptr, nullptr, o.A(s ? SRCPZB : SRCPZA),
+                             o.A(s ? SQ1B : SQ1A), o.A(s ? SQ2B : SQ2A), o.A(s ? SQ3B : SQ3A)};
+    std::vector<double> old[6];
+    for (int v = 0; v < 6; ++v) old[v] = o.a[base + v];
+#pragma omp parallel
+    {
+        FctScratch w;
+        std::vector<double> cu(nz), cv(nz), cs(nz), cout_(nz);
+#pragma omp for schedule(static)
+        for (int x = 0; x < nx; ++x) {
+            for (int z = 0; z < nz; ++z) cu[z] = uz[o.I(x, z)];
+            for (int v = 0; v < 6; ++v) {
+                for (int z = 0; z < nz; ++z) cv[z] = old[v][o.I(x, z)];
+                if (srcs[v]) for (int z = 0; z < nz; ++z) cs[z] = srcs[v][o.I(x, z)];
+                fct_row(nz, cu.data(), cv.data(), srcs[v] ? cs.data() : nullptr, cout_.data(),
+                        o.p.dz, o.p.dt, w);
+                double* dst = o.A(base + v);
+                for (int z = 0; z < nz; ++z) dst[o.I(x, z)] = cout_[z];
+            }
+        }
+    }
+}
+
+void enforce_boundaries(Oracle& o) {               // YM_Fluid.cu:13-30
+    const int nx = o.p.nx, nz = o.p.nz;
+    double *nA = o.A(NA), *nB = o.A(NB);
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < nz; ++z)
+        for (int x = 0; x < nx; ++x) {
+            size_t i = o.I(x, z);
+            if (nA[i] < 0.05) nA[i] = 0.05;
+            if (nB[i] < 0.05) nB[i] = 0.05;
+            if (!o.p.periodic_x && (x == 0 || x == nx - 1)) {
+                o.A(PXA)[i] = 0.0; o.A(PXB)[i] = 0.0;
+                o.A(EZ1)[i] = 0.0; o.A(EZ2)[i] = 0.0; o.A(EZ3)[i] = 0.0;
+            }
+        }
+}
+
+// Lorentz source (YM_Fluid.cu:32-84), one species.
+void lorentz(Oracle& o, int s) {
+    const int nx = o.p.nx, nz = o.p.nz;
+    const int b = s ? NB : NA;
+    const double *n = o.A(b), *px = o.A(b + 1), *pz = o.A(b + 2);
+    const double *Q1 = o.A(b + 3), *Q2 = o.A(b + 4), *Q3 = o.A(b + 5);
+    double* spx = o.A(s ? SRCPXB : SRCPXA);
+    double* spz = o.A(s ? SRCPZB : SRCPZA);
+    const double *Ex[3] = {o.A(EX1), o.A(EX2), o.A(EX3)};
+    const double *Ez[3] = {o.A(EZ1), o.A(EZ2), o.A(EZ3)};
+    const double *By[3] = {o.A(BY1), o.A(BY2), o.A(BY3)};
+    const int per = o.p.periodic_x;
+    const double warm_T = o.p.warm_T;
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < nz; ++z)
+        for (int x = 0; x < nx; ++x) {
+            size_t i = o.I(x, z);
+            int xm = per ? (x + nx - 1) % nx : std::max(0, x - 1);
+            int zm = (z == 0) ? nz - 1 : z - 1;
+            double n_safe = dmax(n[i], 0.05);
+            double vx = px[i] / n_safe;
+            double vz = pz[i] / n_safe;
+            double Exn[3], Ezn[3], Byn[3];
+            for (int c = 0; c < 3; ++c) {
+                Exn[c] = 0.5 * (Ex[c][i] + Ex[c][o.I(xm, z)]);
+                Ezn[c] = 0.5 * (Ez[c][i] + Ez[c][o.I(x, zm)]);
+                Byn[c] = 0.25 * (By[c][i] + By[c][o.I(xm, z)] + By[c][o.I(x, zm)] + By[c][o.I(xm, zm)]);
+            }
+            double q1 = Q1[i], q2 = Q2[i], q3 = Q3[i];
+            double sx = -(q1 * (Exn[0] + vz * Byn[0]) + q2 * (Exn[1] + vz * Byn[1]) + q3 * (Exn[2] + vz * Byn[2]));
+            double sz = -(q1 * (Ezn[0] - vx * Byn[0]) + q2 * (Ezn[1] - vx * Byn[1]) + q3 * (Ezn[2] - vx * Byn[2]));
+            if (warm_T > 0.0) {
+                int xp = per ? (x + 1) % nx : std::min(nx - 1, x + 1);
+                int xm2 = per ? (x + nx - 1) % nx : std::max(0, x - 1);
+                int zp = (z == nz - 1) ? 0 : z + 1;
+                int zm2 = (z == 0) ? nz - 1 : z - 1;
+                double dndx = (n[o.I(xp, z)] - n[o.I(xm2, z)]) / (2.0 * o.p.dx);
+                double dndz = (n[o.I(x, zp)] - n[o.I(x, zm2)]) / (2.0 * o.p.dz);
+                sx -= warm_T * dndx;
+                sz -= warm_T * dndz;
+            }
+            spx[i] = sx;
+            spz[i] = sz;
+        }
+}
+
+// Colour precession source (YM_Fluid.cu:86-108), one species.
+void precession(Oracle& o, int s) {
+    const int nx = o.p.nx, nz = o.p.nz;
+    const int b = s ? NB : NA;
+    const double *n = o.A(b), *pz = o.A(b + 2);
+    const double *Q1 = o.A(b + 3), *Q2 = o.A(b + 4), *Q3 = o.A(b + 5);
+    double* s1 = o.A(s ? SQ1B : SQ1A);
+    double* s2 = o.A(s ? SQ2B : SQ2A);
+    double* s3 = o.A(s ? SQ3B : SQ3A);
+    const double *Az1 = o.A(AZ1), *Az2 = o.A(AZ2), *Az3 = o.A(AZ3);
+    const double al = o.p.alpha;
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < nz; ++z)
+        for (int x = 0; x < nx; ++x) {
+            size_t i = o.I(x, z);
+            int zm = (z == 0) ? nz - 1 : z - 1;
+            double n_safe = dmax(n[i], 0.05);
+            double vz = pz[i] / n_safe;
+            double a1 = 0.5 * (Az1[i] + Az1[o.I(x, zm)]);
+            double a2 = 0.5 * (Az2[i] + Az2[o.I(x, zm)]);
+            double a3 = 0.5 * (Az3[i] + Az3[o.I(x, zm)]);
+            double q1 = Q1[i], q2 = Q2[i], q3 = Q3[i];
+            // su2_cross1/2/3 (YM_Physics.cuh:119-130)
+            s1[i] = al * vz * (q2 * a3 - q3 * a2);
+            s2[i] = al * vz * (q3 * a1 - q1 * a3);
+            s3[i] = al * vz * (q1 * a2 - q2 * a1);
+        }
+}
+
+void sources(Oracle& o) { lorentz(o, 0); lorentz(o, 1); precession(o, 0); precession(o, 1); }
+
+// ---------------------------------------------------------------------------
+// Maxwell (ymgpu2d/YM_Physics.cu)
+// ---------------------------------------------------------------------------
+void currents(Oracle& o) {                          // YM_Physics.cu:9-60
+    const int nx = o.p.nx, nz = o.p.nz, per = o.p.periodic_x;
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < nz; ++z)
+        for (int x = 0; x < nx; ++x) {
+            size_t i = o.I(x, z);
+            int xp = per ? (x + 1) % nx : std::min(nx - 1, x + 1);
+            int zp = (z == nz - 1) ? 0 : z + 1;
+            size_t ixp = o.I(xp, z), izp = o.I(x, zp);
+            double vxr[2], vzt[2], nQx[2][3], nQz[2][3];
+            for (int s = 0; s < 2; ++s) {
+                int b = s ? NB : NA;
+                const double *n = o.A(b), *px = o.A(b + 1), *pz = o.A(b + 2);
+                double n0 = n[i], np = n[ixp], nzp = n[izp];
+                vxr[s] = 0.5 * (px[i] / dmax(n0, 1e-10) + px[ixp] / dmax(np, 1e-10));
+                vzt[s] = 0.5 * (pz[i] / dmax(n0, 1e-10) + pz[izp] / dmax(nzp, 1e-10));
+                for (int c = 0; c < 3; ++c) {
+                    const double* Q = o.A(b + 3 + c);
+                    nQx[s][c] = 0.5 * (n0 * Q[i] + np * Q[ixp]);
+                    nQz[s][c] = 0.5 * (n0 * Q[i] + nzp * Q[izp]);
+                }
+            }
+            for (int c = 0; c < 3; ++c) {
+                o.A(JX1 + c)[i] = -(nQx[0][c] * vxr[0] + nQx[1][c] * vxr[1]);
+                o.A(JZ1 + c)[i] = -(nQz[0][c] * vzt[0] + nQz[1][c] * vzt[1]);
+            }
+        }
+}
+
+void ampere(Oracle& o) {                            // YM_Physics.cu:70-113
+    const int nx = o.p.nx, nz = o.p.nz, per = o.p.periodic_x;
+    const double c2 = o.p.c * o.p.c, dt = o.p.dt, aYM = o.p.alpha;
+    // Ex/Ez at (x,z) depend on By/Az at (x,z),(x,z-1),(x-1,z) which this kernel
+    // does not write, so an in-place loop is race free.
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < nz; ++z)
+        for (int x = 0; x < nx; ++x) {
+            size_t i = o.I(x, z);
+            int zm = (z == 0) ? nz - 1 : z - 1;
+            int xm = per ? (x + nx - 1) % nx : (x == 0 ? 0 : x - 1);
+            double By[3], Byzm[3], Byxm[3], Az[3];
+            for (int c = 0; c < 3; ++c) {
+                By[c] = o.A(BY1 + c)[i];
+                Byzm[c] = o.A(BY1 + c)[o.I(x, zm)];
+                Byxm[c] = o.A(BY1 + c)[o.I(xm, z)];
+                Az[c] = o.A(AZ1 + c)[i];
+            }
+            double cr[3] = {Az[1] * By[2] - Az[2] * By[1],
+                            Az[2] * By[0] - Az[0] * By[2],
+                            Az[0] * By[1] - Az[1] * By[0]};
+            for (int c = 0; c < 3; ++c) {
+                double dBdz = (By[c] - Byzm[c]) / o.p.dz;
+                double* Ex = o.A(EX1 + c);
+                Ex[i] += dt * (c2 * (-dBdz - aYM * cr[c]) - o.A(JX1 + c)[i] / o.p.eps0);
+            }
+            for (int c = 0; c < 3; ++c) {
+                double dBdx = (By[c] - Byxm[c]) / o.p.dx;
+                double* Ez = o.A(EZ1 + c);
+                Ez[i] += dt * (c2 * dBdx - o.A(JZ1 + c)[i] / o.p.eps0);
+            }
+        }
+}
+
+void potential(Oracle& o) {                         // YM_Physics.cu:589-599
+    const double dt = o.p.dt;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)o.N; ++i) {
+        if (!o.p.freeze_az1) o.A(AZ1)[i] -= dt * o.A(EZ1)[i];
+        o.A(AZ2)[i] -= dt * o.A(EZ2)[i];
+        o.A(AZ3)[i] -= dt * o.A(EZ3)[i];
+    }
+}
+
+void faraday(Oracle& o) {                           // YM_Physics.cu:120-151
+    const int nx = o.p.nx, nz = o.p.nz, per = o.p.periodic_x;
+    const double dt = o.p.dt, aYM = o.p.alpha;
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < nz; ++z)
+        for (int x = 0; x < nx; ++x) {
+            size_t i = o.I(x, z);
+            int zp = (z == nz - 1) ? 0 : z + 1;
+            int xp = per ? (x + 1) % nx : std::min(nx - 1, x + 1);
+            double Az[3], Ex[3], dEx[3], dEz[3];
+            for (int c = 0; c < 3; ++c) {
+                const double* ex = o.A(EX1 + c);
+                const double* ez = o.A(EZ1 + c);
+                dEx[c] = (ex[o.I(x, zp)] - ex[i]) / o.p.dz;
+                dEz[c] = (ez[o.I(xp, z)] - ez[i]) / o.p.dx;
+                Az[c] = o.A(AZ1 + c)[i];
+                Ex[c] = ex[i];
+            }
+            double cr[3] = {Az[1] * Ex[2] - Az[2] * Ex[1],
+                            Az[2] * Ex[0] - Az[0] * Ex[2],
+                            Az[0] * Ex[1] - Az[1] * Ex[0]};
+            for (int c = 0; c < 3; ++c)
+                o.A(BY1 + c)[i] -= dt * (dEx[c] - dEz[c] - aYM * cr[c]);
+        }
+}
+
+// ---------------------------------------------------------------------------
+// Filters
+// ---------------------------------------------------------------------------
+const int kCol23[12] = {BY2, BY3, EX2, EX3, EZ2, EZ3, AZ2, AZ3, Q2A, Q3A, Q2B, Q3B};
+// the 15 arrays of kernel_ym_subtract_zmean / _kz_range (YM_Physics.cu:412-430, 505-511)
+const int kFilt15[15] = {BY1, EX1, EZ1, BY2, BY3, EX2, EX3, EZ2, EZ3, AZ2, AZ3, Q2A, Q3A, Q2B, Q3B};
+
+double xi_of(const Oracle& o, int x) {
+    double x_c = (o.p.nx * o.p.dx) * 0.5;
+    return std::fabs((x * o.p.dx - x_c) / o.p.eps);
+}
+
+void sponge(Oracle& o) {                            // YM_Physics.cu:161-182
+#pragma omp parallel for schedule(static)
+    for (int x = 0; x < o.p.nx; ++x) {
+        double xi = xi_of(o, x);
+        if (xi <= o.p.xi_sponge) continue;
+        double sigma = o.p.sigma_sponge * (xi - o.p.xi_sponge);
+        double decay = std::exp(-sigma * o.p.dt);
+        for (int z = 0; z < o.p.nz; ++z)
+            for (int k = 0; k < 12; ++k) o.A(kCol23[k])[o.I(x, z)] *= decay;
+    }
+}
+
+void xicut(Oracle& o) {                             // YM_Physics.cu:192-210
+#pragma omp parallel for schedule(static)
+    for (int x = 0; x < o.p.nx; ++x) {
+        if (xi_of(o, x) <= o.p.xi_cut) continue;
+        for (int z = 0; z < o.p.nz; ++z)
+            for (int k = 0; k < 12; ++k) o.A(kCol23[k])[o.I(x, z)] = 0.0;
+    }
+}
+
+// z-mean removal (YM_Physics.cu:272-327): shared-memory tree sum with strides
+// nz/2, nz/4, ..., 1 (requires a power-of-two NZ, YM_Config.cu:120-123).
+void zmean(Oracle& o) {
+    const int nz = o.p.nz;
+#pragma omp parallel
+    {
+        std::vector<double> buf(nz);
+#pragma omp for schedule(static)
+        for (int x = 0; x < o.p.nx; ++x)
+            for (int k = 0; k < 15; ++k) {
+                double* v = o.A(kFilt15[k]);
+                for (int z = 0; z < nz; ++z) buf[z] = v[o.I(x, z)];
+                for (int stride = nz >> 1; stride > 0; stride >>= 1)
+                    for (int z = 0; z < stride; ++z) buf[z] += buf[z + stride];
+                double inv_nz = 1.0 / (double)nz;
+                double m = buf[0] * inv_nz;
+                for (int z = 0; z < nz; ++z) v[o.I(x, z)] -= m;
+            }
+    }
+}
+
+// Float twiddles of the DFT band filter (YM_Physics.cu:387-388):
+// phase = 2.0f*(float)M_PI*m*z/(float)nz in float arithmetic.
+float band_phase(int m, int z, int nz) {
+    float ph = 2.0f * (float)M_PI;
+    ph = ph * (float)m;
+    ph = ph * (float)z;
+    ph = ph / (float)nz;
+    return ph;
+}
+
+// One column, one array: subtract DFT modes lo..hi with the reference's
+// reduction order: per 32-lane warp a shuffle-down tree (offsets 16..1), then a
+// serial sum over warps; coefficient = sum * (2.0f/(float)nz)
+// (YM_Physics.cu:349-455; kernel_fluid_*_subtract_kz_range :461-584 are the
+// same algorithm).  Requires nz % 32 == 0 (YM_Config.cu:124-127).
+void kz_band_column(double* v, int nz, int lo, int hi, std::vector<double>& tmp) {
+    std::vector<double> delta(nz, 0.0);
+    const int nw = nz / 32;
+    tmp.resize(2 * nz);
+    for (int m = lo; m <= hi; ++m) {
+        double sc = 0.0, ss = 0.0;
+        for (int w = 0; w < nw; ++w) {
+            double* ar = tmp.data();
+            double* br = tmp.data() + 32;
+            for (int l = 0; l < 32; ++l) {
+                int z = w * 32 + l;
+                float ph = band_phase(m, z, nz);
+                float cz = (float)std::cos((double)ph), sz = (float)std::sin((double)ph);
+                ar[l] = v[z] * (double)cz;
+                br[l] = v[z] * (double)sz;
+            }
+            for (int off = 16; off > 0; off >>= 1)
+                for (int l = 0; l < off; ++l) { ar[l] += ar[l + off]; br[l] += br[l + off]; }
+            sc += ar[0];
+            ss += br[0];
+        }
+        float inv2 = 2.0f / (float)nz;
+        double Am = sc * (double)inv2, Bm = ss * (double)inv2;
+        for (int z = 0; z < nz; ++z) {
+            float ph = band_phase(m, z, nz);
+            float cz = (float)std::cos((double)ph), sz = (float)std::sin((double)ph);
+            delta[z] += Am * (double)cz + Bm * (double)sz;
+        }
+    }
+    for (int z = 0; z < nz; ++z) v[z] = v[z] - delta[z];
+}
+
+void kz_band(Oracle& o, const int* ids, int nids, int lo, int hi) {
+    const int nz = o.p.nz;
+#pragma omp parallel
+    {
+        std::vector<double> col(nz), tmp;
+#pragma omp for schedule(static)
+        for (int x = 0; x < o.p.nx; ++x)
+            for (int k = 0; k < nids; ++k) {
+                double* v = o.A(ids[k]);
+                for (int z = 0; z < nz; ++z) col[z] = v[o.I(x, z)];
+                kz_band_column(col.data(), nz, lo, hi, tmp);
+                for (int z = 0; z < nz; ++z) v[o.I(x, z)] = col[z];
+            }
+    }
+}
+
+// Extension: keep only |kz| <= K.  Projection of each column on the modes
+// 0..K with FP64 twiddles cos(2*pi*m*z/nz), coefficients (1/nz, 2/nz) and a
+// sequential sum over z.  (K < nz/2.)
+void keep_modes(Oracle& o, const int* ids, int nids, int K) {
+    const int nz = o.p.nz;
+#pragma omp parallel
+    {
+        std::vector<double> col(nz), out(nz);
+        std::vector<double> am(K + 1), bm(K + 1);
+#pragma omp for schedule(static)
+        for (int x = 0; x < o.p.nx; ++x)
+            for (int k = 0; k < nids; ++k) {
+                double* v = o.A(ids[k]);
+                for (int z = 0; z < nz; ++z) col[z] = v[o.I(x, z)];
+                for (int m = 0; m <= K; ++m) {
+                    double sc = 0.0, ss = 0.0;
+                    for (int z = 0; z < nz; ++z) {
+                        double ph = 2.0 * M_PI * (double)((long)m * z % nz) / (double)nz;
+                        sc += col[z] * std::cos(ph);
+                        ss += col[z] * std::sin(ph);
+                    }
+                    double w = (m == 0) ? 1.0 / nz : 2.0 / nz;
+                    am[m] = sc * w;
+                    bm[m] = ss * w;
+                }
+                for (int z = 0; z < nz; ++z) {
+                    double acc = am[0];
+                    for (int m = 1; m <= K; ++m) {
+                        double ph = 2.0 * M_PI * (double)((long)m * z % nz) / (double)nz;
+                        acc += am[m] * std::cos(ph) + bm[m] * std::sin(ph);
+                    }
+                    out[z] = acc;
+                }
+                for (int z = 0; z < nz; ++z) v[o.I(x, z)] = out[z];
+            }
+    }
+}
+
+// z hyperdiffusion (YM_Physics.cu:228-259), evaluated out of place.
+void hyperdiffuse(Oracle& o) {
+    const int nz = o.p.nz;
+    const double mu = o.p.hyp_diff;
+#pragma omp parallel
+    {
+        std::vector<double> col(nz);
+#pragma omp for schedule(static)
+        for (int x = 0; x < o.p.nx; ++x)
+            for (int k = 0; k < 12; ++k) {
+                double* v = o.A(kCol23[k]);
+                for (int z = 0; z < nz; ++z) col[z] = v[o.I(x, z)];
+                for (int z = 0; z < nz; ++z) {
+                    int zm1 = (z == 0) ? nz - 1 : z - 1;
+                    int zm2 = (z <= 1) ? z - 2 + nz : z - 2;
+                    int zp1 = (z == nz - 1) ? 0 : z + 1;
+                    int zp2 = (z >= nz - 2) ? z + 2 - nz : z + 2;
+                    double d4 = col[zm2] - 4.0 * col[zm1] + 6.0 * col[z] - 4.0 * col[zp1] + col[zp2];
+                    v[o.I(x, z)] = col[z] - mu * d4;
+                }
+            }
+    }
+}
+
+void filters(Oracle& o) {                           // main_ym.cu:365-447
+    const OrcParams& p = o.p;
+    if (p.xi_sponge > 0.0) sponge(o);
+    if (p.xi_cut > 0.0) xicut(o);
+    if (p.suppress_kz0) zmean(o);
+    if (p.kz_suppress_max >= 1) kz_band(o, kFilt15, 15, 1, p.kz_suppress_max);
+    if (p.kz_suppress_hi > p.kz_suppress_max + 1)
+        kz_band(o, kFilt15, 15, p.kz_suppress_max + 2, p.kz_suppress_hi);
+    const int pzs[2] = {PZA, PZB}, pxs[2] = {PXA, PXB}, ns[2] = {NA, NB};
+    if (p.kz_suppress_max >= 1) {
+        kz_band(o, pzs, 2, 1, p.kz_suppress_max);
+        kz_band(o, pxs, 2, 1, p.kz_suppress_max);
+        kz_band(o, ns, 2, 1, p.kz_suppress_max);
+    }
+    if (p.kz_suppress_hi > p.kz_suppress_max + 1) {
+        kz_band(o, pzs, 2, p.kz_suppress_max + 2, p.kz_suppress_hi);
+        kz_band(o, pxs, 2, p.kz_suppress_max + 2, p.kz_suppress_hi);
+        kz_band(o, ns, 2, p.kz_suppress_max + 2, p.kz_suppress_hi);
+    }
+    if (p.kz_keep_max > 0) {
+        const int ids21[21] = {BY1, EX1, EZ1, BY2, BY3, EX2, EX3, EZ2, EZ3, AZ2, AZ3,
+                               Q2A, Q3A, Q2B, Q3B, NA, PXA, PZA, NB, PXB, PZB};
+        keep_modes(o, ids21, 21, p.kz_keep_max);
+    }
+    if (p.hyp_diff > 0.0) hyperdiffuse(o);
+    if (p.suppress_kz0) {
+        std::fill(o.a[BY1].begin(), o.a[BY1].end(), 0.0);
+        std::fill(o.a[EX1].begin(), o.a[EX1].end(), 0.0);
+        std::fill(o.a[EZ1].begin(), o.a[EZ1].end(), 0.0);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Diagnostics
+// ---------------------------------------------------------------------------
+void energy_density(Oracle& o) {                    // YM_Energy.cu:3-21
+    double* e = o.A(EDENS);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)o.N; ++i) {
+        double nA = dmax(o.A(NA)[i], 1e-10), nB = dmax(o.A(NB)[i], 1e-10);
+        double pxA = o.A(PXA)[i], pzA = o.A(PZA)[i], pxB = o.A(PXB)[i], pzB = o.A(PZB)[i];
+        double keA = 0.5 * (pxA * pxA + pzA * pzA) / nA;
+        double keB = 0.5 * (pxB * pxB + pzB * pzB) / nB;
+        double s = 0.0;
+        const int ids[9] = {EX1, EX2, EX3, EZ1, EZ2, EZ3, BY1, BY2, BY3};
+        double acc = o.A(ids[0])[i] * o.A(ids[0])[i];
+        for (int k = 1; k < 9; ++k) acc = acc + o.A(ids[k])[i] * o.A(ids[k])[i];
+        s = 0.5 * acc;
+        e[i] = keA + keB + s;
+    }
+}
+
+double energy(Oracle& o) {                          // YM_Energy.cu:23-32 (serial host sum)
+    energy_density(o);
+    double total = 0.0;
+    const double* e = o.A(EDENS);
+    for (size_t i = 0; i < o.N; ++i) total += e[i];
+    return total;
+}
+
+// Gauss residual, analysis/gauss_check.py:40-115 with the state's own Az1.
+void gauss(Oracle& o, double* out) {
+    const int nx = o.p.nx, nz = o.p.nz;
+    const double DX = o.p.dx, DZ = o.p.dz, al = o.p.alpha;
+    double s2[6] = {0, 0, 0, 0, 0, 0};
+    double sE2 = 0.0;
+    for (int z = 0; z < nz; ++z)
+        for (int x = 0; x < nx; ++x) {
+            size_t i = o.I(x, z);
+            int xm = (x + nx - 1) % nx, zm = (z + nz - 1) % nz;
+            double Ezc[3], dE[3], Az[3];
+            for (int c = 0; c < 3; ++c) {
+                const double* Ex = o.A(EX1 + c);
+                const double* Ez = o.A(EZ1 + c);
+                dE[c] = (Ex[i] - Ex[o.I(xm, z)]) / DX + (Ez[i] - Ez[o.I(x, zm)]) / DZ;
+                Ezc[c] = 0.5 * (Ez[i] + Ez[o.I(x, zm)]);
+                Az[c] = o.A(AZ1 + c)[i];
+            }
+            double nab[3] = {al * (Az[1] * Ezc[2] - Az[2] * Ezc[1]),
+                             al * (Az[2] * Ezc[0] - Az[0] * Ezc[2]),
+                             al * (Az[0] * Ezc[1] - Az[1] * Ezc[0])};
+            double nA = o.A(NA)[i], nB = o.A(NB)[i];
+            for (int c = 0; c < 3; ++c) {
+                double rho = o.A(Q1A + c)[i] * nA + o.A(Q1B + c)[i] * nB;
+                double Gp = dE[c] + nab[c] - rho;
+                double Gm = dE[c] + nab[c] + rho;
+                s2[c] += Gp * Gp;
+                s2[3 + c] += Gm * Gm;
+            }
+            double ae = std::fabs(o.A(EX1)[i]);
+            for (int c = 1; c < 3; ++c) ae = ae + std::fabs(o.A(EX1 + c)[i]);
+            for (int c = 0; c < 3; ++c) ae = ae + std::fabs(o.A(EZ1 + c)[i]);
+            sE2 += ae * ae;
+        }
+    double N = (double)nx * nz;
+    for (int k = 0; k < 6; ++k) out[k] = std::sqrt(s2[k] / N);
+    out[6] = std::sqrt(sE2 / N) / DX + 1e-30;
+}
+
+// One step in the order of main_ym.cu:322-455.
+void step(Oracle& o) {
+    update_velocities(o, 0); update_velocities(o, 1);
+    x_sweep(o, 0); x_sweep(o, 1);
+    z_sweep(o, 0); z_sweep(o, 1);
+    enforce_boundaries(o);
+    currents(o);
+    ampere(o); potential(o); faraday(o);
+    filters(o);
+    sources(o);
+}
+
+// ---------------------------------------------------------------------------
+// Initial state: kernel_ym_init (ymgpu2d/YM_Init.cu:32-257) with its float
+// math (coshf, tanhf, sinf, cosf, expf, logf, sqrtf of float arguments).
+// ---------------------------------------------------------------------------
+double log_cosh_stable(double xi) {                 // YM_Init.cu:11-18
+    if (std::fabs(xi) < 80.0) return (double)logf(coshf((float)xi));
+    double ax = std::fabs(xi);
+    return ax - 0.6931471805599453 + std::log1p(std::exp(-2.0 * ax));
+}
+
+}  // namespace
+
+extern "C" {
+
+void* orc_create(const OrcParams* p) { return new Oracle(*p); }
+void orc_destroy(void* h) { delete static_cast<Oracle*>(h); }
+double* orc_array(void* h, int id) {
+    Oracle* o = static_cast<Oracle*>(h);
+    if (id < 0 || id >= NIDS || o->a[id].empty()) return nullptr;
+    return o->A(id);
+}
+int orc_num_ids(void) { return NIDS; }
+
+void orc_update_velocities(void* h) { Oracle& o = *static_cast<Oracle*>(h); update_velocities(o, 0); update_velocities(o, 1); }
+void orc_x_sweep(void* h, int s) { x_sweep(*static_cast<Oracle*>(h), s); }
+void orc_z_sweep(void* h, int s) { z_sweep(*static_cast<Oracle*>(h), s); }
+void orc_enforce_boundaries(void* h) { enforce_boundaries(*static_cast<Oracle*>(h)); }
+void orc_currents(void* h) { currents(*static_cast<Oracle*>(h)); }
+void orc_ampere(void* h) { ampere(*static_cast<Oracle*>(h)); }
+void orc_potential(void* h) { potential(*static_cast<Oracle*>(h)); }
+void orc_faraday(void* h) { faraday(*static_cast<Oracle*>(h)); }
+void orc_filters(void* h) { filters(*static_cast<Oracle*>(h)); }
+void orc_sources(void* h) { sources(*static_cast<Oracle*>(h)); }
+void orc_step(void* h, int n) { Oracle& o = *static_cast<Oracle*>(h); for (int i = 0; i < n; ++i) step(o); }
+double orc_energy(void* h) { return energy(*static_cast<Oracle*>(h)); }
+void orc_gauss(void* h, double* out) { gauss(*static_cast<Oracle*>(h), out); }
+
+// Single-row FCT for unit tests (clamped ends), out of place.
+void orc_fct_row(int n, const double* u, const double* v, const double* s, double* out,
+                 double dx, double dt) {
+    FctScratch w;
+    fct_row(n, u, v, s, out, dx, dt, w);
+}
+
+// One column of the reference band filter and of the keep-modes projection.
+void orc_kz_band_column(double* v, int nz, int lo, int hi) {
+    std::vector<double> tmp;
+    kz_band_column(v, nz, lo, hi, tmp);
+}
+
+// Initial state.  Arguments are the kernel_ym_init parameters
+// (main_ym.cu:223-225): dx, dz, perturb_amp, V0, epsilon in double (from the
+// float config), alpha_YM = (double)(float)aYM, vz_edge_taper as double.
+// seed: NULL or 6 arrays of nx doubles (by, ex, ez, az, qA, qB; each may be NULL).
+void orc_init(void* h, int k_mode, double perturb_amp, double V0, double epsilon,
+              int run_mode, double alpha_YM, int init_by1_eq, double vz_edge_taper,
+              const double* seed_by, const double* seed_az, const double* seed_qA,
+              const double* seed_qB) {
+    Oracle& o = *static_cast<Oracle*>(h);
+    const int nx = o.p.nx, nz = o.p.nz;
+    const double dx = o.p.dx, dz = o.p.dz;
+    for (int z = 0; z < nz; ++z)
+        for (int x = 0; x < nx; ++x) {
+            size_t idx = o.I(x, z);
+            double k_z = (double)((float)k_mode * 2.0f) * M_PI / (nz * dz);
+            double x_c = (nx * dx) * 0.5;
+            double xi = (x * dx - x_c) / epsilon;
+            double z_val = z * dz;
+            float kzz = (float)(k_z * z_val);
+            double sech_xi = (double)(1.0f / coshf((float)xi));
+            double seed_win = (fabsf((float)xi) < 3.0) ? 1.0 : 0.0;
+            double base_seed = perturb_amp * V0 * sech_xi * seed_win;
+            double az1_eq = -V0 * log_cosh_stable(xi);
+            double by1 = 0, by2 = 0, by3 = 0, az1 = 0, az2 = 0, az3 = 0;
+            double vz_A = +V0 * (double)tanhf((float)xi);
+            double qA_amp = 0, qB_amp = 0;
+            if (run_mode == 0) {
+                by2 = base_seed * (double)sinf(kzz);
+                az1 = az1_eq;
+            } else if (run_mode == 1) {
+                by2 = base_seed * (double)cosf(kzz);
+                by3 = base_seed * (double)sinf(kzz);
+                az1 = az1_eq;
+            } else if (run_mode == 6) {
+                az1 = az1_eq;
+                double az_amp = 0.0;
+                if (seed_az) {
+                    az_amp = perturb_amp * V0 * seed_az[x];
+                    if (seed_by) by2 = perturb_amp * V0 * seed_by[x] * (double)cosf(kzz);
+                    if (seed_by) by3 = perturb_amp * V0 * seed_by[x] * (double)sinf(kzz);
+                    if (seed_qA) qA_amp = perturb_amp * V0 * seed_qA[x];
+                    if (seed_qB) qB_amp = perturb_amp * V0 * seed_qB[x];
+                } else {
+                    double xi_char = (double)(1.0f / sqrtf((float)(alpha_YM * k_z * V0 + (double)1e-12f)));
+                    az_amp = perturb_amp * V0 * (double)expf((float)(-0.5 * xi * xi / (xi_char * xi_char)));
+                }
+                az2 = az_amp * (double)cosf(kzz);
+                az3 = az_amp * (double)sinf(kzz);
+                if (init_by1_eq) {
+                    double mean_lc = (nx * dx) / (4.0 * epsilon) - (double)0.693147f;
+                    by1 = -2.0 * V0 * epsilon * ((double)logf(coshf((float)xi)) - mean_lc);
+                }
+            } else if (run_mode == 2) {
+                by1 = base_seed * (double)sinf(kzz);
+                az1 = 0.0;
+            } else if (run_mode == 4) {
+                double Lx = nx * dx, x_phys = x * dx;
+                double s = (x_phys < Lx * (double)(1.0f / 3.0f) || x_phys >= Lx * (double)(2.0f / 3.0f)) ? 1.0 : -1.0;
+                vz_A = V0 * s;
+                double kx_az1 = 6.0 * M_PI / Lx;
+                az1 = -V0 * (double)cosf((float)(kx_az1 * x_phys));
+                by2 = perturb_amp * V0 * (double)sinf(kzz);
+            } else if (run_mode == 5) {
+                double Lx = nx * dx, x_ph = x * dx;
+                double xi5 = (x_ph - 0.5 * Lx) / epsilon;
+                vz_A = V0 * (double)tanhf((float)xi5);
+                double kx_cos = 2.0 * M_PI / Lx;
+                az1 = -V0 * (double)cosf((float)(kx_cos * x_ph));
+                by1 = -V0 * kx_cos * (double)sinf((float)(kx_cos * x_ph));
+                double sech5 = (double)(1.0f / coshf((float)xi5));
+                double bs5 = perturb_amp * V0 * sech5;
+                by2 = bs5 * (double)cosf(kzz);
+                by3 = bs5 * (double)sinf(kzz);
+            } else {
+                double L = nx * dx;
+                double xi1 = (x * dx - L * 0.25) / epsilon;
+                double xi2 = (x * dx - L * 0.75) / epsilon;
+                float shear = tanhf((float)xi1) - tanhf((float)xi2) - 1.0f;
+                vz_A = V0 * (double)shear;
+                az1 = -V0 * (log_cosh_stable(xi1) - log_cosh_stable(xi2));
+                by1 = V0 * (double)(tanhf((float)xi1) - tanhf((float)xi2)) / epsilon;
+                double sech1 = (double)(1.0f / coshf((float)xi1));
+                double sech2 = (double)(1.0f / coshf((float)xi2));
+                double base_dt = perturb_amp * V0 * (sech1 + sech2);
+                by2 = base_dt * (double)cosf(kzz);
+                by3 = base_dt * (double)sinf(kzz);
+            }
+            if (vz_edge_taper > 0.0 && (run_mode == 1 || run_mode == 6)) {
+                const double dt_tap = 3.0;
+                float axi = fabsf((float)xi);
+                float w = 0.5f * (1.0f - tanhf((float)(((double)axi - vz_edge_taper) / dt_tap)));
+                vz_A *= (double)w;
+                if (init_by1_eq) {
+                    float axi_h = fabsf((float)(xi + 0.5 * dx / epsilon));
+                    double u = ((double)axi_h - vz_edge_taper) / dt_tap;
+                    double I = (double)logf(coshf(axi_h))
+                               - 0.5 * dt_tap * (u + (double)logf(coshf((float)u)) + (double)0.693147f);
+                    double Lxi = (nx * dx) * 0.5 / epsilon;
+                    double vet = vz_edge_taper;
+                    double mean_I = (0.5 * vet * vet - vet * (double)0.693147f
+                                     + (Lxi - vet) * (vet - (double)0.693147f)) / Lxi;
+                    by1 = -2.0 * V0 * epsilon * (I - mean_I);
+                }
+            }
+            o.A(BY1)[idx] = by1; o.A(BY2)[idx] = by2; o.A(BY3)[idx] = by3;
+            o.A(AZ1)[idx] = az1; o.A(AZ2)[idx] = az2; o.A(AZ3)[idx] = az3;
+            for (int c = 0; c < 3; ++c) { o.A(EX1 + c)[idx] = 0.0; o.A(EZ1 + c)[idx] = 0.0; }
+            double cz = (double)cosf(kzz), sz = (double)sinf(kzz);
+            o.A(NA)[idx] = 1.0; o.A(PXA)[idx] = 0.0; o.A(PZA)[idx] = vz_A;
+            o.A(Q1A)[idx] = 1.0; o.A(Q2A)[idx] = qA_amp * cz; o.A(Q3A)[idx] = qA_amp * sz;
+            o.A(NB)[idx] = 1.0; o.A(PXB)[idx] = 0.0; o.A(PZB)[idx] = -vz_A;
+            o.A(Q1B)[idx] = -1.0; o.A(Q2B)[idx] = qB_amp * cz; o.A(Q3B)[idx] = qB_amp * sz;
+        }
+}
+
+// Counter-based white noise (extension key noise_amp/noise_seed): value for
+// (seed, array id, cell) = amp * (2u - 1), u = top 53 bits of splitmix64.
+double orc_noise(uint64_t seed, int id, uint64_t cell, double amp) {
+    uint64_t z = seed * 0x9E3779B97F4A7C15ull + ((uint64_t)(id + 1) << 40) + cell;
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    double u = (double)(z >> 11) * (1.0 / 9007199254740992.0);
+    return amp * (2.0 * u - 1.0);
+}
+
+}  // extern "C"

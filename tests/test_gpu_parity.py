@@ -1,0 +1,210 @@
+"""GPU parity of the HIP library against the FP64 CPU oracle (run on an MI355X).
+
+Bar (stated per test):
+  * FP64 stencil operators (FCT sweeps, sources, currents, Maxwell, sponge,
+    xi_cut, hyperdiffusion) and whole steps without DFT filters: bit-identical
+    to the oracle (both keep the reference's expression order, no FMA
+    contraction).
+  * DFT band / z-mean filters: the GPU sums each column in a different order
+    than the reference's warp tree -> max|diff| <= 1e-12 * max|value|.
+  * energy / Gauss reductions (compensated on the GPU, serial in the oracle):
+    relative 1e-12.
+  * FP32 precision form: relative 2e-5 of each array's scale after 4 steps.
+"""
+import numpy as np
+import pytest
+
+import khs
+import oracle_ref as orc
+from helpers import assert_bitwise, assert_close, make_pair, make_sim, random_state
+
+pytestmark = pytest.mark.gpu
+
+FLUID = ["nA", "PxA", "PzA", "Q1A", "Q2A", "Q3A", "nB", "PxB", "PzB", "Q1B", "Q2B", "Q3B"]
+FIELDS = ["Ex1", "Ex2", "Ex3", "Ez1", "Ez2", "Ez3", "By1", "By2", "By3", "Az1", "Az2", "Az3"]
+
+C1 = dict(nx_override=256, nz_override=128, lx_override=25.6, lz_override=12.8, alpha_YM=1, V0=0.05,
+          eps_override=1.0, courant_override=0.4, k_mode=1, run_mode=0, freeze_override=1,
+          perturb_amp=2e-5, noise_amp=1e-9, noise_seed=0, n_steps=2000)
+
+
+def grid(nx, nz, periodic, **extra):
+    keys = dict(nx_override=nx, nz_override=nz, lx_override=nx * 0.1, lz_override=nz * 0.1,
+                alpha_YM=1.0, V0=0.05, eps_override=1.0, courant_override=0.4, run_mode=6 if periodic else 0,
+                freeze_override=0, perturb_amp=0.01)
+    keys.update(extra)
+    return keys
+
+
+SHAPES = [(64, 48), (300, 40), (256, 128)]
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4, 5, 6])
+def test_init_bitwise(mode):
+    keys = dict(C1)
+    keys.update(run_mode=mode, nx_override=96, nz_override=64, init_by1_eq=1 if mode == 6 else 0,
+                vz_edge_taper=3.0 if mode == 6 else 0.0)
+    cfg, sim = make_sim(**keys)
+    o = orc.Oracle(orc.params_from_config(cfg))
+    o.init_from_config(cfg)
+    for n in orc.STATE:
+        want = o.get(n)
+        if cfg.noise_amp > 0 and n in ("Ex2", "Ex3", "Ez2", "Ez3", "By2", "By3"):
+            cells = np.arange(cfg.nx * cfg.nz, dtype=np.uint64)
+            want = want + np.array([orc.noise(0, orc.IDS[n], int(c), 1e-9) for c in cells]).reshape(want.shape)
+        assert_bitwise(sim.get(n), want, f"mode {mode} {n}")
+
+
+@pytest.mark.parametrize("periodic", [0, 1])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_staged_operators_bitwise(periodic, shape):
+    """x-sweep, z-sweep (+ boundaries), Maxwell: each stage against the oracle."""
+    nx, nz = shape
+    keys = grid(nx, nz, periodic, warm_T=0.01 if periodic else 0.0)
+    st = random_state(nx, nz, seed=nx + nz + periodic)
+    cfg, sim, o = make_pair(st, **keys)
+    o.update_velocities()
+    o.sources()
+    # x sweeps
+    sim.fct_x_sweep(0)
+    sim.fct_x_sweep(1)
+    o.x_sweep(0)
+    o.x_sweep(1)
+    for n in FLUID:
+        assert_bitwise(sim.get_buf(khs.BUF_STAGE, n), o.get(n), f"x-sweep {n}")
+    # z sweeps (+ density floor and x walls of enforce_boundaries)
+    sim.fct_z_sweep(0)
+    sim.fct_z_sweep(1)
+    o.z_sweep(0)
+    o.z_sweep(1)
+    o.enforce_boundaries()
+    for n in FLUID:
+        assert_bitwise(sim.get_buf(khs.BUF_NEXT, n), o.get(n), f"z-sweep {n}")
+    # Maxwell
+    sim.maxwell()
+    o.currents()
+    o.ampere()
+    o.potential()
+    o.faraday()
+    for n in FIELDS:
+        assert_bitwise(sim.get_buf(khs.BUF_NEXT, n), o.get(n), f"maxwell {n}")
+    sim.commit()
+
+
+@pytest.mark.parametrize("periodic", [0, 1])
+def test_derived_bitwise(periodic):
+    nx, nz = 300, 40
+    cfg, sim, o = make_pair(random_state(nx, nz, seed=7), **grid(nx, nz, periodic, warm_T=0.02))
+    o.update_velocities()
+    o.sources()
+    o.currents()
+    for n in ["Jx1", "Jx2", "Jx3", "Jz1", "Jz2", "Jz3", "uxA", "uzA", "uxB", "uzB", "srcpxA", "srcpzA",
+              "srcpxB", "srcpzB", "sQ1A", "sQ2A", "sQ3A", "sQ1B", "sQ2B", "sQ3B"]:
+        assert_bitwise(sim.get(n), o.get(n), n)
+    o.energy()
+    assert_bitwise(sim.get("energy_density"), o.get("energy_density"), "energy density")
+
+
+STEP_CASES = {
+    "walled_c1_shape": dict(C1, n_steps=0),
+    "periodic_mode6_sponge_cut_hyp_warm": grid(128, 64, 1, xi_sponge=4.0, sigma_sponge=5.0, xi_cut=5.5,
+                                               hyp_diff=5e-3, warm_T=0.01, freeze_override=1),
+    "periodic_unfrozen": grid(96, 32, 1, freeze_override=0, alpha_YM=2.0),
+    "walled_unfrozen_odd": grid(70, 36, 0, freeze_override=0),
+}
+
+
+@pytest.mark.parametrize("case", list(STEP_CASES))
+def test_step_bitwise(case):
+    keys = STEP_CASES[case]
+    nx, nz = keys["nx_override"], keys["nz_override"]
+    st = random_state(nx, nz, seed=3, smooth=True)
+    cfg, sim, o = make_pair(st, **keys)
+    o.sources()
+    sim.step(6)
+    o.step(6)
+    for n in orc.STATE:
+        assert_bitwise(sim.get(n), o.get(n), f"{case} {n}")
+
+
+FILTER_CASES = {
+    "zmean_bands": grid(64, 64, 1, suppress_kz0=1, kz_suppress_max=2, kz_suppress_hi=9, hyp_diff=1e-3),
+    "keep4_c1": dict(C1, kz_keep_max=4, nx_override=64),
+    "bands_nyquist": grid(48, 32, 0, kz_suppress_max=3, kz_suppress_hi=16),
+}
+
+
+@pytest.mark.parametrize("case", list(FILTER_CASES))
+def test_step_filters_close(case):
+    """Column DFT filters: coefficient sums in another order -> 1e-12 relative."""
+    keys = FILTER_CASES[case]
+    nx, nz = keys["nx_override"], keys["nz_override"]
+    cfg, sim, o = make_pair(random_state(nx, nz, seed=5, smooth=True), **keys)
+    o.sources()
+    sim.step(3)
+    o.step(3)
+    for n in orc.STATE:
+        assert_close(sim.get(n), o.get(n), 1e-12, f"{case} {n}")
+
+
+def test_energy_gauss_close():
+    nx, nz = 300, 40
+    cfg, sim, o = make_pair(random_state(nx, nz, seed=11), **grid(nx, nz, 1))
+    E, Eo = sim.energy(), o.energy()
+    assert abs(E - Eo) <= 1e-12 * abs(Eo)
+    g, go = sim.gauss(), o.gauss()
+    got = np.array(g["rms_ref"] + g["rms_cont"] + [g["Etot_scale"]])
+    np.testing.assert_allclose(got, go, rtol=1e-12)
+
+
+def test_fp32_form_close():
+    """FP32 bulk arithmetic vs the FP64 oracle on a smooth state, 4 steps."""
+    nx, nz = 128, 64
+    keys = grid(nx, nz, 1, precision="fp32", xi_sponge=4.0, hyp_diff=1e-3)
+    cfg, sim, o = make_pair(random_state(nx, nz, seed=9, smooth=True), **keys)
+    o.sources()
+    sim.step(4)
+    o.step(4)
+    for n in orc.STATE:
+        assert_close(sim.get(n), o.get(n), 2e-5, f"fp32 {n}")
+
+
+def test_full_size_step_bitwise():
+    """One step at the 2048x1024 headline size, bit-identical to the oracle."""
+    keys = dict(C1, nx_override=2048, nz_override=1024, lx_override=204.8, lz_override=102.4,
+                noise_seed=1, n_steps=5000)
+    cfg, sim, o = make_pair(**keys)
+    o.sources()
+    sim.step(2)
+    o.step(2)
+    for n in orc.STATE:
+        assert_bitwise(sim.get(n), o.get(n), f"2048x1024 {n}")
+
+
+def test_determinism_and_staged_equals_fused():
+    keys = grid(256, 128, 1, hyp_diff=1e-3, xi_sponge=6.0)
+    st = random_state(256, 128, seed=21, smooth=True)
+    _, a, _ = make_pair(st, **keys)
+    _, b, _ = make_pair(st, **keys)
+    a.step(5)
+    for _ in range(5):
+        b.fct_x_sweep(0); b.fct_x_sweep(1); b.fct_z_sweep(0); b.fct_z_sweep(1)
+        b.maxwell(); b.filters(); b.commit()
+    for n in orc.STATE:
+        assert_bitwise(a.get(n), b.get(n), n)
+
+
+def test_nan_halts_and_errors():
+    cfg, sim = make_sim(**grid(64, 32, 1, energy_stride=2))
+    v = sim.get("By2")
+    v[3, 5] = np.nan
+    sim.set("By2", v)
+    assert sim.step(10) is True
+    assert sim.step_count == 2
+    with pytest.raises(khs.KhsError) as e:
+        sim.fct_z_sweep(0)
+    assert e.value.code == khs.KHS_ERR_STATE
+    with pytest.raises(khs.KhsError):
+        sim.fct_x_sweep(2)
+    with pytest.raises(khs.KhsError):
+        sim.get(99)
