@@ -1,0 +1,183 @@
+"""Benchmark: cell-updates/s of the KH time step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--calibrate-copy MB]
+
+Workload (the headline config): plain FP64 single run on a 2048x1024 grid,
+x walled, z periodic, dx = dz = 0.1, alpha = 1, V0 = 0.05, a = 1.0, seeded
+kz mode (amplitude 1e-6) plus 1e-9 white noise (seed 1), CFL 0.4, no mode
+filter.  A "step" is one full time step of the reference loop (two FCT
+sweeps per species, Maxwell, sources, and the energy check at the reference
+cadence every (int)(0.5/DT) steps).  All state is resident in HBM before the
+timed region.  The FP32 precision form of the same workload is timed too.
+
+Prints one JSON line (rank 0, N = 1 only; there is one GPU per box).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "kh-simulations_amd"))
+
+METRIC = "cell-updates/s/GPU FP32 vs FP64 at ≤1e-3 growth-rate err; sweep runs/hour"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOAD = dict(nx_override=2048, nz_override=1024, lx_override=204.8, lz_override=102.4, alpha_YM=1,
+                V0=0.05, eps_override=1.0, courant_override=0.4, k_mode=1, run_mode=0, freeze_override=1,
+                perturb_amp=2e-5, noise_amp=1e-9, noise_seed=1, n_steps=5000)
+
+# Compulsory (algorithmic) HBM traffic of each step kernel, in arrays per cell:
+# every input array read once and every output written once, no halo re-reads.
+#   k_fct_x : R fluid 12 + Ex 3 + By 3,            W stage 12
+#   k_fct_z : R stage 12 + step-start fluid 12 + Ez 3 + By 3 + Az 3, W fluid 12
+#   k_maxwell: R new fluid 12 + fields 12,         W E 6 + By 3 + Az2,3 2 (Az1 frozen)
+KERNEL_ARRAYS = {"k_fct_x": 18 + 12, "k_fct_z": 33 + 12, "k_maxwell": 24 + 11}
+# The whole step's algorithmic minimum: R 24 + W 23 arrays (frozen Az1 read only).
+STEP_ARRAYS = 47
+
+
+def run_gpu(precision, steps, warmup):
+    import khs
+
+    keys = dict(WORKLOAD)
+    if precision == "fp32":
+        keys["precision"] = "fp32"
+    cfg = khs.Config(**keys)
+    sim = khs.Simulation(cfg)
+    sim.init()
+    halted = sim.step(warmup)
+    if halted:
+        raise RuntimeError("warm-up halted")
+    sim.sync()
+    sim.timing(True)
+    sim.timing_reset()
+    sim.sync()
+    t0 = time.perf_counter()
+    halted = sim.step(steps)
+    sim.sync()
+    t1 = time.perf_counter()
+    times = sim.kernel_times()
+    sim.timing(False)
+    if halted:
+        raise RuntimeError("timed region halted")
+    E, E0 = sim.energy(), sim.energy0
+    cells = cfg.nx * cfg.nz
+    sim.close()
+    return dict(seconds=t1 - t0, cells=cells, times=times, e_ratio=E / E0, nx=cfg.nx, nz=cfg.nz)
+
+
+def cpu_baseline(budget_s=15.0):
+    """FP64 OpenMP oracle on the host cores, same workload, bounded sample."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import khs
+    import oracle_ref as orc
+
+    cfg = khs.Config(**WORKLOAD)
+    o = orc.Oracle(orc.params_from_config(cfg))
+    o.init_from_config(cfg)
+    o.sources()
+    o.step(1)                          # warm-up (page-in, thread pool)
+    n, t0 = 0, time.perf_counter()
+    while True:
+        o.step(1)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or n >= 200:
+            break
+    cells = cfg.nx * cfg.nz
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=cells * n / el, unit="cell-updates/s", cores=threads, kind="port",
+                sample=f"{n} FP64 steps of the 2048x1024 workload ({el:.1f} s wall, "
+                       f"OMP_NUM_THREADS={threads}, energy check excluded)")
+
+
+def calibrate_copy(mb):
+    """Plain device copy of known size, for scaling rocprofv3 FETCH/WRITE_SIZE."""
+    import torch
+
+    n = mb * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 5}))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB")
+    a = ap.parse_args()
+    if a.calibrate_copy:
+        calibrate_copy(a.calibrate_copy)
+        return
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU per box; the multi-GPU (independent runs per GPU "
+              "or x-slab) path is a design, not a measured configuration. Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    r64 = run_gpu("fp64", a.steps, a.warmup)
+    ms_step = r64["seconds"] / a.steps * 1e3
+    value = r64["cells"] * a.steps / r64["seconds"]
+
+    # per-kernel live roofline (HIP events on the library's stream)
+    kern = {}
+    for name, arrays in KERNEL_ARRAYS.items():
+        ms, cnt = r64["times"].get(name, (0.0, 0))
+        if cnt:
+            avg_s = ms / cnt * 1e-3
+            bytes_alg = arrays * 8 * r64["cells"]
+            kern[name] = dict(avg_us=avg_s * 1e6, launches=cnt, alg_bytes=bytes_alg,
+                              alg_GBs=bytes_alg / avg_s / 1e9)
+    dom = max(kern, key=lambda k: kern[k]["avg_us"])
+    d = kern[dom]
+    step_alg_GBs = STEP_ARRAYS * 8 * r64["cells"] / (ms_step * 1e-3) / 1e9
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded KH equilibrium + 1e-9 white noise, generated on the host)",
+        "config": {"workload": "plain FP64 single run, 2048x1024, x walled / z periodic, dx=dz=0.1, "
+                               "V0=0.05, a=1.0, seed 1, CFL 0.4, no mode filter",
+                   "nx": r64["nx"], "nz": r64["nz"], "precision": "fp64"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": d["alg_GBs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": d["alg_GBs"] / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_launch": d["alg_bytes"], "avg_launch_us": d["avg_us"]},
+        "step_roofline": {"alg_bytes_per_cell": STEP_ARRAYS * 8, "achieved_GBs": step_alg_GBs,
+                          "frac": step_alg_GBs / HBM_PEAK_GBS},
+        "kernels": kern,
+        "energy_ratio_end": r64["e_ratio"],
+    }
+    if not a.no_fp32:
+        r32 = run_gpu("fp32", a.steps, a.warmup)
+        v32 = r32["cells"] * a.steps / r32["seconds"]
+        out["fp32_form"] = {"value": v32, "ms_per_step": r32["seconds"] / a.steps * 1e3,
+                            "speedup_vs_fp64": v32 / value,
+                            "kernels_avg_us": {k: (v[0] / v[1] * 1e3 if v[1] else None)
+                                               for k, v in r32["times"].items()}}
+    if not a.no_cpu:
+        cb = cpu_baseline(a.cpu_seconds)
+        out["cpu_baseline"] = cb
+        out["gpu_over_cpu"] = value / cb["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

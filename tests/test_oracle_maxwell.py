@@ -1,0 +1,179 @@
+"""Pins the oracle's Maxwell, current and source operators (YM_Physics.cu,
+YM_Fluid.cu) with closed forms and symmetries."""
+import numpy as np
+import pytest
+
+import oracle_ref as orc
+
+FIELD_COLOUR = ["Ex", "Ez", "By", "Az"]
+
+
+def _vacuum(nx, nz, dx, dz, dt):
+    """Charge-free fluid (Q = 0, p = 0, n = 1): J = 0 and the fluid is static,
+    so a full step is the bare Yee update."""
+    p = orc.params(nx, nz, dx, dz, dt, alpha=0.0, periodic_x=1, freeze_az1=0)
+    o = orc.Oracle(p)
+    st = {n: np.zeros((nz, nx)) for n in orc.STATE}
+    st["nA"] = np.ones((nz, nx))
+    st["nB"] = np.ones((nz, nx))
+    return o, st
+
+
+@pytest.mark.parametrize("axis", ["z", "x"])
+def test_yee_plane_wave_matches_amplification_matrix(axis):
+    # One Fourier mode of the leapfrog Yee scheme evolves by the 2x2 matrix
+    # M = [[1, -g1], [-g2, 1 + g1 g2]] per step (E updated first, then B with
+    # the new E).  The oracle must reproduce M^N exactly up to rounding.
+    n, d, dt, m, steps = 64, 0.1, 0.04, 3, 50
+    th = 2 * np.pi * m / n
+    j = np.arange(n)
+    if axis == "z":
+        o, st = _vacuum(4, n, 0.1, d, dt)
+        # Ampere: Ex[j] += -dt (By[j] - By[j-1]) / dz ; Faraday: By[j] -= dt (Ex[j+1] - Ex[j]) / dz
+        g1 = dt / d * (1 - np.exp(-1j * th))
+        g2 = dt / d * (np.exp(1j * th) - 1)
+        e_name = "Ex2"
+        shape = lambda prof: np.tile(prof[:, None], (1, 4))
+        take = lambda a: a[:, 0]
+    else:
+        o, st = _vacuum(n, 4, d, 0.1, dt)
+        # Ampere: Ez[j] += dt (By[j] - By[j-1]) / dx ; Faraday: By[j] -= -dt (Ez[j+1] - Ez[j]) / dx
+        g1 = -dt / d * (1 - np.exp(-1j * th))
+        g2 = -dt / d * (np.exp(1j * th) - 1)
+        e_name = "Ez2"
+        shape = lambda prof: np.tile(prof[None, :], (4, 1))
+        take = lambda a: a[0, :]
+    a0, b0 = 0.0 + 0j, 1.0 + 0j
+    st["By2"] = shape(np.real(b0 * np.exp(1j * th * j)))
+    o.set_state(st)
+    o.sources()
+    o.step(steps)
+    M = np.array([[1, -g1], [-g2, 1 + g1 * g2]])
+    aN, bN = np.linalg.matrix_power(M, steps) @ np.array([a0, b0])
+    np.testing.assert_allclose(take(o.get("By2")), np.real(bN * np.exp(1j * th * j)), atol=1e-12)
+    np.testing.assert_allclose(take(o.get(e_name)), np.real(aN * np.exp(1j * th * j)), atol=1e-12)
+    # the scheme is neutrally stable: |eigenvalues| = 1 at this Courant number
+    assert np.allclose(np.abs(np.linalg.eigvals(M)), 1.0, atol=1e-12)
+
+
+def _rotation(seed):
+    r = np.random.default_rng(seed)
+    q, _ = np.linalg.qr(r.standard_normal((3, 3)))
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q
+
+
+def _rotate(st, R):
+    out = dict(st)
+    for f in FIELD_COLOUR:
+        v = np.stack([st[f + c] for c in "123"])
+        w = np.einsum("ab,bzx->azx", R, v)
+        for k, c in enumerate("123"):
+            out[f + c] = w[k]
+    for s in "AB":
+        v = np.stack([st[f"Q{c}{s}"] for c in "123"])
+        w = np.einsum("ab,bzx->azx", R, v)
+        for k, c in enumerate("123"):
+            out[f"Q{c}{s}"] = w[k]
+    return out
+
+
+def test_colour_rotation_covariance_of_maxwell_and_sources():
+    # currents, Ampere, potential, Faraday and the precession source are
+    # built from epsilon^abc and dot products: they commute with any SO(3)
+    # colour rotation.  A wrong sign in one cross-product component breaks it.
+    from helpers import random_state
+    nx, nz = 24, 20
+    st = random_state(nx, nz, seed=4, smooth=True)
+    R = _rotation(5)
+    res = []
+    for s in (st, _rotate(st, R)):
+        o = orc.Oracle(orc.params(nx, nz, 0.1, 0.1, 0.04, alpha=1.7, periodic_x=1, freeze_az1=0))
+        o.set_state(s)
+        o.update_velocities()
+        o.sources()
+        o.currents()
+        o.ampere()
+        o.potential()
+        o.faraday()
+        out = o.state()
+        for c in "123":
+            out["J" + c] = o.get("Jx" + c)
+            out["sQ" + c] = o.get(f"sQ{c}A")
+        out["srcpz"] = o.get("srcpzA")
+        out["srcpx"] = o.get("srcpxA")
+        res.append(out)
+    rot = _rotate(res[0], R)
+    for f in FIELD_COLOUR:
+        for c in "123":
+            np.testing.assert_allclose(res[1][f + c], rot[f + c], atol=1e-13, err_msg=f + c)
+    for name in ("J", "sQ"):
+        v = np.stack([res[0][name + c] for c in "123"])
+        w = np.einsum("ab,bzx->azx", R, v)
+        for k, c in enumerate("123"):
+            np.testing.assert_allclose(res[1][name + c], w[k], atol=1e-13, err_msg=name + c)
+    # the Lorentz force on the fluid is a colour scalar
+    np.testing.assert_allclose(res[1]["srcpz"], res[0]["srcpz"], atol=1e-13)
+    np.testing.assert_allclose(res[1]["srcpx"], res[0]["srcpx"], atol=1e-13)
+
+
+def test_uniform_drift_current_and_sources():
+    nx, nz = 8, 8
+    o = orc.Oracle(orc.params(nx, nz, 0.1, 0.1, 0.04, alpha=2.0, periodic_x=1, freeze_az1=0))
+    st = {n: np.zeros((nz, nx)) for n in orc.STATE}
+    one = np.ones((nz, nx))
+    st.update(nA=2.0 * one, PxA=0.5 * one, PzA=-1.0 * one, Q1A=1.0 * one, Q2A=0.25 * one,
+              nB=1.0 * one, PxB=0.125 * one, PzB=0.5 * one, Q1B=-1.0 * one, Q3B=0.5 * one,
+              Ex1=0.5 * one, Ez2=0.25 * one, By3=2.0 * one, Az1=1.0 * one, Az2=-0.5 * one)
+    o.set_state(st)
+    o.update_velocities()
+    o.sources()
+    o.currents()
+    vA = np.array([0.25, -0.5])      # p / n
+    vB = np.array([0.125, 0.5])
+    QA, QB = np.array([1.0, 0.25, 0.0]), np.array([-1.0, 0.0, 0.5])
+    for c in range(3):
+        Jx = -(2.0 * QA[c] * vA[0] + 1.0 * QB[c] * vB[0])
+        Jz = -(2.0 * QA[c] * vA[1] + 1.0 * QB[c] * vB[1])
+        assert np.all(o.get(f"Jx{c+1}") == Jx) and np.all(o.get(f"Jz{c+1}") == Jz)
+    # Lorentz: F_x = -sum Q^a (E_x^a + v_z B^a), F_z = -sum Q^a (E_z^a - v_x B^a)
+    E_x, E_z, B = np.array([0.5, 0, 0]), np.array([0, 0.25, 0]), np.array([0, 0, 2.0])
+    assert np.allclose(o.get("srcpxA"), -(QA @ (E_x + vA[1] * B)))
+    assert np.allclose(o.get("srcpzA"), -(QA @ (E_z - vA[0] * B)))
+    # precession: sQ = alpha v_z (Q x A)
+    A = np.array([1.0, -0.5, 0.0])
+    sq = 2.0 * vA[1] * np.cross(QA, A)
+    for c in range(3):
+        assert np.allclose(o.get(f"sQ{c+1}A"), sq[c])
+    # |Q| is conserved to first order by precession: Q . sQ = 0
+    assert abs(QA @ sq) < 1e-15
+
+
+def test_energy_closed_form():
+    nx, nz = 10, 6
+    o = orc.Oracle(orc.params(nx, nz, 0.1, 0.1, 0.04))
+    st = {n: np.zeros((nz, nx)) for n in orc.STATE}
+    one = np.ones((nz, nx))
+    st.update(nA=2.0 * one, PxA=1.0 * one, PzA=2.0 * one, nB=0.5 * one, PzB=1.0 * one,
+              Ex2=3.0 * one, By1=1.0 * one, Az2=7.0 * one, Q1A=one)
+    o.set_state(st)
+    want = nx * nz * (0.5 * 5.0 / 2.0 + 0.5 * 1.0 / 0.5 + 0.5 * (9.0 + 1.0))
+    assert o.energy() == want
+
+
+def test_gauss_closed_form():
+    # Ex^1 = sin(2 pi x / Lx) on a periodic grid, rho = 0: G^1 = backward
+    # difference of Ex^1 exactly, G^2 = G^3 = 0.
+    nx, nz, dx = 32, 4, 0.25
+    o = orc.Oracle(orc.params(nx, nz, dx, 0.1, 0.04, alpha=1.0, periodic_x=1))
+    st = {n: np.zeros((nz, nx)) for n in orc.STATE}
+    x = np.arange(nx)
+    ex = np.sin(2 * np.pi * x / nx)
+    st["Ex1"] = np.tile(ex, (nz, 1))
+    o.set_state(st)
+    g = o.gauss()
+    div = (ex - np.roll(ex, 1)) / dx
+    assert abs(g[0] - np.sqrt(np.mean(div ** 2))) < 1e-14
+    assert g[1] == 0.0 and g[2] == 0.0
+    assert abs(g[6] - np.sqrt(np.mean(np.abs(ex) ** 2)) / dx) < 1e-14
