@@ -97,6 +97,10 @@ typedef struct khs_config {
     int      precision;        /* KHS_FP64 or KHS_FP32 */
     int      energy_stride_override;
     int      export_stride_override;  /* <0 disables snapshots */
+    int      fused_sweep;      /* 1: khs_step runs the fused x+z sweep kernel; 0 (default): split */
+    int      autotune;         /* 1 (default): time the tile variants on the first step, keep the fastest */
+    int      x_segment;        /* 0 = tuned, or force 64 / 128 / 256 cells per x-sweep segment */
+    int      z_chunk;          /* 0 = tuned, or force 16 / 32 / 64 rows per z-sweep chunk */
     /* --- derived (khs_derive) --- */
     int    nx, nz;
     int    periodic_x;         /* run_mode >= 1 */
@@ -202,6 +206,9 @@ int khs_timing_enable(khs_ctx* ctx, int on);
 int khs_timing_reset(khs_ctx* ctx);
 int khs_kernel_times(khs_ctx* ctx, double* ms, int* launches, int nslots);
 const char* khs_kernel_name(int slot);
+/* Tile shapes in use: x-sweep segment width (cells) and z-sweep chunk height
+ * (rows); chosen by the autotuner on the first step unless autotune = 0. */
+int khs_get_tuning(khs_ctx* ctx, int* x_segment, int* z_chunk);
 
 #ifdef __cplusplus
 }

@@ -56,6 +56,10 @@ void config_defaults(khs_config& c) {
     c.precision = KHS_FP64;
     c.energy_stride_override = 0;
     c.export_stride_override = 0;
+    c.fused_sweep = 0;
+    c.autotune = 1;
+    c.x_segment = 0;
+    c.z_chunk = 0;
 }
 
 static int parse_int(const std::string& k, const std::string& v) {
@@ -117,6 +121,10 @@ bool config_set(khs_config& c, const std::string& key, const std::string& val) {
     }
     else if (key == "energy_stride")     c.energy_stride_override = parse_int(key, val);
     else if (key == "export_stride")     c.export_stride_override = parse_int(key, val);
+    else if (key == "fused_sweep")       c.fused_sweep = parse_int(key, val);
+    else if (key == "autotune")          c.autotune = parse_int(key, val);
+    else if (key == "x_segment")         c.x_segment = parse_int(key, val);
+    else if (key == "z_chunk")           c.z_chunk = parse_int(key, val);
     else return false;
     return true;
 }
@@ -166,9 +174,12 @@ void config_derive(khs_config& c) {
     if (c.freeze_override >= 0) c.freeze_az1 = c.freeze_override;
     c.periodic_x = (m >= 1) ? 1 : 0;
     // main_ym.cu:244-249
+    // n_steps (extension) fixes the step count and, like target_tu, selects the
+    // physical-time cadences (export every 1 TU, energy check every 0.5 TU).
+    const bool timed = (c.target_tu > 0.0f) || (c.n_steps > 0);
     c.total_steps = (c.target_tu > 0.0f) ? (int)((double)c.target_tu / c.dt) : 2000000;
-    c.export_stride = (c.target_tu > 0.0f) ? std::max(1, (int)(1.0 / c.dt)) : 20000;
-    c.energy_stride = (c.target_tu > 0.0f) ? std::max(1, (int)(0.5 / c.dt)) : 10000;
+    c.export_stride = timed ? std::max(1, (int)(1.0 / c.dt)) : 20000;
+    c.energy_stride = timed ? std::max(1, (int)(0.5 / c.dt)) : 10000;
     c.energy_thresh = (m >= 3 || c.xi_sponge > 0.0) ? 100.0 : 5.0;
     if (c.n_steps > 0) c.total_steps = c.n_steps;
     if (c.energy_stride_override > 0) c.energy_stride = c.energy_stride_override;
@@ -189,6 +200,10 @@ void config_derive(khs_config& c) {
         throw Error(KHS_ERR_CONFIG, "hyp_diff requires NZ >= 5");
     if (c.kz_suppress_hi > NZ / 2 || c.kz_suppress_max > NZ / 2)
         throw Error(KHS_ERR_CONFIG, "kz band edges must not exceed NZ/2");
+    if (c.x_segment != 0 && c.x_segment != 64 && c.x_segment != 128 && c.x_segment != 256)
+        throw Error(KHS_ERR_CONFIG, "x_segment must be 0, 64, 128 or 256");
+    if (c.z_chunk != 0 && c.z_chunk != 16 && c.z_chunk != 32 && c.z_chunk != 64)
+        throw Error(KHS_ERR_CONFIG, "z_chunk must be 0, 16, 32 or 64");
     if (c.run_mode < 0 || c.run_mode > 6)
         throw Error(KHS_ERR_CONFIG, "run_mode must be 0..6");
 }

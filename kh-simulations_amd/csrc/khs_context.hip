@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <functional>
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -40,14 +41,15 @@ namespace khs {
             throw Error(KHS_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_));   \
     } while (0)
 
-enum Slot { K_FCT_X = 0, K_FCT_Z, K_MAXWELL, K_FILTERS, K_ENERGY, K_NSLOTS };
-static const char* kSlotNames[K_NSLOTS] = {"k_fct_x", "k_fct_z", "k_maxwell", "filters", "energy"};
+enum Slot { K_FCT_X = 0, K_FCT_Z, K_FCT_XZ, K_MAXWELL, K_FILTERS, K_ENERGY, K_NSLOTS };
+static const char* kSlotNames[K_NSLOTS] = {"k_fct_x", "k_fct_z", "k_fct_xz", "k_maxwell", "filters", "energy"};
 
 static const int kFilt15[15] = {S_BY1, S_EX1, S_EZ1, S_BY2, S_BY3, S_EX2, S_EX3, S_EZ2, S_EZ3,
                                 S_AZ2, S_AZ3, S_Q2A, S_Q3A, S_Q2B, S_Q3B};
 static const int kFluid6[6] = {S_PZA, S_PZB, S_PXA, S_PXB, S_NA, S_NB};
 static const int kCol23[12] = {S_BY2, S_BY3, S_EX2, S_EX3, S_EZ2, S_EZ3, S_AZ2, S_AZ3,
                                S_Q2A, S_Q3A, S_Q2B, S_Q3B};
+static const int kQ23[4] = {S_Q2A, S_Q3A, S_Q2B, S_Q3B};
 
 struct ModeTable {
     void* c = nullptr;   // [nmodes][nz] of float or double
@@ -91,6 +93,10 @@ struct Ctx {
     std::vector<hipEvent_t> evpool;
     double ms[K_NSLOTS] = {};
     int cnt[K_NSLOTS] = {};
+
+    // tile variants (autotuned on the first step)
+    int xvar = 2, zvar = 1;
+    bool tuned = false;
 
     // step state
     long step = 0;
@@ -155,15 +161,64 @@ struct Ctx {
 
     // ---- stages ----
     template <typename T> void do_x(int sp0, int nsp) {
-        timed(K_FCT_X, [&] { launch_fct_x<T>(state<T>(cur), stg<T>(), kp, sp0, nsp, stream); });
+        timed(K_FCT_X, [&] { launch_fct_x<T>(state<T>(cur), stg<T>(), kp, sp0, nsp, xvar, stream); });
         check_launch();
     }
     template <typename T> void do_z(int sp0, int nsp) {
-        timed(K_FCT_Z, [&] { launch_fct_z<T>(state<T>(cur), stg<T>(), state<T>(1 - cur), kp, sp0, nsp, stream); });
+        timed(K_FCT_Z, [&] { launch_fct_z<T>(state<T>(cur), stg<T>(), state<T>(1 - cur), kp, sp0, nsp, zvar, stream); });
+        check_launch();
+    }
+    // Time every tile variant of the two sweeps on this grid (3 launches each
+    // after a warm-up) and keep the fastest.  Runs before the first sweep of
+    // the context, when the stage and the next-state fluid are scratch; all
+    // variants compute identical bits.
+    template <typename T> int pick(int nvar, const std::function<void(int)>& launch) {
+        hipEvent_t a = get_event(), b = get_event();
+        float best = 1e30f;
+        int bi = 0;
+        for (int v = 0; v < nvar; ++v) {
+            launch(v);
+            HIPCHK(hipEventRecord(a, stream));
+            for (int r = 0; r < 3; ++r) launch(v);
+            HIPCHK(hipEventRecord(b, stream));
+            HIPCHK(hipEventSynchronize(b));
+            float t = 0.f;
+            HIPCHK(hipEventElapsedTime(&t, a, b));
+            if (t < best) { best = t; bi = v; }
+        }
+        check_launch();
+        evpool.push_back(a);
+        evpool.push_back(b);
+        return bi;
+    }
+    template <typename T> void do_autotune() {
+        tuned = true;
+        const int fx = cfg.x_segment == 64 ? 0 : cfg.x_segment == 128 ? 1 : cfg.x_segment == 256 ? 2 : -1;
+        const int fz = cfg.z_chunk == 16 ? 0 : cfg.z_chunk == 32 ? 1 : cfg.z_chunk == 64 ? 2 : -1;
+        if (fx >= 0) xvar = fx;
+        if (fz >= 0) zvar = fz;
+        if (!cfg.autotune) return;
+        if (fx < 0)
+            xvar = pick<T>(kNumXVariants, [&](int v) { launch_fct_x<T>(state<T>(cur), stg<T>(), kp, 0, 2, v, stream); });
+        launch_fct_x<T>(state<T>(cur), stg<T>(), kp, 0, 2, xvar, stream);
+        if (fz < 0)
+            zvar = pick<T>(kNumZVariants, [&](int v) {
+                launch_fct_z<T>(state<T>(cur), stg<T>(), state<T>(1 - cur), kp, 0, 2, v, stream);
+            });
+    }
+    void autotune() {
+        if (tuned) return;
+        fp32 ? do_autotune<float>() : do_autotune<double>();
+    }
+    template <typename T> void do_xz() {
+        timed(K_FCT_XZ, [&] { launch_fct_xz<T>(state<T>(cur), state<T>(1 - cur), kp, stream); });
         check_launch();
     }
     template <typename T> void do_maxwell() {
-        timed(K_MAXWELL, [&] { launch_maxwell<T>(state<T>(cur), state<T>(1 - cur), kp, stream); });
+        timed(K_MAXWELL, [&] {
+            launch_maxwell<T>(state<T>(cur), state<T>(1 - cur), kp, cell_filters ? d_decay : nullptr,
+                              cell_filters ? d_action : nullptr, stream);
+        });
         check_launch();
     }
     template <typename T> void do_modes(const ModeTable& t, const ArrayList<T>& l, int project) {
@@ -183,7 +238,8 @@ struct Ctx {
                    c.kz_suppress_hi > c.kz_suppress_max + 1 || c.kz_keep_max > 0 || c.hyp_diff > 0.0f;
         if (!any) return;
         timed(K_FILTERS, [&] {
-            if (cell_filters) launch_cell_filters<T>(list<T>(k, kCol23, 12, false), d_decay, d_action, kp, stream);
+            // colour-2/3 fields were damped / cut in k_maxwell; the charges here
+            if (cell_filters) launch_cell_filters<T>(list<T>(k, kQ23, 4, false), d_decay, d_action, kp, stream);
             if (c.suppress_kz0) do_modes<T>(t_zmean, list<T>(k, kFilt15, 15, skip_c1), 0);
             // reference band filter: 15 field/charge arrays + (pz, px, n) x 2 species
             int ids21[21];
@@ -200,8 +256,12 @@ struct Ctx {
         });
         check_launch();
     }
-    void x_stage(int sp0, int nsp) { fp32 ? do_x<float>(sp0, nsp) : do_x<double>(sp0, nsp); }
+    void x_stage(int sp0, int nsp) {
+        autotune();
+        fp32 ? do_x<float>(sp0, nsp) : do_x<double>(sp0, nsp);
+    }
     void z_stage(int sp0, int nsp) { fp32 ? do_z<float>(sp0, nsp) : do_z<double>(sp0, nsp); }
+    void xz_stage() { fp32 ? do_xz<float>() : do_xz<double>(); }
     void maxwell_stage() { fp32 ? do_maxwell<float>() : do_maxwell<double>(); }
     void filter_stage() { fp32 ? do_filters<float>() : do_filters<double>(); }
 
@@ -401,8 +461,12 @@ static void commit(Ctx& c) {
 }
 
 static void full_step(Ctx& c) {
-    c.x_stage(0, 2);
-    c.z_stage(0, 2);
+    if (c.cfg.fused_sweep) {
+        c.xz_stage();
+    } else {
+        c.x_stage(0, 2);
+        c.z_stage(0, 2);
+    }
     c.maxwell_stage();
     c.filter_stage();
     commit(c);
@@ -786,6 +850,15 @@ int khs_kernel_times(khs_ctx* ctx, double* ms, int* launches, int nslots) {
             if (launches) launches[i] = c.cnt[i];
         }
     });
+}
+
+int khs_get_tuning(khs_ctx* ctx, int* x_segment, int* z_chunk) {
+    CTX_OR_FAIL(ctx);
+    Ctx& c = *ctx->impl;
+    static const int xs[3] = {64, 128, 256}, zs[3] = {16, 32, 64};
+    if (x_segment) *x_segment = xs[c.xvar];
+    if (z_chunk) *z_chunk = zs[c.zvar];
+    return KHS_OK;
 }
 
 const char* khs_kernel_name(int slot) {

@@ -54,6 +54,15 @@ __device__ __forceinline__ int xp_src(int x, const KParams& p) {   // YM_Physics
 __device__ __forceinline__ int zm_per(int z, int nz) { return z == 0 ? nz - 1 : z - 1; }
 __device__ __forceinline__ int zp_per(int z, int nz) { return z == nz - 1 ? 0 : z + 1; }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b % 8), each with its own L2.  Remap the hardware block id so that XCD k
+// walks the contiguous logical range [k*M, (k+1)*M), M = nb8/8: tiles that are
+// adjacent in logical order (and share halos / field reads) then run on the
+// same XCD at about the same time.  nb8 = grid size rounded up to 8; logical
+// ids >= the real tile count must exit.
+__device__ __forceinline__ int xcd_logical(int b, int nb8) { return (b & 7) * (nb8 >> 3) + (b >> 3); }
+__host__ __device__ __forceinline__ int round8(int n) { return (n + 7) & ~7; }
+
 // Explicit selects: deterministic sign of zero (NaN propagates through the
 // first operand of kmax(a, floor)).
 template <typename T> __device__ __forceinline__ T kmin(T a, T b) { return (b < a) ? b : a; }

@@ -6,14 +6,23 @@
 namespace khs {
 
 // step kernels (khs_step_kernels.hip)
+// Tile variants (bit-identical results): x segment 64/128/256 cells
+// (variant 0/1/2); z chunk 16/32/64 rows (variant 0/1/2).
+constexpr int kNumXVariants = 3;
+constexpr int kNumZVariants = 3;
 template <typename T>
 void launch_fct_x(const State<T>& S, const Stage<T>& out, const KParams& p, int sp0, int nsp,
-                  hipStream_t st);
+                  int variant, hipStream_t st);
 template <typename T>
 void launch_fct_z(const State<T>& S, const Stage<T>& X, const State<T>& O, const KParams& p,
-                  int sp0, int nsp, hipStream_t st);
+                  int sp0, int nsp, int variant, hipStream_t st);
+// fused x + z sweep of both species: state S -> fluid of O
 template <typename T>
-void launch_maxwell(const State<T>& S, const State<T>& O, const KParams& p, hipStream_t st);
+void launch_fct_xz(const State<T>& S, const State<T>& O, const KParams& p, hipStream_t st);
+// decay/action: per-column sponge / xi_cut (nullptr when neither is active)
+template <typename T>
+void launch_maxwell(const State<T>& S, const State<T>& O, const KParams& p, const double* decay,
+                    const int* action, hipStream_t st);
 
 // filters (khs_aux_kernels.hip)
 constexpr int kMaxFilterArrays = 24;

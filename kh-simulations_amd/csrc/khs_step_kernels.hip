@@ -27,11 +27,13 @@ namespace khs {
 // ==========================================================================
 // x sweep
 // ==========================================================================
-constexpr int XB = 256;            // cells per row segment (= threads)
-constexpr int XW = XB + 6;         // segment + 3-cell halo each side
-
-template <typename T>
-__global__ __launch_bounds__(XB) void k_fct_x(State<T> S, Stage<T> out, KParams p, int sp0) {
+// XB cells per row segment (= threads), XW = segment + 3-cell halo each side.
+// XB is tuned per grid (64 / 128 / 256): smaller segments use less LDS and
+// so fit more workgroups per CU, which hides the load latency of phase A.
+template <typename T, int XB>
+__global__ __launch_bounds__(XB) void k_fct_x(State<T> S, Stage<T> out, KParams p, int sp0, int nsp,
+                                              int ntx) {
+    constexpr int XW = XB + 6;
     __shared__ T sU[XW], sS[XW];
     __shared__ T sE[XW], sNU[XW], sMU[XW];
     __shared__ T sV[F_COUNT][XW];      // variable, later the limited interface flux
@@ -39,9 +41,14 @@ __global__ __launch_bounds__(XB) void k_fct_x(State<T> S, Stage<T> out, KParams 
     __shared__ T sLNR[F_COUNT][XW];
 
     const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * XB;
-    const int z = blockIdx.y;
-    const int sp = sp0 + blockIdx.z;
+    // logical tile: species fastest, then x segments, then rows (row z-1,
+    // whose By the px source reads, was staged ntx*nsp tiles earlier)
+    int lt = xcd_logical(blockIdx.x, gridDim.x);
+    if (lt >= nsp * ntx * p.nz) return;
+    const int sp = sp0 + lt % nsp;
+    lt /= nsp;
+    const int x0 = (lt % ntx) * XB;
+    const int z = lt / ntx;
     const int b = S_NA + sp * F_COUNT;
     const size_t row = (size_t)z * p.nx;
     const T dt = T(p.dt);
@@ -123,53 +130,27 @@ __global__ __launch_bounds__(XB) void k_fct_x(State<T> S, Stage<T> out, KParams 
 // ==========================================================================
 // z sweep (register march)
 // ==========================================================================
-constexpr int ZB_X = 64;           // columns per workgroup (one wave wide)
-constexpr int ZB_C = 4;            // z chunks per workgroup (one per wave)
-
-template <typename T, int TZ>
-__global__ __launch_bounds__(ZB_X * ZB_C) void k_fct_z(State<T> S, Stage<T> X, State<T> O,
-                                                       KParams p, int sp0) {
-    const int x = blockIdx.x * ZB_X + (threadIdx.x % ZB_X);
-    const int chunk = blockIdx.y * ZB_C + (threadIdx.x / ZB_X);
-    const int z0 = chunk * TZ;
-    const int sp = sp0 + blockIdx.z;
-    if (x >= p.nx || z0 >= p.nz) return;
-    const int nx = p.nx, nz = p.nz;
-    const int b = S_NA + sp * F_COUNT;
-    const T dt = T(p.dt), dtdz = T(p.dtdz);
-    const int zend = min(z0 + TZ, nz);
-    const bool wall = !p.periodic_x && (x == 0 || x == nx - 1);
-
-    // pipeline registers (names: *_m = previous row / interface)
+// FCT pipeline along z for one column, kept in registers.  push() is fed the
+// extended rows e = z0-3 .. zend+2 in order (rows outside [0, nz) are the
+// clamped end rows, FCT_Sweeps.cu:52,73,78) and yields row e-3 once
+// e >= z0+3.  Per row it evaluates the interface (e-1, e) coefficients, the
+// transport/diffusion fluxes, LORHOT/LNRHOT of row e-1, the antidiffusive flux
+// at e-3/2 and the limited flux at e-5/2, in the reference's order
+// (FCT_Sweeps.cu:88-162).
+template <typename T>
+struct ZPipe {
     T Vm[F_COUNT], Sm[F_COUNT];      // variable and source at row e-1
     T Fm[F_COUNT], Dm[F_COUNT];      // transport / diffusion flux at e-3/2
     T LORm[F_COUNT];                 // LORHOT(e-2)
     T L3[F_COUNT], L2[F_COUNT], L1[F_COUNT];  // LNRHOT(e-4), (e-3), (e-2)
     T FLXm[F_COUNT];                 // antidiffusive flux at e-5/2
     T FLm[F_COUNT];                  // limited flux at e-7/2
-    T Um = T(0), MUm = T(0);         // velocity at e-1, mu at interface e-3/2
+    T Um, MUm;                       // velocity at e-1, mu at interface e-3/2
 
-    for (int e = z0 - 3; e <= zend + 2; ++e) {
-        const int rc = clampi(e, 0, nz - 1);
-        const size_t i = (size_t)x + (size_t)rc * nx;
-        // ---- load row e: x-swept variables, stale velocity, sources ----
-        T V[F_COUNT], Sv[F_COUNT];
-#pragma unroll
-        for (int v = 0; v < F_COUNT; ++v) V[v] = X.a[sp * F_COUNT + v][i];
-        T u;
-        {
-            T n_safe = kmax(S.a[b + F_N][i], T(0.05));
-            u = S.a[b + F_PZ][i] / n_safe;
-        }
-        Sv[F_N] = T(0); Sv[F_PX] = T(0);
-        Sv[F_PZ] = lorentz_pz(S, sp, x, rc, p);
-        {
-            T q[3];
-            precession_at(S, sp, x, rc, p, q);
-            Sv[F_Q1] = q[0]; Sv[F_Q2] = q[1]; Sv[F_Q3] = q[2];
-        }
+    __device__ __forceinline__ bool push(int e, int z0, const T (&V)[F_COUNT], T u,
+                                         const T (&Sv)[F_COUNT], T dt, T dtdz, T (&out)[F_COUNT]) {
+        bool have = false;
         if (e > z0 - 3) {
-            // ---- interface (e-1, e) ----
             T vi = T(0.5) * (Um + u);
             T eps = vi * dtdz;
             T nu, mu;
@@ -177,28 +158,19 @@ __global__ __launch_bounds__(ZB_X * ZB_C) void k_fct_z(State<T> S, Stage<T> X, S
             const bool do_lor = (e >= z0 - 1);
             const bool do_flx = (e >= z0);
             const bool do_fl = (e >= z0 + 2);
-            const bool do_out = (e >= z0 + 3);
+            have = (e >= z0 + 3);
 #pragma unroll
             for (int v = 0; v < F_COUNT; ++v) {
                 T F = T(0.5) * eps * (Vm[v] + V[v]);
                 T D = nu * (V[v] - Vm[v]);
                 if (do_lor) {
-                    // row e-1: LORHOT / LNRHOT
-                    T lor = Vm[v] + dt * Sm[v] - (F - Fm[v]);
+                    T lor = Vm[v] + dt * Sm[v] - (F - Fm[v]);        // row e-1
                     T lnr = lor + (D - Dm[v]);
                     T flx = T(0);
-                    if (do_flx) flx = MUm * (lor - LORm[v]);      // interface e-3/2
+                    if (do_flx) flx = MUm * (lor - LORm[v]);         // interface e-3/2
                     if (do_fl) {
-                        // limited flux at interface e-5/2 (j = e-3)
-                        T fl = fct_limit(FLXm[v], L2[v] - L3[v], L1[v] - L2[v], lnr - L1[v]);
-                        if (do_out) {
-                            T r = L2[v] - (fl - FLm[v]);          // row e-3
-                            const int zo = e - 3;
-                            size_t io = (size_t)x + (size_t)zo * nx;
-                            if (v == F_N) { if (r < T(0.05)) r = T(0.05); }   // YM_Fluid.cu:20-21
-                            if (v == F_PX && wall) r = T(0.0);                // YM_Fluid.cu:23-25
-                            O.a[b + v][io] = r;
-                        }
+                        T fl = fct_limit(FLXm[v], L2[v] - L3[v], L1[v] - L2[v], lnr - L1[v]);  // e-5/2
+                        out[v] = L2[v] - (fl - FLm[v]);               // row e-3
                         FLm[v] = fl;
                     }
                     FLXm[v] = flx;
@@ -212,6 +184,179 @@ __global__ __launch_bounds__(ZB_X * ZB_C) void k_fct_z(State<T> S, Stage<T> X, S
 #pragma unroll
         for (int v = 0; v < F_COUNT; ++v) { Vm[v] = V[v]; Sm[v] = Sv[v]; }
         Um = u;
+        return have;
+    }
+};
+
+// Inputs of the z sweep at real cell (x, rc) from the step-start state: the
+// stale z velocity (YM_FCT.cu:143-146 transposes d_old.uy, set before the
+// x sweep) and the pz / Q^a sources of the previous step's end.
+template <typename T>
+__device__ __forceinline__ void z_inputs(const State<T>& S, int sp, int x, int rc, const KParams& p,
+                                         T& u, T (&Sv)[F_COUNT]) {
+    const int b = S_NA + sp * F_COUNT;
+    const size_t i = (size_t)x + (size_t)rc * p.nx;
+    T n_safe = kmax(S.a[b + F_N][i], T(0.05));
+    u = S.a[b + F_PZ][i] / n_safe;
+    Sv[F_N] = T(0);
+    Sv[F_PX] = T(0);
+    Sv[F_PZ] = lorentz_pz(S, sp, x, rc, p);
+    T q[3];
+    precession_at(S, sp, x, rc, p, q);
+    Sv[F_Q1] = q[0]; Sv[F_Q2] = q[1]; Sv[F_Q3] = q[2];
+}
+
+// Swept row written to the next state with the post-sweep boundary condition
+// of kernel_enforce_boundaries: density floor and x-wall px (YM_Fluid.cu:13-30).
+template <typename T>
+__device__ __forceinline__ void store_fluid(const State<T>& O, int b, size_t io, bool wall, T (&r)[F_COUNT]) {
+    if (r[F_N] < T(0.05)) r[F_N] = T(0.05);
+    if (wall) r[F_PX] = T(0.0);
+#pragma unroll
+    for (int v = 0; v < F_COUNT; ++v) O.a[b + v][io] = r[v];
+}
+
+constexpr int ZB_X = 64;           // columns per workgroup (one wave wide)
+constexpr int ZB_C = 4;            // z chunks per workgroup (one per wave)
+
+template <typename T, int TZ>
+__global__ __launch_bounds__(ZB_X * ZB_C) void k_fct_z(State<T> S, Stage<T> X, State<T> O,
+                                                       KParams p, int sp0, int nsp, int ntx, int ncg) {
+    // logical tile: species fastest (both species read the same fields for
+    // their sources), then 64-column groups, then groups of ZB_C z chunks
+    int lt = xcd_logical(blockIdx.x, gridDim.x);
+    if (lt >= nsp * ntx * ncg) return;
+    const int sp = sp0 + lt % nsp;
+    lt /= nsp;
+    const int x = (lt % ntx) * ZB_X + (threadIdx.x % ZB_X);
+    const int chunk = (lt / ntx) * ZB_C + (threadIdx.x / ZB_X);
+    const int z0 = chunk * TZ;
+    if (x >= p.nx || z0 >= p.nz) return;
+    const int nx = p.nx, nz = p.nz;
+    const int b = S_NA + sp * F_COUNT;
+    const T dt = T(p.dt), dtdz = T(p.dtdz);
+    const int zend = min(z0 + TZ, nz);
+    const bool wall = !p.periodic_x && (x == 0 || x == nx - 1);
+    ZPipe<T> zp;
+    zp.Um = T(0);
+    zp.MUm = T(0);
+    for (int e = z0 - 3; e <= zend + 2; ++e) {
+        const int rc = clampi(e, 0, nz - 1);
+        const size_t i = (size_t)x + (size_t)rc * nx;
+        T V[F_COUNT], Sv[F_COUNT], u, out[F_COUNT];
+#pragma unroll
+        for (int v = 0; v < F_COUNT; ++v) V[v] = X.a[sp * F_COUNT + v][i];
+        z_inputs(S, sp, x, rc, p, u, Sv);
+        if (zp.push(e, z0, V, u, Sv, dt, dtdz, out))
+            store_fluid(O, b, (size_t)x + (size_t)(e - 3) * nx, wall, out);
+    }
+}
+
+// ==========================================================================
+// fused x + z sweep
+// ==========================================================================
+// One wave owns 64 columns of one species and marches down a z chunk.  For
+// every extended row it stages the row segment (+3 halo) in LDS, performs the
+// x sweep there (the k_fct_x phases) and feeds its own column's x-swept values
+// straight into the register z pipeline, so the intermediate fluid never goes
+// to HBM.  The x sweep is recomputed for the 6 halo rows of each chunk.
+// Blocks are numbered XCD-aware: consecutive logical tiles (species pair, then
+// x tiles, then z chunks) land on the same XCD and share its L2.
+constexpr int XZ_W = 64;
+constexpr int XZ_P = XZ_W + 6;
+
+template <typename T, int TZ>
+__global__ __launch_bounds__(XZ_W) void k_fct_xz(State<T> S, State<T> O, KParams p, int ntx, int nch) {
+    __shared__ T sU[XZ_P], sS[XZ_P], sE[XZ_P], sNU[XZ_P], sMU[XZ_P];
+    __shared__ T sV[F_COUNT][XZ_P];
+    __shared__ T sLOR[F_COUNT][XZ_P];
+    __shared__ T sLNR[F_COUNT][XZ_P];
+
+    int lb = xcd_logical(blockIdx.x, gridDim.x);
+    if (lb >= 2 * ntx * nch) return;
+    const int sp = lb & 1;
+    lb >>= 1;
+    const int x0 = (lb % ntx) * XZ_W;
+    const int z0 = (lb / ntx) * TZ;
+    const int lane = threadIdx.x;
+    const int x = x0 + lane;
+    const int nx = p.nx, nz = p.nz;
+    if (z0 >= nz) return;                    // uniform per block
+    const bool active = x < nx;
+    const int b = S_NA + sp * F_COUNT;
+    const T dt = T(p.dt), dtdx = T(p.dtdx), dtdz = T(p.dtdz);
+    const int zend = min(z0 + TZ, nz);
+    const bool wall = !p.periodic_x && (x == 0 || x == nx - 1);
+    ZPipe<T> zp;
+    zp.Um = T(0);
+    zp.MUm = T(0);
+
+    for (int e = z0 - 3; e <= zend + 2; ++e) {
+        const int rc = clampi(e, 0, nz - 1);
+        const size_t row = (size_t)rc * nx;
+        // ---- x sweep of row rc, positions q <-> x = x0-3+q ----
+        for (int q = lane; q < XZ_P; q += XZ_W) {
+            const int xr = map_x(x0 - 3 + q, p);
+            const size_t i = row + xr;
+            T n = S.a[b + F_N][i];
+            T px = S.a[b + F_PX][i];
+            T n_safe = kmax(n, T(0.05));
+            sU[q] = px / n_safe;
+            sS[q] = lorentz_px(S, sp, xr, rc, p);
+            sV[F_N][q] = n;
+            sV[F_PX][q] = px;
+#pragma unroll
+            for (int v = F_PZ; v < F_COUNT; ++v) sV[v][q] = S.a[b + v][i];
+        }
+        __syncthreads();
+        for (int q = lane; q < XZ_P - 1; q += XZ_W) {
+            T vi = T(0.5) * (sU[q] + sU[q + 1]);
+            T eps = vi * dtdx;
+            T nu, mu;
+            fct_nu_mu(eps, nu, mu);
+            sE[q] = eps; sNU[q] = nu; sMU[q] = mu;
+        }
+        __syncthreads();
+        for (int q = 1 + lane; q < XZ_P - 1; q += XZ_W) {
+            const T eL = sE[q - 1], eR = sE[q];
+            const T nL = sNU[q - 1], nR = sNU[q];
+#pragma unroll
+            for (int v = 0; v < F_COUNT; ++v) {
+                const T vm = sV[v][q - 1], v0 = sV[v][q], vp = sV[v][q + 1];
+                T fL = T(0.5) * eL * (vm + v0);
+                T fR = T(0.5) * eR * (v0 + vp);
+                T dL = nL * (v0 - vm);
+                T dR = nR * (vp - v0);
+                T src = (v == F_PX) ? sS[q] : T(0.0);
+                T lor = v0 + dt * src - (fR - fL);
+                sLOR[v][q] = lor;
+                sLNR[v][q] = lor + (dR - dL);
+            }
+        }
+        __syncthreads();
+        for (int q = 2 + lane; q < XZ_P - 3; q += XZ_W) {
+            const T mu = sMU[q];
+#pragma unroll
+            for (int v = 0; v < F_COUNT; ++v) {
+                T flx = mu * (sLOR[v][q + 1] - sLOR[v][q]);
+                T l0 = sLNR[v][q - 1], l1 = sLNR[v][q], l2 = sLNR[v][q + 1], l3 = sLNR[v][q + 2];
+                sV[v][q] = fct_limit(flx, l1 - l0, l2 - l1, l3 - l2);
+            }
+        }
+        __syncthreads();
+        T V[F_COUNT];
+        {
+            const int q = 3 + lane;
+#pragma unroll
+            for (int v = 0; v < F_COUNT; ++v) V[v] = sLNR[v][q] - (sV[v][q] - sV[v][q - 1]);
+        }
+        __syncthreads();                     // LDS is restaged for the next row
+        if (!active) continue;
+        // ---- z pipeline ----
+        T Sv[F_COUNT], u, out[F_COUNT];
+        z_inputs(S, sp, x, rc, p, u, Sv);
+        if (zp.push(e, z0, V, u, Sv, dt, dtdz, out))
+            store_fluid(O, b, (size_t)x + (size_t)(e - 3) * nx, wall, out);
     }
 }
 
@@ -221,11 +366,15 @@ __global__ __launch_bounds__(ZB_X * ZB_C) void k_fct_z(State<T> S, Stage<T> X, S
 constexpr int MX = 64, MZ = 16, MZT = 4;   // tile 64 x 16, 64 x 4 threads
 
 template <typename T>
-__global__ __launch_bounds__(MX * MZT) void k_maxwell(State<T> S, State<T> O, KParams p) {
+__global__ __launch_bounds__(MX * MZT) void k_maxwell(State<T> S, State<T> O, KParams p, int ntx, int ntz,
+                                                      const double* decay, const int* action) {
     __shared__ T sEx[3][MZ + 1][MX + 1];
     __shared__ T sEz[3][MZ + 1][MX + 1];
     const int nx = p.nx, nz = p.nz;
-    const int x0 = blockIdx.x * MX, z0 = blockIdx.y * MZ;
+    // logical tile: x tiles fastest (they share the x +- 1 halo lines)
+    const int lt = xcd_logical(blockIdx.x, gridDim.x);
+    if (lt >= ntx * ntz) return;
+    const int x0 = (lt % ntx) * MX, z0 = (lt / ntx) * MZ;
     const int ncol = min(MX, nx - x0), nrow = min(MZ, nz - z0);
     const T dt = T(p.dt), c2 = T(p.c2), aYM = T(p.alpha), eps0 = T(p.eps0);
     const T dx = T(p.dx), dz = T(p.dz);
@@ -300,15 +449,24 @@ __global__ __launch_bounds__(MX * MZT) void k_maxwell(State<T> S, State<T> O, KP
         T cr[3] = {Az[1] * Ex[2] - Az[2] * Ex[1],
                    Az[2] * Ex[0] - Az[0] * Ex[2],
                    Az[0] * Ex[1] - Az[1] * Ex[0]};
+        // sponge / xi_cut of the colour-2/3 fields (YM_Physics.cu:161-210) act
+        // right after Faraday on values nothing else reads in between, so they
+        // are applied to the stores; the Q2/Q3 part stays in the filter pass.
+        const int act = action ? action[xr] : 0;
+        const T dec = (act == 1) ? T(decay[xr]) : T(1.0);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             T dEx = (sEx[c][iz + 1][ix] - Ex[c]) / dz;
             T dEz = (sEz[c][iz][ixp] - sEz[c][iz][ix]) / dx;
             T by = S.a[S_BY1 + c][i];
-            O.a[S_BY1 + c][i] = by - dt * (dEx - dEz - aYM * cr[c]);
-            O.a[S_EX1 + c][i] = Ex[c];
-            O.a[S_EZ1 + c][i] = sEz[c][iz][ix];
-            if (c > 0 || !p.freeze_az1) O.a[S_AZ1 + c][i] = Az[c];
+            T byn = by - dt * (dEx - dEz - aYM * cr[c]);
+            T exn = Ex[c], ezn = sEz[c][iz][ix], azn = Az[c];
+            if (c > 0 && act == 1) { byn *= dec; exn *= dec; ezn *= dec; azn *= dec; }
+            if (c > 0 && act == 2) { byn = T(0.0); exn = T(0.0); ezn = T(0.0); azn = T(0.0); }
+            O.a[S_BY1 + c][i] = byn;
+            O.a[S_EX1 + c][i] = exn;
+            O.a[S_EZ1 + c][i] = ezn;
+            if (c > 0 || !p.freeze_az1) O.a[S_AZ1 + c][i] = azn;
         }
     }
 }
@@ -316,33 +474,70 @@ __global__ __launch_bounds__(MX * MZT) void k_maxwell(State<T> S, State<T> O, KP
 // ==========================================================================
 // launchers
 // ==========================================================================
+template <typename T, int XB>
+static void launch_fct_x_t(const State<T>& S, const Stage<T>& out, const KParams& p, int sp0, int nsp,
+                           hipStream_t st) {
+    const int ntx = (p.nx + XB - 1) / XB;
+    dim3 grid(round8(ntx * p.nz * nsp));
+    hipLaunchKernelGGL((k_fct_x<T, XB>), grid, dim3(XB), 0, st, S, out, p, sp0, nsp, ntx);
+}
+
 template <typename T>
 void launch_fct_x(const State<T>& S, const Stage<T>& out, const KParams& p, int sp0, int nsp,
-                  hipStream_t st) {
-    dim3 grid((p.nx + XB - 1) / XB, p.nz, nsp);
-    hipLaunchKernelGGL(k_fct_x<T>, grid, dim3(XB), 0, st, S, out, p, sp0);
+                  int variant, hipStream_t st) {
+    switch (variant) {
+        case 0: launch_fct_x_t<T, 64>(S, out, p, sp0, nsp, st); break;
+        case 1: launch_fct_x_t<T, 128>(S, out, p, sp0, nsp, st); break;
+        default: launch_fct_x_t<T, 256>(S, out, p, sp0, nsp, st); break;
+    }
 }
 
+template <typename T, int TZ>
+static void launch_fct_z_t(const State<T>& S, const Stage<T>& X, const State<T>& O, const KParams& p,
+                           int sp0, int nsp, hipStream_t st) {
+    const int nchunk = (p.nz + TZ - 1) / TZ;
+    const int ncg = (nchunk + ZB_C - 1) / ZB_C;
+    const int ntx = (p.nx + ZB_X - 1) / ZB_X;
+    dim3 grid(round8(nsp * ntx * ncg));
+    hipLaunchKernelGGL((k_fct_z<T, TZ>), grid, dim3(ZB_X * ZB_C), 0, st, S, X, O, p, sp0, nsp, ntx, ncg);
+}
+
+// z chunk height: 16 / 32 / 64 rows per wave (halo rows re-read per chunk
+// vs. number of independent column marches in flight)
 template <typename T>
 void launch_fct_z(const State<T>& S, const Stage<T>& X, const State<T>& O, const KParams& p,
-                  int sp0, int nsp, hipStream_t st) {
-    constexpr int TZ = 16;
-    int nchunk = (p.nz + TZ - 1) / TZ;
-    dim3 grid((p.nx + ZB_X - 1) / ZB_X, (nchunk + ZB_C - 1) / ZB_C, nsp);
-    hipLaunchKernelGGL((k_fct_z<T, TZ>), grid, dim3(ZB_X * ZB_C), 0, st, S, X, O, p, sp0);
+                  int sp0, int nsp, int variant, hipStream_t st) {
+    switch (variant) {
+        case 0: launch_fct_z_t<T, 16>(S, X, O, p, sp0, nsp, st); break;
+        case 2: launch_fct_z_t<T, 64>(S, X, O, p, sp0, nsp, st); break;
+        default: launch_fct_z_t<T, 32>(S, X, O, p, sp0, nsp, st); break;
+    }
 }
 
 template <typename T>
-void launch_maxwell(const State<T>& S, const State<T>& O, const KParams& p, hipStream_t st) {
-    dim3 grid((p.nx + MX - 1) / MX, (p.nz + MZ - 1) / MZ);
-    hipLaunchKernelGGL(k_maxwell<T>, grid, dim3(MX * MZT), 0, st, S, O, p);
+void launch_fct_xz(const State<T>& S, const State<T>& O, const KParams& p, hipStream_t st) {
+    constexpr int TZ = 32;
+    const int ntx = (p.nx + XZ_W - 1) / XZ_W;
+    const int nch = (p.nz + TZ - 1) / TZ;
+    dim3 grid(round8(2 * ntx * nch));
+    hipLaunchKernelGGL((k_fct_xz<T, TZ>), grid, dim3(XZ_W), 0, st, S, O, p, ntx, nch);
 }
 
-template void launch_fct_x<double>(const State<double>&, const Stage<double>&, const KParams&, int, int, hipStream_t);
-template void launch_fct_x<float>(const State<float>&, const Stage<float>&, const KParams&, int, int, hipStream_t);
-template void launch_fct_z<double>(const State<double>&, const Stage<double>&, const State<double>&, const KParams&, int, int, hipStream_t);
-template void launch_fct_z<float>(const State<float>&, const Stage<float>&, const State<float>&, const KParams&, int, int, hipStream_t);
-template void launch_maxwell<double>(const State<double>&, const State<double>&, const KParams&, hipStream_t);
-template void launch_maxwell<float>(const State<float>&, const State<float>&, const KParams&, hipStream_t);
+template <typename T>
+void launch_maxwell(const State<T>& S, const State<T>& O, const KParams& p, const double* decay,
+                    const int* action, hipStream_t st) {
+    const int ntx = (p.nx + MX - 1) / MX, ntz = (p.nz + MZ - 1) / MZ;
+    dim3 grid(round8(ntx * ntz));
+    hipLaunchKernelGGL(k_maxwell<T>, grid, dim3(MX * MZT), 0, st, S, O, p, ntx, ntz, decay, action);
+}
+
+template void launch_fct_x<double>(const State<double>&, const Stage<double>&, const KParams&, int, int, int, hipStream_t);
+template void launch_fct_x<float>(const State<float>&, const Stage<float>&, const KParams&, int, int, int, hipStream_t);
+template void launch_fct_z<double>(const State<double>&, const Stage<double>&, const State<double>&, const KParams&, int, int, int, hipStream_t);
+template void launch_fct_z<float>(const State<float>&, const Stage<float>&, const State<float>&, const KParams&, int, int, int, hipStream_t);
+template void launch_fct_xz<double>(const State<double>&, const State<double>&, const KParams&, hipStream_t);
+template void launch_fct_xz<float>(const State<float>&, const State<float>&, const KParams&, hipStream_t);
+template void launch_maxwell<double>(const State<double>&, const State<double>&, const KParams&, const double*, const int*, hipStream_t);
+template void launch_maxwell<float>(const State<float>&, const State<float>&, const KParams&, const double*, const int*, hipStream_t);
 
 }  // namespace khs

@@ -60,6 +60,8 @@ class khs_config(ctypes.Structure):
         ("n_steps", ctypes.c_int), ("noise_amp", ctypes.c_double), ("noise_seed", ctypes.c_uint64),
         ("kz_keep_max", ctypes.c_int), ("precision", ctypes.c_int),
         ("energy_stride_override", ctypes.c_int), ("export_stride_override", ctypes.c_int),
+        ("fused_sweep", ctypes.c_int), ("autotune", ctypes.c_int),
+        ("x_segment", ctypes.c_int), ("z_chunk", ctypes.c_int),
         ("nx", ctypes.c_int), ("nz", ctypes.c_int), ("periodic_x", ctypes.c_int), ("freeze_az1", ctypes.c_int),
         ("lx", ctypes.c_double), ("lz", ctypes.c_double), ("dx", ctypes.c_double), ("dz", ctypes.c_double),
         ("courant", ctypes.c_double), ("dt", ctypes.c_double),
@@ -111,6 +113,7 @@ def load_library(path=None):
         "khs_timing_reset": (c_int, [c_void_p]),
         "khs_kernel_times": (c_int, [c_void_p, dp, P(c_int), c_int]),
         "khs_kernel_name": (c_char_p, [c_int]),
+        "khs_get_tuning": (c_int, [c_void_p, P(c_int), P(c_int)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -131,7 +134,7 @@ def exported_symbols():
             "khs_maxwell", "khs_filters", "khs_commit", "khs_step", "khs_get_step", "khs_energy",
             "khs_energy0", "khs_gauss", "khs_write_snapshot", "khs_energy_csv_open", "khs_energy_csv_row",
             "khs_energy_csv_close", "khs_sync", "khs_timing_enable", "khs_timing_reset", "khs_kernel_times",
-            "khs_kernel_name"]
+            "khs_kernel_name", "khs_get_tuning"]
 
 
 def _check(rc):
@@ -306,6 +309,12 @@ class Simulation:
 
     def timing_reset(self):
         _check(_lib.khs_timing_reset(self._h))
+
+    def tuning(self):
+        """(x-sweep segment width, z-sweep chunk height) in use."""
+        xs, zc = ctypes.c_int(), ctypes.c_int()
+        _check(_lib.khs_get_tuning(self._h, ctypes.byref(xs), ctypes.byref(zc)))
+        return xs.value, zc.value
 
     def kernel_times(self):
         n = 8

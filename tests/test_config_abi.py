@@ -51,6 +51,10 @@ def test_float_parse_quirk_gives_1999_steps():
     c2 = khs.Config(nx_override=256, nz_override=128, lx_override=25.6, lz_override=12.8,
                     courant_override=0.4, target_tu=80, n_steps=2000)
     assert c2.total_steps == 2000
+    # n_steps alone also selects the physical cadences
+    c3 = khs.Config(nx_override=256, nz_override=128, lx_override=25.6, lz_override=12.8,
+                    courant_override=0.4, n_steps=2000)
+    assert (c3.total_steps, c3.energy_stride, c3.export_stride) == (2000, 12, 24)
 
 
 def test_ini_file_and_inline_comments(tmp_path):

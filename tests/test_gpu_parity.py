@@ -114,9 +114,11 @@ STEP_CASES = {
 }
 
 
+@pytest.mark.parametrize("fused", [1, 0])
 @pytest.mark.parametrize("case", list(STEP_CASES))
-def test_step_bitwise(case):
-    keys = STEP_CASES[case]
+def test_step_bitwise(case, fused):
+    """Whole steps (fused x+z sweep kernel, and the split x / z kernels)."""
+    keys = dict(STEP_CASES[case], fused_sweep=fused)
     nx, nz = keys["nx_override"], keys["nz_override"]
     st = random_state(nx, nz, seed=3, smooth=True)
     cfg, sim, o = make_pair(st, **keys)
@@ -179,6 +181,20 @@ def test_full_size_step_bitwise():
     o.step(2)
     for n in orc.STATE:
         assert_bitwise(sim.get(n), o.get(n), f"2048x1024 {n}")
+
+
+@pytest.mark.parametrize("xs,zc", [(64, 16), (128, 64), (256, 32)])
+def test_tile_variants_bitwise(xs, zc):
+    """Every x-segment / z-chunk variant the autotuner may pick gives the same bits."""
+    keys = grid(300, 136, 1, warm_T=0.01, x_segment=xs, z_chunk=zc, autotune=0)
+    st = random_state(300, 136, seed=31, smooth=True)
+    cfg, sim, o = make_pair(st, **keys)
+    assert sim.tuning() == (xs, zc)
+    o.sources()
+    sim.step(3)
+    o.step(3)
+    for n in orc.STATE:
+        assert_bitwise(sim.get(n), o.get(n), f"x{xs} z{zc} {n}")
 
 
 def test_determinism_and_staged_equals_fused():
