@@ -372,6 +372,8 @@ static std::unique_ptr<Ctx> make_ctx(const khs_config& in) {
         if (c.xi_cut > 0.0f && xi > (double)c.xi_cut) action[x] = 2;
     }
     ctx->cell_filters = (c.xi_sponge > 0.0f) || (c.xi_cut > 0.0f);
+    if (c.x_segment) ctx->xvar = c.x_segment == 64 ? 0 : c.x_segment == 128 ? 1 : 2;
+    if (c.z_chunk) ctx->zvar = c.z_chunk == 16 ? 0 : c.z_chunk == 32 ? 1 : 2;
 
     // ---- one device allocation ----
     const bool share_az1 = c.freeze_az1 != 0;
