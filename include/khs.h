@@ -33,6 +33,9 @@ extern "C" {
 #define KHS_FP64 0            /* plain FP64 drop-in form */
 #define KHS_FP32 1            /* precision form: FP32 bulk, FP64 reductions */
 
+#define KHS_MODEL_YM   0      /* ymgpu2d: two-beam SU(2) Yang-Mills plasma */
+#define KHS_MODEL_EMHD 1      /* emhdgpu2d: Abelian electron-MHD fluid */
+
 #define KHS_SPECIES_A 0       /* beam +z, Q = (+1,0,0) at init */
 #define KHS_SPECIES_B 1       /* beam -z, Q = (-1,0,0) at init */
 
@@ -54,7 +57,16 @@ enum khs_array {
     KHS_SRCPXA, KHS_SRCPZA, KHS_SRCPXB, KHS_SRCPZB,     /* kernel_ym_lorentz */
     KHS_SQ1A, KHS_SQ2A, KHS_SQ3A, KHS_SQ1B, KHS_SQ2B, KHS_SQ3B, /* kernel_ym_precession */
     KHS_ENERGY_DENSITY,                                 /* kernel_ym_energy */
-    KHS_N_ARRAY_IDS
+    KHS_N_ARRAY_IDS,
+    /* EMHD model (emhdgpu2d): state ids, then derived ids */
+    KHS_EM_EX = 64, KHS_EM_EY, KHS_EM_EZ, KHS_EM_BX, KHS_EM_BY, KHS_EM_BZ,
+    KHS_EM_AX, KHS_EM_AY, KHS_EM_AZ,
+    KHS_EM_N = 76, KHS_EM_PX, KHS_EM_PY, KHS_EM_PZ,          /* fluid 1 */
+    KHS_EM_N2 = 82, KHS_EM_PX2, KHS_EM_PY2, KHS_EM_PZ2,      /* fluid 2 (extension) */
+    KHS_EM_JX = 96, KHS_EM_JY, KHS_EM_JZ,                    /* kernel_compute_currents */
+    KHS_EM_UX, KHS_EM_UZ, KHS_EM_UX2, KHS_EM_UZ2,            /* update_velocities */
+    KHS_EM_SX, KHS_EM_SY, KHS_EM_SZ, KHS_EM_SX2, KHS_EM_SY2, KHS_EM_SZ2,  /* kernel_compute_lorentz */
+    KHS_EM_ENERGY_DENSITY                                    /* kernel_energy_density */
 };
 
 /* Run configuration.  The first block holds the reference's RunConfig keys
@@ -101,6 +113,19 @@ typedef struct khs_config {
     int      autotune;         /* 1 (default): time the tile variants on the first step, keep the fastest */
     int      x_segment;        /* 0 = tuned, or force 64 / 128 / 256 cells per x-sweep segment */
     int      z_chunk;          /* 0 = tuned, or force 16 / 32 / 64 rows per z-sweep chunk */
+    /* --- model selection and EMHD keys (emhdgpu2d takes k_mode and
+     *     perturb_amp from argv, main_coupled.cu:228-229; grid 256x256, dx=1,
+     *     dt=0.002 fixed, :236-243) --- */
+    int      model;            /* KHS_MODEL_YM (default) or KHS_MODEL_EMHD */
+    int      species;          /* EMHD fluids: 1 (reference) or 2 (extension) */
+    int      periodic;         /* EMHD: 0 = walls in x (reference), 1 = periodic x (extension) */
+    double   charge1, charge2; /* EMHD fluid charges (reference fluid: -1; second fluid: +1) */
+    int      emhd_init;        /* 0 = das_kaw shear equilibrium (reference), 1 = random-phase spectrum */
+    int      smooth_init;      /* EMHD: one 5-point smoothing of By, Ex after init (reference: 1) */
+    int      spectrum_kmax;    /* spectrum init: modes kx, kz = 1..kmax in By */
+    double   spectrum_amp;
+    uint64_t spectrum_seed;
+    double   em_perturb_amp;   /* EMHD das_kaw perturbation (argv[2] as double, default 0.05); set by perturb_amp */
     /* --- derived (khs_derive) --- */
     int    nx, nz;
     int    periodic_x;         /* run_mode >= 1 */

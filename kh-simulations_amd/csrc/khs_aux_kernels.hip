@@ -371,6 +371,88 @@ void launch_derived(const State<T>& S, int id, const KParams& p, double* out, hi
     hipLaunchKernelGGL(k_derived<T>, dim3(blocks), dim3(256), 0, st, S, id, p, out);
 }
 
+
+// --------------------------------------------------------------------------
+// EMHD energy and derived arrays (emhdgpu2d/main_coupled.cu:99-136, 197-222)
+// --------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ double energy_cell_em(const State<T>& S, int nsp, size_t i) {
+    double ke = 0.0;
+    for (int s = 0; s < nsp; ++s) {
+        const int b = em_base(s);
+        double n_safe = kmax((double)S.a[b + EV_N][i], 1e-10);
+        double px = S.a[b + EV_PX][i], py = S.a[b + EV_PY][i], pz = S.a[b + EV_PZ][i];
+        double k = 0.5 * (px * px + py * py + pz * pz) / n_safe;
+        ke = (s == 0) ? k : ke + k;
+    }
+    double ex = S.a[E_EX][i], ey = S.a[E_EY][i], ez = S.a[E_EZ][i];
+    double bx = S.a[E_BX][i], by = S.a[E_BY][i], bz = S.a[E_BZ][i];
+    double em = 0.5 * (ex * ex + ey * ey + ez * ez + bx * bx + by * by + bz * bz);
+    return ke + em;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void k_energy_em(State<T> S, KParams p, double* partials) {
+    const size_t N = (size_t)p.nx * p.nz;
+    CSum acc[1] = {CSum{0.0, 0.0}};
+    for (size_t i = blockIdx.x * (size_t)kRedThreads + threadIdx.x; i < N; i += (size_t)gridDim.x * kRedThreads)
+        cadd(acc[0], energy_cell_em(S, p.nsp, i));
+    block_reduce_store<1>(acc, partials);
+}
+
+template <typename T>
+void launch_energy_em(const State<T>& S, const KParams& p, double* partials, double* out, hipStream_t st) {
+    size_t N = (size_t)p.nx * p.nz;
+    int blocks = (int)std::min<size_t>((N + kRedThreads - 1) / kRedThreads, kRedBlocks);
+    hipLaunchKernelGGL(k_energy_em<T>, dim3(blocks), dim3(kRedThreads), 0, st, S, p, partials);
+    hipLaunchKernelGGL(k_reduce_final<1>, dim3(1), dim3(kRedThreads), 0, st, partials, blocks, out);
+}
+
+template <typename T>
+__global__ void k_derived_em(State<T> S, int which, KParams p, double* out) {
+    const size_t N = (size_t)p.nx * p.nz;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int x = (int)(i % p.nx), z = (int)(i / p.nx);
+        double r = 0.0;
+        if (which <= 2) {                                   // J (kernel_compute_currents)
+            const int xp = xp_src(x, p), zp = zp_per(z, p.nz);
+            const size_t ixp = (size_t)xp + (size_t)z * p.nx, izp = (size_t)x + (size_t)zp * p.nx;
+            T j = T(0);
+            for (int s = 0; s < p.nsp; ++s) {
+                const int b = em_base(s);
+                const T q = T(s ? p.q1 : p.q0), h = T((s ? p.q1 : p.q0) * 0.5);
+                T v;
+                if (which == 0) v = h * (S.a[b + EV_PX][i] + S.a[b + EV_PX][ixp]);
+                else if (which == 1) v = q * S.a[b + EV_PY][i];
+                else v = h * (S.a[b + EV_PZ][i] + S.a[b + EV_PZ][izp]);
+                j = (s == 0) ? v : j + v;
+            }
+            r = (double)j;
+        } else if (which <= 6) {                            // velocities
+            const int s = (which - 3) / 2, comp = (which - 3) % 2;
+            const int b = em_base(s);
+            T n_safe = kmax(S.a[b + EV_N][i], T(0.05));
+            r = (double)(S.a[b + (comp ? EV_PZ : EV_PX)][i] / n_safe);
+        } else if (which <= 12) {                           // Lorentz sources
+            const int s = (which - 7) / 3, c = (which - 7) % 3;
+            T f[3];
+            em_lorentz(S, s, x, z, p, f);
+            r = (double)f[c];
+        } else {
+            r = energy_cell_em(S, p.nsp, i);
+        }
+        out[i] = r;
+    }
+}
+
+template <typename T>
+void launch_derived_em(const State<T>& S, int which, const KParams& p, double* out, hipStream_t st) {
+    size_t N = (size_t)p.nx * p.nz;
+    int blocks = (int)std::min<size_t>((N + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_derived_em<T>, dim3(blocks), dim3(256), 0, st, S, which, p, out);
+}
+
 // explicit instantiations
 #define KHS_INST(T)                                                                                   \
     template void launch_cell_filters<T>(const ArrayList<T>&, const double*, const int*, const KParams&, hipStream_t); \
@@ -380,7 +462,9 @@ void launch_derived(const State<T>& S, int id, const KParams& p, double* out, hi
     template void launch_fill_zero<T>(const ArrayList<T>&, size_t, hipStream_t);                    \
     template void launch_energy<T>(const State<T>&, const KParams&, double*, double*, hipStream_t);  \
     template void launch_gauss<T>(const State<T>&, const KParams&, double*, double*, hipStream_t);   \
-    template void launch_derived<T>(const State<T>&, int, const KParams&, double*, hipStream_t);
+    template void launch_derived<T>(const State<T>&, int, const KParams&, double*, hipStream_t);   \
+    template void launch_energy_em<T>(const State<T>&, const KParams&, double*, double*, hipStream_t); \
+    template void launch_derived_em<T>(const State<T>&, int, const KParams&, double*, hipStream_t);
 KHS_INST(double)
 KHS_INST(float)
 #undef KHS_INST

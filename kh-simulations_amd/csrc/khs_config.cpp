@@ -60,6 +60,17 @@ void config_defaults(khs_config& c) {
     c.autotune = 1;
     c.x_segment = 0;
     c.z_chunk = 0;
+    c.model = KHS_MODEL_YM;
+    c.species = 1;
+    c.periodic = 0;
+    c.charge1 = -1.0;
+    c.charge2 = 1.0;
+    c.emhd_init = 0;
+    c.smooth_init = 1;
+    c.spectrum_kmax = 32;
+    c.spectrum_amp = 1e-5;
+    c.spectrum_seed = 2;
+    c.em_perturb_amp = 0.05;
 }
 
 static int parse_int(const std::string& k, const std::string& v) {
@@ -86,7 +97,10 @@ bool config_set(khs_config& c, const std::string& key, const std::string& val) {
     // reference keys (YM_Config.cu:99-123)
     if      (key == "k_mode")            c.k_mode = parse_int(key, val);
     else if (key == "alpha_YM")          c.alpha_YM = parse_float(key, val);
-    else if (key == "perturb_amp")       c.perturb_amp = parse_float(key, val);
+    else if (key == "perturb_amp") {
+        c.perturb_amp = parse_float(key, val);
+        c.em_perturb_amp = parse_double(key, val);
+    }
     else if (key == "run_mode")          c.run_mode = parse_int(key, val);
     else if (key == "V0")                c.V0 = parse_float(key, val);
     else if (key == "xi_sponge")         c.xi_sponge = parse_float(key, val);
@@ -124,6 +138,24 @@ bool config_set(khs_config& c, const std::string& key, const std::string& val) {
     else if (key == "fused_sweep")       c.fused_sweep = parse_int(key, val);
     else if (key == "autotune")          c.autotune = parse_int(key, val);
     else if (key == "x_segment")         c.x_segment = parse_int(key, val);
+    else if (key == "model") {
+        if (val == "ym") c.model = KHS_MODEL_YM;
+        else if (val == "emhd") c.model = KHS_MODEL_EMHD;
+        else throw Error(KHS_ERR_CONFIG, "model must be ym or emhd, got '" + val + "'");
+    }
+    else if (key == "species")           c.species = parse_int(key, val);
+    else if (key == "periodic")          c.periodic = parse_int(key, val);
+    else if (key == "charge1")           c.charge1 = parse_double(key, val);
+    else if (key == "charge2")           c.charge2 = parse_double(key, val);
+    else if (key == "emhd_init") {
+        if (val == "daskaw" || val == "0") c.emhd_init = 0;
+        else if (val == "spectrum" || val == "1") c.emhd_init = 1;
+        else throw Error(KHS_ERR_CONFIG, "emhd_init must be daskaw or spectrum");
+    }
+    else if (key == "smooth_init")       c.smooth_init = parse_int(key, val);
+    else if (key == "spectrum_kmax")     c.spectrum_kmax = parse_int(key, val);
+    else if (key == "spectrum_amp")      c.spectrum_amp = parse_double(key, val);
+    else if (key == "spectrum_seed")     c.spectrum_seed = (uint64_t)std::stoull(val);
     else if (key == "z_chunk")           c.z_chunk = parse_int(key, val);
     else return false;
     return true;
@@ -155,7 +187,43 @@ void config_load(const std::string& path, khs_config& c, std::string* warnings) 
     config_derive(c);
 }
 
+// emhdgpu2d run (main_coupled.cu:228-260): 256x256, dx = dz = 1, dt = 0.002,
+// energy check every 500 steps, snapshot every 1000, halt at E/E0 > 5.
+static void config_derive_emhd(khs_config& c) {
+    c.nz = (c.nz_override > 0) ? c.nz_override : 256;
+    c.nx = (c.nx_override > 0) ? c.nx_override : 256;
+    c.lx = (c.lx_override > 0.f) ? (double)c.lx_override : (double)c.nx;
+    c.lz = (c.lz_override > 0.f) ? (double)c.lz_override : (double)c.nz;
+    c.dx = c.lx / c.nx;
+    c.dz = c.lz / c.nz;
+    c.courant = (c.courant_override > 0.f) ? (double)c.courant_override : 0.002;
+    c.dt = c.courant * c.dx;
+    c.v0 = 0.1;
+    c.alpha = 0.0;
+    c.eps = 10.0 * c.dx;
+    c.c = 1.0;
+    c.eps_0 = 1.0;
+    c.periodic_x = c.periodic ? 1 : 0;
+    c.freeze_az1 = 0;
+    c.total_steps = (c.target_tu > 0.0f) ? (int)((double)c.target_tu / c.dt) : 500000;
+    if (c.n_steps > 0) c.total_steps = c.n_steps;
+    c.energy_stride = (c.energy_stride_override > 0) ? c.energy_stride_override : 500;
+    c.export_stride = (c.export_stride_override > 0) ? c.export_stride_override : 1000;
+    if (c.export_stride_override < 0) c.export_stride = 0;
+    c.energy_thresh = 5.0;
+    if (c.nx < 4 || c.nz < 4) throw Error(KHS_ERR_CONFIG, "grid too small");
+    if (c.species != 1 && c.species != 2) throw Error(KHS_ERR_CONFIG, "species must be 1 or 2");
+    if (c.x_segment != 0 && c.x_segment != 64 && c.x_segment != 128 && c.x_segment != 256)
+        throw Error(KHS_ERR_CONFIG, "x_segment must be 0, 64, 128 or 256");
+    if (c.z_chunk != 0 && c.z_chunk != 16 && c.z_chunk != 32 && c.z_chunk != 64)
+        throw Error(KHS_ERR_CONFIG, "z_chunk must be 0, 16, 32 or 64");
+    if (c.emhd_init == 1 && (c.spectrum_kmax < 1 || 2 * c.spectrum_kmax >= std::min(c.nx, c.nz)))
+        throw Error(KHS_ERR_CONFIG, "spectrum_kmax must satisfy 1 <= K < min(nx, nz)/2");
+}
+
 void config_derive(khs_config& c) {
+    if (c.model == KHS_MODEL_EMHD) { config_derive_emhd(c); return; }
+    if (c.model != KHS_MODEL_YM) throw Error(KHS_ERR_CONFIG, "unknown model");
     c.nz = (c.nz_override > 0) ? c.nz_override : 64;
     c.nx = (c.nx_override > 0) ? c.nx_override : 768;
     c.lx = (c.lx_override > 0.f) ? (double)c.lx_override : (double)(6.0 * M_PI);
@@ -208,8 +276,10 @@ void config_derive(khs_config& c) {
         throw Error(KHS_ERR_CONFIG, "run_mode must be 0..6");
 }
 
-// Output directory name (main_ym.cu:44-49, 143-170).
+// Output directory name (main_ym.cu:44-49, 143-170; EMHD main_coupled.cu:262-264).
 std::string output_dir_name(const khs_config& c) {
+    if (c.model == KHS_MODEL_EMHD)
+        return "coupled_k" + std::to_string(c.k_mode) + (c.em_perturb_amp == 0.0 ? "_noperturb" : "");
     const int run_mode = c.run_mode;
     const char* mode_tag = (run_mode == 1) ? "_circ"
                          : (run_mode == 2) ? "_emhd"

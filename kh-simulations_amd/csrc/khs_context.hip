@@ -65,6 +65,8 @@ struct Ctx {
     int nx = 0, nz = 0;
     size_t N = 0, stride = 0, esz = 8;
     bool fp32 = false;
+    bool emhd = false;      // emhdgpu2d model (fields 0..8, fluids at 12 / 18)
+    int nsp = 2;            // fluid species of the model
 
     char* dmem = nullptr;
     void* st[2][S_COUNT] = {};
@@ -160,12 +162,20 @@ struct Ctx {
     }
 
     // ---- stages ----
-    template <typename T> void do_x(int sp0, int nsp) {
-        timed(K_FCT_X, [&] { launch_fct_x<T>(state<T>(cur), stg<T>(), kp, sp0, nsp, xvar, stream); });
+    template <typename T> void fct_x(int sp0, int ns, int v) {
+        if (emhd) launch_fct_x<T, EMModel>(state<T>(cur), stg<T>(), kp, sp0, ns, v, stream);
+        else launch_fct_x<T, YMModel>(state<T>(cur), stg<T>(), kp, sp0, ns, v, stream);
+    }
+    template <typename T> void fct_z(int sp0, int ns, int v) {
+        if (emhd) launch_fct_z<T, EMModel>(state<T>(cur), stg<T>(), state<T>(1 - cur), kp, sp0, ns, v, stream);
+        else launch_fct_z<T, YMModel>(state<T>(cur), stg<T>(), state<T>(1 - cur), kp, sp0, ns, v, stream);
+    }
+    template <typename T> void do_x(int sp0, int ns) {
+        timed(K_FCT_X, [&] { fct_x<T>(sp0, ns, xvar); });
         check_launch();
     }
-    template <typename T> void do_z(int sp0, int nsp) {
-        timed(K_FCT_Z, [&] { launch_fct_z<T>(state<T>(cur), stg<T>(), state<T>(1 - cur), kp, sp0, nsp, zvar, stream); });
+    template <typename T> void do_z(int sp0, int ns) {
+        timed(K_FCT_Z, [&] { fct_z<T>(sp0, ns, zvar); });
         check_launch();
     }
     // Time every tile variant of the two sweeps on this grid (3 launches each
@@ -198,13 +208,9 @@ struct Ctx {
         if (fx >= 0) xvar = fx;
         if (fz >= 0) zvar = fz;
         if (!cfg.autotune) return;
-        if (fx < 0)
-            xvar = pick<T>(kNumXVariants, [&](int v) { launch_fct_x<T>(state<T>(cur), stg<T>(), kp, 0, 2, v, stream); });
-        launch_fct_x<T>(state<T>(cur), stg<T>(), kp, 0, 2, xvar, stream);
-        if (fz < 0)
-            zvar = pick<T>(kNumZVariants, [&](int v) {
-                launch_fct_z<T>(state<T>(cur), stg<T>(), state<T>(1 - cur), kp, 0, 2, v, stream);
-            });
+        if (fx < 0) xvar = pick<T>(kNumXVariants, [&](int v) { fct_x<T>(0, nsp, v); });
+        fct_x<T>(0, nsp, xvar);
+        if (fz < 0) zvar = pick<T>(kNumZVariants, [&](int v) { fct_z<T>(0, nsp, v); });
     }
     void autotune() {
         if (tuned) return;
@@ -216,8 +222,11 @@ struct Ctx {
     }
     template <typename T> void do_maxwell() {
         timed(K_MAXWELL, [&] {
-            launch_maxwell<T>(state<T>(cur), state<T>(1 - cur), kp, cell_filters ? d_decay : nullptr,
-                              cell_filters ? d_action : nullptr, stream);
+            if (emhd)
+                launch_emhd_maxwell<T>(state<T>(cur), state<T>(1 - cur), kp, stream);
+            else
+                launch_maxwell<T>(state<T>(cur), state<T>(1 - cur), kp, cell_filters ? d_decay : nullptr,
+                                  cell_filters ? d_action : nullptr, stream);
         });
         check_launch();
     }
@@ -231,6 +240,7 @@ struct Ctx {
                                            t.nmodes, project, d_coef, kp, stream);
     }
     template <typename T> void do_filters() {
+        if (emhd) return;                 // the EMHD step has no filters
         const int k = 1 - cur;
         const khs_config& c = cfg;
         const bool skip_c1 = c.suppress_kz0 != 0;   // colour-1 EM is zeroed at the end anyway
@@ -267,8 +277,13 @@ struct Ctx {
 
     double energy() {
         timed(K_ENERGY, [&] {
-            if (fp32) launch_energy<float>(state<float>(cur), kp, d_partials, d_red, stream);
-            else launch_energy<double>(state<double>(cur), kp, d_partials, d_red, stream);
+            if (emhd) {
+                if (fp32) launch_energy_em<float>(state<float>(cur), kp, d_partials, d_red, stream);
+                else launch_energy_em<double>(state<double>(cur), kp, d_partials, d_red, stream);
+            } else {
+                if (fp32) launch_energy<float>(state<float>(cur), kp, d_partials, d_red, stream);
+                else launch_energy<double>(state<double>(cur), kp, d_partials, d_red, stream);
+            }
         });
         check_launch();
         HIPCHK(hipMemcpyAsync(h_red, d_red, sizeof(double), hipMemcpyDeviceToHost, stream));
@@ -303,6 +318,8 @@ static std::unique_ptr<Ctx> make_ctx(const khs_config& in) {
     config_derive(ctx->cfg);          // validates; idempotent on a derived config
     const khs_config& c = ctx->cfg;
     ctx->nx = c.nx; ctx->nz = c.nz;
+    ctx->emhd = (c.model == KHS_MODEL_EMHD);
+    ctx->nsp = ctx->emhd ? c.species : 2;
     ctx->N = (size_t)c.nx * c.nz;
     ctx->fp32 = (c.precision == KHS_FP32);
     ctx->esz = ctx->fp32 ? 4 : 8;
@@ -319,6 +336,9 @@ static std::unique_ptr<Ctx> make_ctx(const khs_config& in) {
     kp.eps0 = c.eps_0;
     kp.alpha = c.alpha;
     kp.warm_T = (double)c.warm_T;
+    kp.q0 = c.charge1;
+    kp.q1 = c.charge2;
+    kp.nsp = ctx->nsp;
 
     int count;
     if (hipGetDeviceCount(&count) != hipSuccess || count < 1)
@@ -450,6 +470,26 @@ static void download(Ctx& ctx, const void* src, bool isf, double* h) {
     }
 }
 
+// State slot of an array id of the context's model, or -1.
+//   YM   : ids 0..23 are the slots.
+//   EMHD : ids 64..72 fields -> slots 0..8; 76..79 / 82..85 fluid n px py pz
+//          of species 0 / 1 -> slots 12..15 / 18..21.
+static int state_slot(const Ctx& c, int id) {
+    if (!c.emhd) return (id >= 0 && id < KHS_N_STATE) ? id : -1;
+    if (id >= KHS_EM_EX && id <= KHS_EM_AZ) return id - 64;
+    if (id >= KHS_EM_N && id <= KHS_EM_PZ) return id - 64;
+    if (c.nsp > 1 && id >= KHS_EM_N2 && id <= KHS_EM_PZ2) return id - 64;
+    return -1;
+}
+
+// Stage slot of a fluid state slot (species sp, variable v -> sp * NV + v).
+static int stage_slot(const Ctx& c, int slot) {
+    if (!c.emhd) return slot >= S_NA ? slot - S_NA : -1;
+    if (slot < 12) return -1;
+    const int sp = (slot - 12) / 6, v = (slot - 12) % 6;
+    return (v < EV_COUNT) ? sp * EV_COUNT + v : -1;
+}
+
 static void require_idle(const Ctx& c, const char* what) {
     if (c.staged_x || c.staged_z || c.maxwell_done || c.filters_done)
         throw Error(KHS_ERR_STATE, std::string(what) + " called in the middle of a staged step (call khs_commit first)");
@@ -463,11 +503,11 @@ static void commit(Ctx& c) {
 }
 
 static void full_step(Ctx& c) {
-    if (c.cfg.fused_sweep) {
+    if (c.cfg.fused_sweep && !c.emhd) {
         c.xz_stage();
     } else {
-        c.x_stage(0, 2);
-        c.z_stage(0, 2);
+        c.x_stage(0, c.nsp);
+        c.z_stage(0, c.nsp);
     }
     c.maxwell_stage();
     c.filter_stage();
@@ -582,7 +622,8 @@ int khs_init(khs_ctx* ctx, const char* seed_path) {
         khs::SeedProfiles seed;
         if (!sp.empty()) seed = khs::read_seed_file(sp, c.nx);
         std::vector<std::vector<double>> host;
-        khs::host_initial_state(c.cfg, seed, host);
+        if (c.emhd) khs::host_initial_state_em(c.cfg, host);
+        else khs::host_initial_state(c.cfg, seed, host);
         c.cur = 0;
         c.staged_x = c.staged_z = 0;
         c.maxwell_done = c.filters_done = false;
@@ -609,6 +650,18 @@ int khs_array_id(const char* name) {
                                    "sQ1B", "sQ2B", "sQ3B", "energy_density"};
     for (int i = 0; i < (int)(sizeof(dnames) / sizeof(dnames[0])); ++i)
         if (std::strcmp(name, dnames[i]) == 0) return KHS_JX1 + i;
+    // EMHD names (a name shared with the YM set resolves to the YM id above;
+    // the EMHD energy density is "em_energy_density")
+    static const char* enames[] = {"Ex", "Ey", "Ez", "Bx", "By", "Bz", "Ax", "Ay", "Az"};
+    for (int i = 0; i < 9; ++i)
+        if (std::strcmp(name, enames[i]) == 0) return KHS_EM_EX + i;
+    static const char* fnames[] = {"n", "px", "py", "pz", "n2", "px2", "py2", "pz2"};
+    for (int i = 0; i < 8; ++i)
+        if (std::strcmp(name, fnames[i]) == 0) return (i < 4 ? KHS_EM_N : KHS_EM_N2) + (i & 3);
+    static const char* ednames[] = {"Jx", "Jy", "Jz", "ux", "uz", "ux2", "uz2", "sx", "sy", "sz",
+                                    "sx2", "sy2", "sz2", "em_energy_density"};
+    for (int i = 0; i < 14; ++i)
+        if (std::strcmp(name, ednames[i]) == 0) return KHS_EM_JX + i;
     return -1;
 }
 
@@ -617,9 +670,18 @@ int khs_get_array(khs_ctx* ctx, int id, double* host, size_t n) {
     return guard([&] {
         Ctx& c = *ctx->impl;
         if (!host || n != c.N) throw Error(KHS_ERR_ARG, "host buffer must hold nx*nz doubles");
-        if (id >= 0 && id < KHS_N_STATE) {
-            khs::download(c, c.st[c.cur][id], c.fp32, host);
-        } else if (id >= KHS_JX1 && id < KHS_N_ARRAY_IDS) {
+        const int slot = khs::state_slot(c, id);
+        if (slot >= 0) {
+            khs::download(c, c.st[c.cur][slot], c.fp32, host);
+        } else if (c.emhd && id >= KHS_EM_JX && id <= KHS_EM_ENERGY_DENSITY) {
+            const int w = id - KHS_EM_JX;
+            if (c.nsp < 2 && (w == 5 || w == 6 || w >= 10 && w <= 12))
+                throw Error(KHS_ERR_ARG, "array id " + std::to_string(id) + " needs species = 2");
+            if (c.fp32) khs::launch_derived_em<float>(c.state<float>(c.cur), w, c.kp, c.d_derived, c.stream);
+            else khs::launch_derived_em<double>(c.state<double>(c.cur), w, c.kp, c.d_derived, c.stream);
+            c.check_launch();
+            khs::download(c, c.d_derived, false, host);
+        } else if (!c.emhd && id >= KHS_JX1 && id < KHS_N_ARRAY_IDS) {
             if (c.fp32) khs::launch_derived<float>(c.state<float>(c.cur), id, c.kp, c.d_derived, c.stream);
             else khs::launch_derived<double>(c.state<double>(c.cur), id, c.kp, c.d_derived, c.stream);
             c.check_launch();
@@ -635,13 +697,15 @@ int khs_get_array_buf(khs_ctx* ctx, int buf, int id, double* host, size_t n) {
     return guard([&] {
         Ctx& c = *ctx->impl;
         if (!host || n != c.N) throw Error(KHS_ERR_ARG, "host buffer must hold nx*nz doubles");
-        if (id < 0 || id >= KHS_N_STATE) throw Error(KHS_ERR_ARG, "state ids only (0..23)");
+        const int slot = khs::state_slot(c, id);
+        if (slot < 0) throw Error(KHS_ERR_ARG, "state ids of the model only");
         const void* src = nullptr;
-        if (buf == KHS_BUF_CURRENT) src = c.st[c.cur][id];
-        else if (buf == KHS_BUF_NEXT) src = c.st[1 - c.cur][id];
+        if (buf == KHS_BUF_CURRENT) src = c.st[c.cur][slot];
+        else if (buf == KHS_BUF_NEXT) src = c.st[1 - c.cur][slot];
         else if (buf == KHS_BUF_STAGE) {
-            if (id < KHS_NA) throw Error(KHS_ERR_ARG, "the stage holds fluid arrays only");
-            src = c.stage[id - KHS_NA];
+            const int g = khs::stage_slot(c, slot);
+            if (g < 0) throw Error(KHS_ERR_ARG, "the stage holds fluid arrays only");
+            src = c.stage[g];
         } else throw Error(KHS_ERR_ARG, "unknown buffer");
         khs::download(c, src, c.fp32, host);
     });
@@ -652,9 +716,10 @@ int khs_set_array(khs_ctx* ctx, int id, const double* host, size_t n) {
     return guard([&] {
         Ctx& c = *ctx->impl;
         if (!host || n != c.N) throw Error(KHS_ERR_ARG, "host buffer must hold nx*nz doubles");
-        if (id < 0 || id >= KHS_N_STATE) throw Error(KHS_ERR_ARG, "only state arrays (0..23) can be set");
+        const int slot = khs::state_slot(c, id);
+        if (slot < 0) throw Error(KHS_ERR_ARG, "only state arrays of the model can be set");
         khs::require_idle(c, "khs_set_array");
-        khs::upload_state(c, c.cur, id, host);
+        khs::upload_state(c, c.cur, slot, host);
     });
 }
 
@@ -662,7 +727,7 @@ int khs_fct_x_sweep(khs_ctx* ctx, int species) {
     CTX_OR_FAIL(ctx);
     return guard([&] {
         Ctx& c = *ctx->impl;
-        if (species != 0 && species != 1) throw Error(KHS_ERR_ARG, "species must be 0 or 1");
+        if (species < 0 || species >= c.nsp) throw Error(KHS_ERR_ARG, "species out of range");
         if (c.staged_z || c.maxwell_done) throw Error(KHS_ERR_STATE, "x-sweep after z-sweep: commit the step first");
         c.x_stage(species, 1);
         c.staged_x |= (1 << species);
@@ -673,7 +738,7 @@ int khs_fct_z_sweep(khs_ctx* ctx, int species) {
     CTX_OR_FAIL(ctx);
     return guard([&] {
         Ctx& c = *ctx->impl;
-        if (species != 0 && species != 1) throw Error(KHS_ERR_ARG, "species must be 0 or 1");
+        if (species < 0 || species >= c.nsp) throw Error(KHS_ERR_ARG, "species out of range");
         if (!(c.staged_x & (1 << species))) throw Error(KHS_ERR_STATE, "z-sweep before x-sweep of this species");
         c.z_stage(species, 1);
         c.staged_z |= (1 << species);
@@ -684,7 +749,7 @@ int khs_maxwell(khs_ctx* ctx) {
     CTX_OR_FAIL(ctx);
     return guard([&] {
         Ctx& c = *ctx->impl;
-        if (c.staged_z != 3) throw Error(KHS_ERR_STATE, "khs_maxwell needs both species z-swept");
+        if (c.staged_z != (1 << c.nsp) - 1) throw Error(KHS_ERR_STATE, "khs_maxwell needs every species z-swept");
         c.maxwell_stage();
         c.maxwell_done = true;
     });
@@ -758,6 +823,7 @@ int khs_gauss(khs_ctx* ctx, double out[7]) {
     if (!out) { g_last_error = "null argument"; return KHS_ERR_ARG; }
     return guard([&] {
         Ctx& c = *ctx->impl;
+        if (c.emhd) throw Error(KHS_ERR_ARG, "khs_gauss: the colour Gauss law is a YM diagnostic");
         if (c.fp32) khs::launch_gauss<float>(c.state<float>(c.cur), c.kp, c.d_partials, c.d_red, c.stream);
         else khs::launch_gauss<double>(c.state<double>(c.cur), c.kp, c.d_partials, c.d_red, c.stream);
         c.check_launch();
@@ -774,6 +840,24 @@ int khs_write_snapshot(khs_ctx* ctx, const char* dir, int step) {
     if (!dir) { g_last_error = "null argument"; return KHS_ERR_ARG; }
     return guard([&] {
         Ctx& c = *ctx->impl;
+        if (c.emhd) {
+            // emhdgpu2d/main_coupled.cu:286-300: coupled_%05d.csv with X,Z,By,Pz
+            std::vector<double> by(c.N), pz(c.N);
+            khs::download(c, c.st[c.cur][khs::E_BY], c.fp32, by.data());
+            khs::download(c, c.st[c.cur][12 + khs::EV_PZ], c.fp32, pz.data());
+            std::ostringstream fn;
+            fn << dir << "/coupled_" << std::setfill('0') << std::setw(5) << step << ".csv";
+            std::ofstream out(fn.str());
+            if (!out) throw Error(KHS_ERR_IO, "cannot write " + fn.str());
+            out << "X,Z,By,Pz\n";
+            for (int z = 0; z < c.nz; ++z)
+                for (int x = 0; x < c.nx; ++x) {
+                    size_t i = (size_t)x + (size_t)z * c.nx;
+                    out << x * c.cfg.dx << ',' << z * c.cfg.dz << ',' << by[i] << ',' << pz[i] << '\n';
+                }
+            if (!out) throw Error(KHS_ERR_IO, "write failed: " + fn.str());
+            return;
+        }
         // column order of main_ym.cu:291-292
         const int cols[24] = {KHS_BY1, KHS_BY2, KHS_BY3, KHS_AZ2, KHS_AZ3, KHS_PZA, KHS_PXA, KHS_Q1A,
                               KHS_PZB, KHS_PXB, KHS_Q1B, KHS_EX1, KHS_EX2, KHS_EX3, KHS_EZ1, KHS_EZ2,

@@ -31,6 +31,8 @@ struct KParams {
     double dt, dx, dz;
     double dtdx, dtdz;        // dt/dx, dt/dz (the reference evaluates dt_step/dx per call; same value)
     double c2, eps0, alpha, warm_T;
+    double q0, q1;            // EMHD species charges (emhdgpu2d fluid: -1)
+    int nsp;                  // EMHD: 1 or 2 fluids (YM: always 2)
 };
 
 // --------------------------------------------------------------------------
@@ -209,5 +211,85 @@ __device__ __forceinline__ void currents_at(const State<T>& F, int x, int z, con
         Jz[c] = -(nQz[0][c] * vzt[0] + nQz[1][c] * vzt[1]);
     }
 }
+
+// --------------------------------------------------------------------------
+// EMHD (emhdgpu2d) state slots: fields, then fluid species at 12 and 18.
+// --------------------------------------------------------------------------
+enum EmSlot { E_EX = 0, E_EY, E_EZ, E_BX, E_BY, E_BZ, E_AX, E_AY, E_AZ };
+enum EmVar { EV_N = 0, EV_PX, EV_PY, EV_PZ, EV_COUNT };
+__host__ __device__ __forceinline__ int em_base(int sp) { return 12 + 6 * sp; }
+
+// kernel_compute_lorentz (emhdgpu2d/main_coupled.cu:112-136) at (x, z) for
+// species sp with charge q (the reference fluid has q = -1: "-1.0 * (...)").
+template <typename T>
+__device__ __forceinline__ void em_lorentz(const State<T>& S, int sp, int x, int z, const KParams& p,
+                                           T out[3]) {
+    const int nx = p.nx;
+    const int xm = xm_src(x, p), zm = zm_per(z, p.nz);
+    const size_t i = (size_t)x + (size_t)z * nx;
+    const size_t ixm = (size_t)xm + (size_t)z * nx;
+    const size_t izm = (size_t)x + (size_t)zm * nx;
+    const size_t ixmzm = (size_t)xm + (size_t)zm * nx;
+    const T Ex_node = T(0.5) * (S.a[E_EX][i] + S.a[E_EX][ixm]);
+    const T Ey_node = S.a[E_EY][i];
+    const T Ez_node = T(0.5) * (S.a[E_EZ][i] + S.a[E_EZ][izm]);
+    const T Bx_node = T(0.5) * (S.a[E_BX][i] + S.a[E_BX][izm]);
+    const T Bz_node = T(0.5) * (S.a[E_BZ][i] + S.a[E_BZ][ixm]);
+    const T By_node = T(0.25) * (S.a[E_BY][i] + S.a[E_BY][ixm] + S.a[E_BY][izm] + S.a[E_BY][ixmzm]);
+    const int b = em_base(sp);
+    const T n_safe = kmax(S.a[b + EV_N][i], T(0.05));
+    const T vx = S.a[b + EV_PX][i] / n_safe;
+    const T vy = S.a[b + EV_PY][i] / n_safe;
+    const T vz = S.a[b + EV_PZ][i] / n_safe;
+    const T q = T(sp ? p.q1 : p.q0);
+    out[0] = q * (Ex_node + (vy * Bz_node - vz * By_node));
+    out[1] = q * (Ey_node + (vz * Bx_node - vx * Bz_node));
+    out[2] = q * (Ez_node + (vx * By_node - vy * Bx_node));
+}
+
+// --------------------------------------------------------------------------
+// Model policies of the FCT sweeps: number of advected variables, their
+// state slots, which variables carry a source in each sweep, and the
+// source values (evaluated from the step-start state).
+// --------------------------------------------------------------------------
+struct YMModel {                    // ymgpu2d: n, px, pz, Q1..3 per beam
+    static constexpr int NV = F_COUNT;
+    static constexpr int IPX = F_PX, IPZ = F_PZ;
+    static constexpr int NXS = 1;   // x sweep: px only (YM_FCT.cu:37-43)
+    __host__ __device__ static constexpr int base(int sp) { return S_NA + sp * F_COUNT; }
+    __host__ __device__ static constexpr int xslot(int v) { return v == F_PX ? 0 : -1; }
+    template <typename T>
+    __device__ static void x_src(const State<T>& S, int sp, int x, int z, const KParams& p, T (&s)[NXS]) {
+        s[0] = lorentz_px(S, sp, x, z, p);
+    }
+    // z sweep: pz and Q^a (YM_FCT.cu:119-125)
+    template <typename T>
+    __device__ static void z_src(const State<T>& S, int sp, int x, int z, const KParams& p, T (&s)[NV]) {
+        s[F_N] = T(0);
+        s[F_PX] = T(0);
+        s[F_PZ] = lorentz_pz(S, sp, x, z, p);
+        T q[3];
+        precession_at(S, sp, x, z, p, q);
+        s[F_Q1] = q[0]; s[F_Q2] = q[1]; s[F_Q3] = q[2];
+    }
+};
+
+struct EMModel {                    // emhdgpu2d: n, px, py, pz; sources in both sweeps
+    static constexpr int NV = EV_COUNT;
+    static constexpr int IPX = EV_PX, IPZ = EV_PZ;
+    static constexpr int NXS = 3;
+    __host__ __device__ static constexpr int base(int sp) { return 12 + 6 * sp; }
+    __host__ __device__ static constexpr int xslot(int v) { return v == EV_N ? -1 : v - 1; }
+    template <typename T>
+    __device__ static void x_src(const State<T>& S, int sp, int x, int z, const KParams& p, T (&s)[NXS]) {
+        em_lorentz(S, sp, x, z, p, s);
+    }
+    template <typename T>
+    __device__ static void z_src(const State<T>& S, int sp, int x, int z, const KParams& p, T (&s)[NV]) {
+        T f[3];
+        em_lorentz(S, sp, x, z, p, f);
+        s[EV_N] = T(0); s[EV_PX] = f[0]; s[EV_PY] = f[1]; s[EV_PZ] = f[2];
+    }
+};
 
 }  // namespace khs

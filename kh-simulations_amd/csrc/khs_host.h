@@ -35,5 +35,7 @@ SeedProfiles read_seed_file(const std::string& path, int nx);
 void host_initial_state(const khs_config& cfg, const SeedProfiles& seed,
                         std::vector<std::vector<double>>& state);
 double noise_value(uint64_t seed, int id, uint64_t cell, double amp);
+// EMHD initial state (das_kaw equilibrium or the random-phase spectrum).
+void host_initial_state_em(const khs_config& cfg, std::vector<std::vector<double>>& state);
 
 }  // namespace khs

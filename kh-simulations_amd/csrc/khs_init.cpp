@@ -194,4 +194,87 @@ void host_initial_state(const khs_config& cfg, const SeedProfiles& seed,
     }
 }
 
+// --------------------------------------------------------------------------
+// EMHD initial state (slots: fields 0..8 = Ex Ey Ez Bx By Bz Ax Ay Az, fluid
+// f at 12 + 6 f: n px py pz).  init_das_kaw_equilibrium
+// (emhdgpu2d/main_coupled.cu:65-94, double math) followed by the one-time
+// 5-point smoothing of By and Ex (:22-35, :279) evaluated with Jacobi
+// semantics (the reference kernel updates in place and is racy).
+// --------------------------------------------------------------------------
+void host_initial_state_em(const khs_config& cfg, std::vector<std::vector<double>>& st) {
+    const int nx = cfg.nx, nz = cfg.nz;
+    const size_t N = (size_t)nx * nz;
+    st.assign(S_COUNT, std::vector<double>(N, 0.0));
+    const double dx = cfg.dx, dz = cfg.dz;
+    auto I = [&](int x, int z) { return (size_t)x + (size_t)z * nx; };
+    const int EXs = 0, BYs = 4, BXs = 3, BZs = 5, N0 = 12, PZ0 = 15, N1 = 18;
+    if (cfg.emhd_init == 0) {
+        const int k_mode = cfg.k_mode;
+        const double perturb_amp = cfg.em_perturb_amp;
+        for (int z = 0; z < nz; ++z)
+            for (int x = 0; x < nx; ++x) {
+                const size_t idx = I(x, z);
+                const double V0 = 0.1, epsilon = 10.0 * dx;
+                const double k_z = (k_mode * 2.0 * M_PI) / (nz * dz);
+                const double x_center = (nx * dx) / 2.0;
+                const double x_node = x * dx;
+                const double x_norm = (x_node - x_center) / epsilon;
+                st[N0][idx] = 1.0;
+                st[PZ0][idx] = V0 * std::tanh(x_norm);
+                const double x_ex = (x + 0.5) * dx;
+                const double x_norm_ex = (x_ex - x_center) / epsilon;
+                const double B0y_ex = -V0 * epsilon * std::log(std::cosh(x_norm_ex));
+                const double vz_ex = V0 * std::tanh(x_norm_ex);
+                st[EXs][idx] = vz_ex * B0y_ex;
+                const double x_by = (x + 0.5) * dx, z_by = (z + 0.5) * dz;
+                const double x_norm_by = (x_by - x_center) / epsilon;
+                const double B0y_face = -V0 * epsilon * std::log(std::cosh(x_norm_by));
+                st[BYs][idx] = B0y_face + perturb_amp * V0 * (1.0 / std::cosh(x_norm_by)) * std::cos(k_z * z_by);
+                if (cfg.species > 1) st[N1][idx] = 1.0;
+            }
+    } else {
+        // Extension: quiescent fluids and a random-phase spectrum in By,
+        // By = amp sum_{kx,kz=1..K} cos(2 pi (kx x/nx + kz z/nz) + phi(kx,kz)),
+        // phases from the counter-based generator keyed by spectrum_seed.
+        const int K = cfg.spectrum_kmax;
+        const double amp = cfg.spectrum_amp;
+        for (size_t i = 0; i < N; ++i) { st[N0][i] = 1.0; if (cfg.species > 1) st[N1][i] = 1.0; }
+        std::vector<double> gr((size_t)K * nx), gi((size_t)K * nx);
+        for (int kz = 1; kz <= K; ++kz)
+            for (int x = 0; x < nx; ++x) {
+                double re = 0.0, im = 0.0;
+                for (int kx = 1; kx <= K; ++kx) {
+                    double ph = 2.0 * M_PI * (noise_value(cfg.spectrum_seed, 7, (uint64_t)(kx * 4096 + kz), 0.5) + 0.5);
+                    double th = 2.0 * M_PI * (double)((long)kx * x % nx) / nx + ph;
+                    re += std::cos(th);
+                    im += std::sin(th);
+                }
+                gr[(size_t)(kz - 1) * nx + x] = re;
+                gi[(size_t)(kz - 1) * nx + x] = im;
+            }
+        for (int z = 0; z < nz; ++z)
+            for (int x = 0; x < nx; ++x) {
+                double acc = 0.0;
+                for (int kz = 1; kz <= K; ++kz) {
+                    double tz = 2.0 * M_PI * (double)((long)kz * z % nz) / nz;
+                    acc += gr[(size_t)(kz - 1) * nx + x] * std::cos(tz) - gi[(size_t)(kz - 1) * nx + x] * std::sin(tz);
+                }
+                st[BYs][I(x, z)] = amp * acc;
+            }
+    }
+    (void)BXs; (void)BZs;
+    if (cfg.smooth_init) {
+        for (int id : {BYs, EXs}) {
+            std::vector<double> old = st[id];
+            for (int z = 0; z < nz; ++z)
+                for (int x = 0; x < nx; ++x) {
+                    const int xm = std::max(0, x - 1), xp = std::min(nx - 1, x + 1);
+                    const int zm = (z == 0) ? nz - 1 : z - 1, zp = (z == nz - 1) ? 0 : z + 1;
+                    st[id][I(x, z)] = 0.5 * old[I(x, z)] +
+                        0.125 * (old[I(xp, z)] + old[I(xm, z)] + old[I(x, zp)] + old[I(x, zm)]);
+                }
+        }
+    }
+}
+
 }  // namespace khs

@@ -10,12 +10,21 @@ namespace khs {
 // (variant 0/1/2); z chunk 16/32/64 rows (variant 0/1/2).
 constexpr int kNumXVariants = 3;
 constexpr int kNumZVariants = 3;
-template <typename T>
+// M: YMModel (ymgpu2d two-beam SU(2) fluid) or EMModel (emhdgpu2d Abelian fluid)
+template <typename T, typename M>
 void launch_fct_x(const State<T>& S, const Stage<T>& out, const KParams& p, int sp0, int nsp,
                   int variant, hipStream_t st);
-template <typename T>
+template <typename T, typename M>
 void launch_fct_z(const State<T>& S, const Stage<T>& X, const State<T>& O, const KParams& p,
                   int sp0, int nsp, int variant, hipStream_t st);
+template <typename T>
+void launch_emhd_maxwell(const State<T>& S, const State<T>& O, const KParams& p, hipStream_t st);
+// EMHD energy (main_coupled.cu:197-222): out[0] = sum over cells
+template <typename T>
+void launch_energy_em(const State<T>& S, const KParams& p, double* partials, double* out, hipStream_t st);
+// EMHD derived arrays: 0..2 J, 3..6 (ux, uz) per fluid, 7..12 (sx, sy, sz) per fluid, 13 energy density
+template <typename T>
+void launch_derived_em(const State<T>& S, int which, const KParams& p, double* out, hipStream_t st);
 // fused x + z sweep of both species: state S -> fluid of O
 template <typename T>
 void launch_fct_xz(const State<T>& S, const State<T>& O, const KParams& p, hipStream_t st);

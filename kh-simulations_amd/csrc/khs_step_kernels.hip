@@ -30,44 +30,49 @@ namespace khs {
 // XB cells per row segment (= threads), XW = segment + 3-cell halo each side.
 // XB is tuned per grid (64 / 128 / 256): smaller segments use less LDS and
 // so fit more workgroups per CU, which hides the load latency of phase A.
-template <typename T, int XB>
+// M is the model policy (YMModel / EMModel): variables, slots, sources.
+template <typename T, int XB, typename M>
 __global__ __launch_bounds__(XB) void k_fct_x(State<T> S, Stage<T> out, KParams p, int sp0, int nsp,
                                               int ntx) {
     constexpr int XW = XB + 6;
-    __shared__ T sU[XW], sS[XW];
+    constexpr int NV = M::NV;
+    __shared__ T sU[XW];
+    __shared__ T sS[M::NXS][XW];
     __shared__ T sE[XW], sNU[XW], sMU[XW];
-    __shared__ T sV[F_COUNT][XW];      // variable, later the limited interface flux
-    __shared__ T sLOR[F_COUNT][XW];
-    __shared__ T sLNR[F_COUNT][XW];
+    __shared__ T sV[NV][XW];      // variable, later the limited interface flux
+    __shared__ T sLOR[NV][XW];
+    __shared__ T sLNR[NV][XW];
 
     const int tid = threadIdx.x;
     // logical tile: species fastest, then x segments, then rows (row z-1,
-    // whose By the px source reads, was staged ntx*nsp tiles earlier)
+    // whose fields the x source reads, was staged ntx*nsp tiles earlier)
     int lt = xcd_logical(blockIdx.x, gridDim.x);
     if (lt >= nsp * ntx * p.nz) return;
     const int sp = sp0 + lt % nsp;
     lt /= nsp;
     const int x0 = (lt % ntx) * XB;
     const int z = lt / ntx;
-    const int b = S_NA + sp * F_COUNT;
+    const int b = M::base(sp);
     const size_t row = (size_t)z * p.nx;
     const T dt = T(p.dt);
     const T dtdx = T(p.dtdx);
 
-    // Phase A: stage variables, velocity and px source for positions
+    // Phase A: stage variables, velocity and x sources for positions
     // x0-3 .. x0+XB+2 (position q <-> extended x = x0-3+q).
     for (int q = tid; q < XW; q += XB) {
         const int xr = map_x(x0 - 3 + q, p);
         const size_t i = row + xr;
-        T n = S.a[b + F_N][i];
-        T px = S.a[b + F_PX][i];
-        T n_safe = kmax(n, T(0.05));
-        sU[q] = px / n_safe;                               // kernel_update_velocities
-        sS[q] = lorentz_px(S, sp, xr, z, p);
-        sV[F_N][q] = n;
-        sV[F_PX][q] = px;
+        T v[NV];
 #pragma unroll
-        for (int v = F_PZ; v < F_COUNT; ++v) sV[v][q] = S.a[b + v][i];
+        for (int k = 0; k < NV; ++k) v[k] = S.a[b + k][i];
+        T n_safe = kmax(v[0], T(0.05));
+        sU[q] = v[M::IPX] / n_safe;                         // update_velocities
+        T src[M::NXS];
+        M::x_src(S, sp, xr, z, p, src);
+#pragma unroll
+        for (int k = 0; k < M::NXS; ++k) sS[k][q] = src[k];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) sV[k][q] = v[k];
     }
     __syncthreads();
 
@@ -87,13 +92,14 @@ __global__ __launch_bounds__(XB) void k_fct_x(State<T> S, Stage<T> out, KParams 
         const T eL = sE[q - 1], eR = sE[q];
         const T nL = sNU[q - 1], nR = sNU[q];
 #pragma unroll
-        for (int v = 0; v < F_COUNT; ++v) {
+        for (int v = 0; v < NV; ++v) {
             const T vm = sV[v][q - 1], v0 = sV[v][q], vp = sV[v][q + 1];
             T fL = T(0.5) * eL * (vm + v0);
             T fR = T(0.5) * eR * (v0 + vp);
             T dL = nL * (v0 - vm);
             T dR = nR * (vp - v0);
-            T src = (v == F_PX) ? sS[q] : T(0.0);
+            const int sl = M::xslot(v);
+            T src = (sl >= 0) ? sS[sl < 0 ? 0 : sl][q] : T(0.0);
             T lor = v0 + dt * src - (fR - fL);
             sLOR[v][q] = lor;
             sLNR[v][q] = lor + (dR - dL);
@@ -106,7 +112,7 @@ __global__ __launch_bounds__(XB) void k_fct_x(State<T> S, Stage<T> out, KParams 
     for (int q = 2 + tid; q < XW - 3; q += XB) {
         const T mu = sMU[q];
 #pragma unroll
-        for (int v = 0; v < F_COUNT; ++v) {
+        for (int v = 0; v < NV; ++v) {
             T flx = mu * (sLOR[v][q + 1] - sLOR[v][q]);
             T l0 = sLNR[v][q - 1], l1 = sLNR[v][q], l2 = sLNR[v][q + 1], l3 = sLNR[v][q + 2];
             T fl = fct_limit(flx, l1 - l0, l2 - l1, l3 - l2);
@@ -120,9 +126,9 @@ __global__ __launch_bounds__(XB) void k_fct_x(State<T> S, Stage<T> out, KParams 
     if (x < p.nx) {
         const int q = 3 + tid;
 #pragma unroll
-        for (int v = 0; v < F_COUNT; ++v) {
+        for (int v = 0; v < NV; ++v) {
             T r = sLNR[v][q] - (sV[v][q] - sV[v][q - 1]);
-            out.a[sp * F_COUNT + v][row + x] = r;
+            out.a[sp * NV + v][row + x] = r;
         }
     }
 }
@@ -137,18 +143,18 @@ __global__ __launch_bounds__(XB) void k_fct_x(State<T> S, Stage<T> out, KParams 
 // transport/diffusion fluxes, LORHOT/LNRHOT of row e-1, the antidiffusive flux
 // at e-3/2 and the limited flux at e-5/2, in the reference's order
 // (FCT_Sweeps.cu:88-162).
-template <typename T>
+template <typename T, int NV = F_COUNT>
 struct ZPipe {
-    T Vm[F_COUNT], Sm[F_COUNT];      // variable and source at row e-1
-    T Fm[F_COUNT], Dm[F_COUNT];      // transport / diffusion flux at e-3/2
-    T LORm[F_COUNT];                 // LORHOT(e-2)
-    T L3[F_COUNT], L2[F_COUNT], L1[F_COUNT];  // LNRHOT(e-4), (e-3), (e-2)
-    T FLXm[F_COUNT];                 // antidiffusive flux at e-5/2
-    T FLm[F_COUNT];                  // limited flux at e-7/2
+    T Vm[NV], Sm[NV];                // variable and source at row e-1
+    T Fm[NV], Dm[NV];                // transport / diffusion flux at e-3/2
+    T LORm[NV];                      // LORHOT(e-2)
+    T L3[NV], L2[NV], L1[NV];        // LNRHOT(e-4), (e-3), (e-2)
+    T FLXm[NV];                      // antidiffusive flux at e-5/2
+    T FLm[NV];                       // limited flux at e-7/2
     T Um, MUm;                       // velocity at e-1, mu at interface e-3/2
 
-    __device__ __forceinline__ bool push(int e, int z0, const T (&V)[F_COUNT], T u,
-                                         const T (&Sv)[F_COUNT], T dt, T dtdz, T (&out)[F_COUNT]) {
+    __device__ __forceinline__ bool push(int e, int z0, const T (&V)[NV], T u,
+                                         const T (&Sv)[NV], T dt, T dtdz, T (&out)[NV]) {
         bool have = false;
         if (e > z0 - 3) {
             T vi = T(0.5) * (Um + u);
@@ -160,7 +166,7 @@ struct ZPipe {
             const bool do_fl = (e >= z0 + 2);
             have = (e >= z0 + 3);
 #pragma unroll
-            for (int v = 0; v < F_COUNT; ++v) {
+            for (int v = 0; v < NV; ++v) {
                 T F = T(0.5) * eps * (Vm[v] + V[v]);
                 T D = nu * (V[v] - Vm[v]);
                 if (do_lor) {
@@ -182,7 +188,7 @@ struct ZPipe {
             MUm = mu;
         }
 #pragma unroll
-        for (int v = 0; v < F_COUNT; ++v) { Vm[v] = V[v]; Sm[v] = Sv[v]; }
+        for (int v = 0; v < NV; ++v) { Vm[v] = V[v]; Sm[v] = Sv[v]; }
         Um = u;
         return have;
     }
@@ -208,20 +214,21 @@ __device__ __forceinline__ void z_inputs(const State<T>& S, int sp, int x, int r
 
 // Swept row written to the next state with the post-sweep boundary condition
 // of kernel_enforce_boundaries: density floor and x-wall px (YM_Fluid.cu:13-30).
-template <typename T>
-__device__ __forceinline__ void store_fluid(const State<T>& O, int b, size_t io, bool wall, T (&r)[F_COUNT]) {
-    if (r[F_N] < T(0.05)) r[F_N] = T(0.05);
-    if (wall) r[F_PX] = T(0.0);
+template <typename T, int NV, int IPX>
+__device__ __forceinline__ void store_fluid(const State<T>& O, int b, size_t io, bool wall, T (&r)[NV]) {
+    if (r[0] < T(0.05)) r[0] = T(0.05);
+    if (wall) r[IPX] = T(0.0);
 #pragma unroll
-    for (int v = 0; v < F_COUNT; ++v) O.a[b + v][io] = r[v];
+    for (int v = 0; v < NV; ++v) O.a[b + v][io] = r[v];
 }
 
 constexpr int ZB_X = 64;           // columns per workgroup (one wave wide)
 constexpr int ZB_C = 4;            // z chunks per workgroup (one per wave)
 
-template <typename T, int TZ>
+template <typename T, int TZ, typename M>
 __global__ __launch_bounds__(ZB_X * ZB_C) void k_fct_z(State<T> S, Stage<T> X, State<T> O,
                                                        KParams p, int sp0, int nsp, int ntx, int ncg) {
+    constexpr int NV = M::NV;
     // logical tile: species fastest (both species read the same fields for
     // their sources), then 64-column groups, then groups of ZB_C z chunks
     int lt = xcd_logical(blockIdx.x, gridDim.x);
@@ -233,22 +240,25 @@ __global__ __launch_bounds__(ZB_X * ZB_C) void k_fct_z(State<T> S, Stage<T> X, S
     const int z0 = chunk * TZ;
     if (x >= p.nx || z0 >= p.nz) return;
     const int nx = p.nx, nz = p.nz;
-    const int b = S_NA + sp * F_COUNT;
+    const int b = M::base(sp);
     const T dt = T(p.dt), dtdz = T(p.dtdz);
     const int zend = min(z0 + TZ, nz);
     const bool wall = !p.periodic_x && (x == 0 || x == nx - 1);
-    ZPipe<T> zp;
+    ZPipe<T, NV> zp;
     zp.Um = T(0);
     zp.MUm = T(0);
     for (int e = z0 - 3; e <= zend + 2; ++e) {
         const int rc = clampi(e, 0, nz - 1);
         const size_t i = (size_t)x + (size_t)rc * nx;
-        T V[F_COUNT], Sv[F_COUNT], u, out[F_COUNT];
+        T V[NV], Sv[NV], out[NV];
 #pragma unroll
-        for (int v = 0; v < F_COUNT; ++v) V[v] = X.a[sp * F_COUNT + v][i];
-        z_inputs(S, sp, x, rc, p, u, Sv);
+        for (int v = 0; v < NV; ++v) V[v] = X.a[sp * NV + v][i];
+        // stale z velocity (set before the x sweep) and the step-start sources
+        const T n_safe = kmax(S.a[b][i], T(0.05));
+        const T u = S.a[b + M::IPZ][i] / n_safe;
+        M::z_src(S, sp, x, rc, p, Sv);
         if (zp.push(e, z0, V, u, Sv, dt, dtdz, out))
-            store_fluid(O, b, (size_t)x + (size_t)(e - 3) * nx, wall, out);
+            store_fluid<T, NV, M::IPX>(O, b, (size_t)x + (size_t)(e - 3) * nx, wall, out);
     }
 }
 
@@ -356,7 +366,7 @@ __global__ __launch_bounds__(XZ_W) void k_fct_xz(State<T> S, State<T> O, KParams
         T Sv[F_COUNT], u, out[F_COUNT];
         z_inputs(S, sp, x, rc, p, u, Sv);
         if (zp.push(e, z0, V, u, Sv, dt, dtdz, out))
-            store_fluid(O, b, (size_t)x + (size_t)(e - 3) * nx, wall, out);
+            store_fluid<T, F_COUNT, F_PX>(O, b, (size_t)x + (size_t)(e - 3) * nx, wall, out);
     }
 }
 
@@ -472,46 +482,171 @@ __global__ __launch_bounds__(MX * MZT) void k_maxwell(State<T> S, State<T> O, KP
 }
 
 // ==========================================================================
+// EMHD Maxwell: enforce_boundaries (Ey, Ez, Bx walls) + currents + Ampere +
+// vector potential + Faraday (emhdgpu2d/main_coupled.cu:48-63, 99-110;
+// EMHD_Physics.cu:21-98).  Same tiling as k_maxwell; the updated E of the
+// tile plus one halo column on each side and one halo row is kept in LDS
+// (the walled Faraday at x = nx-1 uses E(x-1)).
+// ==========================================================================
+template <typename T>
+__global__ __launch_bounds__(MX * MZT) void k_emhd_maxwell(State<T> S, State<T> O, KParams p, int ntx, int ntz) {
+    __shared__ T sE[3][MZ + 1][MX + 2];     // column index ix + 1
+    const int nx = p.nx, nz = p.nz;
+    const int lt = xcd_logical(blockIdx.x, gridDim.x);
+    if (lt >= ntx * ntz) return;
+    const int x0 = (lt % ntx) * MX, z0 = (lt / ntx) * MZ;
+    const int ncol = min(MX, nx - x0), nrow = min(MZ, nz - z0);
+    const T dt = T(p.dt), c2 = T(p.c2), eps0 = T(p.eps0);
+    const T dx = T(p.dx), dz = T(p.dz);
+    const bool per = p.periodic_x != 0;
+    const int tx = threadIdx.x % MX, tz = threadIdx.x / MX;
+    auto wall = [&](int x) { return !per && (x == 0 || x == nx - 1); };
+
+    auto ampere = [&](int ix, int iz) {
+        int xr;
+        if (ix < 0) xr = x0 - 1;
+        else if (ix < ncol) xr = x0 + ix;
+        else xr = xp_src(x0 + ncol - 1, p);
+        const int zr = (iz < nrow) ? z0 + iz : zp_per(z0 + nrow - 1, nz);
+        const size_t i = (size_t)xr + (size_t)zr * nx;
+        const int zm = zm_per(zr, nz);
+        const size_t izm = (size_t)xr + (size_t)zm * nx;
+        // currents (main_coupled.cu:99-110), summed over the fluids
+        const int xp = xp_src(xr, p), zp = zp_per(zr, nz);
+        const size_t ixp = (size_t)xp + (size_t)zr * nx, izp = (size_t)xr + (size_t)zp * nx;
+        T jx = T(0), jy = T(0), jz = T(0);
+        for (int s = 0; s < p.nsp; ++s) {
+            const int b = em_base(s);
+            const T q = T(s ? p.q1 : p.q0);
+            const T h = T((s ? p.q1 : p.q0) * 0.5);
+            const T a = h * (O.a[b + EV_PX][i] + O.a[b + EV_PX][ixp]);
+            const T bb = q * O.a[b + EV_PY][i];
+            const T c = h * (O.a[b + EV_PZ][i] + O.a[b + EV_PZ][izp]);
+            if (s == 0) { jx = a; jy = bb; jz = c; }
+            else { jx = jx + a; jy = jy + bb; jz = jz + c; }
+        }
+        const bool w = wall(xr);
+        const T Bx0 = w ? T(0) : S.a[E_BX][i], Bxzm = w ? T(0) : S.a[E_BX][izm];
+        T dBy_dz = (S.a[E_BY][i] - S.a[E_BY][izm]) / dz;
+        T curlB_x = -dBy_dz;
+        T dBx_dz = (Bx0 - Bxzm) / dz;
+        T dBz_dx, dBy_dx;
+        if (per) {
+            const size_t ixm = (size_t)((xr + nx - 1) % nx) + (size_t)zr * nx;
+            dBz_dx = (S.a[E_BZ][i] - S.a[E_BZ][ixm]) / dx;
+            dBy_dx = (S.a[E_BY][i] - S.a[E_BY][ixm]) / dx;
+        } else if (xr == 0) {
+            const size_t i1 = (size_t)1 + (size_t)zr * nx;
+            dBz_dx = (S.a[E_BZ][i1] - S.a[E_BZ][i]) / dx;
+            dBy_dx = (S.a[E_BY][i1] - S.a[E_BY][i]) / dx;
+        } else {
+            const size_t ixm = (size_t)(xr - 1) + (size_t)zr * nx;
+            dBz_dx = (S.a[E_BZ][i] - S.a[E_BZ][ixm]) / dx;
+            dBy_dx = (S.a[E_BY][i] - S.a[E_BY][ixm]) / dx;
+        }
+        T curlB_y = dBx_dz - dBz_dx;
+        T curlB_z = dBy_dx;
+        const T ex = S.a[E_EX][i];
+        const T ey = w ? T(0) : S.a[E_EY][i];
+        const T ez = w ? T(0) : S.a[E_EZ][i];
+        sE[0][iz][ix + 1] = ex + dt * (c2 * curlB_x - jx / eps0);
+        sE[1][iz][ix + 1] = ey + dt * (c2 * curlB_y - jy / eps0);
+        sE[2][iz][ix + 1] = ez + dt * (c2 * curlB_z - jz / eps0);
+    };
+
+    for (int k = 0; k < MZ / MZT; ++k) {
+        int iz = tz + MZT * k;
+        if (tx < ncol && iz < nrow) ampere(tx, iz);
+    }
+    {
+        int h = threadIdx.x;
+        if (h < nrow + 1) ampere(ncol, h);                         // right halo column (+ corner)
+        else if (h - (nrow + 1) < ncol) ampere(h - (nrow + 1), nrow);   // halo row
+        else if (x0 > 0 && h - (nrow + 1) - ncol < nrow) ampere(-1, h - (nrow + 1) - ncol);  // left halo column
+    }
+    __syncthreads();
+
+    for (int k = 0; k < MZ / MZT; ++k) {
+        const int iz = tz + MZT * k, ix = tx;
+        if (ix >= ncol || iz >= nrow) continue;
+        const int xr = x0 + ix, zr = z0 + iz;
+        const size_t i = (size_t)xr + (size_t)zr * nx;
+        const T Ex = sE[0][iz][ix + 1], Ey = sE[1][iz][ix + 1], Ez = sE[2][iz][ix + 1];
+        T dEy_dz = (sE[1][iz + 1][ix + 1] - Ey) / dz;
+        T curlE_x = -dEy_dz;
+        T dEx_dz = (sE[0][iz + 1][ix + 1] - Ex) / dz;
+        T dEz_dx, dEy_dx;
+        if (!per && xr == nx - 1) {
+            dEz_dx = (Ez - sE[2][iz][ix]) / dx;
+            dEy_dx = (Ey - sE[1][iz][ix]) / dx;
+        } else {
+            dEz_dx = (sE[2][iz][ix + 2] - Ez) / dx;
+            dEy_dx = (sE[1][iz][ix + 2] - Ey) / dx;
+        }
+        T curlE_y = dEx_dz - dEz_dx;
+        T curlE_z = dEy_dx;
+        const bool w = wall(xr);
+        const T bx = w ? T(0) : S.a[E_BX][i];
+        O.a[E_BX][i] = bx - dt * curlE_x;
+        O.a[E_BY][i] = S.a[E_BY][i] - dt * curlE_y;
+        O.a[E_BZ][i] = S.a[E_BZ][i] - dt * curlE_z;
+        O.a[E_EX][i] = Ex;
+        O.a[E_EY][i] = Ey;
+        O.a[E_EZ][i] = Ez;
+        O.a[E_AX][i] = S.a[E_AX][i] - dt * Ex;
+        O.a[E_AY][i] = S.a[E_AY][i] - dt * Ey;
+        O.a[E_AZ][i] = S.a[E_AZ][i] - dt * Ez;
+    }
+}
+
+// ==========================================================================
 // launchers
 // ==========================================================================
-template <typename T, int XB>
+template <typename T, int XB, typename M>
 static void launch_fct_x_t(const State<T>& S, const Stage<T>& out, const KParams& p, int sp0, int nsp,
                            hipStream_t st) {
     const int ntx = (p.nx + XB - 1) / XB;
     dim3 grid(round8(ntx * p.nz * nsp));
-    hipLaunchKernelGGL((k_fct_x<T, XB>), grid, dim3(XB), 0, st, S, out, p, sp0, nsp, ntx);
+    hipLaunchKernelGGL((k_fct_x<T, XB, M>), grid, dim3(XB), 0, st, S, out, p, sp0, nsp, ntx);
 }
 
-template <typename T>
+template <typename T, typename M>
 void launch_fct_x(const State<T>& S, const Stage<T>& out, const KParams& p, int sp0, int nsp,
                   int variant, hipStream_t st) {
     switch (variant) {
-        case 0: launch_fct_x_t<T, 64>(S, out, p, sp0, nsp, st); break;
-        case 1: launch_fct_x_t<T, 128>(S, out, p, sp0, nsp, st); break;
-        default: launch_fct_x_t<T, 256>(S, out, p, sp0, nsp, st); break;
+        case 0: launch_fct_x_t<T, 64, M>(S, out, p, sp0, nsp, st); break;
+        case 1: launch_fct_x_t<T, 128, M>(S, out, p, sp0, nsp, st); break;
+        default: launch_fct_x_t<T, 256, M>(S, out, p, sp0, nsp, st); break;
     }
 }
 
-template <typename T, int TZ>
+template <typename T, int TZ, typename M>
 static void launch_fct_z_t(const State<T>& S, const Stage<T>& X, const State<T>& O, const KParams& p,
                            int sp0, int nsp, hipStream_t st) {
     const int nchunk = (p.nz + TZ - 1) / TZ;
     const int ncg = (nchunk + ZB_C - 1) / ZB_C;
     const int ntx = (p.nx + ZB_X - 1) / ZB_X;
     dim3 grid(round8(nsp * ntx * ncg));
-    hipLaunchKernelGGL((k_fct_z<T, TZ>), grid, dim3(ZB_X * ZB_C), 0, st, S, X, O, p, sp0, nsp, ntx, ncg);
+    hipLaunchKernelGGL((k_fct_z<T, TZ, M>), grid, dim3(ZB_X * ZB_C), 0, st, S, X, O, p, sp0, nsp, ntx, ncg);
 }
 
 // z chunk height: 16 / 32 / 64 rows per wave (halo rows re-read per chunk
 // vs. number of independent column marches in flight)
-template <typename T>
+template <typename T, typename M>
 void launch_fct_z(const State<T>& S, const Stage<T>& X, const State<T>& O, const KParams& p,
                   int sp0, int nsp, int variant, hipStream_t st) {
     switch (variant) {
-        case 0: launch_fct_z_t<T, 16>(S, X, O, p, sp0, nsp, st); break;
-        case 2: launch_fct_z_t<T, 64>(S, X, O, p, sp0, nsp, st); break;
-        default: launch_fct_z_t<T, 32>(S, X, O, p, sp0, nsp, st); break;
+        case 0: launch_fct_z_t<T, 16, M>(S, X, O, p, sp0, nsp, st); break;
+        case 2: launch_fct_z_t<T, 64, M>(S, X, O, p, sp0, nsp, st); break;
+        default: launch_fct_z_t<T, 32, M>(S, X, O, p, sp0, nsp, st); break;
     }
+}
+
+template <typename T>
+void launch_emhd_maxwell(const State<T>& S, const State<T>& O, const KParams& p, hipStream_t st) {
+    const int ntx = (p.nx + MX - 1) / MX, ntz = (p.nz + MZ - 1) / MZ;
+    dim3 grid(round8(ntx * ntz));
+    hipLaunchKernelGGL(k_emhd_maxwell<T>, grid, dim3(MX * MZT), 0, st, S, O, p, ntx, ntz);
 }
 
 template <typename T>
@@ -531,10 +666,16 @@ void launch_maxwell(const State<T>& S, const State<T>& O, const KParams& p, cons
     hipLaunchKernelGGL(k_maxwell<T>, grid, dim3(MX * MZT), 0, st, S, O, p, ntx, ntz, decay, action);
 }
 
-template void launch_fct_x<double>(const State<double>&, const Stage<double>&, const KParams&, int, int, int, hipStream_t);
-template void launch_fct_x<float>(const State<float>&, const Stage<float>&, const KParams&, int, int, int, hipStream_t);
-template void launch_fct_z<double>(const State<double>&, const Stage<double>&, const State<double>&, const KParams&, int, int, int, hipStream_t);
-template void launch_fct_z<float>(const State<float>&, const Stage<float>&, const State<float>&, const KParams&, int, int, int, hipStream_t);
+#define KHS_SWEEP_INST(T, M)                                                                             \
+    template void launch_fct_x<T, M>(const State<T>&, const Stage<T>&, const KParams&, int, int, int, hipStream_t); \
+    template void launch_fct_z<T, M>(const State<T>&, const Stage<T>&, const State<T>&, const KParams&, int, int, int, hipStream_t);
+KHS_SWEEP_INST(double, YMModel)
+KHS_SWEEP_INST(float, YMModel)
+KHS_SWEEP_INST(double, EMModel)
+KHS_SWEEP_INST(float, EMModel)
+#undef KHS_SWEEP_INST
+template void launch_emhd_maxwell<double>(const State<double>&, const State<double>&, const KParams&, hipStream_t);
+template void launch_emhd_maxwell<float>(const State<float>&, const State<float>&, const KParams&, hipStream_t);
 template void launch_fct_xz<double>(const State<double>&, const State<double>&, const KParams&, hipStream_t);
 template void launch_fct_xz<float>(const State<float>&, const State<float>&, const KParams&, hipStream_t);
 template void launch_maxwell<double>(const State<double>&, const State<double>&, const KParams&, const double*, const int*, hipStream_t);

@@ -37,6 +37,15 @@ DERIVED_NAMES = ["Jx1", "Jx2", "Jx3", "Jz1", "Jz2", "Jz3", "uxA", "uzA", "uxB", 
 ARRAY_IDS = {n: i for i, n in enumerate(STATE_NAMES)}
 ARRAY_IDS.update({n: 32 + i for i, n in enumerate(DERIVED_NAMES)})
 ARRAY_IDS.update({"pxA": 13, "pzA": 14, "pxB": 19, "pzB": 20})
+# EMHD model (model="emhd")
+EM_STATE_NAMES = ["Ex", "Ey", "Ez", "Bx", "By", "Bz", "Ax", "Ay", "Az", "n", "px", "py", "pz",
+                  "n2", "px2", "py2", "pz2"]
+EM_ARRAY_IDS = {n: 64 + i for i, n in enumerate(EM_STATE_NAMES[:9])}
+EM_ARRAY_IDS.update({n: 76 + i for i, n in enumerate(["n", "px", "py", "pz"])})
+EM_ARRAY_IDS.update({n: 82 + i for i, n in enumerate(["n2", "px2", "py2", "pz2"])})
+EM_ARRAY_IDS.update({n: 96 + i for i, n in enumerate(
+    ["Jx", "Jy", "Jz", "ux", "uz", "ux2", "uz2", "sx", "sy", "sz", "sx2", "sy2", "sz2", "energy_density"])})
+MODEL_YM, MODEL_EMHD = 0, 1
 
 
 class KhsError(RuntimeError):
@@ -62,6 +71,10 @@ class khs_config(ctypes.Structure):
         ("energy_stride_override", ctypes.c_int), ("export_stride_override", ctypes.c_int),
         ("fused_sweep", ctypes.c_int), ("autotune", ctypes.c_int),
         ("x_segment", ctypes.c_int), ("z_chunk", ctypes.c_int),
+        ("model", ctypes.c_int), ("species", ctypes.c_int), ("periodic", ctypes.c_int),
+        ("charge1", ctypes.c_double), ("charge2", ctypes.c_double), ("emhd_init", ctypes.c_int),
+        ("smooth_init", ctypes.c_int), ("spectrum_kmax", ctypes.c_int), ("spectrum_amp", ctypes.c_double),
+        ("spectrum_seed", ctypes.c_uint64), ("em_perturb_amp", ctypes.c_double),
         ("nx", ctypes.c_int), ("nz", ctypes.c_int), ("periodic_x", ctypes.c_int), ("freeze_az1", ctypes.c_int),
         ("lx", ctypes.c_double), ("lz", ctypes.c_double), ("dx", ctypes.c_double), ("dz", ctypes.c_double),
         ("courant", ctypes.c_double), ("dt", ctypes.c_double),
@@ -194,6 +207,7 @@ class Simulation:
         _check(_lib.khs_create(ctypes.byref(self.config.struct), ctypes.byref(h)))
         self._h = h
         self.nx, self.nz = self.config.nx, self.config.nz
+        self.model = self.config.model
 
     def close(self):
         if getattr(self, "_h", None):
@@ -216,13 +230,18 @@ class Simulation:
     def init(self, seed_path=None):
         _check(_lib.khs_init(self._h, os.fsencode(seed_path) if seed_path else None))
 
-    @staticmethod
-    def array_id(name):
+    def array_id(self, name):
         if isinstance(name, int):
             return name
-        if name not in ARRAY_IDS:
+        ids = EM_ARRAY_IDS if self.model == MODEL_EMHD else ARRAY_IDS
+        if name not in ids:
             raise KeyError(name)
-        return ARRAY_IDS[name]
+        return ids[name]
+
+    def state_names(self):
+        if self.model == MODEL_EMHD:
+            return EM_STATE_NAMES[: 13 if self.config.species == 1 else 17]
+        return STATE_NAMES
 
     def get(self, name):
         out = np.empty((self.nz, self.nx), dtype=np.float64)
@@ -243,7 +262,7 @@ class Simulation:
                                   a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), a.size))
 
     def state(self):
-        return {n: self.get(n) for n in STATE_NAMES}
+        return {n: self.get(n) for n in self.state_names()}
 
     # ---- operators ----
     def fct_x_sweep(self, species):

@@ -212,3 +212,106 @@ def kz_band_column(col, lo, hi):
 
 def noise(seed, aid, cell, amp):
     return load().orc_noise(seed, aid, cell, amp)
+
+
+# --------------------------------------------------------------------------
+# EMHD oracle (oracle/khs_oracle_emhd.cpp)
+# --------------------------------------------------------------------------
+EM_IDS = {n: i for i, n in enumerate(["Ex", "Ey", "Ez", "Bx", "By", "Bz", "Ax", "Ay", "Az", "Jx", "Jy", "Jz"])}
+for _s, _suf in ((0, ""), (1, "2")):
+    for _k, _n in enumerate(["n", "px", "py", "pz", "ux", "uz", "sx", "sy", "sz"]):
+        EM_IDS[_n + _suf] = 16 + 12 * _s + _k
+EM_IDS["em_energy_density"] = 40
+EM_FIELDS = ["Ex", "Ey", "Ez", "Bx", "By", "Bz", "Ax", "Ay", "Az"]
+
+
+def em_state_names(nsp):
+    f = ["n", "px", "py", "pz"]
+    return EM_FIELDS + f + ([n + "2" for n in f] if nsp > 1 else [])
+
+
+class EmParams(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("nz", ctypes.c_int), ("periodic_x", ctypes.c_int), ("nsp", ctypes.c_int),
+                ("dx", ctypes.c_double), ("dz", ctypes.c_double), ("dt", ctypes.c_double), ("c", ctypes.c_double),
+                ("eps0", ctypes.c_double), ("q0", ctypes.c_double), ("q1", ctypes.c_double)]
+
+
+def em_params_from_config(cfg):
+    return EmParams(cfg.nx, cfg.nz, cfg.periodic_x, cfg.species, cfg.dx, cfg.dz, cfg.dt, cfg.c, cfg.eps_0,
+                    cfg.charge1, cfg.charge2)
+
+
+def _load_em():
+    lib = load()
+    if getattr(lib, "_em_ready", False):
+        return lib
+    v, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    lib.emo_create.restype = v
+    lib.emo_create.argtypes = [ctypes.POINTER(EmParams)]
+    lib.emo_destroy.argtypes = [v]
+    lib.emo_array.restype = ctypes.POINTER(ctypes.c_double)
+    lib.emo_array.argtypes = [v, i]
+    for n in ["emo_update_velocities", "emo_enforce_boundaries", "emo_currents", "emo_ampere",
+              "emo_potential", "emo_faraday", "emo_lorentz"]:
+        getattr(lib, n).argtypes = [v]
+    for n in ["emo_x_sweep", "emo_z_sweep", "emo_step"]:
+        getattr(lib, n).argtypes = [v, i]
+    lib.emo_energy.restype = d
+    lib.emo_energy.argtypes = [v]
+    lib.emo_init_daskaw.argtypes = [v, i, d, i]
+    lib._em_ready = True
+    return lib
+
+
+class EmOracle:
+    """The emhdgpu2d step on the CPU.  After set_state / init call lorentz()
+    once (main_coupled.cu:282) before stepping, as the reference does."""
+
+    def __init__(self, p):
+        self.lib = _load_em()
+        self.p = p
+        self.h = self.lib.emo_create(ctypes.byref(p))
+        self.nx, self.nz, self.nsp = p.nx, p.nz, p.nsp
+
+    def close(self):
+        if self.h:
+            self.lib.emo_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def arr(self, name):
+        i = EM_IDS[name] if isinstance(name, str) else name
+        return np.ctypeslib.as_array(self.lib.emo_array(self.h, i), shape=(self.nz, self.nx))
+
+    def get(self, name):
+        return self.arr(name).copy()
+
+    def set(self, name, a):
+        self.arr(name)[...] = np.asarray(a, dtype=np.float64).reshape(self.nz, self.nx)
+
+    def set_state(self, st):
+        for n, a in st.items():
+            self.set(n, a)
+
+    def state(self):
+        return {n: self.get(n) for n in em_state_names(self.nsp)}
+
+    def init_daskaw(self, k_mode, perturb_amp, smooth=1):
+        self.lib.emo_init_daskaw(self.h, k_mode, perturb_amp, smooth)
+
+    def update_velocities(self): self.lib.emo_update_velocities(self.h)
+    def x_sweep(self, s): self.lib.emo_x_sweep(self.h, s)
+    def z_sweep(self, s): self.lib.emo_z_sweep(self.h, s)
+    def enforce_boundaries(self): self.lib.emo_enforce_boundaries(self.h)
+    def currents(self): self.lib.emo_currents(self.h)
+    def ampere(self): self.lib.emo_ampere(self.h)
+    def potential(self): self.lib.emo_potential(self.h)
+    def faraday(self): self.lib.emo_faraday(self.h)
+    def lorentz(self): self.lib.emo_lorentz(self.h)
+    def step(self, n=1): self.lib.emo_step(self.h, n)
+    def energy(self): return self.lib.emo_energy(self.h)

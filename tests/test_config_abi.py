@@ -139,3 +139,32 @@ def test_noise_is_uniform_and_keyed():
     assert abs(v.mean()) < 2e-11 and abs(v.std() - 1e-9 / np.sqrt(3)) < 2e-11
     w = np.array([orc.noise(8, 10, c, 1e-9) for c in range(100)])
     assert not np.allclose(w, v[:100], rtol=1e-3, atol=0)
+
+
+def test_emhd_config_matches_reference_run():
+    # emhdgpu2d/main_coupled.cu:228-262: 256x256, dx = dz = 1, dt = 0.002,
+    # 500000 steps, energy every 500, export every 1000, halt at E/E0 > 5
+    c = khs.Config(model="emhd")
+    assert (c.nx, c.nz, c.dx, c.dz, c.dt) == (256, 256, 1.0, 1.0, 0.002)
+    assert (c.total_steps, c.energy_stride, c.export_stride, c.energy_thresh) == (500000, 500, 1000, 5.0)
+    assert (c.periodic_x, c.species, c.charge1, c.em_perturb_amp) == (0, 1, -1.0, 0.05)
+    assert c.eps == 10.0 and c.v0 == 0.1 and c.c == 1.0 and c.eps_0 == 1.0
+    assert c.output_dir() == "coupled_k1"
+    assert khs.Config(model="emhd", k_mode=4, perturb_amp=0).output_dir() == "coupled_k4_noperturb"
+    # perturb_amp is a double for EMHD (atof), a float for YM (stof)
+    c = khs.Config(model="emhd", perturb_amp="0.1")
+    assert c.em_perturb_amp == 0.1 and c.perturb_amp == float(np.float32(0.1))
+    for bad in (dict(species=3), dict(emhd_init="spectrum", spectrum_kmax=200)):
+        with pytest.raises(khs.KhsError):
+            khs.Config(model="emhd", **bad)
+
+
+def test_emhd_array_ids():
+    lib = khs.load_library()
+    want = {"Ex": 64, "Az": 72, "n": 76, "pz": 79, "n2": 82, "pz2": 85, "Jx": 96, "uz2": 102, "sz2": 108,
+            "em_energy_density": 109, "Ex1": 0, "energy_density": 52}
+    for name, i in want.items():
+        assert lib.khs_array_id(name.encode()) == i, name
+    for name, i in khs.EM_ARRAY_IDS.items():
+        if name != "energy_density":
+            assert lib.khs_array_id(name.encode()) == i, name
