@@ -45,6 +45,13 @@ SEGMENT = 700
 KERNEL_ARRAYS = {"k_fct_x": 18 + 12, "k_fct_z": 33 + 12, "k_fct_xz": 24 + 12, "k_maxwell": 24 + 11}
 # The whole step's algorithmic minimum: R 24 + W 23 arrays (frozen Az1 read only).
 STEP_ARRAYS = 47
+# HBM bytes per launch of the FP64 step kernels on this workload, measured with
+# rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (separate runs), each scaled
+# by what the counter reads for a plain 256 MiB device copy (fetch x2.0,
+# write x1.0); summary under profiles/.  Keyed by kernel and tile variant.
+PMC_TRAFFIC_MB = {("k_fct_x", 64): 503.8, ("k_fct_x", 128): 503.8, ("k_fct_x", 256): 503.8,
+                  ("k_fct_z", 16): 1056.5, ("k_fct_z", 32): 948.8, ("k_fct_z", 64): 807.9,
+                  ("k_maxwell", None): 846.7}
 
 
 def run_gpu(precision, steps, warmup, fused=1):
@@ -174,6 +181,9 @@ def main():
                               alg_GBs=bytes_alg / avg_s / 1e9)
     dom = max(kern, key=lambda k: kern[k]["avg_us"])
     d = kern[dom]
+    tile = {"k_fct_x": r64["tuning"][0], "k_fct_z": r64["tuning"][1]}.get(dom)
+    traffic = PMC_TRAFFIC_MB.get((dom, tile))
+    traffic = traffic * 1e6 if (traffic and r64["nx"] == 2048 and r64["nz"] == 1024) else None
     step_alg_GBs = STEP_ARRAYS * 8 * r64["cells"] / (ms_step * 1e-3) / 1e9
 
     out = {
@@ -196,8 +206,9 @@ def main():
                    "nx": r64["nx"], "nz": r64["nz"], "precision": "fp64",
                    "sweep": "fused x+z" if fused else "split x / z"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": d["alg_GBs"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": d["alg_GBs"] / HBM_PEAK_GBS, "traffic": None,
-                     "alg_bytes_per_launch": d["alg_bytes"], "avg_launch_us": d["avg_us"]},
+                     "unit": "GB/s", "frac": d["alg_GBs"] / HBM_PEAK_GBS, "traffic": traffic,
+                     "alg_bytes_per_launch": d["alg_bytes"], "avg_launch_us": d["avg_us"],
+                     "traffic_GBs": traffic / (d["avg_us"] * 1e-6) / 1e9 if traffic else None},
         "step_roofline": {"alg_bytes_per_cell": STEP_ARRAYS * 8, "achieved_GBs": step_alg_GBs,
                           "frac": step_alg_GBs / HBM_PEAK_GBS},
         "kernels": kern,
