@@ -1,5 +1,7 @@
-// khs_run — command-line driver with the reference's CLI: `khs_run <config.ini>`
-// (ymgpu2d/main_ym.cu:18-24).  It runs the whole simulation through the C ABI:
+// khs_run — command-line driver with the reference's CLIs:
+//   khs_run <config.ini>                      ymgpu2d/main_ym.cu:18-24
+//   khs_run --emhd [k_mode] [perturb_amp]     emhdgpu2d/main_coupled.cu:227-229
+// (an .ini with "model = emhd" selects the EMHD model as well).  It runs the whole simulation through the C ABI:
 // config parse, init (with optional eigenmode seed), output directory, energy
 // checks and halting every energy_stride steps, CSV snapshots every
 // export_stride steps (main_ym.cu:143-176, 244-491).
@@ -26,23 +28,45 @@ static int fail(const char* what) {
 int main(int argc, char** argv) {
     if (argc < 2) {
         std::cerr << "Usage: khs_run <config.ini>\n"
+                  << "       khs_run --emhd [k_mode] [perturb_amp]\n"
                   << "  Config is a key=value file with the ymgpu2d example.ini keys.\n";
         return 1;
     }
     khs_config cfg;
-    if (khs_load_ini(argv[1], &cfg) != KHS_OK) return fail("config");
+    if (std::strcmp(argv[1], "--emhd") == 0) {
+        // main_coupled.cu:228-229: k_mode = atoi(argv[1]) (default 1),
+        // perturb_amp = atof(argv[2]) (default 0.05)
+        if (khs_config_defaults(&cfg) != KHS_OK) return fail("config");
+        if (khs_config_set(&cfg, "model", "emhd") != KHS_OK) return fail("config");
+        const std::string k = (argc > 2) ? std::to_string(std::atoi(argv[2])) : "1";
+        if (khs_config_set(&cfg, "k_mode", k.c_str()) != KHS_OK) return fail("config");
+        if (argc > 3 && khs_config_set(&cfg, "perturb_amp", argv[3]) != KHS_OK) return fail("config");
+        if (khs_derive(&cfg) != KHS_OK) return fail("config");
+    } else if (khs_load_ini(argv[1], &cfg) != KHS_OK) {
+        return fail("config");
+    }
+    const bool emhd = cfg.model == KHS_MODEL_EMHD;
+    const char* snap_prefix = emhd ? "coupled_" : "ym_";
 
     static const char* mode_names[] = {"NAB_LINEAR", "NAB_CIRC", "EMHD_KH", "NAB_DTANH",
                                        "NAB_STEP", "NAB_TANH_COSAZ", "NAB_CIRC_AZ2"};
-    std::cout << "================================================================\n"
-              << " MI355X SU(2) Yang-Mills KH solver — config: " << argv[1] << "\n"
-              << " k_mode=" << cfg.k_mode << "  alpha_YM=" << cfg.alpha_YM
-              << "  perturb_amp=" << cfg.perturb_amp << "  V0=" << cfg.V0
-              << "  mode=" << mode_names[cfg.run_mode] << "\n"
-              << " NX=" << cfg.nx << "  NZ=" << cfg.nz << "  DX=" << cfg.dx << "  DZ=" << cfg.dz
-              << "  DT=" << cfg.dt << "  steps=" << cfg.total_steps
-              << "  precision=" << (cfg.precision == KHS_FP32 ? "fp32" : "fp64") << "\n"
-              << "================================================================\n";
+    if (emhd)
+        std::cout << "===============================================================\n"
+                  << " MI355X coupled solver: 2D EMHD (" << cfg.species << " fluid"
+                  << (cfg.species > 1 ? "s" : "") << ")\n"
+                  << " k_mode = " << cfg.k_mode << "  perturb_amp = " << cfg.em_perturb_amp
+                  << "  NX=" << cfg.nx << "  NZ=" << cfg.nz << "  DT=" << cfg.dt << "\n"
+                  << "===============================================================\n";
+    else
+        std::cout << "================================================================\n"
+                  << " MI355X SU(2) Yang-Mills KH solver — config: " << argv[1] << "\n"
+                  << " k_mode=" << cfg.k_mode << "  alpha_YM=" << cfg.alpha_YM
+                  << "  perturb_amp=" << cfg.perturb_amp << "  V0=" << cfg.V0
+                  << "  mode=" << mode_names[cfg.run_mode] << "\n"
+                  << " NX=" << cfg.nx << "  NZ=" << cfg.nz << "  DX=" << cfg.dx << "  DZ=" << cfg.dz
+                  << "  DT=" << cfg.dt << "  steps=" << cfg.total_steps
+                  << "  precision=" << (cfg.precision == KHS_FP32 ? "fp32" : "fp64") << "\n"
+                  << "================================================================\n";
 
     khs_ctx* ctx = nullptr;
     if (khs_create(&cfg, &ctx) != KHS_OK) return fail("create");
@@ -53,7 +77,7 @@ int main(int argc, char** argv) {
     fs::create_directories(out_dir);
     for (auto& e : fs::directory_iterator(out_dir)) {
         auto nm = e.path().filename().string();
-        if (nm.rfind("ym_", 0) == 0 && e.path().extension() == ".csv") fs::remove(e.path());
+        if (nm.rfind(snap_prefix, 0) == 0 && e.path().extension() == ".csv") fs::remove(e.path());
     }
 
     if (khs_init(ctx, nullptr) != KHS_OK) return fail("init");
