@@ -214,6 +214,22 @@ int khs_energy0(khs_ctx* ctx, double* E0);
  * out[3..5] = rms G^a with rho = -nQ (continuity-consistent sign),
  * out[6] = Etot_scale = rms(sum|E|)/dx. */
 int khs_gauss(khs_ctx* ctx, double out[7]);
+/* Circular amplitude of the colour-2/3 potential at z mode kz
+ * (analysis/remote_timeseries.py:62-77): F(x) = sum_z (Az2 + i Az3)
+ * e^{-2 pi i kz z/NZ}; out[0] = mean_x |F| / NZ, out[1] = the same for
+ * Az2 - i Az3.  FP64 device reduction.  YM model only. */
+int khs_mode_amplitude(khs_ctx* ctx, int kz, double out[2]);
+
+/* ---------------- growth-rate fits (host only, no context) ---------------- */
+/* Max-R^2 window fit of log(amp) vs t (analysis/batch_analyze.py:44-69).
+ * t_min / t_max: NaN = unbounded.  out = {gamma, t0, t1, r2}; fewer than 4
+ * samples give {0, NaN, NaN, 0}, no window with slope >= 1e-6 gives r2 = -1e9. */
+int khs_fit_growth_max_r2(const double* t, const double* amp, int n, double t_min, double t_max,
+                          double min_win_frac, double out[4]);
+/* Plateau fit (batch_analyze.py:90-152).  out = {gamma, t0, t1, duration,
+ * has_plateau (1/0)}; without a plateau gamma, t0, t1, duration are NaN. */
+int khs_fit_growth_plateau(const double* t, const double* amp, int n, double win_tu,
+                           double min_plateau_tu, double flat_tol, double out[5]);
 
 /* ---------------- output files (byte-compatible with main_ym.cu) ---------------- */
 int khs_write_snapshot(khs_ctx* ctx, const char* dir, int step);      /* ym_%06d.csv, :287-306 */

@@ -30,6 +30,7 @@ SOURCES = [
     ("khs_context.hip", True),
     ("khs_config.cpp", False),
     ("khs_init.cpp", False),
+    ("khs_analysis.cpp", False),
 ]
 
 

@@ -453,6 +453,44 @@ void launch_derived_em(const State<T>& S, int which, const KParams& p, double* o
     hipLaunchKernelGGL(k_derived_em<T>, dim3(blocks), dim3(256), 0, st, S, which, p, out);
 }
 
+// --------------------------------------------------------------------------
+// Circular mode amplitude (ymgpu2d/analysis/remote_timeseries.py:62-77):
+// F(x) = sum_z (Az2 + i Az3) e^{-2 pi i k z / nz}, amp = mean_x |F| / nz, and
+// the same for the conjugate signal (Az2 - i Az3).  FP64 accumulation; the
+// mean over x is a compensated sum.  One thread per column.
+// --------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void k_mode_amp(const T* a2, const T* a3, int kz, KParams p,
+                                                          double* partials) {
+    const int nx = p.nx, nz = p.nz;
+    const int x = blockIdx.x * kRedThreads + threadIdx.x;
+    CSum acc[2] = {CSum{0.0, 0.0}, CSum{0.0, 0.0}};
+    if (x < nx) {
+        double fr = 0.0, fi = 0.0, gr = 0.0, gi = 0.0;
+        for (int z = 0; z < nz; ++z) {
+            const size_t i = (size_t)x + (size_t)z * nx;
+            const double u = (double)a2[i], v = (double)a3[i];
+            double sn, cs;
+            sincospi(2.0 * (double)(((long)kz * z) % nz) / (double)nz, &sn, &cs);
+            fr += u * cs + v * sn;          // (u + i v)(cos - i sin)
+            fi += v * cs - u * sn;
+            gr += u * cs - v * sn;          // (u - i v)(cos - i sin)
+            gi += -v * cs - u * sn;
+        }
+        cadd(acc[0], sqrt(fr * fr + fi * fi));
+        cadd(acc[1], sqrt(gr * gr + gi * gi));
+    }
+    block_reduce_store<2>(acc, partials);
+}
+
+template <typename T>
+void launch_mode_amp(const T* a2, const T* a3, int kz, const KParams& p, double* partials, double* out,
+                     hipStream_t st) {
+    const int blocks = (p.nx + kRedThreads - 1) / kRedThreads;
+    hipLaunchKernelGGL(k_mode_amp<T>, dim3(blocks), dim3(kRedThreads), 0, st, a2, a3, kz, p, partials);
+    hipLaunchKernelGGL(k_reduce_final<2>, dim3(1), dim3(kRedThreads), 0, st, partials, blocks, out);
+}
+
 // explicit instantiations
 #define KHS_INST(T)                                                                                   \
     template void launch_cell_filters<T>(const ArrayList<T>&, const double*, const int*, const KParams&, hipStream_t); \
@@ -464,7 +502,8 @@ void launch_derived_em(const State<T>& S, int which, const KParams& p, double* o
     template void launch_gauss<T>(const State<T>&, const KParams&, double*, double*, hipStream_t);   \
     template void launch_derived<T>(const State<T>&, int, const KParams&, double*, hipStream_t);   \
     template void launch_energy_em<T>(const State<T>&, const KParams&, double*, double*, hipStream_t); \
-    template void launch_derived_em<T>(const State<T>&, int, const KParams&, double*, hipStream_t);
+    template void launch_derived_em<T>(const State<T>&, int, const KParams&, double*, hipStream_t); \
+    template void launch_mode_amp<T>(const T*, const T*, int, const KParams&, double*, double*, hipStream_t);
 KHS_INST(double)
 KHS_INST(float)
 #undef KHS_INST

@@ -835,6 +835,29 @@ int khs_gauss(khs_ctx* ctx, double out[7]) {
     });
 }
 
+int khs_mode_amplitude(khs_ctx* ctx, int kz, double out[2]) {
+    CTX_OR_FAIL(ctx);
+    if (!out) { g_last_error = "null argument"; return KHS_ERR_ARG; }
+    return guard([&] {
+        Ctx& c = *ctx->impl;
+        if (c.emhd) throw Error(KHS_ERR_ARG, "khs_mode_amplitude: YM model only (Az2, Az3)");
+        if (kz < 0) throw Error(KHS_ERR_ARG, "kz must be >= 0");
+        if (c.fp32)
+            khs::launch_mode_amp<float>((const float*)c.st[c.cur][khs::S_AZ2], (const float*)c.st[c.cur][khs::S_AZ3],
+                                        kz, c.kp, c.d_partials, c.d_red, c.stream);
+        else
+            khs::launch_mode_amp<double>((const double*)c.st[c.cur][khs::S_AZ2],
+                                         (const double*)c.st[c.cur][khs::S_AZ3], kz, c.kp, c.d_partials,
+                                         c.d_red, c.stream);
+        c.check_launch();
+        HIPCHK(hipMemcpyAsync(c.h_red, c.d_red, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        HIPCHK(hipStreamSynchronize(c.stream));
+        const double norm = (double)c.nx * (double)c.nz;
+        out[0] = c.h_red[0] / norm;
+        out[1] = c.h_red[1] / norm;
+    });
+}
+
 int khs_write_snapshot(khs_ctx* ctx, const char* dir, int step) {
     CTX_OR_FAIL(ctx);
     if (!dir) { g_last_error = "null argument"; return KHS_ERR_ARG; }

@@ -69,6 +69,11 @@ void launch_energy(const State<T>& S, const KParams& p, double* partials, double
 template <typename T>
 void launch_gauss(const State<T>& S, const KParams& p, double* partials, double* out,
                   hipStream_t st);
+// Circular amplitude of (a2 + i a3) at z mode kz: out[0] = sum_x |F|, out[1] =
+// the same for the conjugate signal (the caller divides by nx * nz).
+template <typename T>
+void launch_mode_amp(const T* a2, const T* a3, int kz, const KParams& p, double* partials, double* out,
+                     hipStream_t st);
 // Derived per-cell array (ids of khs.h >= 32) evaluated from S into out (double).
 template <typename T>
 void launch_derived(const State<T>& S, int id, const KParams& p, double* out, hipStream_t st);

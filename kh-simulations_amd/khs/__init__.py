@@ -97,12 +97,24 @@ def load_library(path=None):
     if not os.path.exists(p):
         raise ImportError(f"{p} not found: build it with `python kh-simulations_amd/build.py`")
     lib = ctypes.CDLL(p)
+    for name, (res, args) in _signatures().items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.khs_config_sizeof() != ctypes.sizeof(khs_config):
+        raise ImportError("khs_config layout mismatch between binding and library")
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _signatures():
     c_int, c_double, c_char_p, c_size_t, c_void_p = (ctypes.c_int, ctypes.c_double, ctypes.c_char_p,
                                                       ctypes.c_size_t, ctypes.c_void_p)
     P = ctypes.POINTER
     cfgp = P(khs_config)
     dp = P(c_double)
-    sigs = {
+    return {
         "khs_last_error": (c_char_p, []), "khs_version": (c_char_p, []), "khs_config_sizeof": (c_size_t, []),
         "khs_config_defaults": (c_int, [cfgp]), "khs_config_set": (c_int, [cfgp, c_char_p, c_char_p]),
         "khs_load_ini": (c_int, [c_char_p, cfgp]), "khs_derive": (c_int, [cfgp]),
@@ -127,27 +139,42 @@ def load_library(path=None):
         "khs_kernel_times": (c_int, [c_void_p, dp, P(c_int), c_int]),
         "khs_kernel_name": (c_char_p, [c_int]),
         "khs_get_tuning": (c_int, [c_void_p, P(c_int), P(c_int)]),
+        "khs_mode_amplitude": (c_int, [c_void_p, c_int, dp]),
+        "khs_fit_growth_max_r2": (c_int, [dp, dp, c_int, c_double, c_double, c_double, dp]),
+        "khs_fit_growth_plateau": (c_int, [dp, dp, c_int, c_double, c_double, c_double, dp]),
     }
-    for name, (res, args) in sigs.items():
-        fn = getattr(lib, name)
-        fn.restype = res
-        fn.argtypes = args
-    if lib.khs_config_sizeof() != ctypes.sizeof(khs_config):
-        raise ImportError("khs_config layout mismatch between binding and library")
-    if path is None:
-        _lib = lib
-    return lib
 
 
 def exported_symbols():
     """Names this binding binds (all entry points of include/khs.h)."""
-    return ["khs_last_error", "khs_version", "khs_config_sizeof", "khs_config_defaults", "khs_config_set",
-            "khs_load_ini", "khs_derive", "khs_output_dir", "khs_create", "khs_destroy", "khs_init",
-            "khs_array_id", "khs_get_array", "khs_set_array", "khs_get_array_buf", "khs_fct_x_sweep", "khs_fct_z_sweep",
-            "khs_maxwell", "khs_filters", "khs_commit", "khs_step", "khs_get_step", "khs_energy",
-            "khs_energy0", "khs_gauss", "khs_write_snapshot", "khs_energy_csv_open", "khs_energy_csv_row",
-            "khs_energy_csv_close", "khs_sync", "khs_timing_enable", "khs_timing_reset", "khs_kernel_times",
-            "khs_kernel_name", "khs_get_tuning"]
+    return list(_signatures())
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def fit_growth_max_r2(t, amp, t_min=None, t_max=None, min_win_frac=0.15):
+    """Max-R^2 growth-rate fit (analysis/batch_analyze.py:44-69) -> (gamma, t0, t1, r2)."""
+    load_library()
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    a = np.ascontiguousarray(amp, dtype=np.float64)
+    out = np.zeros(4)
+    nan = float("nan")
+    _check(_lib.khs_fit_growth_max_r2(_dptr(t), _dptr(a), len(t), nan if t_min is None else t_min,
+                                      nan if t_max is None else t_max, min_win_frac, _dptr(out)))
+    return tuple(float(v) for v in out)
+
+
+def fit_growth_plateau(t, amp, win_tu=8.0, min_plateau_tu=10.0, flat_tol=0.06):
+    """Plateau growth-rate fit (analysis/batch_analyze.py:90-152)."""
+    load_library()
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    a = np.ascontiguousarray(amp, dtype=np.float64)
+    out = np.zeros(5)
+    _check(_lib.khs_fit_growth_plateau(_dptr(t), _dptr(a), len(t), win_tu, min_plateau_tu, flat_tol, _dptr(out)))
+    return {"gamma": float(out[0]), "t0": float(out[1]), "t1": float(out[2]), "duration": float(out[3]),
+            "has_plateau": bool(out[4])}
 
 
 def _check(rc):
@@ -308,6 +335,28 @@ class Simulation:
         _check(_lib.khs_gauss(self._h, out))
         v = list(out)
         return {"rms_ref": v[0:3], "rms_cont": v[3:6], "Etot_scale": v[6]}
+
+    def mode_amplitude(self, kz):
+        """Circular amplitude of Az2 + i Az3 at z mode kz -> (amp, amp_conj)."""
+        out = (ctypes.c_double * 2)()
+        _check(_lib.khs_mode_amplitude(self._h, int(kz), out))
+        return out[0], out[1]
+
+    def timeseries(self, kz, n_steps, sample_stride):
+        """Step n_steps, sampling the mode-kz amplitude at t = 0 and every
+        sample_stride steps (stops early if the run halts)."""
+        ts, amps, conj = [], [], []
+        dt = self.config.dt
+        a, c = self.mode_amplitude(kz)
+        ts.append(self.step_count * dt); amps.append(a); conj.append(c)
+        done, halted = 0, False
+        while done < n_steps and not halted:
+            k = min(sample_stride, n_steps - done)
+            halted = self.step(k)
+            done += k
+            a, c = self.mode_amplitude(kz)
+            ts.append(self.step_count * dt); amps.append(a); conj.append(c)
+        return {"t": np.array(ts), "amp": np.array(amps), "amp_conj": np.array(conj), "halted": halted}
 
     # ---- files ----
     def write_snapshot(self, directory, step):
