@@ -27,6 +27,7 @@ HIP = ["-x", "hip", f"--offload-arch={ARCH}",
 SOURCES = [
     ("khs_step_kernels.hip", True),
     ("khs_aux_kernels.hip", True),
+    ("khs_filter_mfma.hip", True),
     ("khs_context.hip", True),
     ("khs_config.cpp", False),
     ("khs_init.cpp", False),

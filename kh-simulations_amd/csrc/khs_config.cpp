@@ -71,6 +71,7 @@ void config_defaults(khs_config& c) {
     c.spectrum_amp = 1e-5;
     c.spectrum_seed = 2;
     c.em_perturb_amp = 0.05;
+    c.mfma_filter = 1;
 }
 
 static int parse_int(const std::string& k, const std::string& v) {
@@ -152,6 +153,7 @@ bool config_set(khs_config& c, const std::string& key, const std::string& val) {
         else if (val == "spectrum" || val == "1") c.emhd_init = 1;
         else throw Error(KHS_ERR_CONFIG, "emhd_init must be daskaw or spectrum");
     }
+    else if (key == "mfma_filter")       c.mfma_filter = parse_int(key, val);
     else if (key == "smooth_init")       c.smooth_init = parse_int(key, val);
     else if (key == "spectrum_kmax")     c.spectrum_kmax = parse_int(key, val);
     else if (key == "spectrum_amp")      c.spectrum_amp = parse_double(key, val);

@@ -75,6 +75,7 @@ class khs_config(ctypes.Structure):
         ("charge1", ctypes.c_double), ("charge2", ctypes.c_double), ("emhd_init", ctypes.c_int),
         ("smooth_init", ctypes.c_int), ("spectrum_kmax", ctypes.c_int), ("spectrum_amp", ctypes.c_double),
         ("spectrum_seed", ctypes.c_uint64), ("em_perturb_amp", ctypes.c_double),
+        ("mfma_filter", ctypes.c_int),
         ("nx", ctypes.c_int), ("nz", ctypes.c_int), ("periodic_x", ctypes.c_int), ("freeze_az1", ctypes.c_int),
         ("lx", ctypes.c_double), ("lz", ctypes.c_double), ("dx", ctypes.c_double), ("dz", ctypes.c_double),
         ("courant", ctypes.c_double), ("dt", ctypes.c_double),
