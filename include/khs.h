@@ -126,7 +126,7 @@ typedef struct khs_config {
     double   spectrum_amp;
     uint64_t spectrum_seed;
     double   em_perturb_amp;   /* EMHD das_kaw perturbation (argv[2] as double, default 0.05); set by perturb_amp */
-    int      mfma_filter;      /* FP32 form: keep-|kz|<=K filter on MFMA (1, default) or the FP64-sum column kernels (0) */
+    int      mfma_filter;      /* keep-|kz|<=K filter as MFMA contractions (1, default) or the per-column FP64-sum kernels (0) */
     /* --- derived (khs_derive) --- */
     int    nx, nz;
     int    periodic_x;         /* run_mode >= 1 */

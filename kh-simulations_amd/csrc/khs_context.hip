@@ -80,7 +80,7 @@ struct Ctx {
     bool cell_filters = false;
     ModeTable t_zmean, t_lo, t_hi, t_keep;
     double* d_coef = nullptr;
-    float* d_keep_tab = nullptr;     // FP32 MFMA keep filter table (nz x 16)
+    void* d_keep_tab = nullptr;      // MFMA keep filter table (nz x 16, run precision)
 
     // reductions / derived
     double* d_partials = nullptr;
@@ -260,15 +260,10 @@ struct Ctx {
             if (c.kz_suppress_max >= 1) do_modes<T>(t_lo, list<T>(k, ids21, 21, skip_c1), 0);
             if (c.kz_suppress_hi > c.kz_suppress_max + 1) do_modes<T>(t_hi, list<T>(k, ids21, 21, skip_c1), 0);
             if (c.kz_keep_max > 0) {
-                if constexpr (std::is_same<T, float>::value) {
-                    if (d_keep_tab) {
-                        launch_keep_mfma(list<float>(k, ids21, 21, skip_c1), d_keep_tab, (float*)d_coef, kp, stream);
-                    } else {
-                        do_modes<T>(t_keep, list<T>(k, ids21, 21, skip_c1), 1);
-                    }
-                } else {
+                if (d_keep_tab)
+                    launch_keep_mfma<T>(list<T>(k, ids21, 21, skip_c1), (const T*)d_keep_tab, (T*)d_coef, kp, stream);
+                else
                     do_modes<T>(t_keep, list<T>(k, ids21, 21, skip_c1), 1);
-                }
             }
             if (c.hyp_diff > 0.0f) launch_hyperdiffuse<T>(list<T>(k, kCol23, 12, false), (double)c.hyp_diff, kp, stream);
             if (c.suppress_kz0) {
@@ -433,14 +428,13 @@ static std::unique_ptr<Ctx> make_ctx(const khs_config& in) {
     reserve_tab(ctx->t_keep, nkeep, false);
     reserve_tab(ctx->t_zmean, c.suppress_kz0 ? 1 : 0, false);
     int maxm = std::max(std::max(nlo, nhi), std::max(nkeep, c.suppress_kz0 ? 1 : 0));
-    const bool keep_mfma = ctx->fp32 && c.mfma_filter && c.kz_keep_max > 0 &&
-                           keep_mfma_supported(c.nx, c.nz, c.kz_keep_max);
+    const bool keep_mfma = c.mfma_filter && c.kz_keep_max > 0 && keep_mfma_supported(c.nx, c.nz, c.kz_keep_max);
     // coefficient scratch: 21 arrays x 2 x modes doubles (column kernels) or
-    // 21 arrays x 17 rows floats (MFMA keep filter)
+    // 21 arrays x 17 rows of the run precision (MFMA keep filter)
     size_t coef_bytes = (size_t)21 * 2 * maxm * c.nx * sizeof(double);
-    if (keep_mfma) coef_bytes = std::max(coef_bytes, (size_t)21 * 17 * c.nx * sizeof(float));
+    if (keep_mfma) coef_bytes = std::max(coef_bytes, (size_t)21 * 17 * c.nx * ctx->esz);
     if (coef_bytes) reserve((void**)&ctx->d_coef, coef_bytes);
-    if (keep_mfma) reserve((void**)&ctx->d_keep_tab, (size_t)c.nz * 16 * sizeof(float));
+    if (keep_mfma) reserve((void**)&ctx->d_keep_tab, (size_t)c.nz * 16 * ctx->esz);
     reserve((void**)&ctx->d_partials, (size_t)kRedBlocks * 8 * 2 * sizeof(double));
     reserve((void**)&ctx->d_red, 16 * sizeof(double));
     reserve((void**)&ctx->d_derived, ctx->N * sizeof(double));
@@ -459,11 +453,16 @@ static std::unique_ptr<Ctx> make_ctx(const khs_config& in) {
     if (nlo) { up(ctx->t_lo.c, lo_c.data(), lo_c.size() * 4); up(ctx->t_lo.s, lo_s.data(), lo_s.size() * 4); up(ctx->t_lo.w, lo_w.data(), lo_w.size() * 4); }
     if (nhi) { up(ctx->t_hi.c, hi_c.data(), hi_c.size() * 4); up(ctx->t_hi.s, hi_s.data(), hi_s.size() * 4); up(ctx->t_hi.w, hi_w.data(), hi_w.size() * 4); }
     if (nkeep) { up(ctx->t_keep.c, kp_c.data(), kp_c.size() * 8); up(ctx->t_keep.s, kp_s.data(), kp_s.size() * 8); up(ctx->t_keep.w, kp_w.data(), kp_w.size() * 8); }
-    std::vector<float> kt;            // lives until the synchronisation below
-    if (keep_mfma) {
+    std::vector<float> kt;            // live until the synchronisation below
+    std::vector<double> kt64;
+    if (keep_mfma && ctx->fp32) {
         kt.resize((size_t)c.nz * 16);
-        fill_keep_mfma_table(c.nz, c.kz_keep_max, kt.data());
+        fill_keep_mfma_table<float>(c.nz, c.kz_keep_max, kt.data());
         up(ctx->d_keep_tab, kt.data(), kt.size() * sizeof(float));
+    } else if (keep_mfma) {
+        kt64.resize((size_t)c.nz * 16);
+        fill_keep_mfma_table<double>(c.nz, c.kz_keep_max, kt64.data());
+        up(ctx->d_keep_tab, kt64.data(), kt64.size() * sizeof(double));
     }
     if (c.suppress_kz0) { up(ctx->t_zmean.c, zm_c.data(), nz * 8); up(ctx->t_zmean.s, zm_s.data(), nz * 8); up(ctx->t_zmean.w, zm_w.data(), 8); }
     HIPCHK(hipStreamSynchronize(ctx->stream));

@@ -53,12 +53,13 @@ template <typename T, typename TW>
 void launch_column_modes(const ArrayList<T>& arrs, const TW* ctab, const TW* stab,
                          const TW* weight, int nmodes, int project, double* coef,
                          const KParams& p, hipStream_t st);
-// FP32 keep-|kz|<=K projection on MFMA (khs_filter_mfma.hip).  tab: nz x 16
-// floats from fill_keep_mfma_table; coef scratch: arrays * 17 * nx floats.
+// keep-|kz|<=K projection on MFMA (khs_filter_mfma.hip).  tab: nz x 16 values
+// from fill_keep_mfma_table; coef scratch: arrays * 17 * nx values.
 bool keep_mfma_supported(int nx, int nz, int K);
-void fill_keep_mfma_table(int nz, int K, float* tab);
-void launch_keep_mfma(const ArrayList<float>& arrs, const float* tab, float* coef, const KParams& p,
-                      hipStream_t st);
+template <typename T>
+void fill_keep_mfma_table(int nz, int K, T* tab);
+template <typename T>
+void launch_keep_mfma(const ArrayList<T>& arrs, const T* tab, T* coef, const KParams& p, hipStream_t st);
 template <typename T>
 void launch_hyperdiffuse(const ArrayList<T>& arrs, double mu, const KParams& p, hipStream_t st);
 template <typename T>
