@@ -15,7 +15,10 @@ species, Maxwell, sources, the xi_cut, and the energy check at the reference
 cadence every (int)(0.5/DT) steps).  All state is resident in HBM before the
 timed region.  Timed regions longer than SEGMENT steps are split into
 segments that restart from the initial state (untimed) so every timed step is
-physically finite.  The FP32 precision form of the same workload is timed too.
+physically finite.  The FP32 precision form of the same workload is timed too,
+and so is the precision-form workload (4096x2048, V0 = 0.1, a = 0.5, keep
+|kz| <= 8 on MFMA) in FP32 and FP64, with the kz = 1 growth rate of both
+(max-R^2 fit over the same window) and its relative error.
 
 Prints one JSON line (rank 0, N = 1 only; there is one GPU per box).
 """
@@ -35,6 +38,13 @@ WORKLOAD = dict(nx_override=2048, nz_override=1024, lx_override=204.8, lz_overri
                 V0=0.05, eps_override=1.0, courant_override=0.4, k_mode=1, run_mode=0, freeze_override=1,
                 perturb_amp=2e-5, noise_amp=1e-9, noise_seed=1, n_steps=5000, xi_cut=12.8)
 SEGMENT = 700
+# Precision-form workload: 4096x2048, V0 = 0.1, a = 0.5, keep |kz| <= 8 (the
+# column filter runs as MFMA contractions), xi_cut at |xi| = 12.8 for the same
+# reason as above; E/E0 stays below 5 for ~650 steps, so segments are 500.
+WORKLOAD_C3 = dict(nx_override=4096, nz_override=2048, lx_override=409.6, lz_override=204.8, alpha_YM=1,
+                   V0=0.1, eps_override=0.5, courant_override=0.4, k_mode=1, run_mode=0, freeze_override=1,
+                   perturb_amp=1e-5, noise_amp=1e-9, noise_seed=1, n_steps=10000, kz_keep_max=8, xi_cut=12.8)
+SEGMENT_C3 = 500
 
 # Compulsory (algorithmic) HBM traffic of each step kernel, in arrays per cell:
 # every input array read once and every output written once, no halo re-reads.
@@ -54,10 +64,10 @@ PMC_TRAFFIC_MB = {("k_fct_x", 64): 503.8, ("k_fct_x", 128): 503.8, ("k_fct_x", 2
                   ("k_maxwell", None): 846.7}
 
 
-def run_gpu(precision, steps, warmup, fused=1):
+def run_gpu(precision, steps, warmup, fused=1, workload=WORKLOAD, segment=SEGMENT):
     import khs
 
-    keys = dict(WORKLOAD, fused_sweep=fused)
+    keys = dict(workload, fused_sweep=fused)
     if precision == "fp32":
         keys["precision"] = "fp32"
     cfg = khs.Config(**keys)
@@ -66,7 +76,7 @@ def run_gpu(precision, steps, warmup, fused=1):
     times = {}
     e_ratio = None
     while done < steps:
-        seg = min(SEGMENT, steps - done)
+        seg = min(segment, steps - done)
         sim.init()
         if sim.step(warmup):
             raise RuntimeError("warm-up halted")
@@ -92,6 +102,21 @@ def run_gpu(precision, steps, warmup, fused=1):
     sim.close()
     return dict(seconds=total_s, cells=cells, times=times, e_ratio=e_ratio, nx=cfg.nx, nz=cfg.nz,
                 segments=nseg, tuning=tuning)
+
+
+def growth_c3(precision, n_steps=500, stride=10):
+    """kz = 1 growth rate of the precision-form workload (max-R^2 fit of the
+    circular Az2/Az3 amplitude, sampled every `stride` steps from init)."""
+    import khs
+
+    cfg = khs.Config(**WORKLOAD_C3, precision=precision)
+    sim = khs.Simulation(cfg)
+    sim.init()
+    ts = sim.timeseries(1, n_steps, stride)
+    sim.close()
+    if ts["halted"]:
+        raise RuntimeError("growth-rate run halted")
+    return ts
 
 
 def cpu_baseline(budget_s=15.0):
@@ -147,6 +172,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-c3", action="store_true", help="skip the 4096x2048 precision-form leg")
+    ap.add_argument("--c3-steps", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB")
     ap.add_argument("--split", action="store_true", help="separate x and z sweep kernels")
@@ -221,6 +248,25 @@ def main():
                             "speedup_vs_fp64": v32 / value,
                             "kernels_avg_us": {k: (v[0] / v[1] * 1e3 if v[1] else None)
                                                for k, v in r32["times"].items()}}
+    if not a.no_c3:
+        import khs
+        c3 = {}
+        for prec in ("fp32", "fp64"):
+            r = run_gpu(prec, a.c3_steps, 20, 0, WORKLOAD_C3, SEGMENT_C3)
+            c3[prec] = dict(value=r["cells"] * a.c3_steps / r["seconds"],
+                            ms_per_step=r["seconds"] / a.c3_steps * 1e3,
+                            kernels_avg_us={k: (v[0] / v[1] * 1e3 if v[1] else None) for k, v in r["times"].items()})
+        t32, t64 = growth_c3("fp32"), growth_c3("fp64")
+        g64, t0, t1, r2 = khs.fit_growth_max_r2(t64["t"], t64["amp"])
+        g32 = khs.fit_growth_max_r2(t32["t"], t32["amp"], t_min=t0, t_max=t1)[0]
+        g64w = khs.fit_growth_max_r2(t64["t"], t64["amp"], t_min=t0, t_max=t1)[0]
+        out["precision_form"] = {
+            "workload": "4096x2048, V0=0.1, a=0.5, keep |kz|<=8 (MFMA filter), xi_cut=12.8, walled x",
+            "fp32_cell_updates_per_s": c3["fp32"]["value"], "fp64_cell_updates_per_s": c3["fp64"]["value"],
+            "fp32_over_fp64": c3["fp32"]["value"] / c3["fp64"]["value"],
+            "gamma_fp64": g64w, "gamma_fp32": g32, "gamma_rel_err": abs(g32 - g64w) / g64w,
+            "fit_window_t": [t0, t1], "steps": a.c3_steps, "segments_of": SEGMENT_C3,
+            "kernels_avg_us": {p: c3[p]["kernels_avg_us"] for p in c3}}
     if not a.no_cpu:
         cb = cpu_baseline(a.cpu_seconds)
         out["cpu_baseline"] = cb
