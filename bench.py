@@ -229,7 +229,8 @@ def main():
         "config": {"workload": "plain FP64 single run, 2048x1024, x walled / z periodic, dx=dz=0.1, "
                                "V0=0.05, a=1.0, seed 1, CFL 0.4, no mode filter, xi_cut=12.8",
                    "segments": r64["segments"],
-                   "tiles": {"x_segment": r64["tuning"][0], "z_chunk": r64["tuning"][1]},
+                   "tiles": {"x_segment": r64["tuning"][0], "z_chunk": r64["tuning"][1],
+                             "z_prefetch": r64["tuning"][2]},
                    "nx": r64["nx"], "nz": r64["nz"], "precision": "fp64",
                    "sweep": "fused x+z" if fused else "split x / z"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": d["alg_GBs"], "peak": HBM_PEAK_GBS,

@@ -184,11 +184,11 @@ struct Ctx {
     // after a warm-up) and keep the fastest.  Runs before the first sweep of
     // the context, when the stage and the next-state fluid are scratch; all
     // variants compute identical bits.
-    template <typename T> int pick(int nvar, const std::function<void(int)>& launch) {
+    template <typename T> int pick(const std::vector<int>& cand, const std::function<void(int)>& launch) {
         hipEvent_t a = get_event(), b = get_event();
         float best = 1e30f;
-        int bi = 0;
-        for (int v = 0; v < nvar; ++v) {
+        int bi = cand[0];
+        for (int v : cand) {
             launch(v);
             HIPCHK(hipEventRecord(a, stream));
             for (int r = 0; r < 3; ++r) launch(v);
@@ -207,12 +207,20 @@ struct Ctx {
         tuned = true;
         const int fx = cfg.x_segment == 64 ? 0 : cfg.x_segment == 128 ? 1 : cfg.x_segment == 256 ? 2 : -1;
         const int fz = cfg.z_chunk == 16 ? 0 : cfg.z_chunk == 32 ? 1 : cfg.z_chunk == 64 ? 2 : -1;
-        if (fx >= 0) xvar = fx;
-        if (fz >= 0) zvar = fz;
+        const int fp = cfg.z_prefetch;               // -1 = tuned
+        std::vector<int> xc, zc;
+        for (int v = 0; v < kNumXVariants; ++v)
+            if (fx < 0 || v == fx) xc.push_back(v);
+        for (int v = 0; v < kNumZVariants; ++v)
+            if ((fz < 0 || v % 3 == fz) && (fp < 0 || v / 3 == fp)) zc.push_back(v);
+        // without tuning: the forced shape, else 256-cell segments and 32-row
+        // chunks with prefetch in FP32 (registers to spare), without in FP64
+        xvar = (fx >= 0) ? fx : 2;
+        zvar = ((fz >= 0) ? fz : 1) + 3 * ((fp >= 0) ? fp : (fp32 ? 1 : 0));
         if (!cfg.autotune) return;
-        if (fx < 0) xvar = pick<T>(kNumXVariants, [&](int v) { fct_x<T>(0, nsp, v); });
+        if (xc.size() > 1) xvar = pick<T>(xc, [&](int v) { fct_x<T>(0, nsp, v); });
         fct_x<T>(0, nsp, xvar);
-        if (fz < 0) zvar = pick<T>(kNumZVariants, [&](int v) { fct_z<T>(0, nsp, v); });
+        if (zc.size() > 1) zvar = pick<T>(zc, [&](int v) { fct_z<T>(0, nsp, v); });
     }
     void autotune() {
         if (tuned) return;
@@ -401,6 +409,7 @@ static std::unique_ptr<Ctx> make_ctx(const khs_config& in) {
     ctx->cell_filters = (c.xi_sponge > 0.0f) || (c.xi_cut > 0.0f);
     if (c.x_segment) ctx->xvar = c.x_segment == 64 ? 0 : c.x_segment == 128 ? 1 : 2;
     if (c.z_chunk) ctx->zvar = c.z_chunk == 16 ? 0 : c.z_chunk == 32 ? 1 : 2;
+    if (c.z_prefetch == 1) ctx->zvar = ctx->zvar % 3 + 3;
 
     // ---- one device allocation ----
     const bool share_az1 = c.freeze_az1 != 0;
@@ -985,12 +994,13 @@ int khs_kernel_times(khs_ctx* ctx, double* ms, int* launches, int nslots) {
     });
 }
 
-int khs_get_tuning(khs_ctx* ctx, int* x_segment, int* z_chunk) {
+int khs_get_tuning(khs_ctx* ctx, int* x_segment, int* z_chunk, int* z_prefetch) {
     CTX_OR_FAIL(ctx);
     Ctx& c = *ctx->impl;
     static const int xs[3] = {64, 128, 256}, zs[3] = {16, 32, 64};
     if (x_segment) *x_segment = xs[c.xvar];
-    if (z_chunk) *z_chunk = zs[c.zvar];
+    if (z_chunk) *z_chunk = zs[c.zvar % 3];
+    if (z_prefetch) *z_prefetch = c.zvar / 3;
     return KHS_OK;
 }
 

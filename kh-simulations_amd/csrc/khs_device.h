@@ -126,6 +126,34 @@ __device__ __forceinline__ T lorentz_px(const State<T>& S, int sp, int x, int z,
     return s;
 }
 
+// z-component of kernel_ym_lorentz (YM_Fluid.cu:32-84) from values: density,
+// x momentum and charges of the cell, Ez at (x, z) and (x, z-1), By at
+// (x, z), (x-1, z), (x, z-1), (x-1, z-1).  Cold part only.
+template <typename T>
+__device__ __forceinline__ T lorentz_pz_v(T n, T px, const T (&q)[3], const T (&ez)[3], const T (&ezm)[3],
+                                          const T (&by)[3], const T (&byxm)[3], const T (&byzm)[3],
+                                          const T (&byxmzm)[3]) {
+    T n_safe = kmax(n, T(0.05));
+    T vx = px / n_safe;
+    T acc[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        T Ezn = T(0.5) * (ez[c] + ezm[c]);
+        T Byn = T(0.25) * (by[c] + byxm[c] + byzm[c] + byxmzm[c]);
+        acc[c] = q[c] * (Ezn - vx * Byn);
+    }
+    return -(acc[0] + acc[1] + acc[2]);
+}
+
+// Warm-closure pressure term of the z source: -T dn/dz (centred, periodic z).
+template <typename T>
+__device__ __forceinline__ T warm_pz(const State<T>& S, int sp, int x, int z, const KParams& p) {
+    const T* n = S.a[S_NA + sp * F_COUNT + F_N];
+    const int zp = zp_per(z, p.nz), zm = zm_per(z, p.nz);
+    T dndz = (n[(size_t)x + (size_t)zp * p.nx] - n[(size_t)x + (size_t)zm * p.nx]) / (T(2.0) * T(p.dz));
+    return T(p.warm_T) * dndz;
+}
+
 // z-component of kernel_ym_lorentz at real cell (x, z).
 template <typename T>
 __device__ __forceinline__ T lorentz_pz(const State<T>& S, int sp, int x, int z, const KParams& p) {
@@ -136,28 +164,38 @@ __device__ __forceinline__ T lorentz_pz(const State<T>& S, int sp, int x, int z,
     const size_t ixm = (size_t)xm + (size_t)z * nx;
     const size_t izm = (size_t)x + (size_t)zm * nx;
     const size_t ixmzm = (size_t)xm + (size_t)zm * nx;
-    T n_safe = kmax(S.a[b + F_N][i], T(0.05));
-    T vx = S.a[b + F_PX][i] / n_safe;
-    T acc[3];
+    T q[3], ez[3], ezm[3], by[3], byxm[3], byzm[3], byxmzm[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const T* Ez = S.a[S_EZ1 + c];
-        const T* By = S.a[S_BY1 + c];
-        T Ezn = T(0.5) * (Ez[i] + Ez[izm]);
-        T Byn = T(0.25) * (By[i] + By[ixm] + By[izm] + By[ixmzm]);
-        acc[c] = S.a[b + F_Q1 + c][i] * (Ezn - vx * Byn);
+        q[c] = S.a[b + F_Q1 + c][i];
+        ez[c] = S.a[S_EZ1 + c][i];
+        ezm[c] = S.a[S_EZ1 + c][izm];
+        by[c] = S.a[S_BY1 + c][i];
+        byxm[c] = S.a[S_BY1 + c][ixm];
+        byzm[c] = S.a[S_BY1 + c][izm];
+        byxmzm[c] = S.a[S_BY1 + c][ixmzm];
     }
-    T s = -(acc[0] + acc[1] + acc[2]);
-    if (p.warm_T > 0.0) {
-        const T* n = S.a[b + F_N];
-        const int zp = zp_per(z, p.nz);
-        T dndz = (n[(size_t)x + (size_t)zp * nx] - n[izm]) / (T(2.0) * T(p.dz));
-        s -= T(p.warm_T) * dndz;
-    }
+    T s = lorentz_pz_v(S.a[b + F_N][i], S.a[b + F_PX][i], q, ez, ezm, by, byxm, byzm, byxmzm);
+    if (p.warm_T > 0.0) s -= warm_pz(S, sp, x, z, p);
     return s;
 }
 
-// kernel_ym_precession (YM_Fluid.cu:86-108) at real cell (x, z); out[0..2] = sQ^a.
+// kernel_ym_precession (YM_Fluid.cu:86-108) from values: density, z momentum
+// and charges of the cell, Az at (x, z) and (x, z-1); out[0..2] = sQ^a.
+template <typename T>
+__device__ __forceinline__ void precession_v(T n, T pz, const T (&q)[3], const T (&az)[3], const T (&azm)[3],
+                                             T alpha, T out[3]) {
+    T n_safe = kmax(n, T(0.05));
+    T vz = pz / n_safe;
+    T a1 = T(0.5) * (az[0] + azm[0]);
+    T a2 = T(0.5) * (az[1] + azm[1]);
+    T a3 = T(0.5) * (az[2] + azm[2]);
+    out[0] = alpha * vz * (q[1] * a3 - q[2] * a2);
+    out[1] = alpha * vz * (q[2] * a1 - q[0] * a3);
+    out[2] = alpha * vz * (q[0] * a2 - q[1] * a1);
+}
+
+// kernel_ym_precession at real cell (x, z).
 template <typename T>
 __device__ __forceinline__ void precession_at(const State<T>& S, int sp, int x, int z,
                                               const KParams& p, T out[3]) {
@@ -166,16 +204,14 @@ __device__ __forceinline__ void precession_at(const State<T>& S, int sp, int x, 
     const int zm = zm_per(z, p.nz);
     const size_t i = (size_t)x + (size_t)z * nx;
     const size_t izm = (size_t)x + (size_t)zm * nx;
-    T n_safe = kmax(S.a[b + F_N][i], T(0.05));
-    T vz = S.a[b + F_PZ][i] / n_safe;
-    T a1 = T(0.5) * (S.a[S_AZ1][i] + S.a[S_AZ1][izm]);
-    T a2 = T(0.5) * (S.a[S_AZ2][i] + S.a[S_AZ2][izm]);
-    T a3 = T(0.5) * (S.a[S_AZ3][i] + S.a[S_AZ3][izm]);
-    T q1 = S.a[b + F_Q1][i], q2 = S.a[b + F_Q2][i], q3 = S.a[b + F_Q3][i];
-    T al = T(p.alpha);
-    out[0] = al * vz * (q2 * a3 - q3 * a2);
-    out[1] = al * vz * (q3 * a1 - q1 * a3);
-    out[2] = al * vz * (q1 * a2 - q2 * a1);
+    T q[3], az[3], azm[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        q[c] = S.a[b + F_Q1 + c][i];
+        az[c] = S.a[S_AZ1 + c][i];
+        azm[c] = S.a[S_AZ1 + c][izm];
+    }
+    precession_v(S.a[b + F_N][i], S.a[b + F_PZ][i], q, az, azm, T(p.alpha), out);
 }
 
 // kernel_ym_currents (YM_Physics.cu:9-60) at real cell (x, z) from fluid F.
@@ -219,8 +255,44 @@ enum EmSlot { E_EX = 0, E_EY, E_EZ, E_BX, E_BY, E_BZ, E_AX, E_AY, E_AZ };
 enum EmVar { EV_N = 0, EV_PX, EV_PY, EV_PZ, EV_COUNT };
 __host__ __device__ __forceinline__ int em_base(int sp) { return 12 + 6 * sp; }
 
-// kernel_compute_lorentz (emhdgpu2d/main_coupled.cu:112-136) at (x, z) for
-// species sp with charge q (the reference fluid has q = -1: "-1.0 * (...)").
+// Field values around a cell that the EMHD Lorentz force reads from one row:
+// (x, z) and (x-1, z).  The force also needs Ez, Bx, By, By(x-1) of row z-1.
+template <typename T>
+struct EmRow {
+    T ex, exm, ey, ez, bx, bz, bzxm, by, byxm;
+};
+
+template <typename T>
+__device__ __forceinline__ void em_load_row(const State<T>& S, size_t i, size_t ixm, EmRow<T>& r) {
+    r.ex = S.a[E_EX][i]; r.exm = S.a[E_EX][ixm];
+    r.ey = S.a[E_EY][i]; r.ez = S.a[E_EZ][i];
+    r.bx = S.a[E_BX][i];
+    r.bz = S.a[E_BZ][i]; r.bzxm = S.a[E_BZ][ixm];
+    r.by = S.a[E_BY][i]; r.byxm = S.a[E_BY][ixm];
+}
+
+// kernel_compute_lorentz (emhdgpu2d/main_coupled.cu:112-136) from values:
+// row z (c), row z-1 (m), the fluid cell (n, px, py, pz) and the charge q
+// (the reference fluid has q = -1: "-1.0 * (...)").
+template <typename T>
+__device__ __forceinline__ void em_lorentz_v(const EmRow<T>& c, const EmRow<T>& m, T n, T px, T py, T pz, T q,
+                                             T out[3]) {
+    const T Ex_node = T(0.5) * (c.ex + c.exm);
+    const T Ey_node = c.ey;
+    const T Ez_node = T(0.5) * (c.ez + m.ez);
+    const T Bx_node = T(0.5) * (c.bx + m.bx);
+    const T Bz_node = T(0.5) * (c.bz + c.bzxm);
+    const T By_node = T(0.25) * (c.by + c.byxm + m.by + m.byxm);
+    const T n_safe = kmax(n, T(0.05));
+    const T vx = px / n_safe;
+    const T vy = py / n_safe;
+    const T vz = pz / n_safe;
+    out[0] = q * (Ex_node + (vy * Bz_node - vz * By_node));
+    out[1] = q * (Ey_node + (vz * Bx_node - vx * Bz_node));
+    out[2] = q * (Ez_node + (vx * By_node - vy * Bx_node));
+}
+
+// kernel_compute_lorentz at (x, z) for species sp.
 template <typename T>
 __device__ __forceinline__ void em_lorentz(const State<T>& S, int sp, int x, int z, const KParams& p,
                                            T out[3]) {
@@ -230,21 +302,12 @@ __device__ __forceinline__ void em_lorentz(const State<T>& S, int sp, int x, int
     const size_t ixm = (size_t)xm + (size_t)z * nx;
     const size_t izm = (size_t)x + (size_t)zm * nx;
     const size_t ixmzm = (size_t)xm + (size_t)zm * nx;
-    const T Ex_node = T(0.5) * (S.a[E_EX][i] + S.a[E_EX][ixm]);
-    const T Ey_node = S.a[E_EY][i];
-    const T Ez_node = T(0.5) * (S.a[E_EZ][i] + S.a[E_EZ][izm]);
-    const T Bx_node = T(0.5) * (S.a[E_BX][i] + S.a[E_BX][izm]);
-    const T Bz_node = T(0.5) * (S.a[E_BZ][i] + S.a[E_BZ][ixm]);
-    const T By_node = T(0.25) * (S.a[E_BY][i] + S.a[E_BY][ixm] + S.a[E_BY][izm] + S.a[E_BY][ixmzm]);
+    EmRow<T> c, m;
+    em_load_row(S, i, ixm, c);
+    em_load_row(S, izm, ixmzm, m);
     const int b = em_base(sp);
-    const T n_safe = kmax(S.a[b + EV_N][i], T(0.05));
-    const T vx = S.a[b + EV_PX][i] / n_safe;
-    const T vy = S.a[b + EV_PY][i] / n_safe;
-    const T vz = S.a[b + EV_PZ][i] / n_safe;
-    const T q = T(sp ? p.q1 : p.q0);
-    out[0] = q * (Ex_node + (vy * Bz_node - vz * By_node));
-    out[1] = q * (Ey_node + (vz * Bx_node - vx * Bz_node));
-    out[2] = q * (Ez_node + (vx * By_node - vy * Bx_node));
+    em_lorentz_v(c, m, S.a[b + EV_N][i], S.a[b + EV_PX][i], S.a[b + EV_PY][i], S.a[b + EV_PZ][i],
+                 T(sp ? p.q1 : p.q0), out);
 }
 
 // --------------------------------------------------------------------------
@@ -272,6 +335,49 @@ struct YMModel {                    // ymgpu2d: n, px, pz, Q1..3 per beam
         precession_at(S, sp, x, z, p, q);
         s[F_Q1] = q[0]; s[F_Q2] = q[1]; s[F_Q3] = q[2];
     }
+    // Row-marching form of the z sweep's inputs: the field values of one row
+    // that the sources read at (x, z) / (x-1, z); row z-1 comes from the
+    // previous step of the march.
+    template <typename T>
+    struct Row {
+        T ez[3], by[3], byxm[3], az[3];
+    };
+    template <typename T>
+    __device__ static void load_row(const State<T>& S, size_t i, size_t ixm, Row<T>& r) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            r.ez[c] = S.a[S_EZ1 + c][i];
+            r.by[c] = S.a[S_BY1 + c][i];
+            r.byxm[c] = S.a[S_BY1 + c][ixm];
+            r.az[c] = S.a[S_AZ1 + c][i];
+        }
+    }
+    template <typename T>
+    __device__ static void sel_row(bool take, const Row<T>& a, Row<T>& r) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            r.ez[c] = take ? a.ez[c] : r.ez[c];
+            r.by[c] = take ? a.by[c] : r.by[c];
+            r.byxm[c] = take ? a.byxm[c] : r.byxm[c];
+            r.az[c] = take ? a.az[c] : r.az[c];
+        }
+    }
+    // u (stale z velocity) and the z sources from the cell's fluid F (the
+    // species' NV state values) and rows z (c) and z-1 (m).
+    template <typename T>
+    __device__ static void z_src_rows(const T (&F)[NV], const Row<T>& c, const Row<T>& m, const State<T>& S,
+                                      int sp, int x, int z, const KParams& p, T& u, T (&s)[NV]) {
+        const T n_safe = kmax(F[F_N], T(0.05));
+        u = F[F_PZ] / n_safe;
+        const T q[3] = {F[F_Q1], F[F_Q2], F[F_Q3]};
+        s[F_N] = T(0);
+        s[F_PX] = T(0);
+        s[F_PZ] = lorentz_pz_v(F[F_N], F[F_PX], q, c.ez, m.ez, c.by, c.byxm, m.by, m.byxm);
+        if (p.warm_T > 0.0) s[F_PZ] -= warm_pz(S, sp, x, z, p);
+        T o[3];
+        precession_v(F[F_N], F[F_PZ], q, c.az, m.az, T(p.alpha), o);
+        s[F_Q1] = o[0]; s[F_Q2] = o[1]; s[F_Q3] = o[2];
+    }
 };
 
 struct EMModel {                    // emhdgpu2d: n, px, py, pz; sources in both sweeps
@@ -288,6 +394,29 @@ struct EMModel {                    // emhdgpu2d: n, px, py, pz; sources in both
     __device__ static void z_src(const State<T>& S, int sp, int x, int z, const KParams& p, T (&s)[NV]) {
         T f[3];
         em_lorentz(S, sp, x, z, p, f);
+        s[EV_N] = T(0); s[EV_PX] = f[0]; s[EV_PY] = f[1]; s[EV_PZ] = f[2];
+    }
+    template <typename T>
+    using Row = EmRow<T>;
+    template <typename T>
+    __device__ static void load_row(const State<T>& S, size_t i, size_t ixm, Row<T>& r) {
+        em_load_row(S, i, ixm, r);
+    }
+    template <typename T>
+    __device__ static void sel_row(bool take, const Row<T>& a, Row<T>& r) {
+        r.ex = take ? a.ex : r.ex; r.exm = take ? a.exm : r.exm;
+        r.ey = take ? a.ey : r.ey; r.ez = take ? a.ez : r.ez;
+        r.bx = take ? a.bx : r.bx;
+        r.bz = take ? a.bz : r.bz; r.bzxm = take ? a.bzxm : r.bzxm;
+        r.by = take ? a.by : r.by; r.byxm = take ? a.byxm : r.byxm;
+    }
+    template <typename T>
+    __device__ static void z_src_rows(const T (&F)[NV], const Row<T>& c, const Row<T>& m, const State<T>&, int sp,
+                                      int, int, const KParams& p, T& u, T (&s)[NV]) {
+        const T n_safe = kmax(F[EV_N], T(0.05));
+        u = F[EV_PZ] / n_safe;
+        T f[3];
+        em_lorentz_v(c, m, F[EV_N], F[EV_PX], F[EV_PY], F[EV_PZ], T(sp ? p.q1 : p.q0), f);
         s[EV_N] = T(0); s[EV_PX] = f[0]; s[EV_PY] = f[1]; s[EV_PZ] = f[2];
     }
 };

@@ -7,9 +7,10 @@ namespace khs {
 
 // step kernels (khs_step_kernels.hip)
 // Tile variants (bit-identical results): x segment 64/128/256 cells
-// (variant 0/1/2); z chunk 16/32/64 rows (variant 0/1/2).
+// (variant 0/1/2); z chunk 16/32/64 rows (variant % 3) without / with
+// next-row prefetch (variant / 3).
 constexpr int kNumXVariants = 3;
-constexpr int kNumZVariants = 3;
+constexpr int kNumZVariants = 6;
 // M: YMModel (ymgpu2d two-beam SU(2) fluid) or EMModel (emhdgpu2d Abelian fluid)
 template <typename T, typename M>
 void launch_fct_x(const State<T>& S, const Stage<T>& out, const KParams& p, int sp0, int nsp,

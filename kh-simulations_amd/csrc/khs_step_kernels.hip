@@ -225,10 +225,17 @@ __device__ __forceinline__ void store_fluid(const State<T>& O, int b, size_t io,
 constexpr int ZB_X = 64;           // columns per workgroup (one wave wide)
 constexpr int ZB_C = 4;            // z chunks per workgroup (one per wave)
 
-template <typename T, int TZ, typename M>
+// One lane marches one column of one species down a chunk of TZ rows (plus
+// the 3-row FCT halo on each side).  The field values that the z sources
+// read at row z-1 are carried over from the previous row in registers, so
+// every input array is loaded once per row.  PF = 1 issues the loads of row
+// e+1 before the arithmetic of row e (software pipelining); PF = 0 loads each
+// row when it is used (fewer registers).
+template <typename T, int TZ, int PF, typename M>
 __global__ __launch_bounds__(ZB_X * ZB_C) void k_fct_z(State<T> S, Stage<T> X, State<T> O,
                                                        KParams p, int sp0, int nsp, int ntx, int ncg) {
     constexpr int NV = M::NV;
+    typedef typename M::template Row<T> Row;
     // logical tile: species fastest (both species read the same fields for
     // their sources), then 64-column groups, then groups of ZB_C z chunks
     int lt = xcd_logical(blockIdx.x, gridDim.x);
@@ -241,24 +248,50 @@ __global__ __launch_bounds__(ZB_X * ZB_C) void k_fct_z(State<T> S, Stage<T> X, S
     if (x >= p.nx || z0 >= p.nz) return;
     const int nx = p.nx, nz = p.nz;
     const int b = M::base(sp);
+    const int xm = xm_src(x, p);
     const T dt = T(p.dt), dtdz = T(p.dtdz);
     const int zend = min(z0 + TZ, nz);
+    const int elast = zend + 2;
     const bool wall = !p.periodic_x && (x == 0 || x == nx - 1);
+
+    auto load = [&](int r, Row& row, T (&F)[NV], T (&V)[NV]) {
+        const size_t i = (size_t)x + (size_t)r * nx;
+        M::load_row(S, i, (size_t)xm + (size_t)r * nx, row);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) F[v] = S.a[b + v][i];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) V[v] = X.a[sp * NV + v][i];
+    };
+
+    // rows outside [0, nz) are the clamped end rows (FCT_Sweeps.cu:52,73,78)
+    int rc = clampi(z0 - 3, 0, nz - 1);
+    Row prev, cur;
+    T F[NV], V[NV];
+    {
+        T Fd[NV], Vd[NV];
+        load(zm_per(rc, nz), prev, Fd, Vd);
+    }
+    load(rc, cur, F, V);
     ZPipe<T, NV> zp;
     zp.Um = T(0);
     zp.MUm = T(0);
-    for (int e = z0 - 3; e <= zend + 2; ++e) {
-        const int rc = clampi(e, 0, nz - 1);
-        const size_t i = (size_t)x + (size_t)rc * nx;
-        T V[NV], Sv[NV], out[NV];
-#pragma unroll
-        for (int v = 0; v < NV; ++v) V[v] = X.a[sp * NV + v][i];
-        // stale z velocity (set before the x sweep) and the step-start sources
-        const T n_safe = kmax(S.a[b][i], T(0.05));
-        const T u = S.a[b + M::IPZ][i] / n_safe;
-        M::z_src(S, sp, x, rc, p, Sv);
+    for (int e = z0 - 3; e <= elast; ++e) {
+        const int rn = clampi(min(e + 1, elast), 0, nz - 1);
+        Row nrow;
+        T nF[NV], nV[NV];
+        if (PF) load(rn, nrow, nF, nV);
+        T u, Sv[NV], out[NV];
+        M::z_src_rows(F, cur, prev, S, sp, x, rc, p, u, Sv);
         if (zp.push(e, z0, V, u, Sv, dt, dtdz, out))
             store_fluid<T, NV, M::IPX>(O, b, (size_t)x + (size_t)(e - 3) * nx, wall, out);
+        if (!PF) load(rn, nrow, nF, nV);
+        // advance: row z-1 of the next row is this row unless the clamp holds
+        // the row index at an end
+        M::sel_row(rn != rc, cur, prev);
+        cur = nrow;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { F[v] = nF[v]; V[v] = nV[v]; }
+        rc = rn;
     }
 }
 
@@ -620,25 +653,29 @@ void launch_fct_x(const State<T>& S, const Stage<T>& out, const KParams& p, int 
     }
 }
 
-template <typename T, int TZ, typename M>
+template <typename T, int TZ, int PF, typename M>
 static void launch_fct_z_t(const State<T>& S, const Stage<T>& X, const State<T>& O, const KParams& p,
                            int sp0, int nsp, hipStream_t st) {
     const int nchunk = (p.nz + TZ - 1) / TZ;
     const int ncg = (nchunk + ZB_C - 1) / ZB_C;
     const int ntx = (p.nx + ZB_X - 1) / ZB_X;
     dim3 grid(round8(nsp * ntx * ncg));
-    hipLaunchKernelGGL((k_fct_z<T, TZ, M>), grid, dim3(ZB_X * ZB_C), 0, st, S, X, O, p, sp0, nsp, ntx, ncg);
+    hipLaunchKernelGGL((k_fct_z<T, TZ, PF, M>), grid, dim3(ZB_X * ZB_C), 0, st, S, X, O, p, sp0, nsp, ntx, ncg);
 }
 
-// z chunk height: 16 / 32 / 64 rows per wave (halo rows re-read per chunk
-// vs. number of independent column marches in flight)
+// Variant v: chunk height 16 / 32 / 64 rows (v % 3; halo rows re-read per
+// chunk vs. independent column marches in flight), prefetch of the next row
+// off / on (v / 3).
 template <typename T, typename M>
 void launch_fct_z(const State<T>& S, const Stage<T>& X, const State<T>& O, const KParams& p,
                   int sp0, int nsp, int variant, hipStream_t st) {
     switch (variant) {
-        case 0: launch_fct_z_t<T, 16, M>(S, X, O, p, sp0, nsp, st); break;
-        case 2: launch_fct_z_t<T, 64, M>(S, X, O, p, sp0, nsp, st); break;
-        default: launch_fct_z_t<T, 32, M>(S, X, O, p, sp0, nsp, st); break;
+        case 0: launch_fct_z_t<T, 16, 0, M>(S, X, O, p, sp0, nsp, st); break;
+        case 1: launch_fct_z_t<T, 32, 0, M>(S, X, O, p, sp0, nsp, st); break;
+        case 2: launch_fct_z_t<T, 64, 0, M>(S, X, O, p, sp0, nsp, st); break;
+        case 3: launch_fct_z_t<T, 16, 1, M>(S, X, O, p, sp0, nsp, st); break;
+        case 5: launch_fct_z_t<T, 64, 1, M>(S, X, O, p, sp0, nsp, st); break;
+        default: launch_fct_z_t<T, 32, 1, M>(S, X, O, p, sp0, nsp, st); break;
     }
 }
 

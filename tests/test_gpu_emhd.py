@@ -122,6 +122,19 @@ def test_step_bitwise(case):
         assert_bitwise(sim.get(n), o.get(n), f"{case} {n}")
 
 
+@pytest.mark.parametrize("zc,pf", [(16, 1), (64, 0), (32, 1)])
+def test_tile_variants_bitwise(zc, pf):
+    nx, nz, periodic, nsp = 130, 70, 0, 2
+    keys = em_keys(nx, nz, periodic, nsp, z_chunk=zc, z_prefetch=pf, x_segment=64, autotune=0)
+    cfg, sim, o = em_pair(em_random(nx, nz, nsp, seed=4, smooth=True), **keys)
+    assert sim.tuning() == (64, zc, pf)
+    o.lorentz()
+    sim.step(5)
+    o.step(5)
+    for n in orc.em_state_names(nsp):
+        assert_bitwise(sim.get(n), o.get(n), f"z{zc} pf{pf} {n}")
+
+
 def test_daskaw_steps_bitwise():
     """The reference run itself (256x256, dx = 1, dt = 0.002), 20 steps from init."""
     cfg = khs.Config(model="emhd", k_mode=1, n_steps=20)
