@@ -208,7 +208,7 @@ def main():
                               alg_GBs=bytes_alg / avg_s / 1e9)
     dom = max(kern, key=lambda k: kern[k]["avg_us"])
     d = kern[dom]
-    tile = {"k_fct_x": r64["tuning"][0], "k_fct_z": r64["tuning"][1]}.get(dom)
+    tile = {"k_fct_x": r64["tuning"]["x_segment"], "k_fct_z": r64["tuning"]["z_chunk"]}.get(dom)
     traffic = PMC_TRAFFIC_MB.get((dom, tile))
     traffic = traffic * 1e6 if (traffic and r64["nx"] == 2048 and r64["nz"] == 1024) else None
     step_alg_GBs = STEP_ARRAYS * 8 * r64["cells"] / (ms_step * 1e-3) / 1e9
@@ -229,8 +229,7 @@ def main():
         "config": {"workload": "plain FP64 single run, 2048x1024, x walled / z periodic, dx=dz=0.1, "
                                "V0=0.05, a=1.0, seed 1, CFL 0.4, no mode filter, xi_cut=12.8",
                    "segments": r64["segments"],
-                   "tiles": {"x_segment": r64["tuning"][0], "z_chunk": r64["tuning"][1],
-                             "z_prefetch": r64["tuning"][2]},
+                   "tiles": r64["tuning"],
                    "nx": r64["nx"], "nz": r64["nz"], "precision": "fp64",
                    "sweep": "fused x+z" if fused else "split x / z"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": d["alg_GBs"], "peak": HBM_PEAK_GBS,

@@ -126,6 +126,7 @@ typedef struct khs_config {
     double   spectrum_amp;
     uint64_t spectrum_seed;
     double   em_perturb_amp;   /* EMHD das_kaw perturbation (argv[2] as double, default 0.05); set by perturb_amp */
+    int      x_rows;           /* x sweep rows per workgroup: 0 = autotuned (default), 1, 4 or 16 */
     int      z_prefetch;       /* z sweep: -1 = autotuned (default), 0 = load each row when used, 1 = prefetch the next row */
     int      mfma_filter;      /* keep-|kz|<=K filter as MFMA contractions (1, default) or the per-column FP64-sum kernels (0) */
     /* --- derived (khs_derive) --- */
@@ -249,10 +250,10 @@ int khs_timing_enable(khs_ctx* ctx, int on);
 int khs_timing_reset(khs_ctx* ctx);
 int khs_kernel_times(khs_ctx* ctx, double* ms, int* launches, int nslots);
 const char* khs_kernel_name(int slot);
-/* Tile shapes in use: x-sweep segment width (cells), z-sweep chunk height
- * (rows) and whether the z sweep prefetches the next row (0/1); chosen by the
- * autotuner on the first step unless autotune = 0. */
-int khs_get_tuning(khs_ctx* ctx, int* x_segment, int* z_chunk, int* z_prefetch);
+/* Tile shapes in use: out = {x-sweep segment width (cells), x-sweep rows per
+ * workgroup, z-sweep chunk height (rows), z-sweep next-row prefetch (0/1)};
+ * chosen by the autotuner on the first step unless autotune = 0. */
+int khs_get_tuning(khs_ctx* ctx, int out[4]);
 
 #ifdef __cplusplus
 }

@@ -73,6 +73,7 @@ void config_defaults(khs_config& c) {
     c.em_perturb_amp = 0.05;
     c.mfma_filter = 1;
     c.z_prefetch = -1;
+    c.x_rows = 0;
 }
 
 static int parse_int(const std::string& k, const std::string& v) {
@@ -154,6 +155,7 @@ bool config_set(khs_config& c, const std::string& key, const std::string& val) {
         else if (val == "spectrum" || val == "1") c.emhd_init = 1;
         else throw Error(KHS_ERR_CONFIG, "emhd_init must be daskaw or spectrum");
     }
+    else if (key == "x_rows")            c.x_rows = parse_int(key, val);
     else if (key == "z_prefetch")        c.z_prefetch = parse_int(key, val);
     else if (key == "mfma_filter")       c.mfma_filter = parse_int(key, val);
     else if (key == "smooth_init")       c.smooth_init = parse_int(key, val);
@@ -222,6 +224,8 @@ static void config_derive_emhd(khs_config& c) {
     if (c.z_chunk != 0 && c.z_chunk != 16 && c.z_chunk != 32 && c.z_chunk != 64)
         throw Error(KHS_ERR_CONFIG, "z_chunk must be 0, 16, 32 or 64");
     if (c.z_prefetch < -1 || c.z_prefetch > 1) throw Error(KHS_ERR_CONFIG, "z_prefetch must be -1, 0 or 1");
+    if (c.x_rows != 0 && c.x_rows != 1 && c.x_rows != 4 && c.x_rows != 16)
+        throw Error(KHS_ERR_CONFIG, "x_rows must be 0, 1, 4 or 16");
     if (c.emhd_init == 1 && (c.spectrum_kmax < 1 || 2 * c.spectrum_kmax >= std::min(c.nx, c.nz)))
         throw Error(KHS_ERR_CONFIG, "spectrum_kmax must satisfy 1 <= K < min(nx, nz)/2");
 }
@@ -278,6 +282,8 @@ void config_derive(khs_config& c) {
     if (c.z_chunk != 0 && c.z_chunk != 16 && c.z_chunk != 32 && c.z_chunk != 64)
         throw Error(KHS_ERR_CONFIG, "z_chunk must be 0, 16, 32 or 64");
     if (c.z_prefetch < -1 || c.z_prefetch > 1) throw Error(KHS_ERR_CONFIG, "z_prefetch must be -1, 0 or 1");
+    if (c.x_rows != 0 && c.x_rows != 1 && c.x_rows != 4 && c.x_rows != 16)
+        throw Error(KHS_ERR_CONFIG, "x_rows must be 0, 1, 4 or 16");
     if (c.run_mode < 0 || c.run_mode > 6)
         throw Error(KHS_ERR_CONFIG, "run_mode must be 0..6");
 }

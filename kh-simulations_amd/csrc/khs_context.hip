@@ -209,13 +209,14 @@ struct Ctx {
         const int fz = cfg.z_chunk == 16 ? 0 : cfg.z_chunk == 32 ? 1 : cfg.z_chunk == 64 ? 2 : -1;
         const int fp = cfg.z_prefetch;               // -1 = tuned
         std::vector<int> xc, zc;
+        const int fr = cfg.x_rows == 1 ? 0 : cfg.x_rows == 4 ? 1 : cfg.x_rows == 16 ? 2 : -1;
         for (int v = 0; v < kNumXVariants; ++v)
-            if (fx < 0 || v == fx) xc.push_back(v);
+            if ((fx < 0 || v % 3 == fx) && (fr < 0 || v / 3 == fr)) xc.push_back(v);
         for (int v = 0; v < kNumZVariants; ++v)
             if ((fz < 0 || v % 3 == fz) && (fp < 0 || v / 3 == fp)) zc.push_back(v);
         // without tuning: the forced shape, else 256-cell segments and 32-row
         // chunks with prefetch in FP32 (registers to spare), without in FP64
-        xvar = (fx >= 0) ? fx : 2;
+        xvar = ((fx >= 0) ? fx : 2) + 3 * ((fr >= 0) ? fr : 1);
         zvar = ((fz >= 0) ? fz : 1) + 3 * ((fp >= 0) ? fp : (fp32 ? 1 : 0));
         if (!cfg.autotune) return;
         if (xc.size() > 1) xvar = pick<T>(xc, [&](int v) { fct_x<T>(0, nsp, v); });
@@ -408,6 +409,7 @@ static std::unique_ptr<Ctx> make_ctx(const khs_config& in) {
     }
     ctx->cell_filters = (c.xi_sponge > 0.0f) || (c.xi_cut > 0.0f);
     if (c.x_segment) ctx->xvar = c.x_segment == 64 ? 0 : c.x_segment == 128 ? 1 : 2;
+    ctx->xvar += 3 * (c.x_rows == 1 ? 0 : c.x_rows == 16 ? 2 : 1);
     if (c.z_chunk) ctx->zvar = c.z_chunk == 16 ? 0 : c.z_chunk == 32 ? 1 : 2;
     if (c.z_prefetch == 1) ctx->zvar = ctx->zvar % 3 + 3;
 
@@ -994,13 +996,15 @@ int khs_kernel_times(khs_ctx* ctx, double* ms, int* launches, int nslots) {
     });
 }
 
-int khs_get_tuning(khs_ctx* ctx, int* x_segment, int* z_chunk, int* z_prefetch) {
+int khs_get_tuning(khs_ctx* ctx, int out[4]) {
     CTX_OR_FAIL(ctx);
+    if (!out) { g_last_error = "null argument"; return KHS_ERR_ARG; }
     Ctx& c = *ctx->impl;
-    static const int xs[3] = {64, 128, 256}, zs[3] = {16, 32, 64};
-    if (x_segment) *x_segment = xs[c.xvar];
-    if (z_chunk) *z_chunk = zs[c.zvar % 3];
-    if (z_prefetch) *z_prefetch = c.zvar / 3;
+    static const int xs[3] = {64, 128, 256}, xr[3] = {1, 4, 16}, zs[3] = {16, 32, 64};
+    out[0] = xs[c.xvar % 3];
+    out[1] = xr[c.xvar / 3];
+    out[2] = zs[c.zvar % 3];
+    out[3] = c.zvar / 3;
     return KHS_OK;
 }
 

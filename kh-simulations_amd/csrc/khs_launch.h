@@ -7,9 +7,9 @@ namespace khs {
 
 // step kernels (khs_step_kernels.hip)
 // Tile variants (bit-identical results): x segment 64/128/256 cells
-// (variant 0/1/2); z chunk 16/32/64 rows (variant % 3) without / with
-// next-row prefetch (variant / 3).
-constexpr int kNumXVariants = 3;
+// (variant % 3) with 1/4/16 rows per workgroup (variant / 3); z chunk
+// 16/32/64 rows (variant % 3) without / with next-row prefetch (variant / 3).
+constexpr int kNumXVariants = 9;
 constexpr int kNumZVariants = 6;
 // M: YMModel (ymgpu2d two-beam SU(2) fluid) or EMModel (emhdgpu2d Abelian fluid)
 template <typename T, typename M>

@@ -33,7 +33,7 @@ namespace khs {
 // M is the model policy (YMModel / EMModel): variables, slots, sources.
 template <typename T, int XB, typename M>
 __global__ __launch_bounds__(XB) void k_fct_x(State<T> S, Stage<T> out, KParams p, int sp0, int nsp,
-                                              int ntx) {
+                                              int ntx, int rows) {
     constexpr int XW = XB + 6;
     constexpr int NV = M::NV;
     __shared__ T sU[XW];
@@ -44,18 +44,22 @@ __global__ __launch_bounds__(XB) void k_fct_x(State<T> S, Stage<T> out, KParams 
     __shared__ T sLNR[NV][XW];
 
     const int tid = threadIdx.x;
-    // logical tile: species fastest, then x segments, then rows (row z-1,
-    // whose fields the x source reads, was staged ntx*nsp tiles earlier)
+    // logical tile: species fastest, then x segments, then groups of `rows`
+    // rows (one block sweeps its rows in turn: fewer, longer-lived
+    // workgroups; row z-1, whose fields the x source reads, is in cache)
+    const int nrg = (p.nz + rows - 1) / rows;
     int lt = xcd_logical(blockIdx.x, gridDim.x);
-    if (lt >= nsp * ntx * p.nz) return;
+    if (lt >= nsp * ntx * nrg) return;
     const int sp = sp0 + lt % nsp;
     lt /= nsp;
     const int x0 = (lt % ntx) * XB;
-    const int z = lt / ntx;
+    const int zb = (lt / ntx) * rows;
+    const int ze = min(zb + rows, p.nz);
     const int b = M::base(sp);
-    const size_t row = (size_t)z * p.nx;
     const T dt = T(p.dt);
     const T dtdx = T(p.dtdx);
+    for (int z = zb; z < ze; ++z) {
+    const size_t row = (size_t)z * p.nx;
 
     // Phase A: stage variables, velocity and x sources for positions
     // x0-3 .. x0+XB+2 (position q <-> extended x = x0-3+q).
@@ -130,6 +134,8 @@ __global__ __launch_bounds__(XB) void k_fct_x(State<T> S, Stage<T> out, KParams 
             T r = sLNR[v][q] - (sV[v][q] - sV[v][q - 1]);
             out.a[sp * NV + v][row + x] = r;
         }
+    }
+    __syncthreads();                  // LDS is restaged for the next row
     }
 }
 
@@ -637,19 +643,24 @@ __global__ __launch_bounds__(MX * MZT) void k_emhd_maxwell(State<T> S, State<T> 
 // ==========================================================================
 template <typename T, int XB, typename M>
 static void launch_fct_x_t(const State<T>& S, const Stage<T>& out, const KParams& p, int sp0, int nsp,
-                           hipStream_t st) {
+                           int rows, hipStream_t st) {
     const int ntx = (p.nx + XB - 1) / XB;
-    dim3 grid(round8(ntx * p.nz * nsp));
-    hipLaunchKernelGGL((k_fct_x<T, XB, M>), grid, dim3(XB), 0, st, S, out, p, sp0, nsp, ntx);
+    const int nrg = (p.nz + rows - 1) / rows;
+    dim3 grid(round8(ntx * nrg * nsp));
+    hipLaunchKernelGGL((k_fct_x<T, XB, M>), grid, dim3(XB), 0, st, S, out, p, sp0, nsp, ntx, rows);
 }
 
+// Variant v: segment width 64 / 128 / 256 cells (v % 3), rows per workgroup
+// 1 / 4 / 16 (v / 3).
 template <typename T, typename M>
 void launch_fct_x(const State<T>& S, const Stage<T>& out, const KParams& p, int sp0, int nsp,
                   int variant, hipStream_t st) {
-    switch (variant) {
-        case 0: launch_fct_x_t<T, 64, M>(S, out, p, sp0, nsp, st); break;
-        case 1: launch_fct_x_t<T, 128, M>(S, out, p, sp0, nsp, st); break;
-        default: launch_fct_x_t<T, 256, M>(S, out, p, sp0, nsp, st); break;
+    static const int kRows[3] = {1, 4, 16};
+    const int rows = kRows[(variant / 3) % 3];
+    switch (variant % 3) {
+        case 0: launch_fct_x_t<T, 64, M>(S, out, p, sp0, nsp, rows, st); break;
+        case 1: launch_fct_x_t<T, 128, M>(S, out, p, sp0, nsp, rows, st); break;
+        default: launch_fct_x_t<T, 256, M>(S, out, p, sp0, nsp, rows, st); break;
     }
 }
 

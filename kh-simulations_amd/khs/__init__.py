@@ -75,7 +75,7 @@ class khs_config(ctypes.Structure):
         ("charge1", ctypes.c_double), ("charge2", ctypes.c_double), ("emhd_init", ctypes.c_int),
         ("smooth_init", ctypes.c_int), ("spectrum_kmax", ctypes.c_int), ("spectrum_amp", ctypes.c_double),
         ("spectrum_seed", ctypes.c_uint64), ("em_perturb_amp", ctypes.c_double),
-        ("z_prefetch", ctypes.c_int), ("mfma_filter", ctypes.c_int),
+        ("x_rows", ctypes.c_int), ("z_prefetch", ctypes.c_int), ("mfma_filter", ctypes.c_int),
         ("nx", ctypes.c_int), ("nz", ctypes.c_int), ("periodic_x", ctypes.c_int), ("freeze_az1", ctypes.c_int),
         ("lx", ctypes.c_double), ("lz", ctypes.c_double), ("dx", ctypes.c_double), ("dz", ctypes.c_double),
         ("courant", ctypes.c_double), ("dt", ctypes.c_double),
@@ -139,7 +139,7 @@ def _signatures():
         "khs_timing_reset": (c_int, [c_void_p]),
         "khs_kernel_times": (c_int, [c_void_p, dp, P(c_int), c_int]),
         "khs_kernel_name": (c_char_p, [c_int]),
-        "khs_get_tuning": (c_int, [c_void_p, P(c_int), P(c_int), P(c_int)]),
+        "khs_get_tuning": (c_int, [c_void_p, P(c_int)]),
         "khs_mode_amplitude": (c_int, [c_void_p, c_int, dp]),
         "khs_fit_growth_max_r2": (c_int, [dp, dp, c_int, c_double, c_double, c_double, dp]),
         "khs_fit_growth_plateau": (c_int, [dp, dp, c_int, c_double, c_double, c_double, dp]),
@@ -380,10 +380,10 @@ class Simulation:
         _check(_lib.khs_timing_reset(self._h))
 
     def tuning(self):
-        """(x-sweep segment width, z-sweep chunk height, z-sweep prefetch) in use."""
-        xs, zc, pf = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        _check(_lib.khs_get_tuning(self._h, ctypes.byref(xs), ctypes.byref(zc), ctypes.byref(pf)))
-        return xs.value, zc.value, pf.value
+        """Tile shapes in use: {x_segment, x_rows, z_chunk, z_prefetch}."""
+        out = (ctypes.c_int * 4)()
+        _check(_lib.khs_get_tuning(self._h, out))
+        return dict(x_segment=out[0], x_rows=out[1], z_chunk=out[2], z_prefetch=out[3])
 
     def kernel_times(self):
         n = 8

@@ -183,19 +183,19 @@ def test_full_size_step_bitwise():
         assert_bitwise(sim.get(n), o.get(n), f"2048x1024 {n}")
 
 
-@pytest.mark.parametrize("xs,zc,pf", [(64, 16, 0), (128, 64, 1), (256, 32, 0), (64, 32, 1), (128, 16, 1),
-                                      (256, 64, 0)])
-def test_tile_variants_bitwise(xs, zc, pf):
-    """Every x-segment / z-chunk / z-prefetch variant the autotuner may pick gives the same bits."""
-    keys = grid(300, 136, 1, warm_T=0.01, x_segment=xs, z_chunk=zc, z_prefetch=pf, autotune=0)
+@pytest.mark.parametrize("xs,xr,zc,pf", [(64, 1, 16, 0), (128, 4, 64, 1), (256, 16, 32, 0), (64, 16, 32, 1),
+                                         (128, 1, 16, 1), (256, 4, 64, 0)])
+def test_tile_variants_bitwise(xs, xr, zc, pf):
+    """Every x-segment / x-rows / z-chunk / z-prefetch variant the autotuner may pick gives the same bits."""
+    keys = grid(300, 136, 1, warm_T=0.01, x_segment=xs, x_rows=xr, z_chunk=zc, z_prefetch=pf, autotune=0)
     st = random_state(300, 136, seed=31, smooth=True)
     cfg, sim, o = make_pair(st, **keys)
-    assert sim.tuning() == (xs, zc, pf)
+    assert sim.tuning() == dict(x_segment=xs, x_rows=xr, z_chunk=zc, z_prefetch=pf)
     o.sources()
     sim.step(3)
     o.step(3)
     for n in orc.STATE:
-        assert_bitwise(sim.get(n), o.get(n), f"x{xs} z{zc} pf{pf} {n}")
+        assert_bitwise(sim.get(n), o.get(n), f"x{xs}r{xr} z{zc} pf{pf} {n}")
 
 
 def test_determinism_and_staged_equals_fused():
