@@ -18,7 +18,8 @@ segments that restart from the initial state (untimed) so every timed step is
 physically finite.  The FP32 precision form of the same workload is timed too,
 and so is the precision-form workload (4096x2048, V0 = 0.1, a = 0.5, keep
 |kz| <= 8 on MFMA) in FP32 and FP64, with the kz = 1 growth rate of both
-(max-R^2 fit over the same window) and its relative error.
+(max-R^2 fit over the same window) and its relative error, and the batched
+sweep (64 uneven FP32 runs through captured graphs: runs/hour).
 
 Prints one JSON line (rank 0, N = 1 only; there is one GPU per box).
 """
@@ -119,6 +120,31 @@ def growth_c3(precision, n_steps=500, stride=10):
     return ts
 
 
+def sweep(n=64, steps=3000, lanes=8, segment=50):
+    """Batched-sweep workload: n uneven-grid FP32 runs (khs.sweep), stepped
+    through captured graphs, then the same runs one after another without
+    graphs.  A run ends at `steps` or when it halts (E/E0 rule)."""
+    import khs
+    from khs.sweep import sweep_configs
+
+    cfgs = sweep_configs(n, steps)
+    cells = [c["nx_override"] * c["nz_override"] for c in cfgs]
+    out = {"workload": f"{n} runs, NX in 256..2048 x NZ in 128..1024, V0~U[0.01,0.2], a~U[0.5,2], kz 1..3, "
+                       f"half with suppress_kz0, FP32, {steps} steps or halt, xi_cut=12.8",
+           "lanes": lanes, "segment": segment}
+    for mode in ("graph", "eager"):
+        b = khs.Batch(cfgs, lanes=lanes, segment=segment)
+        b.init()
+        t0 = time.perf_counter()
+        done, halted = (b.run if mode == "graph" else b.run_eager)(steps)
+        el = time.perf_counter() - t0
+        b.close()
+        out[mode] = {"seconds": el, "runs_per_hour": n / el * 3600.0, "halted": int(sum(halted)),
+                     "cell_updates_per_s": sum(c * d for c, d in zip(cells, done)) / el}
+    out["graph_over_eager"] = out["eager"]["seconds"] / out["graph"]["seconds"]
+    return out
+
+
 def cpu_baseline(budget_s=15.0):
     """FP64 OpenMP oracle on the host cores, same workload, bounded sample."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
@@ -174,6 +200,7 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-c3", action="store_true", help="skip the 4096x2048 precision-form leg")
     ap.add_argument("--c3-steps", type=int, default=200)
+    ap.add_argument("--no-sweep", action="store_true", help="skip the batched-sweep (runs/hour) leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB")
     ap.add_argument("--split", action="store_true", help="separate x and z sweep kernels")
@@ -267,6 +294,8 @@ def main():
             "gamma_fp64": g64w, "gamma_fp32": g32, "gamma_rel_err": abs(g32 - g64w) / g64w,
             "fit_window_t": [t0, t1], "steps": a.c3_steps, "segments_of": SEGMENT_C3,
             "kernels_avg_us": {p: c3[p]["kernels_avg_us"] for p in c3}}
+    if not a.no_sweep:
+        out["sweep"] = sweep()
     if not a.no_cpu:
         cb = cpu_baseline(a.cpu_seconds)
         out["cpu_baseline"] = cb

@@ -255,6 +255,27 @@ const char* khs_kernel_name(int slot);
  * chosen by the autotuner on the first step unless autotune = 0. */
 int khs_get_tuning(khs_ctx* ctx, int out[4]);
 
+/* ---------------- batches of runs (parameter sweeps) ---------------- */
+/* No counterpart in the reference, which runs one configuration per process
+ * (scripts/run_multigpu_abi_template.sh).  A batch owns one context per config;
+ * khs_batch_run advances every active run by nsteps: `segment` (even) steps of
+ * all runs plus their energy reductions form one captured HIP graph, the runs
+ * spread over `lanes` streams so that small grids execute concurrently.  The
+ * halting rule (NaN/Inf, E/E0 > threshold) is applied after every segment
+ * rather than every energy_stride steps.  steps_done[i] / halted[i] (n
+ * entries each, may be NULL) report each run's step count and halt flag. */
+typedef struct khs_batch khs_batch;
+int khs_batch_create(const khs_config* cfgs, int n, int lanes, int segment, khs_batch** out);
+int khs_batch_destroy(khs_batch* b);
+int khs_batch_init(khs_batch* b);
+int khs_batch_run(khs_batch* b, int nsteps, int* steps_done, int* halted);
+/* The same stepping without graphs: each run in turn through khs_step
+ * (halting every energy_stride steps), for comparison. */
+int khs_batch_run_eager(khs_batch* b, int nsteps, int* steps_done, int* halted);
+int khs_batch_get_array(khs_batch* b, int run, int id, double* host, size_t n);
+int khs_batch_set_array(khs_batch* b, int run, int id, const double* host, size_t n);
+int khs_batch_energy(khs_batch* b, int run, double* E, double* E0);
+
 #ifdef __cplusplus
 }
 #endif

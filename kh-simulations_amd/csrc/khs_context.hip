@@ -291,7 +291,9 @@ struct Ctx {
     void maxwell_stage() { fp32 ? do_maxwell<float>() : do_maxwell<double>(); }
     void filter_stage() { fp32 ? do_filters<float>() : do_filters<double>(); }
 
-    double energy() {
+    // Energy reduction and its copy into the pinned h_red[0], without waiting
+    // (also used inside a captured graph).
+    void energy_async() {
         timed(K_ENERGY, [&] {
             if (emhd) {
                 if (fp32) launch_energy_em<float>(state<float>(cur), kp, d_partials, d_red, stream);
@@ -303,6 +305,9 @@ struct Ctx {
         });
         check_launch();
         HIPCHK(hipMemcpyAsync(h_red, d_red, sizeof(double), hipMemcpyDeviceToHost, stream));
+    }
+    double energy() {
+        energy_async();
         HIPCHK(hipStreamSynchronize(stream));
         return h_red[0];
     }
@@ -555,6 +560,82 @@ static void write_energy_row(Ctx& c, long step, double E) {
     double t_now = step * c.cfg.dt;
     c.ecsv << step << ',' << t_now << ',' << E << ',' << ratio << '\n';
     c.ecsv.flush();
+}
+
+// --------------------------------------------------------------------------
+// Batch of independent runs (parameter sweeps).  The runs are spread over
+// `lanes` streams; K steps of every active run, followed by each run's energy
+// reduction and its copy to pinned host memory, are captured once into a HIP
+// graph and replayed, so a K-step segment of the whole sweep is one graph
+// launch.  Halting (NaN/Inf or E/E0 above the run's threshold) is checked on
+// the host after each segment; a halted run leaves the graph, which is then
+// captured again.  K is even so that every replay starts from the same
+// ping-pong buffers.
+// --------------------------------------------------------------------------
+struct Batch {
+    std::vector<std::unique_ptr<Ctx>> runs;
+    std::vector<hipStream_t> lanes;
+    hipStream_t origin = nullptr;
+    std::vector<hipEvent_t> ev;              // fork event + one join event per lane
+    int seg = 50;                             // steps per graph segment (even)
+    std::vector<int> active;                  // 1 while a run has not halted
+    hipGraphExec_t exec = nullptr;
+    int exec_k = 0;
+    std::vector<int> exec_set;                // runs captured in exec
+    ~Batch();
+    void release_graph() {
+        if (exec) hipGraphExecDestroy(exec);
+        exec = nullptr;
+        exec_k = 0;
+        exec_set.clear();
+    }
+    void capture(int K) {
+        release_graph();
+        std::vector<int> set;
+        for (int i = 0; i < (int)runs.size(); ++i)
+            if (active[i]) set.push_back(i);
+        if (set.empty()) return;
+        for (int i : set) runs[i]->autotune();          // tuning launches and waits: before capture
+        for (int i : set) HIPCHK(hipStreamSynchronize(runs[i]->stream));
+        HIPCHK(hipStreamBeginCapture(origin, hipStreamCaptureModeRelaxed));
+        HIPCHK(hipEventRecord(ev[0], origin));
+        for (auto l : lanes) HIPCHK(hipStreamWaitEvent(l, ev[0], 0));
+        for (size_t k = 0; k < set.size(); ++k) {
+            Ctx& c = *runs[set[k]];
+            hipStream_t own = c.stream;
+            c.stream = lanes[k % lanes.size()];
+            const int cur0 = c.cur;
+            const long step0 = c.step;
+            for (int s = 0; s < K; ++s) full_step(c);
+            c.energy_async();
+            c.cur = cur0;                                   // the graph does the stepping
+            c.step = step0;
+            c.stream = own;
+        }
+        for (size_t l = 0; l < lanes.size(); ++l) {
+            HIPCHK(hipEventRecord(ev[1 + l], lanes[l]));
+            HIPCHK(hipStreamWaitEvent(origin, ev[1 + l], 0));
+        }
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamEndCapture(origin, &g));
+        hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            exec = nullptr;
+            throw Error(KHS_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+        exec_k = K;
+        exec_set = set;
+    }
+};
+
+Batch::~Batch() {
+    release_graph();
+    if (origin) hipStreamSynchronize(origin);
+    for (auto e : ev) hipEventDestroy(e);
+    for (auto l : lanes) hipStreamDestroy(l);
+    if (origin) hipStreamDestroy(origin);
+    runs.clear();
 }
 
 }  // namespace khs
@@ -1011,6 +1092,165 @@ int khs_get_tuning(khs_ctx* ctx, int out[4]) {
 const char* khs_kernel_name(int slot) {
     if (slot < 0 || slot >= khs::K_NSLOTS) return nullptr;
     return khs::kSlotNames[slot];
+}
+
+}  // extern "C"
+
+// ==========================================================================
+// C ABI: batches
+// ==========================================================================
+struct khs_batch {
+    std::unique_ptr<khs::Batch> impl;
+};
+
+extern "C" {
+
+int khs_batch_create(const khs_config* cfgs, int n, int lanes, int segment, khs_batch** out) {
+    if (!cfgs || !out || n < 1 || lanes < 1 || segment < 2 || segment % 2) {
+        g_last_error = "khs_batch_create: need cfgs, n >= 1, lanes >= 1 and an even segment >= 2";
+        return KHS_ERR_ARG;
+    }
+    *out = nullptr;
+    return guard([&] {
+        auto b = std::make_unique<khs::Batch>();
+        b->seg = segment;
+        for (int i = 0; i < n; ++i) b->runs.push_back(khs::make_ctx(cfgs[i]));
+        HIPCHK(hipStreamCreateWithFlags(&b->origin, hipStreamNonBlocking));
+        for (int l = 0; l < std::min(lanes, n); ++l) {
+            hipStream_t s;
+            HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+            b->lanes.push_back(s);
+        }
+        for (size_t l = 0; l <= b->lanes.size(); ++l) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            b->ev.push_back(e);
+        }
+        b->active.assign(n, 1);
+        auto h = new khs_batch;
+        h->impl = std::move(b);
+        *out = h;
+    });
+}
+
+int khs_batch_destroy(khs_batch* b) {
+    if (!b) return KHS_OK;
+    int rc = guard([&] { b->impl.reset(); });
+    delete b;
+    return rc;
+}
+
+int khs_batch_init(khs_batch* b) {
+    if (!b || !b->impl) { g_last_error = "null batch"; return KHS_ERR_ARG; }
+    return guard([&] {
+        khs::Batch& B = *b->impl;
+        B.release_graph();
+        for (size_t i = 0; i < B.runs.size(); ++i) {
+            khs_ctx tmp;
+            tmp.impl = std::move(B.runs[i]);
+            const int rc = khs_init(&tmp, nullptr);
+            B.runs[i] = std::move(tmp.impl);
+            if (rc != KHS_OK) throw Error(rc, "run " + std::to_string(i) + ": " + g_last_error);
+            B.active[i] = 1;
+        }
+    });
+}
+
+int khs_batch_run(khs_batch* b, int nsteps, int* steps_done, int* halted) {
+    if (!b || !b->impl) { g_last_error = "null batch"; return KHS_ERR_ARG; }
+    return guard([&] {
+        khs::Batch& B = *b->impl;
+        const int n = (int)B.runs.size();
+        int left = nsteps;
+        while (left > 0) {
+            bool any = false;
+            for (int i = 0; i < n; ++i) any = any || B.active[i];
+            if (!any) break;
+            if (left == 1) {                      // odd tail: one eager step per run
+                for (int i = 0; i < n; ++i)
+                    if (B.active[i]) khs::full_step(*B.runs[i]);
+                left = 0;
+                break;
+            }
+            const int K = std::min(B.seg, left - (left % 2));
+            std::vector<int> set;
+            for (int i = 0; i < n; ++i)
+                if (B.active[i]) set.push_back(i);
+            if (!B.exec || B.exec_k != K || B.exec_set != set) B.capture(K);
+            HIPCHK(hipGraphLaunch(B.exec, B.origin));
+            HIPCHK(hipStreamSynchronize(B.origin));
+            for (int i : set) {
+                Ctx& c = *B.runs[i];
+                c.step += K;
+                const double E = c.h_red[0];
+                khs::write_energy_row(c, c.step, E);
+                const double ratio = E / c.E0;
+                if (std::isnan(E) || std::isinf(E) || ratio > c.cfg.energy_thresh) B.active[i] = 0;
+            }
+            left -= K;
+        }
+        for (int i = 0; i < n; ++i) {
+            if (steps_done) steps_done[i] = (int)B.runs[i]->step;
+            if (halted) halted[i] = B.active[i] ? 0 : 1;
+        }
+    });
+}
+
+int khs_batch_run_eager(khs_batch* b, int nsteps, int* steps_done, int* halted) {
+    if (!b || !b->impl) { g_last_error = "null batch"; return KHS_ERR_ARG; }
+    return guard([&] {
+        khs::Batch& B = *b->impl;
+        for (size_t i = 0; i < B.runs.size(); ++i) {
+            if (!B.active[i]) continue;
+            khs_ctx tmp;
+            tmp.impl = std::move(B.runs[i]);
+            int h = 0;
+            const int rc = khs_step(&tmp, nsteps, &h);
+            B.runs[i] = std::move(tmp.impl);
+            if (rc != KHS_OK) throw Error(rc, "run " + std::to_string(i) + ": " + g_last_error);
+            if (h) B.active[i] = 0;
+        }
+        for (size_t i = 0; i < B.runs.size(); ++i) {
+            if (steps_done) steps_done[i] = (int)B.runs[i]->step;
+            if (halted) halted[i] = B.active[i] ? 0 : 1;
+        }
+    });
+}
+
+int khs_batch_get_array(khs_batch* b, int run, int id, double* host, size_t n) {
+    if (!b || !b->impl || run < 0 || run >= (int)b->impl->runs.size()) {
+        g_last_error = "bad batch or run index";
+        return KHS_ERR_ARG;
+    }
+    khs_ctx tmp;
+    tmp.impl = std::move(b->impl->runs[run]);
+    const int rc = khs_get_array(&tmp, id, host, n);
+    b->impl->runs[run] = std::move(tmp.impl);
+    return rc;
+}
+
+int khs_batch_set_array(khs_batch* b, int run, int id, const double* host, size_t n) {
+    if (!b || !b->impl || run < 0 || run >= (int)b->impl->runs.size()) {
+        g_last_error = "bad batch or run index";
+        return KHS_ERR_ARG;
+    }
+    khs_ctx tmp;
+    tmp.impl = std::move(b->impl->runs[run]);
+    const int rc = khs_set_array(&tmp, id, host, n);
+    b->impl->runs[run] = std::move(tmp.impl);
+    return rc;
+}
+
+int khs_batch_energy(khs_batch* b, int run, double* E, double* E0) {
+    if (!b || !b->impl || run < 0 || run >= (int)b->impl->runs.size()) {
+        g_last_error = "bad batch or run index";
+        return KHS_ERR_ARG;
+    }
+    return guard([&] {
+        Ctx& c = *b->impl->runs[run];
+        if (E) *E = c.energy();
+        if (E0) *E0 = c.E0;
+    });
 }
 
 }  // extern "C"

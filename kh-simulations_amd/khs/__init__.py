@@ -143,6 +143,13 @@ def _signatures():
         "khs_mode_amplitude": (c_int, [c_void_p, c_int, dp]),
         "khs_fit_growth_max_r2": (c_int, [dp, dp, c_int, c_double, c_double, c_double, dp]),
         "khs_fit_growth_plateau": (c_int, [dp, dp, c_int, c_double, c_double, c_double, dp]),
+        "khs_batch_create": (c_int, [cfgp, c_int, c_int, c_int, P(c_void_p)]),
+        "khs_batch_destroy": (c_int, [c_void_p]), "khs_batch_init": (c_int, [c_void_p]),
+        "khs_batch_run": (c_int, [c_void_p, c_int, P(c_int), P(c_int)]),
+        "khs_batch_run_eager": (c_int, [c_void_p, c_int, P(c_int), P(c_int)]),
+        "khs_batch_get_array": (c_int, [c_void_p, c_int, c_int, dp, c_size_t]),
+        "khs_batch_set_array": (c_int, [c_void_p, c_int, c_int, dp, c_size_t]),
+        "khs_batch_energy": (c_int, [c_void_p, c_int, dp, dp]),
     }
 
 
@@ -397,3 +404,67 @@ class Simulation:
                 break
             out[nm.decode()] = (ms[i], cnt[i])
         return out
+
+
+class Batch:
+    """Independent runs stepped together through captured HIP graphs
+    (khs_batch_*).  configs: list of Config."""
+
+    def __init__(self, configs, lanes=4, segment=50):
+        load_library()
+        self.configs = [c if isinstance(c, Config) else Config(**c) for c in configs]
+        n = len(self.configs)
+        arr = (khs_config * n)(*[c.struct for c in self.configs])
+        h = ctypes.c_void_p()
+        _check(_lib.khs_batch_create(arr, n, int(lanes), int(segment), ctypes.byref(h)))
+        self._h = h
+        self.n = n
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.khs_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init(self):
+        _check(_lib.khs_batch_init(self._h))
+
+    def _run(self, fn, nsteps):
+        done = (ctypes.c_int * self.n)()
+        halted = (ctypes.c_int * self.n)()
+        _check(fn(self._h, int(nsteps), done, halted))
+        return list(done), [bool(h) for h in halted]
+
+    def run(self, nsteps):
+        """Graph-captured stepping -> (steps per run, halted flags)."""
+        return self._run(_lib.khs_batch_run, nsteps)
+
+    def run_eager(self, nsteps):
+        return self._run(_lib.khs_batch_run_eager, nsteps)
+
+    def get(self, run, name):
+        if not 0 <= run < self.n:
+            raise KhsError(-1, f"run index {run} out of range")
+        c = self.configs[run]
+        ids = EM_ARRAY_IDS if c.model == MODEL_EMHD else ARRAY_IDS
+        out = np.empty((c.nz, c.nx), dtype=np.float64)
+        _check(_lib.khs_batch_get_array(self._h, int(run), ids[name] if isinstance(name, str) else name,
+                                        out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), out.size))
+        return out
+
+    def set(self, run, name, arr):
+        c = self.configs[run]
+        ids = EM_ARRAY_IDS if c.model == MODEL_EMHD else ARRAY_IDS
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(c.nz, c.nx)
+        _check(_lib.khs_batch_set_array(self._h, int(run), ids[name] if isinstance(name, str) else name,
+                                        a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), a.size))
+
+    def energy(self, run):
+        e, e0 = ctypes.c_double(), ctypes.c_double()
+        _check(_lib.khs_batch_energy(self._h, int(run), ctypes.byref(e), ctypes.byref(e0)))
+        return e.value, e0.value

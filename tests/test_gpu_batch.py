@@ -1,0 +1,77 @@
+"""Batched runs through captured HIP graphs (khs_batch_*): every graph replay
+gives the same bits as stepping each run on its own (eager launches), with
+eager work and synchronisations between the replays; a run that halts leaves
+the graph and the others continue bit-identically."""
+import numpy as np
+import pytest
+
+import khs
+import oracle_ref as orc
+from helpers import assert_bitwise
+
+pytestmark = pytest.mark.gpu
+
+
+def small_configs():
+    base = dict(alpha_YM=1.0, courant_override=0.4, run_mode=0, freeze_override=1, perturb_amp=1e-3,
+                noise_amp=1e-9, xi_cut=4.0, n_steps=1000)
+    grids = [(128, 64, 0.05, 1.0, 0, "fp32"), (256, 128, 0.1, 0.7, 1, "fp32"), (96, 32, 0.02, 1.5, 0, "fp64"),
+             (64, 64, 0.15, 0.5, 1, "fp32"), (200, 48, 0.08, 1.2, 0, "fp64")]
+    out = []
+    for i, (nx, nz, v0, a, kz0, prec) in enumerate(grids):
+        out.append(dict(base, nx_override=nx, nz_override=nz, lx_override=nx * 0.1, lz_override=nz * 0.1,
+                        V0=v0, eps_override=a, k_mode=1 + i % 3, suppress_kz0=kz0, noise_seed=i, precision=prec))
+    return out
+
+
+def test_batch_graph_replays_match_eager():
+    cfgs = small_configs()
+    batch = khs.Batch(cfgs, lanes=2, segment=10)
+    batch.init()
+    sims = []
+    for c in cfgs:
+        s = khs.Simulation(khs.Config(**c))
+        s.init()
+        sims.append(s)
+    other = khs.Simulation(khs.Config(**cfgs[1]))     # unrelated eager work between replays
+    other.init()
+    for rep in range(3):
+        done, halted = batch.run(10)
+        assert done == [10 * (rep + 1)] * len(cfgs) and not any(halted)
+        other.step(3)
+        other.sync()
+        for i, s in enumerate(sims):
+            s.step(10)
+            for n in ("PzA", "Q2B", "By2", "Ez3", "Az2", "nB"):
+                assert_bitwise(batch.get(i, n), s.get(n), f"replay {rep} run {i} {n}")
+            E, E0 = batch.energy(i)
+            assert E == s.energy()
+    # odd tail (one eager step per run) and a longer call spanning segments
+    done, halted = batch.run(21)
+    for i, s in enumerate(sims):
+        s.step(21)
+        assert done[i] == 51
+        assert_bitwise(batch.get(i, "PxA"), s.get("PxA"), f"tail run {i}")
+
+
+def test_batch_halted_run_leaves_graph():
+    cfgs = small_configs()[:3]
+    batch = khs.Batch(cfgs, lanes=3, segment=4)
+    batch.init()
+    ref = khs.Simulation(khs.Config(**cfgs[0]))
+    ref.init()
+    done, halted = batch.run(4)
+    ref.step(4)
+    arr = batch.get(1, "nA")
+    arr[5, 5] = np.nan                               # run 1 goes NaN -> halts at the next check
+    batch.set(1, "nA", arr)
+    done, halted = batch.run(4)
+    ref.step(4)
+    assert halted == [False, True, False] and done == [8, 8, 8]
+    done, halted = batch.run(8)                      # graph captured again without run 1
+    ref.step(8)
+    assert halted == [False, True, False] and done == [16, 8, 16]
+    for n in ("PzA", "By2", "Az3"):
+        assert_bitwise(batch.get(0, n), ref.get(n), f"run 0 {n}")
+    with pytest.raises(khs.KhsError):
+        batch.get(7, "nA")
