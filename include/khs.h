@@ -223,6 +223,13 @@ int khs_gauss(khs_ctx* ctx, double out[7]);
  * Az2 - i Az3.  FP64 device reduction.  YM model only. */
 int khs_mode_amplitude(khs_ctx* ctx, int kz, double out[2]);
 
+/* 2-D Fourier coefficients of a state array (id of the run's model) at
+ * kx, kz = 1..kmax (kmax <= 32): out[2 (kz-1) kmax + 2 (kx-1) + {0, 1}] =
+ * {Re, Im} of sum_{x,z} f e^{-2 pi i (kx x / NX + kz z / NZ)}.  The FP32 form
+ * contracts z and then x on MFMA (v_mfma_f32_16x16x4_f32); the FP64 form
+ * with plain FP64 sums.  Used for the space-time dispersion of wave runs. */
+int khs_mode_spectrum(khs_ctx* ctx, int id, int kmax, double* out);
+
 /* ---------------- growth-rate fits (host only, no context) ---------------- */
 /* Max-R^2 window fit of log(amp) vs t (analysis/batch_analyze.py:44-69).
  * t_min / t_max: NaN = unbounded.  out = {gamma, t0, t1, r2}; fewer than 4

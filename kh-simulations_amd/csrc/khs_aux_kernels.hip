@@ -491,6 +491,35 @@ void launch_mode_amp(const T* a2, const T* a3, int kz, const KParams& p, double*
     hipLaunchKernelGGL(k_reduce_final<2>, dim3(1), dim3(kRedThreads), 0, st, partials, blocks, out);
 }
 
+// --------------------------------------------------------------------------
+// 2-D mode spectrum, FP64 form (plain FP64 sums; the FP32 form runs on MFMA
+// in khs_filter_mfma.hip).  Tables have 64 columns: r < K cos, K <= r < 2K sin.
+// --------------------------------------------------------------------------
+__global__ void k_spec_z_f64(const double* __restrict__ f, const double* __restrict__ tab, double* __restrict__ C,
+                             int nx, int nz, int rows) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (x >= nx || r >= rows) return;
+    double s = 0.0;
+    for (int z = 0; z < nz; ++z) s += f[(size_t)z * nx + x] * tab[(size_t)z * 64 + r];
+    C[(size_t)r * nx + x] = s;
+}
+
+__global__ void k_spec_x_f64(const double* __restrict__ C, const double* __restrict__ utab, double* __restrict__ D,
+                             int nx, int rows) {
+    const int s = threadIdx.x, r = blockIdx.x;
+    if (r >= rows || s >= rows) return;
+    double acc = 0.0;
+    for (int x = 0; x < nx; ++x) acc += C[(size_t)r * nx + x] * utab[(size_t)x * 64 + s];
+    D[(size_t)r * 64 + s] = acc;
+}
+
+void launch_spectrum_f64(const double* f, const double* ztab, const double* xtab, double* C, double* D, int nx,
+                         int nz, int rows, hipStream_t st) {
+    hipLaunchKernelGGL(k_spec_z_f64, dim3((nx + 127) / 128, rows), dim3(128), 0, st, f, ztab, C, nx, nz, rows);
+    hipLaunchKernelGGL(k_spec_x_f64, dim3(rows), dim3(64), 0, st, C, xtab, D, nx, rows);
+}
+
 // explicit instantiations
 #define KHS_INST(T)                                                                                   \
     template void launch_cell_filters<T>(const ArrayList<T>&, const double*, const int*, const KParams&, hipStream_t); \

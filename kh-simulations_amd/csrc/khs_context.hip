@@ -81,6 +81,10 @@ struct Ctx {
     ModeTable t_zmean, t_lo, t_hi, t_keep;
     double* d_coef = nullptr;
     void* d_keep_tab = nullptr;      // MFMA keep filter table (nz x 16, run precision)
+    // mode spectrum (allocated on first use)
+    char* spec_mem = nullptr;
+    void *spec_zt = nullptr, *spec_xt = nullptr, *spec_C = nullptr, *spec_D = nullptr;
+    int spec_K = 0;
 
     // reductions / derived
     double* d_partials = nullptr;
@@ -318,6 +322,7 @@ Ctx::~Ctx() {
     for (auto& p : pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
     for (auto e : evpool) hipEventDestroy(e);
     if (dmem) hipFree(dmem);
+    if (spec_mem) hipFree(spec_mem);
     if (h_red) hipHostFree(h_red);
     if (stream) hipStreamDestroy(stream);
 }
@@ -971,6 +976,82 @@ int khs_mode_amplitude(khs_ctx* ctx, int kz, double out[2]) {
         const double norm = (double)c.nx * (double)c.nz;
         out[0] = c.h_red[0] / norm;
         out[1] = c.h_red[1] / norm;
+    });
+}
+
+int khs_mode_spectrum(khs_ctx* ctx, int id, int kmax, double* out) {
+    CTX_OR_FAIL(ctx);
+    if (!out) { g_last_error = "null argument"; return KHS_ERR_ARG; }
+    return guard([&] {
+        Ctx& c = *ctx->impl;
+        const int slot = khs::state_slot(c, id);
+        if (slot < 0) throw Error(KHS_ERR_ARG, "khs_mode_spectrum: state arrays of the model only");
+        if (kmax < 1 || kmax > 32 || 2 * kmax > std::min(c.nx, c.nz))
+            throw Error(KHS_ERR_ARG, "kmax must be 1..32 and below half the grid");
+        if (c.fp32 && !khs::spectrum_mfma_supported(c.nx, c.nz, kmax))
+            throw Error(KHS_ERR_ARG, "FP32 spectrum needs nx % 4 == 0 and nz % 16 == 0");
+        const size_t e = c.esz;
+        if (!c.spec_mem) {
+            const size_t zt = (size_t)c.nz * 64 * e, xt = (size_t)c.nx * 64 * e, cb = (size_t)64 * c.nx * e;
+            auto ru = [](size_t v) { return (v + 255) / 256 * 256; };
+            HIPCHK(hipMalloc((void**)&c.spec_mem, ru(zt) + ru(xt) + ru(cb) + ru(64 * 64 * e)));
+            c.spec_zt = c.spec_mem;
+            c.spec_xt = c.spec_mem + ru(zt);
+            c.spec_C = c.spec_mem + ru(zt) + ru(xt);
+            c.spec_D = c.spec_mem + ru(zt) + ru(xt) + ru(cb);
+            c.spec_K = 0;
+        }
+        if (c.spec_K != kmax) {
+            std::vector<float> zf((size_t)c.nz * 64), xf((size_t)c.nx * 64);
+            khs::fill_spectrum_table(c.nz, kmax, zf.data());
+            khs::fill_spectrum_table(c.nx, kmax, xf.data());
+            if (c.fp32) {
+                HIPCHK(hipMemcpy(c.spec_zt, zf.data(), zf.size() * 4, hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpy(c.spec_xt, xf.data(), xf.size() * 4, hipMemcpyHostToDevice));
+            } else {
+                auto dtab = [&](int n, void* dst) {
+                    std::vector<double> t((size_t)n * 64, 0.0);
+                    for (int z = 0; z < n; ++z)
+                        for (int r = 0; r < 2 * kmax; ++r) {
+                            const int m = (r < kmax) ? r + 1 : r - kmax + 1;
+                            const double ph = 2.0 * M_PI * (double)((long)m * z % n) / (double)n;
+                            t[(size_t)z * 64 + r] = (r < kmax) ? std::cos(ph) : std::sin(ph);
+                        }
+                    HIPCHK(hipMemcpy(dst, t.data(), t.size() * 8, hipMemcpyHostToDevice));
+                };
+                dtab(c.nz, c.spec_zt);
+                dtab(c.nx, c.spec_xt);
+            }
+            c.spec_K = kmax;
+        }
+        std::vector<double> D(64 * 64);
+        if (c.fp32) {
+            khs::launch_spectrum_mfma((const float*)c.st[c.cur][slot], (const float*)c.spec_zt,
+                                      (const float*)c.spec_xt, (float*)c.spec_C, (float*)c.spec_D, c.nx, c.nz,
+                                      c.stream);
+            c.check_launch();
+            std::vector<float> Df(64 * 64);
+            HIPCHK(hipMemcpyAsync(Df.data(), c.spec_D, Df.size() * 4, hipMemcpyDeviceToHost, c.stream));
+            HIPCHK(hipStreamSynchronize(c.stream));
+            for (size_t i = 0; i < D.size(); ++i) D[i] = Df[i];
+        } else {
+            khs::launch_spectrum_f64((const double*)c.st[c.cur][slot], (const double*)c.spec_zt,
+                                     (const double*)c.spec_xt, (double*)c.spec_C, (double*)c.spec_D, c.nx, c.nz,
+                                     2 * kmax, c.stream);
+            c.check_launch();
+            HIPCHK(hipMemcpyAsync(D.data(), c.spec_D, D.size() * 8, hipMemcpyDeviceToHost, c.stream));
+            HIPCHK(hipStreamSynchronize(c.stream));
+        }
+        // D[r][s]: r = z row (cos kz | sin kz), s = x column (cos kx | sin kx);
+        // F = sum f e^{-i(a + b)} -> Re = CC - SS, Im = -(CS + SC)
+        const int K = kmax;
+        for (int kz = 0; kz < K; ++kz)
+            for (int kx = 0; kx < K; ++kx) {
+                const double cc = D[(size_t)kz * 64 + kx], cs = D[(size_t)kz * 64 + K + kx];
+                const double sc = D[(size_t)(K + kz) * 64 + kx], ss = D[(size_t)(K + kz) * 64 + K + kx];
+                out[2 * ((size_t)kz * K + kx) + 0] = cc - ss;
+                out[2 * ((size_t)kz * K + kx) + 1] = -(cs + sc);
+            }
     });
 }
 

@@ -186,3 +186,114 @@ template void launch_keep_mfma<double>(const ArrayList<double>&, const double*, 
                                        hipStream_t);
 
 }  // namespace khs
+
+namespace khs {
+
+// --------------------------------------------------------------------------
+// 2-D mode spectrum of one array on MFMA (FP32 form):
+//   C[r][x] = sum_z T[z][r] f[z][x]            (z contraction, MT x 16 rows)
+//   D[r][s] = sum_x C[r][x] U[x][s]            (x contraction, 64 columns)
+// with T[z][r] = cos/sin(2 pi kz z / nz) for kz = 1..K (rows r < K: cos,
+// K <= r < 2K: sin) and U[x][s] likewise along x.  D holds the four real
+// products (cos-cos, cos-sin, sin-cos, sin-sin) from which the complex
+// coefficients sum f e^{-2 pi i (kx x / nx + kz z / nz)} follow.
+// --------------------------------------------------------------------------
+constexpr int SP_ROWS = 64;      // 2K <= 64 (K <= 32)
+
+// z contraction: one 64-column strip per block, the 4 waves split z.
+__global__ __launch_bounds__(256) void k_spec_z_mfma(const float* __restrict__ f, const float* __restrict__ tab,
+                                                     float* __restrict__ C, int nx, int nz) {
+    __shared__ float red[SP_ROWS][64];
+    const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int n = l & 15, kq = l >> 4;
+    const int x0 = blockIdx.x * 64;
+    const int col = x0 + 4 * n;
+    const bool ok = col < nx;
+    const f4 zero = f4{0.f, 0.f, 0.f, 0.f};
+    f4 acc[4][4];                                   // [M tile][N tile]
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[t][j] = zero;
+    const int zper = nz >> 2;
+    const int zb = wave * zper, ze = zb + zper;
+    for (int z0 = zb; z0 < ze; z0 += 4) {
+        const int z = z0 + kq;
+        const f4 b = ok ? *reinterpret_cast<const f4*>(f + (size_t)z * nx + col) : zero;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const float at = tab[(size_t)z * SP_ROWS + 16 * t + n];
+            acc[t][0] = mfma16(at, b.x, acc[t][0]);
+            acc[t][1] = mfma16(at, b.y, acc[t][1]);
+            acc[t][2] = mfma16(at, b.z, acc[t][2]);
+            acc[t][3] = mfma16(at, b.w, acc[t][3]);
+        }
+    }
+    // sum the four z quarters through LDS, one wave at a time
+    for (int w = 0; w < 4; ++w) {
+        if (wave == w) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        float& r = red[16 * t + drow(0.f, kq, e)][4 * n + j];
+                        r = (w == 0) ? acc[t][j][e] : r + acc[t][j][e];
+                    }
+        }
+        __syncthreads();
+    }
+    for (int o = threadIdx.x; o < SP_ROWS * 64; o += 256) {
+        const int r = o >> 6, c = o & 63;
+        if (x0 + c < nx) C[(size_t)r * nx + x0 + c] = red[r][c];
+    }
+}
+
+// x contraction: D[64][64] = C[64][nx] U[nx][64], one block; wave w owns
+// output row tile w and all four column tiles.
+__global__ __launch_bounds__(256) void k_spec_x_mfma(const float* __restrict__ C, const float* __restrict__ utab,
+                                                     float* __restrict__ D, int nx) {
+    const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int n = l & 15, kq = l >> 4;
+    const f4 zero = f4{0.f, 0.f, 0.f, 0.f};
+    f4 acc[4] = {zero, zero, zero, zero};
+    for (int x0 = 0; x0 < nx; x0 += 4) {
+        const int x = x0 + kq;
+        const float a = C[(size_t)(16 * wave + n) * nx + x];          // A[i = n][k = kq]
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float b = utab[(size_t)x * SP_ROWS + 16 * j + n];   // B[k = kq][j' = n]
+            acc[j] = mfma16(a, b, acc[j]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) D[(size_t)(16 * wave + drow(0.f, kq, e)) * SP_ROWS + 16 * j + n] = acc[j][e];
+}
+
+bool spectrum_mfma_supported(int nx, int nz, int K) {
+    return K >= 1 && 2 * K <= SP_ROWS && nx % 4 == 0 && nz % 16 == 0;
+}
+
+void fill_spectrum_table(int n, int K, float* tab) {
+    for (int z = 0; z < n; ++z)
+        for (int r = 0; r < SP_ROWS; ++r) {
+            double v = 0.0;
+            if (r < 2 * K) {
+                const int m = (r < K) ? r + 1 : r - K + 1;
+                const double ph = 2.0 * M_PI * (double)((long)m * z % n) / (double)n;
+                v = (r < K) ? std::cos(ph) : std::sin(ph);
+            }
+            tab[(size_t)z * SP_ROWS + r] = (float)v;
+        }
+}
+
+void launch_spectrum_mfma(const float* f, const float* ztab, const float* xtab, float* C, float* D, int nx, int nz,
+                          hipStream_t st) {
+    hipLaunchKernelGGL(k_spec_z_mfma, dim3((nx + 63) / 64), dim3(256), 0, st, f, ztab, C, nx, nz);
+    hipLaunchKernelGGL(k_spec_x_mfma, dim3(1), dim3(256), 0, st, C, xtab, D, nx);
+}
+
+}  // namespace khs

@@ -61,6 +61,14 @@ template <typename T>
 void fill_keep_mfma_table(int nz, int K, T* tab);
 template <typename T>
 void launch_keep_mfma(const ArrayList<T>& arrs, const T* tab, T* coef, const KParams& p, hipStream_t st);
+// 2-D mode spectrum (kx, kz = 1..K): tables of 64 columns per grid point
+// (cos for r < K, sin for K <= r < 2K), C scratch 64 x nx, D output 64 x 64.
+bool spectrum_mfma_supported(int nx, int nz, int K);
+void fill_spectrum_table(int n, int K, float* tab);
+void launch_spectrum_mfma(const float* f, const float* ztab, const float* xtab, float* C, float* D, int nx, int nz,
+                          hipStream_t st);
+void launch_spectrum_f64(const double* f, const double* ztab, const double* xtab, double* C, double* D, int nx,
+                         int nz, int rows, hipStream_t st);
 template <typename T>
 void launch_hyperdiffuse(const ArrayList<T>& arrs, double mu, const KParams& p, hipStream_t st);
 template <typename T>

@@ -141,6 +141,7 @@ def _signatures():
         "khs_kernel_name": (c_char_p, [c_int]),
         "khs_get_tuning": (c_int, [c_void_p, P(c_int)]),
         "khs_mode_amplitude": (c_int, [c_void_p, c_int, dp]),
+        "khs_mode_spectrum": (c_int, [c_void_p, c_int, c_int, dp]),
         "khs_fit_growth_max_r2": (c_int, [dp, dp, c_int, c_double, c_double, c_double, dp]),
         "khs_fit_growth_plateau": (c_int, [dp, dp, c_int, c_double, c_double, c_double, dp]),
         "khs_batch_create": (c_int, [cfgp, c_int, c_int, c_int, P(c_void_p)]),
@@ -349,6 +350,13 @@ class Simulation:
         out = (ctypes.c_double * 2)()
         _check(_lib.khs_mode_amplitude(self._h, int(kz), out))
         return out[0], out[1]
+
+    def mode_spectrum(self, name, kmax):
+        """Complex 2-D Fourier coefficients [kz-1, kx-1] (kx, kz = 1..kmax) of a state array."""
+        out = np.zeros(2 * kmax * kmax)
+        _check(_lib.khs_mode_spectrum(self._h, self.array_id(name), int(kmax),
+                                      out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return (out[0::2] + 1j * out[1::2]).reshape(kmax, kmax)
 
     def timeseries(self, kz, n_steps, sample_stride):
         """Step n_steps, sampling the mode-kz amplitude at t = 0 and every
