@@ -19,7 +19,8 @@ physically finite.  The FP32 precision form of the same workload is timed too,
 and so is the precision-form workload (4096x2048, V0 = 0.1, a = 0.5, keep
 |kz| <= 8 on MFMA) in FP32 and FP64, with the kz = 1 growth rate of both
 (max-R^2 fit over the same window) and its relative error, and the batched
-sweep (64 uneven FP32 runs through captured graphs: runs/hour).
+sweep (64 uneven FP32 runs through captured graphs: runs/hour), and the
+EMHD 1024x1024 FP32 wave run with its MFMA space-time dispersion check.
 
 Prints one JSON line (rank 0, N = 1 only; there is one GPU per box).
 """
@@ -145,6 +146,44 @@ def sweep(n=64, steps=3000, lanes=8, segment=50):
     return out
 
 
+def emhd_waves(n=1024, steps=20000, stride=20, kmax=32, timed_steps=2000):
+    """Two-fluid EMHD wave run: n x n periodic, FP32, random-phase By spectrum
+    (kx, kz = 1..32, amplitude 1e-5, seed 2).  The step is timed on its own;
+    then the whole run records the By spectrum (MFMA z and x contractions)
+    every `stride` steps and every mode's frequency is compared with the
+    dispersion relation of the discretised model."""
+    import numpy as np
+    import khs
+    from khs.dispersion import emhd_wave_omega, peak_frequency, record_spectrum
+
+    keys = dict(model="emhd", nx_override=n, nz_override=n, periodic=1, species=2, emhd_init="spectrum",
+                spectrum_kmax=kmax, spectrum_amp=1e-5, spectrum_seed=2, smooth_init=0, precision="fp32",
+                n_steps=steps)
+    sim = khs.Simulation(khs.Config(**keys))
+    sim.init()
+    sim.step(20)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step(timed_steps)
+    sim.sync()
+    el = time.perf_counter() - t0
+    sim.init()
+    t1 = time.perf_counter()
+    t, F = record_spectrum(sim, "By", kmax, steps, stride)
+    el_run = time.perf_counter() - t1
+    cfg = sim.config
+    sim.close()
+    kx, kz = np.meshgrid(np.arange(1, kmax + 1), np.arange(1, kmax + 1))
+    want = emhd_wave_omega(kx, kz, cfg.nx, cfg.nz, cfg.dx, cfg.dz, charges=(-1.0, 1.0), densities=(1.0, 1.0))
+    got = np.array([[peak_frequency(t, F[:, i, j]) for j in range(kmax)] for i in range(kmax)])
+    err = np.abs(got / want - 1.0)
+    return {"workload": f"EMHD two-fluid {n}x{n} periodic FP32, random-phase By kx,kz=1..{kmax}, amp 1e-5, seed 2",
+            "cell_updates_per_s": n * n * timed_steps / el, "ms_per_step": el / timed_steps * 1e3,
+            "run_steps": steps, "spectrum_stride": stride, "run_seconds_with_spectra": el_run,
+            "modes": kmax * kmax, "omega_rel_err_max": float(err.max()), "omega_rel_err_median": float(np.median(err)),
+            "relation": "omega^2 = 2 sum q^2 n + k_Yee^2 (both FCT sweeps apply the Lorentz impulse)"}
+
+
 def cpu_baseline(budget_s=15.0):
     """FP64 OpenMP oracle on the host cores, same workload, bounded sample."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
@@ -201,6 +240,7 @@ def main():
     ap.add_argument("--no-c3", action="store_true", help="skip the 4096x2048 precision-form leg")
     ap.add_argument("--c3-steps", type=int, default=200)
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched-sweep (runs/hour) leg")
+    ap.add_argument("--no-c5", action="store_true", help="skip the EMHD wave-dispersion leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB")
     ap.add_argument("--split", action="store_true", help="separate x and z sweep kernels")
@@ -296,6 +336,8 @@ def main():
             "kernels_avg_us": {p: c3[p]["kernels_avg_us"] for p in c3}}
     if not a.no_sweep:
         out["sweep"] = sweep()
+    if not a.no_c5:
+        out["emhd_waves"] = emhd_waves()
     if not a.no_cpu:
         cb = cpu_baseline(a.cpu_seconds)
         out["cpu_baseline"] = cb

@@ -34,12 +34,18 @@ def record_spectrum(sim, name, kmax, n_steps, stride):
 
 
 def peak_frequency(t, series, pad=64):
-    """Angular frequency of the strongest line in a uniformly sampled series."""
-    x = np.asarray(series)
+    """Angular frequency |omega| of the strongest line in a uniformly sampled
+    (real or complex) series; for a complex series the powers at +omega and
+    -omega are added, so a standing wave F0 cos(omega t) is found whatever
+    the phase of F0."""
+    x = np.asarray(series, dtype=complex)
     x = x - x.mean()
     n = len(x)
     w = np.hanning(n)
-    spec = np.abs(np.fft.rfft(np.real(x) * w, pad * n))
+    full = np.abs(np.fft.fft(x * w, pad * n))
+    m = pad * n
+    spec = full[: m // 2 + 1].copy()
+    spec[1: m // 2] += full[m - 1: m // 2: -1]
     spec[0] = 0.0
     i = int(np.argmax(spec))
     i = min(max(i, 1), len(spec) - 2)
