@@ -126,6 +126,8 @@ typedef struct khs_config {
     double   spectrum_amp;
     uint64_t spectrum_seed;
     double   em_perturb_amp;   /* EMHD das_kaw perturbation (argv[2] as double, default 0.05); set by perturb_amp */
+    int      x_serial;         /* x sweep phases per variable: -1 = autotuned (default), 0 = all at once, 1 = one at a time */
+    int      maxwell_kernel;   /* YM Maxwell: -1 = autotuned (default), 0 = LDS tiles, 1 = per cell */
     int      x_rows;           /* x sweep rows per workgroup: 0 = autotuned (default), 1, 4 or 16 */
     int      z_prefetch;       /* z sweep: -1 = autotuned (default), 0 = load each row when used, 1 = prefetch the next row */
     int      mfma_filter;      /* keep-|kz|<=K filter as MFMA contractions (1, default) or the per-column FP64-sum kernels (0) */
@@ -258,9 +260,10 @@ int khs_timing_reset(khs_ctx* ctx);
 int khs_kernel_times(khs_ctx* ctx, double* ms, int* launches, int nslots);
 const char* khs_kernel_name(int slot);
 /* Tile shapes in use: out = {x-sweep segment width (cells), x-sweep rows per
- * workgroup, z-sweep chunk height (rows), z-sweep next-row prefetch (0/1)};
- * chosen by the autotuner on the first step unless autotune = 0. */
-int khs_get_tuning(khs_ctx* ctx, int out[4]);
+ * workgroup, z-sweep chunk height (rows), z-sweep next-row prefetch (0/1),
+ * YM Maxwell kernel (0 = LDS tiles, 1 = per cell), x-sweep variables one at a
+ * time (0/1)}; chosen by the autotuner on the first step unless autotune = 0. */
+int khs_get_tuning(khs_ctx* ctx, int out[6]);
 
 /* ---------------- batches of runs (parameter sweeps) ---------------- */
 /* No counterpart in the reference, which runs one configuration per process

@@ -103,7 +103,7 @@ struct Ctx {
     int cnt[K_NSLOTS] = {};
 
     // tile variants (autotuned on the first step)
-    int xvar = 2, zvar = 1;
+    int xvar = 2, zvar = 1, mvar = 0;
     bool tuned = false;
 
     // step state
@@ -214,18 +214,28 @@ struct Ctx {
         const int fp = cfg.z_prefetch;               // -1 = tuned
         std::vector<int> xc, zc;
         const int fr = cfg.x_rows == 1 ? 0 : cfg.x_rows == 4 ? 1 : cfg.x_rows == 16 ? 2 : -1;
+        const int fv = cfg.x_serial;             // -1 = tuned
         for (int v = 0; v < kNumXVariants; ++v)
-            if ((fx < 0 || v % 3 == fx) && (fr < 0 || v / 3 == fr)) xc.push_back(v);
+            if ((fx < 0 || v % 3 == fx) && (fr < 0 || (v / 3) % 3 == fr) && (fv < 0 || v / 9 == fv)) xc.push_back(v);
         for (int v = 0; v < kNumZVariants; ++v)
             if ((fz < 0 || v % 3 == fz) && (fp < 0 || v / 3 == fp)) zc.push_back(v);
         // without tuning: the forced shape, else 256-cell segments and 32-row
         // chunks with prefetch in FP32 (registers to spare), without in FP64
-        xvar = ((fx >= 0) ? fx : 2) + 3 * ((fr >= 0) ? fr : 1);
+        xvar = ((fx >= 0) ? fx : 2) + 3 * ((fr >= 0) ? fr : 1) + 9 * ((fv >= 0) ? fv : 1);
         zvar = ((fz >= 0) ? fz : 1) + 3 * ((fp >= 0) ? fp : (fp32 ? 1 : 0));
+        mvar = (cfg.maxwell_kernel >= 0) ? cfg.maxwell_kernel : 1;
         if (!cfg.autotune) return;
         if (xc.size() > 1) xvar = pick<T>(xc, [&](int v) { fct_x<T>(0, nsp, v); });
         fct_x<T>(0, nsp, xvar);
         if (zc.size() > 1) zvar = pick<T>(zc, [&](int v) { fct_z<T>(0, nsp, v); });
+        if (!emhd && cfg.maxwell_kernel < 0) {
+            std::vector<int> mc = {0, 1};
+            // writes the next state's fields from the current state: scratch here
+            mvar = pick<T>(mc, [&](int v) {
+                launch_maxwell<T>(state<T>(cur), state<T>(1 - cur), kp, cell_filters ? d_decay : nullptr,
+                                  cell_filters ? d_action : nullptr, v, stream);
+            });
+        }
     }
     void autotune() {
         if (tuned) return;
@@ -241,7 +251,7 @@ struct Ctx {
                 launch_emhd_maxwell<T>(state<T>(cur), state<T>(1 - cur), kp, stream);
             else
                 launch_maxwell<T>(state<T>(cur), state<T>(1 - cur), kp, cell_filters ? d_decay : nullptr,
-                                  cell_filters ? d_action : nullptr, stream);
+                                  cell_filters ? d_action : nullptr, mvar, stream);
         });
         check_launch();
     }
@@ -419,9 +429,10 @@ static std::unique_ptr<Ctx> make_ctx(const khs_config& in) {
     }
     ctx->cell_filters = (c.xi_sponge > 0.0f) || (c.xi_cut > 0.0f);
     if (c.x_segment) ctx->xvar = c.x_segment == 64 ? 0 : c.x_segment == 128 ? 1 : 2;
-    ctx->xvar += 3 * (c.x_rows == 1 ? 0 : c.x_rows == 16 ? 2 : 1);
+    ctx->xvar += 3 * (c.x_rows == 1 ? 0 : c.x_rows == 16 ? 2 : 1) + 9 * (c.x_serial == 0 ? 0 : 1);
     if (c.z_chunk) ctx->zvar = c.z_chunk == 16 ? 0 : c.z_chunk == 32 ? 1 : 2;
     if (c.z_prefetch == 1) ctx->zvar = ctx->zvar % 3 + 3;
+    if (c.maxwell_kernel >= 0) ctx->mvar = c.maxwell_kernel;
 
     // ---- one device allocation ----
     const bool share_az1 = c.freeze_az1 != 0;
@@ -1158,15 +1169,17 @@ int khs_kernel_times(khs_ctx* ctx, double* ms, int* launches, int nslots) {
     });
 }
 
-int khs_get_tuning(khs_ctx* ctx, int out[4]) {
+int khs_get_tuning(khs_ctx* ctx, int out[6]) {
     CTX_OR_FAIL(ctx);
     if (!out) { g_last_error = "null argument"; return KHS_ERR_ARG; }
     Ctx& c = *ctx->impl;
     static const int xs[3] = {64, 128, 256}, xr[3] = {1, 4, 16}, zs[3] = {16, 32, 64};
     out[0] = xs[c.xvar % 3];
-    out[1] = xr[c.xvar / 3];
+    out[1] = xr[(c.xvar / 3) % 3];
     out[2] = zs[c.zvar % 3];
     out[3] = c.zvar / 3;
+    out[4] = c.mvar;
+    out[5] = c.xvar / 9;
     return KHS_OK;
 }
 

@@ -7,10 +7,12 @@ namespace khs {
 
 // step kernels (khs_step_kernels.hip)
 // Tile variants (bit-identical results): x segment 64/128/256 cells
-// (variant % 3) with 1/4/16 rows per workgroup (variant / 3); z chunk
-// 16/32/64 rows (variant % 3) without / with next-row prefetch (variant / 3).
-constexpr int kNumXVariants = 9;
+// (variant % 3) with 1/4/16 rows per workgroup ((variant / 3) % 3), all
+// variables or one at a time through LDS (variant / 9); z chunk 16/32/64 rows
+// (variant % 3) without / with next-row prefetch (variant / 3).
+constexpr int kNumXVariants = 18;
 constexpr int kNumZVariants = 6;
+constexpr int kNumMaxwellVariants = 2;
 // M: YMModel (ymgpu2d two-beam SU(2) fluid) or EMModel (emhdgpu2d Abelian fluid)
 template <typename T, typename M>
 void launch_fct_x(const State<T>& S, const Stage<T>& out, const KParams& p, int sp0, int nsp,
@@ -29,10 +31,12 @@ void launch_derived_em(const State<T>& S, int which, const KParams& p, double* o
 // fused x + z sweep of both species: state S -> fluid of O
 template <typename T>
 void launch_fct_xz(const State<T>& S, const State<T>& O, const KParams& p, hipStream_t st);
-// decay/action: per-column sponge / xi_cut (nullptr when neither is active)
+// decay/action: per-column sponge / xi_cut (nullptr when neither is active);
+// variant 0: 64x16 tiles sharing the new E through LDS, 1: one thread per
+// cell recomputing its neighbours' E (bit-identical)
 template <typename T>
 void launch_maxwell(const State<T>& S, const State<T>& O, const KParams& p, const double* decay,
-                    const int* action, hipStream_t st);
+                    const int* action, int variant, hipStream_t st);
 
 // filters (khs_aux_kernels.hip)
 constexpr int kMaxFilterArrays = 24;

@@ -30,18 +30,23 @@ namespace khs {
 // XB cells per row segment (= threads), XW = segment + 3-cell halo each side.
 // XB is tuned per grid (64 / 128 / 256): smaller segments use less LDS and
 // so fit more workgroups per CU, which hides the load latency of phase A.
-// M is the model policy (YMModel / EMModel): variables, slots, sources.
-template <typename T, int XB, typename M>
+// VS = 1 runs phases C-E one variable at a time through single LOR / LNR /
+// flux rows (less LDS, more workgroups per CU, 3 barriers per variable);
+// VS = 0 keeps all variables' rows.  M is the model policy (YMModel /
+// EMModel): variables, slots, sources.
+template <typename T, int XB, typename M, int VS>
 __global__ __launch_bounds__(XB) void k_fct_x(State<T> S, Stage<T> out, KParams p, int sp0, int nsp,
                                               int ntx, int rows) {
     constexpr int XW = XB + 6;
     constexpr int NV = M::NV;
+    constexpr int NR = VS ? 1 : NV;   // LOR / LNR rows held at once
     __shared__ T sU[XW];
     __shared__ T sS[M::NXS][XW];
     __shared__ T sE[XW], sNU[XW], sMU[XW];
-    __shared__ T sV[NV][XW];      // variable, later the limited interface flux
-    __shared__ T sLOR[NV][XW];
-    __shared__ T sLNR[NV][XW];
+    __shared__ T sV[NV][XW];      // variable, later (VS = 0) the limited interface flux
+    __shared__ T sLOR[NR][XW];
+    __shared__ T sLNR[NR][XW];
+    __shared__ T sFL[VS ? XW : 1];
 
     const int tid = threadIdx.x;
     // logical tile: species fastest, then x segments, then groups of `rows`
@@ -90,6 +95,7 @@ __global__ __launch_bounds__(XB) void k_fct_x(State<T> S, Stage<T> out, KParams 
     }
     __syncthreads();
 
+    if constexpr (VS == 0) {
     // Phase C: transported (LORHOT) and diffused (LNRHOT) low-order values at
     // positions 1 .. XW-2 (FCT_Sweeps.cu:88-103).
     for (int q = 1 + tid; q < XW - 1; q += XB) {
@@ -134,6 +140,39 @@ __global__ __launch_bounds__(XB) void k_fct_x(State<T> S, Stage<T> out, KParams 
             T r = sLNR[v][q] - (sV[v][q] - sV[v][q - 1]);
             out.a[sp * NV + v][row + x] = r;
         }
+    }
+    } else {
+    // the same phases C, D, E one variable at a time
+    const int x = x0 + tid;
+#pragma unroll 1
+    for (int v = 0; v < NV; ++v) {
+        const int sl = M::xslot(v);
+        for (int q = 1 + tid; q < XW - 1; q += XB) {
+            const T eL = sE[q - 1], eR = sE[q];
+            const T nL = sNU[q - 1], nR = sNU[q];
+            const T vm = sV[v][q - 1], v0 = sV[v][q], vp = sV[v][q + 1];
+            T fL = T(0.5) * eL * (vm + v0);
+            T fR = T(0.5) * eR * (v0 + vp);
+            T dL = nL * (v0 - vm);
+            T dR = nR * (vp - v0);
+            T src = (sl >= 0) ? sS[sl < 0 ? 0 : sl][q] : T(0.0);
+            T lor = v0 + dt * src - (fR - fL);
+            sLOR[0][q] = lor;
+            sLNR[0][q] = lor + (dR - dL);
+        }
+        __syncthreads();
+        for (int q = 2 + tid; q < XW - 3; q += XB) {
+            T flx = sMU[q] * (sLOR[0][q + 1] - sLOR[0][q]);
+            T l0 = sLNR[0][q - 1], l1 = sLNR[0][q], l2 = sLNR[0][q + 1], l3 = sLNR[0][q + 2];
+            sFL[q] = fct_limit(flx, l1 - l0, l2 - l1, l3 - l2);
+        }
+        __syncthreads();
+        if (x < p.nx) {
+            const int q = 3 + tid;
+            out.a[sp * NV + v][row + x] = sLNR[0][q] - (sFL[q] - sFL[q - 1]);
+        }
+        __syncthreads();              // sLOR / sLNR / sFL are rewritten for the next variable
+    }
     }
     __syncthreads();                  // LDS is restaged for the next row
     }
@@ -520,6 +559,146 @@ __global__ __launch_bounds__(MX * MZT) void k_maxwell(State<T> S, State<T> O, KP
     }
 }
 
+// --------------------------------------------------------------------------
+// Maxwell, one thread per cell, no LDS: the thread evaluates Ampere at its
+// cell (Ex, Ez), Ez at x+1 and Ex at z+1 itself (the neighbours' values come
+// from the same formulas on the same inputs, so the results are the bits the
+// tiled kernel shares through LDS), then potential and Faraday.  Blocks of
+// 64 x 4 cells in XCD-aware row-band order: the z+1 / x+1 re-reads hit L2.
+// --------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void jx_at(const State<T>& F, int x, int z, const KParams& p, T Jx[3]) {
+    const int nx = p.nx;
+    const size_t i = (size_t)x + (size_t)z * nx;
+    const size_t ixp = (size_t)xp_src(x, p) + (size_t)z * nx;
+    T vxr[2], nQx[2][3];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const int b = S_NA + s * F_COUNT;
+        const T n0 = F.a[b + F_N][i], np = F.a[b + F_N][ixp];
+        vxr[s] = T(0.5) * (F.a[b + F_PX][i] / kmax(n0, T(1e-10)) + F.a[b + F_PX][ixp] / kmax(np, T(1e-10)));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) nQx[s][c] = T(0.5) * (n0 * F.a[b + F_Q1 + c][i] + np * F.a[b + F_Q1 + c][ixp]);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) Jx[c] = -(nQx[0][c] * vxr[0] + nQx[1][c] * vxr[1]);
+}
+
+template <typename T>
+__device__ __forceinline__ void jz_at(const State<T>& F, int x, int z, const KParams& p, T Jz[3]) {
+    const int nx = p.nx;
+    const size_t i = (size_t)x + (size_t)z * nx;
+    const size_t izp = (size_t)x + (size_t)zp_per(z, p.nz) * nx;
+    T vzt[2], nQz[2][3];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const int b = S_NA + s * F_COUNT;
+        const T n0 = F.a[b + F_N][i], nzp = F.a[b + F_N][izp];
+        vzt[s] = T(0.5) * (F.a[b + F_PZ][i] / kmax(n0, T(1e-10)) + F.a[b + F_PZ][izp] / kmax(nzp, T(1e-10)));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) nQz[s][c] = T(0.5) * (n0 * F.a[b + F_Q1 + c][i] + nzp * F.a[b + F_Q1 + c][izp]);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) Jz[c] = -(nQz[0][c] * vzt[0] + nQz[1][c] * vzt[1]);
+}
+
+// Ampere (YM_Physics.cu:70-113) for Ex^a at (x, z) from state S and new fluid O.
+template <typename T>
+__device__ __forceinline__ void ampere_ex(const State<T>& S, const State<T>& O, int x, int z, const KParams& p,
+                                          T out[3]) {
+    const int nx = p.nx;
+    const size_t i = (size_t)x + (size_t)z * nx;
+    const size_t izm = (size_t)x + (size_t)zm_per(z, p.nz) * nx;
+    T Jx[3];
+    jx_at(O, x, z, p, Jx);
+    T By[3], Byzm[3], Az[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        By[c] = S.a[S_BY1 + c][i];
+        Byzm[c] = S.a[S_BY1 + c][izm];
+        Az[c] = S.a[S_AZ1 + c][i];
+    }
+    const T cr[3] = {Az[1] * By[2] - Az[2] * By[1], Az[2] * By[0] - Az[0] * By[2], Az[0] * By[1] - Az[1] * By[0]};
+    const T dt = T(p.dt), c2 = T(p.c2), aYM = T(p.alpha), eps0 = T(p.eps0), dz = T(p.dz);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        T dBdz = (By[c] - Byzm[c]) / dz;
+        out[c] = S.a[S_EX1 + c][i] + dt * (c2 * (-dBdz - aYM * cr[c]) - Jx[c] / eps0);
+    }
+}
+
+// Ampere for Ez^a at (x, z); walled x: Ez = 0 at the walls first
+// (kernel_enforce_boundaries, YM_Fluid.cu:23-26).
+template <typename T>
+__device__ __forceinline__ void ampere_ez(const State<T>& S, const State<T>& O, int x, int z, const KParams& p,
+                                          T out[3]) {
+    const int nx = p.nx;
+    const size_t i = (size_t)x + (size_t)z * nx;
+    const int xm = p.periodic_x ? (x + nx - 1) % nx : (x == 0 ? 0 : x - 1);
+    const size_t ixm = (size_t)xm + (size_t)z * nx;
+    T Jz[3];
+    jz_at(O, x, z, p, Jz);
+    const bool wall = !p.periodic_x && (x == 0 || x == nx - 1);
+    const T dt = T(p.dt), c2 = T(p.c2), eps0 = T(p.eps0), dx = T(p.dx);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        T dBdx = (S.a[S_BY1 + c][i] - S.a[S_BY1 + c][ixm]) / dx;
+        T ez = wall ? T(0.0) : S.a[S_EZ1 + c][i];
+        out[c] = ez + dt * (c2 * dBdx - Jz[c] / eps0);
+    }
+}
+
+constexpr int MC_X = 64, MC_Z = 4;
+
+template <typename T>
+__global__ __launch_bounds__(MC_X * MC_Z) void k_maxwell_cell(State<T> S, State<T> O, KParams p, int ntx, int ntz,
+                                                              const double* decay, const int* action) {
+    const int nx = p.nx, nz = p.nz;
+    const int lt = xcd_logical(blockIdx.x, gridDim.x);
+    if (lt >= ntx * ntz) return;
+    const int x = (lt % ntx) * MC_X + (int)(threadIdx.x % MC_X);
+    const int z = (lt / ntx) * MC_Z + (int)(threadIdx.x / MC_X);
+    if (x >= nx || z >= nz) return;
+    const size_t i = (size_t)x + (size_t)z * nx;
+    const T dt = T(p.dt), aYM = T(p.alpha), dx = T(p.dx), dz = T(p.dz);
+    T Ex[3], Ez[3], Exzp[3], Ezxp[3];
+    ampere_ex(S, O, x, z, p, Ex);
+    ampere_ez(S, O, x, z, p, Ez);
+    // Faraday's neighbours; walled x: E(x+1) of the last column is its own
+    const int xq = (!p.periodic_x && x == nx - 1) ? x : xp_src(x, p);
+    if (xq == x) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Ezxp[c] = Ez[c];
+    } else {
+        ampere_ez(S, O, xq, z, p, Ezxp);
+    }
+    ampere_ex(S, O, x, zp_per(z, nz), p, Exzp);
+    // kernel_ym_potential (YM_Physics.cu:589-599), kernel_ym_faraday (:120-151)
+    T Az[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        Az[c] = S.a[S_AZ1 + c][i];
+        if (c > 0 || !p.freeze_az1) Az[c] = Az[c] - dt * Ez[c];
+    }
+    const T cr[3] = {Az[1] * Ex[2] - Az[2] * Ex[1], Az[2] * Ex[0] - Az[0] * Ex[2], Az[0] * Ex[1] - Az[1] * Ex[0]};
+    const int act = action ? action[x] : 0;
+    const T dec = (act == 1) ? T(decay[x]) : T(1.0);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        T dEx = (Exzp[c] - Ex[c]) / dz;
+        T dEz = (Ezxp[c] - Ez[c]) / dx;
+        T by = S.a[S_BY1 + c][i];
+        T byn = by - dt * (dEx - dEz - aYM * cr[c]);
+        T exn = Ex[c], ezn = Ez[c], azn = Az[c];
+        if (c > 0 && act == 1) { byn *= dec; exn *= dec; ezn *= dec; azn *= dec; }
+        if (c > 0 && act == 2) { byn = T(0.0); exn = T(0.0); ezn = T(0.0); azn = T(0.0); }
+        O.a[S_BY1 + c][i] = byn;
+        O.a[S_EX1 + c][i] = exn;
+        O.a[S_EZ1 + c][i] = ezn;
+        if (c > 0 || !p.freeze_az1) O.a[S_AZ1 + c][i] = azn;
+    }
+}
+
 // ==========================================================================
 // EMHD Maxwell: enforce_boundaries (Ey, Ez, Bx walls) + currents + Ampere +
 // vector potential + Faraday (emhdgpu2d/main_coupled.cu:48-63, 99-110;
@@ -643,24 +822,28 @@ __global__ __launch_bounds__(MX * MZT) void k_emhd_maxwell(State<T> S, State<T> 
 // ==========================================================================
 template <typename T, int XB, typename M>
 static void launch_fct_x_t(const State<T>& S, const Stage<T>& out, const KParams& p, int sp0, int nsp,
-                           int rows, hipStream_t st) {
+                           int rows, int vs, hipStream_t st) {
     const int ntx = (p.nx + XB - 1) / XB;
     const int nrg = (p.nz + rows - 1) / rows;
     dim3 grid(round8(ntx * nrg * nsp));
-    hipLaunchKernelGGL((k_fct_x<T, XB, M>), grid, dim3(XB), 0, st, S, out, p, sp0, nsp, ntx, rows);
+    if (vs)
+        hipLaunchKernelGGL((k_fct_x<T, XB, M, 1>), grid, dim3(XB), 0, st, S, out, p, sp0, nsp, ntx, rows);
+    else
+        hipLaunchKernelGGL((k_fct_x<T, XB, M, 0>), grid, dim3(XB), 0, st, S, out, p, sp0, nsp, ntx, rows);
 }
 
 // Variant v: segment width 64 / 128 / 256 cells (v % 3), rows per workgroup
-// 1 / 4 / 16 (v / 3).
+// 1 / 4 / 16 ((v / 3) % 3), all variables / one variable at a time (v / 9).
 template <typename T, typename M>
 void launch_fct_x(const State<T>& S, const Stage<T>& out, const KParams& p, int sp0, int nsp,
                   int variant, hipStream_t st) {
     static const int kRows[3] = {1, 4, 16};
     const int rows = kRows[(variant / 3) % 3];
+    const int vs = variant / 9;
     switch (variant % 3) {
-        case 0: launch_fct_x_t<T, 64, M>(S, out, p, sp0, nsp, rows, st); break;
-        case 1: launch_fct_x_t<T, 128, M>(S, out, p, sp0, nsp, rows, st); break;
-        default: launch_fct_x_t<T, 256, M>(S, out, p, sp0, nsp, rows, st); break;
+        case 0: launch_fct_x_t<T, 64, M>(S, out, p, sp0, nsp, rows, vs, st); break;
+        case 1: launch_fct_x_t<T, 128, M>(S, out, p, sp0, nsp, rows, vs, st); break;
+        default: launch_fct_x_t<T, 256, M>(S, out, p, sp0, nsp, rows, vs, st); break;
     }
 }
 
@@ -708,7 +891,13 @@ void launch_fct_xz(const State<T>& S, const State<T>& O, const KParams& p, hipSt
 
 template <typename T>
 void launch_maxwell(const State<T>& S, const State<T>& O, const KParams& p, const double* decay,
-                    const int* action, hipStream_t st) {
+                    const int* action, int variant, hipStream_t st) {
+    if (variant == 1) {
+        const int ntx = (p.nx + MC_X - 1) / MC_X, ntz = (p.nz + MC_Z - 1) / MC_Z;
+        hipLaunchKernelGGL(k_maxwell_cell<T>, dim3(round8(ntx * ntz)), dim3(MC_X * MC_Z), 0, st, S, O, p, ntx, ntz,
+                           decay, action);
+        return;
+    }
     const int ntx = (p.nx + MX - 1) / MX, ntz = (p.nz + MZ - 1) / MZ;
     dim3 grid(round8(ntx * ntz));
     hipLaunchKernelGGL(k_maxwell<T>, grid, dim3(MX * MZT), 0, st, S, O, p, ntx, ntz, decay, action);
@@ -726,7 +915,7 @@ template void launch_emhd_maxwell<double>(const State<double>&, const State<doub
 template void launch_emhd_maxwell<float>(const State<float>&, const State<float>&, const KParams&, hipStream_t);
 template void launch_fct_xz<double>(const State<double>&, const State<double>&, const KParams&, hipStream_t);
 template void launch_fct_xz<float>(const State<float>&, const State<float>&, const KParams&, hipStream_t);
-template void launch_maxwell<double>(const State<double>&, const State<double>&, const KParams&, const double*, const int*, hipStream_t);
-template void launch_maxwell<float>(const State<float>&, const State<float>&, const KParams&, const double*, const int*, hipStream_t);
+template void launch_maxwell<double>(const State<double>&, const State<double>&, const KParams&, const double*, const int*, int, hipStream_t);
+template void launch_maxwell<float>(const State<float>&, const State<float>&, const KParams&, const double*, const int*, int, hipStream_t);
 
 }  // namespace khs

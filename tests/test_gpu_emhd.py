@@ -127,7 +127,8 @@ def test_tile_variants_bitwise(zc, pf):
     nx, nz, periodic, nsp = 130, 70, 0, 2
     keys = em_keys(nx, nz, periodic, nsp, z_chunk=zc, z_prefetch=pf, x_segment=64, x_rows=16, autotune=0)
     cfg, sim, o = em_pair(em_random(nx, nz, nsp, seed=4, smooth=True), **keys)
-    assert sim.tuning() == dict(x_segment=64, x_rows=16, z_chunk=zc, z_prefetch=pf)
+    t = sim.tuning()
+    assert (t["x_segment"], t["x_rows"], t["z_chunk"], t["z_prefetch"]) == (64, 16, zc, pf)
     o.lorentz()
     sim.step(5)
     o.step(5)
