@@ -76,6 +76,7 @@ void config_defaults(khs_config& c) {
     c.x_rows = 0;
     c.maxwell_kernel = -1;
     c.x_serial = -1;
+    c.xz_variant = -1;
 }
 
 static int parse_int(const std::string& k, const std::string& v) {
@@ -157,6 +158,7 @@ bool config_set(khs_config& c, const std::string& key, const std::string& val) {
         else if (val == "spectrum" || val == "1") c.emhd_init = 1;
         else throw Error(KHS_ERR_CONFIG, "emhd_init must be daskaw or spectrum");
     }
+    else if (key == "xz_variant")        c.xz_variant = parse_int(key, val);
     else if (key == "x_serial")          c.x_serial = parse_int(key, val);
     else if (key == "maxwell_kernel")    c.maxwell_kernel = parse_int(key, val);
     else if (key == "x_rows")            c.x_rows = parse_int(key, val);
@@ -232,6 +234,7 @@ static void config_derive_emhd(khs_config& c) {
         throw Error(KHS_ERR_CONFIG, "x_rows must be 0, 1, 4 or 16");
     if (c.maxwell_kernel < -1 || c.maxwell_kernel > 1) throw Error(KHS_ERR_CONFIG, "maxwell_kernel must be -1, 0 or 1");
     if (c.x_serial < -1 || c.x_serial > 1) throw Error(KHS_ERR_CONFIG, "x_serial must be -1, 0 or 1");
+    if (c.xz_variant < -1 || c.xz_variant > 3) throw Error(KHS_ERR_CONFIG, "xz_variant must be -1..3");
     if (c.emhd_init == 1 && (c.spectrum_kmax < 1 || 2 * c.spectrum_kmax >= std::min(c.nx, c.nz)))
         throw Error(KHS_ERR_CONFIG, "spectrum_kmax must satisfy 1 <= K < min(nx, nz)/2");
 }
@@ -292,6 +295,7 @@ void config_derive(khs_config& c) {
         throw Error(KHS_ERR_CONFIG, "x_rows must be 0, 1, 4 or 16");
     if (c.maxwell_kernel < -1 || c.maxwell_kernel > 1) throw Error(KHS_ERR_CONFIG, "maxwell_kernel must be -1, 0 or 1");
     if (c.x_serial < -1 || c.x_serial > 1) throw Error(KHS_ERR_CONFIG, "x_serial must be -1, 0 or 1");
+    if (c.xz_variant < -1 || c.xz_variant > 3) throw Error(KHS_ERR_CONFIG, "xz_variant must be -1..3");
     if (c.run_mode < 0 || c.run_mode > 6)
         throw Error(KHS_ERR_CONFIG, "run_mode must be 0..6");
 }

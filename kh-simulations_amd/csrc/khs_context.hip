@@ -103,7 +103,7 @@ struct Ctx {
     int cnt[K_NSLOTS] = {};
 
     // tile variants (autotuned on the first step)
-    int xvar = 2, zvar = 1, mvar = 0;
+    int xvar = 2, zvar = 1, mvar = 0, xzvar = 2;
     bool tuned = false;
 
     // step state
@@ -223,11 +223,21 @@ struct Ctx {
         // chunks with prefetch in FP32 (registers to spare), without in FP64
         xvar = ((fx >= 0) ? fx : 2) + 3 * ((fr >= 0) ? fr : 1) + 9 * ((fv >= 0) ? fv : 1);
         zvar = ((fz >= 0) ? fz : 1) + 3 * ((fp >= 0) ? fp : (fp32 ? 1 : 0));
-        mvar = (cfg.maxwell_kernel >= 0) ? cfg.maxwell_kernel : 1;
+        mvar = (cfg.maxwell_kernel >= 0) ? cfg.maxwell_kernel : 0;
+        xzvar = (cfg.xz_variant >= 0) ? cfg.xz_variant : 2;
         if (!cfg.autotune) return;
         if (xc.size() > 1) xvar = pick<T>(xc, [&](int v) { fct_x<T>(0, nsp, v); });
         fct_x<T>(0, nsp, xvar);
         if (zc.size() > 1) zvar = pick<T>(zc, [&](int v) { fct_z<T>(0, nsp, v); });
+        if (!emhd && cfg.fused_sweep) {
+            if (cfg.xz_variant >= 0) {
+                xzvar = cfg.xz_variant;
+            } else {
+                std::vector<int> xzc;
+                for (int v = 0; v < kNumXZVariants; ++v) xzc.push_back(v);
+                xzvar = pick<T>(xzc, [&](int v) { launch_fct_xz<T>(state<T>(cur), state<T>(1 - cur), kp, v, stream); });
+            }
+        }
         if (!emhd && cfg.maxwell_kernel < 0) {
             std::vector<int> mc = {0, 1};
             // writes the next state's fields from the current state: scratch here
@@ -242,7 +252,7 @@ struct Ctx {
         fp32 ? do_autotune<float>() : do_autotune<double>();
     }
     template <typename T> void do_xz() {
-        timed(K_FCT_XZ, [&] { launch_fct_xz<T>(state<T>(cur), state<T>(1 - cur), kp, stream); });
+        timed(K_FCT_XZ, [&] { launch_fct_xz<T>(state<T>(cur), state<T>(1 - cur), kp, xzvar, stream); });
         check_launch();
     }
     template <typename T> void do_maxwell() {
@@ -301,7 +311,10 @@ struct Ctx {
         fp32 ? do_x<float>(sp0, nsp) : do_x<double>(sp0, nsp);
     }
     void z_stage(int sp0, int nsp) { fp32 ? do_z<float>(sp0, nsp) : do_z<double>(sp0, nsp); }
-    void xz_stage() { fp32 ? do_xz<float>() : do_xz<double>(); }
+    void xz_stage() {
+        autotune();
+        fp32 ? do_xz<float>() : do_xz<double>();
+    }
     void maxwell_stage() { fp32 ? do_maxwell<float>() : do_maxwell<double>(); }
     void filter_stage() { fp32 ? do_filters<float>() : do_filters<double>(); }
 
@@ -433,6 +446,7 @@ static std::unique_ptr<Ctx> make_ctx(const khs_config& in) {
     if (c.z_chunk) ctx->zvar = c.z_chunk == 16 ? 0 : c.z_chunk == 32 ? 1 : 2;
     if (c.z_prefetch == 1) ctx->zvar = ctx->zvar % 3 + 3;
     if (c.maxwell_kernel >= 0) ctx->mvar = c.maxwell_kernel;
+    if (c.xz_variant >= 0) ctx->xzvar = c.xz_variant;
 
     // ---- one device allocation ----
     const bool share_az1 = c.freeze_az1 != 0;
@@ -1169,7 +1183,7 @@ int khs_kernel_times(khs_ctx* ctx, double* ms, int* launches, int nslots) {
     });
 }
 
-int khs_get_tuning(khs_ctx* ctx, int out[6]) {
+int khs_get_tuning(khs_ctx* ctx, int out[7]) {
     CTX_OR_FAIL(ctx);
     if (!out) { g_last_error = "null argument"; return KHS_ERR_ARG; }
     Ctx& c = *ctx->impl;
@@ -1180,6 +1194,7 @@ int khs_get_tuning(khs_ctx* ctx, int out[6]) {
     out[3] = c.zvar / 3;
     out[4] = c.mvar;
     out[5] = c.xvar / 9;
+    out[6] = c.xzvar;
     return KHS_OK;
 }
 

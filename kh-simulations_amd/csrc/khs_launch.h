@@ -28,9 +28,11 @@ void launch_energy_em(const State<T>& S, const KParams& p, double* partials, dou
 // EMHD derived arrays: 0..2 J, 3..6 (ux, uz) per fluid, 7..12 (sx, sy, sz) per fluid, 13 energy density
 template <typename T>
 void launch_derived_em(const State<T>& S, int which, const KParams& p, double* out, hipStream_t st);
-// fused x + z sweep of both species: state S -> fluid of O
+// fused x + z sweep of both species: state S -> fluid of O (variants: see
+// launch_fct_xz in khs_step_kernels.hip)
+constexpr int kNumXZVariants = 4;
 template <typename T>
-void launch_fct_xz(const State<T>& S, const State<T>& O, const KParams& p, hipStream_t st);
+void launch_fct_xz(const State<T>& S, const State<T>& O, const KParams& p, int variant, hipStream_t st);
 // decay/action: per-column sponge / xi_cut (nullptr when neither is active);
 // variant 0: 64x16 tiles sharing the new E through LDS, 1: one thread per
 // cell recomputing its neighbours' E (bit-identical)

@@ -880,8 +880,196 @@ void launch_emhd_maxwell(const State<T>& S, const State<T>& O, const KParams& p,
     hipLaunchKernelGGL(k_emhd_maxwell<T>, grid, dim3(MX * MZT), 0, st, S, O, p, ntx, ntz);
 }
 
+// --------------------------------------------------------------------------
+// Fused x + z sweep, four waves per workgroup.  The workgroup owns 128
+// columns of one species and marches a chunk of TZ rows (+3 halo rows each
+// side).  Per extended row: all 256 threads stage the 134-position row
+// segment, x-sweep it in LDS three variables at a time, and leave the x-swept
+// values of the 128 columns in LDS; then waves 0/1 (columns 0-63 / 64-127)
+// push n, px, pz and waves 2/3 push Q1, Q2, Q3 into their register FCT
+// pipelines along z, with the z sources from row-carried field values.  Only
+// the step-start state is read and only the swept fluid is written.
+// --------------------------------------------------------------------------
+constexpr int XZ2_C = 128;            // columns per workgroup
+constexpr int XZ2_P = XZ2_C + 6;      // staged positions (3-cell halo each side)
+
+template <typename T, int TZ>
+__global__ __launch_bounds__(256) void k_fct_xz2(State<T> S, State<T> O, KParams p, int ntx, int nch) {
+    __shared__ T sU[XZ2_P], sS[XZ2_P], sE[XZ2_P], sNU[XZ2_P], sMU[XZ2_P];
+    __shared__ T sV[F_COUNT][XZ2_P];
+    __shared__ T sLOR[3][XZ2_P], sLNR[3][XZ2_P], sFL[3][XZ2_P];
+    __shared__ T sXS[F_COUNT][XZ2_C];
+
+    int lb = xcd_logical(blockIdx.x, gridDim.x);
+    if (lb >= 2 * ntx * nch) return;                 // uniform per block
+    const int sp = lb & 1;
+    lb >>= 1;
+    const int x0 = (lb % ntx) * XZ2_C;
+    const int z0 = (lb / ntx) * TZ;
+    const int nx = p.nx, nz = p.nz;
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int g = wave >> 1;                         // variable group: 0 = n, px, pz; 1 = Q1..Q3
+    const int c = (wave & 1) * 64 + lane;            // column within the workgroup
+    const int x = x0 + c;
+    const bool active = x < nx;
+    const int xz = active ? x : nx - 1;              // in-bounds column for loads
+    const int xm = xm_src(xz, p);
+    const int b = S_NA + sp * F_COUNT;
+    const T dt = T(p.dt), dtdx = T(p.dtdx), dtdz = T(p.dtdz);
+    const int zend = min(z0 + TZ, nz);
+    const int elast = zend + 2;
+    const bool wall = !p.periodic_x && (x == 0 || x == nx - 1);
+
+    typedef typename YMModel::template Row<T> Row;
+    auto load_row = [&](int r, Row& row) {
+        const size_t i = (size_t)xz + (size_t)r * nx, ixm = (size_t)xm + (size_t)r * nx;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (g == 0) {
+                row.ez[k] = S.a[S_EZ1 + k][i];
+                row.by[k] = S.a[S_BY1 + k][i];
+                row.byxm[k] = S.a[S_BY1 + k][ixm];
+                row.az[k] = T(0);
+            } else {
+                row.ez[k] = T(0); row.by[k] = T(0); row.byxm[k] = T(0);
+                row.az[k] = S.a[S_AZ1 + k][i];
+            }
+        }
+    };
+
+    int rc = clampi(z0 - 3, 0, nz - 1);
+    Row prev, cur;
+    load_row(zm_per(rc, nz), prev);
+    load_row(rc, cur);
+    ZPipe<T, 3> zp;
+    zp.Um = T(0);
+    zp.MUm = T(0);
+
+    for (int e = z0 - 3; e <= elast; ++e) {
+        const size_t row = (size_t)rc * nx;
+        // ---- phase A: stage positions q <-> x = x0-3+q ----
+        if (tid < XZ2_P) {
+            const int q = tid;
+            const int xr = map_x(x0 - 3 + q, p);
+            const size_t i = row + xr;
+            T v[F_COUNT];
+#pragma unroll
+            for (int k = 0; k < F_COUNT; ++k) v[k] = S.a[b + k][i];
+            const T n_safe = kmax(v[F_N], T(0.05));
+            sU[q] = v[F_PX] / n_safe;
+            sS[q] = lorentz_px(S, sp, xr, rc, p);
+#pragma unroll
+            for (int k = 0; k < F_COUNT; ++k) sV[k][q] = v[k];
+        }
+        __syncthreads();
+        // ---- phase B: interface coefficients ----
+        if (tid < XZ2_P - 1) {
+            const int q = tid;
+            T vi = T(0.5) * (sU[q] + sU[q + 1]);
+            T eps = vi * dtdx;
+            T nu, mu;
+            fct_nu_mu(eps, nu, mu);
+            sE[q] = eps; sNU[q] = nu; sMU[q] = mu;
+        }
+        __syncthreads();
+        // ---- phases C, D, E for variables 0-2, then 3-5 ----
+#pragma unroll 1
+        for (int v0 = 0; v0 < F_COUNT; v0 += 3) {
+            if (tid >= 1 && tid < XZ2_P - 1) {
+                const int q = tid;
+                const T eL = sE[q - 1], eR = sE[q];
+                const T nL = sNU[q - 1], nR = sNU[q];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    const int v = v0 + j;
+                    const T vm = sV[v][q - 1], vv = sV[v][q], vp = sV[v][q + 1];
+                    T fL = T(0.5) * eL * (vm + vv);
+                    T fR = T(0.5) * eR * (vv + vp);
+                    T dL = nL * (vv - vm);
+                    T dR = nR * (vp - vv);
+                    T src = (v == F_PX) ? sS[q] : T(0.0);
+                    T lor = vv + dt * src - (fR - fL);
+                    sLOR[j][q] = lor;
+                    sLNR[j][q] = lor + (dR - dL);
+                }
+            }
+            __syncthreads();
+            if (tid >= 2 && tid < XZ2_P - 3) {
+                const int q = tid;
+                const T mu = sMU[q];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    T flx = mu * (sLOR[j][q + 1] - sLOR[j][q]);
+                    T l0 = sLNR[j][q - 1], l1 = sLNR[j][q], l2 = sLNR[j][q + 1], l3 = sLNR[j][q + 2];
+                    sFL[j][q] = fct_limit(flx, l1 - l0, l2 - l1, l3 - l2);
+                }
+            }
+            __syncthreads();
+            if (tid < XZ2_C) {
+                const int q = 3 + tid;
+#pragma unroll
+                for (int j = 0; j < 3; ++j) sXS[v0 + j][tid] = sLNR[j][q] - (sFL[j][q] - sFL[j][q - 1]);
+            }
+            __syncthreads();
+        }
+        // ---- z pipeline of this wave's three variables ----
+        const int rn = clampi(min(e + 1, elast), 0, nz - 1);
+        Row nrow;
+        load_row(rn, nrow);
+        {
+            const int q = 3 + c;
+            T F[F_COUNT];
+#pragma unroll
+            for (int k = 0; k < F_COUNT; ++k) F[k] = sV[k][q];
+            T V3[3], S3[3], out[3];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) V3[j] = sXS[3 * g + j][c];
+            const T n_safe = kmax(F[F_N], T(0.05));
+            const T u = F[F_PZ] / n_safe;
+            const T qq[3] = {F[F_Q1], F[F_Q2], F[F_Q3]};
+            if (g == 0) {
+                S3[0] = T(0);
+                S3[1] = T(0);
+                S3[2] = lorentz_pz_v(F[F_N], F[F_PX], qq, cur.ez, prev.ez, cur.by, cur.byxm, prev.by, prev.byxm);
+                if (p.warm_T > 0.0) S3[2] -= warm_pz(S, sp, xz, rc, p);
+            } else {
+                precession_v(F[F_N], F[F_PZ], qq, cur.az, prev.az, T(p.alpha), S3);
+            }
+            if (zp.push(e, z0, V3, u, S3, dt, dtdz, out) && active) {
+                const size_t io = (size_t)x + (size_t)(e - 3) * nx;
+                if (g == 0) {
+                    if (out[0] < T(0.05)) out[0] = T(0.05);     // kernel_enforce_boundaries
+                    if (wall) out[1] = T(0.0);
+                }
+#pragma unroll
+                for (int j = 0; j < 3; ++j) O.a[b + 3 * g + j][io] = out[j];
+            }
+        }
+        YMModel::sel_row(rn != rc, cur, prev);
+        cur = nrow;
+        rc = rn;
+        __syncthreads();                              // LDS is restaged for the next row
+    }
+}
+
+template <typename T, int TZ>
+static void launch_fct_xz2_t(const State<T>& S, const State<T>& O, const KParams& p, hipStream_t st) {
+    const int ntx = (p.nx + XZ2_C - 1) / XZ2_C;
+    const int nch = (p.nz + TZ - 1) / TZ;
+    hipLaunchKernelGGL((k_fct_xz2<T, TZ>), dim3(round8(2 * ntx * nch)), dim3(256), 0, st, S, O, p, ntx, nch);
+}
+
+// Variant 0: one wave per 64 columns, 32-row chunks; 1/2/3: four waves per
+// 128 columns (variable groups split across waves), 16/32/64-row chunks.
 template <typename T>
-void launch_fct_xz(const State<T>& S, const State<T>& O, const KParams& p, hipStream_t st) {
+void launch_fct_xz(const State<T>& S, const State<T>& O, const KParams& p, int variant, hipStream_t st) {
+    switch (variant) {
+        case 1: launch_fct_xz2_t<T, 16>(S, O, p, st); return;
+        case 2: launch_fct_xz2_t<T, 32>(S, O, p, st); return;
+        case 3: launch_fct_xz2_t<T, 64>(S, O, p, st); return;
+        default: break;
+    }
     constexpr int TZ = 32;
     const int ntx = (p.nx + XZ_W - 1) / XZ_W;
     const int nch = (p.nz + TZ - 1) / TZ;
@@ -913,8 +1101,8 @@ KHS_SWEEP_INST(float, EMModel)
 #undef KHS_SWEEP_INST
 template void launch_emhd_maxwell<double>(const State<double>&, const State<double>&, const KParams&, hipStream_t);
 template void launch_emhd_maxwell<float>(const State<float>&, const State<float>&, const KParams&, hipStream_t);
-template void launch_fct_xz<double>(const State<double>&, const State<double>&, const KParams&, hipStream_t);
-template void launch_fct_xz<float>(const State<float>&, const State<float>&, const KParams&, hipStream_t);
+template void launch_fct_xz<double>(const State<double>&, const State<double>&, const KParams&, int, hipStream_t);
+template void launch_fct_xz<float>(const State<float>&, const State<float>&, const KParams&, int, hipStream_t);
 template void launch_maxwell<double>(const State<double>&, const State<double>&, const KParams&, const double*, const int*, int, hipStream_t);
 template void launch_maxwell<float>(const State<float>&, const State<float>&, const KParams&, const double*, const int*, int, hipStream_t);
 

@@ -192,12 +192,28 @@ def test_tile_variants_bitwise(xs, xr, zc, pf, mk, xv, periodic):
                 maxwell_kernel=mk, x_serial=xv, autotune=0, xi_sponge=8.0, xi_cut=12.0)
     st = random_state(300, 136, seed=31, smooth=True)
     cfg, sim, o = make_pair(st, **keys)
-    assert sim.tuning() == dict(x_segment=xs, x_rows=xr, z_chunk=zc, z_prefetch=pf, maxwell=mk, x_serial=xv)
+    t = sim.tuning()
+    assert t == dict(x_segment=xs, x_rows=xr, z_chunk=zc, z_prefetch=pf, maxwell=mk, x_serial=xv, xz_variant=t["xz_variant"])
     o.sources()
     sim.step(3)
     o.step(3)
     for n in orc.STATE:
         assert_bitwise(sim.get(n), o.get(n), f"x{xs}r{xr} z{zc} pf{pf} {n}")
+
+
+@pytest.mark.parametrize("xzv", [0, 1, 2, 3])
+@pytest.mark.parametrize("shape,periodic", [((300, 136), 1), ((70, 36), 0), ((256, 64), 0)])
+def test_fused_sweep_variants_bitwise(xzv, shape, periodic):
+    """Every fused x+z sweep kernel variant gives the oracle's bits."""
+    nx, nz = shape
+    keys = grid(nx, nz, periodic, warm_T=0.01 if periodic else 0.0, fused_sweep=1, xz_variant=xzv, autotune=0)
+    cfg, sim, o = make_pair(random_state(nx, nz, seed=nx + xzv, smooth=True), **keys)
+    assert sim.tuning()["xz_variant"] == xzv
+    o.sources()
+    sim.step(4)
+    o.step(4)
+    for n in orc.STATE:
+        assert_bitwise(sim.get(n), o.get(n), f"xz{xzv} {nx}x{nz} {n}")
 
 
 def test_determinism_and_staged_equals_fused():
