@@ -128,7 +128,7 @@ typedef struct khs_config {
     double   em_perturb_amp;   /* EMHD das_kaw perturbation (argv[2] as double, default 0.05); set by perturb_amp */
     int      xz_variant;       /* fused sweep kernel variant: -1 = autotuned (default), 0..3 */
     int      x_serial;         /* x sweep phases per variable: -1 = autotuned (default), 0 = all at once, 1 = one at a time */
-    int      maxwell_kernel;   /* YM Maxwell: -1 = autotuned (default), 0 = LDS tiles, 1 = per cell */
+    int      maxwell_kernel;   /* YM Maxwell: -1 = autotuned (default), 0/2/3 = 64x16/64x4/64x8 LDS tiles, 1 = per cell */
     int      x_rows;           /* x sweep rows per workgroup: 0 = autotuned (default), 1, 4 or 16 */
     int      z_prefetch;       /* z sweep: -1 = autotuned (default), 0 = load each row when used, 1 = prefetch the next row */
     int      mfma_filter;      /* keep-|kz|<=K filter as MFMA contractions (1, default) or the per-column FP64-sum kernels (0) */
@@ -262,7 +262,7 @@ int khs_kernel_times(khs_ctx* ctx, double* ms, int* launches, int nslots);
 const char* khs_kernel_name(int slot);
 /* Tile shapes in use: out = {x-sweep segment width (cells), x-sweep rows per
  * workgroup, z-sweep chunk height (rows), z-sweep next-row prefetch (0/1),
- * YM Maxwell kernel (0 = LDS tiles, 1 = per cell), x-sweep variables one at a
+ * YM Maxwell kernel (0/2/3 = 64x16/64x4/64x8 LDS tiles, 1 = per cell), x-sweep variables one at a
  * time (0/1), fused-sweep variant (0: one wave per 64 columns; 1/2/3: four
  * waves per 128 columns with 16/32/64-row chunks)}; chosen by the autotuner
  * on the first step unless autotune = 0. */

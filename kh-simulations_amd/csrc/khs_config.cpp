@@ -232,7 +232,7 @@ static void config_derive_emhd(khs_config& c) {
     if (c.z_prefetch < -1 || c.z_prefetch > 1) throw Error(KHS_ERR_CONFIG, "z_prefetch must be -1, 0 or 1");
     if (c.x_rows != 0 && c.x_rows != 1 && c.x_rows != 4 && c.x_rows != 16)
         throw Error(KHS_ERR_CONFIG, "x_rows must be 0, 1, 4 or 16");
-    if (c.maxwell_kernel < -1 || c.maxwell_kernel > 1) throw Error(KHS_ERR_CONFIG, "maxwell_kernel must be -1, 0 or 1");
+    if (c.maxwell_kernel < -1 || c.maxwell_kernel > 3) throw Error(KHS_ERR_CONFIG, "maxwell_kernel must be -1..3");
     if (c.x_serial < -1 || c.x_serial > 1) throw Error(KHS_ERR_CONFIG, "x_serial must be -1, 0 or 1");
     if (c.xz_variant < -1 || c.xz_variant > 3) throw Error(KHS_ERR_CONFIG, "xz_variant must be -1..3");
     if (c.emhd_init == 1 && (c.spectrum_kmax < 1 || 2 * c.spectrum_kmax >= std::min(c.nx, c.nz)))
@@ -293,7 +293,7 @@ void config_derive(khs_config& c) {
     if (c.z_prefetch < -1 || c.z_prefetch > 1) throw Error(KHS_ERR_CONFIG, "z_prefetch must be -1, 0 or 1");
     if (c.x_rows != 0 && c.x_rows != 1 && c.x_rows != 4 && c.x_rows != 16)
         throw Error(KHS_ERR_CONFIG, "x_rows must be 0, 1, 4 or 16");
-    if (c.maxwell_kernel < -1 || c.maxwell_kernel > 1) throw Error(KHS_ERR_CONFIG, "maxwell_kernel must be -1, 0 or 1");
+    if (c.maxwell_kernel < -1 || c.maxwell_kernel > 3) throw Error(KHS_ERR_CONFIG, "maxwell_kernel must be -1..3");
     if (c.x_serial < -1 || c.x_serial > 1) throw Error(KHS_ERR_CONFIG, "x_serial must be -1, 0 or 1");
     if (c.xz_variant < -1 || c.xz_variant > 3) throw Error(KHS_ERR_CONFIG, "xz_variant must be -1..3");
     if (c.run_mode < 0 || c.run_mode > 6)

@@ -239,7 +239,7 @@ struct Ctx {
             }
         }
         if (!emhd && cfg.maxwell_kernel < 0) {
-            std::vector<int> mc = {0, 1};
+            std::vector<int> mc = {0, 1, 2, 3};
             // writes the next state's fields from the current state: scratch here
             mvar = pick<T>(mc, [&](int v) {
                 launch_maxwell<T>(state<T>(cur), state<T>(1 - cur), kp, cell_filters ? d_decay : nullptr,

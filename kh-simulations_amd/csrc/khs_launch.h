@@ -12,7 +12,7 @@ namespace khs {
 // (variant % 3) without / with next-row prefetch (variant / 3).
 constexpr int kNumXVariants = 18;
 constexpr int kNumZVariants = 6;
-constexpr int kNumMaxwellVariants = 2;
+constexpr int kNumMaxwellVariants = 4;
 // M: YMModel (ymgpu2d two-beam SU(2) fluid) or EMModel (emhdgpu2d Abelian fluid)
 template <typename T, typename M>
 void launch_fct_x(const State<T>& S, const Stage<T>& out, const KParams& p, int sp0, int nsp,
@@ -34,8 +34,8 @@ constexpr int kNumXZVariants = 4;
 template <typename T>
 void launch_fct_xz(const State<T>& S, const State<T>& O, const KParams& p, int variant, hipStream_t st);
 // decay/action: per-column sponge / xi_cut (nullptr when neither is active);
-// variant 0: 64x16 tiles sharing the new E through LDS, 1: one thread per
-// cell recomputing its neighbours' E (bit-identical)
+// variant 0/2/3: 64x16 / 64x4 / 64x8 tiles sharing the new E through LDS,
+// 1: one thread per cell recomputing its neighbours' E (all bit-identical)
 template <typename T>
 void launch_maxwell(const State<T>& S, const State<T>& O, const KParams& p, const double* decay,
                     const int* action, int variant, hipStream_t st);

@@ -453,17 +453,17 @@ __global__ __launch_bounds__(XZ_W) void k_fct_xz(State<T> S, State<T> O, KParams
 // ==========================================================================
 constexpr int MX = 64, MZ = 16, MZT = 4;   // tile 64 x 16, 64 x 4 threads
 
-template <typename T>
+template <typename T, int TM>
 __global__ __launch_bounds__(MX * MZT) void k_maxwell(State<T> S, State<T> O, KParams p, int ntx, int ntz,
                                                       const double* decay, const int* action) {
-    __shared__ T sEx[3][MZ + 1][MX + 1];
-    __shared__ T sEz[3][MZ + 1][MX + 1];
+    __shared__ T sEx[3][TM + 1][MX + 1];
+    __shared__ T sEz[3][TM + 1][MX + 1];
     const int nx = p.nx, nz = p.nz;
     // logical tile: x tiles fastest (they share the x +- 1 halo lines)
     const int lt = xcd_logical(blockIdx.x, gridDim.x);
     if (lt >= ntx * ntz) return;
-    const int x0 = (lt % ntx) * MX, z0 = (lt / ntx) * MZ;
-    const int ncol = min(MX, nx - x0), nrow = min(MZ, nz - z0);
+    const int x0 = (lt % ntx) * MX, z0 = (lt / ntx) * TM;
+    const int ncol = min(MX, nx - x0), nrow = min(TM, nz - z0);
     const T dt = T(p.dt), c2 = T(p.c2), aYM = T(p.alpha), eps0 = T(p.eps0);
     const T dx = T(p.dx), dz = T(p.dz);
     const int tx = threadIdx.x % MX, tz = threadIdx.x / MX;
@@ -505,7 +505,7 @@ __global__ __launch_bounds__(MX * MZT) void k_maxwell(State<T> S, State<T> O, KP
     };
 
     // own positions: (tx, tz + MZT*k); halo: column ncol and row nrow
-    for (int k = 0; k < MZ / MZT; ++k) {
+    for (int k = 0; k < TM / MZT; ++k) {
         int iz = tz + MZT * k;
         if (tx < ncol && iz < nrow) ampere(tx, iz);
     }
@@ -516,7 +516,7 @@ __global__ __launch_bounds__(MX * MZT) void k_maxwell(State<T> S, State<T> O, KP
     }
     __syncthreads();
 
-    for (int k = 0; k < MZ / MZT; ++k) {
+    for (int k = 0; k < TM / MZT; ++k) {
         int iz = tz + MZT * k;
         int ix = tx;
         if (ix >= ncol || iz >= nrow) continue;
@@ -1086,9 +1086,15 @@ void launch_maxwell(const State<T>& S, const State<T>& O, const KParams& p, cons
                            decay, action);
         return;
     }
-    const int ntx = (p.nx + MX - 1) / MX, ntz = (p.nz + MZ - 1) / MZ;
+    const int tm = (variant == 2) ? 4 : (variant == 3) ? 8 : 16;
+    const int ntx = (p.nx + MX - 1) / MX, ntz = (p.nz + tm - 1) / tm;
     dim3 grid(round8(ntx * ntz));
-    hipLaunchKernelGGL(k_maxwell<T>, grid, dim3(MX * MZT), 0, st, S, O, p, ntx, ntz, decay, action);
+    if (tm == 4)
+        hipLaunchKernelGGL((k_maxwell<T, 4>), grid, dim3(MX * MZT), 0, st, S, O, p, ntx, ntz, decay, action);
+    else if (tm == 8)
+        hipLaunchKernelGGL((k_maxwell<T, 8>), grid, dim3(MX * MZT), 0, st, S, O, p, ntx, ntz, decay, action);
+    else
+        hipLaunchKernelGGL((k_maxwell<T, 16>), grid, dim3(MX * MZT), 0, st, S, O, p, ntx, ntz, decay, action);
 }
 
 #define KHS_SWEEP_INST(T, M)                                                                             \

@@ -183,8 +183,8 @@ def test_full_size_step_bitwise():
         assert_bitwise(sim.get(n), o.get(n), f"2048x1024 {n}")
 
 
-@pytest.mark.parametrize("xs,xr,zc,pf,mk,xv", [(64, 1, 16, 0, 0, 1), (128, 4, 64, 1, 1, 0), (256, 16, 32, 0, 1, 1),
-                                               (64, 16, 32, 1, 0, 0), (128, 1, 16, 1, 1, 1), (256, 4, 64, 0, 0, 0)])
+@pytest.mark.parametrize("xs,xr,zc,pf,mk,xv", [(64, 1, 16, 0, 0, 1), (128, 4, 64, 1, 1, 0), (256, 16, 32, 0, 2, 1),
+                                               (64, 16, 32, 1, 3, 0), (128, 1, 16, 1, 1, 1), (256, 4, 64, 0, 2, 0)])
 @pytest.mark.parametrize("periodic", [1, 0])
 def test_tile_variants_bitwise(xs, xr, zc, pf, mk, xv, periodic):
     """Every tile / kernel variant the autotuner may pick gives the same bits."""
