@@ -65,6 +65,18 @@ __device__ __forceinline__ int zp_per(int z, int nz) { return z == nz - 1 ? 0 : 
 __device__ __forceinline__ int xcd_logical(int b, int nb8) { return (b & 7) * (nb8 >> 3) + (b >> 3); }
 __host__ __device__ __forceinline__ int round8(int n) { return (n + 7) & ~7; }
 
+// Element i of an array through a 32-bit byte offset (every array is below
+// 4 GiB): the load can then take the array base from scalar registers and one
+// vector offset shared by all arrays of a cell.
+template <typename T>
+__device__ __forceinline__ T ld32(const T* a, unsigned i) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(a) + i * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void st32(T* a, unsigned i, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(a) + i * (unsigned)sizeof(T)) = v;
+}
+
 // Explicit selects: deterministic sign of zero (NaN propagates through the
 // first operand of kmax(a, floor)).
 template <typename T> __device__ __forceinline__ T kmin(T a, T b) { return (b < a) ? b : a; }
@@ -101,26 +113,26 @@ __device__ __forceinline__ T lorentz_px(const State<T>& S, int sp, int x, int z,
     const int nx = p.nx;
     const int b = S_NA + sp * F_COUNT;
     const int xm = xm_src(x, p), zm = zm_per(z, p.nz);
-    const size_t i = (size_t)x + (size_t)z * nx;
-    const size_t ixm = (size_t)xm + (size_t)z * nx;
-    const size_t izm = (size_t)x + (size_t)zm * nx;
-    const size_t ixmzm = (size_t)xm + (size_t)zm * nx;
-    T n_safe = kmax(S.a[b + F_N][i], T(0.05));
-    T vz = S.a[b + F_PZ][i] / n_safe;
+    const unsigned i = (unsigned)x + (unsigned)z * (unsigned)nx;
+    const unsigned ixm = (unsigned)xm + (unsigned)z * (unsigned)nx;
+    const unsigned izm = (unsigned)x + (unsigned)zm * (unsigned)nx;
+    const unsigned ixmzm = (unsigned)xm + (unsigned)zm * (unsigned)nx;
+    T n_safe = kmax(ld32(S.a[b + F_N], i), T(0.05));
+    T vz = ld32(S.a[b + F_PZ], i) / n_safe;
     T acc[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const T* Ex = S.a[S_EX1 + c];
         const T* By = S.a[S_BY1 + c];
-        T Exn = T(0.5) * (Ex[i] + Ex[ixm]);
-        T Byn = T(0.25) * (By[i] + By[ixm] + By[izm] + By[ixmzm]);
-        acc[c] = S.a[b + F_Q1 + c][i] * (Exn + vz * Byn);
+        T Exn = T(0.5) * (ld32(Ex, i) + ld32(Ex, ixm));
+        T Byn = T(0.25) * (ld32(By, i) + ld32(By, ixm) + ld32(By, izm) + ld32(By, ixmzm));
+        acc[c] = ld32(S.a[b + F_Q1 + c], i) * (Exn + vz * Byn);
     }
     T s = -(acc[0] + acc[1] + acc[2]);
     if (p.warm_T > 0.0) {
         const T* n = S.a[b + F_N];
         const int xp = xp_src(x, p);
-        T dndx = (n[(size_t)xp + (size_t)z * nx] - n[ixm]) / (T(2.0) * T(p.dx));
+        T dndx = (ld32(n, (unsigned)xp + (unsigned)z * (unsigned)nx) - ld32(n, ixm)) / (T(2.0) * T(p.dx));
         s -= T(p.warm_T) * dndx;
     }
     return s;
@@ -160,22 +172,22 @@ __device__ __forceinline__ T lorentz_pz(const State<T>& S, int sp, int x, int z,
     const int nx = p.nx;
     const int b = S_NA + sp * F_COUNT;
     const int xm = xm_src(x, p), zm = zm_per(z, p.nz);
-    const size_t i = (size_t)x + (size_t)z * nx;
-    const size_t ixm = (size_t)xm + (size_t)z * nx;
-    const size_t izm = (size_t)x + (size_t)zm * nx;
-    const size_t ixmzm = (size_t)xm + (size_t)zm * nx;
+    const unsigned i = (unsigned)x + (unsigned)z * (unsigned)nx;
+    const unsigned ixm = (unsigned)xm + (unsigned)z * (unsigned)nx;
+    const unsigned izm = (unsigned)x + (unsigned)zm * (unsigned)nx;
+    const unsigned ixmzm = (unsigned)xm + (unsigned)zm * (unsigned)nx;
     T q[3], ez[3], ezm[3], by[3], byxm[3], byzm[3], byxmzm[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        q[c] = S.a[b + F_Q1 + c][i];
-        ez[c] = S.a[S_EZ1 + c][i];
-        ezm[c] = S.a[S_EZ1 + c][izm];
-        by[c] = S.a[S_BY1 + c][i];
-        byxm[c] = S.a[S_BY1 + c][ixm];
-        byzm[c] = S.a[S_BY1 + c][izm];
-        byxmzm[c] = S.a[S_BY1 + c][ixmzm];
+        q[c] = ld32(S.a[b + F_Q1 + c], i);
+        ez[c] = ld32(S.a[S_EZ1 + c], i);
+        ezm[c] = ld32(S.a[S_EZ1 + c], izm);
+        by[c] = ld32(S.a[S_BY1 + c], i);
+        byxm[c] = ld32(S.a[S_BY1 + c], ixm);
+        byzm[c] = ld32(S.a[S_BY1 + c], izm);
+        byxmzm[c] = ld32(S.a[S_BY1 + c], ixmzm);
     }
-    T s = lorentz_pz_v(S.a[b + F_N][i], S.a[b + F_PX][i], q, ez, ezm, by, byxm, byzm, byxmzm);
+    T s = lorentz_pz_v(ld32(S.a[b + F_N], i), ld32(S.a[b + F_PX], i), q, ez, ezm, by, byxm, byzm, byxmzm);
     if (p.warm_T > 0.0) s -= warm_pz(S, sp, x, z, p);
     return s;
 }
@@ -202,16 +214,16 @@ __device__ __forceinline__ void precession_at(const State<T>& S, int sp, int x, 
     const int nx = p.nx;
     const int b = S_NA + sp * F_COUNT;
     const int zm = zm_per(z, p.nz);
-    const size_t i = (size_t)x + (size_t)z * nx;
-    const size_t izm = (size_t)x + (size_t)zm * nx;
+    const unsigned i = (unsigned)x + (unsigned)z * (unsigned)nx;
+    const unsigned izm = (unsigned)x + (unsigned)zm * (unsigned)nx;
     T q[3], az[3], azm[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        q[c] = S.a[b + F_Q1 + c][i];
-        az[c] = S.a[S_AZ1 + c][i];
-        azm[c] = S.a[S_AZ1 + c][izm];
+        q[c] = ld32(S.a[b + F_Q1 + c], i);
+        az[c] = ld32(S.a[S_AZ1 + c], i);
+        azm[c] = ld32(S.a[S_AZ1 + c], izm);
     }
-    precession_v(S.a[b + F_N][i], S.a[b + F_PZ][i], q, az, azm, T(p.alpha), out);
+    precession_v(ld32(S.a[b + F_N], i), ld32(S.a[b + F_PZ], i), q, az, azm, T(p.alpha), out);
 }
 
 // kernel_ym_currents (YM_Physics.cu:9-60) at real cell (x, z) from fluid F.
@@ -220,9 +232,9 @@ __device__ __forceinline__ void currents_at(const State<T>& F, int x, int z, con
                                             T Jx[3], T Jz[3]) {
     const int nx = p.nx;
     const int xp = xp_src(x, p), zp = zp_per(z, p.nz);
-    const size_t i = (size_t)x + (size_t)z * nx;
-    const size_t ixp = (size_t)xp + (size_t)z * nx;
-    const size_t izp = (size_t)x + (size_t)zp * nx;
+    const unsigned i = (unsigned)x + (unsigned)z * (unsigned)nx;
+    const unsigned ixp = (unsigned)xp + (unsigned)z * (unsigned)nx;
+    const unsigned izp = (unsigned)x + (unsigned)zp * (unsigned)nx;
     T vxr[2], vzt[2], nQx[2][3], nQz[2][3];
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
@@ -230,15 +242,15 @@ __device__ __forceinline__ void currents_at(const State<T>& F, int x, int z, con
         const T* n = F.a[b + F_N];
         const T* px = F.a[b + F_PX];
         const T* pz = F.a[b + F_PZ];
-        T n0 = n[i], np = n[ixp], nzp = n[izp];
-        vxr[s] = T(0.5) * (px[i] / kmax(n0, T(1e-10)) + px[ixp] / kmax(np, T(1e-10)));
-        vzt[s] = T(0.5) * (pz[i] / kmax(n0, T(1e-10)) + pz[izp] / kmax(nzp, T(1e-10)));
+        T n0 = ld32(n, i), np = ld32(n, ixp), nzp = ld32(n, izp);
+        vxr[s] = T(0.5) * (ld32(px, i) / kmax(n0, T(1e-10)) + ld32(px, ixp) / kmax(np, T(1e-10)));
+        vzt[s] = T(0.5) * (ld32(pz, i) / kmax(n0, T(1e-10)) + ld32(pz, izp) / kmax(nzp, T(1e-10)));
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const T* Q = F.a[b + F_Q1 + c];
-            T q0 = Q[i];
-            nQx[s][c] = T(0.5) * (n0 * q0 + np * Q[ixp]);
-            nQz[s][c] = T(0.5) * (n0 * q0 + nzp * Q[izp]);
+            T q0 = ld32(Q, i);
+            nQx[s][c] = T(0.5) * (n0 * q0 + np * ld32(Q, ixp));
+            nQz[s][c] = T(0.5) * (n0 * q0 + nzp * ld32(Q, izp));
         }
     }
 #pragma unroll
@@ -263,12 +275,12 @@ struct EmRow {
 };
 
 template <typename T>
-__device__ __forceinline__ void em_load_row(const State<T>& S, size_t i, size_t ixm, EmRow<T>& r) {
-    r.ex = S.a[E_EX][i]; r.exm = S.a[E_EX][ixm];
-    r.ey = S.a[E_EY][i]; r.ez = S.a[E_EZ][i];
-    r.bx = S.a[E_BX][i];
-    r.bz = S.a[E_BZ][i]; r.bzxm = S.a[E_BZ][ixm];
-    r.by = S.a[E_BY][i]; r.byxm = S.a[E_BY][ixm];
+__device__ __forceinline__ void em_load_row(const State<T>& S, unsigned i, unsigned ixm, EmRow<T>& r) {
+    r.ex = ld32(S.a[E_EX], i); r.exm = ld32(S.a[E_EX], ixm);
+    r.ey = ld32(S.a[E_EY], i); r.ez = ld32(S.a[E_EZ], i);
+    r.bx = ld32(S.a[E_BX], i);
+    r.bz = ld32(S.a[E_BZ], i); r.bzxm = ld32(S.a[E_BZ], ixm);
+    r.by = ld32(S.a[E_BY], i); r.byxm = ld32(S.a[E_BY], ixm);
 }
 
 // kernel_compute_lorentz (emhdgpu2d/main_coupled.cu:112-136) from values:
@@ -343,13 +355,13 @@ struct YMModel {                    // ymgpu2d: n, px, pz, Q1..3 per beam
         T ez[3], by[3], byxm[3], az[3];
     };
     template <typename T>
-    __device__ static void load_row(const State<T>& S, size_t i, size_t ixm, Row<T>& r) {
+    __device__ static void load_row(const State<T>& S, unsigned i, unsigned ixm, Row<T>& r) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            r.ez[c] = S.a[S_EZ1 + c][i];
-            r.by[c] = S.a[S_BY1 + c][i];
-            r.byxm[c] = S.a[S_BY1 + c][ixm];
-            r.az[c] = S.a[S_AZ1 + c][i];
+            r.ez[c] = ld32(S.a[S_EZ1 + c], i);
+            r.by[c] = ld32(S.a[S_BY1 + c], i);
+            r.byxm[c] = ld32(S.a[S_BY1 + c], ixm);
+            r.az[c] = ld32(S.a[S_AZ1 + c], i);
         }
     }
     template <typename T>
@@ -399,7 +411,7 @@ struct EMModel {                    // emhdgpu2d: n, px, py, pz; sources in both
     template <typename T>
     using Row = EmRow<T>;
     template <typename T>
-    __device__ static void load_row(const State<T>& S, size_t i, size_t ixm, Row<T>& r) {
+    __device__ static void load_row(const State<T>& S, unsigned i, unsigned ixm, Row<T>& r) {
         em_load_row(S, i, ixm, r);
     }
     template <typename T>

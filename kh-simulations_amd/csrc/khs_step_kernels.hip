@@ -70,10 +70,10 @@ __global__ __launch_bounds__(XB) void k_fct_x(State<T> S, Stage<T> out, KParams 
     // x0-3 .. x0+XB+2 (position q <-> extended x = x0-3+q).
     for (int q = tid; q < XW; q += XB) {
         const int xr = map_x(x0 - 3 + q, p);
-        const size_t i = row + xr;
+        const unsigned i = (unsigned)row + (unsigned)xr;
         T v[NV];
 #pragma unroll
-        for (int k = 0; k < NV; ++k) v[k] = S.a[b + k][i];
+        for (int k = 0; k < NV; ++k) v[k] = ld32(S.a[b + k], i);
         T n_safe = kmax(v[0], T(0.05));
         sU[q] = v[M::IPX] / n_safe;                         // update_velocities
         T src[M::NXS];
@@ -138,7 +138,7 @@ __global__ __launch_bounds__(XB) void k_fct_x(State<T> S, Stage<T> out, KParams 
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             T r = sLNR[v][q] - (sV[v][q] - sV[v][q - 1]);
-            out.a[sp * NV + v][row + x] = r;
+            st32(out.a[sp * NV + v], (unsigned)row + (unsigned)x, r);
         }
     }
     } else {
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(XB) void k_fct_x(State<T> S, Stage<T> out, KParams 
         __syncthreads();
         if (x < p.nx) {
             const int q = 3 + tid;
-            out.a[sp * NV + v][row + x] = sLNR[0][q] - (sFL[q] - sFL[q - 1]);
+            st32(out.a[sp * NV + v], (unsigned)row + (unsigned)x, sLNR[0][q] - (sFL[q] - sFL[q - 1]));
         }
         __syncthreads();              // sLOR / sLNR / sFL are rewritten for the next variable
     }
@@ -300,12 +300,12 @@ __global__ __launch_bounds__(ZB_X * ZB_C) void k_fct_z(State<T> S, Stage<T> X, S
     const bool wall = !p.periodic_x && (x == 0 || x == nx - 1);
 
     auto load = [&](int r, Row& row, T (&F)[NV], T (&V)[NV]) {
-        const size_t i = (size_t)x + (size_t)r * nx;
-        M::load_row(S, i, (size_t)xm + (size_t)r * nx, row);
+        const unsigned i = (unsigned)x + (unsigned)r * (unsigned)nx;
+        M::load_row(S, i, (unsigned)xm + (unsigned)r * (unsigned)nx, row);
 #pragma unroll
-        for (int v = 0; v < NV; ++v) F[v] = S.a[b + v][i];
+        for (int v = 0; v < NV; ++v) F[v] = ld32(S.a[b + v], i);
 #pragma unroll
-        for (int v = 0; v < NV; ++v) V[v] = X.a[sp * NV + v][i];
+        for (int v = 0; v < NV; ++v) V[v] = ld32(X.a[sp * NV + v], i);
     };
 
     // rows outside [0, nz) are the clamped end rows (FCT_Sweeps.cu:52,73,78)
@@ -327,8 +327,13 @@ __global__ __launch_bounds__(ZB_X * ZB_C) void k_fct_z(State<T> S, Stage<T> X, S
         if (PF) load(rn, nrow, nF, nV);
         T u, Sv[NV], out[NV];
         M::z_src_rows(F, cur, prev, S, sp, x, rc, p, u, Sv);
-        if (zp.push(e, z0, V, u, Sv, dt, dtdz, out))
-            store_fluid<T, NV, M::IPX>(O, b, (size_t)x + (size_t)(e - 3) * nx, wall, out);
+        if (zp.push(e, z0, V, u, Sv, dt, dtdz, out)) {
+            if (out[0] < T(0.05)) out[0] = T(0.05);          // kernel_enforce_boundaries
+            if (wall) out[M::IPX] = T(0.0);
+            const unsigned io = (unsigned)x + (unsigned)(e - 3) * (unsigned)nx;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) st32(O.a[b + v], io, out[v]);
+        }
         if (!PF) load(rn, nrow, nF, nV);
         // advance: row z-1 of the next row is this row unless the clamp holds
         // the row index at an end
@@ -472,20 +477,20 @@ __global__ __launch_bounds__(MX * MZT) void k_maxwell(State<T> S, State<T> O, KP
     auto ampere = [&](int ix, int iz) {
         int xr = (ix < ncol) ? x0 + ix : xp_src(x0 + ncol - 1, p);
         int zr = (iz < nrow) ? z0 + iz : zp_per(z0 + nrow - 1, nz);
-        const size_t i = (size_t)xr + (size_t)zr * nx;
+        const unsigned i = (unsigned)xr + (unsigned)zr * (unsigned)nx;
         const int zm = zm_per(zr, nz);
         const int xm = p.periodic_x ? (xr + nx - 1) % nx : (xr == 0 ? 0 : xr - 1);
-        const size_t izm = (size_t)xr + (size_t)zm * nx, ixm = (size_t)xm + (size_t)zr * nx;
+        const unsigned izm = (unsigned)xr + (unsigned)zm * (unsigned)nx, ixm = (unsigned)xm + (unsigned)zr * (unsigned)nx;
         T Jx[3], Jz[3];
         currents_at(O, xr, zr, p, Jx, Jz);
         const bool wall = !p.periodic_x && (xr == 0 || xr == nx - 1);   // YM_Fluid.cu:223-226
         T By[3], Byzm[3], Byxm[3], Az[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            By[c] = S.a[S_BY1 + c][i];
-            Byzm[c] = S.a[S_BY1 + c][izm];
-            Byxm[c] = S.a[S_BY1 + c][ixm];
-            Az[c] = S.a[S_AZ1 + c][i];
+            By[c] = ld32(S.a[S_BY1 + c], i);
+            Byzm[c] = ld32(S.a[S_BY1 + c], izm);
+            Byxm[c] = ld32(S.a[S_BY1 + c], ixm);
+            Az[c] = ld32(S.a[S_AZ1 + c], i);
         }
         T cr[3] = {Az[1] * By[2] - Az[2] * By[1],
                    Az[2] * By[0] - Az[0] * By[2],
@@ -493,13 +498,13 @@ __global__ __launch_bounds__(MX * MZT) void k_maxwell(State<T> S, State<T> O, KP
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             T dBdz = (By[c] - Byzm[c]) / dz;
-            T ex = S.a[S_EX1 + c][i];
+            T ex = ld32(S.a[S_EX1 + c], i);
             sEx[c][iz][ix] = ex + dt * (c2 * (-dBdz - aYM * cr[c]) - Jx[c] / eps0);
         }
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             T dBdx = (By[c] - Byxm[c]) / dx;
-            T ez = wall ? T(0.0) : S.a[S_EZ1 + c][i];
+            T ez = wall ? T(0.0) : ld32(S.a[S_EZ1 + c], i);
             sEz[c][iz][ix] = ez + dt * (c2 * dBdx - Jz[c] / eps0);
         }
     };
@@ -521,12 +526,12 @@ __global__ __launch_bounds__(MX * MZT) void k_maxwell(State<T> S, State<T> O, KP
         int ix = tx;
         if (ix >= ncol || iz >= nrow) continue;
         const int xr = x0 + ix, zr = z0 + iz;
-        const size_t i = (size_t)xr + (size_t)zr * nx;
+        const unsigned i = (unsigned)xr + (unsigned)zr * (unsigned)nx;
         // kernel_ym_potential (YM_Physics.cu:589-599)
         T Az[3], Ex[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            Az[c] = S.a[S_AZ1 + c][i];
+            Az[c] = ld32(S.a[S_AZ1 + c], i);
             if (c > 0 || !p.freeze_az1) Az[c] = Az[c] - dt * sEz[c][iz][ix];
             Ex[c] = sEx[c][iz][ix];
         }
@@ -546,15 +551,15 @@ __global__ __launch_bounds__(MX * MZT) void k_maxwell(State<T> S, State<T> O, KP
         for (int c = 0; c < 3; ++c) {
             T dEx = (sEx[c][iz + 1][ix] - Ex[c]) / dz;
             T dEz = (sEz[c][iz][ixp] - sEz[c][iz][ix]) / dx;
-            T by = S.a[S_BY1 + c][i];
+            T by = ld32(S.a[S_BY1 + c], i);
             T byn = by - dt * (dEx - dEz - aYM * cr[c]);
             T exn = Ex[c], ezn = sEz[c][iz][ix], azn = Az[c];
             if (c > 0 && act == 1) { byn *= dec; exn *= dec; ezn *= dec; azn *= dec; }
             if (c > 0 && act == 2) { byn = T(0.0); exn = T(0.0); ezn = T(0.0); azn = T(0.0); }
-            O.a[S_BY1 + c][i] = byn;
-            O.a[S_EX1 + c][i] = exn;
-            O.a[S_EZ1 + c][i] = ezn;
-            if (c > 0 || !p.freeze_az1) O.a[S_AZ1 + c][i] = azn;
+            st32(O.a[S_BY1 + c], i, byn);
+            st32(O.a[S_EX1 + c], i, exn);
+            st32(O.a[S_EZ1 + c], i, ezn);
+            if (c > 0 || !p.freeze_az1) st32(O.a[S_AZ1 + c], i, azn);
         }
     }
 }
