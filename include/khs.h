@@ -126,6 +126,7 @@ typedef struct khs_config {
     double   spectrum_amp;
     uint64_t spectrum_seed;
     double   em_perturb_amp;   /* EMHD das_kaw perturbation (argv[2] as double, default 0.05); set by perturb_amp */
+    int      z_groups;         /* z sweep: -1 = autotuned (default), 0 = all variables per lane, 1 = two variable groups on two waves */
     int      xz_variant;       /* fused sweep kernel variant: -1 = autotuned (default), 0..3 */
     int      x_serial;         /* x sweep phases per variable: -1 = autotuned (default), 0 = all at once, 1 = one at a time */
     int      maxwell_kernel;   /* YM Maxwell: -1 = autotuned (default), 0/2/3 = 64x16/64x4/64x8 LDS tiles, 1 = per cell */
@@ -264,9 +265,10 @@ const char* khs_kernel_name(int slot);
  * workgroup, z-sweep chunk height (rows), z-sweep next-row prefetch (0/1),
  * YM Maxwell kernel (0/2/3 = 64x16/64x4/64x8 LDS tiles, 1 = per cell), x-sweep variables one at a
  * time (0/1), fused-sweep variant (0: one wave per 64 columns; 1/2/3: four
- * waves per 128 columns with 16/32/64-row chunks)}; chosen by the autotuner
- * on the first step unless autotune = 0. */
-int khs_get_tuning(khs_ctx* ctx, int out[7]);
+ * waves per 128 columns with 16/32/64-row chunks), z-sweep variables split
+ * over two waves (0/1)}; chosen by the autotuner on the first step unless
+ * autotune = 0. */
+int khs_get_tuning(khs_ctx* ctx, int out[8]);
 
 /* ---------------- batches of runs (parameter sweeps) ---------------- */
 /* No counterpart in the reference, which runs one configuration per process

@@ -77,6 +77,7 @@ void config_defaults(khs_config& c) {
     c.maxwell_kernel = -1;
     c.x_serial = -1;
     c.xz_variant = -1;
+    c.z_groups = -1;
 }
 
 static int parse_int(const std::string& k, const std::string& v) {
@@ -158,6 +159,7 @@ bool config_set(khs_config& c, const std::string& key, const std::string& val) {
         else if (val == "spectrum" || val == "1") c.emhd_init = 1;
         else throw Error(KHS_ERR_CONFIG, "emhd_init must be daskaw or spectrum");
     }
+    else if (key == "z_groups")          c.z_groups = parse_int(key, val);
     else if (key == "xz_variant")        c.xz_variant = parse_int(key, val);
     else if (key == "x_serial")          c.x_serial = parse_int(key, val);
     else if (key == "maxwell_kernel")    c.maxwell_kernel = parse_int(key, val);
@@ -235,6 +237,7 @@ static void config_derive_emhd(khs_config& c) {
     if (c.maxwell_kernel < -1 || c.maxwell_kernel > 3) throw Error(KHS_ERR_CONFIG, "maxwell_kernel must be -1..3");
     if (c.x_serial < -1 || c.x_serial > 1) throw Error(KHS_ERR_CONFIG, "x_serial must be -1, 0 or 1");
     if (c.xz_variant < -1 || c.xz_variant > 3) throw Error(KHS_ERR_CONFIG, "xz_variant must be -1..3");
+    if (c.z_groups < -1 || c.z_groups > 1) throw Error(KHS_ERR_CONFIG, "z_groups must be -1, 0 or 1");
     if (c.emhd_init == 1 && (c.spectrum_kmax < 1 || 2 * c.spectrum_kmax >= std::min(c.nx, c.nz)))
         throw Error(KHS_ERR_CONFIG, "spectrum_kmax must satisfy 1 <= K < min(nx, nz)/2");
 }
@@ -296,6 +299,7 @@ void config_derive(khs_config& c) {
     if (c.maxwell_kernel < -1 || c.maxwell_kernel > 3) throw Error(KHS_ERR_CONFIG, "maxwell_kernel must be -1..3");
     if (c.x_serial < -1 || c.x_serial > 1) throw Error(KHS_ERR_CONFIG, "x_serial must be -1, 0 or 1");
     if (c.xz_variant < -1 || c.xz_variant > 3) throw Error(KHS_ERR_CONFIG, "xz_variant must be -1..3");
+    if (c.z_groups < -1 || c.z_groups > 1) throw Error(KHS_ERR_CONFIG, "z_groups must be -1, 0 or 1");
     if (c.run_mode < 0 || c.run_mode > 6)
         throw Error(KHS_ERR_CONFIG, "run_mode must be 0..6");
 }

@@ -217,12 +217,13 @@ struct Ctx {
         const int fv = cfg.x_serial;             // -1 = tuned
         for (int v = 0; v < kNumXVariants; ++v)
             if ((fx < 0 || v % 3 == fx) && (fr < 0 || (v / 3) % 3 == fr) && (fv < 0 || v / 9 == fv)) xc.push_back(v);
+        const int fg = cfg.z_groups;             // -1 = tuned
         for (int v = 0; v < kNumZVariants; ++v)
-            if ((fz < 0 || v % 3 == fz) && (fp < 0 || v / 3 == fp)) zc.push_back(v);
+            if ((fz < 0 || v % 3 == fz) && (fp < 0 || (v / 3) % 2 == fp) && (fg < 0 || v / 6 == fg)) zc.push_back(v);
         // without tuning: the forced shape, else 256-cell segments and 32-row
         // chunks with prefetch in FP32 (registers to spare), without in FP64
         xvar = ((fx >= 0) ? fx : 2) + 3 * ((fr >= 0) ? fr : 1) + 9 * ((fv >= 0) ? fv : 1);
-        zvar = ((fz >= 0) ? fz : 1) + 3 * ((fp >= 0) ? fp : (fp32 ? 1 : 0));
+        zvar = ((fz >= 0) ? fz : 1) + 3 * ((fp >= 0) ? fp : (fp32 ? 1 : 0)) + 6 * ((fg >= 0) ? fg : 0);
         mvar = (cfg.maxwell_kernel >= 0) ? cfg.maxwell_kernel : 0;
         xzvar = (cfg.xz_variant >= 0) ? cfg.xz_variant : 2;
         if (!cfg.autotune) return;
@@ -445,6 +446,7 @@ static std::unique_ptr<Ctx> make_ctx(const khs_config& in) {
     ctx->xvar += 3 * (c.x_rows == 1 ? 0 : c.x_rows == 16 ? 2 : 1) + 9 * (c.x_serial == 0 ? 0 : 1);
     if (c.z_chunk) ctx->zvar = c.z_chunk == 16 ? 0 : c.z_chunk == 32 ? 1 : 2;
     if (c.z_prefetch == 1) ctx->zvar = ctx->zvar % 3 + 3;
+    if (c.z_groups == 1) ctx->zvar += 6;
     if (c.maxwell_kernel >= 0) ctx->mvar = c.maxwell_kernel;
     if (c.xz_variant >= 0) ctx->xzvar = c.xz_variant;
 
@@ -1183,7 +1185,7 @@ int khs_kernel_times(khs_ctx* ctx, double* ms, int* launches, int nslots) {
     });
 }
 
-int khs_get_tuning(khs_ctx* ctx, int out[7]) {
+int khs_get_tuning(khs_ctx* ctx, int out[8]) {
     CTX_OR_FAIL(ctx);
     if (!out) { g_last_error = "null argument"; return KHS_ERR_ARG; }
     Ctx& c = *ctx->impl;
@@ -1191,10 +1193,11 @@ int khs_get_tuning(khs_ctx* ctx, int out[7]) {
     out[0] = xs[c.xvar % 3];
     out[1] = xr[(c.xvar / 3) % 3];
     out[2] = zs[c.zvar % 3];
-    out[3] = c.zvar / 3;
+    out[3] = (c.zvar / 3) % 2;
     out[4] = c.mvar;
     out[5] = c.xvar / 9;
     out[6] = c.xzvar;
+    out[7] = c.zvar / 6;
     return KHS_OK;
 }
 

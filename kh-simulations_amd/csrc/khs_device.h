@@ -390,6 +390,50 @@ struct YMModel {                    // ymgpu2d: n, px, pz, Q1..3 per beam
         precession_v(F[F_N], F[F_PZ], q, c.az, m.az, T(p.alpha), o);
         s[F_Q1] = o[0]; s[F_Q2] = o[1]; s[F_Q3] = o[2];
     }
+    // Variable groups of the z sweep (one wave each): G = 0 -> n, px, pz
+    // (source on pz from the Lorentz force), G = 1 -> Q1..Q3 (precession).
+    static constexpr int NG = 3;
+    template <int G, typename T>
+    __device__ static void load_row_g(const State<T>& S, unsigned i, unsigned ixm, Row<T>& r) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (G == 0) {
+                r.ez[c] = ld32(S.a[S_EZ1 + c], i);
+                r.by[c] = ld32(S.a[S_BY1 + c], i);
+                r.byxm[c] = ld32(S.a[S_BY1 + c], ixm);
+            } else {
+                r.az[c] = ld32(S.a[S_AZ1 + c], i);
+            }
+        }
+    }
+    template <int G, typename T>
+    __device__ static void sel_row_g(bool take, const Row<T>& a, Row<T>& r) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (G == 0) {
+                r.ez[c] = take ? a.ez[c] : r.ez[c];
+                r.by[c] = take ? a.by[c] : r.by[c];
+                r.byxm[c] = take ? a.byxm[c] : r.byxm[c];
+            } else {
+                r.az[c] = take ? a.az[c] : r.az[c];
+            }
+        }
+    }
+    template <int G, typename T>
+    __device__ static void z_src_rows_g(const T (&F)[NV], const Row<T>& c, const Row<T>& m, const State<T>& S,
+                                        int sp, int x, int z, const KParams& p, T& u, T (&s)[NG]) {
+        const T n_safe = kmax(F[F_N], T(0.05));
+        u = F[F_PZ] / n_safe;
+        const T q[3] = {F[F_Q1], F[F_Q2], F[F_Q3]};
+        if (G == 0) {
+            s[0] = T(0);
+            s[1] = T(0);
+            s[2] = lorentz_pz_v(F[F_N], F[F_PX], q, c.ez, m.ez, c.by, c.byxm, m.by, m.byxm);
+            if (p.warm_T > 0.0) s[2] -= warm_pz(S, sp, x, z, p);
+        } else {
+            precession_v(F[F_N], F[F_PZ], q, c.az, m.az, T(p.alpha), s);
+        }
+    }
 };
 
 struct EMModel {                    // emhdgpu2d: n, px, py, pz; sources in both sweeps
@@ -430,6 +474,27 @@ struct EMModel {                    // emhdgpu2d: n, px, py, pz; sources in both
         T f[3];
         em_lorentz_v(c, m, F[EV_N], F[EV_PX], F[EV_PY], F[EV_PZ], T(sp ? p.q1 : p.q0), f);
         s[EV_N] = T(0); s[EV_PX] = f[0]; s[EV_PY] = f[1]; s[EV_PZ] = f[2];
+    }
+    // variable groups: G = 0 -> n, px; G = 1 -> py, pz (both need the full
+    // Lorentz force, which is cheap)
+    static constexpr int NG = 2;
+    template <int G, typename T>
+    __device__ static void load_row_g(const State<T>& S, unsigned i, unsigned ixm, Row<T>& r) {
+        em_load_row(S, i, ixm, r);
+    }
+    template <int G, typename T>
+    __device__ static void sel_row_g(bool take, const Row<T>& a, Row<T>& r) {
+        sel_row(take, a, r);
+    }
+    template <int G, typename T>
+    __device__ static void z_src_rows_g(const T (&F)[NV], const Row<T>& c, const Row<T>& m, const State<T>&,
+                                        int sp, int, int, const KParams& p, T& u, T (&s)[NG]) {
+        const T n_safe = kmax(F[EV_N], T(0.05));
+        u = F[EV_PZ] / n_safe;
+        T f[3];
+        em_lorentz_v(c, m, F[EV_N], F[EV_PX], F[EV_PY], F[EV_PZ], T(sp ? p.q1 : p.q0), f);
+        if (G == 0) { s[0] = T(0); s[1] = f[0]; }
+        else { s[0] = f[1]; s[1] = f[2]; }
     }
 };
 

@@ -9,9 +9,10 @@ namespace khs {
 // Tile variants (bit-identical results): x segment 64/128/256 cells
 // (variant % 3) with 1/4/16 rows per workgroup ((variant / 3) % 3), all
 // variables or one at a time through LDS (variant / 9); z chunk 16/32/64 rows
-// (variant % 3) without / with next-row prefetch (variant / 3).
+// (variant % 3) without / with next-row prefetch ((variant / 3) % 2), one lane
+// per column or two waves per column (variable groups, variant / 6).
 constexpr int kNumXVariants = 18;
-constexpr int kNumZVariants = 6;
+constexpr int kNumZVariants = 12;
 constexpr int kNumMaxwellVariants = 4;
 // M: YMModel (ymgpu2d two-beam SU(2) fluid) or EMModel (emhdgpu2d Abelian fluid)
 template <typename T, typename M>

@@ -345,6 +345,85 @@ __global__ __launch_bounds__(ZB_X * ZB_C) void k_fct_z(State<T> S, Stage<T> X, S
     }
 }
 
+// The same z march with the variables split into two groups (model's
+// NG = NV/2 each) on two waves: half the pipeline state per lane, so more
+// waves fit; both waves of a column read the cell's fluid for the sources
+// (the second read hits L2).  A workgroup is 64 columns x 2 chunks x 2 groups.
+template <typename T, int TZ, int PF, typename M, int G>
+__device__ __forceinline__ void z_march_g(const State<T>& S, const Stage<T>& X, const State<T>& O,
+                                          const KParams& p, int sp, int x, int z0) {
+    constexpr int NV = M::NV, NG = M::NG;
+    typedef typename M::template Row<T> Row;
+    const int nx = p.nx, nz = p.nz;
+    const int b = M::base(sp);
+    const int xm = xm_src(x, p);
+    const T dt = T(p.dt), dtdz = T(p.dtdz);
+    const int zend = min(z0 + TZ, nz);
+    const int elast = zend + 2;
+    const bool wall = !p.periodic_x && (x == 0 || x == nx - 1);
+
+    auto load = [&](int r, Row& row, T (&F)[NV], T (&V)[NG]) {
+        const unsigned i = (unsigned)x + (unsigned)r * (unsigned)nx;
+        M::template load_row_g<G>(S, i, (unsigned)xm + (unsigned)r * (unsigned)nx, row);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) F[v] = ld32(S.a[b + v], i);
+#pragma unroll
+        for (int j = 0; j < NG; ++j) V[j] = ld32(X.a[sp * NV + G * NG + j], i);
+    };
+
+    int rc = clampi(z0 - 3, 0, nz - 1);
+    Row prev, cur;
+    T F[NV], V[NG];
+    {
+        T Fd[NV], Vd[NG];
+        load(zm_per(rc, nz), prev, Fd, Vd);
+    }
+    load(rc, cur, F, V);
+    ZPipe<T, NG> zp;
+    zp.Um = T(0);
+    zp.MUm = T(0);
+    for (int e = z0 - 3; e <= elast; ++e) {
+        const int rn = clampi(min(e + 1, elast), 0, nz - 1);
+        Row nrow;
+        T nF[NV], nV[NG];
+        if (PF) load(rn, nrow, nF, nV);
+        T u, Sv[NG], out[NG];
+        M::template z_src_rows_g<G>(F, cur, prev, S, sp, x, rc, p, u, Sv);
+        if (zp.push(e, z0, V, u, Sv, dt, dtdz, out)) {
+            if (G == 0) {
+                if (out[0] < T(0.05)) out[0] = T(0.05);      // kernel_enforce_boundaries
+                if (wall) out[M::IPX] = T(0.0);
+            }
+            const unsigned io = (unsigned)x + (unsigned)(e - 3) * (unsigned)nx;
+#pragma unroll
+            for (int j = 0; j < NG; ++j) st32(O.a[b + G * NG + j], io, out[j]);
+        }
+        if (!PF) load(rn, nrow, nF, nV);
+        M::template sel_row_g<G>(rn != rc, cur, prev);
+        cur = nrow;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) F[v] = nF[v];
+#pragma unroll
+        for (int j = 0; j < NG; ++j) V[j] = nV[j];
+        rc = rn;
+    }
+}
+
+template <typename T, int TZ, int PF, typename M>
+__global__ __launch_bounds__(256) void k_fct_zg(State<T> S, Stage<T> X, State<T> O, KParams p, int sp0, int nsp,
+                                                int ntx, int ncg) {
+    int lt = xcd_logical(blockIdx.x, gridDim.x);
+    if (lt >= nsp * ntx * ncg) return;
+    const int sp = sp0 + lt % nsp;
+    lt /= nsp;
+    const int x = (lt % ntx) * 64 + (int)(threadIdx.x & 63);
+    const int w = (int)(threadIdx.x >> 6);
+    const int z0 = ((lt / ntx) * 2 + (w >> 1)) * TZ;
+    if (x >= p.nx || z0 >= p.nz) return;
+    if ((w & 1) == 0) z_march_g<T, TZ, PF, M, 0>(S, X, O, p, sp, x, z0);
+    else z_march_g<T, TZ, PF, M, 1>(S, X, O, p, sp, x, z0);
+}
+
 // ==========================================================================
 // fused x + z sweep
 // ==========================================================================
@@ -854,27 +933,35 @@ void launch_fct_x(const State<T>& S, const Stage<T>& out, const KParams& p, int 
 
 template <typename T, int TZ, int PF, typename M>
 static void launch_fct_z_t(const State<T>& S, const Stage<T>& X, const State<T>& O, const KParams& p,
-                           int sp0, int nsp, hipStream_t st) {
+                           int sp0, int nsp, int grouped, hipStream_t st) {
     const int nchunk = (p.nz + TZ - 1) / TZ;
-    const int ncg = (nchunk + ZB_C - 1) / ZB_C;
     const int ntx = (p.nx + ZB_X - 1) / ZB_X;
+    if (grouped) {
+        const int ncg = (nchunk + 1) / 2;
+        hipLaunchKernelGGL((k_fct_zg<T, TZ, PF, M>), dim3(round8(nsp * ntx * ncg)), dim3(256), 0, st, S, X, O, p,
+                           sp0, nsp, ntx, ncg);
+        return;
+    }
+    const int ncg = (nchunk + ZB_C - 1) / ZB_C;
     dim3 grid(round8(nsp * ntx * ncg));
     hipLaunchKernelGGL((k_fct_z<T, TZ, PF, M>), grid, dim3(ZB_X * ZB_C), 0, st, S, X, O, p, sp0, nsp, ntx, ncg);
 }
 
 // Variant v: chunk height 16 / 32 / 64 rows (v % 3; halo rows re-read per
 // chunk vs. independent column marches in flight), prefetch of the next row
-// off / on (v / 3).
+// off / on ((v / 3) % 2), all variables on one lane / split into two groups
+// on two waves (v / 6).
 template <typename T, typename M>
 void launch_fct_z(const State<T>& S, const Stage<T>& X, const State<T>& O, const KParams& p,
                   int sp0, int nsp, int variant, hipStream_t st) {
-    switch (variant) {
-        case 0: launch_fct_z_t<T, 16, 0, M>(S, X, O, p, sp0, nsp, st); break;
-        case 1: launch_fct_z_t<T, 32, 0, M>(S, X, O, p, sp0, nsp, st); break;
-        case 2: launch_fct_z_t<T, 64, 0, M>(S, X, O, p, sp0, nsp, st); break;
-        case 3: launch_fct_z_t<T, 16, 1, M>(S, X, O, p, sp0, nsp, st); break;
-        case 5: launch_fct_z_t<T, 64, 1, M>(S, X, O, p, sp0, nsp, st); break;
-        default: launch_fct_z_t<T, 32, 1, M>(S, X, O, p, sp0, nsp, st); break;
+    const int gr = variant / 6;
+    switch (variant % 6) {
+        case 0: launch_fct_z_t<T, 16, 0, M>(S, X, O, p, sp0, nsp, gr, st); break;
+        case 1: launch_fct_z_t<T, 32, 0, M>(S, X, O, p, sp0, nsp, gr, st); break;
+        case 2: launch_fct_z_t<T, 64, 0, M>(S, X, O, p, sp0, nsp, gr, st); break;
+        case 3: launch_fct_z_t<T, 16, 1, M>(S, X, O, p, sp0, nsp, gr, st); break;
+        case 5: launch_fct_z_t<T, 64, 1, M>(S, X, O, p, sp0, nsp, gr, st); break;
+        default: launch_fct_z_t<T, 32, 1, M>(S, X, O, p, sp0, nsp, gr, st); break;
     }
 }
 

@@ -122,10 +122,11 @@ def test_step_bitwise(case):
         assert_bitwise(sim.get(n), o.get(n), f"{case} {n}")
 
 
-@pytest.mark.parametrize("zc,pf", [(16, 1), (64, 0), (32, 1)])
-def test_tile_variants_bitwise(zc, pf):
+@pytest.mark.parametrize("zc,pf,zg", [(16, 1, 0), (64, 0, 1), (32, 1, 1), (16, 0, 1)])
+def test_tile_variants_bitwise(zc, pf, zg):
     nx, nz, periodic, nsp = 130, 70, 0, 2
-    keys = em_keys(nx, nz, periodic, nsp, z_chunk=zc, z_prefetch=pf, x_segment=64, x_rows=16, autotune=0)
+    keys = em_keys(nx, nz, periodic, nsp, z_chunk=zc, z_prefetch=pf, z_groups=zg, x_segment=64, x_rows=16,
+                   autotune=0)
     cfg, sim, o = em_pair(em_random(nx, nz, nsp, seed=4, smooth=True), **keys)
     t = sim.tuning()
     assert (t["x_segment"], t["x_rows"], t["z_chunk"], t["z_prefetch"]) == (64, 16, zc, pf)

@@ -183,17 +183,18 @@ def test_full_size_step_bitwise():
         assert_bitwise(sim.get(n), o.get(n), f"2048x1024 {n}")
 
 
-@pytest.mark.parametrize("xs,xr,zc,pf,mk,xv", [(64, 1, 16, 0, 0, 1), (128, 4, 64, 1, 1, 0), (256, 16, 32, 0, 2, 1),
-                                               (64, 16, 32, 1, 3, 0), (128, 1, 16, 1, 1, 1), (256, 4, 64, 0, 2, 0)])
+@pytest.mark.parametrize("xs,xr,zc,pf,mk,xv,zg", [(64, 1, 16, 0, 0, 1, 0), (128, 4, 64, 1, 1, 0, 1), (256, 16, 32, 0, 2, 1, 1),
+                                                  (64, 16, 32, 1, 3, 0, 0), (128, 1, 16, 1, 1, 1, 1), (256, 4, 64, 0, 2, 0, 1)])
 @pytest.mark.parametrize("periodic", [1, 0])
-def test_tile_variants_bitwise(xs, xr, zc, pf, mk, xv, periodic):
+def test_tile_variants_bitwise(xs, xr, zc, pf, mk, xv, zg, periodic):
     """Every tile / kernel variant the autotuner may pick gives the same bits."""
     keys = grid(300, 136, periodic, warm_T=0.01, x_segment=xs, x_rows=xr, z_chunk=zc, z_prefetch=pf,
-                maxwell_kernel=mk, x_serial=xv, autotune=0, xi_sponge=8.0, xi_cut=12.0)
+                maxwell_kernel=mk, x_serial=xv, z_groups=zg, autotune=0, xi_sponge=8.0, xi_cut=12.0)
     st = random_state(300, 136, seed=31, smooth=True)
     cfg, sim, o = make_pair(st, **keys)
     t = sim.tuning()
-    assert t == dict(x_segment=xs, x_rows=xr, z_chunk=zc, z_prefetch=pf, maxwell=mk, x_serial=xv, xz_variant=t["xz_variant"])
+    assert t == dict(x_segment=xs, x_rows=xr, z_chunk=zc, z_prefetch=pf, maxwell=mk, x_serial=xv,
+                     xz_variant=t["xz_variant"], z_groups=zg)
     o.sources()
     sim.step(3)
     o.step(3)
