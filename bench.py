@@ -60,10 +60,33 @@ STEP_ARRAYS = 47
 # HBM bytes per launch of the FP64 step kernels on this workload, measured with
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (separate runs), each scaled
 # by what the counter reads for a plain 256 MiB device copy (fetch x2.0,
-# write x1.0); summary under profiles/.  Keyed by kernel and tile variant.
-PMC_TRAFFIC_MB = {("k_fct_x", 64): 503.8, ("k_fct_x", 128): 503.8, ("k_fct_x", 256): 503.8,
-                  ("k_fct_z", 16): 1056.5, ("k_fct_z", 32): 948.8, ("k_fct_z", 64): 807.9,
-                  ("k_maxwell", None): 846.7}
+# write x1.0); summary under profiles/.  Keyed by kernel instance.
+PMC_TRAFFIC_MB = {
+    "k_fct_x<double, 64, YMModel, 0>": 511.3, "k_fct_x<double, 128, YMModel, 0>": 511.3,
+    "k_fct_x<double, 256, YMModel, 0>": 509.2, "k_fct_x<double, 64, YMModel, 1>": 523.8,
+    "k_fct_x<double, 128, YMModel, 1>": 522.6, "k_fct_x<double, 256, YMModel, 1>": 509.8,
+    "k_fct_z<double, 16, 0, YMModel>": 981.4, "k_fct_z<double, 16, 1, YMModel>": 972.3,
+    "k_fct_z<double, 32, 0, YMModel>": 874.4, "k_fct_z<double, 32, 1, YMModel>": 875.7,
+    "k_fct_z<double, 64, 0, YMModel>": 806.5, "k_fct_z<double, 64, 1, YMModel>": 823.2,
+    "k_fct_zg<double, 16, 0, YMModel>": 1051.1, "k_fct_zg<double, 16, 1, YMModel>": 994.0,
+    "k_fct_zg<double, 32, 0, YMModel>": 982.7, "k_fct_zg<double, 32, 1, YMModel>": 930.7,
+    "k_fct_zg<double, 64, 0, YMModel>": 932.8, "k_fct_zg<double, 64, 1, YMModel>": 958.2,
+    "k_maxwell<double, 16>": 847.0, "k_maxwell<double, 4>": 717.9, "k_maxwell<double, 8>": 856.2,
+    "k_maxwell_cell<double>": 594.0,
+}
+
+
+def kernel_instance(slot, t):
+    """Kernel instance (template arguments) that the tuned slot launches."""
+    if slot == "k_fct_x":
+        return f"k_fct_x<double, {t['x_segment']}, YMModel, {t['x_serial']}>"
+    if slot == "k_fct_z":
+        name = "k_fct_zg" if t["z_groups"] else "k_fct_z"
+        return f"{name}<double, {t['z_chunk']}, {t['z_prefetch']}, YMModel>"
+    if slot == "k_maxwell":
+        return {0: "k_maxwell<double, 16>", 1: "k_maxwell_cell<double>", 2: "k_maxwell<double, 4>",
+                3: "k_maxwell<double, 8>"}[t["maxwell"]]
+    return None
 
 
 def run_gpu(precision, steps, warmup, fused=1, workload=WORKLOAD, segment=SEGMENT):
@@ -275,9 +298,16 @@ def main():
                               alg_GBs=bytes_alg / avg_s / 1e9)
     dom = max(kern, key=lambda k: kern[k]["avg_us"])
     d = kern[dom]
-    tile = {"k_fct_x": r64["tuning"]["x_segment"], "k_fct_z": r64["tuning"]["z_chunk"]}.get(dom)
-    traffic = PMC_TRAFFIC_MB.get((dom, tile))
+    inst = kernel_instance(dom, r64["tuning"])
+    traffic = PMC_TRAFFIC_MB.get(inst)
     traffic = traffic * 1e6 if (traffic and r64["nx"] == 2048 and r64["nz"] == 1024) else None
+    for name, kd in kern.items():
+        ki = kernel_instance(name, r64["tuning"])
+        t_mb = PMC_TRAFFIC_MB.get(ki)
+        kd["instance"] = ki
+        if t_mb:
+            kd["pmc_traffic_bytes"] = t_mb * 1e6
+            kd["pmc_traffic_GBs"] = t_mb * 1e6 / (kd["avg_us"] * 1e-6) / 1e9
     step_alg_GBs = STEP_ARRAYS * 8 * r64["cells"] / (ms_step * 1e-3) / 1e9
 
     out = {
@@ -301,7 +331,7 @@ def main():
                    "sweep": "fused x+z" if fused else "split x / z"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": d["alg_GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": d["alg_GBs"] / HBM_PEAK_GBS, "traffic": traffic,
-                     "alg_bytes_per_launch": d["alg_bytes"], "avg_launch_us": d["avg_us"],
+                     "kernel_instance": inst, "alg_bytes_per_launch": d["alg_bytes"], "avg_launch_us": d["avg_us"],
                      "traffic_GBs": traffic / (d["avg_us"] * 1e-6) / 1e9 if traffic else None},
         "step_roofline": {"alg_bytes_per_cell": STEP_ARRAYS * 8, "achieved_GBs": step_alg_GBs,
                           "frac": step_alg_GBs / HBM_PEAK_GBS},
