@@ -79,10 +79,15 @@ PMC_TRAFFIC_MB = {
 def kernel_instance(slot, t):
     """Kernel instance (template arguments) that the tuned slot launches."""
     if slot == "k_fct_x":
+        if t.get("x_lane_cells", 0) > 0:
+            return f"k_fct_xr<double, {t['x_lane_cells']}>"
         return f"k_fct_x<double, {t['x_segment']}, YMModel, {t['x_serial']}>"
     if slot == "k_fct_z":
         name = "k_fct_zg" if t["z_groups"] else "k_fct_z"
         return f"{name}<double, {t['z_chunk']}, {t['z_prefetch']}, YMModel>"
+    if slot == "k_maxwell" and t["maxwell"] >= 4:
+        tz, pf = {4: (16, 1), 5: (32, 1), 6: (16, 0), 7: (8, 1), 8: (8, 0), 9: (4, 1)}[t["maxwell"]]
+        return f"k_maxwell_zm<double, {tz}, {pf}>"
     if slot == "k_maxwell":
         return {0: "k_maxwell<double, 16>", 1: "k_maxwell_cell<double>", 2: "k_maxwell<double, 4>",
                 3: "k_maxwell<double, 8>"}[t["maxwell"]]
@@ -138,6 +143,8 @@ def growth_c3(precision, n_steps=500, stride=10):
     sim = khs.Simulation(cfg)
     sim.init()
     ts = sim.timeseries(1, n_steps, stride)
+    ts["energy_ratio"] = sim.energy() / sim.energy0
+    ts["gauss"] = sim.gauss()
     sim.close()
     if ts["halted"]:
         raise RuntimeError("growth-rate run halted")
@@ -362,6 +369,10 @@ def main():
             "fp32_cell_updates_per_s": c3["fp32"]["value"], "fp64_cell_updates_per_s": c3["fp64"]["value"],
             "fp32_over_fp64": c3["fp32"]["value"] / c3["fp64"]["value"],
             "gamma_fp64": g64w, "gamma_fp32": g32, "gamma_rel_err": abs(g32 - g64w) / g64w,
+            "energy_ratio_fp64": t64["energy_ratio"], "energy_ratio_fp32": t32["energy_ratio"],
+            "energy_drift_err": abs(t32["energy_ratio"] - t64["energy_ratio"]),
+            "gauss_rms_fp64": t64["gauss"]["rms_ref"], "gauss_rms_fp32": t32["gauss"]["rms_ref"],
+            "gauss_rel_scale": t64["gauss"]["Etot_scale"],
             "fit_window_t": [t0, t1], "steps": a.c3_steps, "segments_of": SEGMENT_C3,
             "kernels_avg_us": {p: c3[p]["kernels_avg_us"] for p in c3}}
     if not a.no_sweep:

@@ -129,10 +129,13 @@ typedef struct khs_config {
     int      z_groups;         /* z sweep: -1 = autotuned (default), 0 = all variables per lane, 1 = two variable groups on two waves */
     int      xz_variant;       /* fused sweep kernel variant: -1 = autotuned (default), 0..3 */
     int      x_serial;         /* x sweep phases per variable: -1 = autotuned (default), 0 = all at once, 1 = one at a time */
-    int      maxwell_kernel;   /* YM Maxwell: -1 = autotuned (default), 0/2/3 = 64x16/64x4/64x8 LDS tiles, 1 = per cell */
+    int      maxwell_kernel;   /* YM Maxwell: -1 = autotuned (default), 0/2/3 = 64x16/64x4/64x8 LDS tiles, 1 = per cell,
+                                  4..9 = z march with 16/32/16/8/8/4-row chunks (6, 8: without next-row prefetch) */
     int      x_rows;           /* x sweep rows per workgroup: 0 = autotuned (default), 1, 4 or 16 */
     int      z_prefetch;       /* z sweep: -1 = autotuned (default), 0 = load each row when used, 1 = prefetch the next row */
     int      mfma_filter;      /* keep-|kz|<=K filter as MFMA contractions (1, default) or the per-column FP64-sum kernels (0) */
+    int      x_lane_cells;     /* x sweep form: -1 = autotuned (default), 0 = LDS-staged segments, 4 or 8 = register form with that many cells per lane (YM) */
+    int      snake_order;      /* 1 (default): consecutive step kernels walk the grid in opposite directions (cache reuse); 0: all forward */
     /* --- derived (khs_derive) --- */
     int    nx, nz;
     int    periodic_x;         /* run_mode >= 1 */
@@ -263,12 +266,14 @@ int khs_kernel_times(khs_ctx* ctx, double* ms, int* launches, int nslots);
 const char* khs_kernel_name(int slot);
 /* Tile shapes in use: out = {x-sweep segment width (cells), x-sweep rows per
  * workgroup, z-sweep chunk height (rows), z-sweep next-row prefetch (0/1),
- * YM Maxwell kernel (0/2/3 = 64x16/64x4/64x8 LDS tiles, 1 = per cell), x-sweep variables one at a
+ * YM Maxwell kernel (0/2/3 = 64x16/64x4/64x8 LDS tiles, 1 = per cell, 4..9 = z march), x-sweep variables one at a
  * time (0/1), fused-sweep variant (0: one wave per 64 columns; 1/2/3: four
  * waves per 128 columns with 16/32/64-row chunks), z-sweep variables split
- * over two waves (0/1)}; chosen by the autotuner on the first step unless
- * autotune = 0. */
-int khs_get_tuning(khs_ctx* ctx, int out[8]);
+ * over two waves (0/1), x-sweep cells per lane of the register form (0 = LDS
+ * form; the register form reports its 62*C-cell wave segment, 1 row, and 1
+ * for variables one at a time)}; chosen by the autotuner on the first step
+ * unless autotune = 0. */
+int khs_get_tuning(khs_ctx* ctx, int out[9]);
 
 /* ---------------- batches of runs (parameter sweeps) ---------------- */
 /* No counterpart in the reference, which runs one configuration per process

@@ -78,6 +78,8 @@ void config_defaults(khs_config& c) {
     c.x_serial = -1;
     c.xz_variant = -1;
     c.z_groups = -1;
+    c.x_lane_cells = -1;
+    c.snake_order = 1;
 }
 
 static int parse_int(const std::string& k, const std::string& v) {
@@ -166,6 +168,8 @@ bool config_set(khs_config& c, const std::string& key, const std::string& val) {
     else if (key == "x_rows")            c.x_rows = parse_int(key, val);
     else if (key == "z_prefetch")        c.z_prefetch = parse_int(key, val);
     else if (key == "mfma_filter")       c.mfma_filter = parse_int(key, val);
+    else if (key == "x_lane_cells")      c.x_lane_cells = parse_int(key, val);
+    else if (key == "snake_order")       c.snake_order = parse_int(key, val);
     else if (key == "smooth_init")       c.smooth_init = parse_int(key, val);
     else if (key == "spectrum_kmax")     c.spectrum_kmax = parse_int(key, val);
     else if (key == "spectrum_amp")      c.spectrum_amp = parse_double(key, val);
@@ -234,10 +238,13 @@ static void config_derive_emhd(khs_config& c) {
     if (c.z_prefetch < -1 || c.z_prefetch > 1) throw Error(KHS_ERR_CONFIG, "z_prefetch must be -1, 0 or 1");
     if (c.x_rows != 0 && c.x_rows != 1 && c.x_rows != 4 && c.x_rows != 16)
         throw Error(KHS_ERR_CONFIG, "x_rows must be 0, 1, 4 or 16");
-    if (c.maxwell_kernel < -1 || c.maxwell_kernel > 3) throw Error(KHS_ERR_CONFIG, "maxwell_kernel must be -1..3");
+    if (c.maxwell_kernel < -1 || c.maxwell_kernel > 9) throw Error(KHS_ERR_CONFIG, "maxwell_kernel must be -1..9");
     if (c.x_serial < -1 || c.x_serial > 1) throw Error(KHS_ERR_CONFIG, "x_serial must be -1, 0 or 1");
     if (c.xz_variant < -1 || c.xz_variant > 3) throw Error(KHS_ERR_CONFIG, "xz_variant must be -1..3");
     if (c.z_groups < -1 || c.z_groups > 1) throw Error(KHS_ERR_CONFIG, "z_groups must be -1, 0 or 1");
+    if (c.x_lane_cells != -1 && c.x_lane_cells != 0 && c.x_lane_cells != 4 && c.x_lane_cells != 8)
+        throw Error(KHS_ERR_CONFIG, "x_lane_cells must be -1, 0, 4 or 8");
+    if (c.snake_order != 0 && c.snake_order != 1) throw Error(KHS_ERR_CONFIG, "snake_order must be 0 or 1");
     if (c.emhd_init == 1 && (c.spectrum_kmax < 1 || 2 * c.spectrum_kmax >= std::min(c.nx, c.nz)))
         throw Error(KHS_ERR_CONFIG, "spectrum_kmax must satisfy 1 <= K < min(nx, nz)/2");
 }
@@ -296,10 +303,13 @@ void config_derive(khs_config& c) {
     if (c.z_prefetch < -1 || c.z_prefetch > 1) throw Error(KHS_ERR_CONFIG, "z_prefetch must be -1, 0 or 1");
     if (c.x_rows != 0 && c.x_rows != 1 && c.x_rows != 4 && c.x_rows != 16)
         throw Error(KHS_ERR_CONFIG, "x_rows must be 0, 1, 4 or 16");
-    if (c.maxwell_kernel < -1 || c.maxwell_kernel > 3) throw Error(KHS_ERR_CONFIG, "maxwell_kernel must be -1..3");
+    if (c.maxwell_kernel < -1 || c.maxwell_kernel > 9) throw Error(KHS_ERR_CONFIG, "maxwell_kernel must be -1..9");
     if (c.x_serial < -1 || c.x_serial > 1) throw Error(KHS_ERR_CONFIG, "x_serial must be -1, 0 or 1");
     if (c.xz_variant < -1 || c.xz_variant > 3) throw Error(KHS_ERR_CONFIG, "xz_variant must be -1..3");
     if (c.z_groups < -1 || c.z_groups > 1) throw Error(KHS_ERR_CONFIG, "z_groups must be -1, 0 or 1");
+    if (c.x_lane_cells != -1 && c.x_lane_cells != 0 && c.x_lane_cells != 4 && c.x_lane_cells != 8)
+        throw Error(KHS_ERR_CONFIG, "x_lane_cells must be -1, 0, 4 or 8");
+    if (c.snake_order != 0 && c.snake_order != 1) throw Error(KHS_ERR_CONFIG, "snake_order must be 0 or 1");
     if (c.run_mode < 0 || c.run_mode > 6)
         throw Error(KHS_ERR_CONFIG, "run_mode must be 0..6");
 }

@@ -176,11 +176,19 @@ struct Ctx {
         if (emhd) launch_fct_z<T, EMModel>(state<T>(cur), stg<T>(), state<T>(1 - cur), kp, sp0, ns, v, stream);
         else launch_fct_z<T, YMModel>(state<T>(cur), stg<T>(), state<T>(1 - cur), kp, sp0, ns, v, stream);
     }
+    // Direction of the step kernels' tile walk: x and Maxwell one way, the z
+    // sweep between them the other, flipping every step, so each kernel
+    // starts where its predecessor ended.
+    void set_order(int stage) {
+        kp.rev = cfg.snake_order ? (int)((step + (stage == 1)) & 1) : 0;
+    }
     template <typename T> void do_x(int sp0, int ns) {
+        set_order(0);
         timed(K_FCT_X, [&] { fct_x<T>(sp0, ns, xvar); });
         check_launch();
     }
     template <typename T> void do_z(int sp0, int ns) {
+        set_order(1);
         timed(K_FCT_Z, [&] { fct_z<T>(sp0, ns, zvar); });
         check_launch();
     }
@@ -215,14 +223,22 @@ struct Ctx {
         std::vector<int> xc, zc;
         const int fr = cfg.x_rows == 1 ? 0 : cfg.x_rows == 4 ? 1 : cfg.x_rows == 16 ? 2 : -1;
         const int fv = cfg.x_serial;             // -1 = tuned
-        for (int v = 0; v < kNumXVariants; ++v)
-            if ((fx < 0 || v % 3 == fx) && (fr < 0 || (v / 3) % 3 == fr) && (fv < 0 || v / 9 == fv)) xc.push_back(v);
+        const int fl = cfg.x_lane_cells;         // -1 = tuned, 0 = LDS form, 4 / 8 = register form
+        const bool lds_keys = fx >= 0 || fr >= 0 || fv >= 0;
+        for (int v = 0; v < 18; ++v)
+            if ((fl <= 0) && (fx < 0 || v % 3 == fx) && (fr < 0 || (v / 3) % 3 == fr) && (fv < 0 || v / 9 == fv))
+                xc.push_back(v);
+        if (!emhd && !lds_keys) {
+            if (fl < 0 || fl == 4) xc.push_back(18);
+            if (fl < 0 || fl == 8) xc.push_back(19);
+        }
         const int fg = cfg.z_groups;             // -1 = tuned
         for (int v = 0; v < kNumZVariants; ++v)
             if ((fz < 0 || v % 3 == fz) && (fp < 0 || (v / 3) % 2 == fp) && (fg < 0 || v / 6 == fg)) zc.push_back(v);
         // without tuning: the forced shape, else 256-cell segments and 32-row
         // chunks with prefetch in FP32 (registers to spare), without in FP64
         xvar = ((fx >= 0) ? fx : 2) + 3 * ((fr >= 0) ? fr : 1) + 9 * ((fv >= 0) ? fv : 1);
+        if (!emhd && fl > 0) xvar = fl == 4 ? 18 : 19;
         zvar = ((fz >= 0) ? fz : 1) + 3 * ((fp >= 0) ? fp : (fp32 ? 1 : 0)) + 6 * ((fg >= 0) ? fg : 0);
         mvar = (cfg.maxwell_kernel >= 0) ? cfg.maxwell_kernel : 0;
         xzvar = (cfg.xz_variant >= 0) ? cfg.xz_variant : 2;
@@ -240,7 +256,8 @@ struct Ctx {
             }
         }
         if (!emhd && cfg.maxwell_kernel < 0) {
-            std::vector<int> mc = {0, 1, 2, 3};
+            std::vector<int> mc;
+            for (int v = 0; v < kNumMaxwellVariants; ++v) mc.push_back(v);
             // writes the next state's fields from the current state: scratch here
             mvar = pick<T>(mc, [&](int v) {
                 launch_maxwell<T>(state<T>(cur), state<T>(1 - cur), kp, cell_filters ? d_decay : nullptr,
@@ -257,6 +274,7 @@ struct Ctx {
         check_launch();
     }
     template <typename T> void do_maxwell() {
+        set_order(2);
         timed(K_MAXWELL, [&] {
             if (emhd)
                 launch_emhd_maxwell<T>(state<T>(cur), state<T>(1 - cur), kp, stream);
@@ -444,6 +462,7 @@ static std::unique_ptr<Ctx> make_ctx(const khs_config& in) {
     ctx->cell_filters = (c.xi_sponge > 0.0f) || (c.xi_cut > 0.0f);
     if (c.x_segment) ctx->xvar = c.x_segment == 64 ? 0 : c.x_segment == 128 ? 1 : 2;
     ctx->xvar += 3 * (c.x_rows == 1 ? 0 : c.x_rows == 16 ? 2 : 1) + 9 * (c.x_serial == 0 ? 0 : 1);
+    if (c.model != KHS_MODEL_EMHD && c.x_lane_cells > 0) ctx->xvar = c.x_lane_cells == 4 ? 18 : 19;
     if (c.z_chunk) ctx->zvar = c.z_chunk == 16 ? 0 : c.z_chunk == 32 ? 1 : 2;
     if (c.z_prefetch == 1) ctx->zvar = ctx->zvar % 3 + 3;
     if (c.z_groups == 1) ctx->zvar += 6;
@@ -1185,7 +1204,7 @@ int khs_kernel_times(khs_ctx* ctx, double* ms, int* launches, int nslots) {
     });
 }
 
-int khs_get_tuning(khs_ctx* ctx, int out[8]) {
+int khs_get_tuning(khs_ctx* ctx, int out[9]) {
     CTX_OR_FAIL(ctx);
     if (!out) { g_last_error = "null argument"; return KHS_ERR_ARG; }
     Ctx& c = *ctx->impl;
@@ -1198,6 +1217,14 @@ int khs_get_tuning(khs_ctx* ctx, int out[8]) {
     out[5] = c.xvar / 9;
     out[6] = c.xzvar;
     out[7] = c.zvar / 6;
+    out[8] = 0;
+    if (c.xvar >= 18) {
+        const int cells = c.xvar == 18 ? 4 : 8;
+        out[0] = 62 * cells;
+        out[1] = 1;
+        out[5] = 1;
+        out[8] = cells;
+    }
     return KHS_OK;
 }
 

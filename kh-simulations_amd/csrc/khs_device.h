@@ -33,6 +33,7 @@ struct KParams {
     double c2, eps0, alpha, warm_T;
     double q0, q1;            // EMHD species charges (emhdgpu2d fluid: -1)
     int nsp;                  // EMHD: 1 or 2 fluids (YM: always 2)
+    int rev;                  // walk the launch's tiles in reverse logical order
 };
 
 // --------------------------------------------------------------------------
@@ -64,6 +65,11 @@ __device__ __forceinline__ int zp_per(int z, int nz) { return z == nz - 1 ? 0 : 
 // ids >= the real tile count must exit.
 __device__ __forceinline__ int xcd_logical(int b, int nb8) { return (b & 7) * (nb8 >> 3) + (b >> 3); }
 __host__ __device__ __forceinline__ int round8(int n) { return (n + 7) & ~7; }
+// Logical tile order of a step kernel, forward or reversed (p.rev).  The
+// step alternates the direction from one kernel to the next, so a kernel
+// starts on the rows its predecessor touched last, which are still in the
+// 256 MB memory-side cache.
+__device__ __forceinline__ int tile_order(int lt, int total, int rev) { return rev ? total - 1 - lt : lt; }
 
 // Element i of an array through a 32-bit byte offset (every array is below
 // 4 GiB): the load can then take the array base from scalar registers and one
@@ -76,6 +82,24 @@ template <typename T>
 __device__ __forceinline__ void st32(T* a, unsigned i, T v) {
     *reinterpret_cast<T*>(reinterpret_cast<char*>(a) + i * (unsigned)sizeof(T)) = v;
 }
+
+// Whole-wavefront lane shifts (DPP wave_shr:1 / wave_shl:1 of GFX9):
+// lane_from_left(v) in lane l is v of lane l-1, lane_from_right(v) is v of
+// lane l+1; lane 0 (from the left) and lane 63 (from the right) receive 0.
+// Every lane of the wavefront must be active.
+__device__ __forceinline__ int dpp_shr1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xF, 0xF, true); }
+__device__ __forceinline__ int dpp_shl1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xF, 0xF, true); }
+__device__ __forceinline__ float lane_from_left(float v) { return __int_as_float(dpp_shr1(__float_as_int(v))); }
+__device__ __forceinline__ float lane_from_right(float v) { return __int_as_float(dpp_shl1(__float_as_int(v))); }
+__device__ __forceinline__ double lane_shift64(double v, bool left) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const int lo = (int)(unsigned)b, hi = (int)(unsigned)(b >> 32);
+    const int rlo = left ? dpp_shr1(lo) : dpp_shl1(lo);
+    const int rhi = left ? dpp_shr1(hi) : dpp_shl1(hi);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)rhi << 32) | (unsigned)rlo));
+}
+__device__ __forceinline__ double lane_from_left(double v) { return lane_shift64(v, true); }
+__device__ __forceinline__ double lane_from_right(double v) { return lane_shift64(v, false); }
 
 // Explicit selects: deterministic sign of zero (NaN propagates through the
 // first operand of kmax(a, floor)).

@@ -10,10 +10,12 @@ namespace khs {
 // (variant % 3) with 1/4/16 rows per workgroup ((variant / 3) % 3), all
 // variables or one at a time through LDS (variant / 9); z chunk 16/32/64 rows
 // (variant % 3) without / with next-row prefetch ((variant / 3) % 2), one lane
-// per column or two waves per column (variable groups, variant / 6).
-constexpr int kNumXVariants = 18;
+// per column or two waves per column (variable groups, variant / 6).  x
+// variants 18 / 19: register form, 4 / 8 cells per lane, lane neighbours by
+// DPP (YM model).
+constexpr int kNumXVariants = 20;
 constexpr int kNumZVariants = 12;
-constexpr int kNumMaxwellVariants = 4;
+constexpr int kNumMaxwellVariants = 10;
 // M: YMModel (ymgpu2d two-beam SU(2) fluid) or EMModel (emhdgpu2d Abelian fluid)
 template <typename T, typename M>
 void launch_fct_x(const State<T>& S, const Stage<T>& out, const KParams& p, int sp0, int nsp,
@@ -36,7 +38,9 @@ template <typename T>
 void launch_fct_xz(const State<T>& S, const State<T>& O, const KParams& p, int variant, hipStream_t st);
 // decay/action: per-column sponge / xi_cut (nullptr when neither is active);
 // variant 0/2/3: 64x16 / 64x4 / 64x8 tiles sharing the new E through LDS,
-// 1: one thread per cell recomputing its neighbours' E (all bit-identical)
+// 1: one thread per cell recomputing its neighbours' E, 4..9: z march, one
+// lane per column, chunks of 16/32/16/8/8/4 rows with / with / without / with
+// / without / with next-row prefetch; all bit-identical
 template <typename T>
 void launch_maxwell(const State<T>& S, const State<T>& O, const KParams& p, const double* decay,
                     const int* action, int variant, hipStream_t st);

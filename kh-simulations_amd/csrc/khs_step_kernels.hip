@@ -19,6 +19,8 @@
 //
 // Reference semantics: ymgpu2d/FCT_Sweeps.cu:39-177, YM_FCT.cu:30-135,
 // YM_Fluid.cu:3-108, YM_Physics.cu:9-151,589-599, main_ym.cu:322-363.
+#include <type_traits>
+
 #include "khs_device.h"
 #include "khs_launch.h"
 
@@ -55,6 +57,7 @@ __global__ __launch_bounds__(XB) void k_fct_x(State<T> S, Stage<T> out, KParams 
     const int nrg = (p.nz + rows - 1) / rows;
     int lt = xcd_logical(blockIdx.x, gridDim.x);
     if (lt >= nsp * ntx * nrg) return;
+    lt = tile_order(lt, nsp * ntx * nrg, p.rev);
     const int sp = sp0 + lt % nsp;
     lt /= nsp;
     const int x0 = (lt % ntx) * XB;
@@ -179,6 +182,195 @@ __global__ __launch_bounds__(XB) void k_fct_x(State<T> S, Stage<T> out, KParams 
 }
 
 // ==========================================================================
+// x sweep, register form
+// ==========================================================================
+// One wavefront sweeps a row segment with no LDS and no barriers.  Lane l owns
+// the C consecutive cells xl = x0 - C + C*l .. xl+C-1 in registers and takes
+// the values the FCT stencil needs across its run boundaries from lanes l-1 /
+// l+1 by whole-wave DPP shifts.  Lanes 0 and 63 are halo lanes: the stencil
+// reaches 3 cells, so with C >= 4 lane 1 needs only what lane 0 computes from
+// its own cells 1..3 (and lane 62 only lane 63's cells 0..2); their outer
+// values, fed by the shifts' zero fill, are never used.  The wave writes the
+// 62*C cells of lanes 1..62.  Every quantity is evaluated at the same extended
+// positions (map_x) and with the same expressions as in k_fct_x, so the two
+// forms agree bit for bit.  Lanes whose run is inside the row load and store
+// with 16-byte vectors when rows are C-aligned; a lane with cells outside the
+// row (domain ends) loads each cell through map_x and evaluates its px source
+// cell by cell (the x-1 / x+1 neighbours of such cells follow xm_src / xp_src
+// of the mapped cell, not the lane shift).  YM model only.
+constexpr int XR_OUT_LANES = 62;
+
+template <typename T, int C>
+__device__ __forceinline__ void load_run(const T* a, unsigned row, int xl, bool vec, const KParams& p,
+                                         T (&o)[C]) {
+    if (vec) {
+        constexpr int N = 16 / sizeof(T);
+        typedef T V __attribute__((ext_vector_type(N)));
+        const V* q = reinterpret_cast<const V*>(reinterpret_cast<const char*>(a) +
+                                                (row + (unsigned)xl) * (unsigned)sizeof(T));
+#pragma unroll
+        for (int k = 0; k < C / N; ++k) {
+            const V w = q[k];
+#pragma unroll
+            for (int j = 0; j < N; ++j) o[k * N + j] = w[j];
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < C; ++j) o[j] = ld32(a, row + (unsigned)map_x(xl + j, p));
+    }
+}
+
+template <typename T, int C>
+__device__ __forceinline__ void store_run(T* a, unsigned row, int xl, bool vec, int nx, const T (&o)[C]) {
+    if (vec) {
+        constexpr int N = 16 / sizeof(T);
+        typedef T V __attribute__((ext_vector_type(N)));
+        V* q = reinterpret_cast<V*>(reinterpret_cast<char*>(a) + (row + (unsigned)xl) * (unsigned)sizeof(T));
+#pragma unroll
+        for (int k = 0; k < C / N; ++k) {
+            V w;
+#pragma unroll
+            for (int j = 0; j < N; ++j) w[j] = o[k * N + j];
+            q[k] = w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < C; ++j)
+            if (xl + j < nx) st32(a, row + (unsigned)(xl + j), o[j]);
+    }
+}
+
+template <typename T, int C>
+__global__ __launch_bounds__(256) void k_fct_xr(State<T> S, Stage<T> out, KParams p, int sp0, int nsp, int nseg) {
+    static_assert(C >= 4 && C % (16 / sizeof(T)) == 0, "run length");
+    constexpr int NV = F_COUNT;
+    const int lane = threadIdx.x & 63;
+    int wl = xcd_logical(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+    if (wl >= nsp * nseg * p.nz) return;       // whole wavefronts leave together
+    wl = tile_order(wl, nsp * nseg * p.nz, p.rev);
+    const int sp = sp0 + wl % nsp;
+    const int seg = (wl / nsp) % nseg;
+    const int z = wl / (nsp * nseg);
+    const int nx = p.nx;
+    const int xl = seg * (XR_OUT_LANES * C) - C + C * lane;
+    const unsigned row = (unsigned)z * (unsigned)nx;
+    const unsigned rowm = (unsigned)zm_per(z, p.nz) * (unsigned)nx;
+    const bool inside = xl >= 0 && xl + C <= nx;
+    const bool vec = inside && (nx % C) == 0;
+    const int b = YMModel::base(sp);
+    const T dt = T(p.dt);
+    const T dtdx = T(p.dtdx);
+
+    T v[NV][C];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) load_run<T, C>(S.a[b + k], row, xl, vec, p, v[k]);
+
+    // update_velocities and the px Lorentz source (kernel_ym_lorentz,
+    // YM_Fluid.cu:32-84) of every cell of the run
+    T u[C], src[C];
+#pragma unroll
+    for (int j = 0; j < C; ++j) u[j] = v[F_PX][j] / kmax(v[F_N][j], T(0.05));
+    {
+        T acc[3][C];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            T ex[C], by[C], bzm[C];
+            load_run<T, C>(S.a[S_EX1 + c], row, xl, vec, p, ex);
+            load_run<T, C>(S.a[S_BY1 + c], row, xl, vec, p, by);
+            load_run<T, C>(S.a[S_BY1 + c], rowm, xl, vec, p, bzm);
+            const T exl = lane_from_left(ex[C - 1]);
+            const T byl = lane_from_left(by[C - 1]);
+            const T bzml = lane_from_left(bzm[C - 1]);
+#pragma unroll
+            for (int j = 0; j < C; ++j) {
+                const T n_safe = kmax(v[F_N][j], T(0.05));
+                const T vz = v[F_PZ][j] / n_safe;
+                const T exm = j ? ex[j - 1] : exl;
+                const T bym = j ? by[j - 1] : byl;
+                const T bzmm = j ? bzm[j - 1] : bzml;
+                T Exn = T(0.5) * (ex[j] + exm);
+                T Byn = T(0.25) * (by[j] + bym + bzm[j] + bzmm);
+                acc[c][j] = v[F_Q1 + c][j] * (Exn + vz * Byn);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < C; ++j) src[j] = -(acc[0][j] + acc[1][j] + acc[2][j]);
+        if (p.warm_T > 0.0) {
+            const T nl = lane_from_left(v[F_N][C - 1]);
+            const T nr = lane_from_right(v[F_N][0]);
+#pragma unroll
+            for (int j = 0; j < C; ++j) {
+                const T nm = j ? v[F_N][j - 1] : nl;
+                const T np = (j + 1 < C) ? v[F_N][j + 1] : nr;
+                T dndx = (np - nm) / (T(2.0) * T(p.dx));
+                src[j] -= T(p.warm_T) * dndx;
+            }
+        }
+    }
+    if (!inside) {
+#pragma unroll
+        for (int j = 0; j < C; ++j) src[j] = lorentz_px(S, sp, map_x(xl + j, p), z, p);
+    }
+
+    // interface j+1/2 coefficients (compute_eps, compute_nu_mu)
+    T eps[C], nu[C], mu[C];
+    {
+        const T ur = lane_from_right(u[0]);
+#pragma unroll
+        for (int j = 0; j < C; ++j) {
+            T vi = T(0.5) * (u[j] + ((j + 1 < C) ? u[j + 1] : ur));
+            eps[j] = vi * dtdx;
+            fct_nu_mu(eps[j], nu[j], mu[j]);
+        }
+    }
+    const T epsl = lane_from_left(eps[C - 1]);
+    const T nul = lane_from_left(nu[C - 1]);
+    const bool store = lane >= 1 && lane <= XR_OUT_LANES;
+
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        // LORHOT / LNRHOT (FCT_Sweeps.cu:88-103)
+        const T vl = lane_from_left(v[k][C - 1]);
+        const T vr = lane_from_right(v[k][0]);
+        T lor[C], lnr[C];
+#pragma unroll
+        for (int j = 0; j < C; ++j) {
+            const T vm = j ? v[k][j - 1] : vl, v0 = v[k][j], vp = (j + 1 < C) ? v[k][j + 1] : vr;
+            const T eL = j ? eps[j - 1] : epsl, eR = eps[j];
+            const T nL = j ? nu[j - 1] : nul, nR = nu[j];
+            T fL = T(0.5) * eL * (vm + v0);
+            T fR = T(0.5) * eR * (v0 + vp);
+            T dL = nL * (v0 - vm);
+            T dR = nR * (vp - v0);
+            T sj = (k == F_PX) ? src[j] : T(0.0);
+            T lo = v0 + dt * sj - (fR - fL);
+            lor[j] = lo;
+            lnr[j] = lo + (dR - dL);
+        }
+        // limited antidiffusive flux at interfaces j+1/2 (FCT_Sweeps.cu:139, 155-160)
+        const T lorr = lane_from_right(lor[0]);
+        const T lnrl = lane_from_left(lnr[C - 1]);
+        const T lnrr = lane_from_right(lnr[0]);
+        const T lnrr2 = lane_from_right(lnr[1]);
+        T fl[C];
+#pragma unroll
+        for (int j = 0; j < C; ++j) {
+            T flx = mu[j] * (((j + 1 < C) ? lor[j + 1] : lorr) - lor[j]);
+            const T l0 = j ? lnr[j - 1] : lnrl;
+            const T l1 = lnr[j];
+            const T l2 = (j + 1 < C) ? lnr[j + 1] : lnrr;
+            const T l3 = (j + 2 < C) ? lnr[j + 2] : ((j + 2 == C) ? lnrr : lnrr2);
+            fl[j] = fct_limit(flx, l1 - l0, l2 - l1, l3 - l2);
+        }
+        const T fll = lane_from_left(fl[C - 1]);
+        T r[C];
+#pragma unroll
+        for (int j = 0; j < C; ++j) r[j] = lnr[j] - (fl[j] - (j ? fl[j - 1] : fll));
+        if (store) store_run<T, C>(out.a[sp * NV + k], row, xl, vec, nx, r);
+    }
+}
+
+// ==========================================================================
 // z sweep (register march)
 // ==========================================================================
 // FCT pipeline along z for one column, kept in registers.  push() is fed the
@@ -285,6 +477,7 @@ __global__ __launch_bounds__(ZB_X * ZB_C) void k_fct_z(State<T> S, Stage<T> X, S
     // their sources), then 64-column groups, then groups of ZB_C z chunks
     int lt = xcd_logical(blockIdx.x, gridDim.x);
     if (lt >= nsp * ntx * ncg) return;
+    lt = tile_order(lt, nsp * ntx * ncg, p.rev);
     const int sp = sp0 + lt % nsp;
     lt /= nsp;
     const int x = (lt % ntx) * ZB_X + (threadIdx.x % ZB_X);
@@ -414,6 +607,7 @@ __global__ __launch_bounds__(256) void k_fct_zg(State<T> S, Stage<T> X, State<T>
                                                 int ntx, int ncg) {
     int lt = xcd_logical(blockIdx.x, gridDim.x);
     if (lt >= nsp * ntx * ncg) return;
+    lt = tile_order(lt, nsp * ntx * ncg, p.rev);
     const int sp = sp0 + lt % nsp;
     lt /= nsp;
     const int x = (lt % ntx) * 64 + (int)(threadIdx.x & 63);
@@ -544,8 +738,8 @@ __global__ __launch_bounds__(MX * MZT) void k_maxwell(State<T> S, State<T> O, KP
     __shared__ T sEz[3][TM + 1][MX + 1];
     const int nx = p.nx, nz = p.nz;
     // logical tile: x tiles fastest (they share the x +- 1 halo lines)
-    const int lt = xcd_logical(blockIdx.x, gridDim.x);
-    if (lt >= ntx * ntz) return;
+    const int lt = tile_order(xcd_logical(blockIdx.x, gridDim.x), ntx * ntz, p.rev);
+    if (lt >= ntx * ntz || lt < 0) return;
     const int x0 = (lt % ntx) * MX, z0 = (lt / ntx) * TM;
     const int ncol = min(MX, nx - x0), nrow = min(TM, nz - z0);
     const T dt = T(p.dt), c2 = T(p.c2), aYM = T(p.alpha), eps0 = T(p.eps0);
@@ -738,8 +932,8 @@ template <typename T>
 __global__ __launch_bounds__(MC_X * MC_Z) void k_maxwell_cell(State<T> S, State<T> O, KParams p, int ntx, int ntz,
                                                               const double* decay, const int* action) {
     const int nx = p.nx, nz = p.nz;
-    const int lt = xcd_logical(blockIdx.x, gridDim.x);
-    if (lt >= ntx * ntz) return;
+    const int lt = tile_order(xcd_logical(blockIdx.x, gridDim.x), ntx * ntz, p.rev);
+    if (lt >= ntx * ntz || lt < 0) return;
     const int x = (lt % ntx) * MC_X + (int)(threadIdx.x % MC_X);
     const int z = (lt / ntx) * MC_Z + (int)(threadIdx.x / MC_X);
     if (x >= nx || z >= nz) return;
@@ -783,6 +977,195 @@ __global__ __launch_bounds__(MC_X * MC_Z) void k_maxwell_cell(State<T> S, State<
     }
 }
 
+// --------------------------------------------------------------------------
+// Maxwell as a z march.  A wavefront owns 64 adjacent columns and marches down
+// a chunk of TZ rows; lane l holds column x = 63*w + l.  Per row it loads the
+// new fluid of both species and the step-start fields of its cell once:
+//   * Jx at x+1/2 takes the x+1 quantities (px/n, nQ) from lane l+1 by a DPP
+//     shift, Jz at z+1/2 the row below from the next iteration;
+//   * Ez needs By at x-1 (loaded with the row: the line is lane l-1's, so
+//     the load hits the cache), Faraday the new Ez at x+1 (lane l+1) and the
+//     new Ex at z+1 (next iteration), so row z is finished one iteration
+//     after it is loaded.
+// Lane 63 is a halo lane (its x+1 neighbours are in the next wavefront) and
+// stores nothing: 63 output columns per wavefront, one extra row per chunk.
+// The arithmetic is that of currents_at / ampere / potential / Faraday in
+// k_maxwell, operation for operation, so all Maxwell variants agree bit for
+// bit.  PF = 1 loads row e+1 before the arithmetic of row e.
+// --------------------------------------------------------------------------
+constexpr int MZM_OUT = 63;
+
+template <typename T>
+struct MRow {                  // one row of one column: inputs of the march
+    T n[2], px[2], pz[2], q[2][3];
+    T by[3], byxm[3], az[3], ex[3], ez[3];
+};
+
+template <typename T>
+__device__ __forceinline__ void mrow_load(const State<T>& S, const State<T>& O, unsigned i, unsigned ixm,
+                                          MRow<T>& r) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const int b = S_NA + s * F_COUNT;
+        r.n[s] = ld32(O.a[b + F_N], i);
+        r.px[s] = ld32(O.a[b + F_PX], i);
+        r.pz[s] = ld32(O.a[b + F_PZ], i);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) r.q[s][c] = ld32(O.a[b + F_Q1 + c], i);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        r.by[c] = ld32(S.a[S_BY1 + c], i);
+        r.byxm[c] = ld32(S.a[S_BY1 + c], ixm);
+        r.az[c] = ld32(S.a[S_AZ1 + c], i);
+        r.ex[c] = ld32(S.a[S_EX1 + c], i);
+        r.ez[c] = ld32(S.a[S_EZ1 + c], i);
+    }
+}
+
+template <typename T, int TZ, int PF>
+__global__ __launch_bounds__(256) void k_maxwell_zm(State<T> S, State<T> O, KParams p, int nwx, int nch,
+                                                   const double* decay, const int* action) {
+    const int lane = threadIdx.x & 63;
+    int wl = xcd_logical(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+    if (wl >= nwx * nch) return;             // whole wavefronts leave together
+    wl = tile_order(wl, nwx * nch, p.rev);
+    const int nx = p.nx, nz = p.nz;
+    const bool per = p.periodic_x != 0;
+    const int x = (wl % nwx) * MZM_OUT + lane;
+    // lanes past the last column mirror the x+1 neighbour of column nx-1
+    const int xc = x < nx ? x : (per ? x % nx : nx - 1);
+    const int xm = per ? (xc + nx - 1) % nx : (xc == 0 ? 0 : xc - 1);
+    const bool wall = !per && (xc == 0 || xc == nx - 1);     // YM_Fluid.cu:23-26
+    const bool last_wall = !per && xc == nx - 1;              // Faraday: E(x+1) := E(x)
+    const bool store = lane < MZM_OUT && x < nx;
+    const int z0 = (wl / nwx) * TZ;
+    const int z1 = min(z0 + TZ, nz);
+    const T dt = T(p.dt), c2 = T(p.c2), aYM = T(p.alpha), eps0 = T(p.eps0);
+    const T dx = T(p.dx), dz = T(p.dz);
+    const int act = action ? action[xc] : 0;
+    const T dec = (act == 1) ? T(decay[xc]) : T(1.0);
+
+    // carried from row e-1
+    T byp[3], byxmp[3], azp[3], ezp[3], exn_p[3];
+    T uzp[2], nqp[2][3];
+    {
+        const unsigned im = (unsigned)xc + (unsigned)zm_per(z0, nz) * (unsigned)nx;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) byp[c] = ld32(S.a[S_BY1 + c], im);
+    }
+    MRow<T> cur, nxt;
+    auto rbase = [&](int e) { return (unsigned)(e < nz ? e : e - nz) * (unsigned)nx; };
+    mrow_load(S, O, rbase(z0) + (unsigned)xc, rbase(z0) + (unsigned)xm, cur);
+
+#pragma unroll 1
+    for (int e = z0; e <= z1; ++e) {
+        if (PF && e < z1) mrow_load(S, O, rbase(e + 1) + (unsigned)xc, rbase(e + 1) + (unsigned)xm, nxt);
+        // cell quantities of the currents (kernel_ym_currents, YM_Physics.cu:9-60)
+        T ux[2], uz[2], nq[2][3];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const T nf = kmax(cur.n[s], T(1e-10));
+            ux[s] = cur.px[s] / nf;
+            uz[s] = cur.pz[s] / nf;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) nq[s][c] = cur.n[s] * cur.q[s][c];
+        }
+        // Jx at (x+1/2, e) with the x+1 values of lane l+1
+        T Jx[3];
+        {
+            T vxr[2], nQx[2][3];
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                vxr[s] = T(0.5) * (ux[s] + lane_from_right(ux[s]));
+#pragma unroll
+                for (int c = 0; c < 3; ++c) nQx[s][c] = T(0.5) * (nq[s][c] + lane_from_right(nq[s][c]));
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) Jx[c] = -(nQx[0][c] * vxr[0] + nQx[1][c] * vxr[1]);
+        }
+        // Ampere for Ex at row e (YM_Physics.cu:70-113)
+        T exn[3];
+        {
+            const T cr[3] = {cur.az[1] * cur.by[2] - cur.az[2] * cur.by[1],
+                             cur.az[2] * cur.by[0] - cur.az[0] * cur.by[2],
+                             cur.az[0] * cur.by[1] - cur.az[1] * cur.by[0]};
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                T dBdz = (cur.by[c] - byp[c]) / dz;
+                exn[c] = cur.ex[c] + dt * (c2 * (-dBdz - aYM * cr[c]) - Jx[c] / eps0);
+            }
+        }
+        if (e > z0) {
+            // row e-1: Jz at (x, e-1/2), Ampere for Ez, potential, Faraday
+            T ezn[3];
+            {
+                T vzt[2], nQz[2][3];
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    vzt[s] = T(0.5) * (uzp[s] + uz[s]);
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) nQz[s][c] = T(0.5) * (nqp[s][c] + nq[s][c]);
+                }
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    T Jz = -(nQz[0][c] * vzt[0] + nQz[1][c] * vzt[1]);
+                    T dBdx = (byp[c] - byxmp[c]) / dx;
+                    T ez = wall ? T(0.0) : ezp[c];
+                    ezn[c] = ez + dt * (c2 * dBdx - Jz / eps0);
+                }
+            }
+            T ezr[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                ezr[c] = lane_from_right(ezn[c]);
+                if (last_wall) ezr[c] = ezn[c];
+            }
+            // kernel_ym_potential (YM_Physics.cu:589-599), kernel_ym_faraday (:120-151)
+            T Az[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                Az[c] = azp[c];
+                if (c > 0 || !p.freeze_az1) Az[c] = Az[c] - dt * ezn[c];
+            }
+            const T cr[3] = {Az[1] * exn_p[2] - Az[2] * exn_p[1],
+                             Az[2] * exn_p[0] - Az[0] * exn_p[2],
+                             Az[0] * exn_p[1] - Az[1] * exn_p[0]};
+            if (store) {
+                const unsigned i = rbase(e - 1) + (unsigned)xc;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    T dEx = (exn[c] - exn_p[c]) / dz;
+                    T dEz = (ezr[c] - ezn[c]) / dx;
+                    T byn = byp[c] - dt * (dEx - dEz - aYM * cr[c]);
+                    T exo = exn_p[c], ezo = ezn[c], azo = Az[c];
+                    if (c > 0 && act == 1) { byn *= dec; exo *= dec; ezo *= dec; azo *= dec; }
+                    if (c > 0 && act == 2) { byn = T(0.0); exo = T(0.0); ezo = T(0.0); azo = T(0.0); }
+                    st32(O.a[S_BY1 + c], i, byn);
+                    st32(O.a[S_EX1 + c], i, exo);
+                    st32(O.a[S_EZ1 + c], i, ezo);
+                    if (c > 0 || !p.freeze_az1) st32(O.a[S_AZ1 + c], i, azo);
+                }
+            }
+        }
+        // carry row e
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            byp[c] = cur.by[c]; byxmp[c] = cur.byxm[c]; azp[c] = cur.az[c]; ezp[c] = cur.ez[c]; exn_p[c] = exn[c];
+        }
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            uzp[s] = uz[s];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) nqp[s][c] = nq[s][c];
+        }
+        if (e < z1) {
+            if (PF) cur = nxt;
+            else mrow_load(S, O, rbase(e + 1) + (unsigned)xc, rbase(e + 1) + (unsigned)xm, cur);
+        }
+    }
+}
+
 // ==========================================================================
 // EMHD Maxwell: enforce_boundaries (Ey, Ez, Bx walls) + currents + Ampere +
 // vector potential + Faraday (emhdgpu2d/main_coupled.cu:48-63, 99-110;
@@ -794,8 +1177,8 @@ template <typename T>
 __global__ __launch_bounds__(MX * MZT) void k_emhd_maxwell(State<T> S, State<T> O, KParams p, int ntx, int ntz) {
     __shared__ T sE[3][MZ + 1][MX + 2];     // column index ix + 1
     const int nx = p.nx, nz = p.nz;
-    const int lt = xcd_logical(blockIdx.x, gridDim.x);
-    if (lt >= ntx * ntz) return;
+    const int lt = tile_order(xcd_logical(blockIdx.x, gridDim.x), ntx * ntz, p.rev);
+    if (lt >= ntx * ntz || lt < 0) return;
     const int x0 = (lt % ntx) * MX, z0 = (lt / ntx) * MZ;
     const int ncol = min(MX, nx - x0), nrow = min(MZ, nz - z0);
     const T dt = T(p.dt), c2 = T(p.c2), eps0 = T(p.eps0);
@@ -916,11 +1299,30 @@ static void launch_fct_x_t(const State<T>& S, const Stage<T>& out, const KParams
         hipLaunchKernelGGL((k_fct_x<T, XB, M, 0>), grid, dim3(XB), 0, st, S, out, p, sp0, nsp, ntx, rows);
 }
 
-// Variant v: segment width 64 / 128 / 256 cells (v % 3), rows per workgroup
-// 1 / 4 / 16 ((v / 3) % 3), all variables / one variable at a time (v / 9).
+template <typename T, int C>
+static void launch_fct_xr_t(const State<T>& S, const Stage<T>& out, const KParams& p, int sp0, int nsp,
+                            hipStream_t st) {
+    const int nseg = (p.nx + XR_OUT_LANES * C - 1) / (XR_OUT_LANES * C);
+    const int waves = nsp * nseg * p.nz;
+    hipLaunchKernelGGL((k_fct_xr<T, C>), dim3(round8((waves + 3) / 4)), dim3(256), 0, st, S, out, p, sp0, nsp,
+                       nseg);
+}
+
+// Variant v < 18: LDS-staged segments of 64 / 128 / 256 cells (v % 3), rows
+// per workgroup 1 / 4 / 16 ((v / 3) % 3), all variables / one variable at a
+// time (v / 9).  v = 18 / 19: register form with 4 / 8 cells per lane (YM
+// model; the EMHD model takes the default LDS shape instead).
 template <typename T, typename M>
 void launch_fct_x(const State<T>& S, const Stage<T>& out, const KParams& p, int sp0, int nsp,
                   int variant, hipStream_t st) {
+    if (variant >= 18) {
+        if constexpr (std::is_same<M, YMModel>::value) {
+            if (variant == 18) launch_fct_xr_t<T, 4>(S, out, p, sp0, nsp, st);
+            else launch_fct_xr_t<T, 8>(S, out, p, sp0, nsp, st);
+            return;
+        }
+        variant = 14;
+    }
     static const int kRows[3] = {1, 4, 16};
     const int rows = kRows[(variant / 3) % 3];
     const int vs = variant / 9;
@@ -1169,9 +1571,28 @@ void launch_fct_xz(const State<T>& S, const State<T>& O, const KParams& p, int v
     hipLaunchKernelGGL((k_fct_xz<T, TZ>), grid, dim3(XZ_W), 0, st, S, O, p, ntx, nch);
 }
 
+template <typename T, int TZ, int PF>
+static void launch_maxwell_zm_t(const State<T>& S, const State<T>& O, const KParams& p, const double* decay,
+                                const int* action, hipStream_t st) {
+    const int nwx = (p.nx + MZM_OUT - 1) / MZM_OUT;
+    const int nch = (p.nz + TZ - 1) / TZ;
+    const int waves = nwx * nch;
+    hipLaunchKernelGGL((k_maxwell_zm<T, TZ, PF>), dim3(round8((waves + 3) / 4)), dim3(256), 0, st, S, O, p, nwx, nch,
+                       decay, action);
+}
+
 template <typename T>
 void launch_maxwell(const State<T>& S, const State<T>& O, const KParams& p, const double* decay,
                     const int* action, int variant, hipStream_t st) {
+    switch (variant) {
+        case 4: launch_maxwell_zm_t<T, 16, 1>(S, O, p, decay, action, st); return;
+        case 5: launch_maxwell_zm_t<T, 32, 1>(S, O, p, decay, action, st); return;
+        case 6: launch_maxwell_zm_t<T, 16, 0>(S, O, p, decay, action, st); return;
+        case 7: launch_maxwell_zm_t<T, 8, 1>(S, O, p, decay, action, st); return;
+        case 8: launch_maxwell_zm_t<T, 8, 0>(S, O, p, decay, action, st); return;
+        case 9: launch_maxwell_zm_t<T, 4, 1>(S, O, p, decay, action, st); return;
+        default: break;
+    }
     if (variant == 1) {
         const int ntx = (p.nx + MC_X - 1) / MC_X, ntz = (p.nz + MC_Z - 1) / MC_Z;
         hipLaunchKernelGGL(k_maxwell_cell<T>, dim3(round8(ntx * ntz)), dim3(MC_X * MC_Z), 0, st, S, O, p, ntx, ntz,

@@ -85,6 +85,8 @@ def test_output_dir_name_c1():
     (dict(kz_suppress_max=2, nz_override=40), "multiple of 32"),
     (dict(kz_keep_max=40, nz_override=64), "kz_keep_max"),
     (dict(run_mode=9), "run_mode"),
+    (dict(x_lane_cells=6), "x_lane_cells"),
+    (dict(z_groups=2), "z_groups"),
 ])
 def test_constraint_errors_are_codes_not_exits(keys, msg):
     with pytest.raises(khs.KhsError) as e:

@@ -194,12 +194,50 @@ def test_tile_variants_bitwise(xs, xr, zc, pf, mk, xv, zg, periodic):
     cfg, sim, o = make_pair(st, **keys)
     t = sim.tuning()
     assert t == dict(x_segment=xs, x_rows=xr, z_chunk=zc, z_prefetch=pf, maxwell=mk, x_serial=xv,
-                     xz_variant=t["xz_variant"], z_groups=zg)
+                     xz_variant=t["xz_variant"], z_groups=zg, x_lane_cells=0)
     o.sources()
     sim.step(3)
     o.step(3)
     for n in orc.STATE:
         assert_bitwise(sim.get(n), o.get(n), f"x{xs}r{xr} z{zc} pf{pf} {n}")
+
+
+@pytest.mark.parametrize("cells", [4, 8])
+@pytest.mark.parametrize("shape,periodic,warm", [((300, 136), 1, 0.01), ((256, 64), 0, 0.0), ((70, 36), 0, 0.01),
+                                                 ((1000, 40), 0, 0.01), ((1000, 40), 1, 0.0), ((500, 24), 1, 0.01)])
+def test_x_register_form_bitwise(cells, shape, periodic, warm):
+    """Register-form x sweep (lane runs of 4 / 8 cells, DPP neighbours; vector
+    and per-cell loads, one and several wave segments per row, both ends)
+    gives the oracle's bits."""
+    nx, nz = shape
+    keys = grid(nx, nz, periodic, warm_T=warm, x_lane_cells=cells, autotune=0, xi_sponge=8.0)
+    cfg, sim, o = make_pair(random_state(nx, nz, seed=nx + cells, smooth=True), **keys)
+    t = sim.tuning()
+    assert (t["x_lane_cells"], t["x_segment"]) == (cells, 62 * cells)
+    o.sources()
+    sim.step(3)
+    o.step(3)
+    for n in orc.STATE:
+        assert_bitwise(sim.get(n), o.get(n), f"x lane cells {cells} {shape} {n}")
+
+
+@pytest.mark.parametrize("mk", [4, 5, 6, 7, 8, 9])
+@pytest.mark.parametrize("shape,periodic,freeze", [((300, 136), 1, 1), ((70, 36), 0, 1), ((256, 64), 0, 0),
+                                                   ((130, 100), 1, 0)])
+def test_maxwell_march_bitwise(mk, shape, periodic, freeze):
+    """Maxwell as a z march (63 output columns per wavefront, chunk ends that
+    wrap in z, walls and periodic x, sponge and xi cut, Az1 frozen or not)
+    gives the oracle's bits."""
+    nx, nz = shape
+    keys = grid(nx, nz, periodic, maxwell_kernel=mk, autotune=0, xi_sponge=8.0, xi_cut=12.0,
+                freeze_override=freeze)
+    cfg, sim, o = make_pair(random_state(nx, nz, seed=nx * 7 + mk, smooth=True), **keys)
+    assert sim.tuning()["maxwell"] == mk
+    o.sources()
+    sim.step(3)
+    o.step(3)
+    for n in orc.STATE:
+        assert_bitwise(sim.get(n), o.get(n), f"maxwell {mk} {shape} {n}")
 
 
 @pytest.mark.parametrize("xzv", [0, 1, 2, 3])
