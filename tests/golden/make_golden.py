@@ -8,6 +8,12 @@
       ymgpu2d/eigenmode_kz1_a1.00_V0.05_EPS0.15.npz (the linear eigensolver's
       growth rate at alpha = 1, V0 = 0.05, EPS = 0.15, kz = 1, NX = 768),
       read with numpy.load(allow_pickle=False).
+  eigenmode_kz1_a1.00_V0.05_EPS0.15_vec.npz  the six complex profiles b, ex,
+      ez, a, qA, qB of the same file (the eigenvector, 6 x 768).
+  eigensolver_grid_cache.npz  kz_vals (80), series (8 x (alpha, V0,
+      xi_sponge)) and gamma (8 x 80, NaN where no localised mode) of
+      ymgpu2d/presentation/eigensolver_grid_cache.npz (NX = 384, EPS = 0.15,
+      Lx = 6 pi; presentation/compute_eigensolver_grid.py).
 """
 import os
 import shutil
@@ -27,6 +33,11 @@ def main():
     np.savez(os.path.join(HERE, "eigenmode_kz1_a1.00_V0.05_EPS0.15_gamma.npz"),
              gamma=np.array([d["gamma"].real, d["gamma"].imag]), gamma_wkb=d["gamma_wkb"], alpha=d["alpha"],
              V0=d["V0"], EPS=d["EPS"], kz_int=d["kz_int"])
+    np.savez_compressed(os.path.join(HERE, "eigenmode_kz1_a1.00_V0.05_EPS0.15_vec.npz"),
+                        **{k: d[k] for k in ("b", "ex", "ez", "a", "qA", "qB")})
+    g = np.load(os.path.join(src, "presentation", "eigensolver_grid_cache.npz"), allow_pickle=False)
+    np.savez_compressed(os.path.join(HERE, "eigensolver_grid_cache.npz"),
+                        kz_vals=g["kz_vals"], series=g["series"], gamma=g["gamma"])
 
 
 if __name__ == "__main__":
