@@ -254,6 +254,22 @@ int khs_energy_csv_open(khs_ctx* ctx, const char* path);              /* header 
 int khs_energy_csv_row(khs_ctx* ctx, int step, double E);            /* step,time,E,E/E0, :462 */
 int khs_energy_csv_close(khs_ctx* ctx);
 
+/* ---------------- binary checkpoint / restart ---------------- */
+/* No counterpart in the reference (its runs restart only from a seed file).
+ * khs_checkpoint_write stores the committed state: an 80-byte header
+ * {char magic[8] = "KHSCKPT", uint32 version = 1, int32 model, nx, nz, fp32,
+ * species, periodic_x, n_arrays, int64 step, double E0, dt, dx, dz}, then
+ * n_arrays x nx*nz FP64 arrays (state slots 0..23 in the order of enum
+ * khs_array; the FP32 form widens exactly), then a uint64 FNV-1a hash of the
+ * array bytes.  The file is written to path + ".tmp" and renamed.
+ * khs_checkpoint_read loads such a file into a context created from the same
+ * configuration (model, grid, precision, species, x boundary and dt / dx / dz
+ * must match: KHS_ERR_CONFIG otherwise; KHS_ERR_IO for a short, corrupt or
+ * foreign file) and restores the step count and E0, so stepping on gives the
+ * bits of an uninterrupted run.  Both need a committed step (KHS_ERR_STATE). */
+int khs_checkpoint_write(khs_ctx* ctx, const char* path);
+int khs_checkpoint_read(khs_ctx* ctx, const char* path);
+
 /* ---------------- timing ---------------- */
 int khs_sync(khs_ctx* ctx);
 /* Enable per-kernel HIP-event timing on the context's stream.  While enabled,

@@ -1101,6 +1101,94 @@ int khs_mode_spectrum(khs_ctx* ctx, int id, int kmax, double* out) {
     });
 }
 
+namespace {
+struct CkptHeader {
+    char magic[8];
+    uint32_t version;
+    int32_t model, nx, nz, fp32, nsp, periodic_x, narrays;
+    int64_t step;
+    double E0, dt, dx, dz;
+};
+static_assert(sizeof(CkptHeader) == 80, "checkpoint header layout");
+constexpr char kCkptMagic[8] = {'K', 'H', 'S', 'C', 'K', 'P', 'T', 0};
+
+uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
+    const unsigned char* b = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n; ++i) { h ^= b[i]; h *= 1099511628211ull; }
+    return h;
+}
+
+CkptHeader ckpt_header(const khs::Ctx& c) {
+    CkptHeader h{};
+    std::memcpy(h.magic, kCkptMagic, 8);
+    h.version = 1;
+    h.model = c.emhd ? KHS_MODEL_EMHD : KHS_MODEL_YM;
+    h.nx = c.nx; h.nz = c.nz; h.fp32 = c.fp32 ? 1 : 0; h.nsp = c.nsp;
+    h.periodic_x = c.kp.periodic_x; h.narrays = khs::S_COUNT;
+    h.step = c.step; h.E0 = c.E0; h.dt = c.kp.dt; h.dx = c.kp.dx; h.dz = c.kp.dz;
+    return h;
+}
+}  // namespace
+
+int khs_checkpoint_write(khs_ctx* ctx, const char* path) {
+    CTX_OR_FAIL(ctx);
+    return guard([&] {
+        Ctx& c = *ctx->impl;
+        if (!path) throw Error(KHS_ERR_ARG, "null path");
+        khs::require_idle(c, "khs_checkpoint_write");
+        const std::string tmp = std::string(path) + ".tmp";
+        std::ofstream f(tmp, std::ios::binary | std::ios::trunc);
+        if (!f) throw Error(KHS_ERR_IO, "cannot open " + tmp);
+        const CkptHeader h = ckpt_header(c);
+        f.write(reinterpret_cast<const char*>(&h), sizeof(h));
+        std::vector<double> a(c.N);
+        uint64_t hash = 14695981039346656037ull;
+        for (int s = 0; s < khs::S_COUNT; ++s) {
+            khs::download(c, c.st[c.cur][s], c.fp32, a.data());
+            f.write(reinterpret_cast<const char*>(a.data()), (std::streamsize)(c.N * sizeof(double)));
+            hash = fnv1a(hash, a.data(), c.N * sizeof(double));
+        }
+        f.write(reinterpret_cast<const char*>(&hash), sizeof(hash));
+        f.close();
+        if (!f) throw Error(KHS_ERR_IO, "write failed: " + tmp);
+        if (std::rename(tmp.c_str(), path) != 0) throw Error(KHS_ERR_IO, std::string("cannot rename to ") + path);
+    });
+}
+
+int khs_checkpoint_read(khs_ctx* ctx, const char* path) {
+    CTX_OR_FAIL(ctx);
+    return guard([&] {
+        Ctx& c = *ctx->impl;
+        if (!path) throw Error(KHS_ERR_ARG, "null path");
+        khs::require_idle(c, "khs_checkpoint_read");
+        std::ifstream f(path, std::ios::binary);
+        if (!f) throw Error(KHS_ERR_IO, std::string("cannot open ") + path);
+        CkptHeader h{};
+        if (!f.read(reinterpret_cast<char*>(&h), sizeof(h)) || std::memcmp(h.magic, kCkptMagic, 8) != 0 ||
+            h.version != 1)
+            throw Error(KHS_ERR_IO, std::string(path) + " is not a khs checkpoint");
+        const CkptHeader want = ckpt_header(c);
+        if (h.model != want.model || h.nx != want.nx || h.nz != want.nz || h.fp32 != want.fp32 ||
+            h.nsp != want.nsp || h.periodic_x != want.periodic_x || h.narrays != want.narrays ||
+            std::memcmp(&h.dt, &want.dt, sizeof(double)) != 0 || std::memcmp(&h.dx, &want.dx, sizeof(double)) != 0 ||
+            std::memcmp(&h.dz, &want.dz, sizeof(double)) != 0)
+            throw Error(KHS_ERR_CONFIG, std::string(path) + " was written by a different configuration");
+        std::vector<std::vector<double>> a(khs::S_COUNT, std::vector<double>(c.N));
+        uint64_t hash = 14695981039346656037ull;
+        for (int s = 0; s < khs::S_COUNT; ++s) {
+            if (!f.read(reinterpret_cast<char*>(a[s].data()), (std::streamsize)(c.N * sizeof(double))))
+                throw Error(KHS_ERR_IO, std::string(path) + " is truncated");
+            hash = fnv1a(hash, a[s].data(), c.N * sizeof(double));
+        }
+        uint64_t stored = 0;
+        if (!f.read(reinterpret_cast<char*>(&stored), sizeof(stored)) || stored != hash)
+            throw Error(KHS_ERR_IO, std::string(path) + ": checksum mismatch");
+        for (int s = 0; s < khs::S_COUNT; ++s) khs::upload_state(c, c.cur, s, a[s].data());
+        c.step = h.step;
+        c.E0 = h.E0;
+    });
+}
+
 int khs_write_snapshot(khs_ctx* ctx, const char* dir, int step) {
     CTX_OR_FAIL(ctx);
     if (!dir) { g_last_error = "null argument"; return KHS_ERR_ARG; }

@@ -5,6 +5,9 @@
 // config parse, init (with optional eigenmode seed), output directory, energy
 // checks and halting every energy_stride steps, CSV snapshots every
 // export_stride steps (main_ym.cu:143-176, 244-491).
+// Options (no counterpart in the reference): --checkpoint-every N writes the
+// binary checkpoint <output dir>/checkpoint.khs every N steps;
+// --restart FILE continues a run from such a checkpoint.
 #include <sys/stat.h>
 
 #include <cmath>
@@ -15,6 +18,7 @@
 #include <iomanip>
 #include <iostream>
 #include <string>
+#include <vector>
 
 #include "../../include/khs.h"
 
@@ -26,9 +30,19 @@ static int fail(const char* what) {
 }
 
 int main(int argc, char** argv) {
+    std::string restart;
+    int ckpt_every = 0;
+    std::vector<char*> pos;
+    for (int i = 0; i < argc; ++i) {
+        if (std::strcmp(argv[i], "--restart") == 0 && i + 1 < argc) { restart = argv[++i]; continue; }
+        if (std::strcmp(argv[i], "--checkpoint-every") == 0 && i + 1 < argc) { ckpt_every = std::atoi(argv[++i]); continue; }
+        pos.push_back(argv[i]);
+    }
+    argc = (int)pos.size();
+    argv = pos.data();
     if (argc < 2) {
-        std::cerr << "Usage: khs_run <config.ini>\n"
-                  << "       khs_run --emhd [k_mode] [perturb_amp]\n"
+        std::cerr << "Usage: khs_run <config.ini> [--checkpoint-every N] [--restart FILE]\n"
+                  << "       khs_run --emhd [k_mode] [perturb_amp] [--checkpoint-every N] [--restart FILE]\n"
                   << "  Config is a key=value file with the ymgpu2d example.ini keys.\n";
         return 1;
     }
@@ -75,26 +89,36 @@ int main(int argc, char** argv) {
     if (khs_output_dir(&cfg, dirbuf, sizeof(dirbuf)) != KHS_OK) return fail("output dir");
     std::string out_dir = dirbuf;
     fs::create_directories(out_dir);
-    for (auto& e : fs::directory_iterator(out_dir)) {
-        auto nm = e.path().filename().string();
-        if (nm.rfind(snap_prefix, 0) == 0 && e.path().extension() == ".csv") fs::remove(e.path());
-    }
+    if (restart.empty())
+        for (auto& e : fs::directory_iterator(out_dir)) {
+            auto nm = e.path().filename().string();
+            if (nm.rfind(snap_prefix, 0) == 0 && e.path().extension() == ".csv") fs::remove(e.path());
+        }
 
     if (khs_init(ctx, nullptr) != KHS_OK) return fail("init");
+    if (!restart.empty()) {
+        if (khs_checkpoint_read(ctx, restart.c_str()) != KHS_OK) return fail("restart");
+        int s0 = 0;
+        khs_get_step(ctx, &s0, nullptr);
+        std::cout << " Restarted from " << restart << " at step " << s0 << "\n";
+    }
     double E0 = 0.0;
     khs_energy0(ctx, &E0);
     if (khs_energy_csv_open(ctx, (out_dir + "/energy.csv").c_str()) != KHS_OK) return fail("energy.csv");
     std::cout << " Initial energy: " << E0 << "\n";
-    if (cfg.export_stride > 0 && khs_write_snapshot(ctx, out_dir.c_str(), 0) != KHS_OK)
+    if (restart.empty() && cfg.export_stride > 0 && khs_write_snapshot(ctx, out_dir.c_str(), 0) != KHS_OK)
         return fail("snapshot");
 
     const int total = cfg.total_steps;
     const int es = cfg.energy_stride, xs = cfg.export_stride;
     int step = 0, halted = 0;
+    khs_get_step(ctx, &step, nullptr);
+    const std::string ckpt_path = out_dir + "/checkpoint.khs";
     while (step < total && !halted) {
         int next = total;
         if (es > 0) next = std::min(next, (step / es + 1) * es);
         if (xs > 0) next = std::min(next, (step / xs + 1) * xs);
+        if (ckpt_every > 0) next = std::min(next, (step / ckpt_every + 1) * ckpt_every);
         if (khs_step(ctx, next - step, &halted) != KHS_OK) return fail("step");
         khs_get_step(ctx, &step, nullptr);
         if (es > 0 && step % es == 0) {
@@ -110,6 +134,9 @@ int main(int argc, char** argv) {
         }
         if (!halted && xs > 0 && step % xs == 0 && khs_write_snapshot(ctx, out_dir.c_str(), step) != KHS_OK)
             return fail("snapshot");
+        if (!halted && ckpt_every > 0 && step % ckpt_every == 0 &&
+            khs_checkpoint_write(ctx, ckpt_path.c_str()) != KHS_OK)
+            return fail("checkpoint");
     }
     if (!halted) std::cout << "\n Run completed normally.\n";
     khs_energy_csv_close(ctx);

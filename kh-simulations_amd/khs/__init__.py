@@ -140,6 +140,8 @@ def _signatures():
         "khs_kernel_times": (c_int, [c_void_p, dp, P(c_int), c_int]),
         "khs_kernel_name": (c_char_p, [c_int]),
         "khs_get_tuning": (c_int, [c_void_p, P(c_int)]),
+        "khs_checkpoint_write": (c_int, [c_void_p, c_char_p]),
+        "khs_checkpoint_read": (c_int, [c_void_p, c_char_p]),
         "khs_mode_amplitude": (c_int, [c_void_p, c_int, dp]),
         "khs_mode_spectrum": (c_int, [c_void_p, c_int, c_int, dp]),
         "khs_fit_growth_max_r2": (c_int, [dp, dp, c_int, c_double, c_double, c_double, dp]),
@@ -375,6 +377,14 @@ class Simulation:
         return {"t": np.array(ts), "amp": np.array(amps), "amp_conj": np.array(conj), "halted": halted}
 
     # ---- files ----
+    def checkpoint(self, path):
+        """Binary FP64 checkpoint of the committed state (khs_checkpoint_write)."""
+        _check(_lib.khs_checkpoint_write(self._h, os.fsencode(path)))
+
+    def restore(self, path):
+        """Load a checkpoint of the same configuration; step and E0 follow it."""
+        _check(_lib.khs_checkpoint_read(self._h, os.fsencode(path)))
+
     def write_snapshot(self, directory, step):
         _check(_lib.khs_write_snapshot(self._h, os.fsencode(directory), int(step)))
 
