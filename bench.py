@@ -54,9 +54,11 @@ SEGMENT_C3 = 500
 #   k_fct_z : R stage 12 + step-start fluid 12 + Ez 3 + By 3 + Az 3, W fluid 12
 #   k_maxwell: R new fluid 12 + fields 12,         W E 6 + By 3 + Az2,3 2 (Az1 frozen)
 #   k_fct_xz: R fluid 12 + Ex, Ez, By, Az 12,       W fluid 12
-KERNEL_ARRAYS = {"k_fct_x": 18 + 12, "k_fct_z": 33 + 12, "k_fct_xz": 24 + 12, "k_maxwell": 24 + 11}
+KERNEL_ARRAYS = {"k_fct_x": 18 + 12, "k_fct_z": 33 + 12, "k_fct_xz": 24 + 12, "k_maxwell": 24 + 11,
+                 "k_fct_zm": 36 + 23}
 # The whole step's algorithmic minimum: R 24 + W 23 arrays (frozen Az1 read only).
 STEP_ARRAYS = 47
+ZM_ROWS = {1: 16, 2: 32, 3: 64}       # chunk height of the fused z sweep + Maxwell variants
 # HBM bytes per launch of the FP64 step kernels on this workload, measured with
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (separate runs), each scaled
 # by what the counter reads for a plain 256 MiB device copy (fetch x2.0,
@@ -85,6 +87,8 @@ def kernel_instance(slot, t):
     if slot == "k_fct_z":
         name = "k_fct_zg" if t["z_groups"] else "k_fct_z"
         return f"{name}<double, {t['z_chunk']}, {t['z_prefetch']}, YMModel>"
+    if slot == "k_fct_zm":
+        return f"k_fct_zmx<double, {ZM_ROWS.get(t.get('zm_fused', 0), 0)}>"
     if slot == "k_maxwell" and t["maxwell"] >= 4:
         tz, pf = {4: (16, 1), 5: (32, 1), 6: (16, 0), 7: (8, 1), 8: (8, 0), 9: (4, 1)}[t["maxwell"]]
         return f"k_maxwell_zm<double, {tz}, {pf}>"
@@ -94,7 +98,10 @@ def kernel_instance(slot, t):
     return None
 
 
-def run_gpu(precision, steps, warmup, fused=1, workload=WORKLOAD, segment=SEGMENT):
+def run_gpu(precision, steps, warmup, fused=1, workload=WORKLOAD, segment=SEGMENT, kernel_steps=120):
+    """Timed khs_step over `steps` steps (segments restarting from init; whole
+    energy strides replay as captured graphs), then `kernel_steps` eager steps
+    with HIP events around every launch for the per-kernel durations."""
     import khs
 
     keys = dict(workload, fused_sweep=fused)
@@ -105,33 +112,43 @@ def run_gpu(precision, steps, warmup, fused=1, workload=WORKLOAD, segment=SEGMEN
     total_s, done, nseg = 0.0, 0, 0
     times = {}
     e_ratio = None
+    sim.init()
+    if sim.step(2 * cfg.energy_stride + warmup):      # autotune and graph capture, untimed
+        raise RuntimeError("warm-up halted")
     while done < steps:
         seg = min(segment, steps - done)
         sim.init()
         if sim.step(warmup):
             raise RuntimeError("warm-up halted")
         sim.sync()
-        sim.timing(True)
-        sim.timing_reset()
-        sim.sync()
         t0 = time.perf_counter()
         halted = sim.step(seg)
         sim.sync()
         total_s += time.perf_counter() - t0
-        for k, (ms, n) in sim.kernel_times().items():
-            a, b = times.get(k, (0.0, 0))
-            times[k] = (a + ms, b + n)
-        sim.timing(False)
         if halted:
             raise RuntimeError("timed region halted")
         e_ratio = sim.energy() / sim.energy0
         done += seg
         nseg += 1
+    sim.init()
+    if sim.step(warmup):
+        raise RuntimeError("warm-up halted")
+    sim.sync()
+    sim.timing(True)
+    sim.timing_reset()
+    t0 = time.perf_counter()
+    if sim.step(kernel_steps):
+        raise RuntimeError("kernel-timing steps halted")
+    sim.sync()
+    eager_s = time.perf_counter() - t0
+    for k, (ms, n) in sim.kernel_times().items():
+        times[k] = (ms, n)
+    sim.timing(False)
     cells = cfg.nx * cfg.nz
     tuning = sim.tuning()
     sim.close()
     return dict(seconds=total_s, cells=cells, times=times, e_ratio=e_ratio, nx=cfg.nx, nz=cfg.nz,
-                segments=nseg, tuning=tuning)
+                segments=nseg, tuning=tuning, eager_ms_per_step=eager_s / kernel_steps * 1e3)
 
 
 def growth_c3(precision, n_steps=500, stride=10):
@@ -325,6 +342,7 @@ def main():
         "steps": a.steps,
         "warmup": a.warmup,
         "ms_per_step": ms_step,
+        "eager_ms_per_step": r64["eager_ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

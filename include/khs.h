@@ -136,6 +136,8 @@ typedef struct khs_config {
     int      mfma_filter;      /* keep-|kz|<=K filter as MFMA contractions (1, default) or the per-column FP64-sum kernels (0) */
     int      x_lane_cells;     /* x sweep form: -1 = autotuned (default), 0 = LDS-staged segments, 4 or 8 = register form with that many cells per lane (YM) */
     int      snake_order;      /* 1 (default): consecutive step kernels walk the grid in opposite directions (cache reuse); 0: all forward */
+    int      zm_fused;         /* YM step: -1 = autotuned (default), 0 = split z sweep and Maxwell, 1/2/3 = one fused kernel with 16/32/64-row chunks */
+    int      step_graphs;      /* 1 (default): khs_step replays whole energy strides (even) as one captured HIP graph; 0: eager launches */
     /* --- derived (khs_derive) --- */
     int    nx, nz;
     int    periodic_x;         /* run_mode >= 1 */
@@ -287,9 +289,10 @@ const char* khs_kernel_name(int slot);
  * waves per 128 columns with 16/32/64-row chunks), z-sweep variables split
  * over two waves (0/1), x-sweep cells per lane of the register form (0 = LDS
  * form; the register form reports its 62*C-cell wave segment, 1 row, and 1
- * for variables one at a time)}; chosen by the autotuner on the first step
+ * for variables one at a time), fused z sweep + Maxwell kernel (0 = split
+ * kernels, 1/2/3 = fused with 16/32/64-row chunks)}; chosen by the autotuner on the first step
  * unless autotune = 0. */
-int khs_get_tuning(khs_ctx* ctx, int out[9]);
+int khs_get_tuning(khs_ctx* ctx, int out[10]);
 
 /* ---------------- batches of runs (parameter sweeps) ---------------- */
 /* No counterpart in the reference, which runs one configuration per process
@@ -311,6 +314,11 @@ int khs_batch_run_eager(khs_batch* b, int nsteps, int* steps_done, int* halted);
 int khs_batch_get_array(khs_batch* b, int run, int id, double* host, size_t n);
 int khs_batch_set_array(khs_batch* b, int run, int id, const double* host, size_t n);
 int khs_batch_energy(khs_batch* b, int run, double* E, double* E0);
+/* Step limit of one run (max_step <= 0: none, the default): khs_batch_run
+ * stops the run there without marking it halted. */
+int khs_batch_set_step_limit(khs_batch* b, int run, int max_step);
+/* khs_mode_amplitude of one run of the batch (campaign growth-rate sampling) */
+int khs_batch_mode_amplitude(khs_batch* b, int run, int kz, double out[2]);
 
 #ifdef __cplusplus
 }

@@ -80,6 +80,8 @@ void config_defaults(khs_config& c) {
     c.z_groups = -1;
     c.x_lane_cells = -1;
     c.snake_order = 1;
+    c.zm_fused = -1;
+    c.step_graphs = 1;
 }
 
 static int parse_int(const std::string& k, const std::string& v) {
@@ -170,6 +172,8 @@ bool config_set(khs_config& c, const std::string& key, const std::string& val) {
     else if (key == "mfma_filter")       c.mfma_filter = parse_int(key, val);
     else if (key == "x_lane_cells")      c.x_lane_cells = parse_int(key, val);
     else if (key == "snake_order")       c.snake_order = parse_int(key, val);
+    else if (key == "zm_fused")          c.zm_fused = parse_int(key, val);
+    else if (key == "step_graphs")       c.step_graphs = parse_int(key, val);
     else if (key == "smooth_init")       c.smooth_init = parse_int(key, val);
     else if (key == "spectrum_kmax")     c.spectrum_kmax = parse_int(key, val);
     else if (key == "spectrum_amp")      c.spectrum_amp = parse_double(key, val);
@@ -245,6 +249,7 @@ static void config_derive_emhd(khs_config& c) {
     if (c.x_lane_cells != -1 && c.x_lane_cells != 0 && c.x_lane_cells != 4 && c.x_lane_cells != 8)
         throw Error(KHS_ERR_CONFIG, "x_lane_cells must be -1, 0, 4 or 8");
     if (c.snake_order != 0 && c.snake_order != 1) throw Error(KHS_ERR_CONFIG, "snake_order must be 0 or 1");
+    if (c.zm_fused < -1 || c.zm_fused > 3) throw Error(KHS_ERR_CONFIG, "zm_fused must be -1..3");
     if (c.emhd_init == 1 && (c.spectrum_kmax < 1 || 2 * c.spectrum_kmax >= std::min(c.nx, c.nz)))
         throw Error(KHS_ERR_CONFIG, "spectrum_kmax must satisfy 1 <= K < min(nx, nz)/2");
 }
@@ -310,6 +315,7 @@ void config_derive(khs_config& c) {
     if (c.x_lane_cells != -1 && c.x_lane_cells != 0 && c.x_lane_cells != 4 && c.x_lane_cells != 8)
         throw Error(KHS_ERR_CONFIG, "x_lane_cells must be -1, 0, 4 or 8");
     if (c.snake_order != 0 && c.snake_order != 1) throw Error(KHS_ERR_CONFIG, "snake_order must be 0 or 1");
+    if (c.zm_fused < -1 || c.zm_fused > 3) throw Error(KHS_ERR_CONFIG, "zm_fused must be -1..3");
     if (c.run_mode < 0 || c.run_mode > 6)
         throw Error(KHS_ERR_CONFIG, "run_mode must be 0..6");
 }

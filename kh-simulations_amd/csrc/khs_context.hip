@@ -24,6 +24,7 @@
 #include <cstring>
 #include <fstream>
 #include <iomanip>
+#include <limits>
 #include <memory>
 #include <sstream>
 #include <string>
@@ -42,8 +43,9 @@ namespace khs {
             throw Error(KHS_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_));   \
     } while (0)
 
-enum Slot { K_FCT_X = 0, K_FCT_Z, K_FCT_XZ, K_MAXWELL, K_FILTERS, K_ENERGY, K_NSLOTS };
-static const char* kSlotNames[K_NSLOTS] = {"k_fct_x", "k_fct_z", "k_fct_xz", "k_maxwell", "filters", "energy"};
+enum Slot { K_FCT_X = 0, K_FCT_Z, K_FCT_XZ, K_MAXWELL, K_FILTERS, K_ENERGY, K_FCT_ZM, K_NSLOTS };
+static const char* kSlotNames[K_NSLOTS] = {"k_fct_x", "k_fct_z", "k_fct_xz", "k_maxwell", "filters", "energy",
+                                           "k_fct_zm"};
 
 static const int kFilt15[15] = {S_BY1, S_EX1, S_EZ1, S_BY2, S_BY3, S_EX2, S_EX3, S_EZ2, S_EZ3,
                                 S_AZ2, S_AZ3, S_Q2A, S_Q3A, S_Q2B, S_Q3B};
@@ -104,6 +106,10 @@ struct Ctx {
 
     // tile variants (autotuned on the first step)
     int xvar = 2, zvar = 1, mvar = 0, xzvar = 2;
+    int zmvar = 0;                            // 0: split z sweep + Maxwell; 1..3: fused kernel
+    // one energy stride of steps + the energy reduction, captured once
+    hipGraphExec_t sgraph = nullptr;
+    int sgraph_cur = -1;
     bool tuned = false;
 
     // step state
@@ -255,6 +261,22 @@ struct Ctx {
                 xzvar = pick<T>(xzc, [&](int v) { launch_fct_xz<T>(state<T>(cur), state<T>(1 - cur), kp, v, stream); });
             }
         }
+        zmvar = (!emhd && cfg.zm_fused > 0) ? cfg.zm_fused : 0;
+        if (!emhd && cfg.zm_fused < 0 && !cfg.fused_sweep) {
+            // split z sweep + (tuned) Maxwell against the fused kernel's chunk heights
+            if (cfg.maxwell_kernel < 0) {
+                std::vector<int> mc;
+                for (int v = 0; v < kNumMaxwellVariants; ++v) mc.push_back(v);
+                mvar = pick<T>(mc, [&](int v) {
+                    launch_maxwell<T>(state<T>(cur), state<T>(1 - cur), kp, cell_filters ? d_decay : nullptr,
+                                      cell_filters ? d_action : nullptr, v, stream);
+                });
+            }
+            std::vector<int> zmc;
+            for (int v = 0; v < kNumZMVariants; ++v) zmc.push_back(v);
+            zmvar = pick<T>(zmc, [&](int v) { zm_launch<T>(v); });
+            return;
+        }
         if (!emhd && cfg.maxwell_kernel < 0) {
             std::vector<int> mc;
             for (int v = 0; v < kNumMaxwellVariants; ++v) mc.push_back(v);
@@ -271,6 +293,26 @@ struct Ctx {
     }
     template <typename T> void do_xz() {
         timed(K_FCT_XZ, [&] { launch_fct_xz<T>(state<T>(cur), state<T>(1 - cur), kp, xzvar, stream); });
+        check_launch();
+    }
+    // z sweep of both species then Maxwell, as the split kernels (v = 0) or
+    // the fused kernel (v = 1..3); writes the next state
+    template <typename T> void zm_launch(int v) {
+        const double* dec = cell_filters ? d_decay : nullptr;
+        const int* actp = cell_filters ? d_action : nullptr;
+        if (v == 0) {
+            fct_z<T>(0, nsp, zvar);
+            launch_maxwell<T>(state<T>(cur), state<T>(1 - cur), kp, dec, actp, mvar, stream);
+        } else {
+            launch_fct_zmx<T>(state<T>(cur), stg<T>(), state<T>(1 - cur), kp, dec, actp, v, stream);
+        }
+    }
+    template <typename T> void do_zm() {
+        set_order(1);
+        timed(K_FCT_ZM, [&] {
+            launch_fct_zmx<T>(state<T>(cur), stg<T>(), state<T>(1 - cur), kp, cell_filters ? d_decay : nullptr,
+                              cell_filters ? d_action : nullptr, zmvar, stream);
+        });
         check_launch();
     }
     template <typename T> void do_maxwell() {
@@ -335,6 +377,7 @@ struct Ctx {
         fp32 ? do_xz<float>() : do_xz<double>();
     }
     void maxwell_stage() { fp32 ? do_maxwell<float>() : do_maxwell<double>(); }
+    void zm_stage() { fp32 ? do_zm<float>() : do_zm<double>(); }
     void filter_stage() { fp32 ? do_filters<float>() : do_filters<double>(); }
 
     // Energy reduction and its copy into the pinned h_red[0], without waiting
@@ -361,6 +404,7 @@ struct Ctx {
 
 Ctx::~Ctx() {
     if (stream) hipStreamSynchronize(stream);
+    if (sgraph) hipGraphExecDestroy(sgraph);
     for (auto& p : pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
     for (auto e : evpool) hipEventDestroy(e);
     if (dmem) hipFree(dmem);
@@ -468,6 +512,7 @@ static std::unique_ptr<Ctx> make_ctx(const khs_config& in) {
     if (c.z_groups == 1) ctx->zvar += 6;
     if (c.maxwell_kernel >= 0) ctx->mvar = c.maxwell_kernel;
     if (c.xz_variant >= 0) ctx->xzvar = c.xz_variant;
+    if (c.model != KHS_MODEL_EMHD && c.zm_fused > 0) ctx->zmvar = c.zm_fused;
 
     // ---- one device allocation ----
     const bool share_az1 = c.freeze_az1 != 0;
@@ -593,14 +638,47 @@ static void commit(Ctx& c) {
     c.step += 1;
 }
 
+static void full_step(Ctx& c);
+
+// Capture `stride` full steps and the energy reduction (into the pinned
+// h_red[0]) as one HIP graph, starting from ping-pong buffer c.cur; stride
+// is even, so every replay starts and ends on the same buffer.
+static void capture_stride(Ctx& c, int stride) {
+    if (c.sgraph) hipGraphExecDestroy(c.sgraph);
+    c.sgraph = nullptr;
+    c.autotune();                               // tuning launches and waits: before capture
+    HIPCHK(hipStreamSynchronize(c.stream));
+    const int cur0 = c.cur;
+    const long step0 = c.step;
+    HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeRelaxed));
+    for (int s = 0; s < stride; ++s) full_step(c);
+    c.energy_async();
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamEndCapture(c.stream, &g));
+    c.cur = cur0;                               // the graph does the stepping
+    c.step = step0;
+    hipError_t e = hipGraphInstantiate(&c.sgraph, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        c.sgraph = nullptr;
+        throw Error(KHS_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    c.sgraph_cur = cur0;
+}
+
 static void full_step(Ctx& c) {
     if (c.cfg.fused_sweep && !c.emhd) {
         c.xz_stage();
+        c.maxwell_stage();
     } else {
-        c.x_stage(0, c.nsp);
-        c.z_stage(0, c.nsp);
+        c.x_stage(0, c.nsp);                    // autotunes on the first call
+        if (c.zmvar > 0) {
+            c.zm_stage();
+        } else {
+            c.z_stage(0, c.nsp);
+            c.maxwell_stage();
+        }
     }
-    c.maxwell_stage();
     c.filter_stage();
     commit(c);
 }
@@ -630,6 +708,7 @@ struct Batch {
     std::vector<hipEvent_t> ev;              // fork event + one join event per lane
     int seg = 50;                             // steps per graph segment (even)
     std::vector<int> active;                  // 1 while a run has not halted
+    std::vector<long> cap;                    // a run stops (not halted) at this step
     hipGraphExec_t exec = nullptr;
     int exec_k = 0;
     std::vector<int> exec_set;                // runs captured in exec
@@ -640,11 +719,8 @@ struct Batch {
         exec_k = 0;
         exec_set.clear();
     }
-    void capture(int K) {
+    void capture(int K, const std::vector<int>& set) {
         release_graph();
-        std::vector<int> set;
-        for (int i = 0; i < (int)runs.size(); ++i)
-            if (active[i]) set.push_back(i);
         if (set.empty()) return;
         for (int i : set) runs[i]->autotune();          // tuning launches and waits: before capture
         for (int i : set) HIPCHK(hipStreamSynchronize(runs[i]->stream));
@@ -949,7 +1025,25 @@ int khs_step(khs_ctx* ctx, int nsteps, int* halted) {
         Ctx& c = *ctx->impl;
         khs::require_idle(c, "khs_step");
         const int stride = c.cfg.energy_stride;
-        for (int s = 0; s < nsteps; ++s) {
+        // whole energy strides run as one captured graph (no per-kernel launch
+        // cost on the host); kernel timing needs the eager path
+        const bool graphs = c.cfg.step_graphs && !c.timing && stride >= 2 && stride % 2 == 0;
+        for (int s = 0; s < nsteps;) {
+            if (graphs && c.step % stride == 0 && nsteps - s >= stride) {
+                if (!c.sgraph || c.sgraph_cur != c.cur) khs::capture_stride(c, stride);
+                HIPCHK(hipGraphLaunch(c.sgraph, c.stream));
+                HIPCHK(hipStreamSynchronize(c.stream));
+                c.step += stride;
+                s += stride;
+                const double E = c.h_red[0];
+                khs::write_energy_row(c, c.step, E);
+                if (std::isnan(E) || std::isinf(E) || E / c.E0 > c.cfg.energy_thresh) {
+                    if (halted) *halted = 1;
+                    break;
+                }
+                continue;
+            }
+            ++s;
             khs::full_step(c);
             if (stride > 0 && c.step % stride == 0) {
                 double E = c.energy();
@@ -1002,11 +1096,8 @@ int khs_gauss(khs_ctx* ctx, double out[7]) {
     });
 }
 
-int khs_mode_amplitude(khs_ctx* ctx, int kz, double out[2]) {
-    CTX_OR_FAIL(ctx);
-    if (!out) { g_last_error = "null argument"; return KHS_ERR_ARG; }
-    return guard([&] {
-        Ctx& c = *ctx->impl;
+namespace {
+void mode_amplitude(Ctx& c, int kz, double out[2]) {
         if (c.emhd) throw Error(KHS_ERR_ARG, "khs_mode_amplitude: YM model only (Az2, Az3)");
         if (kz < 0) throw Error(KHS_ERR_ARG, "kz must be >= 0");
         if (c.fp32)
@@ -1022,7 +1113,13 @@ int khs_mode_amplitude(khs_ctx* ctx, int kz, double out[2]) {
         const double norm = (double)c.nx * (double)c.nz;
         out[0] = c.h_red[0] / norm;
         out[1] = c.h_red[1] / norm;
-    });
+}
+}  // namespace
+
+int khs_mode_amplitude(khs_ctx* ctx, int kz, double out[2]) {
+    CTX_OR_FAIL(ctx);
+    if (!out) { g_last_error = "null argument"; return KHS_ERR_ARG; }
+    return guard([&] { mode_amplitude(*ctx->impl, kz, out); });
 }
 
 int khs_mode_spectrum(khs_ctx* ctx, int id, int kmax, double* out) {
@@ -1292,7 +1389,7 @@ int khs_kernel_times(khs_ctx* ctx, double* ms, int* launches, int nslots) {
     });
 }
 
-int khs_get_tuning(khs_ctx* ctx, int out[9]) {
+int khs_get_tuning(khs_ctx* ctx, int out[10]) {
     CTX_OR_FAIL(ctx);
     if (!out) { g_last_error = "null argument"; return KHS_ERR_ARG; }
     Ctx& c = *ctx->impl;
@@ -1306,6 +1403,7 @@ int khs_get_tuning(khs_ctx* ctx, int out[9]) {
     out[6] = c.xzvar;
     out[7] = c.zvar / 6;
     out[8] = 0;
+    out[9] = c.zmvar;
     if (c.xvar >= 18) {
         const int cells = c.xvar == 18 ? 4 : 8;
         out[0] = 62 * cells;
@@ -1354,6 +1452,7 @@ int khs_batch_create(const khs_config* cfgs, int n, int lanes, int segment, khs_
             b->ev.push_back(e);
         }
         b->active.assign(n, 1);
+        b->cap.assign(n, std::numeric_limits<long>::max());
         auto h = new khs_batch;
         h->impl = std::move(b);
         *out = h;
@@ -1390,20 +1489,23 @@ int khs_batch_run(khs_batch* b, int nsteps, int* steps_done, int* halted) {
         const int n = (int)B.runs.size();
         int left = nsteps;
         while (left > 0) {
-            bool any = false;
-            for (int i = 0; i < n; ++i) any = any || B.active[i];
-            if (!any) break;
-            if (left == 1) {                      // odd tail: one eager step per run
-                for (int i = 0; i < n; ++i)
-                    if (B.active[i]) khs::full_step(*B.runs[i]);
-                left = 0;
-                break;
-            }
-            const int K = std::min(B.seg, left - (left % 2));
+            // runs that have neither halted nor reached their step limit
             std::vector<int> set;
+            long rem = left;
             for (int i = 0; i < n; ++i)
-                if (B.active[i]) set.push_back(i);
-            if (!B.exec || B.exec_k != K || B.exec_set != set) B.capture(K);
+                if (B.active[i] && B.runs[i]->step < B.cap[i]) {
+                    set.push_back(i);
+                    rem = std::min(rem, B.cap[i] - B.runs[i]->step);
+                }
+            if (set.empty()) break;
+            int K = (int)std::min<long>(B.seg, rem);
+            if (K < 2) {                          // odd remainder: one eager step per run
+                for (int i : set) khs::full_step(*B.runs[i]);
+                left -= 1;
+                continue;
+            }
+            K -= K % 2;
+            if (!B.exec || B.exec_k != K || B.exec_set != set) B.capture(K, set);
             HIPCHK(hipGraphLaunch(B.exec, B.origin));
             HIPCHK(hipStreamSynchronize(B.origin));
             for (int i : set) {
@@ -1466,6 +1568,23 @@ int khs_batch_set_array(khs_batch* b, int run, int id, const double* host, size_
     const int rc = khs_set_array(&tmp, id, host, n);
     b->impl->runs[run] = std::move(tmp.impl);
     return rc;
+}
+
+int khs_batch_set_step_limit(khs_batch* b, int run, int max_step) {
+    if (!b || !b->impl || run < 0 || run >= (int)b->impl->runs.size()) {
+        g_last_error = "bad batch or run index";
+        return KHS_ERR_ARG;
+    }
+    b->impl->cap[run] = max_step > 0 ? (long)max_step : std::numeric_limits<long>::max();
+    return KHS_OK;
+}
+
+int khs_batch_mode_amplitude(khs_batch* b, int run, int kz, double out[2]) {
+    if (!b || !b->impl || run < 0 || run >= (int)b->impl->runs.size() || !out) {
+        g_last_error = "bad batch, run index or null argument";
+        return KHS_ERR_ARG;
+    }
+    return guard([&] { mode_amplitude(*b->impl->runs[run], kz, out); });
 }
 
 int khs_batch_energy(khs_batch* b, int run, double* E, double* E0) {

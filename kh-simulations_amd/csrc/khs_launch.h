@@ -41,6 +41,12 @@ void launch_fct_xz(const State<T>& S, const State<T>& O, const KParams& p, int v
 // 1: one thread per cell recomputing its neighbours' E, 4..9: z march, one
 // lane per column, chunks of 16/32/16/8/8/4 rows with / with / without / with
 // / without / with next-row prefetch; all bit-identical
+// fused z sweep (both species) + Maxwell of the YM model: stage X and state S
+// -> fluid and fields of O; variant 1 / 2 / 3 = 16 / 32 / 64-row chunks
+constexpr int kNumZMVariants = 4;          // 0 = split z sweep and Maxwell kernels
+template <typename T>
+void launch_fct_zmx(const State<T>& S, const Stage<T>& X, const State<T>& O, const KParams& p, const double* decay,
+                    const int* action, int variant, hipStream_t st);
 template <typename T>
 void launch_maxwell(const State<T>& S, const State<T>& O, const KParams& p, const double* decay,
                     const int* action, int variant, hipStream_t st);

@@ -1166,6 +1166,243 @@ __global__ __launch_bounds__(256) void k_maxwell_zm(State<T> S, State<T> O, KPar
     }
 }
 
+// --------------------------------------------------------------------------
+// Fused z sweep + Maxwell (YM).  A workgroup is three wavefronts on the same
+// 64 columns (lane l: column x = 63*w + l, as in k_maxwell_zm) and z chunk.
+// Wavefronts 0 and 1 march species 0 and 1 through the z-sweep pipeline of
+// k_fct_z; for every finished row they store the new fluid and hand it to
+// wavefront 2 through LDS (double-buffered, one barrier per row), which runs
+// the Maxwell march step of k_maxwell_zm on both species' new fluid.  The new
+// fluid is thus never read back from memory, and the fields are read once
+// for the z sources and Maxwell together (Maxwell's read of a row a few
+// iterations after the z sources' hits the cache).  The pipelines run one row
+// past the chunk, because the Maxwell step of the chunk's last row needs the
+// new fluid of the row below; in the last chunk that row is row 0 (z is
+// periodic for Maxwell but clamped for the FCT), whose new fluid a second,
+// 7-row pipeline computes from the clamped rows 0,0,0,0,1,2,3 exactly as the
+// chunk that owns it does.  Every value is computed by the expressions of
+// k_fct_z and k_maxwell_zm in the same order, so the results are those of the
+// split kernels bit for bit.
+// --------------------------------------------------------------------------
+template <typename T, int TZ>
+__global__ __launch_bounds__(192) void k_fct_zmx(State<T> S, Stage<T> X, State<T> O, KParams p, int nwx, int nch,
+                                                const double* decay, const int* action) {
+    typedef YMModel M;
+    constexpr int NV = M::NV;
+    typedef typename M::template Row<T> Row;
+    __shared__ T sF[2][2][NV][64];             // [buffer][species][variable][lane]
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;           // 0, 1: z sweep of that species; 2: Maxwell
+    int lb = xcd_logical(blockIdx.x, gridDim.x);
+    if (lb >= nwx * nch) return;               // whole workgroups leave together
+    lb = tile_order(lb, nwx * nch, p.rev);
+    const int nx = p.nx, nz = p.nz;
+    const bool per = p.periodic_x != 0;
+    const int x = (lb % nwx) * MZM_OUT + lane;
+    const int xc = x < nx ? x : (per ? x % nx : nx - 1);
+    const int xm = per ? (xc + nx - 1) % nx : (xc == 0 ? 0 : xc - 1);      // xm_src(xc)
+    const bool wall = !per && (xc == 0 || xc == nx - 1);
+    const bool store = lane < MZM_OUT && x < nx;
+    const int z0 = (lb / nwx) * TZ;
+    const int zend = min(z0 + TZ, nz);
+    const bool last = zend == nz;
+    const int r_end = last ? nz : zend;        // Maxwell steps z0 .. r_end (r_end: the row below)
+    const T dt = T(p.dt);
+    auto rbase = [&](int r) { return (unsigned)(r < nz ? r : r - nz) * (unsigned)nx; };
+
+    if (wv == 2) {
+        // ---- Maxwell march (k_maxwell_zm) on the new fluid from LDS ----
+        const bool last_wall = !per && xc == nx - 1;      // Faraday: E(x+1) := E(x)
+        const T c2 = T(p.c2), aYM = T(p.alpha), eps0 = T(p.eps0), dx = T(p.dx), dz = T(p.dz);
+        const int act = action ? action[xc] : 0;
+        const T dec = (act == 1) ? T(decay[xc]) : T(1.0);
+        T byp[3], byxmp[3], azp[3], ezp[3], exn_p[3], uzp[2], nqp[2][3];
+        T fby[3], fbyxm[3], faz[3], fex[3], fez[3];       // field row of the next step
+        auto fload = [&](int r) {
+            const unsigned i = rbase(r) + (unsigned)xc, im = rbase(r) + (unsigned)xm;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                fby[c] = ld32(S.a[S_BY1 + c], i);
+                fbyxm[c] = ld32(S.a[S_BY1 + c], im);
+                faz[c] = ld32(S.a[S_AZ1 + c], i);
+                fex[c] = ld32(S.a[S_EX1 + c], i);
+                fez[c] = ld32(S.a[S_EZ1 + c], i);
+            }
+        };
+        {
+            const unsigned im = (unsigned)zm_per(z0, nz) * (unsigned)nx + (unsigned)xc;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) byp[c] = ld32(S.a[S_BY1 + c], im);
+        }
+        fload(z0);
+        int buf = 0;
+#pragma unroll 1
+        for (int r = z0; r <= r_end; ++r) {
+            T by[3], byxm[3], az[3], ex[3], ez[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) { by[c] = fby[c]; byxm[c] = fbyxm[c]; az[c] = faz[c]; ex[c] = fex[c]; ez[c] = fez[c]; }
+            if (r < r_end) fload(r + 1);
+            __syncthreads();                       // the new fluid of row r is in sF[buf]
+            T ux[2], uz[2], nq[2][3];
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const T n = sF[buf][s][F_N][lane];
+                const T nf = kmax(n, T(1e-10));
+                ux[s] = sF[buf][s][F_PX][lane] / nf;
+                uz[s] = sF[buf][s][F_PZ][lane] / nf;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) nq[s][c] = n * sF[buf][s][F_Q1 + c][lane];
+            }
+            buf ^= 1;
+            // Jx at (x+1/2, r) with the x+1 values of lane l+1 (kernel_ym_currents)
+            T Jx[3];
+            {
+                T vxr[2], nQx[2][3];
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    vxr[s] = T(0.5) * (ux[s] + lane_from_right(ux[s]));
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) nQx[s][c] = T(0.5) * (nq[s][c] + lane_from_right(nq[s][c]));
+                }
+#pragma unroll
+                for (int c = 0; c < 3; ++c) Jx[c] = -(nQx[0][c] * vxr[0] + nQx[1][c] * vxr[1]);
+            }
+            // Ampere for Ex at row r
+            T exn[3];
+            {
+                const T cr[3] = {az[1] * by[2] - az[2] * by[1], az[2] * by[0] - az[0] * by[2],
+                                 az[0] * by[1] - az[1] * by[0]};
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    T dBdz = (by[c] - byp[c]) / dz;
+                    exn[c] = ex[c] + dt * (c2 * (-dBdz - aYM * cr[c]) - Jx[c] / eps0);
+                }
+            }
+            if (r > z0) {
+                // row r-1: Jz, Ampere for Ez, potential, Faraday
+                T ezn[3];
+                {
+                    T vzt[2], nQz[2][3];
+#pragma unroll
+                    for (int s = 0; s < 2; ++s) {
+                        vzt[s] = T(0.5) * (uzp[s] + uz[s]);
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) nQz[s][c] = T(0.5) * (nqp[s][c] + nq[s][c]);
+                    }
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        T Jz = -(nQz[0][c] * vzt[0] + nQz[1][c] * vzt[1]);
+                        T dBdx = (byp[c] - byxmp[c]) / dx;
+                        T ez0 = wall ? T(0.0) : ezp[c];
+                        ezn[c] = ez0 + dt * (c2 * dBdx - Jz / eps0);
+                    }
+                }
+                T ezr[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    ezr[c] = lane_from_right(ezn[c]);
+                    if (last_wall) ezr[c] = ezn[c];
+                }
+                T Az[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    Az[c] = azp[c];
+                    if (c > 0 || !p.freeze_az1) Az[c] = Az[c] - dt * ezn[c];
+                }
+                const T cr[3] = {Az[1] * exn_p[2] - Az[2] * exn_p[1], Az[2] * exn_p[0] - Az[0] * exn_p[2],
+                                 Az[0] * exn_p[1] - Az[1] * exn_p[0]};
+                if (store) {
+                    const unsigned i = rbase(r - 1) + (unsigned)xc;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        T dEx = (exn[c] - exn_p[c]) / dz;
+                        T dEz = (ezr[c] - ezn[c]) / dx;
+                        T byn = byp[c] - dt * (dEx - dEz - aYM * cr[c]);
+                        T exo = exn_p[c], ezo = ezn[c], azo = Az[c];
+                        if (c > 0 && act == 1) { byn *= dec; exo *= dec; ezo *= dec; azo *= dec; }
+                        if (c > 0 && act == 2) { byn = T(0.0); exo = T(0.0); ezo = T(0.0); azo = T(0.0); }
+                        st32(O.a[S_BY1 + c], i, byn);
+                        st32(O.a[S_EX1 + c], i, exo);
+                        st32(O.a[S_EZ1 + c], i, ezo);
+                        if (c > 0 || !p.freeze_az1) st32(O.a[S_AZ1 + c], i, azo);
+                    }
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) { byp[c] = by[c]; byxmp[c] = byxm[c]; azp[c] = az[c]; ezp[c] = ez[c]; exn_p[c] = exn[c]; }
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                uzp[s] = uz[s];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) nqp[s][c] = nq[s][c];
+            }
+        }
+        return;
+    }
+
+    // ---- z-sweep pipeline of species wv (k_fct_z with next-row prefetch) ----
+    const int sp = wv;
+    const int b = M::base(sp);
+    const T dtdz = T(p.dtdz);
+    auto zload = [&](int r, Row& row, T (&F)[NV], T (&V)[NV]) {
+        const unsigned i = (unsigned)xc + (unsigned)r * (unsigned)nx;
+        M::load_row(S, i, (unsigned)xm + (unsigned)r * (unsigned)nx, row);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) F[v] = ld32(S.a[b + v], i);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) V[v] = ld32(X.a[sp * NV + v], i);
+    };
+    int buf = 0;
+    // the chunk's own rows z0 .. (last ? nz-1 : zend) -- zend only for the
+    // Maxwell step -- then, in the last chunk, row 0 again for the step of
+    // row nz-1
+#pragma unroll 1
+    for (int pass = 0; pass < (last ? 2 : 1); ++pass) {
+        const bool own = pass == 0;
+        const int q0 = own ? z0 : 0;
+        const int elast = own ? (last ? nz + 2 : zend + 3) : 3;
+        int rc = clampi(q0 - 3, 0, nz - 1);
+        Row prev, cur;
+        T F[NV], V[NV];
+        {
+            T Fd[NV], Vd[NV];
+            zload(zm_per(rc, nz), prev, Fd, Vd);
+        }
+        zload(rc, cur, F, V);
+        ZPipe<T, NV> zp;
+        zp.Um = T(0);
+        zp.MUm = T(0);
+#pragma unroll 1
+        for (int e = q0 - 3; e <= elast; ++e) {
+            const int rn = clampi(min(e + 1, elast), 0, nz - 1);
+            Row nrow;
+            T nF[NV], nV[NV];
+            zload(rn, nrow, nF, nV);
+            T u, Sv[NV], out[NV];
+            M::z_src_rows(F, cur, prev, S, sp, xc, rc, p, u, Sv);
+            if (zp.push(e, q0, V, u, Sv, dt, dtdz, out)) {
+                const int r = e - 3;
+                if (out[0] < T(0.05)) out[0] = T(0.05);          // kernel_enforce_boundaries
+                if (wall) out[M::IPX] = T(0.0);
+                if (own && store && r < zend) {
+                    const unsigned io = (unsigned)xc + (unsigned)r * (unsigned)nx;
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) st32(O.a[b + v], io, out[v]);
+                }
+#pragma unroll
+                for (int v = 0; v < NV; ++v) sF[buf][sp][v][lane] = out[v];
+                __syncthreads();
+                buf ^= 1;
+            }
+            M::sel_row(rn != rc, cur, prev);
+            cur = nrow;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) { F[v] = nF[v]; V[v] = nV[v]; }
+            rc = rn;
+        }
+    }
+}
+
 // ==========================================================================
 // EMHD Maxwell: enforce_boundaries (Ey, Ez, Bx walls) + currents + Ampere +
 // vector potential + Faraday (emhdgpu2d/main_coupled.cu:48-63, 99-110;
@@ -1581,6 +1818,22 @@ static void launch_maxwell_zm_t(const State<T>& S, const State<T>& O, const KPar
                        decay, action);
 }
 
+// Fused z sweep + Maxwell, variant 1 / 2 / 3: 16 / 32 / 64-row chunks.
+template <typename T>
+void launch_fct_zmx(const State<T>& S, const Stage<T>& X, const State<T>& O, const KParams& p, const double* decay,
+                    const int* action, int variant, hipStream_t st) {
+    const int nwx = (p.nx + MZM_OUT - 1) / MZM_OUT;
+    const int tz = variant == 1 ? 16 : variant == 3 ? 64 : 32;
+    const int nch = (p.nz + tz - 1) / tz;
+    dim3 grid(round8(nwx * nch));
+    if (tz == 16)
+        hipLaunchKernelGGL((k_fct_zmx<T, 16>), grid, dim3(192), 0, st, S, X, O, p, nwx, nch, decay, action);
+    else if (tz == 64)
+        hipLaunchKernelGGL((k_fct_zmx<T, 64>), grid, dim3(192), 0, st, S, X, O, p, nwx, nch, decay, action);
+    else
+        hipLaunchKernelGGL((k_fct_zmx<T, 32>), grid, dim3(192), 0, st, S, X, O, p, nwx, nch, decay, action);
+}
+
 template <typename T>
 void launch_maxwell(const State<T>& S, const State<T>& O, const KParams& p, const double* decay,
                     const int* action, int variant, hipStream_t st) {
@@ -1622,6 +1875,10 @@ template void launch_emhd_maxwell<double>(const State<double>&, const State<doub
 template void launch_emhd_maxwell<float>(const State<float>&, const State<float>&, const KParams&, hipStream_t);
 template void launch_fct_xz<double>(const State<double>&, const State<double>&, const KParams&, int, hipStream_t);
 template void launch_fct_xz<float>(const State<float>&, const State<float>&, const KParams&, int, hipStream_t);
+template void launch_fct_zmx<double>(const State<double>&, const Stage<double>&, const State<double>&, const KParams&,
+                                     const double*, const int*, int, hipStream_t);
+template void launch_fct_zmx<float>(const State<float>&, const Stage<float>&, const State<float>&, const KParams&,
+                                    const double*, const int*, int, hipStream_t);
 template void launch_maxwell<double>(const State<double>&, const State<double>&, const KParams&, const double*, const int*, int, hipStream_t);
 template void launch_maxwell<float>(const State<float>&, const State<float>&, const KParams&, const double*, const int*, int, hipStream_t);
 

@@ -75,7 +75,7 @@ class khs_config(ctypes.Structure):
         ("charge1", ctypes.c_double), ("charge2", ctypes.c_double), ("emhd_init", ctypes.c_int),
         ("smooth_init", ctypes.c_int), ("spectrum_kmax", ctypes.c_int), ("spectrum_amp", ctypes.c_double),
         ("spectrum_seed", ctypes.c_uint64), ("em_perturb_amp", ctypes.c_double),
-        ("z_groups", ctypes.c_int), ("xz_variant", ctypes.c_int), ("x_serial", ctypes.c_int), ("maxwell_kernel", ctypes.c_int), ("x_rows", ctypes.c_int), ("z_prefetch", ctypes.c_int), ("mfma_filter", ctypes.c_int), ("x_lane_cells", ctypes.c_int), ("snake_order", ctypes.c_int),
+        ("z_groups", ctypes.c_int), ("xz_variant", ctypes.c_int), ("x_serial", ctypes.c_int), ("maxwell_kernel", ctypes.c_int), ("x_rows", ctypes.c_int), ("z_prefetch", ctypes.c_int), ("mfma_filter", ctypes.c_int), ("x_lane_cells", ctypes.c_int), ("snake_order", ctypes.c_int), ("zm_fused", ctypes.c_int), ("step_graphs", ctypes.c_int),
         ("nx", ctypes.c_int), ("nz", ctypes.c_int), ("periodic_x", ctypes.c_int), ("freeze_az1", ctypes.c_int),
         ("lx", ctypes.c_double), ("lz", ctypes.c_double), ("dx", ctypes.c_double), ("dz", ctypes.c_double),
         ("courant", ctypes.c_double), ("dt", ctypes.c_double),
@@ -153,6 +153,8 @@ def _signatures():
         "khs_batch_get_array": (c_int, [c_void_p, c_int, c_int, dp, c_size_t]),
         "khs_batch_set_array": (c_int, [c_void_p, c_int, c_int, dp, c_size_t]),
         "khs_batch_energy": (c_int, [c_void_p, c_int, dp, dp]),
+        "khs_batch_mode_amplitude": (c_int, [c_void_p, c_int, c_int, dp]),
+        "khs_batch_set_step_limit": (c_int, [c_void_p, c_int, c_int]),
     }
 
 
@@ -406,10 +408,10 @@ class Simulation:
 
     def tuning(self):
         """Tile shapes and kernel variants in use (khs_get_tuning)."""
-        out = (ctypes.c_int * 9)()
+        out = (ctypes.c_int * 10)()
         _check(_lib.khs_get_tuning(self._h, out))
         return dict(x_segment=out[0], x_rows=out[1], z_chunk=out[2], z_prefetch=out[3], maxwell=out[4],
-                    x_serial=out[5], xz_variant=out[6], z_groups=out[7], x_lane_cells=out[8])
+                    x_serial=out[5], xz_variant=out[6], z_groups=out[7], x_lane_cells=out[8], zm_fused=out[9])
 
     def kernel_times(self):
         n = 8
@@ -487,3 +489,13 @@ class Batch:
         e, e0 = ctypes.c_double(), ctypes.c_double()
         _check(_lib.khs_batch_energy(self._h, int(run), ctypes.byref(e), ctypes.byref(e0)))
         return e.value, e0.value
+
+    def set_step_limit(self, run, max_step):
+        """run stops at max_step (<= 0: no limit) without being marked halted."""
+        _check(_lib.khs_batch_set_step_limit(self._h, int(run), int(max_step)))
+
+    def mode_amplitude(self, run, kz):
+        """Circular Az2 + i Az3 amplitude of run `run` at z mode kz -> (amp, amp_conj)."""
+        out = (ctypes.c_double * 2)()
+        _check(_lib.khs_batch_mode_amplitude(self._h, int(run), int(kz), out))
+        return out[0], out[1]

@@ -194,7 +194,7 @@ def test_tile_variants_bitwise(xs, xr, zc, pf, mk, xv, zg, periodic):
     cfg, sim, o = make_pair(st, **keys)
     t = sim.tuning()
     assert t == dict(x_segment=xs, x_rows=xr, z_chunk=zc, z_prefetch=pf, maxwell=mk, x_serial=xv,
-                     xz_variant=t["xz_variant"], z_groups=zg, x_lane_cells=0)
+                     xz_variant=t["xz_variant"], z_groups=zg, x_lane_cells=0, zm_fused=0)
     o.sources()
     sim.step(3)
     o.step(3)
@@ -219,6 +219,27 @@ def test_x_register_form_bitwise(cells, shape, periodic, warm):
     o.step(3)
     for n in orc.STATE:
         assert_bitwise(sim.get(n), o.get(n), f"x lane cells {cells} {shape} {n}")
+
+
+@pytest.mark.parametrize("zm", [1, 2, 3])
+@pytest.mark.parametrize("shape,periodic,freeze,warm", [((300, 136), 1, 1, 0.01), ((70, 36), 0, 1, 0.0),
+                                                        ((256, 64), 0, 0, 0.01), ((130, 100), 1, 0, 0.0),
+                                                        ((64, 16), 0, 1, 0.0)])
+def test_fused_z_maxwell_bitwise(zm, shape, periodic, freeze, warm):
+    """Fused z sweep + Maxwell kernel (three wavefronts per workgroup, LDS
+    hand-over, row 0 recomputed for the last chunk) gives the oracle's bits:
+    chunk heights that do and do not divide nz, a single chunk, walls and
+    periodic x, warm sources, sponge and cut, Az1 frozen or not."""
+    nx, nz = shape
+    keys = grid(nx, nz, periodic, zm_fused=zm, autotune=0, xi_sponge=8.0, xi_cut=12.0, freeze_override=freeze,
+                warm_T=warm)
+    cfg, sim, o = make_pair(random_state(nx, nz, seed=nx * 3 + zm, smooth=True), **keys)
+    assert sim.tuning()["zm_fused"] == zm
+    o.sources()
+    sim.step(3)
+    o.step(3)
+    for n in orc.STATE:
+        assert_bitwise(sim.get(n), o.get(n), f"zm {zm} {shape} {n}")
 
 
 @pytest.mark.parametrize("mk", [4, 5, 6, 7, 8, 9])
