@@ -64,17 +64,23 @@ ZM_ROWS = {1: 16, 2: 32, 3: 64}       # chunk height of the fused z sweep + Maxw
 # by what the counter reads for a plain 256 MiB device copy (fetch x2.0,
 # write x1.0); summary under profiles/.  Keyed by kernel instance.
 PMC_TRAFFIC_MB = {
-    "k_fct_x<double, 64, YMModel, 0>": 511.3, "k_fct_x<double, 128, YMModel, 0>": 511.3,
-    "k_fct_x<double, 256, YMModel, 0>": 509.2, "k_fct_x<double, 64, YMModel, 1>": 523.8,
-    "k_fct_x<double, 128, YMModel, 1>": 522.6, "k_fct_x<double, 256, YMModel, 1>": 509.8,
-    "k_fct_z<double, 16, 0, YMModel>": 981.4, "k_fct_z<double, 16, 1, YMModel>": 972.3,
-    "k_fct_z<double, 32, 0, YMModel>": 874.4, "k_fct_z<double, 32, 1, YMModel>": 875.7,
-    "k_fct_z<double, 64, 0, YMModel>": 806.5, "k_fct_z<double, 64, 1, YMModel>": 823.2,
-    "k_fct_zg<double, 16, 0, YMModel>": 1051.1, "k_fct_zg<double, 16, 1, YMModel>": 994.0,
-    "k_fct_zg<double, 32, 0, YMModel>": 982.7, "k_fct_zg<double, 32, 1, YMModel>": 930.7,
-    "k_fct_zg<double, 64, 0, YMModel>": 932.8, "k_fct_zg<double, 64, 1, YMModel>": 958.2,
-    "k_maxwell<double, 16>": 847.0, "k_maxwell<double, 4>": 717.9, "k_maxwell<double, 8>": 856.2,
-    "k_maxwell_cell<double>": 594.0,
+    "k_fct_x<double, 128, YMModel, 0>": 511.5, "k_fct_x<double, 128, YMModel, 1>": 524.2,
+    "k_fct_x<double, 256, YMModel, 0>": 509.4, "k_fct_x<double, 256, YMModel, 1>": 524.5,
+    "k_fct_x<double, 64, YMModel, 0>": 511.9, "k_fct_x<double, 64, YMModel, 1>": 524.1,
+    "k_fct_xr<double, 4>": 506.4, "k_fct_xr<double, 8>": 505.8,
+    "k_fct_z<double, 16, 0, YMModel>": 985.3, "k_fct_z<double, 16, 1, YMModel>": 972.6,
+    "k_fct_z<double, 32, 0, YMModel>": 874.0, "k_fct_z<double, 32, 1, YMModel>": 869.4,
+    "k_fct_z<double, 64, 0, YMModel>": 806.7, "k_fct_z<double, 64, 1, YMModel>": 822.7,
+    "k_fct_zg<double, 16, 0, YMModel>": 1050.8, "k_fct_zg<double, 16, 1, YMModel>": 994.5,
+    "k_fct_zg<double, 32, 0, YMModel>": 987.2, "k_fct_zg<double, 32, 1, YMModel>": 930.8,
+    "k_fct_zg<double, 64, 0, YMModel>": 933.8, "k_fct_zg<double, 64, 1, YMModel>": 952.9,
+    "k_fct_zmx<double, 16>": 1262.3, "k_fct_zmx<double, 32>": 1143.5,
+    "k_fct_zmx<double, 64>": 1101.5, "k_maxwell<double, 16>": 842.9,
+    "k_maxwell<double, 4>": 712.0, "k_maxwell<double, 8>": 854.2,
+    "k_maxwell_cell<double>": 596.4, "k_maxwell_zm<double, 16, 0>": 652.1,
+    "k_maxwell_zm<double, 16, 1>": 636.3, "k_maxwell_zm<double, 32, 1>": 621.9,
+    "k_maxwell_zm<double, 4, 1>": 693.8, "k_maxwell_zm<double, 8, 0>": 688.2,
+    "k_maxwell_zm<double, 8, 1>": 659.3,
 }
 
 
