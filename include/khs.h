@@ -132,7 +132,7 @@ typedef struct khs_config {
     int      maxwell_kernel;   /* YM Maxwell: -1 = autotuned (default), 0/2/3 = 64x16/64x4/64x8 LDS tiles, 1 = per cell,
                                   4..9 = z march with 16/32/16/8/8/4-row chunks (6, 8: without next-row prefetch) */
     int      x_rows;           /* x sweep rows per workgroup: 0 = autotuned (default), 1, 4 or 16 */
-    int      z_prefetch;       /* z sweep: -1 = autotuned (default), 0 = load each row when used, 1 = prefetch the next row */
+    int      z_prefetch;       /* z sweep: -1 = autotuned (default), 0 = load each row when used, 1 / 2 = load one / two rows ahead */
     int      mfma_filter;      /* keep-|kz|<=K filter as MFMA contractions (1, default) or the per-column FP64-sum kernels (0) */
     int      x_lane_cells;     /* x sweep form: -1 = autotuned (default), 0 = LDS-staged segments, 4 or 8 = register form with that many cells per lane (YM) */
     int      snake_order;      /* 1 (default): consecutive step kernels walk the grid in opposite directions (cache reuse); 0: all forward */
@@ -204,7 +204,24 @@ int khs_get_array_buf(khs_ctx* ctx, int buf, int id, double* host, size_t n);
  * (they are computed from the end-of-step state, main_ym.cu:450-455). */
 int khs_fct_x_sweep(khs_ctx* ctx, int species);   /* ym_fct_x_sweep[_periodic], YM_FCT.cu:30-91 */
 int khs_fct_z_sweep(khs_ctx* ctx, int species);   /* ym_fct_z_sweep, YM_FCT.cu:93-135; + enforce_boundaries on the fluid */
-int khs_maxwell(khs_ctx* ctx);                    /* enforce_boundaries(Ez) + currents + Ampere + potential + Faraday */
+int khs_maxwell(khs_ctx* ctx);
+/* kernel_update_velocities (YM_Fluid.cu:3-11; EMHD update_velocities): ux =
+ * px / max(n, 0.05) and uz = pz / max(n, 0.05) of species s of the committed
+ * state, into host buffers of nx*nz doubles.  The sweeps evaluate the same
+ * values on the fly; this exports them. */
+int khs_fluid_velocities(khs_ctx* ctx, int species, double* ux, double* uz, size_t n);
+/* kernel_ym_lorentz / kernel_ym_precession (YM_Fluid.cu:32-108): the sources
+ * that the next step's sweeps add for species s (px and pz momentum sources,
+ * then the three colour precession rates in sQ[0..3n)), evaluated on the
+ * committed state.  EMHD: kernel_compute_lorentz's (sx, sy, sz) into src_px,
+ * src_pz and sQ[0..n) (sy).  Host buffers of n = nx*nz doubles (sQ 3n). */
+int khs_sources(khs_ctx* ctx, int species, double* src_px, double* src_pz, double* sQ, size_t n);
+/* kernel_enforce_boundaries (YM_Fluid.cu:13-30; EMHD main_coupled.cu:48-63)
+ * on the committed state, in place: density floor 0.05; with walls in x,
+ * px = 0 and Ez^a = 0 (EMHD: Ey, Ez, Bx = 0) in the wall columns.  khs_step
+ * and khs_fct_z_sweep / khs_maxwell apply the same conditions as part of
+ * their outputs; this operator serves callers that write arrays directly. */
+int khs_enforce_boundaries(khs_ctx* ctx);                    /* enforce_boundaries(Ez) + currents + Ampere + potential + Faraday */
 int khs_filters(khs_ctx* ctx);                    /* sponge, xi_cut, z-mean, kz bands, hyperdiffusion, colour-1 zeroing */
 int khs_commit(khs_ctx* ctx);
 

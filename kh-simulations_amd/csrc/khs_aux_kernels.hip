@@ -165,6 +165,39 @@ void launch_fill_zero(const ArrayList<T>& arrs, size_t n, hipStream_t st) {
     hipLaunchKernelGGL(k_fill_zero<T>, dim3(blocks), dim3(256), 0, st, arrs, n);
 }
 
+// kernel_enforce_boundaries (YM_Fluid.cu:13-30; EMHD main_coupled.cu:48-63)
+// on one state set, in place: density floor 0.05 of every fluid; with walls
+// in x, px = 0 in the wall columns and the wall fields (YM: Ez^a; EMHD: Ey,
+// Ez, Bx) = 0.  Cell-local.
+template <typename T>
+__global__ void k_enforce_boundaries(State<T> S, KParams p, int emhd) {
+    const size_t n = (size_t)p.nx * p.nz;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int x = (int)(i % (size_t)p.nx);
+        const bool wall = !p.periodic_x && (x == 0 || x == p.nx - 1);
+        for (int s = 0; s < p.nsp; ++s) {
+            const int b = emhd ? 12 + 6 * s : S_NA + s * F_COUNT;
+            T* nn = S.a[b];
+            if (nn[i] < T(0.05)) nn[i] = T(0.05);
+            if (wall) S.a[b + 1][i] = T(0.0);                 // px (both models: slot b + 1)
+        }
+        if (wall) {
+            if (emhd) {
+                S.a[1][i] = T(0.0); S.a[2][i] = T(0.0); S.a[3][i] = T(0.0);   // Ey, Ez, Bx
+            } else {
+                S.a[S_EZ1][i] = T(0.0); S.a[S_EZ2][i] = T(0.0); S.a[S_EZ3][i] = T(0.0);
+            }
+        }
+    }
+}
+
+template <typename T>
+void launch_enforce_boundaries(const State<T>& S, const KParams& p, int emhd, hipStream_t st) {
+    const size_t n = (size_t)p.nx * p.nz;
+    int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_enforce_boundaries<T>, dim3(blocks), dim3(256), 0, st, S, p, emhd);
+}
+
 // --------------------------------------------------------------------------
 // compensated reductions
 // --------------------------------------------------------------------------
@@ -527,6 +560,7 @@ void launch_spectrum_f64(const double* f, const double* ztab, const double* xtab
     template void launch_column_modes<T, double>(const ArrayList<T>&, const double*, const double*, const double*, int, int, double*, const KParams&, hipStream_t); \
     template void launch_hyperdiffuse<T>(const ArrayList<T>&, double, const KParams&, hipStream_t); \
     template void launch_fill_zero<T>(const ArrayList<T>&, size_t, hipStream_t);                    \
+    template void launch_enforce_boundaries<T>(const State<T>&, const KParams&, int, hipStream_t);  \
     template void launch_energy<T>(const State<T>&, const KParams&, double*, double*, hipStream_t);  \
     template void launch_gauss<T>(const State<T>&, const KParams&, double*, double*, hipStream_t);   \
     template void launch_derived<T>(const State<T>&, int, const KParams&, double*, hipStream_t);   \

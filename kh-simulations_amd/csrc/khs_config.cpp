@@ -239,7 +239,7 @@ static void config_derive_emhd(khs_config& c) {
         throw Error(KHS_ERR_CONFIG, "x_segment must be 0, 64, 128 or 256");
     if (c.z_chunk != 0 && c.z_chunk != 16 && c.z_chunk != 32 && c.z_chunk != 64)
         throw Error(KHS_ERR_CONFIG, "z_chunk must be 0, 16, 32 or 64");
-    if (c.z_prefetch < -1 || c.z_prefetch > 1) throw Error(KHS_ERR_CONFIG, "z_prefetch must be -1, 0 or 1");
+    if (c.z_prefetch < -1 || c.z_prefetch > 2) throw Error(KHS_ERR_CONFIG, "z_prefetch must be -1, 0, 1 or 2");
     if (c.x_rows != 0 && c.x_rows != 1 && c.x_rows != 4 && c.x_rows != 16)
         throw Error(KHS_ERR_CONFIG, "x_rows must be 0, 1, 4 or 16");
     if (c.maxwell_kernel < -1 || c.maxwell_kernel > 9) throw Error(KHS_ERR_CONFIG, "maxwell_kernel must be -1..9");
@@ -305,7 +305,7 @@ void config_derive(khs_config& c) {
         throw Error(KHS_ERR_CONFIG, "x_segment must be 0, 64, 128 or 256");
     if (c.z_chunk != 0 && c.z_chunk != 16 && c.z_chunk != 32 && c.z_chunk != 64)
         throw Error(KHS_ERR_CONFIG, "z_chunk must be 0, 16, 32 or 64");
-    if (c.z_prefetch < -1 || c.z_prefetch > 1) throw Error(KHS_ERR_CONFIG, "z_prefetch must be -1, 0 or 1");
+    if (c.z_prefetch < -1 || c.z_prefetch > 2) throw Error(KHS_ERR_CONFIG, "z_prefetch must be -1, 0, 1 or 2");
     if (c.x_rows != 0 && c.x_rows != 1 && c.x_rows != 4 && c.x_rows != 16)
         throw Error(KHS_ERR_CONFIG, "x_rows must be 0, 1, 4 or 16");
     if (c.maxwell_kernel < -1 || c.maxwell_kernel > 9) throw Error(KHS_ERR_CONFIG, "maxwell_kernel must be -1..9");

@@ -240,12 +240,14 @@ struct Ctx {
         }
         const int fg = cfg.z_groups;             // -1 = tuned
         for (int v = 0; v < kNumZVariants; ++v)
-            if ((fz < 0 || v % 3 == fz) && (fp < 0 || (v / 3) % 2 == fp) && (fg < 0 || v / 6 == fg)) zc.push_back(v);
+            if ((fz < 0 || v % 3 == fz) && (fp < 0 || z_prefetch_of(v) == fp) && (fg < 0 || z_groups_of(v) == fg))
+                zc.push_back(v);
         // without tuning: the forced shape, else 256-cell segments and 32-row
         // chunks with prefetch in FP32 (registers to spare), without in FP64
         xvar = ((fx >= 0) ? fx : 2) + 3 * ((fr >= 0) ? fr : 1) + 9 * ((fv >= 0) ? fv : 1);
         if (!emhd && fl > 0) xvar = fl == 4 ? 18 : 19;
         zvar = ((fz >= 0) ? fz : 1) + 3 * ((fp >= 0) ? fp : (fp32 ? 1 : 0)) + 6 * ((fg >= 0) ? fg : 0);
+        if (fp == 2) zvar = 12 + ((fz >= 0) ? fz : 1);
         mvar = (cfg.maxwell_kernel >= 0) ? cfg.maxwell_kernel : 0;
         xzvar = (cfg.xz_variant >= 0) ? cfg.xz_variant : 2;
         if (!cfg.autotune) return;
@@ -509,6 +511,7 @@ static std::unique_ptr<Ctx> make_ctx(const khs_config& in) {
     if (c.model != KHS_MODEL_EMHD && c.x_lane_cells > 0) ctx->xvar = c.x_lane_cells == 4 ? 18 : 19;
     if (c.z_chunk) ctx->zvar = c.z_chunk == 16 ? 0 : c.z_chunk == 32 ? 1 : 2;
     if (c.z_prefetch == 1) ctx->zvar = ctx->zvar % 3 + 3;
+    if (c.z_prefetch == 2) ctx->zvar = ctx->zvar % 3 + 12;
     if (c.z_groups == 1) ctx->zvar += 6;
     if (c.maxwell_kernel >= 0) ctx->mvar = c.maxwell_kernel;
     if (c.xz_variant >= 0) ctx->xzvar = c.xz_variant;
@@ -988,6 +991,51 @@ int khs_fct_z_sweep(khs_ctx* ctx, int species) {
     });
 }
 
+int khs_fluid_velocities(khs_ctx* ctx, int species, double* ux, double* uz, size_t n) {
+    CTX_OR_FAIL(ctx);
+    Ctx& c = *ctx->impl;
+    if (species < 0 || species >= c.nsp) { g_last_error = "species out of range"; return KHS_ERR_ARG; }
+    const int idx = c.emhd ? (species ? KHS_EM_UX2 : KHS_EM_UX) : (species ? KHS_UXB : KHS_UXA);
+    const int idz = c.emhd ? (species ? KHS_EM_UZ2 : KHS_EM_UZ) : (species ? KHS_UZB : KHS_UZA);
+    const int rc = khs_get_array(ctx, idx, ux, n);
+    return rc != KHS_OK ? rc : khs_get_array(ctx, idz, uz, n);
+}
+
+int khs_sources(khs_ctx* ctx, int species, double* src_px, double* src_pz, double* sQ, size_t n) {
+    CTX_OR_FAIL(ctx);
+    Ctx& c = *ctx->impl;
+    if (species < 0 || species >= c.nsp) { g_last_error = "species out of range"; return KHS_ERR_ARG; }
+    if (!sQ) { g_last_error = "null argument"; return KHS_ERR_ARG; }
+    int ids[5];
+    int nq;
+    if (c.emhd) {
+        const int o = species ? 3 : 0;
+        ids[0] = KHS_EM_SX + o; ids[1] = KHS_EM_SZ + o; ids[2] = KHS_EM_SY + o;
+        nq = 1;
+    } else {
+        ids[0] = species ? KHS_SRCPXB : KHS_SRCPXA;
+        ids[1] = species ? KHS_SRCPZB : KHS_SRCPZA;
+        for (int k = 0; k < 3; ++k) ids[2 + k] = (species ? KHS_SQ1B : KHS_SQ1A) + k;
+        nq = 3;
+    }
+    int rc = khs_get_array(ctx, ids[0], src_px, n);
+    if (rc == KHS_OK) rc = khs_get_array(ctx, ids[1], src_pz, n);
+    for (int k = 0; k < nq && rc == KHS_OK; ++k) rc = khs_get_array(ctx, ids[2 + k], sQ + (size_t)k * n, n);
+    return rc;
+}
+
+int khs_enforce_boundaries(khs_ctx* ctx) {
+    CTX_OR_FAIL(ctx);
+    return guard([&] {
+        Ctx& c = *ctx->impl;
+        khs::require_idle(c, "khs_enforce_boundaries");
+        if (c.fp32) khs::launch_enforce_boundaries<float>(c.state<float>(c.cur), c.kp, c.emhd ? 1 : 0, c.stream);
+        else khs::launch_enforce_boundaries<double>(c.state<double>(c.cur), c.kp, c.emhd ? 1 : 0, c.stream);
+        c.check_launch();
+        HIPCHK(hipStreamSynchronize(c.stream));
+    });
+}
+
 int khs_maxwell(khs_ctx* ctx) {
     CTX_OR_FAIL(ctx);
     return guard([&] {
@@ -1397,11 +1445,11 @@ int khs_get_tuning(khs_ctx* ctx, int out[10]) {
     out[0] = xs[c.xvar % 3];
     out[1] = xr[(c.xvar / 3) % 3];
     out[2] = zs[c.zvar % 3];
-    out[3] = (c.zvar / 3) % 2;
+    out[3] = khs::z_prefetch_of(c.zvar);
     out[4] = c.mvar;
     out[5] = c.xvar / 9;
     out[6] = c.xzvar;
-    out[7] = c.zvar / 6;
+    out[7] = khs::z_groups_of(c.zvar);
     out[8] = 0;
     out[9] = c.zmvar;
     if (c.xvar >= 18) {

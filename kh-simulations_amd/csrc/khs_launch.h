@@ -14,7 +14,9 @@ namespace khs {
 // variants 18 / 19: register form, 4 / 8 cells per lane, lane neighbours by
 // DPP (YM model).
 constexpr int kNumXVariants = 20;
-constexpr int kNumZVariants = 12;
+constexpr int kNumZVariants = 15;          // 12..14: 16/32/64-row chunks, loads two rows ahead
+inline int z_prefetch_of(int v) { return v >= 12 ? 2 : (v / 3) % 2; }
+inline int z_groups_of(int v) { return v >= 12 ? 0 : v / 6; }
 constexpr int kNumMaxwellVariants = 10;
 // M: YMModel (ymgpu2d two-beam SU(2) fluid) or EMModel (emhdgpu2d Abelian fluid)
 template <typename T, typename M>
@@ -90,6 +92,8 @@ template <typename T>
 void launch_hyperdiffuse(const ArrayList<T>& arrs, double mu, const KParams& p, hipStream_t st);
 template <typename T>
 void launch_fill_zero(const ArrayList<T>& arrs, size_t n, hipStream_t st);
+template <typename T>
+void launch_enforce_boundaries(const State<T>& S, const KParams& p, int emhd, hipStream_t st);
 
 // diagnostics
 constexpr int kRedBlocks = 1024;

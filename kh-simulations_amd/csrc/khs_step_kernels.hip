@@ -538,6 +538,79 @@ __global__ __launch_bounds__(ZB_X * ZB_C) void k_fct_z(State<T> S, Stage<T> X, S
     }
 }
 
+// The same z march with the loads issued two rows ahead: at occupancy one
+// (the pipeline state of six variables fills the register file) a one-row
+// prefetch leaves part of the load latency exposed; the extra row of inputs
+// lives in the otherwise unused accumulation registers.
+template <typename T, int TZ, typename M>
+__global__ __launch_bounds__(ZB_X * ZB_C) void k_fct_z2(State<T> S, Stage<T> X, State<T> O,
+                                                        KParams p, int sp0, int nsp, int ntx, int ncg) {
+    constexpr int NV = M::NV;
+    typedef typename M::template Row<T> Row;
+    int lt = xcd_logical(blockIdx.x, gridDim.x);
+    if (lt >= nsp * ntx * ncg) return;
+    lt = tile_order(lt, nsp * ntx * ncg, p.rev);
+    const int sp = sp0 + lt % nsp;
+    lt /= nsp;
+    const int x = (lt % ntx) * ZB_X + (threadIdx.x % ZB_X);
+    const int chunk = (lt / ntx) * ZB_C + (threadIdx.x / ZB_X);
+    const int z0 = chunk * TZ;
+    if (x >= p.nx || z0 >= p.nz) return;
+    const int nx = p.nx, nz = p.nz;
+    const int b = M::base(sp);
+    const int xm = xm_src(x, p);
+    const T dt = T(p.dt), dtdz = T(p.dtdz);
+    const int zend = min(z0 + TZ, nz);
+    const int elast = zend + 2;
+    const bool wall = !p.periodic_x && (x == 0 || x == nx - 1);
+
+    auto load = [&](int r, Row& row, T (&F)[NV], T (&V)[NV]) {
+        const unsigned i = (unsigned)x + (unsigned)r * (unsigned)nx;
+        M::load_row(S, i, (unsigned)xm + (unsigned)r * (unsigned)nx, row);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) F[v] = ld32(S.a[b + v], i);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) V[v] = ld32(X.a[sp * NV + v], i);
+    };
+    auto row_at = [&](int e) { return clampi(min(e, elast), 0, nz - 1); };
+
+    int rc = row_at(z0 - 3);
+    Row prev, cur, nrow;
+    T F[NV], V[NV], nF[NV], nV[NV];
+    {
+        T Fd[NV], Vd[NV];
+        load(zm_per(rc, nz), prev, Fd, Vd);
+    }
+    load(rc, cur, F, V);
+    int rn = row_at(z0 - 2);
+    load(rn, nrow, nF, nV);
+    ZPipe<T, NV> zp;
+    zp.Um = T(0);
+    zp.MUm = T(0);
+    for (int e = z0 - 3; e <= elast; ++e) {
+        const int rnn = row_at(e + 2);
+        Row nnrow;
+        T nnF[NV], nnV[NV];
+        load(rnn, nnrow, nnF, nnV);
+        T u, Sv[NV], out[NV];
+        M::z_src_rows(F, cur, prev, S, sp, x, rc, p, u, Sv);
+        if (zp.push(e, z0, V, u, Sv, dt, dtdz, out)) {
+            if (out[0] < T(0.05)) out[0] = T(0.05);          // kernel_enforce_boundaries
+            if (wall) out[M::IPX] = T(0.0);
+            const unsigned io = (unsigned)x + (unsigned)(e - 3) * (unsigned)nx;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) st32(O.a[b + v], io, out[v]);
+        }
+        M::sel_row(rn != rc, cur, prev);
+        cur = nrow;
+        nrow = nnrow;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { F[v] = nF[v]; V[v] = nV[v]; nF[v] = nnF[v]; nV[v] = nnV[v]; }
+        rc = rn;
+        rn = rnn;
+    }
+}
+
 // The same z march with the variables split into two groups (model's
 // NG = NV/2 each) on two waves: half the pipeline state per lane, so more
 // waves fit; both waves of a column read the cell's fluid for the sources
@@ -1575,6 +1648,12 @@ static void launch_fct_z_t(const State<T>& S, const Stage<T>& X, const State<T>&
                            int sp0, int nsp, int grouped, hipStream_t st) {
     const int nchunk = (p.nz + TZ - 1) / TZ;
     const int ntx = (p.nx + ZB_X - 1) / ZB_X;
+    if (PF == 2) {
+        const int ncg = (nchunk + ZB_C - 1) / ZB_C;
+        hipLaunchKernelGGL((k_fct_z2<T, TZ, M>), dim3(round8(nsp * ntx * ncg)), dim3(ZB_X * ZB_C), 0, st, S, X, O, p,
+                           sp0, nsp, ntx, ncg);
+        return;
+    }
     if (grouped) {
         const int ncg = (nchunk + 1) / 2;
         hipLaunchKernelGGL((k_fct_zg<T, TZ, PF, M>), dim3(round8(nsp * ntx * ncg)), dim3(256), 0, st, S, X, O, p,
@@ -1593,6 +1672,12 @@ static void launch_fct_z_t(const State<T>& S, const Stage<T>& X, const State<T>&
 template <typename T, typename M>
 void launch_fct_z(const State<T>& S, const Stage<T>& X, const State<T>& O, const KParams& p,
                   int sp0, int nsp, int variant, hipStream_t st) {
+    switch (variant) {                       // two-row prefetch
+        case 12: launch_fct_z_t<T, 16, 2, M>(S, X, O, p, sp0, nsp, 0, st); return;
+        case 13: launch_fct_z_t<T, 32, 2, M>(S, X, O, p, sp0, nsp, 0, st); return;
+        case 14: launch_fct_z_t<T, 64, 2, M>(S, X, O, p, sp0, nsp, 0, st); return;
+        default: break;
+    }
     const int gr = variant / 6;
     switch (variant % 6) {
         case 0: launch_fct_z_t<T, 16, 0, M>(S, X, O, p, sp0, nsp, gr, st); break;

@@ -140,6 +140,9 @@ def _signatures():
         "khs_kernel_times": (c_int, [c_void_p, dp, P(c_int), c_int]),
         "khs_kernel_name": (c_char_p, [c_int]),
         "khs_get_tuning": (c_int, [c_void_p, P(c_int)]),
+        "khs_fluid_velocities": (c_int, [c_void_p, c_int, dp, dp, c_size_t]),
+        "khs_sources": (c_int, [c_void_p, c_int, dp, dp, dp, c_size_t]),
+        "khs_enforce_boundaries": (c_int, [c_void_p]),
         "khs_checkpoint_write": (c_int, [c_void_p, c_char_p]),
         "khs_checkpoint_read": (c_int, [c_void_p, c_char_p]),
         "khs_mode_amplitude": (c_int, [c_void_p, c_int, dp]),
@@ -310,6 +313,25 @@ class Simulation:
 
     def fct_z_sweep(self, species):
         _check(_lib.khs_fct_z_sweep(self._h, species))
+
+    def fluid_velocities(self, species):
+        """(ux, uz) of species s (kernel_update_velocities) -> two (nz, nx) arrays."""
+        c = self.config
+        ux, uz = np.empty((c.nz, c.nx)), np.empty((c.nz, c.nx))
+        _check(_lib.khs_fluid_velocities(self._h, int(species), _dptr(ux), _dptr(uz), ux.size))
+        return ux, uz
+
+    def sources(self, species):
+        """(src_px, src_pz, sQ) of species s; sQ has 3 (YM) or 1 (EMHD: sy) rows."""
+        c = self.config
+        px, pz = np.empty((c.nz, c.nx)), np.empty((c.nz, c.nx))
+        nq = 1 if c.model == MODEL_EMHD else 3
+        sq = np.empty((nq, c.nz, c.nx))
+        _check(_lib.khs_sources(self._h, int(species), _dptr(px), _dptr(pz), _dptr(sq), px.size))
+        return px, pz, sq
+
+    def enforce_boundaries(self):
+        _check(_lib.khs_enforce_boundaries(self._h))
 
     def maxwell(self):
         _check(_lib.khs_maxwell(self._h))

@@ -122,7 +122,7 @@ def test_step_bitwise(case):
         assert_bitwise(sim.get(n), o.get(n), f"{case} {n}")
 
 
-@pytest.mark.parametrize("zc,pf,zg", [(16, 1, 0), (64, 0, 1), (32, 1, 1), (16, 0, 1)])
+@pytest.mark.parametrize("zc,pf,zg", [(16, 1, 0), (64, 0, 1), (32, 1, 1), (16, 0, 1), (32, 2, 0)])
 def test_tile_variants_bitwise(zc, pf, zg):
     nx, nz, periodic, nsp = 130, 70, 0, 2
     keys = em_keys(nx, nz, periodic, nsp, z_chunk=zc, z_prefetch=pf, z_groups=zg, x_segment=64, x_rows=16,
@@ -167,6 +167,31 @@ def test_derived_bitwise(periodic, nsp):
     Eo = o.energy()
     assert_bitwise(sim.get("energy_density"), o.get("em_energy_density"), "energy density")
     assert abs(sim.energy() - Eo) <= 1e-12 * abs(Eo)
+
+
+@pytest.mark.parametrize("periodic,nsp", [(0, 2), (1, 1)])
+def test_operator_entry_points_bitwise(periodic, nsp):
+    """khs_fluid_velocities, khs_sources (sx, sz, sy) and
+    khs_enforce_boundaries against the EMHD oracle."""
+    nx, nz = 150, 40
+    st = em_random(nx, nz, nsp, seed=27 + nsp)
+    st["n"][2, :5] = 0.01
+    cfg, sim, o = em_pair(st, **em_keys(nx, nz, periodic, nsp))
+    o.lorentz()
+    o.update_velocities()
+    for s in range(nsp):
+        tag = "2" if s else ""
+        ux, uz = sim.fluid_velocities(s)
+        assert_bitwise(ux, o.get("ux" + tag), "ux" + tag)
+        assert_bitwise(uz, o.get("uz" + tag), "uz" + tag)
+        sx, sz, sy = sim.sources(s)
+        assert_bitwise(sx, o.get("sx" + tag), "sx" + tag)
+        assert_bitwise(sz, o.get("sz" + tag), "sz" + tag)
+        assert_bitwise(sy[0], o.get("sy" + tag), "sy" + tag)
+    sim.enforce_boundaries()
+    o.enforce_boundaries()
+    for n in orc.em_state_names(nsp):
+        assert_bitwise(sim.get(n), o.get(n), f"enforce_boundaries {n}")
 
 
 def test_fp32_form_close():

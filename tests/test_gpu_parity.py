@@ -105,6 +105,35 @@ def test_derived_bitwise(periodic):
     assert_bitwise(sim.get("energy_density"), o.get("energy_density"), "energy density")
 
 
+@pytest.mark.parametrize("periodic", [0, 1])
+def test_operator_entry_points_bitwise(periodic):
+    """khs_fluid_velocities, khs_sources and khs_enforce_boundaries against
+    the oracle's update_velocities, sources and enforce_boundaries."""
+    nx, nz = 130, 40
+    st = random_state(nx, nz, seed=17 + periodic)
+    # densities below the floor and wall momenta / fields for the boundary operator
+    st["nA"][3, :7] = 0.01
+    st["nB"][5, -4:] = -0.2
+    cfg, sim, o = make_pair(st, **grid(nx, nz, periodic, warm_T=0.02))
+    o.update_velocities()
+    o.sources()
+    for s, tag in ((0, "A"), (1, "B")):
+        ux, uz = sim.fluid_velocities(s)
+        assert_bitwise(ux, o.get("ux" + tag), "ux" + tag)
+        assert_bitwise(uz, o.get("uz" + tag), "uz" + tag)
+        px, pz, sq = sim.sources(s)
+        assert_bitwise(px, o.get("srcpx" + tag), "srcpx" + tag)
+        assert_bitwise(pz, o.get("srcpz" + tag), "srcpz" + tag)
+        for k in range(3):
+            assert_bitwise(sq[k], o.get(f"sQ{k + 1}{tag}"), f"sQ{k + 1}{tag}")
+    sim.enforce_boundaries()
+    o.enforce_boundaries()
+    for n in orc.STATE:
+        assert_bitwise(sim.get(n), o.get(n), f"enforce_boundaries {n}")
+    with pytest.raises(khs.KhsError):
+        sim.fluid_velocities(2)
+
+
 STEP_CASES = {
     "walled_c1_shape": dict(C1, n_steps=0),
     "periodic_mode6_sponge_cut_hyp_warm": grid(128, 64, 1, xi_sponge=4.0, sigma_sponge=5.0, xi_cut=5.5,
@@ -219,6 +248,23 @@ def test_x_register_form_bitwise(cells, shape, periodic, warm):
     o.step(3)
     for n in orc.STATE:
         assert_bitwise(sim.get(n), o.get(n), f"x lane cells {cells} {shape} {n}")
+
+
+@pytest.mark.parametrize("zc", [16, 32, 64])
+@pytest.mark.parametrize("shape,periodic,warm", [((300, 136), 1, 0.01), ((70, 36), 0, 0.0), ((128, 200), 0, 0.01)])
+def test_z_two_row_prefetch_bitwise(zc, shape, periodic, warm):
+    """z march with loads two rows ahead (chunk ends, clamped z ends) gives
+    the oracle's bits."""
+    nx, nz = shape
+    keys = grid(nx, nz, periodic, z_chunk=zc, z_prefetch=2, autotune=0, warm_T=warm, xi_sponge=8.0)
+    cfg, sim, o = make_pair(random_state(nx, nz, seed=nz + zc, smooth=True), **keys)
+    t = sim.tuning()
+    assert (t["z_chunk"], t["z_prefetch"], t["z_groups"]) == (zc, 2, 0)
+    o.sources()
+    sim.step(3)
+    o.step(3)
+    for n in orc.STATE:
+        assert_bitwise(sim.get(n), o.get(n), f"z{zc} pf2 {shape} {n}")
 
 
 @pytest.mark.parametrize("zm", [1, 2, 3])
