@@ -174,10 +174,13 @@ def growth_c3(precision, n_steps=500, stride=10):
     return ts
 
 
-def sweep(n=64, steps=3000, lanes=8, segment=50):
-    """Batched-sweep workload: n uneven-grid FP32 runs (khs.sweep), stepped
-    through captured graphs, then the same runs one after another without
-    graphs.  A run ends at `steps` or when it halts (E/E0 rule)."""
+def sweep(n=64, steps=3000, lanes=8, segment=50, threads=8):
+    """Batched-sweep workload: n uneven-grid FP32 runs (khs.sweep) stepped
+    three ways: all runs' segments of `segment` steps as one captured graph
+    over `lanes` streams ("graph"); `threads` host threads each stepping its
+    share of the runs on their own streams with graph-replayed energy strides
+    ("concurrent"); the runs one after another ("eager").  A run ends at
+    `steps` or when it halts (E/E0 rule)."""
     import khs
     from khs.sweep import sweep_configs
 
@@ -185,17 +188,23 @@ def sweep(n=64, steps=3000, lanes=8, segment=50):
     cells = [c["nx_override"] * c["nz_override"] for c in cfgs]
     out = {"workload": f"{n} runs, NX in 256..2048 x NZ in 128..1024, V0~U[0.01,0.2], a~U[0.5,2], kz 1..3, "
                        f"half with suppress_kz0, FP32, {steps} steps or halt, xi_cut=12.8",
-           "lanes": lanes, "segment": segment}
-    for mode in ("graph", "eager"):
+           "lanes": lanes, "segment": segment, "threads": threads}
+    for mode in ("graph", "concurrent", "eager"):
         b = khs.Batch(cfgs, lanes=lanes, segment=segment)
         b.init()
+        run = {"graph": b.run, "eager": b.run_eager,
+               "concurrent": lambda k: b.run_concurrent(k, threads=threads)}[mode]
         t0 = time.perf_counter()
-        done, halted = (b.run if mode == "graph" else b.run_eager)(steps)
+        done, halted = run(steps)
         el = time.perf_counter() - t0
         b.close()
         out[mode] = {"seconds": el, "runs_per_hour": n / el * 3600.0, "halted": int(sum(halted)),
                      "cell_updates_per_s": sum(c * d for c, d in zip(cells, done)) / el}
     out["graph_over_eager"] = out["eager"]["seconds"] / out["graph"]["seconds"]
+    out["concurrent_over_eager"] = out["eager"]["seconds"] / out["concurrent"]["seconds"]
+    best = min(("graph", "concurrent"), key=lambda m: out[m]["seconds"])
+    out["runs_per_hour"] = out[best]["runs_per_hour"]
+    out["best_mode"] = best
     return out
 
 

@@ -328,6 +328,13 @@ int khs_batch_run(khs_batch* b, int nsteps, int* steps_done, int* halted);
 /* The same stepping without graphs: each run in turn through khs_step
  * (halting every energy_stride steps), for comparison. */
 int khs_batch_run_eager(khs_batch* b, int nsteps, int* steps_done, int* halted);
+/* The runs stepped concurrently: dealt to `threads` host threads by longest
+ * processing time first, each thread driving its runs with khs_step (whole
+ * energy strides replayed as captured graphs, the reference's halting rule
+ * at every energy check) on the run's own stream, so that small,
+ * latency-bound runs overlap with each other and with large ones.  Step
+ * limits (khs_batch_set_step_limit) apply. */
+int khs_batch_run_concurrent(khs_batch* b, int nsteps, int threads, int* steps_done, int* halted);
 int khs_batch_get_array(khs_batch* b, int run, int id, double* host, size_t n);
 int khs_batch_set_array(khs_batch* b, int run, int id, const double* host, size_t n);
 int khs_batch_energy(khs_batch* b, int run, double* E, double* E0);

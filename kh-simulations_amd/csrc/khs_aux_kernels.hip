@@ -2,10 +2,11 @@
 //
 //   k_cell_filters  sponge damping / xi_cut zeroing of the colour-2/3 set
 //                   (YM_Physics.cu:161-210), one pass, per-column action table.
-//   k_col_coef      DFT coefficients of whole z columns: one lane per x column
-//   k_col_apply     (coalesced), modes in registers; then subtraction (reference
-//                   band filter, YM_Physics.cu:349-584; z-mean removal :272-327)
-//                   or projection (keep |kz| <= K extension).
+//   k_col_coef2     DFT coefficients of whole z columns: 64 columns per block
+//   k_col_apply     (coalesced), z split over 8 row groups, modes in registers;
+//                   then subtraction (reference band filter,
+//                   YM_Physics.cu:349-584; z-mean removal :272-327) or
+//                   projection (keep |kz| <= K extension).
 //   k_hyperdiff     4th-order z hyperdiffusion (YM_Physics.cu:228-259) as an
 //                   in-place column march with a register window of the old
 //                   values, i.e. Jacobi semantics without a scratch copy.
@@ -50,37 +51,57 @@ void launch_cell_filters(const ArrayList<T>& arrs, const double* decay, const in
 // --------------------------------------------------------------------------
 // column DFT filter
 // --------------------------------------------------------------------------
-constexpr int CM = 16;   // modes per coefficient pass
 constexpr int CZ = 64;   // z rows per apply block
 
+// DFT coefficients of whole z columns, FP64 sums: a block is 64 columns x CG
+// groups of rows; every group sums its rows for CM2 modes at a time, and the
+// groups' partial sums are added in group order through LDS.  A column costs
+// nz / CG dependent iterations, which is what narrow, tall grids need (a
+// 256 x 1024 grid has only 4 column strips).
+constexpr int CG = 8;      // z groups per block
+constexpr int CM2 = 8;     // modes per pass
+
 template <typename T, typename TW>
-__global__ __launch_bounds__(64) void k_col_coef(ArrayList<T> arrs, const TW* ctab, const TW* stab,
-                                                 const TW* weight, int nmodes, double* coef, KParams p) {
-    const int x = blockIdx.x * 64 + threadIdx.x;
+__global__ __launch_bounds__(64 * CG) void k_col_coef2(ArrayList<T> arrs, const TW* ctab, const TW* stab,
+                                                       const TW* weight, int nmodes, double* coef, KParams p) {
+    __shared__ double red[CG][CM2][2][64];
+    const int lane = threadIdx.x & 63;
+    const int g = threadIdx.x >> 6;
+    const int x = blockIdx.x * 64 + lane;
     const int k = blockIdx.y;
-    const int m0 = blockIdx.z * CM;
-    if (x >= p.nx) return;
-    const int mc = min(CM, nmodes - m0);
-    const T* v = arrs.a[k];
-    double sc[CM], ss[CM];
+    const int m0 = blockIdx.z * CM2;
+    const int mc = min(CM2, nmodes - m0);
+    const int rows = (p.nz + CG - 1) / CG;
+    const int z0 = g * rows, z1 = min(p.nz, z0 + rows);
+    double sc[CM2], ss[CM2];
 #pragma unroll
-    for (int j = 0; j < CM; ++j) { sc[j] = 0.0; ss[j] = 0.0; }
-    for (int z = 0; z < p.nz; ++z) {
-        const double val = (double)v[(size_t)x + (size_t)z * p.nx];
+    for (int j = 0; j < CM2; ++j) { sc[j] = 0.0; ss[j] = 0.0; }
+    if (x < p.nx) {
+        const T* v = arrs.a[k];
+        for (int z = z0; z < z1; ++z) {
+            const double val = (double)v[(size_t)x + (size_t)z * p.nx];
 #pragma unroll
-        for (int j = 0; j < CM; ++j) {
-            if (j < mc) {
-                sc[j] += val * (double)ctab[(size_t)(m0 + j) * p.nz + z];
-                ss[j] += val * (double)stab[(size_t)(m0 + j) * p.nz + z];
+            for (int j = 0; j < CM2; ++j) {
+                if (j < mc) {
+                    sc[j] += val * (double)ctab[(size_t)(m0 + j) * p.nz + z];
+                    ss[j] += val * (double)stab[(size_t)(m0 + j) * p.nz + z];
+                }
             }
         }
     }
 #pragma unroll
-    for (int j = 0; j < CM; ++j) {
-        if (j < mc) {
-            const size_t base = ((size_t)k * nmodes + (m0 + j)) * 2;
-            coef[(base + 0) * p.nx + x] = sc[j] * (double)weight[m0 + j];
-            coef[(base + 1) * p.nx + x] = ss[j] * (double)weight[m0 + j];
+    for (int j = 0; j < CM2; ++j) { red[g][j][0][lane] = sc[j]; red[g][j][1][lane] = ss[j]; }
+    __syncthreads();
+    if (g == 0 && x < p.nx) {
+#pragma unroll
+        for (int j = 0; j < CM2; ++j) {
+            if (j < mc) {
+                double a = red[0][j][0][lane], b = red[0][j][1][lane];
+                for (int q = 1; q < CG; ++q) { a += red[q][j][0][lane]; b += red[q][j][1][lane]; }
+                const size_t base = ((size_t)k * nmodes + (m0 + j)) * 2;
+                coef[(base + 0) * p.nx + x] = a * (double)weight[m0 + j];
+                coef[(base + 1) * p.nx + x] = b * (double)weight[m0 + j];
+            }
         }
     }
 }
@@ -113,8 +134,8 @@ void launch_column_modes(const ArrayList<T>& arrs, const TW* ctab, const TW* sta
                          const TW* weight, int nmodes, int project, double* coef,
                          const KParams& p, hipStream_t st) {
     if (arrs.n == 0 || nmodes == 0) return;
-    dim3 g1((p.nx + 63) / 64, arrs.n, (nmodes + CM - 1) / CM);
-    hipLaunchKernelGGL((k_col_coef<T, TW>), g1, dim3(64), 0, st, arrs, ctab, stab, weight, nmodes, coef, p);
+    dim3 g1((p.nx + 63) / 64, arrs.n, (nmodes + CM2 - 1) / CM2);
+    hipLaunchKernelGGL((k_col_coef2<T, TW>), g1, dim3(64 * CG), 0, st, arrs, ctab, stab, weight, nmodes, coef, p);
     dim3 g2((p.nx + 63) / 64, arrs.n, (p.nz + CZ - 1) / CZ);
     hipLaunchKernelGGL((k_col_apply<T, TW>), g2, dim3(64), 0, st, arrs, ctab, stab, nmodes, project, coef, p);
 }

@@ -27,6 +27,7 @@
 #include <limits>
 #include <memory>
 #include <sstream>
+#include <thread>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -730,10 +731,22 @@ struct Batch {
         HIPCHK(hipStreamBeginCapture(origin, hipStreamCaptureModeRelaxed));
         HIPCHK(hipEventRecord(ev[0], origin));
         for (auto l : lanes) HIPCHK(hipStreamWaitEvent(l, ev[0], 0));
-        for (size_t k = 0; k < set.size(); ++k) {
+        // runs dealt to lanes by longest processing time first (cells per
+        // step), so that the lanes' chains of kernels end together
+        std::vector<int> order(set.size()), lane_of(set.size());
+        for (size_t k = 0; k < set.size(); ++k) order[k] = (int)k;
+        auto cost = [&](int k) { return (double)runs[set[k]]->nx * (double)runs[set[k]]->nz; };
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+        std::vector<double> load(lanes.size(), 0.0);
+        for (int k : order) {
+            const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            lane_of[k] = l;
+            load[l] += cost(k);
+        }
+        for (int k : order) {
             Ctx& c = *runs[set[k]];
             hipStream_t own = c.stream;
-            c.stream = lanes[k % lanes.size()];
+            c.stream = lanes[lane_of[k]];
             const int cur0 = c.cur;
             const long step0 = c.step;
             for (int s = 0; s < K; ++s) full_step(c);
@@ -1588,6 +1601,59 @@ int khs_batch_run_eager(khs_batch* b, int nsteps, int* steps_done, int* halted) 
             if (h) B.active[i] = 0;
         }
         for (size_t i = 0; i < B.runs.size(); ++i) {
+            if (steps_done) steps_done[i] = (int)B.runs[i]->step;
+            if (halted) halted[i] = B.active[i] ? 0 : 1;
+        }
+    });
+}
+
+int khs_batch_run_concurrent(khs_batch* b, int nsteps, int threads, int* steps_done, int* halted) {
+    if (!b || !b->impl) { g_last_error = "null batch"; return KHS_ERR_ARG; }
+    if (threads < 1) { g_last_error = "threads must be >= 1"; return KHS_ERR_ARG; }
+    return guard([&] {
+        khs::Batch& B = *b->impl;
+        const int n = (int)B.runs.size();
+        // runs dealt to host threads by longest processing time first (cells x
+        // steps to go); each thread steps its runs one after another with
+        // khs_step (graph-replayed energy strides) on the run's own stream
+        std::vector<std::vector<int>> part(threads);
+        std::vector<double> load(threads, 0.0);
+        std::vector<int> order;
+        std::vector<double> cost(n, 0.0);
+        for (int i = 0; i < n; ++i) {
+            const Ctx& c = *B.runs[i];
+            const long todo = std::min<long>(nsteps, B.cap[i] - c.step);
+            if (!B.active[i] || todo <= 0) continue;
+            cost[i] = (double)c.nx * (double)c.nz * (double)todo;
+            order.push_back(i);
+        }
+        std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return cost[a] > cost[c]; });
+        for (int i : order) {
+            const int t = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            part[t].push_back(i);
+            load[t] += cost[i];
+        }
+        std::vector<int> rc(threads, KHS_OK);
+        std::vector<std::string> msg(threads);
+        auto work = [&](int t) {
+            for (int i : part[t]) {
+                khs_ctx tmp;
+                tmp.impl = std::move(B.runs[i]);
+                const int todo = (int)std::min<long>(nsteps, B.cap[i] - tmp.impl->step);
+                int h = 0;
+                const int r = khs_step(&tmp, todo, &h);
+                B.runs[i] = std::move(tmp.impl);
+                if (r != KHS_OK) { rc[t] = r; msg[t] = "run " + std::to_string(i) + ": " + g_last_error; return; }
+                if (h) B.active[i] = 0;
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int t = 1; t < threads; ++t) pool.emplace_back(work, t);
+        work(0);
+        for (auto& th : pool) th.join();
+        for (int t = 0; t < threads; ++t)
+            if (rc[t] != KHS_OK) throw Error(rc[t], msg[t]);
+        for (int i = 0; i < n; ++i) {
             if (steps_done) steps_done[i] = (int)B.runs[i]->step;
             if (halted) halted[i] = B.active[i] ? 0 : 1;
         }

@@ -158,6 +158,7 @@ def _signatures():
         "khs_batch_energy": (c_int, [c_void_p, c_int, dp, dp]),
         "khs_batch_mode_amplitude": (c_int, [c_void_p, c_int, c_int, dp]),
         "khs_batch_set_step_limit": (c_int, [c_void_p, c_int, c_int]),
+        "khs_batch_run_concurrent": (c_int, [c_void_p, c_int, c_int, P(c_int), P(c_int)]),
     }
 
 
@@ -489,6 +490,10 @@ class Batch:
 
     def run_eager(self, nsteps):
         return self._run(_lib.khs_batch_run_eager, nsteps)
+
+    def run_concurrent(self, nsteps, threads=8):
+        """Runs stepped by `threads` host threads on their own streams."""
+        return self._run(lambda h, n, d, hl: _lib.khs_batch_run_concurrent(h, n, int(threads), d, hl), nsteps)
 
     def get(self, run, name):
         if not 0 <= run < self.n:

@@ -75,3 +75,27 @@ def test_batch_halted_run_leaves_graph():
         assert_bitwise(batch.get(0, n), ref.get(n), f"run 0 {n}")
     with pytest.raises(khs.KhsError):
         batch.get(7, "nA")
+
+
+def test_batch_concurrent_matches_single_runs():
+    """Runs stepped by several host threads on their own streams end in the
+    bits of each run stepped alone; per-run step limits stop runs exactly."""
+    cfgs = small_configs()
+    batch = khs.Batch(cfgs, lanes=2, segment=10)
+    batch.init()
+    limits = [40, 25, 40, 13, 40]
+    for i, n in enumerate(limits):
+        batch.set_step_limit(i, n)
+    done, halted = batch.run_concurrent(30, threads=3)
+    assert done == [min(30, n) for n in limits] and not any(halted)
+    done, halted = batch.run_concurrent(30, threads=4)
+    assert done == limits and not any(halted)
+    for i, c in enumerate(cfgs):
+        s = khs.Simulation(khs.Config(**c))
+        s.init()
+        s.step(limits[i])
+        for n in ("PzA", "Q2B", "By2", "Ez3", "Az2", "nB"):
+            assert_bitwise(batch.get(i, n), s.get(n), f"concurrent run {i} {n}")
+        assert batch.energy(i)[0] == s.energy()
+        s.close()
+    batch.close()

@@ -162,6 +162,8 @@ FILTER_CASES = {
     "zmean_bands": grid(64, 64, 1, suppress_kz0=1, kz_suppress_max=2, kz_suppress_hi=9, hyp_diff=1e-3),
     "keep4_c1": dict(C1, kz_keep_max=4, nx_override=64),
     "bands_nyquist": grid(48, 32, 0, kz_suppress_max=3, kz_suppress_hi=16),
+    "zmean_bands_tall": grid(70, 512, 0, suppress_kz0=1, kz_suppress_max=2, kz_suppress_hi=20),
+    "keep_tall_float_twiddles": grid(64, 200, 1, kz_keep_max=6, mfma_filter=0),
 }
 
 
