@@ -10,12 +10,15 @@ batch_analyze.py).  Here:
   * int_kz_keys / half_kz_keys restate the two .ini templates (:89-116,
     :119-168) and xi_sponge_for their sponge radius (:33-37);
   * a manifest (JSON: {"defaults": {...}, "runs": [{...}, ...],
-    "sample_stride": S, "lanes": L, "segment": G}, or a list of .ini files)
-    names the runs;
+    "sample_stride": S, "lanes": L, "segment": G, "mode": "concurrent" |
+    "graph"}, or a directory / glob of .ini files) names the runs;
   * lpt_assign deals them to GPUs by longest processing time first (cost =
     cells x steps), instead of round-robin;
   * run_part steps one GPU's share as one khs.Batch (each run stops at its own
-    n_steps / target_tu or when it halts), sampling the circular Az2 + i Az3
+    n_steps / target_tu or when it halts; the runs advance concurrently on
+    host threads and their own streams, whole energy strides as captured
+    graphs, or with "mode": "graph" as one graph per segment of all runs),
+    sampling the circular Az2 + i Az3
     amplitude at each run's k_mode every S steps, and fits the growth rate
     (khs.fit_growth_max_r2 and khs.fit_growth_plateau);
   * main() is the command line; with --gpus N > 1 it starts one process per
@@ -62,7 +65,7 @@ def half_kz_keys(alpha, V0, kz_phys):
 
 def load_manifest(path):
     """-> (list of (name, keys dict or .ini path), options dict)."""
-    opts = dict(sample_stride=20, lanes=4, segment=20)
+    opts = dict(sample_stride=24, lanes=4, segment=24, mode="concurrent")
     if os.path.isdir(path) or any(ch in path for ch in "*?["):
         files = sorted(glob.glob(os.path.join(path, "*.ini") if os.path.isdir(path) else path))
         return [(os.path.splitext(os.path.basename(f))[0], f) for f in files], opts
@@ -78,7 +81,7 @@ def load_manifest(path):
         runs.append((name, keys))
     for k in opts:
         if k in m:
-            opts[k] = int(m[k])
+            opts[k] = m[k] if k == "mode" else int(m[k])
     return runs, opts
 
 
@@ -145,7 +148,8 @@ def run_part(runs, opts, out_root):
             live = [i for i in range(n) if not halted[i] and steps[i] < cfgs[i].total_steps]
             if not live:
                 break
-            done, hl = b.run(stride)
+            done, hl = (b.run(stride) if opts.get("mode") == "graph"
+                        else b.run_concurrent(stride, threads=int(opts["lanes"])))
             for i in live:
                 if done[i] != steps[i]:
                     steps[i] = done[i]

@@ -55,7 +55,7 @@ def test_load_manifest_json_and_ini(tmp_path):
     runs, opts = C.load_manifest(str(p))
     assert [r[0] for r in runs] == ["a", "run001"]
     assert runs[1][1] == {"nx_override": 96, "nz_override": 64, "n_steps": 30, "k_mode": 2}
-    assert opts["sample_stride"] == 10 and opts["lanes"] == 4
+    assert opts["sample_stride"] == 10 and opts["lanes"] == 4 and opts["mode"] == "concurrent"
     d = tmp_path / "inis"
     d.mkdir()
     (d / "r1.ini").write_text("k_mode = 2\nnz_override = 64\n")

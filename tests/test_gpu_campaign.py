@@ -22,15 +22,17 @@ RUNS = [dict(name="w128_k1", nx_override=128, lx_override=12.8, k_mode=1, n_step
              precision="fp32")]
 
 
-def manifest(tmp_path):
-    m = {"defaults": BASE, "runs": RUNS, "sample_stride": 10, "lanes": 2, "segment": 10}
+def manifest(tmp_path, mode="concurrent"):
+    m = {"defaults": BASE, "runs": RUNS, "sample_stride": 10, "lanes": 2, "segment": 10, "mode": mode}
     p = tmp_path / "campaign.json"
     p.write_text(json.dumps(m))
     return str(p)
 
 
-def test_campaign_part_matches_single_runs(tmp_path):
-    runs, opts = C.load_manifest(manifest(tmp_path))
+@pytest.mark.parametrize("mode", ["concurrent", "graph"])
+def test_campaign_part_matches_single_runs(tmp_path, mode):
+    runs, opts = C.load_manifest(manifest(tmp_path, mode))
+    assert opts["mode"] == mode
     out = str(tmp_path / "out")
     rows = C.run_part(runs, opts, out)
     assert [r["name"] for r in rows] == [r["name"] for r in RUNS]
