@@ -18,6 +18,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <mutex>
+#include <map>
 #include <chrono>
 #include <functional>
 #include <cmath>
@@ -62,6 +65,16 @@ struct ModeTable {
     int nmodes = 0;
     bool is_float = false;
 };
+
+// process-wide cache of autotuned kernel choices, keyed by Ctx::tune_key()
+static std::map<std::vector<long>, std::array<int, 5>>& tune_cache() {
+    static std::map<std::vector<long>, std::array<int, 5>> m;
+    return m;
+}
+static std::mutex& tune_cache_mutex() {
+    static std::mutex m;
+    return m;
+}
 
 struct Ctx {
     khs_config cfg{};
@@ -222,6 +235,12 @@ struct Ctx {
         evpool.push_back(b);
         return bi;
     }
+    // the kernel-shape keys and everything else the timing depends on
+    std::vector<long> tune_key() const {
+        return {nx, nz, fp32, emhd, nsp, kp.periodic_x, cell_filters, cfg.fused_sweep, cfg.x_segment, cfg.x_rows,
+                cfg.x_serial, cfg.x_lane_cells, cfg.z_chunk, cfg.z_prefetch, cfg.z_groups, cfg.maxwell_kernel,
+                cfg.zm_fused, cfg.xz_variant};
+    }
     template <typename T> void do_autotune() {
         tuned = true;
         const int fx = cfg.x_segment == 64 ? 0 : cfg.x_segment == 128 ? 1 : cfg.x_segment == 256 ? 2 : -1;
@@ -252,6 +271,26 @@ struct Ctx {
         mvar = (cfg.maxwell_kernel >= 0) ? cfg.maxwell_kernel : 0;
         xzvar = (cfg.xz_variant >= 0) ? cfg.xz_variant : 2;
         if (!cfg.autotune) return;
+        // contexts of the same shape in one process (sweeps, campaigns) reuse
+        // the first one's choice: every variant computes the same bits
+        const std::vector<long> key = tune_key();
+        {
+            std::lock_guard<std::mutex> lk(tune_cache_mutex());
+            auto it = tune_cache().find(key);
+            if (it != tune_cache().end()) {
+                xvar = it->second[0]; zvar = it->second[1]; mvar = it->second[2];
+                xzvar = it->second[3]; zmvar = it->second[4];
+                return;
+            }
+        }
+        struct Store {
+            Ctx* c;
+            std::vector<long> key;
+            ~Store() {
+                std::lock_guard<std::mutex> lk(tune_cache_mutex());
+                tune_cache()[key] = {c->xvar, c->zvar, c->mvar, c->xzvar, c->zmvar};
+            }
+        } store{this, key};
         if (xc.size() > 1) xvar = pick<T>(xc, [&](int v) { fct_x<T>(0, nsp, v); });
         fct_x<T>(0, nsp, xvar);
         if (zc.size() > 1) zvar = pick<T>(zc, [&](int v) { fct_z<T>(0, nsp, v); });

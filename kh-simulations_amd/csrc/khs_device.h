@@ -82,6 +82,12 @@ template <typename T>
 __device__ __forceinline__ void st32(T* a, unsigned i, T v) {
     *reinterpret_cast<T*>(reinterpret_cast<char*>(a) + i * (unsigned)sizeof(T)) = v;
 }
+// The same store marked non-temporal (streaming): the line is not kept in
+// the caches for reuse, which suits outputs that no kernel reads soon.
+template <typename T>
+__device__ __forceinline__ void st32_nt(T* a, unsigned i, T v) {
+    __builtin_nontemporal_store(v, reinterpret_cast<T*>(reinterpret_cast<char*>(a) + i * (unsigned)sizeof(T)));
+}
 
 // Whole-wavefront lane shifts (DPP wave_shr:1 / wave_shl:1 of GFX9):
 // lane_from_left(v) in lane l is v of lane l-1, lane_from_right(v) is v of

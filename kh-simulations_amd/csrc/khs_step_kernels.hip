@@ -141,7 +141,7 @@ __global__ __launch_bounds__(XB) void k_fct_x(State<T> S, Stage<T> out, KParams 
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             T r = sLNR[v][q] - (sV[v][q] - sV[v][q - 1]);
-            st32(out.a[sp * NV + v], (unsigned)row + (unsigned)x, r);
+            st32_nt(out.a[sp * NV + v], (unsigned)row + (unsigned)x, r);
         }
     }
     } else {
@@ -172,7 +172,7 @@ __global__ __launch_bounds__(XB) void k_fct_x(State<T> S, Stage<T> out, KParams 
         __syncthreads();
         if (x < p.nx) {
             const int q = 3 + tid;
-            st32(out.a[sp * NV + v], (unsigned)row + (unsigned)x, sLNR[0][q] - (sFL[q] - sFL[q - 1]));
+            st32_nt(out.a[sp * NV + v], (unsigned)row + (unsigned)x, sLNR[0][q] - (sFL[q] - sFL[q - 1]));
         }
         __syncthreads();              // sLOR / sLNR / sFL are rewritten for the next variable
     }
@@ -231,12 +231,12 @@ __device__ __forceinline__ void store_run(T* a, unsigned row, int xl, bool vec, 
             V w;
 #pragma unroll
             for (int j = 0; j < N; ++j) w[j] = o[k * N + j];
-            q[k] = w;
+            __builtin_nontemporal_store(w, q + k);
         }
     } else {
 #pragma unroll
         for (int j = 0; j < C; ++j)
-            if (xl + j < nx) st32(a, row + (unsigned)(xl + j), o[j]);
+            if (xl + j < nx) st32_nt(a, row + (unsigned)(xl + j), o[j]);
     }
 }
 
@@ -525,7 +525,7 @@ __global__ __launch_bounds__(ZB_X * ZB_C) void k_fct_z(State<T> S, Stage<T> X, S
             if (wall) out[M::IPX] = T(0.0);
             const unsigned io = (unsigned)x + (unsigned)(e - 3) * (unsigned)nx;
 #pragma unroll
-            for (int v = 0; v < NV; ++v) st32(O.a[b + v], io, out[v]);
+            for (int v = 0; v < NV; ++v) st32_nt(O.a[b + v], io, out[v]);
         }
         if (!PF) load(rn, nrow, nF, nV);
         // advance: row z-1 of the next row is this row unless the clamp holds
@@ -599,7 +599,7 @@ __global__ __launch_bounds__(ZB_X * ZB_C) void k_fct_z2(State<T> S, Stage<T> X, 
             if (wall) out[M::IPX] = T(0.0);
             const unsigned io = (unsigned)x + (unsigned)(e - 3) * (unsigned)nx;
 #pragma unroll
-            for (int v = 0; v < NV; ++v) st32(O.a[b + v], io, out[v]);
+            for (int v = 0; v < NV; ++v) st32_nt(O.a[b + v], io, out[v]);
         }
         M::sel_row(rn != rc, cur, prev);
         cur = nrow;
