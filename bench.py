@@ -266,9 +266,19 @@ def cpu_baseline(budget_s=15.0):
             break
     cells = cfg.nx * cfg.nz
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return dict(value=cells * n / el, unit="cell-updates/s", cores=threads, kind="port",
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), None)
+    except OSError:
+        pass
+    triad = orc.stream_triad()
+    value = cells * n / el
+    return dict(value=value, unit="cell-updates/s", cores=threads, kind="port",
                 sample=f"{n} FP64 steps of the 2048x1024 workload ({el:.1f} s wall, "
-                       f"OMP_NUM_THREADS={threads}, energy check excluded)")
+                       f"OMP_NUM_THREADS={threads}, energy check excluded)",
+                cpu_model=model, stream_triad_GBs=triad,
+                roofline_frac=(value * STEP_ARRAYS * 8 / 1e9 / triad) if triad else None)
 
 
 def calibrate_copy(mb):

@@ -23,6 +23,7 @@
 // header comment of tests/oracle_ref.py.
 
 #include <cmath>
+#include <omp.h>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
@@ -764,6 +765,28 @@ void orc_fct_row(int n, const double* u, const double* v, const double* s, doubl
                  double dx, double dt) {
     FctScratch w;
     fct_row(n, u, v, s, out, dx, dt, w);
+}
+
+// Host memory-bandwidth roofline for the CPU baseline: the STREAM triad
+// a = b + 3 c on n doubles per array with the OpenMP threads (first touch by
+// the same static schedule), best of `reps` passes, in GB/s (24 n bytes).
+double orc_stream_triad(long n, int reps) {
+    double* a = static_cast<double*>(std::malloc(sizeof(double) * n));
+    double* b = static_cast<double*>(std::malloc(sizeof(double) * n));
+    double* c = static_cast<double*>(std::malloc(sizeof(double) * n));
+    if (!a || !b || !c) { std::free(a); std::free(b); std::free(c); return 0.0; }
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) { a[i] = 0.0; b[i] = 1.0; c[i] = 2.0; }
+    double best = 1e30;
+    for (int r = 0; r < reps; ++r) {
+        const double t0 = omp_get_wtime();
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < n; ++i) a[i] = b[i] + 3.0 * c[i];
+        best = std::min(best, omp_get_wtime() - t0);
+    }
+    const double check = a[n / 2];
+    std::free(a); std::free(b); std::free(c);
+    return check == 7.0 ? 24.0 * (double)n / best * 1e-9 : 0.0;
 }
 
 // One column of the reference band filter and of the keep-modes projection.

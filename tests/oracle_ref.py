@@ -95,6 +95,8 @@ def load():
     lib.orc_gauss.argtypes = [v, dp]
     lib.orc_fct_row.argtypes = [i, dp, dp, dp, dp, d, d]
     lib.orc_kz_band_column.argtypes = [dp, i, i, i]
+    lib.orc_stream_triad.argtypes = [ctypes.c_long, i]
+    lib.orc_stream_triad.restype = d
     lib.orc_init.argtypes = [v, i, d, d, d, i, d, i, d, dp, dp, dp, dp]
     lib.orc_noise.restype = d
     lib.orc_noise.argtypes = [ctypes.c_uint64, i, ctypes.c_uint64, d]
@@ -191,6 +193,11 @@ class Oracle:
         out = np.zeros(7)
         self.lib.orc_gauss(self.h, _ptr(out))
         return out
+
+
+def stream_triad(n=1 << 25, reps=5):
+    """Host STREAM-triad bandwidth (GB/s) with the oracle's OpenMP threads."""
+    return load().orc_stream_triad(n, reps)
 
 
 def fct_row(u, v, s, dx, dt):
