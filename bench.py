@@ -95,6 +95,12 @@ def kernel_instance(slot, t):
         return f"{name}<double, {t['z_chunk']}, {t['z_prefetch']}, YMModel>"
     if slot == "k_fct_zm":
         return f"k_fct_zmx<double, {ZM_ROWS.get(t.get('zm_fused', 0), 0)}>"
+    if slot == "k_fct_xz":
+        v = t["xz_variant"]
+        name, tz = {0: ("k_fct_xz", 32), 1: ("k_fct_xz2", 16), 2: ("k_fct_xz2", 32), 3: ("k_fct_xz2", 64),
+                    4: ("k_fct_xzr", 32), 5: ("k_fct_xzr", 64), 6: ("k_fct_xzr", 128), 7: ("k_fct_xzg", 16),
+                    8: ("k_fct_xzg", 32), 9: ("k_fct_xzg", 64)}[v]
+        return f"{name}<double, {tz}>"
     if slot == "k_maxwell" and t["maxwell"] >= 4:
         tz, pf = {4: (16, 1), 5: (32, 1), 6: (16, 0), 7: (8, 1), 8: (8, 0), 9: (4, 1)}[t["maxwell"]]
         return f"k_maxwell_zm<double, {tz}, {pf}>"
@@ -104,7 +110,7 @@ def kernel_instance(slot, t):
     return None
 
 
-def run_gpu(precision, steps, warmup, fused=1, workload=WORKLOAD, segment=SEGMENT, kernel_steps=120):
+def run_gpu(precision, steps, warmup, fused=-1, workload=WORKLOAD, segment=SEGMENT, kernel_steps=120):
     """Timed khs_step over `steps` steps (segments restarting from init; whole
     energy strides replay as captured graphs), then `kernel_steps` eager steps
     with HIP events around every launch for the per-kernel durations."""
@@ -315,8 +321,8 @@ def main():
     ap.add_argument("--no-c5", action="store_true", help="skip the EMHD wave-dispersion leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB")
-    ap.add_argument("--split", action="store_true", help="separate x and z sweep kernels")
-    ap.add_argument("--fused", action="store_true", help="fused x+z sweep kernel")
+    ap.add_argument("--split", action="store_true", help="force the separate x and z sweep kernels")
+    ap.add_argument("--fused", action="store_true", help="force the fused x+z sweep kernel")
     a = ap.parse_args()
     if a.calibrate_copy:
         calibrate_copy(a.calibrate_copy)
@@ -326,7 +332,7 @@ def main():
               "or x-slab) path is a design, not a measured configuration. Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
-    fused = 1 if a.fused else 0
+    fused = 1 if a.fused else (0 if a.split else -1)     # -1: the autotuner's choice
     r64 = run_gpu("fp64", a.steps, a.warmup, fused)
     ms_step = r64["seconds"] / a.steps * 1e3
     value = r64["cells"] * a.steps / r64["seconds"]
@@ -378,7 +384,7 @@ def main():
                    "segments": r64["segments"],
                    "tiles": r64["tuning"],
                    "nx": r64["nx"], "nz": r64["nz"], "precision": "fp64",
-                   "sweep": "fused x+z" if fused else "split x / z"},
+                   "sweep": "fused x+z" if r64["tuning"]["fused_sweep"] else "split x / z"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": d["alg_GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": d["alg_GBs"] / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel_instance": inst, "alg_bytes_per_launch": d["alg_bytes"], "avg_launch_us": d["avg_us"],
@@ -399,7 +405,7 @@ def main():
         import khs
         c3 = {}
         for prec in ("fp32", "fp64"):
-            r = run_gpu(prec, a.c3_steps, 20, 0, WORKLOAD_C3, SEGMENT_C3)
+            r = run_gpu(prec, a.c3_steps, 20, fused, WORKLOAD_C3, SEGMENT_C3)
             c3[prec] = dict(value=r["cells"] * a.c3_steps / r["seconds"],
                             ms_per_step=r["seconds"] / a.c3_steps * 1e3,
                             kernels_avg_us={k: (v[0] / v[1] * 1e3 if v[1] else None) for k, v in r["times"].items()})

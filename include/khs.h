@@ -109,7 +109,9 @@ typedef struct khs_config {
     int      precision;        /* KHS_FP64 or KHS_FP32 */
     int      energy_stride_override;
     int      export_stride_override;  /* <0 disables snapshots */
-    int      fused_sweep;      /* 1: khs_step runs the fused x+z sweep kernel; 0 (default): split */
+    int      fused_sweep;      /* YM: 1 = khs_step runs the fused x+z sweep kernel, 0 = the split x / z
+                                  sweeps, -1 (default) = the autotuner's choice (fused without tuning;
+                                  split when a split-sweep tile shape is forced) */
     int      autotune;         /* 1 (default): time the tile variants on the first step, keep the fastest */
     int      x_segment;        /* 0 = tuned, or force 64 / 128 / 256 cells per x-sweep segment */
     int      z_chunk;          /* 0 = tuned, or force 16 / 32 / 64 rows per z-sweep chunk */
@@ -127,7 +129,7 @@ typedef struct khs_config {
     uint64_t spectrum_seed;
     double   em_perturb_amp;   /* EMHD das_kaw perturbation (argv[2] as double, default 0.05); set by perturb_amp */
     int      z_groups;         /* z sweep: -1 = autotuned (default), 0 = all variables per lane, 1 = two variable groups on two waves */
-    int      xz_variant;       /* fused sweep kernel variant: -1 = autotuned (default), 0..3 */
+    int      xz_variant;       /* fused sweep kernel variant: -1 = autotuned (default), 0..9 */
     int      x_serial;         /* x sweep phases per variable: -1 = autotuned (default), 0 = all at once, 1 = one at a time */
     int      maxwell_kernel;   /* YM Maxwell: -1 = autotuned (default), 0/2/3 = 64x16/64x4/64x8 LDS tiles, 1 = per cell,
                                   4..9 = z march with 16/32/16/8/8/4-row chunks (6, 8: without next-row prefetch) */
@@ -303,13 +305,15 @@ const char* khs_kernel_name(int slot);
  * workgroup, z-sweep chunk height (rows), z-sweep next-row prefetch (0/1),
  * YM Maxwell kernel (0/2/3 = 64x16/64x4/64x8 LDS tiles, 1 = per cell, 4..9 = z march), x-sweep variables one at a
  * time (0/1), fused-sweep variant (0: one wave per 64 columns; 1/2/3: four
- * waves per 128 columns with 16/32/64-row chunks), z-sweep variables split
+ * waves per 128 columns with 16/32/64-row chunks; 4/5/6: register x sweep in
+ * the z march, 32/64/128-row chunks; 7/8/9: the same with the variables on two
+ * waves, 16/32/64-row chunks), z-sweep variables split
  * over two waves (0/1), x-sweep cells per lane of the register form (0 = LDS
  * form; the register form reports its 62*C-cell wave segment, 1 row, and 1
  * for variables one at a time), fused z sweep + Maxwell kernel (0 = split
- * kernels, 1/2/3 = fused with 16/32/64-row chunks)}; chosen by the autotuner on the first step
- * unless autotune = 0. */
-int khs_get_tuning(khs_ctx* ctx, int out[10]);
+ * kernels, 1/2/3 = fused with 16/32/64-row chunks), fused x+z sweep in use (0/1)}; chosen by the
+ * autotuner on the first step unless autotune = 0. */
+int khs_get_tuning(khs_ctx* ctx, int out[11]);
 
 /* ---------------- batches of runs (parameter sweeps) ---------------- */
 /* No counterpart in the reference, which runs one configuration per process

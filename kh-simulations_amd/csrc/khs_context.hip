@@ -67,8 +67,8 @@ struct ModeTable {
 };
 
 // process-wide cache of autotuned kernel choices, keyed by Ctx::tune_key()
-static std::map<std::vector<long>, std::array<int, 5>>& tune_cache() {
-    static std::map<std::vector<long>, std::array<int, 5>> m;
+static std::map<std::vector<long>, std::array<int, 6>>& tune_cache() {
+    static std::map<std::vector<long>, std::array<int, 6>> m;
     return m;
 }
 static std::mutex& tune_cache_mutex() {
@@ -119,7 +119,8 @@ struct Ctx {
     int cnt[K_NSLOTS] = {};
 
     // tile variants (autotuned on the first step)
-    int xvar = 2, zvar = 1, mvar = 0, xzvar = 2;
+    int xvar = 2, zvar = 1, mvar = 0, xzvar = 7;
+    bool fused = false;                 // khs_step runs the fused x+z sweep
     int zmvar = 0;                            // 0: split z sweep + Maxwell; 1..3: fused kernel
     // one energy stride of steps + the energy reduction, captured once
     hipGraphExec_t sgraph = nullptr;
@@ -216,7 +217,8 @@ struct Ctx {
     // after a warm-up) and keep the fastest.  Runs before the first sweep of
     // the context, when the stage and the next-state fluid are scratch; all
     // variants compute identical bits.
-    template <typename T> int pick(const std::vector<int>& cand, const std::function<void(int)>& launch) {
+    template <typename T> int pick(const std::vector<int>& cand, const std::function<void(int)>& launch,
+                                   float* best_ms = nullptr) {
         hipEvent_t a = get_event(), b = get_event();
         float best = 1e30f;
         int bi = cand[0];
@@ -233,6 +235,7 @@ struct Ctx {
         check_launch();
         evpool.push_back(a);
         evpool.push_back(b);
+        if (best_ms) *best_ms = best;
         return bi;
     }
     // the kernel-shape keys and everything else the timing depends on
@@ -269,7 +272,13 @@ struct Ctx {
         zvar = ((fz >= 0) ? fz : 1) + 3 * ((fp >= 0) ? fp : (fp32 ? 1 : 0)) + 6 * ((fg >= 0) ? fg : 0);
         if (fp == 2) zvar = 12 + ((fz >= 0) ? fz : 1);
         mvar = (cfg.maxwell_kernel >= 0) ? cfg.maxwell_kernel : 0;
-        xzvar = (cfg.xz_variant >= 0) ? cfg.xz_variant : 2;
+        // fused x+z sweep (YM): forced by fused_sweep = 1; with -1 it is the
+        // untuned default and the autotuner's candidate against the split
+        // sweeps, unless a split-sweep shape is forced
+        const bool split_keys = lds_keys || fl >= 0 || fz >= 0 || fp >= 0 || fg >= 0 || cfg.zm_fused > 0;
+        const bool try_fused = !emhd && cfg.fused_sweep < 0 && !split_keys;
+        fused = !emhd && (cfg.fused_sweep > 0 || try_fused);
+        xzvar = (cfg.xz_variant >= 0) ? cfg.xz_variant : 7;
         if (!cfg.autotune) return;
         // contexts of the same shape in one process (sweeps, campaigns) reuse
         // the first one's choice: every variant computes the same bits
@@ -279,7 +288,7 @@ struct Ctx {
             auto it = tune_cache().find(key);
             if (it != tune_cache().end()) {
                 xvar = it->second[0]; zvar = it->second[1]; mvar = it->second[2];
-                xzvar = it->second[3]; zmvar = it->second[4];
+                xzvar = it->second[3]; zmvar = it->second[4]; fused = it->second[5] != 0;
                 return;
             }
         }
@@ -288,23 +297,24 @@ struct Ctx {
             std::vector<long> key;
             ~Store() {
                 std::lock_guard<std::mutex> lk(tune_cache_mutex());
-                tune_cache()[key] = {c->xvar, c->zvar, c->mvar, c->xzvar, c->zmvar};
+                tune_cache()[key] = {c->xvar, c->zvar, c->mvar, c->xzvar, c->zmvar, c->fused ? 1 : 0};
             }
         } store{this, key};
-        if (xc.size() > 1) xvar = pick<T>(xc, [&](int v) { fct_x<T>(0, nsp, v); });
+        float tx = 0.f, tz = 0.f, txz = 0.f;
+        if (xc.size() > 1 || try_fused) xvar = pick<T>(xc, [&](int v) { fct_x<T>(0, nsp, v); }, &tx);
         fct_x<T>(0, nsp, xvar);
-        if (zc.size() > 1) zvar = pick<T>(zc, [&](int v) { fct_z<T>(0, nsp, v); });
-        if (!emhd && cfg.fused_sweep) {
-            if (cfg.xz_variant >= 0) {
-                xzvar = cfg.xz_variant;
-            } else {
-                std::vector<int> xzc;
-                for (int v = 0; v < kNumXZVariants; ++v) xzc.push_back(v);
-                xzvar = pick<T>(xzc, [&](int v) { launch_fct_xz<T>(state<T>(cur), state<T>(1 - cur), kp, v, stream); });
-            }
+        if (zc.size() > 1 || try_fused) zvar = pick<T>(zc, [&](int v) { fct_z<T>(0, nsp, v); }, &tz);
+        if (fused) {
+            std::vector<int> xzc;
+            for (int v = 0; v < kNumXZVariants; ++v)
+                if (cfg.xz_variant < 0 || v == cfg.xz_variant) xzc.push_back(v);
+            if (xzc.size() > 1 || try_fused)
+                xzvar = pick<T>(xzc, [&](int v) { launch_fct_xz<T>(state<T>(cur), state<T>(1 - cur), kp, v, stream); },
+                                &txz);
+            if (try_fused) fused = txz < tx + tz;
         }
         zmvar = (!emhd && cfg.zm_fused > 0) ? cfg.zm_fused : 0;
-        if (!emhd && cfg.zm_fused < 0 && !cfg.fused_sweep) {
+        if (!emhd && cfg.zm_fused < 0 && !fused) {
             // split z sweep + (tuned) Maxwell against the fused kernel's chunk heights
             if (cfg.maxwell_kernel < 0) {
                 std::vector<int> mc;
@@ -710,7 +720,8 @@ static void capture_stride(Ctx& c, int stride) {
 }
 
 static void full_step(Ctx& c) {
-    if (c.cfg.fused_sweep && !c.emhd) {
+    c.autotune();                               // decides split or fused sweeps on the first step
+    if (c.fused) {
         c.xz_stage();
         c.maxwell_stage();
     } else {
@@ -1489,7 +1500,7 @@ int khs_kernel_times(khs_ctx* ctx, double* ms, int* launches, int nslots) {
     });
 }
 
-int khs_get_tuning(khs_ctx* ctx, int out[10]) {
+int khs_get_tuning(khs_ctx* ctx, int out[11]) {
     CTX_OR_FAIL(ctx);
     if (!out) { g_last_error = "null argument"; return KHS_ERR_ARG; }
     Ctx& c = *ctx->impl;
@@ -1504,6 +1515,7 @@ int khs_get_tuning(khs_ctx* ctx, int out[10]) {
     out[7] = khs::z_groups_of(c.zvar);
     out[8] = 0;
     out[9] = c.zmvar;
+    out[10] = c.fused ? 1 : 0;
     if (c.xvar >= 18) {
         const int cells = c.xvar == 18 ? 4 : 8;
         out[0] = 62 * cells;

@@ -1869,6 +1869,317 @@ __global__ __launch_bounds__(256) void k_fct_xz2(State<T> S, State<T> O, KParams
     }
 }
 
+// --------------------------------------------------------------------------
+// Fused x + z sweep in registers (YM).  Lane l of a wavefront owns column
+// x = 58*g - 3 + l and marches a chunk of TZ rows (+3 halo rows each side)
+// down z exactly as k_fct_z does; for every extended row it first performs
+// the x sweep of that row at its column, taking the FCT stencil's neighbours
+// (velocity, variables, LORHOT / LNRHOT, limited fluxes, the px source's x-1
+// fields) from lanes l-1 / l+1 by DPP shifts, and pushes the x-swept values
+// straight into the z pipeline.  The stage never goes to memory and the
+// step-start state is read once per row for both sweeps.  The x stencil
+// reaches 3 cells, so lanes 0-2 and 61-63 are halo lanes: a wavefront stores
+// the 58 columns of lanes 3..60.  A lane whose column lies outside the grid
+// loads through map_x and evaluates its px source cell by cell (the source's
+// x-1 / x+1 cells follow xm_src / xp_src of the mapped cell, not the lane
+// shift), as in k_fct_xr.  Every value is computed by the expressions of
+// k_fct_x and k_fct_z in the same order, so the results are those of the
+// split sweeps bit for bit.
+// --------------------------------------------------------------------------
+constexpr int XZR_OUT = 58;
+
+template <typename T, int TZ>
+__global__ __launch_bounds__(256) void k_fct_xzr(State<T> S, State<T> O, KParams p, int ngx, int nch) {
+    typedef YMModel M;
+    constexpr int NV = F_COUNT;
+    typedef typename M::template Row<T> Row;
+    const int lane = threadIdx.x & 63;
+    int wl = xcd_logical(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+    if (wl >= 2 * ngx * nch) return;          // whole wavefronts leave together
+    wl = tile_order(wl, 2 * ngx * nch, p.rev);
+    const int sp = wl % 2;
+    const int g = (wl / 2) % ngx;
+    const int ch = wl / (2 * ngx);
+    const int nx = p.nx, nz = p.nz;
+    const int x = g * XZR_OUT - 3 + lane;      // extended column of this lane
+    const int xc = map_x(x, p);                // the real column it reads
+    const bool inside = x >= 0 && x < nx;
+    const bool store = lane >= 3 && lane < 3 + XZR_OUT && inside;
+    const int xm = xm_src(xc, p);
+    const bool wall = !p.periodic_x && (xc == 0 || xc == nx - 1);
+    const int b = M::base(sp);
+    const T dt = T(p.dt), dtdx = T(p.dtdx), dtdz = T(p.dtdz);
+    const int z0 = ch * TZ;
+    const int zend = min(z0 + TZ, nz);
+    const int elast = zend + 2;
+
+    auto load = [&](int r, Row& row, T (&F)[NV], T (&ex)[3]) {
+        const unsigned i = (unsigned)xc + (unsigned)r * (unsigned)nx;
+        M::load_row(S, i, (unsigned)xm + (unsigned)r * (unsigned)nx, row);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) F[v] = ld32(S.a[b + v], i);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) ex[c] = ld32(S.a[S_EX1 + c], i);
+    };
+
+    int rc = clampi(z0 - 3, 0, nz - 1);
+    Row prev, cur;
+    T F[NV], EX[3];
+    {
+        T Fd[NV], Ed[3];
+        load(zm_per(rc, nz), prev, Fd, Ed);
+    }
+    load(rc, cur, F, EX);
+    ZPipe<T, NV> zp;
+    zp.Um = T(0);
+    zp.MUm = T(0);
+    for (int e = z0 - 3; e <= elast; ++e) {
+        const int rn = clampi(min(e + 1, elast), 0, nz - 1);
+        Row nrow;
+        T nF[NV], nEX[3];
+        load(rn, nrow, nF, nEX);
+
+        // ---- x sweep of row rc at this lane's column (k_fct_x, YM_FCT.cu:30-91) ----
+        // px source (kernel_ym_lorentz, YM_Fluid.cu:32-84) of the cell
+        T src;
+        {
+            const T n_safe = kmax(F[F_N], T(0.05));
+            const T vz = F[F_PZ] / n_safe;
+            T acc[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const T exm = lane_from_left(EX[c]);
+                T Exn = T(0.5) * (EX[c] + exm);
+                T Byn = T(0.25) * (cur.by[c] + cur.byxm[c] + prev.by[c] + prev.byxm[c]);
+                acc[c] = F[F_Q1 + c] * (Exn + vz * Byn);
+            }
+            src = -(acc[0] + acc[1] + acc[2]);
+            if (p.warm_T > 0.0) {
+                const T nl = lane_from_left(F[F_N]);
+                const T nr = lane_from_right(F[F_N]);
+                T dndx = (nr - nl) / (T(2.0) * T(p.dx));
+                src -= T(p.warm_T) * dndx;
+            }
+            if (!inside) src = lorentz_px(S, sp, xc, rc, p);
+        }
+        T V[NV];
+        {
+            const T u = F[F_PX] / kmax(F[F_N], T(0.05));          // update_velocities
+            const T ur = lane_from_right(u);
+            T vi = T(0.5) * (u + ur);
+            const T eps = vi * dtdx;
+            T nu, mu;
+            fct_nu_mu(eps, nu, mu);
+            const T epsl = lane_from_left(eps), nul = lane_from_left(nu);
+#pragma unroll
+            for (int k = 0; k < NV; ++k) {
+                const T v0 = F[k];
+                const T vm = lane_from_left(v0), vp = lane_from_right(v0);
+                T fL = T(0.5) * epsl * (vm + v0);
+                T fR = T(0.5) * eps * (v0 + vp);
+                T dL = nul * (v0 - vm);
+                T dR = nu * (vp - v0);
+                T sk = (k == F_PX) ? src : T(0.0);
+                T lor = v0 + dt * sk - (fR - fL);
+                T lnr = lor + (dR - dL);
+                const T lorr = lane_from_right(lor);
+                const T lnrl = lane_from_left(lnr);
+                const T lnrr = lane_from_right(lnr);
+                const T lnrr2 = lane_from_right(lnrr);
+                T flx = mu * (lorr - lor);
+                T fl = fct_limit(flx, lnr - lnrl, lnrr - lnr, lnrr2 - lnrr);
+                const T fll = lane_from_left(fl);
+                V[k] = lnr - (fl - fll);
+            }
+        }
+
+        // ---- z sweep (k_fct_z) of the x-swept row ----
+        T uz, Sv[NV], out[NV];
+        M::z_src_rows(F, cur, prev, S, sp, xc, rc, p, uz, Sv);
+        if (zp.push(e, z0, V, uz, Sv, dt, dtdz, out)) {
+            if (out[0] < T(0.05)) out[0] = T(0.05);          // kernel_enforce_boundaries
+            if (wall) out[M::IPX] = T(0.0);
+            if (store) {
+                const unsigned io = (unsigned)xc + (unsigned)(e - 3) * (unsigned)nx;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) st32_nt(O.a[b + v], io, out[v]);
+            }
+        }
+        M::sel_row(rn != rc, cur, prev);
+        cur = nrow;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) F[v] = nF[v];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) EX[c] = nEX[c];
+        rc = rn;
+    }
+}
+
+// The same fused march with the variables split into two groups on two waves
+// of a workgroup (G = 0: n, px, pz; G = 1: Q1..Q3), as k_fct_zg splits the z
+// sweep: half the pipeline state and half the x-sweep arithmetic per lane, so
+// twice the waves fit on a SIMD.  Both waves compute the interface
+// coefficients from the cell's n and px; only G = 0 reads Ex and the
+// By / Ez rows (px and pz sources), only G = 1 the Az rows (precession).
+template <typename T, int TZ, int G>
+__device__ __forceinline__ void xz_march_g(const State<T>& S, const State<T>& O, const KParams& p, int sp, int g,
+                                           int ch, int lane) {
+    typedef YMModel M;
+    constexpr int NV = F_COUNT, NG = M::NG;
+    typedef typename M::template Row<T> Row;
+    const int nx = p.nx, nz = p.nz;
+    const int x = g * XZR_OUT - 3 + lane;
+    const int xc = map_x(x, p);
+    const bool inside = x >= 0 && x < nx;
+    const bool store = lane >= 3 && lane < 3 + XZR_OUT && inside;
+    const int xm = xm_src(xc, p);
+    const bool wall = !p.periodic_x && (xc == 0 || xc == nx - 1);
+    const int b = M::base(sp);
+    const T dt = T(p.dt), dtdx = T(p.dtdx), dtdz = T(p.dtdz);
+    const int z0 = ch * TZ;
+    const int zend = min(z0 + TZ, nz);
+    const int elast = zend + 2;
+
+    auto load = [&](int r, Row& row, T (&F)[NV], T (&ex)[3]) {
+        const unsigned i = (unsigned)xc + (unsigned)r * (unsigned)nx;
+        M::template load_row_g<G>(S, i, (unsigned)xm + (unsigned)r * (unsigned)nx, row);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) F[v] = ld32(S.a[b + v], i);
+        if (G == 0) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) ex[c] = ld32(S.a[S_EX1 + c], i);
+        }
+    };
+
+    int rc = clampi(z0 - 3, 0, nz - 1);
+    Row prev, cur;
+    T F[NV], EX[3];
+    {
+        T Fd[NV], Ed[3];
+        load(zm_per(rc, nz), prev, Fd, Ed);
+    }
+    load(rc, cur, F, EX);
+    ZPipe<T, NG> zp;
+    zp.Um = T(0);
+    zp.MUm = T(0);
+    for (int e = z0 - 3; e <= elast; ++e) {
+        const int rn = clampi(min(e + 1, elast), 0, nz - 1);
+        Row nrow;
+        T nF[NV], nEX[3];
+        load(rn, nrow, nF, nEX);
+
+        // ---- x sweep of row rc at this lane's column (YM_FCT.cu:30-91) ----
+        T src = T(0.0);
+        if (G == 0) {                          // px source (kernel_ym_lorentz, YM_Fluid.cu:32-84)
+            const T n_safe = kmax(F[F_N], T(0.05));
+            const T vz = F[F_PZ] / n_safe;
+            T acc[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const T exm = lane_from_left(EX[c]);
+                T Exn = T(0.5) * (EX[c] + exm);
+                T Byn = T(0.25) * (cur.by[c] + cur.byxm[c] + prev.by[c] + prev.byxm[c]);
+                acc[c] = F[F_Q1 + c] * (Exn + vz * Byn);
+            }
+            src = -(acc[0] + acc[1] + acc[2]);
+            if (p.warm_T > 0.0) {
+                const T nl = lane_from_left(F[F_N]);
+                const T nr = lane_from_right(F[F_N]);
+                T dndx = (nr - nl) / (T(2.0) * T(p.dx));
+                src -= T(p.warm_T) * dndx;
+            }
+            if (!inside) src = lorentz_px(S, sp, xc, rc, p);
+        }
+        T V[NG];
+        {
+            const T u = F[F_PX] / kmax(F[F_N], T(0.05));          // update_velocities
+            const T ur = lane_from_right(u);
+            T vi = T(0.5) * (u + ur);
+            const T eps = vi * dtdx;
+            T nu, mu;
+            fct_nu_mu(eps, nu, mu);
+            const T epsl = lane_from_left(eps), nul = lane_from_left(nu);
+#pragma unroll
+            for (int j = 0; j < NG; ++j) {
+                const int k = G * NG + j;
+                const T v0 = F[k];
+                const T vm = lane_from_left(v0), vp = lane_from_right(v0);
+                T fL = T(0.5) * epsl * (vm + v0);
+                T fR = T(0.5) * eps * (v0 + vp);
+                T dL = nul * (v0 - vm);
+                T dR = nu * (vp - v0);
+                T sk = (k == F_PX) ? src : T(0.0);
+                T lor = v0 + dt * sk - (fR - fL);
+                T lnr = lor + (dR - dL);
+                const T lorr = lane_from_right(lor);
+                const T lnrl = lane_from_left(lnr);
+                const T lnrr = lane_from_right(lnr);
+                const T lnrr2 = lane_from_right(lnrr);
+                T flx = mu * (lorr - lor);
+                T fl = fct_limit(flx, lnr - lnrl, lnrr - lnr, lnrr2 - lnrr);
+                const T fll = lane_from_left(fl);
+                V[j] = lnr - (fl - fll);
+            }
+        }
+
+        // ---- z sweep (k_fct_zg) of the x-swept row ----
+        T uz, Sv[NG], out[NG];
+        M::template z_src_rows_g<G>(F, cur, prev, S, sp, xc, rc, p, uz, Sv);
+        if (zp.push(e, z0, V, uz, Sv, dt, dtdz, out)) {
+            if (G == 0) {
+                if (out[0] < T(0.05)) out[0] = T(0.05);      // kernel_enforce_boundaries
+                if (wall) out[M::IPX] = T(0.0);
+            }
+            if (store) {
+                const unsigned io = (unsigned)xc + (unsigned)(e - 3) * (unsigned)nx;
+#pragma unroll
+                for (int j = 0; j < NG; ++j) st32_nt(O.a[b + G * NG + j], io, out[j]);
+            }
+        }
+        M::template sel_row_g<G>(rn != rc, cur, prev);
+        cur = nrow;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) F[v] = nF[v];
+        if (G == 0) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) EX[c] = nEX[c];
+        }
+        rc = rn;
+    }
+}
+
+// Workgroup = 2 column groups x 2 variable groups (waves 2i, 2i+1 share the
+// columns of tile i, so the second read of the cell's fluid hits the cache).
+template <typename T, int TZ>
+__global__ __launch_bounds__(256) void k_fct_xzg(State<T> S, State<T> O, KParams p, int ngx, int nch) {
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    int wl = __builtin_amdgcn_readfirstlane(xcd_logical(blockIdx.x, gridDim.x) * 2 + (w >> 1));
+    if (wl >= 2 * ngx * nch) return;          // whole wavefronts leave together
+    wl = tile_order(wl, 2 * ngx * nch, p.rev);
+    const int sp = wl % 2;
+    const int g = (wl / 2) % ngx;
+    const int ch = wl / (2 * ngx);
+    if ((w & 1) == 0) xz_march_g<T, TZ, 0>(S, O, p, sp, g, ch, lane);
+    else xz_march_g<T, TZ, 1>(S, O, p, sp, g, ch, lane);
+}
+
+template <typename T, int TZ>
+static void launch_fct_xzg_t(const State<T>& S, const State<T>& O, const KParams& p, hipStream_t st) {
+    const int ngx = (p.nx + XZR_OUT - 1) / XZR_OUT;
+    const int nch = (p.nz + TZ - 1) / TZ;
+    const int tiles = 2 * ngx * nch;
+    hipLaunchKernelGGL((k_fct_xzg<T, TZ>), dim3(round8((tiles + 1) / 2)), dim3(256), 0, st, S, O, p, ngx, nch);
+}
+
+template <typename T, int TZ>
+static void launch_fct_xzr_t(const State<T>& S, const State<T>& O, const KParams& p, hipStream_t st) {
+    const int ngx = (p.nx + XZR_OUT - 1) / XZR_OUT;
+    const int nch = (p.nz + TZ - 1) / TZ;
+    const int waves = 2 * ngx * nch;
+    hipLaunchKernelGGL((k_fct_xzr<T, TZ>), dim3(round8((waves + 3) / 4)), dim3(256), 0, st, S, O, p, ngx, nch);
+}
+
 template <typename T, int TZ>
 static void launch_fct_xz2_t(const State<T>& S, const State<T>& O, const KParams& p, hipStream_t st) {
     const int ntx = (p.nx + XZ2_C - 1) / XZ2_C;
@@ -1877,13 +2188,22 @@ static void launch_fct_xz2_t(const State<T>& S, const State<T>& O, const KParams
 }
 
 // Variant 0: one wave per 64 columns, 32-row chunks; 1/2/3: four waves per
-// 128 columns (variable groups split across waves), 16/32/64-row chunks.
+// 128 columns (variable groups split across waves), 16/32/64-row chunks;
+// 4/5/6: the x sweep in registers (DPP) inside the z march, 32/64/128-row
+// chunks; 7/8/9: the same with the variables on two waves, 16/32/64-row
+// chunks.
 template <typename T>
 void launch_fct_xz(const State<T>& S, const State<T>& O, const KParams& p, int variant, hipStream_t st) {
     switch (variant) {
         case 1: launch_fct_xz2_t<T, 16>(S, O, p, st); return;
         case 2: launch_fct_xz2_t<T, 32>(S, O, p, st); return;
         case 3: launch_fct_xz2_t<T, 64>(S, O, p, st); return;
+        case 4: launch_fct_xzr_t<T, 32>(S, O, p, st); return;
+        case 5: launch_fct_xzr_t<T, 64>(S, O, p, st); return;
+        case 6: launch_fct_xzr_t<T, 128>(S, O, p, st); return;
+        case 7: launch_fct_xzg_t<T, 16>(S, O, p, st); return;
+        case 8: launch_fct_xzg_t<T, 32>(S, O, p, st); return;
+        case 9: launch_fct_xzg_t<T, 64>(S, O, p, st); return;
         default: break;
     }
     constexpr int TZ = 32;

@@ -431,10 +431,11 @@ class Simulation:
 
     def tuning(self):
         """Tile shapes and kernel variants in use (khs_get_tuning)."""
-        out = (ctypes.c_int * 10)()
+        out = (ctypes.c_int * 11)()
         _check(_lib.khs_get_tuning(self._h, out))
         return dict(x_segment=out[0], x_rows=out[1], z_chunk=out[2], z_prefetch=out[3], maxwell=out[4],
-                    x_serial=out[5], xz_variant=out[6], z_groups=out[7], x_lane_cells=out[8], zm_fused=out[9])
+                    x_serial=out[5], xz_variant=out[6], z_groups=out[7], x_lane_cells=out[8], zm_fused=out[9],
+                    fused_sweep=out[10])
 
     def kernel_times(self):
         n = 8

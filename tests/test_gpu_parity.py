@@ -143,10 +143,11 @@ STEP_CASES = {
 }
 
 
-@pytest.mark.parametrize("fused", [1, 0])
+@pytest.mark.parametrize("fused", [1, 0, -1])
 @pytest.mark.parametrize("case", list(STEP_CASES))
 def test_step_bitwise(case, fused):
-    """Whole steps (fused x+z sweep kernel, and the split x / z kernels)."""
+    """Whole steps (fused x+z sweep kernel, the split x / z kernels, and the
+    autotuner's choice between them)."""
     keys = dict(STEP_CASES[case], fused_sweep=fused)
     nx, nz = keys["nx_override"], keys["nz_override"]
     st = random_state(nx, nz, seed=3, smooth=True)
@@ -225,7 +226,7 @@ def test_tile_variants_bitwise(xs, xr, zc, pf, mk, xv, zg, periodic):
     cfg, sim, o = make_pair(st, **keys)
     t = sim.tuning()
     assert t == dict(x_segment=xs, x_rows=xr, z_chunk=zc, z_prefetch=pf, maxwell=mk, x_serial=xv,
-                     xz_variant=t["xz_variant"], z_groups=zg, x_lane_cells=0, zm_fused=0)
+                     xz_variant=t["xz_variant"], z_groups=zg, x_lane_cells=0, zm_fused=0, fused_sweep=0)
     o.sources()
     sim.step(3)
     o.step(3)
@@ -309,12 +310,16 @@ def test_maxwell_march_bitwise(mk, shape, periodic, freeze):
         assert_bitwise(sim.get(n), o.get(n), f"maxwell {mk} {shape} {n}")
 
 
-@pytest.mark.parametrize("xzv", [0, 1, 2, 3])
-@pytest.mark.parametrize("shape,periodic", [((300, 136), 1), ((70, 36), 0), ((256, 64), 0)])
-def test_fused_sweep_variants_bitwise(xzv, shape, periodic):
-    """Every fused x+z sweep kernel variant gives the oracle's bits."""
+@pytest.mark.parametrize("xzv", list(range(10)))
+@pytest.mark.parametrize("shape,periodic,warm", [((300, 136), 1, 0.01), ((70, 36), 0, 0.0), ((256, 64), 0, 0.0),
+                                                 ((1000, 40), 0, 0.01), ((116, 300), 1, 0.0)])
+def test_fused_sweep_variants_bitwise(xzv, shape, periodic, warm):
+    """Every fused x+z sweep kernel variant (LDS-staged x sweep; register x
+    sweep inside the z march: several wave columns per row, both x ends,
+    columns a multiple of the 58 a wave stores, several z chunks) gives the
+    oracle's bits."""
     nx, nz = shape
-    keys = grid(nx, nz, periodic, warm_T=0.01 if periodic else 0.0, fused_sweep=1, xz_variant=xzv, autotune=0)
+    keys = grid(nx, nz, periodic, warm_T=warm, fused_sweep=1, xz_variant=xzv, autotune=0)
     cfg, sim, o = make_pair(random_state(nx, nz, seed=nx + xzv, smooth=True), **keys)
     assert sim.tuning()["xz_variant"] == xzv
     o.sources()
