@@ -35,7 +35,7 @@ template <typename T>
 void launch_derived_em(const State<T>& S, int which, const KParams& p, double* out, hipStream_t st);
 // fused x + z sweep of both species: state S -> fluid of O (variants: see
 // launch_fct_xz in khs_step_kernels.hip)
-constexpr int kNumXZVariants = 10;
+constexpr int kNumXZVariants = 11;
 template <typename T>
 void launch_fct_xz(const State<T>& S, const State<T>& O, const KParams& p, int variant, hipStream_t st);
 // decay/action: per-column sponge / xi_cut (nullptr when neither is active);

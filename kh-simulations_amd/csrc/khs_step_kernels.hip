@@ -429,6 +429,35 @@ struct ZPipe {
         Um = u;
         return have;
     }
+    // push() for a row e >= z0 + 3, where every stage is active: the same
+    // expressions without the warm-up conditions, so that an unrolled march
+    // keeps the pipeline in renamed registers instead of moving it.
+    __device__ __forceinline__ void push_steady(const T (&V)[NV], T u, const T (&Sv)[NV], T dt, T dtdz,
+                                                T (&out)[NV]) {
+        T vi = T(0.5) * (Um + u);
+        T eps = vi * dtdz;
+        T nu, mu;
+        fct_nu_mu(eps, nu, mu);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            T F = T(0.5) * eps * (Vm[v] + V[v]);
+            T D = nu * (V[v] - Vm[v]);
+            T lor = Vm[v] + dt * Sm[v] - (F - Fm[v]);
+            T lnr = lor + (D - Dm[v]);
+            T flx = MUm * (lor - LORm[v]);
+            T fl = fct_limit(FLXm[v], L2[v] - L3[v], L1[v] - L2[v], lnr - L1[v]);
+            out[v] = L2[v] - (fl - FLm[v]);
+            FLm[v] = fl;
+            FLXm[v] = flx;
+            LORm[v] = lor;
+            L3[v] = L2[v]; L2[v] = L1[v]; L1[v] = lnr;
+            Fm[v] = F; Dm[v] = D;
+        }
+        MUm = mu;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { Vm[v] = V[v]; Sm[v] = Sv[v]; }
+        Um = u;
+    }
 };
 
 // Inputs of the z sweep at real cell (x, rc) from the step-start state: the
@@ -2062,8 +2091,15 @@ __device__ __forceinline__ void xz_march_g(const State<T>& S, const State<T>& O,
     ZPipe<T, NG> zp;
     zp.Um = T(0);
     zp.MUm = T(0);
-    for (int e = z0 - 3; e <= elast; ++e) {
-        const int rn = clampi(min(e + 1, elast), 0, nz - 1);
+
+    // One extended row e (real row rc): load row rn ahead, x-sweep row rc at
+    // this lane's column, push it through the z pipeline.  MODE 0: the
+    // pipeline's warm-up rows (e < z0 + 3) and the clamped rows at the end of
+    // the grid (rn may equal rc); MODE 1: steady rows, where every pipeline
+    // stage is active and row rn = rc + 1.
+    auto row = [&](int e, auto mode) {
+        constexpr int MODE = decltype(mode)::value;
+        const int rn = MODE ? rc + 1 : clampi(min(e + 1, elast), 0, nz - 1);
         Row nrow;
         T nF[NV], nEX[3];
         load(rn, nrow, nF, nEX);
@@ -2125,7 +2161,10 @@ __device__ __forceinline__ void xz_march_g(const State<T>& S, const State<T>& O,
         // ---- z sweep (k_fct_zg) of the x-swept row ----
         T uz, Sv[NG], out[NG];
         M::template z_src_rows_g<G>(F, cur, prev, S, sp, xc, rc, p, uz, Sv);
-        if (zp.push(e, z0, V, uz, Sv, dt, dtdz, out)) {
+        bool have = true;
+        if (MODE) zp.push_steady(V, uz, Sv, dt, dtdz, out);
+        else have = zp.push(e, z0, V, uz, Sv, dt, dtdz, out);
+        if (have) {
             if (G == 0) {
                 if (out[0] < T(0.05)) out[0] = T(0.05);      // kernel_enforce_boundaries
                 if (wall) out[M::IPX] = T(0.0);
@@ -2136,7 +2175,8 @@ __device__ __forceinline__ void xz_march_g(const State<T>& S, const State<T>& O,
                 for (int j = 0; j < NG; ++j) st32_nt(O.a[b + G * NG + j], io, out[j]);
             }
         }
-        M::template sel_row_g<G>(rn != rc, cur, prev);
+        if (MODE) prev = cur;
+        else M::template sel_row_g<G>(rn != rc, cur, prev);
         cur = nrow;
 #pragma unroll
         for (int v = 0; v < NV; ++v) F[v] = nF[v];
@@ -2145,31 +2185,50 @@ __device__ __forceinline__ void xz_march_g(const State<T>& S, const State<T>& O,
             for (int c = 0; c < 3; ++c) EX[c] = nEX[c];
         }
         rc = rn;
+    };
+
+    // rows e = z0-3 .. z0+2 fill the pipeline; steady rows run while e+1 is
+    // a real row below elast; the rest (clamped at the grid's end) as warm-up
+    int e = z0 - 3;
+    for (; e < z0 + 3; ++e) row(e, std::integral_constant<int, 0>());
+    const int esteady = min(elast - 1, nz - 2);
+    // three rows per iteration: the pipeline's three-row LNRHOT window then
+    // returns to the same registers, so it is renamed rather than moved
+    for (; e + 2 <= esteady; e += 3) {
+        row(e, std::integral_constant<int, 1>());
+        row(e + 1, std::integral_constant<int, 1>());
+        row(e + 2, std::integral_constant<int, 1>());
     }
+    for (; e <= esteady; ++e) row(e, std::integral_constant<int, 1>());
+    for (; e <= elast; ++e) row(e, std::integral_constant<int, 0>());
 }
 
 // Workgroup = 2 column groups x 2 variable groups (waves 2i, 2i+1 share the
 // columns of tile i, so the second read of the cell's fluid hits the cache).
-template <typename T, int TZ>
-__global__ __launch_bounds__(256) void k_fct_xzg(State<T> S, State<T> O, KParams p, int ngx, int nch) {
+template <typename T, int TZ, int BS>
+__global__ __launch_bounds__(BS) void k_fct_xzg(State<T> S, State<T> O, KParams p, int ngx, int nch) {
+    constexpr int TPB = BS / 128;             // tiles per workgroup
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    int wl = __builtin_amdgcn_readfirstlane(xcd_logical(blockIdx.x, gridDim.x) * 2 + (w >> 1));
+    const int grp = w & 1;
+    int wl = __builtin_amdgcn_readfirstlane(xcd_logical(blockIdx.x, gridDim.x) * TPB + (w >> 1));
     if (wl >= 2 * ngx * nch) return;          // whole wavefronts leave together
     wl = tile_order(wl, 2 * ngx * nch, p.rev);
     const int sp = wl % 2;
     const int g = (wl / 2) % ngx;
     const int ch = wl / (2 * ngx);
-    if ((w & 1) == 0) xz_march_g<T, TZ, 0>(S, O, p, sp, g, ch, lane);
+    if (grp == 0) xz_march_g<T, TZ, 0>(S, O, p, sp, g, ch, lane);
     else xz_march_g<T, TZ, 1>(S, O, p, sp, g, ch, lane);
 }
 
-template <typename T, int TZ>
+template <typename T, int TZ, int BS = 256>
 static void launch_fct_xzg_t(const State<T>& S, const State<T>& O, const KParams& p, hipStream_t st) {
+    constexpr int TPB = BS / 128;
     const int ngx = (p.nx + XZR_OUT - 1) / XZR_OUT;
     const int nch = (p.nz + TZ - 1) / TZ;
     const int tiles = 2 * ngx * nch;
-    hipLaunchKernelGGL((k_fct_xzg<T, TZ>), dim3(round8((tiles + 1) / 2)), dim3(256), 0, st, S, O, p, ngx, nch);
+    hipLaunchKernelGGL((k_fct_xzg<T, TZ, BS>), dim3(round8((tiles + TPB - 1) / TPB)), dim3(BS), 0, st, S, O, p,
+                       ngx, nch);
 }
 
 template <typename T, int TZ>
@@ -2191,7 +2250,8 @@ static void launch_fct_xz2_t(const State<T>& S, const State<T>& O, const KParams
 // 128 columns (variable groups split across waves), 16/32/64-row chunks;
 // 4/5/6: the x sweep in registers (DPP) inside the z march, 32/64/128-row
 // chunks; 7/8/9: the same with the variables on two waves, 16/32/64-row
-// chunks.
+// chunks (two column tiles per workgroup), 10: 32-row chunks, one tile per
+// workgroup.
 template <typename T>
 void launch_fct_xz(const State<T>& S, const State<T>& O, const KParams& p, int variant, hipStream_t st) {
     switch (variant) {
@@ -2204,6 +2264,7 @@ void launch_fct_xz(const State<T>& S, const State<T>& O, const KParams& p, int v
         case 7: launch_fct_xzg_t<T, 16>(S, O, p, st); return;
         case 8: launch_fct_xzg_t<T, 32>(S, O, p, st); return;
         case 9: launch_fct_xzg_t<T, 64>(S, O, p, st); return;
+        case 10: launch_fct_xzg_t<T, 32, 128>(S, O, p, st); return;
         default: break;
     }
     constexpr int TZ = 32;

@@ -310,7 +310,7 @@ def test_maxwell_march_bitwise(mk, shape, periodic, freeze):
         assert_bitwise(sim.get(n), o.get(n), f"maxwell {mk} {shape} {n}")
 
 
-@pytest.mark.parametrize("xzv", list(range(10)))
+@pytest.mark.parametrize("xzv", list(range(11)))
 @pytest.mark.parametrize("shape,periodic,warm", [((300, 136), 1, 0.01), ((70, 36), 0, 0.0), ((256, 64), 0, 0.0),
                                                  ((1000, 40), 0, 0.01), ((116, 300), 1, 0.0)])
 def test_fused_sweep_variants_bitwise(xzv, shape, periodic, warm):
