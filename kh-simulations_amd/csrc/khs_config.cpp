@@ -211,6 +211,12 @@ void config_load(const std::string& path, khs_config& c, std::string* warnings) 
 
 // emhdgpu2d run (main_coupled.cu:228-260): 256x256, dx = dz = 1, dt = 0.002,
 // energy check every 500 steps, snapshot every 1000, halt at E/E0 > 5.
+// The kernels address each field array with 32-bit byte offsets.
+static void check_grid_bytes(long nx, long nz) {
+    if (nx * nz * (long)sizeof(double) >= (1L << 32))
+        throw Error(KHS_ERR_CONFIG, "grid too large: NX*NZ must stay below 2^29 cells (each field array under 4 GiB)");
+}
+
 static void config_derive_emhd(khs_config& c) {
     c.nz = (c.nz_override > 0) ? c.nz_override : 256;
     c.nx = (c.nx_override > 0) ? c.nx_override : 256;
@@ -234,6 +240,7 @@ static void config_derive_emhd(khs_config& c) {
     if (c.export_stride_override < 0) c.export_stride = 0;
     c.energy_thresh = 5.0;
     if (c.nx < 4 || c.nz < 4) throw Error(KHS_ERR_CONFIG, "grid too small");
+    check_grid_bytes(c.nx, c.nz);
     if (c.species != 1 && c.species != 2) throw Error(KHS_ERR_CONFIG, "species must be 1 or 2");
     if (c.x_segment != 0 && c.x_segment != 64 && c.x_segment != 128 && c.x_segment != 256)
         throw Error(KHS_ERR_CONFIG, "x_segment must be 0, 64, 128 or 256");
@@ -292,6 +299,7 @@ void config_derive(khs_config& c) {
     const int NZ = c.nz, NX = c.nx;
     if (NX < 4 || NZ < 4)
         throw Error(KHS_ERR_CONFIG, "grid too small (NX=" + std::to_string(NX) + ", NZ=" + std::to_string(NZ) + ")");
+    check_grid_bytes(NX, NZ);
     if (c.suppress_kz0 && (NZ & (NZ - 1)) != 0)
         throw Error(KHS_ERR_CONFIG, "suppress_kz0=1 requires NZ to be a power of 2 (got NZ=" + std::to_string(NZ) + ")");
     if ((c.kz_suppress_max >= 1 || c.kz_suppress_hi > 0) && (NZ % 32 != 0))

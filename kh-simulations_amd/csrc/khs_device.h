@@ -89,6 +89,25 @@ __device__ __forceinline__ void st32_nt(T* a, unsigned i, T v) {
     __builtin_nontemporal_store(v, reinterpret_cast<T*>(reinterpret_cast<char*>(a) + i * (unsigned)sizeof(T)));
 }
 
+// Raw buffer stores (non-temporal).  The buffer's range check drops a store
+// whose byte offset is at or beyond the buffer's size, so lanes that must not
+// write take offset kNoStore instead of a branch around the store: a store
+// under a lane-divergent branch makes the compiler's wait counting assume the
+// branch was not taken, and every later wait for an earlier load then also
+// waits for the store to complete.  Arrays stay below 4 GiB (the config
+// checks the grid), so kNoStore is always out of range.
+constexpr unsigned kNoStore = 0xFFFFFFF0u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ void buf_st_nt(double v, __amdgpu_buffer_rsrc_t r, unsigned off) {
+    typedef unsigned U2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, off, 0, 2);
+}
+__device__ __forceinline__ void buf_st_nt(float v, __amdgpu_buffer_rsrc_t r, unsigned off) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 2);
+}
+
 // Whole-wavefront lane shifts (DPP wave_shr:1 / wave_shl:1 of GFX9):
 // lane_from_left(v) in lane l is v of lane l-1, lane_from_right(v) is v of
 // lane l+1; lane 0 (from the left) and lane 63 (from the right) receive 0.

@@ -2044,17 +2044,30 @@ __global__ __launch_bounds__(256) void k_fct_xzr(State<T> S, State<T> O, KParams
     }
 }
 
-// The same fused march with the variables split into two groups on two waves
-// of a workgroup (G = 0: n, px, pz; G = 1: Q1..Q3), as k_fct_zg splits the z
-// sweep: half the pipeline state and half the x-sweep arithmetic per lane, so
-// twice the waves fit on a SIMD.  Both waves compute the interface
-// coefficients from the cell's n and px; only G = 0 reads Ex and the
-// By / Ez rows (px and pz sources), only G = 1 the Az rows (precession).
-template <typename T, int TZ, int G>
+// The same fused march with the variables split into groups on separate
+// waves of a workgroup, as k_fct_zg splits the z sweep: a smaller pipeline
+// and less x-sweep arithmetic per lane, so more waves fit on a SIMD.  Every
+// wave computes the interface coefficients from the cell's n and px and the
+// z velocity from n and pz; each reads only the field rows its sources use.
+// A group is variables VB .. VB+NGV-1 (FluidVar order) with the rows it
+// reads: Ex and By (px source), Ez and By (pz source), Az (precession).
+template <int VB_, int NGV_, bool EX_, bool BY_, bool EZ_, bool AZ_>
+struct XzGroup {
+    static constexpr int VB = VB_, NGV = NGV_;
+    static constexpr bool EX = EX_, BY = BY_, EZ = EZ_, AZ = AZ_;
+    static constexpr bool has(int k) { return k >= VB && k < VB + NGV; }
+};
+// two waves: {n, px, pz}, {Q1, Q2, Q3} (a three-wave split {n, px}, {pz},
+// {Q} reaches occupancy 3 in FP64 but repeats the shared loads and
+// coefficients once more and measured slower)
+typedef XzGroup<F_N, 3, true, true, true, false> XzN3;
+typedef XzGroup<F_Q1, 3, false, false, false, true> XzQ3;
+
+template <typename T, int TZ, typename GR>
 __device__ __forceinline__ void xz_march_g(const State<T>& S, const State<T>& O, const KParams& p, int sp, int g,
                                            int ch, int lane) {
     typedef YMModel M;
-    constexpr int NV = F_COUNT, NG = M::NG;
+    constexpr int NV = F_COUNT, NG = GR::NGV, VB = GR::VB;
     typedef typename M::template Row<T> Row;
     const int nx = p.nx, nz = p.nz;
     const int x = g * XZR_OUT - 3 + lane;
@@ -2068,13 +2081,26 @@ __device__ __forceinline__ void xz_march_g(const State<T>& S, const State<T>& O,
     const int z0 = ch * TZ;
     const int zend = min(z0 + TZ, nz);
     const int elast = zend + 2;
+    const unsigned bytes = (unsigned)nx * (unsigned)nz * (unsigned)sizeof(T);
+    __amdgpu_buffer_rsrc_t orsc[NG];
+#pragma unroll
+    for (int j = 0; j < NG; ++j) orsc[j] = buf_rsrc(O.a[b + VB + j], bytes);
 
     auto load = [&](int r, Row& row, T (&F)[NV], T (&ex)[3]) {
         const unsigned i = (unsigned)xc + (unsigned)r * (unsigned)nx;
-        M::template load_row_g<G>(S, i, (unsigned)xm + (unsigned)r * (unsigned)nx, row);
+        const unsigned ixm = (unsigned)xm + (unsigned)r * (unsigned)nx;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (GR::EZ) row.ez[c] = ld32(S.a[S_EZ1 + c], i);
+            if (GR::BY) {
+                row.by[c] = ld32(S.a[S_BY1 + c], i);
+                row.byxm[c] = ld32(S.a[S_BY1 + c], ixm);
+            }
+            if (GR::AZ) row.az[c] = ld32(S.a[S_AZ1 + c], i);
+        }
 #pragma unroll
         for (int v = 0; v < NV; ++v) F[v] = ld32(S.a[b + v], i);
-        if (G == 0) {
+        if (GR::EX) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) ex[c] = ld32(S.a[S_EX1 + c], i);
         }
@@ -2106,7 +2132,7 @@ __device__ __forceinline__ void xz_march_g(const State<T>& S, const State<T>& O,
 
         // ---- x sweep of row rc at this lane's column (YM_FCT.cu:30-91) ----
         T src = T(0.0);
-        if (G == 0) {                          // px source (kernel_ym_lorentz, YM_Fluid.cu:32-84)
+        if (GR::has(F_PX)) {                   // px source (kernel_ym_lorentz, YM_Fluid.cu:32-84)
             const T n_safe = kmax(F[F_N], T(0.05));
             const T vz = F[F_PZ] / n_safe;
             T acc[3];
@@ -2137,7 +2163,7 @@ __device__ __forceinline__ void xz_march_g(const State<T>& S, const State<T>& O,
             const T epsl = lane_from_left(eps), nul = lane_from_left(nu);
 #pragma unroll
             for (int j = 0; j < NG; ++j) {
-                const int k = G * NG + j;
+                const int k = VB + j;
                 const T v0 = F[k];
                 const T vm = lane_from_left(v0), vp = lane_from_right(v0);
                 T fL = T(0.5) * epsl * (vm + v0);
@@ -2158,29 +2184,44 @@ __device__ __forceinline__ void xz_march_g(const State<T>& S, const State<T>& O,
             }
         }
 
-        // ---- z sweep (k_fct_zg) of the x-swept row ----
-        T uz, Sv[NG], out[NG];
-        M::template z_src_rows_g<G>(F, cur, prev, S, sp, xc, rc, p, uz, Sv);
+        // ---- z sweep of the x-swept row: stale z velocity and the z sources
+        // (YMModel::z_src_rows: pz Lorentz force, precession of Q) ----
+        T Sv[NG], out[NG];
+        const T uz = F[F_PZ] / kmax(F[F_N], T(0.05));
+        {
+            const T q[3] = {F[F_Q1], F[F_Q2], F[F_Q3]};
+            T spz = T(0), sq[3] = {T(0), T(0), T(0)};
+            if (GR::has(F_PZ)) {
+                spz = lorentz_pz_v(F[F_N], F[F_PX], q, cur.ez, prev.ez, cur.by, cur.byxm, prev.by, prev.byxm);
+                if (p.warm_T > 0.0) spz -= warm_pz(S, sp, xc, rc, p);
+            }
+            if (GR::has(F_Q1)) precession_v(F[F_N], F[F_PZ], q, cur.az, prev.az, T(p.alpha), sq);
+#pragma unroll
+            for (int j = 0; j < NG; ++j) {
+                const int k = VB + j;
+                Sv[j] = (k == F_PZ) ? spz : (k >= F_Q1) ? sq[k >= F_Q1 ? k - F_Q1 : 0] : T(0);
+            }
+        }
         bool have = true;
         if (MODE) zp.push_steady(V, uz, Sv, dt, dtdz, out);
         else have = zp.push(e, z0, V, uz, Sv, dt, dtdz, out);
         if (have) {
-            if (G == 0) {
-                if (out[0] < T(0.05)) out[0] = T(0.05);      // kernel_enforce_boundaries
-                if (wall) out[M::IPX] = T(0.0);
-            }
-            if (store) {
-                const unsigned io = (unsigned)xc + (unsigned)(e - 3) * (unsigned)nx;
+            if (GR::has(F_N) && out[F_N - VB] < T(0.05)) out[F_N - VB] = T(0.05);   // kernel_enforce_boundaries
+            if (GR::has(F_PX) && wall) out[F_PX - VB] = T(0.0);
+            const unsigned off =
+                store ? ((unsigned)xc + (unsigned)(e - 3) * (unsigned)nx) * (unsigned)sizeof(T) : kNoStore;
 #pragma unroll
-                for (int j = 0; j < NG; ++j) st32_nt(O.a[b + G * NG + j], io, out[j]);
-            }
+            for (int j = 0; j < NG; ++j) buf_st_nt(out[j], orsc[j], off);
         }
-        if (MODE) prev = cur;
-        else M::template sel_row_g<G>(rn != rc, cur, prev);
+        if (MODE) {
+            prev = cur;
+        } else if (rn != rc) {
+            prev = cur;
+        }
         cur = nrow;
 #pragma unroll
         for (int v = 0; v < NV; ++v) F[v] = nF[v];
-        if (G == 0) {
+        if (GR::EX) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) EX[c] = nEX[c];
         }
@@ -2217,8 +2258,8 @@ __global__ __launch_bounds__(BS) void k_fct_xzg(State<T> S, State<T> O, KParams 
     const int sp = wl % 2;
     const int g = (wl / 2) % ngx;
     const int ch = wl / (2 * ngx);
-    if (grp == 0) xz_march_g<T, TZ, 0>(S, O, p, sp, g, ch, lane);
-    else xz_march_g<T, TZ, 1>(S, O, p, sp, g, ch, lane);
+    if (grp == 0) xz_march_g<T, TZ, XzN3>(S, O, p, sp, g, ch, lane);
+    else xz_march_g<T, TZ, XzQ3>(S, O, p, sp, g, ch, lane);
 }
 
 template <typename T, int TZ, int BS = 256>
