@@ -64,23 +64,24 @@ ZM_ROWS = {1: 16, 2: 32, 3: 64}       # chunk height of the fused z sweep + Maxw
 # by what the counter reads for a plain 256 MiB device copy (fetch x2.0,
 # write x1.0); summary under profiles/.  Keyed by kernel instance.
 PMC_TRAFFIC_MB = {
-    "k_fct_x<double, 128, YMModel, 0>": 511.5, "k_fct_x<double, 128, YMModel, 1>": 524.2,
-    "k_fct_x<double, 256, YMModel, 0>": 509.4, "k_fct_x<double, 256, YMModel, 1>": 524.5,
-    "k_fct_x<double, 64, YMModel, 0>": 511.9, "k_fct_x<double, 64, YMModel, 1>": 524.1,
-    "k_fct_xr<double, 4>": 506.4, "k_fct_xr<double, 8>": 505.8,
-    "k_fct_z<double, 16, 0, YMModel>": 985.3, "k_fct_z<double, 16, 1, YMModel>": 972.6,
-    "k_fct_z<double, 32, 0, YMModel>": 874.0, "k_fct_z<double, 32, 1, YMModel>": 869.4,
-    "k_fct_z<double, 64, 0, YMModel>": 806.7, "k_fct_z<double, 64, 1, YMModel>": 822.7,
-    "k_fct_zg<double, 16, 0, YMModel>": 1050.8, "k_fct_zg<double, 16, 1, YMModel>": 994.5,
-    "k_fct_zg<double, 32, 0, YMModel>": 987.2, "k_fct_zg<double, 32, 1, YMModel>": 930.8,
-    "k_fct_zg<double, 64, 0, YMModel>": 933.8, "k_fct_zg<double, 64, 1, YMModel>": 952.9,
-    "k_fct_zmx<double, 16>": 1262.3, "k_fct_zmx<double, 32>": 1143.5,
-    "k_fct_zmx<double, 64>": 1101.5, "k_maxwell<double, 16>": 842.9,
-    "k_maxwell<double, 4>": 712.0, "k_maxwell<double, 8>": 854.2,
-    "k_maxwell_cell<double>": 596.4, "k_maxwell_zm<double, 16, 0>": 652.1,
-    "k_maxwell_zm<double, 16, 1>": 636.3, "k_maxwell_zm<double, 32, 1>": 621.9,
-    "k_maxwell_zm<double, 4, 1>": 693.8, "k_maxwell_zm<double, 8, 0>": 688.2,
-    "k_maxwell_zm<double, 8, 1>": 659.3,
+    "k_fct_x<double, 128, YMModel, 0>": 507.7, "k_fct_x<double, 128, YMModel, 1>": 512.6,
+    "k_fct_x<double, 256, YMModel, 0>": 507.5, "k_fct_x<double, 256, YMModel, 1>": 512.3,
+    "k_fct_x<double, 64, YMModel, 0>": 507.4, "k_fct_x<double, 64, YMModel, 1>": 512.7,
+    "k_fct_xr<double, 4>": 507.7, "k_fct_xr<double, 8>": 528.7, "k_fct_xz2<double, 16>": 706.7,
+    "k_fct_xz2<double, 32>": 666.2, "k_fct_xz2<double, 64>": 642.0, "k_fct_xz<double, 32>": 680.7,
+    "k_fct_xzg<double, 16, 256>": 671.1, "k_fct_xzg<double, 32, 128>": 772.2, "k_fct_xzg<double, 32, 256>": 694.5,
+    "k_fct_xzg<double, 64, 256>": 758.4, "k_fct_xzr<double, 128>": 648.2, "k_fct_xzr<double, 32>": 681.2,
+    "k_fct_xzr<double, 64>": 661.0, "k_fct_z2<double, 16, YMModel>": 970.4, "k_fct_z2<double, 32, YMModel>": 864.8,
+    "k_fct_z2<double, 64, YMModel>": 811.5, "k_fct_z<double, 16, 0, YMModel>": 979.2,
+    "k_fct_z<double, 16, 1, YMModel>": 971.1, "k_fct_z<double, 32, 0, YMModel>": 870.0,
+    "k_fct_z<double, 32, 1, YMModel>": 868.1, "k_fct_z<double, 64, 0, YMModel>": 806.5,
+    "k_fct_z<double, 64, 1, YMModel>": 814.9, "k_fct_zg<double, 16, 0, YMModel>": 1054.6,
+    "k_fct_zg<double, 16, 1, YMModel>": 993.5, "k_fct_zg<double, 32, 0, YMModel>": 985.3,
+    "k_fct_zg<double, 32, 1, YMModel>": 926.1, "k_fct_zg<double, 64, 0, YMModel>": 929.8,
+    "k_fct_zg<double, 64, 1, YMModel>": 949.5, "k_maxwell<double, 16>": 846.3, "k_maxwell<double, 4>": 719.5,
+    "k_maxwell<double, 8>": 859.3, "k_maxwell_cell<double>": 597.5, "k_maxwell_zm<double, 16, 0>": 652.9,
+    "k_maxwell_zm<double, 16, 1>": 636.2, "k_maxwell_zm<double, 32, 1>": 620.7,
+    "k_maxwell_zm<double, 4, 1>": 702.0, "k_maxwell_zm<double, 8, 0>": 688.9, "k_maxwell_zm<double, 8, 1>": 659.2,
 }
 
 
