@@ -2160,16 +2160,18 @@ __device__ __forceinline__ void xz_march_g(const State<T>& S, const State<T>& O,
             const T eps = vi * dtdx;
             T nu, mu;
             fct_nu_mu(eps, nu, mu);
-            const T epsl = lane_from_left(eps), nul = lane_from_left(nu);
 #pragma unroll
             for (int j = 0; j < NG; ++j) {
                 const int k = VB + j;
                 const T v0 = F[k];
-                const T vm = lane_from_left(v0), vp = lane_from_right(v0);
-                T fL = T(0.5) * epsl * (vm + v0);
+                const T vp = lane_from_right(v0);
+                // the left interface's fluxes are the left lane's right ones
+                // (0.5 eps(x-1) (v(x-1) + v(x)), nu(x-1) (v(x) - v(x-1)):
+                // the same operands in the same order)
                 T fR = T(0.5) * eps * (v0 + vp);
-                T dL = nul * (v0 - vm);
                 T dR = nu * (vp - v0);
+                const T fL = lane_from_left(fR);
+                const T dL = lane_from_left(dR);
                 T sk = (k == F_PX) ? src : T(0.0);
                 T lor = v0 + dt * sk - (fR - fL);
                 T lnr = lor + (dR - dL);
