@@ -2063,7 +2063,12 @@ struct XzGroup {
 typedef XzGroup<F_N, 3, true, true, true, false> XzN3;
 typedef XzGroup<F_Q1, 3, false, false, false, true> XzQ3;
 
-template <typename T, int TZ, typename GR>
+// WARM: warm_T > 0 (pressure terms in both sources).  EDGE: some lane's
+// column lies outside the grid (the first and last column tiles); interior
+// wavefronts take the instantiation without the per-cell source fallback, so
+// their steady rows are free of branches and the compiler can overlap one
+// row's loads and x sweep with the previous row's z pipeline.
+template <typename T, int TZ, typename GR, bool WARM, bool EDGE>
 __device__ __forceinline__ void xz_march_g(const State<T>& S, const State<T>& O, const KParams& p, int sp, int g,
                                            int ch, int lane) {
     typedef YMModel M;
@@ -2072,7 +2077,7 @@ __device__ __forceinline__ void xz_march_g(const State<T>& S, const State<T>& O,
     const int nx = p.nx, nz = p.nz;
     const int x = g * XZR_OUT - 3 + lane;
     const int xc = map_x(x, p);
-    const bool inside = x >= 0 && x < nx;
+    const bool inside = !EDGE || (x >= 0 && x < nx);
     const bool store = lane >= 3 && lane < 3 + XZR_OUT && inside;
     const int xm = xm_src(xc, p);
     const bool wall = !p.periodic_x && (xc == 0 || xc == nx - 1);
@@ -2144,13 +2149,13 @@ __device__ __forceinline__ void xz_march_g(const State<T>& S, const State<T>& O,
                 acc[c] = F[F_Q1 + c] * (Exn + vz * Byn);
             }
             src = -(acc[0] + acc[1] + acc[2]);
-            if (p.warm_T > 0.0) {
+            if (WARM) {
                 const T nl = lane_from_left(F[F_N]);
                 const T nr = lane_from_right(F[F_N]);
                 T dndx = (nr - nl) / (T(2.0) * T(p.dx));
                 src -= T(p.warm_T) * dndx;
             }
-            if (!inside) src = lorentz_px(S, sp, xc, rc, p);
+            if (EDGE && !inside) src = lorentz_px(S, sp, xc, rc, p);
         }
         T V[NG];
         {
@@ -2195,7 +2200,7 @@ __device__ __forceinline__ void xz_march_g(const State<T>& S, const State<T>& O,
             T spz = T(0), sq[3] = {T(0), T(0), T(0)};
             if (GR::has(F_PZ)) {
                 spz = lorentz_pz_v(F[F_N], F[F_PX], q, cur.ez, prev.ez, cur.by, cur.byxm, prev.by, prev.byxm);
-                if (p.warm_T > 0.0) spz -= warm_pz(S, sp, xc, rc, p);
+                if (WARM) spz -= warm_pz(S, sp, xc, rc, p);
             }
             if (GR::has(F_Q1)) precession_v(F[F_N], F[F_PZ], q, cur.az, prev.az, T(p.alpha), sq);
 #pragma unroll
@@ -2249,7 +2254,8 @@ __device__ __forceinline__ void xz_march_g(const State<T>& S, const State<T>& O,
 // Workgroup = 2 column groups x 2 variable groups (waves 2i, 2i+1 share the
 // columns of tile i, so the second read of the cell's fluid hits the cache).
 template <typename T, int TZ, int BS>
-__global__ __launch_bounds__(BS) void k_fct_xzg(State<T> S, State<T> O, KParams p, int ngx, int nch) {
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_fct_xzg(State<T> S, State<T> O, KParams p, int ngx,
+                                                                                  int nch) {
     constexpr int TPB = BS / 128;             // tiles per workgroup
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -2260,8 +2266,20 @@ __global__ __launch_bounds__(BS) void k_fct_xzg(State<T> S, State<T> O, KParams 
     const int sp = wl % 2;
     const int g = (wl / 2) % ngx;
     const int ch = wl / (2 * ngx);
-    if (grp == 0) xz_march_g<T, TZ, XzN3>(S, O, p, sp, g, ch, lane);
-    else xz_march_g<T, TZ, XzQ3>(S, O, p, sp, g, ch, lane);
+    const bool warm = p.warm_T > 0.0;
+    const bool edge = g == 0 || g * XZR_OUT + 60 >= p.nx;   // lanes cover columns 58g-3 .. 58g+60
+    if (grp == 0) {
+        if (warm) {
+            if (edge) xz_march_g<T, TZ, XzN3, true, true>(S, O, p, sp, g, ch, lane);
+            else xz_march_g<T, TZ, XzN3, true, false>(S, O, p, sp, g, ch, lane);
+        } else {
+            if (edge) xz_march_g<T, TZ, XzN3, false, true>(S, O, p, sp, g, ch, lane);
+            else xz_march_g<T, TZ, XzN3, false, false>(S, O, p, sp, g, ch, lane);
+        }
+    } else {
+        if (edge) xz_march_g<T, TZ, XzQ3, false, true>(S, O, p, sp, g, ch, lane);   // no warm terms
+        else xz_march_g<T, TZ, XzQ3, false, false>(S, O, p, sp, g, ch, lane);
+    }
 }
 
 template <typename T, int TZ, int BS = 256>
