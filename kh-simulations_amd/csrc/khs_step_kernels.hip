@@ -2099,7 +2099,9 @@ __device__ __forceinline__ void xz_march_g(const State<T>& S, const State<T>& O,
             if (GR::EZ) row.ez[c] = ld32(S.a[S_EZ1 + c], i);
             if (GR::BY) {
                 row.by[c] = ld32(S.a[S_BY1 + c], i);
-                row.byxm[c] = ld32(S.a[S_BY1 + c], ixm);
+                // interior wavefronts: By(x-1) is lane l-1's By (lane 0's
+                // value is unused: it is a halo lane)
+                row.byxm[c] = EDGE ? ld32(S.a[S_BY1 + c], ixm) : lane_from_left(row.by[c]);
             }
             if (GR::AZ) row.az[c] = ld32(S.a[S_AZ1 + c], i);
         }
