@@ -107,6 +107,14 @@ __device__ __forceinline__ void buf_st_nt(double v, __amdgpu_buffer_rsrc_t r, un
 __device__ __forceinline__ void buf_st_nt(float v, __amdgpu_buffer_rsrc_t r, unsigned off) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 2);
 }
+// the same without the non-temporal hint (outputs read again soon)
+__device__ __forceinline__ void buf_st(double v, __amdgpu_buffer_rsrc_t r, unsigned off) {
+    typedef unsigned U2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, off, 0, 0);
+}
+__device__ __forceinline__ void buf_st(float v, __amdgpu_buffer_rsrc_t r, unsigned off) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
+}
 
 // Whole-wavefront lane shifts (DPP wave_shr:1 / wave_shl:1 of GFX9):
 // lane_from_left(v) in lane l is v of lane l-1, lane_from_right(v) is v of

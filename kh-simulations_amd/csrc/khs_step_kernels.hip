@@ -1129,7 +1129,8 @@ template <typename T, int TZ, int PF>
 __global__ __launch_bounds__(256) void k_maxwell_zm(State<T> S, State<T> O, KParams p, int nwx, int nch,
                                                    const double* decay, const int* action) {
     const int lane = threadIdx.x & 63;
-    int wl = xcd_logical(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+    // the wavefront index is uniform: scalar loop control
+    int wl = __builtin_amdgcn_readfirstlane(xcd_logical(blockIdx.x, gridDim.x) * 4 + (int)(threadIdx.x >> 6));
     if (wl >= nwx * nch) return;             // whole wavefronts leave together
     wl = tile_order(wl, nwx * nch, p.rev);
     const int nx = p.nx, nz = p.nz;
@@ -1147,6 +1148,17 @@ __global__ __launch_bounds__(256) void k_maxwell_zm(State<T> S, State<T> O, KPar
     const T dx = T(p.dx), dz = T(p.dz);
     const int act = action ? action[xc] : 0;
     const T dec = (act == 1) ? T(decay[xc]) : T(1.0);
+    // halo lane 63 and lanes past the last column store through range-checked
+    // buffer stores at an out-of-range offset: no branch around the stores
+    const unsigned bytes = (unsigned)nx * (unsigned)nz * (unsigned)sizeof(T);
+    __amdgpu_buffer_rsrc_t rby[3], rex[3], rez[3], raz[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        rby[c] = buf_rsrc(O.a[S_BY1 + c], bytes);
+        rex[c] = buf_rsrc(O.a[S_EX1 + c], bytes);
+        rez[c] = buf_rsrc(O.a[S_EZ1 + c], bytes);
+        raz[c] = buf_rsrc(O.a[S_AZ1 + c], bytes);
+    }
 
     // carried from row e-1
     T byp[3], byxmp[3], azp[3], ezp[3], exn_p[3];
@@ -1233,8 +1245,8 @@ __global__ __launch_bounds__(256) void k_maxwell_zm(State<T> S, State<T> O, KPar
             const T cr[3] = {Az[1] * exn_p[2] - Az[2] * exn_p[1],
                              Az[2] * exn_p[0] - Az[0] * exn_p[2],
                              Az[0] * exn_p[1] - Az[1] * exn_p[0]};
-            if (store) {
-                const unsigned i = rbase(e - 1) + (unsigned)xc;
+            {
+                const unsigned off = store ? (rbase(e - 1) + (unsigned)xc) * (unsigned)sizeof(T) : kNoStore;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     T dEx = (exn[c] - exn_p[c]) / dz;
@@ -1243,10 +1255,10 @@ __global__ __launch_bounds__(256) void k_maxwell_zm(State<T> S, State<T> O, KPar
                     T exo = exn_p[c], ezo = ezn[c], azo = Az[c];
                     if (c > 0 && act == 1) { byn *= dec; exo *= dec; ezo *= dec; azo *= dec; }
                     if (c > 0 && act == 2) { byn = T(0.0); exo = T(0.0); ezo = T(0.0); azo = T(0.0); }
-                    st32(O.a[S_BY1 + c], i, byn);
-                    st32(O.a[S_EX1 + c], i, exo);
-                    st32(O.a[S_EZ1 + c], i, ezo);
-                    if (c > 0 || !p.freeze_az1) st32(O.a[S_AZ1 + c], i, azo);
+                    buf_st(byn, rby[c], off);
+                    buf_st(exo, rex[c], off);
+                    buf_st(ezo, rez[c], off);
+                    if (c > 0 || !p.freeze_az1) buf_st(azo, raz[c], off);
                 }
             }
         }
