@@ -100,16 +100,18 @@ def kernel_instance(slot, t):
         v = t["xz_variant"]
         name, tz = {0: ("k_fct_xz", 32), 1: ("k_fct_xz2", 16), 2: ("k_fct_xz2", 32), 3: ("k_fct_xz2", 64),
                     4: ("k_fct_xzr", 32), 5: ("k_fct_xzr", 64), 6: ("k_fct_xzr", 128), 7: ("k_fct_xzg", 16),
-                    8: ("k_fct_xzg", 32), 9: ("k_fct_xzg", 64), 10: ("k_fct_xzg", 32)}[v]
+                    8: ("k_fct_xzg", 32), 9: ("k_fct_xzg", 64), 10: ("k_fct_xzg", 32)}.get(v, (None, 0))
+        if name is None:
+            return None
         if name == "k_fct_xzg":
             return f"{name}<double, {tz}, {128 if v == 10 else 256}>"
         return f"{name}<double, {tz}>"
     if slot == "k_maxwell" and t["maxwell"] >= 4:
-        tz, pf = {4: (16, 1), 5: (32, 1), 6: (16, 0), 7: (8, 1), 8: (8, 0), 9: (4, 1)}[t["maxwell"]]
-        return f"k_maxwell_zm<double, {tz}, {pf}>"
+        tzpf = {4: (16, 1), 5: (32, 1), 6: (16, 0), 7: (8, 1), 8: (8, 0), 9: (4, 1)}.get(t["maxwell"])
+        return f"k_maxwell_zm<double, {tzpf[0]}, {tzpf[1]}>" if tzpf else None
     if slot == "k_maxwell":
         return {0: "k_maxwell<double, 16>", 1: "k_maxwell_cell<double>", 2: "k_maxwell<double, 4>",
-                3: "k_maxwell<double, 8>"}[t["maxwell"]]
+                3: "k_maxwell<double, 8>"}.get(t["maxwell"])
     return None
 
 
