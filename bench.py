@@ -257,6 +257,35 @@ def emhd_waves(n=1024, steps=20000, stride=20, kmax=32, timed_steps=2000):
             "relation": "omega^2 = 2 sum q^2 n + k_Yee^2 (both FCT sweeps apply the Lorentz impulse)"}
 
 
+def profile_leg(leg, steps, warmup, fused=-1):
+    """One short leg for a profiler: the kernels of the precision form (C3,
+    both precisions) or of the EMHD wave run with its spectrum (C5)."""
+    import khs
+
+    if leg == "c3":
+        out = {}
+        for prec in ("fp32", "fp64"):
+            r = run_gpu(prec, steps, warmup, fused, WORKLOAD_C3, SEGMENT_C3, kernel_steps=steps)
+            out[prec] = dict(ms_per_step=r["seconds"] / steps * 1e3, tiles=r["tuning"],
+                             kernels_avg_us={k: v[0] / v[1] * 1e3 for k, v in r["times"].items() if v[1]})
+        return {"leg": "c3", "workload": "4096x2048 keep |kz|<=8 (MFMA filter)", "steps": steps, **out}
+    from khs.dispersion import record_spectrum
+
+    keys = dict(model="emhd", nx_override=1024, nz_override=1024, periodic=1, species=2, emhd_init="spectrum",
+                spectrum_kmax=32, spectrum_amp=1e-5, spectrum_seed=2, smooth_init=0, precision="fp32",
+                n_steps=steps)
+    sim = khs.Simulation(khs.Config(**keys))
+    sim.init()
+    sim.step(warmup)
+    sim.sync()
+    t0 = time.perf_counter()
+    record_spectrum(sim, "By", 32, steps, 20)
+    el = time.perf_counter() - t0
+    sim.close()
+    return {"leg": "c5", "workload": "EMHD two-fluid 1024x1024 FP32, By spectrum kx,kz=1..32 every 20 steps",
+            "steps": steps, "seconds": el}
+
+
 def cpu_baseline(budget_s=15.0):
     """FP64 OpenMP oracle on the host cores, same workload, bounded sample."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
@@ -326,6 +355,10 @@ def main():
     ap.add_argument("--no-c5", action="store_true", help="skip the EMHD wave-dispersion leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB")
+    ap.add_argument("--profile-leg", choices=["c3", "c5"], default=None,
+                    help="run one short leg for a profiler and print its kernel times: c3 = the 4096x2048 "
+                         "precision form (FP32 and FP64, MFMA keep filter), c5 = the EMHD 1024^2 run with "
+                         "its MFMA space-time spectrum every 20 steps")
     ap.add_argument("--split", action="store_true", help="force the separate x and z sweep kernels")
     ap.add_argument("--fused", action="store_true", help="force the fused x+z sweep kernel")
     a = ap.parse_args()
@@ -338,6 +371,9 @@ def main():
         sys.exit(2)
 
     fused = 1 if a.fused else (0 if a.split else -1)     # -1: the autotuner's choice
+    if a.profile_leg:
+        print(json.dumps(profile_leg(a.profile_leg, a.steps, a.warmup, fused)))
+        return
     r64 = run_gpu("fp64", a.steps, a.warmup, fused)
     ms_step = r64["seconds"] / a.steps * 1e3
     value = r64["cells"] * a.steps / r64["seconds"]

@@ -99,7 +99,7 @@ struct Ctx {
     void* d_keep_tab = nullptr;      // MFMA keep filter table (nz x 16, run precision)
     // mode spectrum (allocated on first use)
     char* spec_mem = nullptr;
-    void *spec_zt = nullptr, *spec_xt = nullptr, *spec_C = nullptr, *spec_D = nullptr;
+    void *spec_zt = nullptr, *spec_xt = nullptr, *spec_C = nullptr, *spec_D = nullptr, *spec_P = nullptr;
     int spec_K = 0;
 
     // reductions / derived
@@ -1246,13 +1246,16 @@ int khs_mode_spectrum(khs_ctx* ctx, int id, int kmax, double* out) {
             throw Error(KHS_ERR_ARG, "FP32 spectrum needs nx % 4 == 0 and nz % 16 == 0");
         const size_t e = c.esz;
         if (!c.spec_mem) {
-            const size_t zt = (size_t)c.nz * 64 * e, xt = (size_t)c.nx * 64 * e, cb = (size_t)64 * c.nx * e;
+            const size_t zs = c.fp32 ? (size_t)khs::spectrum_z_slices(c.nx, c.nz) : 1;
+            const size_t zt = (size_t)c.nz * 64 * e, xt = (size_t)c.nx * 64 * e, cb = zs * 64 * c.nx * e;
+            const size_t pb = (size_t)khs::spectrum_x_blocks(c.nx) * 64 * 64 * 4;   // FP32 x partials
             auto ru = [](size_t v) { return (v + 255) / 256 * 256; };
-            HIPCHK(hipMalloc((void**)&c.spec_mem, ru(zt) + ru(xt) + ru(cb) + ru(64 * 64 * e)));
+            HIPCHK(hipMalloc((void**)&c.spec_mem, ru(zt) + ru(xt) + ru(cb) + ru(64 * 64 * e) + ru(pb)));
             c.spec_zt = c.spec_mem;
             c.spec_xt = c.spec_mem + ru(zt);
             c.spec_C = c.spec_mem + ru(zt) + ru(xt);
             c.spec_D = c.spec_mem + ru(zt) + ru(xt) + ru(cb);
+            c.spec_P = c.spec_mem + ru(zt) + ru(xt) + ru(cb) + ru(64 * 64 * e);
             c.spec_K = 0;
         }
         if (c.spec_K != kmax) {
@@ -1281,8 +1284,8 @@ int khs_mode_spectrum(khs_ctx* ctx, int id, int kmax, double* out) {
         std::vector<double> D(64 * 64);
         if (c.fp32) {
             khs::launch_spectrum_mfma((const float*)c.st[c.cur][slot], (const float*)c.spec_zt,
-                                      (const float*)c.spec_xt, (float*)c.spec_C, (float*)c.spec_D, c.nx, c.nz,
-                                      c.stream);
+                                      (const float*)c.spec_xt, (float*)c.spec_C, (float*)c.spec_P,
+                                      (float*)c.spec_D, c.nx, c.nz, c.stream);
             c.check_launch();
             std::vector<float> Df(64 * 64);
             HIPCHK(hipMemcpyAsync(Df.data(), c.spec_D, Df.size() * 4, hipMemcpyDeviceToHost, c.stream));

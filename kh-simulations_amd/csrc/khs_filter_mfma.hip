@@ -69,10 +69,31 @@ __global__ __launch_bounds__(256) void k_keep_coef_mfma(ArrayList<T> arrs, const
     v4 msum = zero;
     const int zper = nz >> 2;                      // nz % 16 == 0
     const int zb = wave * zper, ze = zb + zper;
-    for (int z0 = zb; z0 < ze; z0 += 4) {
+    // four 4-row steps per iteration with their loads issued first (more
+    // loads in flight per wave); the MFMA and sum order is the one-step order
+    int z0 = zb;
+    for (; z0 + 16 <= ze; z0 += 16) {
+        v4 b[4];
+        T at[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int z = z0 + 4 * u + kq;
+            b[u] = ok ? *reinterpret_cast<const v4*>(v + (size_t)z * nx + col) : zero;
+            at[u] = tab[(size_t)z * KF_ROWS + n];              // A[i = n][k = kq]
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            acc[0] = mfma16(at[u], b[u].x, acc[0]);
+            acc[1] = mfma16(at[u], b[u].y, acc[1]);
+            acc[2] = mfma16(at[u], b[u].z, acc[2]);
+            acc[3] = mfma16(at[u], b[u].w, acc[3]);
+            msum += b[u];
+        }
+    }
+    for (; z0 < ze; z0 += 4) {
         const int z = z0 + kq;
         const v4 b = ok ? *reinterpret_cast<const v4*>(v + (size_t)z * nx + col) : zero;
-        const T at = tab[(size_t)z * KF_ROWS + n];           // A[i = n][k = kq]
+        const T at = tab[(size_t)z * KF_ROWS + n];
         acc[0] = mfma16(at, b.x, acc[0]);
         acc[1] = mfma16(at, b.y, acc[1]);
         acc[2] = mfma16(at, b.z, acc[2]);
@@ -200,7 +221,9 @@ namespace khs {
 // --------------------------------------------------------------------------
 constexpr int SP_ROWS = 64;      // 2K <= 64 (K <= 32)
 
-// z contraction: one 64-column strip per block, the 4 waves split z.
+// z contraction: one 64-column strip and one of gridDim.y z slices per
+// block, the 4 waves split the slice; slice y writes its partial rows to
+// C + y * 64 * nx (the x contraction adds the slices in order).
 __global__ __launch_bounds__(256) void k_spec_z_mfma(const float* __restrict__ f, const float* __restrict__ tab,
                                                      float* __restrict__ C, int nx, int nz) {
     __shared__ float red[SP_ROWS][64];
@@ -215,8 +238,9 @@ __global__ __launch_bounds__(256) void k_spec_z_mfma(const float* __restrict__ f
     for (int t = 0; t < 4; ++t)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[t][j] = zero;
-    const int zper = nz >> 2;
-    const int zb = wave * zper, ze = zb + zper;
+    const int zslice = nz / (int)gridDim.y;         // nz % (16 * gridDim.y) == 0
+    const int zper = zslice >> 2;
+    const int zb = (int)blockIdx.y * zslice + wave * zper, ze = zb + zper;
     for (int z0 = zb; z0 < ze; z0 += 4) {
         const int z = z0 + kq;
         const f4 b = ok ? *reinterpret_cast<const f4*>(f + (size_t)z * nx + col) : zero;
@@ -244,33 +268,52 @@ __global__ __launch_bounds__(256) void k_spec_z_mfma(const float* __restrict__ f
         }
         __syncthreads();
     }
+    float* __restrict__ Cs = C + (size_t)blockIdx.y * SP_ROWS * nx;
     for (int o = threadIdx.x; o < SP_ROWS * 64; o += 256) {
         const int r = o >> 6, c = o & 63;
-        if (x0 + c < nx) C[(size_t)r * nx + x0 + c] = red[r][c];
+        if (x0 + c < nx) Cs[(size_t)r * nx + x0 + c] = red[r][c];
     }
 }
 
 // x contraction: D[64][64] = C[64][nx] U[nx][64], one block; wave w owns
 // output row tile w and all four column tiles.
+// x contraction, split over x: block b sums the 64 columns 64 b .. 64 b + 63
+// into its own 64 x 64 partial product (4 waves x 16 rows r, 4 tiles of 16
+// columns s); k_spec_x_sum adds the partials in block order.
 __global__ __launch_bounds__(256) void k_spec_x_mfma(const float* __restrict__ C, const float* __restrict__ utab,
-                                                     float* __restrict__ D, int nx) {
+                                                     float* __restrict__ P, int nx) {
     const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int n = l & 15, kq = l >> 4;
     const f4 zero = f4{0.f, 0.f, 0.f, 0.f};
     f4 acc[4] = {zero, zero, zero, zero};
-    for (int x0 = 0; x0 < nx; x0 += 4) {
+    const int xb = blockIdx.x * 64;
+#pragma unroll 4
+    for (int x0 = xb; x0 < xb + 64; x0 += 4) {
         const int x = x0 + kq;
-        const float a = C[(size_t)(16 * wave + n) * nx + x];          // A[i = n][k = kq]
+        const bool ok = x < nx;
+        const float a = ok ? C[(size_t)(16 * wave + n) * nx + x] : 0.f;   // A[i = n][k = kq]
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const float b = utab[(size_t)x * SP_ROWS + 16 * j + n];   // B[k = kq][j' = n]
+            const float b = ok ? utab[(size_t)x * SP_ROWS + 16 * j + n] : 0.f;   // B[k = kq][j' = n]
             acc[j] = mfma16(a, b, acc[j]);
         }
     }
+    float* __restrict__ out = P + (size_t)blockIdx.x * SP_ROWS * SP_ROWS;
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) D[(size_t)(16 * wave + drow(0.f, kq, e)) * SP_ROWS + 16 * j + n] = acc[j][e];
+        for (int e = 0; e < 4; ++e) out[(size_t)(16 * wave + drow(0.f, kq, e)) * SP_ROWS + 16 * j + n] = acc[j][e];
+}
+
+// out[o] = sum over k < nparts of in[k * stride + o], in order of k (one
+// output per thread): the z slices of C into slice 0, the x partials into D
+__global__ __launch_bounds__(256) void k_spec_sum(const float* in, float* out, int n,
+                                                  size_t stride, int nparts) {
+    const int o = blockIdx.x * 256 + threadIdx.x;
+    if (o >= n) return;
+    float s = 0.f;
+    for (int k = 0; k < nparts; ++k) s += in[(size_t)k * stride + o];
+    out[o] = s;
 }
 
 bool spectrum_mfma_supported(int nx, int nz, int K) {
@@ -290,10 +333,25 @@ void fill_spectrum_table(int n, int K, float* tab) {
         }
 }
 
-void launch_spectrum_mfma(const float* f, const float* ztab, const float* xtab, float* C, float* D, int nx, int nz,
-                          hipStream_t st) {
-    hipLaunchKernelGGL(k_spec_z_mfma, dim3((nx + 63) / 64), dim3(256), 0, st, f, ztab, C, nx, nz);
-    hipLaunchKernelGGL(k_spec_x_mfma, dim3(1), dim3(256), 0, st, C, xtab, D, nx);
+int spectrum_x_blocks(int nx) { return (nx + 63) / 64; }
+
+// z slices of the z contraction: about 256 blocks in all, each wave at
+// least 4 rows (nz % (16 * slices) == 0)
+int spectrum_z_slices(int nx, int nz) {
+    int zs = 1;
+    while (zs < 16 && spectrum_x_blocks(nx) * zs < 256 && nz % (16 * 2 * zs) == 0) zs *= 2;
+    return zs;
+}
+
+void launch_spectrum_mfma(const float* f, const float* ztab, const float* xtab, float* C, float* P, float* D, int nx,
+                          int nz, hipStream_t st) {
+    const int nb = spectrum_x_blocks(nx), zs = spectrum_z_slices(nx, nz);
+    const int nc = SP_ROWS * nx, nd = SP_ROWS * SP_ROWS;
+    hipLaunchKernelGGL(k_spec_z_mfma, dim3(nb, zs), dim3(256), 0, st, f, ztab, C, nx, nz);
+    if (zs > 1)   // slices 1.. added onto slice 0 in place: each output reads only its own column of parts
+        hipLaunchKernelGGL(k_spec_sum, dim3((nc + 255) / 256), dim3(256), 0, st, C, C, nc, (size_t)nc, zs);
+    hipLaunchKernelGGL(k_spec_x_mfma, dim3(nb), dim3(256), 0, st, C, xtab, P, nx);
+    hipLaunchKernelGGL(k_spec_sum, dim3((nd + 255) / 256), dim3(256), 0, st, P, D, nd, (size_t)nd, nb);
 }
 
 }  // namespace khs

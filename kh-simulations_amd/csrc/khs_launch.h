@@ -84,8 +84,13 @@ void launch_keep_mfma(const ArrayList<T>& arrs, const T* tab, T* coef, const KPa
 // (cos for r < K, sin for K <= r < 2K), C scratch 64 x nx, D output 64 x 64.
 bool spectrum_mfma_supported(int nx, int nz, int K);
 void fill_spectrum_table(int n, int K, float* tab);
-void launch_spectrum_mfma(const float* f, const float* ztab, const float* xtab, float* C, float* D, int nx, int nz,
-                          hipStream_t st);
+// scratch of the FP32 spectrum: C holds spectrum_z_slices(nx, nz) x 64 x nx
+// partial z contractions, P spectrum_x_blocks(nx) x 64 x 64 partial x
+// contractions
+int spectrum_x_blocks(int nx);
+int spectrum_z_slices(int nx, int nz);
+void launch_spectrum_mfma(const float* f, const float* ztab, const float* xtab, float* C, float* P, float* D, int nx,
+                          int nz, hipStream_t st);
 void launch_spectrum_f64(const double* f, const double* ztab, const double* xtab, double* C, double* D, int nx,
                          int nz, int rows, hipStream_t st);
 template <typename T>
