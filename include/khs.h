@@ -109,9 +109,9 @@ typedef struct khs_config {
     int      precision;        /* KHS_FP64 or KHS_FP32 */
     int      energy_stride_override;
     int      export_stride_override;  /* <0 disables snapshots */
-    int      fused_sweep;      /* YM: 1 = khs_step runs the fused x+z sweep kernel, 0 = the split x / z
-                                  sweeps, -1 (default) = the autotuner's choice (fused without tuning;
-                                  split when a split-sweep tile shape is forced) */
+    int      fused_sweep;      /* 1 = khs_step runs the fused x+z sweep kernel (YM and EMHD), 0 = the
+                                  split x / z sweeps, -1 (default) = the autotuner's choice (fused without
+                                  tuning; split when a split-sweep tile shape is forced) */
     int      autotune;         /* 1 (default): time the tile variants on the first step, keep the fastest */
     int      x_segment;        /* 0 = tuned, or force 64 / 128 / 256 cells per x-sweep segment */
     int      z_chunk;          /* 0 = tuned, or force 16 / 32 / 64 rows per z-sweep chunk */
@@ -129,7 +129,7 @@ typedef struct khs_config {
     uint64_t spectrum_seed;
     double   em_perturb_amp;   /* EMHD das_kaw perturbation (argv[2] as double, default 0.05); set by perturb_amp */
     int      z_groups;         /* z sweep: -1 = autotuned (default), 0 = all variables per lane, 1 = two variable groups on two waves */
-    int      xz_variant;       /* fused sweep kernel variant: -1 = autotuned (default), 0..10 */
+    int      xz_variant;       /* fused sweep kernel variant: -1 = autotuned (default), YM 0..10, EMHD 0..2 */
     int      x_serial;         /* x sweep phases per variable: -1 = autotuned (default), 0 = all at once, 1 = one at a time */
     int      maxwell_kernel;   /* YM Maxwell: -1 = autotuned (default), 0/2/3 = 64x16/64x4/64x8 LDS tiles, 1 = per cell,
                                   4..9 = z march with 16/32/16/8/8/4-row chunks (6, 8: without next-row prefetch) */

@@ -276,9 +276,9 @@ struct Ctx {
         // untuned default and the autotuner's candidate against the split
         // sweeps, unless a split-sweep shape is forced
         const bool split_keys = lds_keys || fl >= 0 || fz >= 0 || fp >= 0 || fg >= 0 || cfg.zm_fused > 0;
-        const bool try_fused = !emhd && cfg.fused_sweep < 0 && !split_keys;
-        fused = !emhd && (cfg.fused_sweep > 0 || try_fused);
-        xzvar = (cfg.xz_variant >= 0) ? cfg.xz_variant : 7;
+        const bool try_fused = cfg.fused_sweep < 0 && !split_keys;
+        fused = cfg.fused_sweep > 0 || try_fused;
+        xzvar = (cfg.xz_variant >= 0) ? cfg.xz_variant : (emhd ? 1 : 7);
         if (!cfg.autotune) return;
         // contexts of the same shape in one process (sweeps, campaigns) reuse
         // the first one's choice: every variant computes the same bits
@@ -306,11 +306,10 @@ struct Ctx {
         if (zc.size() > 1 || try_fused) zvar = pick<T>(zc, [&](int v) { fct_z<T>(0, nsp, v); }, &tz);
         if (fused) {
             std::vector<int> xzc;
-            for (int v = 0; v < kNumXZVariants; ++v)
+            for (int v = 0; v < (emhd ? kNumXZEmVariants : kNumXZVariants); ++v)
                 if (cfg.xz_variant < 0 || v == cfg.xz_variant) xzc.push_back(v);
             if (xzc.size() > 1 || try_fused)
-                xzvar = pick<T>(xzc, [&](int v) { launch_fct_xz<T>(state<T>(cur), state<T>(1 - cur), kp, v, stream); },
-                                &txz);
+                xzvar = pick<T>(xzc, [&](int v) { fct_xz<T>(v); }, &txz);
             if (try_fused) fused = txz < tx + tz;
         }
         zmvar = (!emhd && cfg.zm_fused > 0) ? cfg.zm_fused : 0;
@@ -343,8 +342,13 @@ struct Ctx {
         if (tuned) return;
         fp32 ? do_autotune<float>() : do_autotune<double>();
     }
+    // fused x+z sweep of every species: current state -> next state
+    template <typename T> void fct_xz(int v) {
+        if (emhd) launch_fct_xz_em<T>(state<T>(cur), state<T>(1 - cur), kp, nsp, v, stream);
+        else launch_fct_xz<T>(state<T>(cur), state<T>(1 - cur), kp, v, stream);
+    }
     template <typename T> void do_xz() {
-        timed(K_FCT_XZ, [&] { launch_fct_xz<T>(state<T>(cur), state<T>(1 - cur), kp, xzvar, stream); });
+        timed(K_FCT_XZ, [&] { fct_xz<T>(xzvar); });
         check_launch();
     }
     // z sweep of both species then Maxwell, as the split kernels (v = 0) or

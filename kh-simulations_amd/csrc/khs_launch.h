@@ -36,6 +36,10 @@ void launch_derived_em(const State<T>& S, int which, const KParams& p, double* o
 // fused x + z sweep of both species: state S -> fluid of O (variants: see
 // launch_fct_xz in khs_step_kernels.hip)
 constexpr int kNumXZVariants = 11;
+// EMHD fused x+z sweep of species 0..nsp-1: variants 0/1/2 (16/32/64-row chunks)
+constexpr int kNumXZEmVariants = 3;
+template <typename T>
+void launch_fct_xz_em(const State<T>& S, const State<T>& O, const KParams& p, int nsp, int variant, hipStream_t st);
 template <typename T>
 void launch_fct_xz(const State<T>& S, const State<T>& O, const KParams& p, int variant, hipStream_t st);
 // decay/action: per-column sponge / xi_cut (nullptr when neither is active);

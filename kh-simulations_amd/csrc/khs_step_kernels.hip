@@ -2296,6 +2296,188 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_
     }
 }
 
+// --------------------------------------------------------------------------
+// Fused x + z sweep of the EMHD model (n, px, py, pz of one species per
+// wavefront), the same register march as xz_march_g: lane l owns column
+// 58 g - 3 + l, the x sweep of every row takes its neighbours by DPP and feeds
+// the z pipeline.  Both sweeps use the Lorentz force of the step-start state
+// (main_coupled.cu:112-136) on px, py, pz, so it is evaluated once per cell
+// from the carried field rows.  EDGE wavefronts (some column outside the
+// grid) load the x-1 fields through xm_src of the lane's real column, which
+// makes the row form equal to em_lorentz at that column for every lane;
+// interior wavefronts take Ex, Bz, By at x-1 from lane l-1.
+// --------------------------------------------------------------------------
+template <typename T, int TZ, bool EDGE>
+__device__ __forceinline__ void xz_march_em(const State<T>& S, const State<T>& O, const KParams& p, int sp, int g,
+                                            int ch, int lane) {
+    constexpr int NV = EV_COUNT;
+    typedef EmRow<T> Row;
+    const int nx = p.nx, nz = p.nz;
+    const int x = g * XZR_OUT - 3 + lane;
+    const int xc = map_x(x, p);
+    const bool inside = !EDGE || (x >= 0 && x < nx);
+    const bool store = lane >= 3 && lane < 3 + XZR_OUT && inside;
+    const int xm = xm_src(xc, p);
+    const bool wall = !p.periodic_x && (xc == 0 || xc == nx - 1);
+    const int b = em_base(sp);
+    const T q = T(sp ? p.q1 : p.q0);
+    const T dt = T(p.dt), dtdx = T(p.dtdx), dtdz = T(p.dtdz);
+    const int z0 = ch * TZ;
+    const int zend = min(z0 + TZ, nz);
+    const int elast = zend + 2;
+    const unsigned bytes = (unsigned)nx * (unsigned)nz * (unsigned)sizeof(T);
+    __amdgpu_buffer_rsrc_t orsc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) orsc[v] = buf_rsrc(O.a[b + v], bytes);
+
+    auto load = [&](int r, Row& row, T (&F)[NV]) {
+        const unsigned i = (unsigned)xc + (unsigned)r * (unsigned)nx;
+        const unsigned ixm = (unsigned)xm + (unsigned)r * (unsigned)nx;
+        row.ex = ld32(S.a[E_EX], i);
+        row.ey = ld32(S.a[E_EY], i);
+        row.ez = ld32(S.a[E_EZ], i);
+        row.bx = ld32(S.a[E_BX], i);
+        row.bz = ld32(S.a[E_BZ], i);
+        row.by = ld32(S.a[E_BY], i);
+        if (EDGE) {
+            row.exm = ld32(S.a[E_EX], ixm);
+            row.bzxm = ld32(S.a[E_BZ], ixm);
+            row.byxm = ld32(S.a[E_BY], ixm);
+        } else {                               // lane 0 is a halo lane: its value is unused
+            row.exm = lane_from_left(row.ex);
+            row.bzxm = lane_from_left(row.bz);
+            row.byxm = lane_from_left(row.by);
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) F[v] = ld32(S.a[b + v], i);
+    };
+
+    int rc = clampi(z0 - 3, 0, nz - 1);
+    Row prev, cur;
+    T F[NV];
+    {
+        T Fd[NV];
+        load(zm_per(rc, nz), prev, Fd);
+    }
+    load(rc, cur, F);
+    ZPipe<T, NV> zp;
+    zp.Um = T(0);
+    zp.MUm = T(0);
+
+    auto row = [&](int e, auto mode) {
+        constexpr int MODE = decltype(mode)::value;
+        const int rn = MODE ? rc + 1 : clampi(min(e + 1, elast), 0, nz - 1);
+        Row nrow;
+        T nF[NV];
+        load(rn, nrow, nF);
+
+        // Lorentz force of the cell (both sweeps' source on px, py, pz)
+        T f[3];
+        em_lorentz_v(cur, prev, F[EV_N], F[EV_PX], F[EV_PY], F[EV_PZ], q, f);
+
+        // ---- x sweep of row rc at this lane's column (FCT_Sweeps.cu:184-197) ----
+        T V[NV];
+        {
+            const T u = F[EV_PX] / kmax(F[EV_N], T(0.05));        // update_velocities
+            const T ur = lane_from_right(u);
+            T vi = T(0.5) * (u + ur);
+            const T eps = vi * dtdx;
+            T nu, mu;
+            fct_nu_mu(eps, nu, mu);
+#pragma unroll
+            for (int k = 0; k < NV; ++k) {
+                const T v0 = F[k];
+                const T vp = lane_from_right(v0);
+                T fR = T(0.5) * eps * (v0 + vp);
+                T dR = nu * (vp - v0);
+                const T fL = lane_from_left(fR);                    // the left lane's right fluxes
+                const T dL = lane_from_left(dR);
+                T sk = (k == EV_N) ? T(0.0) : f[k == EV_N ? 0 : k - 1];
+                T lor = v0 + dt * sk - (fR - fL);
+                T lnr = lor + (dR - dL);
+                const T lorr = lane_from_right(lor);
+                const T lnrl = lane_from_left(lnr);
+                const T lnrr = lane_from_right(lnr);
+                const T lnrr2 = lane_from_right(lnrr);
+                T flx = mu * (lorr - lor);
+                T fl = fct_limit(flx, lnr - lnrl, lnrr - lnr, lnrr2 - lnrr);
+                const T fll = lane_from_left(fl);
+                V[k] = lnr - (fl - fll);
+            }
+        }
+
+        // ---- z sweep (FCT_Sweeps.cu:199-232) of the x-swept row ----
+        const T uz = F[EV_PZ] / kmax(F[EV_N], T(0.05));
+        const T Sv[NV] = {T(0), f[0], f[1], f[2]};
+        T out[NV];
+        bool have = true;
+        if (MODE) zp.push_steady(V, uz, Sv, dt, dtdz, out);
+        else have = zp.push(e, z0, V, uz, Sv, dt, dtdz, out);
+        if (have) {
+            if (out[0] < T(0.05)) out[0] = T(0.05);          // enforce_boundaries (fluid)
+            if (wall) out[EV_PX] = T(0.0);
+            const unsigned off =
+                store ? ((unsigned)xc + (unsigned)(e - 3) * (unsigned)nx) * (unsigned)sizeof(T) : kNoStore;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) buf_st_nt(out[v], orsc[v], off);
+        }
+        if (MODE) {
+            prev = cur;
+        } else if (rn != rc) {
+            prev = cur;
+        }
+        cur = nrow;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) F[v] = nF[v];
+        rc = rn;
+    };
+
+    int e = z0 - 3;
+    for (; e < z0 + 3; ++e) row(e, std::integral_constant<int, 0>());
+    const int esteady = min(elast - 1, nz - 2);
+    for (; e + 2 <= esteady; e += 3) {
+        row(e, std::integral_constant<int, 1>());
+        row(e + 1, std::integral_constant<int, 1>());
+        row(e + 2, std::integral_constant<int, 1>());
+    }
+    for (; e <= esteady; ++e) row(e, std::integral_constant<int, 1>());
+    for (; e <= elast; ++e) row(e, std::integral_constant<int, 0>());
+}
+
+// one wavefront per (species, 58-column tile, z chunk), 4 per workgroup
+template <typename T, int TZ>
+__global__ __launch_bounds__(256) void k_fct_xze(State<T> S, State<T> O, KParams p, int nsp, int ngx, int nch) {
+    const int lane = threadIdx.x & 63;
+    int wl = __builtin_amdgcn_readfirstlane(xcd_logical(blockIdx.x, gridDim.x) * 4 + (int)(threadIdx.x >> 6));
+    if (wl >= nsp * ngx * nch) return;        // whole wavefronts leave together
+    wl = tile_order(wl, nsp * ngx * nch, p.rev);
+    const int sp = wl % nsp;
+    const int g = (wl / nsp) % ngx;
+    const int ch = wl / (nsp * ngx);
+    if (g == 0 || g * XZR_OUT + 60 >= p.nx) xz_march_em<T, TZ, true>(S, O, p, sp, g, ch, lane);
+    else xz_march_em<T, TZ, false>(S, O, p, sp, g, ch, lane);
+}
+
+template <typename T, int TZ>
+static void launch_fct_xze_t(const State<T>& S, const State<T>& O, const KParams& p, int nsp, hipStream_t st) {
+    const int ngx = (p.nx + XZR_OUT - 1) / XZR_OUT;
+    const int nch = (p.nz + TZ - 1) / TZ;
+    const int waves = nsp * ngx * nch;
+    hipLaunchKernelGGL((k_fct_xze<T, TZ>), dim3(round8((waves + 3) / 4)), dim3(256), 0, st, S, O, p, nsp, ngx, nch);
+}
+
+// EMHD fused sweep of species 0..nsp-1, variants 0/1/2: 16/32/64-row chunks
+template <typename T>
+void launch_fct_xz_em(const State<T>& S, const State<T>& O, const KParams& p, int nsp, int variant, hipStream_t st) {
+    if (variant == 0) launch_fct_xze_t<T, 16>(S, O, p, nsp, st);
+    else if (variant == 2) launch_fct_xze_t<T, 64>(S, O, p, nsp, st);
+    else launch_fct_xze_t<T, 32>(S, O, p, nsp, st);
+}
+template void launch_fct_xz_em<double>(const State<double>&, const State<double>&, const KParams&, int, int,
+                                       hipStream_t);
+template void launch_fct_xz_em<float>(const State<float>&, const State<float>&, const KParams&, int, int,
+                                      hipStream_t);
+
 template <typename T, int TZ, int BS = 256>
 static void launch_fct_xzg_t(const State<T>& S, const State<T>& O, const KParams& p, hipStream_t st) {
     constexpr int TPB = BS / 128;

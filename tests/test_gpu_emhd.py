@@ -137,6 +137,25 @@ def test_tile_variants_bitwise(zc, pf, zg):
         assert_bitwise(sim.get(n), o.get(n), f"z{zc} pf{pf} {n}")
 
 
+@pytest.mark.parametrize("xzv", [0, 1, 2])
+@pytest.mark.parametrize("shape,periodic,nsp", [((130, 70), 0, 2), ((200, 96), 1, 1), ((64, 40), 1, 2),
+                                                ((116, 150), 0, 1)])
+def test_fused_sweep_variants_bitwise(xzv, shape, periodic, nsp):
+    """Fused x+z sweep (register x sweep in the z march, Lorentz force once
+    per cell): walls and periodic x, one and two species, one and several
+    column tiles and z chunks, give the oracle's bits."""
+    nx, nz = shape
+    keys = em_keys(nx, nz, periodic, nsp, fused_sweep=1, xz_variant=xzv, autotune=0)
+    cfg, sim, o = em_pair(em_random(nx, nz, nsp, seed=nx + xzv, smooth=True), **keys)
+    o.lorentz()
+    sim.step(5)
+    o.step(5)
+    t = sim.tuning()
+    assert (t["fused_sweep"], t["xz_variant"]) == (1, xzv)
+    for n in orc.em_state_names(nsp):
+        assert_bitwise(sim.get(n), o.get(n), f"xz{xzv} {shape} {n}")
+
+
 def test_daskaw_steps_bitwise():
     """The reference run itself (256x256, dx = 1, dt = 0.002), 20 steps from init."""
     cfg = khs.Config(model="emhd", k_mode=1, n_steps=20)
