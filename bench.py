@@ -100,11 +100,12 @@ def kernel_instance(slot, t):
         v = t["xz_variant"]
         name, tz = {0: ("k_fct_xz", 32), 1: ("k_fct_xz2", 16), 2: ("k_fct_xz2", 32), 3: ("k_fct_xz2", 64),
                     4: ("k_fct_xzr", 32), 5: ("k_fct_xzr", 64), 6: ("k_fct_xzr", 128), 7: ("k_fct_xzg", 16),
-                    8: ("k_fct_xzg", 32), 9: ("k_fct_xzg", 64), 10: ("k_fct_xzg", 32)}.get(v, (None, 0))
+                    8: ("k_fct_xzg", 32), 9: ("k_fct_xzg", 64), 10: ("k_fct_xzg", 32),
+                    11: ("k_fct_xzg", 8)}.get(v, (None, 0))
         if name is None:
             return None
         if name == "k_fct_xzg":
-            return f"{name}<double, {tz}, {128 if v == 10 else 256}>"
+            return f"{name}<double, {tz}, {128 if v >= 10 else 256}>"
         return f"{name}<double, {tz}>"
     if slot == "k_maxwell" and t["maxwell"] >= 4:
         tzpf = {4: (16, 1), 5: (32, 1), 6: (16, 0), 7: (8, 1), 8: (8, 0), 9: (4, 1)}.get(t["maxwell"])
@@ -262,6 +263,17 @@ def profile_leg(leg, steps, warmup, fused=-1):
     both precisions) or of the EMHD wave run with its spectrum (C5)."""
     import khs
 
+    if leg in ("sweep", "sweep-graph"):
+        from khs.sweep import sweep_configs
+
+        b = khs.Batch(sweep_configs(64, steps), lanes=8, segment=50)
+        b.init()
+        t0 = time.perf_counter()
+        done, halted = b.run_concurrent(steps, threads=8) if leg == "sweep" else b.run(steps)
+        el = time.perf_counter() - t0
+        b.close()
+        return {"leg": leg, "runs": 64, "steps": steps, "seconds": el, "halted": int(sum(halted)),
+                "run_steps": int(sum(done))}
     if leg == "c3":
         out = {}
         for prec in ("fp32", "fp64"):
@@ -355,10 +367,11 @@ def main():
     ap.add_argument("--no-c5", action="store_true", help="skip the EMHD wave-dispersion leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB")
-    ap.add_argument("--profile-leg", choices=["c3", "c5"], default=None,
+    ap.add_argument("--profile-leg", choices=["c3", "c5", "sweep", "sweep-graph"], default=None,
                     help="run one short leg for a profiler and print its kernel times: c3 = the 4096x2048 "
                          "precision form (FP32 and FP64, MFMA keep filter), c5 = the EMHD 1024^2 run with "
-                         "its MFMA space-time spectrum every 20 steps")
+                         "its MFMA space-time spectrum every 20 steps, sweep / sweep-graph = the 64-run "
+                         "sweep in the concurrent / single-thread graph mode (--steps steps per run)")
     ap.add_argument("--split", action="store_true", help="force the separate x and z sweep kernels")
     ap.add_argument("--fused", action="store_true", help="force the fused x+z sweep kernel")
     a = ap.parse_args()

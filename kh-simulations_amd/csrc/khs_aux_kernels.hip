@@ -49,6 +49,66 @@ void launch_cell_filters(const ArrayList<T>& arrs, const double* decay, const in
 }
 
 // --------------------------------------------------------------------------
+// z-mean removal with the charge cut and the colour-1 zeroing in one pass
+// (the filter set of suppress_kz0 runs without bands, keep or hyperdiffusion:
+// main_ym.cu:365-447 in that case is sponge / xi_cut on Q2, Q3, z-mean of 15
+// arrays, colour-1 EM set to zero).  A workgroup owns a 64-column strip of
+// one array: arrays [0, nmean) lose their column mean (the last nact of them
+// first take the column action: 1 = times decay[x], 2 = zero; per column, so
+// it commutes with the column mean), arrays [nmean, n) are set to zero.  The
+// column sum is FP64 over 4 row groups added in group order; the mean is
+// sum * w (w = 1 / nz, as in the column DFT filter's mode-0 weight).
+// --------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_zmean_cut(ArrayList<T> arrs, int nmean, int nact, const double* decay,
+                                                   const int* action, double w, KParams p) {
+    __shared__ double red[4][64];
+    const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int x = blockIdx.x * 64 + lane;
+    const int k = blockIdx.y;
+    const int nx = p.nx, nz = p.nz;
+    const int rows = (nz + 3) / 4;
+    const int z0 = g * rows, z1 = min(nz, z0 + rows);
+    T* v = arrs.a[k];
+    if (k >= nmean) {
+        if (x < nx)
+            for (int z = z0; z < z1; ++z) v[(size_t)x + (size_t)z * nx] = T(0.0);
+        return;                                        // whole workgroups leave together
+    }
+    int act = 0;
+    T d = T(1.0);
+    if (x < nx && k >= nmean - nact && action) {
+        act = action[x];
+        if (act == 1) d = T(decay[x]);
+    }
+    auto val = [&](size_t i) {                         // the value after the column action
+        T a = v[i];
+        if (act == 1) a *= d;
+        else if (act == 2) a = T(0.0);
+        return a;
+    };
+    double s = 0.0;
+    if (x < nx)
+        for (int z = z0; z < z1; ++z) s += (double)val((size_t)x + (size_t)z * nx);
+    red[g][lane] = s;
+    __syncthreads();
+    const double mean = (((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane]) * w;
+    if (x < nx)
+        for (int z = z0; z < z1; ++z) {
+            const size_t i = (size_t)x + (size_t)z * nx;
+            v[i] = (T)((double)val(i) - mean);
+        }
+}
+
+template <typename T>
+void launch_zmean_cut(const ArrayList<T>& arrs, int nmean, int nact, const double* decay, const int* action,
+                      const KParams& p, hipStream_t st) {
+    if (arrs.n == 0) return;
+    hipLaunchKernelGGL(k_zmean_cut<T>, dim3((p.nx + 63) / 64, arrs.n), dim3(256), 0, st, arrs, nmean, nact, decay,
+                       action, 1.0 / (double)p.nz, p);
+}
+
+// --------------------------------------------------------------------------
 // column DFT filter
 // --------------------------------------------------------------------------
 constexpr int CZ = 64;   // z rows per apply block
@@ -577,6 +637,7 @@ void launch_spectrum_f64(const double* f, const double* ztab, const double* xtab
 // explicit instantiations
 #define KHS_INST(T)                                                                                   \
     template void launch_cell_filters<T>(const ArrayList<T>&, const double*, const int*, const KParams&, hipStream_t); \
+    template void launch_zmean_cut<T>(const ArrayList<T>&, int, int, const double*, const int*, const KParams&, hipStream_t); \
     template void launch_column_modes<T, float>(const ArrayList<T>&, const float*, const float*, const float*, int, int, double*, const KParams&, hipStream_t); \
     template void launch_column_modes<T, double>(const ArrayList<T>&, const double*, const double*, const double*, int, int, double*, const KParams&, hipStream_t); \
     template void launch_hyperdiffuse<T>(const ArrayList<T>&, double, const KParams&, hipStream_t); \

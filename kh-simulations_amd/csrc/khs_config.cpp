@@ -241,7 +241,7 @@ static void config_derive_emhd(khs_config& c) {
     c.energy_thresh = 5.0;
     if (c.nx < 4 || c.nz < 4) throw Error(KHS_ERR_CONFIG, "grid too small");
     check_grid_bytes(c.nx, c.nz);
-    if (c.xz_variant > 2) throw Error(KHS_ERR_CONFIG, "xz_variant must be -1..2 for the EMHD model");
+    if (c.xz_variant > 3) throw Error(KHS_ERR_CONFIG, "xz_variant must be -1..3 for the EMHD model");
     if (c.species != 1 && c.species != 2) throw Error(KHS_ERR_CONFIG, "species must be 1 or 2");
     if (c.x_segment != 0 && c.x_segment != 64 && c.x_segment != 128 && c.x_segment != 256)
         throw Error(KHS_ERR_CONFIG, "x_segment must be 0, 64, 128 or 256");
@@ -252,7 +252,7 @@ static void config_derive_emhd(khs_config& c) {
         throw Error(KHS_ERR_CONFIG, "x_rows must be 0, 1, 4 or 16");
     if (c.maxwell_kernel < -1 || c.maxwell_kernel > 9) throw Error(KHS_ERR_CONFIG, "maxwell_kernel must be -1..9");
     if (c.x_serial < -1 || c.x_serial > 1) throw Error(KHS_ERR_CONFIG, "x_serial must be -1, 0 or 1");
-    if (c.xz_variant < -1 || c.xz_variant > 10) throw Error(KHS_ERR_CONFIG, "xz_variant must be -1..10");
+    if (c.xz_variant < -1 || c.xz_variant > 11) throw Error(KHS_ERR_CONFIG, "xz_variant must be -1..11");
     if (c.fused_sweep < -1 || c.fused_sweep > 1) throw Error(KHS_ERR_CONFIG, "fused_sweep must be -1, 0 or 1");
     if (c.z_groups < -1 || c.z_groups > 1) throw Error(KHS_ERR_CONFIG, "z_groups must be -1, 0 or 1");
     if (c.x_lane_cells != -1 && c.x_lane_cells != 0 && c.x_lane_cells != 4 && c.x_lane_cells != 8)
@@ -320,7 +320,7 @@ void config_derive(khs_config& c) {
         throw Error(KHS_ERR_CONFIG, "x_rows must be 0, 1, 4 or 16");
     if (c.maxwell_kernel < -1 || c.maxwell_kernel > 9) throw Error(KHS_ERR_CONFIG, "maxwell_kernel must be -1..9");
     if (c.x_serial < -1 || c.x_serial > 1) throw Error(KHS_ERR_CONFIG, "x_serial must be -1, 0 or 1");
-    if (c.xz_variant < -1 || c.xz_variant > 10) throw Error(KHS_ERR_CONFIG, "xz_variant must be -1..10");
+    if (c.xz_variant < -1 || c.xz_variant > 11) throw Error(KHS_ERR_CONFIG, "xz_variant must be -1..11");
     if (c.fused_sweep < -1 || c.fused_sweep > 1) throw Error(KHS_ERR_CONFIG, "fused_sweep must be -1, 0 or 1");
     if (c.z_groups < -1 || c.z_groups > 1) throw Error(KHS_ERR_CONFIG, "z_groups must be -1, 0 or 1");
     if (c.x_lane_cells != -1 && c.x_lane_cells != 0 && c.x_lane_cells != 4 && c.x_lane_cells != 8)

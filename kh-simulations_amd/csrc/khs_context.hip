@@ -399,6 +399,24 @@ struct Ctx {
         bool any = cell_filters || c.suppress_kz0 || c.kz_suppress_max >= 1 ||
                    c.kz_suppress_hi > c.kz_suppress_max + 1 || c.kz_keep_max > 0 || c.hyp_diff > 0.0f;
         if (!any) return;
+        const bool bands = c.kz_suppress_max >= 1 || c.kz_suppress_hi > c.kz_suppress_max + 1;
+        if (c.suppress_kz0 && !bands && c.kz_keep_max == 0 && c.hyp_diff <= 0.0f) {
+            // z-mean only: charge cut, z-mean of the 12 colour-2/3 arrays
+            // (kFilt15 without colour 1, charges last) and colour-1 zeroing in
+            // one launch; the same filters as below, with the z-mean sums
+            // in another order
+            timed(K_FILTERS, [&] {
+                int ids[15];
+                int n = 0;
+                for (int id : kFilt15)
+                    if (id != S_BY1 && id != S_EX1 && id != S_EZ1) ids[n++] = id;
+                ids[n++] = S_BY1; ids[n++] = S_EX1; ids[n++] = S_EZ1;
+                launch_zmean_cut<T>(list<T>(k, ids, 15, false), 12, cell_filters ? 4 : 0, d_decay,
+                                    cell_filters ? d_action : nullptr, kp, stream);
+            });
+            check_launch();
+            return;
+        }
         timed(K_FILTERS, [&] {
             // colour-2/3 fields were damped / cut in k_maxwell; the charges here
             if (cell_filters) launch_cell_filters<T>(list<T>(k, kQ23, 4, false), d_decay, d_action, kp, stream);

@@ -35,9 +35,9 @@ template <typename T>
 void launch_derived_em(const State<T>& S, int which, const KParams& p, double* out, hipStream_t st);
 // fused x + z sweep of both species: state S -> fluid of O (variants: see
 // launch_fct_xz in khs_step_kernels.hip)
-constexpr int kNumXZVariants = 11;
-// EMHD fused x+z sweep of species 0..nsp-1: variants 0/1/2 (16/32/64-row chunks)
-constexpr int kNumXZEmVariants = 3;
+constexpr int kNumXZVariants = 12;
+// EMHD fused x+z sweep of species 0..nsp-1: variants 0/1/2/3 (16/32/64/8-row chunks)
+constexpr int kNumXZEmVariants = 4;
 template <typename T>
 void launch_fct_xz_em(const State<T>& S, const State<T>& O, const KParams& p, int nsp, int variant, hipStream_t st);
 template <typename T>
@@ -69,6 +69,12 @@ struct ArrayList {
 template <typename T>
 void launch_cell_filters(const ArrayList<T>& arrs, const double* decay, const int* action,
                          const KParams& p, hipStream_t st);
+// z-mean removal of arrays [0, nmean) (the last nact after the per-column
+// action of launch_cell_filters, when action is given) and zeroing of arrays
+// [nmean, n), one workgroup per 64-column strip and array.
+template <typename T>
+void launch_zmean_cut(const ArrayList<T>& arrs, int nmean, int nact, const double* decay, const int* action,
+                      const KParams& p, hipStream_t st);
 // Column DFT filter.  coef scratch: arrays * 2 * nmodes * nx doubles.
 //   project == 0: v -= sum_m (a_m c_m(z) + b_m s_m(z))   (reference band subtraction)
 //   project == 1: v  = sum_m (a_m c_m(z) + b_m s_m(z))   (keep-modes projection)

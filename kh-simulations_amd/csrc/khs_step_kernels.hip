@@ -2466,11 +2466,12 @@ static void launch_fct_xze_t(const State<T>& S, const State<T>& O, const KParams
     hipLaunchKernelGGL((k_fct_xze<T, TZ>), dim3(round8((waves + 3) / 4)), dim3(256), 0, st, S, O, p, nsp, ngx, nch);
 }
 
-// EMHD fused sweep of species 0..nsp-1, variants 0/1/2: 16/32/64-row chunks
+// EMHD fused sweep of species 0..nsp-1, variants 0/1/2/3: 16/32/64/8-row chunks
 template <typename T>
 void launch_fct_xz_em(const State<T>& S, const State<T>& O, const KParams& p, int nsp, int variant, hipStream_t st) {
     if (variant == 0) launch_fct_xze_t<T, 16>(S, O, p, nsp, st);
     else if (variant == 2) launch_fct_xze_t<T, 64>(S, O, p, nsp, st);
+    else if (variant == 3) launch_fct_xze_t<T, 8>(S, O, p, nsp, st);
     else launch_fct_xze_t<T, 32>(S, O, p, nsp, st);
 }
 template void launch_fct_xz_em<double>(const State<double>&, const State<double>&, const KParams&, int, int,
@@ -2507,8 +2508,8 @@ static void launch_fct_xz2_t(const State<T>& S, const State<T>& O, const KParams
 // 128 columns (variable groups split across waves), 16/32/64-row chunks;
 // 4/5/6: the x sweep in registers (DPP) inside the z march, 32/64/128-row
 // chunks; 7/8/9: the same with the variables on two waves, 16/32/64-row
-// chunks (two column tiles per workgroup), 10: 32-row chunks, one tile per
-// workgroup.
+// chunks (two column tiles per workgroup), 10 / 11: 32 / 8-row chunks, one
+// tile per workgroup (short chunks: lower latency on small grids).
 template <typename T>
 void launch_fct_xz(const State<T>& S, const State<T>& O, const KParams& p, int variant, hipStream_t st) {
     switch (variant) {
@@ -2522,6 +2523,7 @@ void launch_fct_xz(const State<T>& S, const State<T>& O, const KParams& p, int v
         case 8: launch_fct_xzg_t<T, 32>(S, O, p, st); return;
         case 9: launch_fct_xzg_t<T, 64>(S, O, p, st); return;
         case 10: launch_fct_xzg_t<T, 32, 128>(S, O, p, st); return;
+        case 11: launch_fct_xzg_t<T, 8, 128>(S, O, p, st); return;
         default: break;
     }
     constexpr int TZ = 32;

@@ -88,10 +88,10 @@ def test_output_dir_name_c1():
     (dict(x_lane_cells=6), "x_lane_cells"),
     (dict(z_groups=2), "z_groups"),
     (dict(fused_sweep=2), "fused_sweep"),
-    (dict(xz_variant=11), "xz_variant"),
+    (dict(xz_variant=12), "xz_variant"),
     (dict(nx_override=32768, nz_override=16384), "grid too large"),
     (dict(model="emhd", nx_override=32768, nz_override=16384), "grid too large"),
-    (dict(model="emhd", xz_variant=3), "xz_variant"),
+    (dict(model="emhd", xz_variant=4), "xz_variant"),
 ])
 def test_constraint_errors_are_codes_not_exits(keys, msg):
     with pytest.raises(khs.KhsError) as e:

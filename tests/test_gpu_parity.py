@@ -165,6 +165,10 @@ FILTER_CASES = {
     "bands_nyquist": grid(48, 32, 0, kz_suppress_max=3, kz_suppress_hi=16),
     "zmean_bands_tall": grid(70, 512, 0, suppress_kz0=1, kz_suppress_max=2, kz_suppress_hi=20),
     "keep_tall_float_twiddles": grid(64, 200, 1, kz_keep_max=6, mfma_filter=0),
+    # z-mean only: the one-launch cut + z-mean + colour-1 zeroing kernel
+    "zmean_only_ragged": grid(70, 32, 0, suppress_kz0=1),
+    "zmean_xicut": grid(96, 64, 0, suppress_kz0=1, xi_cut=3.0),
+    "zmean_sponge_periodic": grid(128, 128, 1, suppress_kz0=1, xi_sponge=4.0, sigma_sponge=1.0),
 }
 
 
@@ -310,7 +314,7 @@ def test_maxwell_march_bitwise(mk, shape, periodic, freeze):
         assert_bitwise(sim.get(n), o.get(n), f"maxwell {mk} {shape} {n}")
 
 
-@pytest.mark.parametrize("xzv", list(range(11)))
+@pytest.mark.parametrize("xzv", list(range(12)))
 @pytest.mark.parametrize("shape,periodic,warm", [((300, 136), 1, 0.01), ((70, 36), 0, 0.0), ((256, 64), 0, 0.0),
                                                  ((1000, 40), 0, 0.01), ((116, 300), 1, 0.0)])
 def test_fused_sweep_variants_bitwise(xzv, shape, periodic, warm):
