@@ -56,22 +56,26 @@ void launch_cell_filters(const ArrayList<T>& arrs, const double* decay, const in
 // one array: arrays [0, nmean) lose their column mean (the last nact of them
 // first take the column action: 1 = times decay[x], 2 = zero; per column, so
 // it commutes with the column mean), arrays [nmean, n) are set to zero.  The
-// column sum is FP64 over 4 row groups added in group order; the mean is
-// sum * w (w = 1 / nz, as in the column DFT filter's mode-0 weight).
+// column sum is FP64 over ZM_G row groups (each with four running sums)
+// added in group order; the mean is sum * w (w = 1 / nz, as in the column
+// DFT filter's mode-0 weight).
 // --------------------------------------------------------------------------
+constexpr int ZM_G = 16;     // row groups per workgroup of k_zmean_cut
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_zmean_cut(ArrayList<T> arrs, int nmean, int nact, const double* decay,
-                                                   const int* action, double w, KParams p) {
-    __shared__ double red[4][64];
+__global__ __launch_bounds__(64 * ZM_G) void k_zmean_cut(ArrayList<T> arrs, int nmean, int nact, const double* decay,
+                                                         const int* action, double w, KParams p) {
+    __shared__ double red[ZM_G][64];
     const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
     const int x = blockIdx.x * 64 + lane;
     const int k = blockIdx.y;
     const int nx = p.nx, nz = p.nz;
-    const int rows = (nz + 3) / 4;
+    const int rows = (nz + ZM_G - 1) / ZM_G;
     const int z0 = g * rows, z1 = min(nz, z0 + rows);
     T* v = arrs.a[k];
     if (k >= nmean) {
         if (x < nx)
+#pragma unroll 4
             for (int z = z0; z < z1; ++z) v[(size_t)x + (size_t)z * nx] = T(0.0);
         return;                                        // whole workgroups leave together
     }
@@ -87,13 +91,25 @@ __global__ __launch_bounds__(256) void k_zmean_cut(ArrayList<T> arrs, int nmean,
         else if (act == 2) a = T(0.0);
         return a;
     };
-    double s = 0.0;
-    if (x < nx)
-        for (int z = z0; z < z1; ++z) s += (double)val((size_t)x + (size_t)z * nx);
-    red[g][lane] = s;
+    // four running sums (four loads in flight), added in order
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    if (x < nx) {
+        int z = z0;
+        for (; z + 4 <= z1; z += 4) {
+            const T a0 = val((size_t)x + (size_t)z * nx), a1 = val((size_t)x + (size_t)(z + 1) * nx);
+            const T a2 = val((size_t)x + (size_t)(z + 2) * nx), a3 = val((size_t)x + (size_t)(z + 3) * nx);
+            s0 += (double)a0; s1 += (double)a1; s2 += (double)a2; s3 += (double)a3;
+        }
+        for (; z < z1; ++z) s0 += (double)val((size_t)x + (size_t)z * nx);
+    }
+    red[g][lane] = (s0 + s1) + (s2 + s3);
     __syncthreads();
-    const double mean = (((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane]) * w;
+    double tot = 0.0;
+#pragma unroll
+    for (int q = 0; q < ZM_G; ++q) tot += red[q][lane];
+    const double mean = tot * w;
     if (x < nx)
+#pragma unroll 4
         for (int z = z0; z < z1; ++z) {
             const size_t i = (size_t)x + (size_t)z * nx;
             v[i] = (T)((double)val(i) - mean);
@@ -104,8 +120,8 @@ template <typename T>
 void launch_zmean_cut(const ArrayList<T>& arrs, int nmean, int nact, const double* decay, const int* action,
                       const KParams& p, hipStream_t st) {
     if (arrs.n == 0) return;
-    hipLaunchKernelGGL(k_zmean_cut<T>, dim3((p.nx + 63) / 64, arrs.n), dim3(256), 0, st, arrs, nmean, nact, decay,
-                       action, 1.0 / (double)p.nz, p);
+    hipLaunchKernelGGL(k_zmean_cut<T>, dim3((p.nx + 63) / 64, arrs.n), dim3(64 * ZM_G), 0, st, arrs, nmean, nact,
+                       decay, action, 1.0 / (double)p.nz, p);
 }
 
 // --------------------------------------------------------------------------
