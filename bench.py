@@ -186,7 +186,7 @@ def growth_c3(precision, n_steps=500, stride=10):
     return ts
 
 
-def sweep(n=64, steps=3000, lanes=8, segment=50, threads=8):
+def sweep(n=64, steps=3000, lanes=8, segment=50, threads=16):
     """Batched-sweep workload: n uneven-grid FP32 runs (khs.sweep) stepped
     three ways: all runs' segments of `segment` steps as one captured graph
     over `lanes` streams ("graph"); `threads` host threads each stepping its
@@ -269,7 +269,7 @@ def profile_leg(leg, steps, warmup, fused=-1):
         b = khs.Batch(sweep_configs(64, steps), lanes=8, segment=50)
         b.init()
         t0 = time.perf_counter()
-        done, halted = b.run_concurrent(steps, threads=8) if leg == "sweep" else b.run(steps)
+        done, halted = b.run_concurrent(steps, threads=16) if leg == "sweep" else b.run(steps)
         el = time.perf_counter() - t0
         b.close()
         return {"leg": leg, "runs": 64, "steps": steps, "seconds": el, "halted": int(sum(halted)),
