@@ -22,7 +22,7 @@ and so is the precision-form workload (4096x2048, V0 = 0.1, a = 0.5, keep
 sweep (64 uneven FP32 runs through captured graphs: runs/hour), and the
 EMHD 1024x1024 FP32 wave run with its MFMA space-time dispersion check.
 
-Prints one JSON line (rank 0, N = 1 only; there is one GPU per box).
+Prints one JSON line (one GPU: N = 1 only).
 """
 import argparse
 import json
@@ -379,8 +379,8 @@ def main():
         calibrate_copy(a.calibrate_copy)
         return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU per box; the multi-GPU (independent runs per GPU "
-              "or x-slab) path is a design, not a measured configuration. Use --gpus 1.", file=sys.stderr)
+        print("bench.py: measures one GPU only; the multi-GPU path (independent runs per GPU or x-slabs) "
+              "is a design, not a measured configuration. Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     fused = 1 if a.fused else (0 if a.split else -1)     # -1: the autotuner's choice
