@@ -64,24 +64,25 @@ ZM_ROWS = {1: 16, 2: 32, 3: 64}       # chunk height of the fused z sweep + Maxw
 # by what the counter reads for a plain 256 MiB device copy (fetch x2.0,
 # write x1.0); summary under profiles/.  Keyed by kernel instance.
 PMC_TRAFFIC_MB = {
-    "k_fct_x<double, 128, YMModel, 0>": 507.7, "k_fct_x<double, 128, YMModel, 1>": 512.6,
-    "k_fct_x<double, 256, YMModel, 0>": 507.5, "k_fct_x<double, 256, YMModel, 1>": 512.3,
-    "k_fct_x<double, 64, YMModel, 0>": 507.4, "k_fct_x<double, 64, YMModel, 1>": 512.7,
-    "k_fct_xr<double, 4>": 507.7, "k_fct_xr<double, 8>": 528.7, "k_fct_xz2<double, 16>": 706.7,
-    "k_fct_xz2<double, 32>": 666.2, "k_fct_xz2<double, 64>": 642.0, "k_fct_xz<double, 32>": 680.7,
-    "k_fct_xzg<double, 16, 256>": 671.1, "k_fct_xzg<double, 32, 128>": 772.2, "k_fct_xzg<double, 32, 256>": 694.5,
-    "k_fct_xzg<double, 64, 256>": 758.4, "k_fct_xzr<double, 128>": 648.2, "k_fct_xzr<double, 32>": 681.2,
-    "k_fct_xzr<double, 64>": 661.0, "k_fct_z2<double, 16, YMModel>": 970.4, "k_fct_z2<double, 32, YMModel>": 864.8,
-    "k_fct_z2<double, 64, YMModel>": 811.5, "k_fct_z<double, 16, 0, YMModel>": 979.2,
-    "k_fct_z<double, 16, 1, YMModel>": 971.1, "k_fct_z<double, 32, 0, YMModel>": 870.0,
-    "k_fct_z<double, 32, 1, YMModel>": 868.1, "k_fct_z<double, 64, 0, YMModel>": 806.5,
-    "k_fct_z<double, 64, 1, YMModel>": 814.9, "k_fct_zg<double, 16, 0, YMModel>": 1054.6,
-    "k_fct_zg<double, 16, 1, YMModel>": 993.5, "k_fct_zg<double, 32, 0, YMModel>": 985.3,
-    "k_fct_zg<double, 32, 1, YMModel>": 926.1, "k_fct_zg<double, 64, 0, YMModel>": 929.8,
-    "k_fct_zg<double, 64, 1, YMModel>": 949.5, "k_maxwell<double, 16>": 846.3, "k_maxwell<double, 4>": 719.5,
-    "k_maxwell<double, 8>": 859.3, "k_maxwell_cell<double>": 597.5, "k_maxwell_zm<double, 16, 0>": 652.9,
-    "k_maxwell_zm<double, 16, 1>": 636.2, "k_maxwell_zm<double, 32, 1>": 620.7,
-    "k_maxwell_zm<double, 4, 1>": 702.0, "k_maxwell_zm<double, 8, 0>": 688.9, "k_maxwell_zm<double, 8, 1>": 659.2,
+    "k_fct_x<double, 128, YMModel, 0>": 507.6, "k_fct_x<double, 128, YMModel, 1>": 512.2,
+    "k_fct_x<double, 256, YMModel, 0>": 507.6, "k_fct_x<double, 256, YMModel, 1>": 512.3,
+    "k_fct_x<double, 64, YMModel, 0>": 507.4, "k_fct_x<double, 64, YMModel, 1>": 512.8,
+    "k_fct_xr<double, 4>": 507.7, "k_fct_xr<double, 8>": 528.5, "k_fct_xz2<double, 16>": 706.8,
+    "k_fct_xz2<double, 32>": 665.9, "k_fct_xz2<double, 64>": 642.0, "k_fct_xz<double, 32>": 680.7,
+    "k_fct_xzg<double, 16, 256>": 677.1, "k_fct_xzg<double, 32, 128>": 725.2, "k_fct_xzg<double, 32, 256>": 704.2,
+    "k_fct_xzg<double, 64, 256>": 699.4, "k_fct_xzg<double, 8, 128>": 657.4, "k_fct_xzr<double, 128>": 648.8,
+    "k_fct_xzr<double, 32>": 680.6, "k_fct_xzr<double, 64>": 661.0, "k_fct_z2<double, 16, YMModel>": 970.3,
+    "k_fct_z2<double, 32, YMModel>": 865.2, "k_fct_z2<double, 64, YMModel>": 811.3,
+    "k_fct_z<double, 16, 0, YMModel>": 978.6, "k_fct_z<double, 16, 1, YMModel>": 971.1,
+    "k_fct_z<double, 32, 0, YMModel>": 870.4, "k_fct_z<double, 32, 1, YMModel>": 869.1,
+    "k_fct_z<double, 64, 0, YMModel>": 806.5, "k_fct_z<double, 64, 1, YMModel>": 815.4,
+    "k_fct_zg<double, 16, 0, YMModel>": 1051.1, "k_fct_zg<double, 16, 1, YMModel>": 993.1,
+    "k_fct_zg<double, 32, 0, YMModel>": 982.8, "k_fct_zg<double, 32, 1, YMModel>": 927.8,
+    "k_fct_zg<double, 64, 0, YMModel>": 936.6, "k_fct_zg<double, 64, 1, YMModel>": 940.0,
+    "k_maxwell<double, 16>": 846.5, "k_maxwell<double, 4>": 718.5, "k_maxwell<double, 8>": 859.2,
+    "k_maxwell_cell<double>": 597.7, "k_maxwell_zm<double, 16, 0>": 646.3, "k_maxwell_zm<double, 16, 1>": 640.0,
+    "k_maxwell_zm<double, 32, 1>": 614.0, "k_maxwell_zm<double, 4, 1>": 702.4, "k_maxwell_zm<double, 8, 0>": 678.3,
+    "k_maxwell_zm<double, 8, 1>": 662.8,
 }
 
 
