@@ -132,7 +132,8 @@ typedef struct khs_config {
     int      xz_variant;       /* fused sweep kernel variant: -1 = autotuned (default), YM 0..11, EMHD 0..3 */
     int      x_serial;         /* x sweep phases per variable: -1 = autotuned (default), 0 = all at once, 1 = one at a time */
     int      maxwell_kernel;   /* YM Maxwell: -1 = autotuned (default), 0/2/3 = 64x16/64x4/64x8 LDS tiles, 1 = per cell,
-                                  4..9 = z march with 16/32/16/8/8/4-row chunks (6, 8: without next-row prefetch) */
+                                  4..9 = z march with 16/32/16/8/8/4-row chunks (6, 8: without next-row prefetch);
+                                  EMHD: 0 = 64x16 tiles, 1/2/3 = z march with 4/8/16-row chunks */
     int      x_rows;           /* x sweep rows per workgroup: 0 = autotuned (default), 1, 4 or 16 */
     int      z_prefetch;       /* z sweep: -1 = autotuned (default), 0 = load each row when used, 1 / 2 = load one / two rows ahead */
     int      mfma_filter;      /* keep-|kz|<=K filter as MFMA contractions (1, default) or the per-column FP64-sum kernels (0) */

@@ -242,6 +242,7 @@ static void config_derive_emhd(khs_config& c) {
     if (c.nx < 4 || c.nz < 4) throw Error(KHS_ERR_CONFIG, "grid too small");
     check_grid_bytes(c.nx, c.nz);
     if (c.xz_variant > 3) throw Error(KHS_ERR_CONFIG, "xz_variant must be -1..3 for the EMHD model");
+    if (c.maxwell_kernel > 3) throw Error(KHS_ERR_CONFIG, "maxwell_kernel must be -1..3 for the EMHD model");
     if (c.species != 1 && c.species != 2) throw Error(KHS_ERR_CONFIG, "species must be 1 or 2");
     if (c.x_segment != 0 && c.x_segment != 64 && c.x_segment != 128 && c.x_segment != 256)
         throw Error(KHS_ERR_CONFIG, "x_segment must be 0, 64, 128 or 256");

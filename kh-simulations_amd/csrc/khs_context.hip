@@ -336,6 +336,12 @@ struct Ctx {
             zmvar = pick<T>(zmc, [&](int v) { zm_launch<T>(v); });
             return;
         }
+        if (emhd && cfg.maxwell_kernel < 0) {
+            std::vector<int> mc;
+            for (int v = 0; v < kNumEmMaxwellVariants; ++v) mc.push_back(v);
+            // writes the next state's fields from the current state: scratch here
+            mvar = pick<T>(mc, [&](int v) { launch_emhd_maxwell<T>(state<T>(cur), state<T>(1 - cur), kp, v, stream); });
+        }
         if (!emhd && cfg.maxwell_kernel < 0) {
             std::vector<int> mc;
             for (int v = 0; v < kNumMaxwellVariants; ++v) mc.push_back(v);
@@ -383,7 +389,7 @@ struct Ctx {
         set_order(2);
         timed(K_MAXWELL, [&] {
             if (emhd)
-                launch_emhd_maxwell<T>(state<T>(cur), state<T>(1 - cur), kp, stream);
+                launch_emhd_maxwell<T>(state<T>(cur), state<T>(1 - cur), kp, mvar, stream);
             else
                 launch_maxwell<T>(state<T>(cur), state<T>(1 - cur), kp, cell_filters ? d_decay : nullptr,
                                   cell_filters ? d_action : nullptr, mvar, stream);

@@ -26,7 +26,8 @@ template <typename T, typename M>
 void launch_fct_z(const State<T>& S, const Stage<T>& X, const State<T>& O, const KParams& p,
                   int sp0, int nsp, int variant, hipStream_t st);
 template <typename T>
-void launch_emhd_maxwell(const State<T>& S, const State<T>& O, const KParams& p, hipStream_t st);
+void launch_emhd_maxwell(const State<T>& S, const State<T>& O, const KParams& p, int variant, hipStream_t st);
+constexpr int kNumEmMaxwellVariants = 4;   // 0: 64 x 16 tiles; 1/2/3: z march, 4/8/16-row chunks
 // EMHD energy (main_coupled.cu:197-222): out[0] = sum over cells
 template <typename T>
 void launch_energy_em(const State<T>& S, const KParams& p, double* partials, double* out, hipStream_t st);

@@ -1730,8 +1730,183 @@ void launch_fct_z(const State<T>& S, const Stage<T>& X, const State<T>& O, const
     }
 }
 
+// --------------------------------------------------------------------------
+// EMHD Maxwell as a z march (the k_emhd_maxwell update, same expressions).
+// Lane l of a wavefront owns column 62 g - 1 + l; lanes 0 and 63 are halo
+// lanes, so a wavefront stores the 62 columns of lanes 1..62.  Ampere for
+// row e needs the new pz of row e+1 (Jz) and the old Bx, By of row e-1 (the
+// previous iteration's row); the x+1 momentum of Jx comes from lane l+1, B
+// at the Ampere x-neighbour (x-1, or x+1 at the x = 0 wall) is loaded through
+// the reference's index rule.  Faraday for row e-1 runs one iteration later,
+// when the new E of row e exists; the new Ey, Ez of x+1 (or x-1 at the
+// x = nx-1 wall) come from lane l+1 (l-1).  Lane 63's new Ex lacks its x+1
+// momentum but no stored lane reads it; lane 0's and 63's new Ey, Ez are
+// complete.  A chunk of TZ rows computes Ampere for TZ+1 rows (the last is
+// the next chunk's first, z periodic).
+// --------------------------------------------------------------------------
+constexpr int EMZ_OUT = 62;
+
 template <typename T>
-void launch_emhd_maxwell(const State<T>& S, const State<T>& O, const KParams& p, hipStream_t st) {
+struct EmFieldRow {
+    T ex, ey, ez, bx, by, bz, byo, bzo;   // old fields at (x, z); By, Bz at Ampere's x-neighbour
+};
+
+template <typename T, int TZ>
+__global__ __launch_bounds__(256) void k_emhd_maxwell_zm(State<T> S, State<T> O, KParams p, int nwx, int nch) {
+    const int lane = threadIdx.x & 63;
+    int wl = __builtin_amdgcn_readfirstlane(xcd_logical(blockIdx.x, gridDim.x) * 4 + (int)(threadIdx.x >> 6));
+    if (wl >= nwx * nch) return;             // whole wavefronts leave together
+    wl = tile_order(wl, nwx * nch, p.rev);
+    const int nx = p.nx, nz = p.nz;
+    const bool per = p.periodic_x != 0;
+    const int x = (wl % nwx) * EMZ_OUT - 1 + lane;
+    const int xc = map_x(x, p);
+    const bool store = lane >= 1 && lane <= EMZ_OUT && x < nx;
+    const bool w = !per && (xc == 0 || xc == nx - 1);
+    const bool fwd = !per && xc == 0;                           // Ampere: B(1) - B(0) at the x = 0 wall
+    const int xo = per ? (xc + nx - 1) % nx : (xc == 0 ? 1 : xc - 1);
+    const bool bwd = !per && xc == nx - 1;                      // Faraday: E(x) - E(x-1) at the x = nx-1 wall
+    const int z0 = (wl / nwx) * TZ;
+    const int z1 = min(z0 + TZ, nz);
+    const int nsp = p.nsp;
+    const T dt = T(p.dt), c2 = T(p.c2), eps0 = T(p.eps0);
+    const T dx = T(p.dx), dz = T(p.dz);
+    const unsigned bytes = (unsigned)nx * (unsigned)nz * (unsigned)sizeof(T);
+    __amdgpu_buffer_rsrc_t ro[9];
+#pragma unroll
+    for (int a = 0; a < 9; ++a) ro[a] = buf_rsrc(O.a[E_EX + a], bytes);
+    auto rw = [&](int e) { return e < nz ? e : e - nz; };      // e <= nz + 1
+    auto load_fields = [&](int r, EmFieldRow<T>& f) {
+        const unsigned i = (unsigned)xc + (unsigned)r * (unsigned)nx;
+        const unsigned io = (unsigned)xo + (unsigned)r * (unsigned)nx;
+        f.ex = ld32(S.a[E_EX], i); f.ey = ld32(S.a[E_EY], i); f.ez = ld32(S.a[E_EZ], i);
+        f.bx = ld32(S.a[E_BX], i); f.by = ld32(S.a[E_BY], i); f.bz = ld32(S.a[E_BZ], i);
+        f.byo = ld32(S.a[E_BY], io); f.bzo = ld32(S.a[E_BZ], io);
+    };
+    auto load_fluid = [&](int r, T (&px)[2], T (&py)[2], T (&pz)[2]) {
+        const unsigned i = (unsigned)xc + (unsigned)r * (unsigned)nx;
+        for (int s = 0; s < 2; ++s) {
+            const int b = em_base(s);
+            px[s] = s < nsp ? ld32(O.a[b + EV_PX], i) : T(0);
+            py[s] = s < nsp ? ld32(O.a[b + EV_PY], i) : T(0);
+            pz[s] = s < nsp ? ld32(O.a[b + EV_PZ], i) : T(0);
+        }
+    };
+
+    EmFieldRow<T> cur, nxt;
+    T px[2], py[2], pz[2];
+    load_fields(z0, cur);
+    load_fluid(z0, px, py, pz);
+    T bxm, bym;                                                 // old Bx, By of row e-1
+    {
+        const unsigned im = (unsigned)xc + (unsigned)zm_per(z0, nz) * (unsigned)nx;
+        bxm = ld32(S.a[E_BX], im);
+        bym = ld32(S.a[E_BY], im);
+    }
+    T exp_ = T(0), eyp = T(0), ezp = T(0);                     // new E of row e-1
+    T bxp = T(0), byp = T(0), bzp = T(0);                      // old B of row e-1
+
+#pragma unroll 1
+    for (int e = z0; e <= z1; ++e) {
+        const int rz = rw(e), rzn = rw(e + 1);
+        if (e < z1) load_fields(rzn, nxt);
+        T pxn[2], pyn[2], pzn[2];
+        load_fluid(rzn, pxn, pyn, pzn);
+
+        // ---- Ampere for row e (EMHD_Physics.cu:21-98, currents main_coupled.cu:99-110) ----
+        T jx = T(0), jy = T(0), jz = T(0);
+        for (int s = 0; s < 2; ++s) {
+            if (s >= nsp) break;
+            const T q = T(s ? p.q1 : p.q0);
+            const T h = T((s ? p.q1 : p.q0) * 0.5);
+            const T a = h * (px[s] + lane_from_right(px[s]));
+            const T bb = q * py[s];
+            const T c = h * (pz[s] + pzn[s]);
+            if (s == 0) { jx = a; jy = bb; jz = c; }
+            else { jx = jx + a; jy = jy + bb; jz = jz + c; }
+        }
+        T exn, eyn, ezn;
+        {
+            const T Bx0 = w ? T(0) : cur.bx, Bxzm = w ? T(0) : bxm;
+            T dBy_dz = (cur.by - bym) / dz;
+            T curlB_x = -dBy_dz;
+            T dBx_dz = (Bx0 - Bxzm) / dz;
+            T dBz_dx, dBy_dx;
+            if (fwd) {
+                dBz_dx = (cur.bzo - cur.bz) / dx;
+                dBy_dx = (cur.byo - cur.by) / dx;
+            } else {
+                dBz_dx = (cur.bz - cur.bzo) / dx;
+                dBy_dx = (cur.by - cur.byo) / dx;
+            }
+            T curlB_y = dBx_dz - dBz_dx;
+            T curlB_z = dBy_dx;
+            const T ey = w ? T(0) : cur.ey;
+            const T ez = w ? T(0) : cur.ez;
+            exn = cur.ex + dt * (c2 * curlB_x - jx / eps0);
+            eyn = ey + dt * (c2 * curlB_y - jy / eps0);
+            ezn = ez + dt * (c2 * curlB_z - jz / eps0);
+        }
+
+        // ---- Faraday and the potential for row e-1 ----
+        const T eyr = lane_from_right(eyp), ezr = lane_from_right(ezp);
+        const T eyl = lane_from_left(eyp), ezl = lane_from_left(ezp);
+        if (e > z0) {
+            T dEy_dz = (eyn - eyp) / dz;
+            T curlE_x = -dEy_dz;
+            T dEx_dz = (exn - exp_) / dz;
+            T dEz_dx, dEy_dx;
+            if (bwd) {
+                dEz_dx = (ezp - ezl) / dx;
+                dEy_dx = (eyp - eyl) / dx;
+            } else {
+                dEz_dx = (ezr - ezp) / dx;
+                dEy_dx = (eyr - eyp) / dx;
+            }
+            T curlE_y = dEx_dz - dEz_dx;
+            T curlE_z = dEy_dx;
+            const T bx = w ? T(0) : bxp;
+            const unsigned i = (unsigned)xc + (unsigned)(e - 1) * (unsigned)nx;
+            const T ax = ld32(S.a[E_AX], i), ay = ld32(S.a[E_AY], i), az = ld32(S.a[E_AZ], i);
+            const unsigned off = store ? i * (unsigned)sizeof(T) : kNoStore;
+            buf_st(bx - dt * curlE_x, ro[E_BX - E_EX], off);
+            buf_st(byp - dt * curlE_y, ro[E_BY - E_EX], off);
+            buf_st(bzp - dt * curlE_z, ro[E_BZ - E_EX], off);
+            buf_st(exp_, ro[0], off);
+            buf_st(eyp, ro[E_EY - E_EX], off);
+            buf_st(ezp, ro[E_EZ - E_EX], off);
+            buf_st(ax - dt * exp_, ro[E_AX - E_EX], off);
+            buf_st(ay - dt * eyp, ro[E_AY - E_EX], off);
+            buf_st(az - dt * ezp, ro[E_AZ - E_EX], off);
+        }
+        // carry row e
+        exp_ = exn; eyp = eyn; ezp = ezn;
+        bxp = cur.bx; byp = cur.by; bzp = cur.bz;
+        bxm = cur.bx; bym = cur.by;
+        if (e < z1) cur = nxt;
+        for (int s = 0; s < 2; ++s) { px[s] = pxn[s]; py[s] = pyn[s]; pz[s] = pzn[s]; }
+        (void)rz;
+    }
+}
+
+template <typename T, int TZ>
+static void launch_emhd_maxwell_zm_t(const State<T>& S, const State<T>& O, const KParams& p, hipStream_t st) {
+    const int nwx = (p.nx + EMZ_OUT - 1) / EMZ_OUT;
+    const int nch = (p.nz + TZ - 1) / TZ;
+    const int waves = nwx * nch;
+    hipLaunchKernelGGL((k_emhd_maxwell_zm<T, TZ>), dim3(round8((waves + 3) / 4)), dim3(256), 0, st, S, O, p, nwx,
+                       nch);
+}
+
+// variant 0: 64 x 16 tiles; 1 / 2 / 3: z march with 4 / 8 / 16-row chunks
+template <typename T>
+void launch_emhd_maxwell(const State<T>& S, const State<T>& O, const KParams& p, int variant, hipStream_t st) {
+    switch (variant) {
+        case 1: launch_emhd_maxwell_zm_t<T, 4>(S, O, p, st); return;
+        case 2: launch_emhd_maxwell_zm_t<T, 8>(S, O, p, st); return;
+        case 3: launch_emhd_maxwell_zm_t<T, 16>(S, O, p, st); return;
+        default: break;
+    }
     const int ntx = (p.nx + MX - 1) / MX, ntz = (p.nz + MZ - 1) / MZ;
     dim3 grid(round8(ntx * ntz));
     hipLaunchKernelGGL(k_emhd_maxwell<T>, grid, dim3(MX * MZT), 0, st, S, O, p, ntx, ntz);
@@ -2596,8 +2771,9 @@ KHS_SWEEP_INST(float, YMModel)
 KHS_SWEEP_INST(double, EMModel)
 KHS_SWEEP_INST(float, EMModel)
 #undef KHS_SWEEP_INST
-template void launch_emhd_maxwell<double>(const State<double>&, const State<double>&, const KParams&, hipStream_t);
-template void launch_emhd_maxwell<float>(const State<float>&, const State<float>&, const KParams&, hipStream_t);
+template void launch_emhd_maxwell<double>(const State<double>&, const State<double>&, const KParams&, int,
+                                          hipStream_t);
+template void launch_emhd_maxwell<float>(const State<float>&, const State<float>&, const KParams&, int, hipStream_t);
 template void launch_fct_xz<double>(const State<double>&, const State<double>&, const KParams&, int, hipStream_t);
 template void launch_fct_xz<float>(const State<float>&, const State<float>&, const KParams&, int, hipStream_t);
 template void launch_fct_zmx<double>(const State<double>&, const Stage<double>&, const State<double>&, const KParams&,

@@ -156,6 +156,43 @@ def test_fused_sweep_variants_bitwise(xzv, shape, periodic, nsp):
         assert_bitwise(sim.get(n), o.get(n), f"xz{xzv} {shape} {n}")
 
 
+@pytest.mark.parametrize("mk", [0, 1, 2, 3])
+@pytest.mark.parametrize("shape,periodic,nsp", [((130, 70), 0, 2), ((200, 96), 1, 1), ((64, 40), 1, 2),
+                                                ((124, 33), 0, 1), ((125, 64), 1, 2)])
+def test_maxwell_variants_bitwise(mk, shape, periodic, nsp):
+    """EMHD Maxwell as tiles and as a z march (62 columns per wavefront, both
+    walls' one-sided differences, periodic x and z wrap, chunk heights that do
+    and do not divide nz, one and two fluids) gives the oracle's bits, as an
+    operator and in whole steps."""
+    nx, nz = shape
+    keys = em_keys(nx, nz, periodic, nsp, maxwell_kernel=mk, autotune=0)
+    cfg, sim, o = em_pair(em_random(nx, nz, nsp, seed=nx * 3 + mk, smooth=True), **keys)
+    o.lorentz()
+    o.update_velocities()
+    for s in range(nsp):
+        sim.fct_x_sweep(s)
+        o.x_sweep(s)
+    for s in range(nsp):
+        sim.fct_z_sweep(s)
+        o.z_sweep(s)
+    o.enforce_boundaries()
+    sim.maxwell()
+    o.currents()
+    o.ampere()
+    o.potential()
+    o.faraday()
+    for n in orc.EM_FIELDS:
+        assert_bitwise(sim.get_buf(khs.BUF_NEXT, n), o.get(n), f"maxwell {mk} {shape} {n}")
+    sim.commit()
+    o2 = orc.EmOracle(orc.em_params_from_config(cfg))
+    o2.set_state(sim.state())
+    o2.lorentz()
+    sim.step(4)
+    o2.step(4)
+    for n in orc.em_state_names(nsp):
+        assert_bitwise(sim.get(n), o2.get(n), f"maxwell {mk} {shape} steps {n}")
+
+
 def test_daskaw_steps_bitwise():
     """The reference run itself (256x256, dx = 1, dt = 0.002), 20 steps from init."""
     cfg = khs.Config(model="emhd", k_mode=1, n_steps=20)
