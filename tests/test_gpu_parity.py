@@ -333,6 +333,24 @@ def test_fused_sweep_variants_bitwise(xzv, shape, periodic, warm):
         assert_bitwise(sim.get(n), o.get(n), f"xz{xzv} {nx}x{nz} {n}")
 
 
+@pytest.mark.parametrize("extra,want", [(dict(fused_sweep=1), 1), (dict(fused_sweep=0), 0),
+                                        (dict(fused_sweep=-1, z_chunk=32), 0), (dict(fused_sweep=-1), None),
+                                        (dict(fused_sweep=-1, autotune=0), 1)])
+def test_sweep_form_choice(extra, want):
+    """The sweep form in use (khs_get_tuning): forced by fused_sweep = 0 / 1,
+    split when a split-sweep tile shape is forced, fused without tuning, the
+    timed choice otherwise; every form gives the oracle's bits."""
+    keys = grid(300, 136, 1, **extra)
+    cfg, sim, o = make_pair(random_state(300, 136, seed=17, smooth=True), **keys)
+    o.sources()
+    sim.step(2)
+    o.step(2)
+    got = sim.tuning()["fused_sweep"]
+    assert got in (0, 1) and (want is None or got == want)
+    for n in orc.STATE:
+        assert_bitwise(sim.get(n), o.get(n), f"{extra} {n}")
+
+
 def test_determinism_and_staged_equals_fused():
     keys = grid(256, 128, 1, hyp_diff=1e-3, xi_sponge=6.0)
     st = random_state(256, 128, seed=21, smooth=True)
