@@ -59,6 +59,13 @@ KERNEL_ARRAYS = {"k_fct_x": 18 + 12, "k_fct_z": 33 + 12, "k_fct_xz": 24 + 12, "k
 # The whole step's algorithmic minimum: R 24 + W 23 arrays (frozen Az1 read only).
 STEP_ARRAYS = 47
 ZM_ROWS = {1: 16, 2: 32, 3: 64}       # chunk height of the fused z sweep + Maxwell variants
+# What limits the step kernels other than HBM bytes, from rocprofv3 SQ counters
+# on this workload (profiles/sq_counters_2048x1024_fp64.md), keyed by slot.
+LIMITER = {
+    "k_fct_xz": "FP64 VALU issue and latency at 2 waves per SIMD: SIMD VALU busy 0.43 of the kernel's "
+                "cycles, waves waiting on memory counters 0.32 of their life (profiles/sq_counters_2048x1024_fp64.md)",
+    "k_maxwell": "HBM: 702 MB per launch (1.20x algorithmic) at 5.6 TB/s; waves wait on memory 0.47 of their life",
+}
 # HBM bytes per launch of the FP64 step kernels on this workload, measured with
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (separate runs), each scaled
 # by what the counter reads for a plain 256 MiB device copy (fetch x2.0,
@@ -443,7 +450,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": d["alg_GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": d["alg_GBs"] / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel_instance": inst, "alg_bytes_per_launch": d["alg_bytes"], "avg_launch_us": d["avg_us"],
-                     "traffic_GBs": traffic / (d["avg_us"] * 1e-6) / 1e9 if traffic else None},
+                     "traffic_GBs": traffic / (d["avg_us"] * 1e-6) / 1e9 if traffic else None,
+                     "limiter": LIMITER.get(dom)},
         "step_roofline": {"alg_bytes_per_cell": STEP_ARRAYS * 8, "achieved_GBs": step_alg_GBs,
                           "frac": step_alg_GBs / HBM_PEAK_GBS},
         "kernels": kern,
