@@ -241,7 +241,7 @@ static void config_derive_emhd(khs_config& c) {
     c.energy_thresh = 5.0;
     if (c.nx < 4 || c.nz < 4) throw Error(KHS_ERR_CONFIG, "grid too small");
     check_grid_bytes(c.nx, c.nz);
-    if (c.xz_variant > 3) throw Error(KHS_ERR_CONFIG, "xz_variant must be -1..3 for the EMHD model");
+    if (c.xz_variant > 6) throw Error(KHS_ERR_CONFIG, "xz_variant must be -1..6 for the EMHD model");
     if (c.maxwell_kernel > 3) throw Error(KHS_ERR_CONFIG, "maxwell_kernel must be -1..3 for the EMHD model");
     if (c.species != 1 && c.species != 2) throw Error(KHS_ERR_CONFIG, "species must be 1 or 2");
     if (c.x_segment != 0 && c.x_segment != 64 && c.x_segment != 128 && c.x_segment != 256)

@@ -286,7 +286,7 @@ struct Ctx {
         const bool split_keys = lds_keys || fl >= 0 || fz >= 0 || fp >= 0 || fg >= 0 || cfg.zm_fused > 0;
         const bool try_fused = cfg.fused_sweep < 0 && !split_keys;
         fused = cfg.fused_sweep > 0 || try_fused;
-        xzvar = (cfg.xz_variant >= 0) ? cfg.xz_variant : (emhd ? 1 : 7);
+        xzvar = (cfg.xz_variant >= 0) ? cfg.xz_variant : (emhd ? 5 : 7);
         if (!cfg.autotune) return;
         // contexts of the same shape in one process (sweeps, campaigns) reuse
         // the first one's choice: every variant computes the same bits

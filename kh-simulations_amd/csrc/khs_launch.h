@@ -37,8 +37,9 @@ void launch_derived_em(const State<T>& S, int which, const KParams& p, double* o
 // fused x + z sweep of both species: state S -> fluid of O (variants: see
 // launch_fct_xz in khs_step_kernels.hip)
 constexpr int kNumXZVariants = 12;
-// EMHD fused x+z sweep of species 0..nsp-1: variants 0/1/2/3 (16/32/64/8-row chunks)
-constexpr int kNumXZEmVariants = 4;
+// EMHD fused x+z sweep of species 0..nsp-1: variants 0/1/2/3 (16/32/64/8-row chunks), 4/5/6
+// (two wavefronts per species, 8/16/32-row chunks)
+constexpr int kNumXZEmVariants = 7;
 template <typename T>
 void launch_fct_xz_em(const State<T>& S, const State<T>& O, const KParams& p, int nsp, int variant, hipStream_t st);
 template <typename T>

@@ -2482,7 +2482,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_
 // makes the row form equal to em_lorentz at that column for every lane;
 // interior wavefronts take Ex, Bz, By at x-1 from lane l-1.
 // --------------------------------------------------------------------------
-template <typename T, int TZ, bool EDGE>
+// VB, NG: the variables this wavefront advects (all four, or {n, px} /
+// {py, pz} when a species is split over two wavefronts).
+template <typename T, int TZ, bool EDGE, int VB = 0, int NG = EV_COUNT>
 __device__ __forceinline__ void xz_march_em(const State<T>& S, const State<T>& O, const KParams& p, int sp, int g,
                                             int ch, int lane) {
     constexpr int NV = EV_COUNT;
@@ -2501,9 +2503,9 @@ __device__ __forceinline__ void xz_march_em(const State<T>& S, const State<T>& O
     const int zend = min(z0 + TZ, nz);
     const int elast = zend + 2;
     const unsigned bytes = (unsigned)nx * (unsigned)nz * (unsigned)sizeof(T);
-    __amdgpu_buffer_rsrc_t orsc[NV];
+    __amdgpu_buffer_rsrc_t orsc[NG];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) orsc[v] = buf_rsrc(O.a[b + v], bytes);
+    for (int j = 0; j < NG; ++j) orsc[j] = buf_rsrc(O.a[b + VB + j], bytes);
 
     auto load = [&](int r, Row& row, T (&F)[NV]) {
         const unsigned i = (unsigned)xc + (unsigned)r * (unsigned)nx;
@@ -2535,7 +2537,7 @@ __device__ __forceinline__ void xz_march_em(const State<T>& S, const State<T>& O
         load(zm_per(rc, nz), prev, Fd);
     }
     load(rc, cur, F);
-    ZPipe<T, NV> zp;
+    ZPipe<T, NG> zp;
     zp.Um = T(0);
     zp.MUm = T(0);
 
@@ -2551,7 +2553,7 @@ __device__ __forceinline__ void xz_march_em(const State<T>& S, const State<T>& O
         em_lorentz_v(cur, prev, F[EV_N], F[EV_PX], F[EV_PY], F[EV_PZ], q, f);
 
         // ---- x sweep of row rc at this lane's column (FCT_Sweeps.cu:184-197) ----
-        T V[NV];
+        T V[NG];
         {
             const T u = F[EV_PX] / kmax(F[EV_N], T(0.05));        // update_velocities
             const T ur = lane_from_right(u);
@@ -2560,7 +2562,8 @@ __device__ __forceinline__ void xz_march_em(const State<T>& S, const State<T>& O
             T nu, mu;
             fct_nu_mu(eps, nu, mu);
 #pragma unroll
-            for (int k = 0; k < NV; ++k) {
+            for (int j = 0; j < NG; ++j) {
+                const int k = VB + j;
                 const T v0 = F[k];
                 const T vp = lane_from_right(v0);
                 T fR = T(0.5) * eps * (v0 + vp);
@@ -2577,24 +2580,26 @@ __device__ __forceinline__ void xz_march_em(const State<T>& S, const State<T>& O
                 T flx = mu * (lorr - lor);
                 T fl = fct_limit(flx, lnr - lnrl, lnrr - lnr, lnrr2 - lnrr);
                 const T fll = lane_from_left(fl);
-                V[k] = lnr - (fl - fll);
+                V[j] = lnr - (fl - fll);
             }
         }
 
         // ---- z sweep (FCT_Sweeps.cu:199-232) of the x-swept row ----
         const T uz = F[EV_PZ] / kmax(F[EV_N], T(0.05));
-        const T Sv[NV] = {T(0), f[0], f[1], f[2]};
-        T out[NV];
+        const T Sall[NV] = {T(0), f[0], f[1], f[2]};
+        T Sv[NG], out[NG];
+#pragma unroll
+        for (int j = 0; j < NG; ++j) Sv[j] = Sall[VB + j];
         bool have = true;
         if (MODE) zp.push_steady(V, uz, Sv, dt, dtdz, out);
         else have = zp.push(e, z0, V, uz, Sv, dt, dtdz, out);
         if (have) {
-            if (out[0] < T(0.05)) out[0] = T(0.05);          // enforce_boundaries (fluid)
-            if (wall) out[EV_PX] = T(0.0);
+            if (VB == EV_N && out[0] < T(0.05)) out[0] = T(0.05);          // enforce_boundaries (fluid)
+            if (VB <= EV_PX && EV_PX < VB + NG && wall) out[EV_PX - VB] = T(0.0);
             const unsigned off =
                 store ? ((unsigned)xc + (unsigned)(e - 3) * (unsigned)nx) * (unsigned)sizeof(T) : kNoStore;
 #pragma unroll
-            for (int v = 0; v < NV; ++v) buf_st_nt(out[v], orsc[v], off);
+            for (int j = 0; j < NG; ++j) buf_st_nt(out[j], orsc[j], off);
         }
         if (MODE) {
             prev = cur;
@@ -2633,6 +2638,37 @@ __global__ __launch_bounds__(256) void k_fct_xze(State<T> S, State<T> O, KParams
     else xz_march_em<T, TZ, false>(S, O, p, sp, g, ch, lane);
 }
 
+// the same with each species' variables on two wavefronts ({n, px}, {py, pz}):
+// 2 waves per (species, tile, chunk), 4 per workgroup
+template <typename T, int TZ>
+__global__ __launch_bounds__(256) void k_fct_xze2(State<T> S, State<T> O, KParams p, int nsp, int ngx, int nch) {
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    int wl = __builtin_amdgcn_readfirstlane(xcd_logical(blockIdx.x, gridDim.x) * 2 + (w >> 1));
+    if (wl >= nsp * ngx * nch) return;        // both wavefronts of a tile leave together
+    wl = tile_order(wl, nsp * ngx * nch, p.rev);
+    const int sp = wl % nsp;
+    const int g = (wl / nsp) % ngx;
+    const int ch = wl / (nsp * ngx);
+    const bool edge = g == 0 || g * XZR_OUT + 60 >= p.nx;
+    if ((w & 1) == 0) {
+        if (edge) xz_march_em<T, TZ, true, EV_N, 2>(S, O, p, sp, g, ch, lane);
+        else xz_march_em<T, TZ, false, EV_N, 2>(S, O, p, sp, g, ch, lane);
+    } else {
+        if (edge) xz_march_em<T, TZ, true, EV_PY, 2>(S, O, p, sp, g, ch, lane);
+        else xz_march_em<T, TZ, false, EV_PY, 2>(S, O, p, sp, g, ch, lane);
+    }
+}
+
+template <typename T, int TZ>
+static void launch_fct_xze2_t(const State<T>& S, const State<T>& O, const KParams& p, int nsp, hipStream_t st) {
+    const int ngx = (p.nx + XZR_OUT - 1) / XZR_OUT;
+    const int nch = (p.nz + TZ - 1) / TZ;
+    const int tiles = nsp * ngx * nch;
+    hipLaunchKernelGGL((k_fct_xze2<T, TZ>), dim3(round8((tiles + 1) / 2)), dim3(256), 0, st, S, O, p, nsp, ngx,
+                       nch);
+}
+
 template <typename T, int TZ>
 static void launch_fct_xze_t(const State<T>& S, const State<T>& O, const KParams& p, int nsp, hipStream_t st) {
     const int ngx = (p.nx + XZR_OUT - 1) / XZR_OUT;
@@ -2641,10 +2677,14 @@ static void launch_fct_xze_t(const State<T>& S, const State<T>& O, const KParams
     hipLaunchKernelGGL((k_fct_xze<T, TZ>), dim3(round8((waves + 3) / 4)), dim3(256), 0, st, S, O, p, nsp, ngx, nch);
 }
 
-// EMHD fused sweep of species 0..nsp-1, variants 0/1/2/3: 16/32/64/8-row chunks
+// EMHD fused sweep of species 0..nsp-1, variants 0/1/2/3: 16/32/64/8-row
+// chunks; 4/5/6: the variables on two wavefronts, 8/16/32-row chunks
 template <typename T>
 void launch_fct_xz_em(const State<T>& S, const State<T>& O, const KParams& p, int nsp, int variant, hipStream_t st) {
-    if (variant == 0) launch_fct_xze_t<T, 16>(S, O, p, nsp, st);
+    if (variant == 4) launch_fct_xze2_t<T, 8>(S, O, p, nsp, st);
+    else if (variant == 5) launch_fct_xze2_t<T, 16>(S, O, p, nsp, st);
+    else if (variant == 6) launch_fct_xze2_t<T, 32>(S, O, p, nsp, st);
+    else if (variant == 0) launch_fct_xze_t<T, 16>(S, O, p, nsp, st);
     else if (variant == 2) launch_fct_xze_t<T, 64>(S, O, p, nsp, st);
     else if (variant == 3) launch_fct_xze_t<T, 8>(S, O, p, nsp, st);
     else launch_fct_xze_t<T, 32>(S, O, p, nsp, st);

@@ -91,7 +91,7 @@ def test_output_dir_name_c1():
     (dict(xz_variant=12), "xz_variant"),
     (dict(nx_override=32768, nz_override=16384), "grid too large"),
     (dict(model="emhd", nx_override=32768, nz_override=16384), "grid too large"),
-    (dict(model="emhd", xz_variant=4), "xz_variant"),
+    (dict(model="emhd", xz_variant=7), "xz_variant"),
 ])
 def test_constraint_errors_are_codes_not_exits(keys, msg):
     with pytest.raises(khs.KhsError) as e:

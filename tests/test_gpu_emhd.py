@@ -137,7 +137,7 @@ def test_tile_variants_bitwise(zc, pf, zg):
         assert_bitwise(sim.get(n), o.get(n), f"z{zc} pf{pf} {n}")
 
 
-@pytest.mark.parametrize("xzv", [0, 1, 2, 3])
+@pytest.mark.parametrize("xzv", [0, 1, 2, 3, 4, 5, 6])
 @pytest.mark.parametrize("shape,periodic,nsp", [((130, 70), 0, 2), ((200, 96), 1, 1), ((64, 40), 1, 2),
                                                 ((116, 150), 0, 1)])
 def test_fused_sweep_variants_bitwise(xzv, shape, periodic, nsp):
