@@ -48,79 +48,91 @@ __device__ __forceinline__ d4 mfma16(double a, double b, d4 c) {
 constexpr int KF_ROWS = 16;        // mode rows (cos 1..K, sin 1..K, K <= 8)
 constexpr int KF_ZR = 256;         // z rows per synthesis block
 
-// Coefficients of one 64-column strip of one array; the 4 waves split z.
-template <typename T>
+// Coefficients of one strip of 64 * Q columns of one array; the 4 waves split
+// z.  Lane n of a 16-lane row group holds columns x0 + 4 (Q n + q) + j
+// (q < Q, j < 4) with Q 16-byte loads, tile 4 q + j.
+template <typename T, int Q>
 __global__ __launch_bounds__(256) void k_keep_coef_mfma(ArrayList<T> arrs, const T* __restrict__ tab,
                                                         T* __restrict__ coef, int nx, int nz) {
     typedef typename V4<T>::type v4;
-    __shared__ T red[4][KF_ROWS][64];
-    __shared__ T redm[4][4][64];
+    constexpr int W = 64 * Q;                      // columns per strip
+    __shared__ T red[4][KF_ROWS][W];
+    __shared__ T redm[4][4][W];
     const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int n = l & 15, kq = l >> 4;
-    const int x0 = blockIdx.x * 64;
+    const int x0 = blockIdx.x * W;
     const int a = blockIdx.y;
     const T* __restrict__ v = arrs.a[a];
-    const int col = x0 + 4 * n;
-    const bool ok = col < nx;                      // nx % 4 == 0: all four or none
-    const v4 zero = v4{T(0), T(0), T(0), T(0)};
-    v4 acc[4];
+    bool ok[Q];                                    // nx % 4 == 0: all four or none
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[j] = zero;
-    v4 msum = zero;
+    for (int q = 0; q < Q; ++q) ok[q] = x0 + 4 * (Q * n + q) < nx;
+    const v4 zero = v4{T(0), T(0), T(0), T(0)};
+    v4 acc[4 * Q];
+#pragma unroll
+    for (int j = 0; j < 4 * Q; ++j) acc[j] = zero;
+    v4 msum[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) msum[q] = zero;
     const int zper = nz >> 2;                      // nz % 16 == 0
     const int zb = wave * zper, ze = zb + zper;
+    auto ld = [&](int z, int q) {
+        return ok[q] ? *reinterpret_cast<const v4*>(v + (size_t)z * nx + x0 + 4 * (Q * n + q)) : zero;
+    };
+    auto step = [&](T at, const v4 (&b)[Q]) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            acc[4 * q + 0] = mfma16(at, b[q].x, acc[4 * q + 0]);
+            acc[4 * q + 1] = mfma16(at, b[q].y, acc[4 * q + 1]);
+            acc[4 * q + 2] = mfma16(at, b[q].z, acc[4 * q + 2]);
+            acc[4 * q + 3] = mfma16(at, b[q].w, acc[4 * q + 3]);
+            msum[q] += b[q];
+        }
+    };
     // four 4-row steps per iteration with their loads issued first (more
     // loads in flight per wave); the MFMA and sum order is the one-step order
     int z0 = zb;
     for (; z0 + 16 <= ze; z0 += 16) {
-        v4 b[4];
+        v4 b[4][Q];
         T at[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int z = z0 + 4 * u + kq;
-            b[u] = ok ? *reinterpret_cast<const v4*>(v + (size_t)z * nx + col) : zero;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) b[u][q] = ld(z, q);
             at[u] = tab[(size_t)z * KF_ROWS + n];              // A[i = n][k = kq]
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            acc[0] = mfma16(at[u], b[u].x, acc[0]);
-            acc[1] = mfma16(at[u], b[u].y, acc[1]);
-            acc[2] = mfma16(at[u], b[u].z, acc[2]);
-            acc[3] = mfma16(at[u], b[u].w, acc[3]);
-            msum += b[u];
-        }
+        for (int u = 0; u < 4; ++u) step(at[u], b[u]);
     }
     for (; z0 < ze; z0 += 4) {
         const int z = z0 + kq;
-        const v4 b = ok ? *reinterpret_cast<const v4*>(v + (size_t)z * nx + col) : zero;
-        const T at = tab[(size_t)z * KF_ROWS + n];
-        acc[0] = mfma16(at, b.x, acc[0]);
-        acc[1] = mfma16(at, b.y, acc[1]);
-        acc[2] = mfma16(at, b.z, acc[2]);
-        acc[3] = mfma16(at, b.w, acc[3]);
-        msum += b;
+        v4 b[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) b[q] = ld(z, q);
+        step(tab[(size_t)z * KF_ROWS + n], b);
     }
-    // lane holds, for tile j, rows drow(kq, e) of column 4 n + j
+    // lane holds, for tile 4 q + j, rows drow(kq, e) of column 4 (Q n + q) + j
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int q = 0; q < Q; ++q)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) red[wave][drow(T(0), kq, e)][4 * n + j] = acc[j][e];
+        for (int j = 0; j < 4; ++j) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) redm[wave][kq][4 * n + j] = msum[j];
+            for (int e = 0; e < 4; ++e) red[wave][drow(T(0), kq, e)][4 * (Q * n + q) + j] = acc[4 * q + j][e];
+            redm[wave][kq][4 * (Q * n + q) + j] = msum[q][j];
+        }
     __syncthreads();
     const T w = T(2) / T(nz), w0 = T(1) / T(nz);
     const size_t base = (size_t)a * (KF_ROWS + 1);
-    for (int o = threadIdx.x; o < KF_ROWS * 64; o += 256) {
-        const int r = o >> 6, c = o & 63;
+    for (int o = threadIdx.x; o < KF_ROWS * W; o += 256) {
+        const int r = o / W, c = o % W;
         if (x0 + c >= nx) continue;
         const T s = (red[0][r][c] + red[1][r][c]) + (red[2][r][c] + red[3][r][c]);
         coef[(base + 1 + r) * nx + x0 + c] = s * w;
     }
-    if (threadIdx.x < 64) {
-        const int c = threadIdx.x;
+    for (int c = threadIdx.x; c < W; c += 256) {
         if (x0 + c < nx) {
             T s = T(0);
-            for (int q = 0; q < 16; ++q) s += redm[q >> 2][q & 3][c];
+            for (int qq = 0; qq < 16; ++qq) s += redm[qq >> 2][qq & 3][c];
             coef[base * nx + x0 + c] = s * w0;
         }
     }
@@ -195,7 +207,12 @@ void fill_keep_mfma_table(int nz, int K, T* tab) {
 template <typename T>
 void launch_keep_mfma(const ArrayList<T>& arrs, const T* tab, T* coef, const KParams& p, hipStream_t st) {
     const int nxb = (p.nx + 63) / 64;
-    hipLaunchKernelGGL(k_keep_coef_mfma<T>, dim3(nxb, arrs.n), dim3(256), 0, st, arrs, tab, coef, p.nx, p.nz);
+    // 128-column strips: a row group reads 512 (FP32) or 1024 (FP64) contiguous
+    // bytes per row; 4096x2048, 21 arrays: FP32 pass 7% faster than 64-column
+    // strips and than 256-column ones (LDS-bound occupancy), FP64 2%
+    constexpr int Q = 2;
+    hipLaunchKernelGGL((k_keep_coef_mfma<T, Q>), dim3((p.nx + 64 * Q - 1) / (64 * Q), arrs.n), dim3(256), 0, st,
+                       arrs, tab, coef, p.nx, p.nz);
     hipLaunchKernelGGL(k_keep_synth_mfma<T>, dim3(nxb, arrs.n, (p.nz + KF_ZR - 1) / KF_ZR), dim3(256), 0, st,
                        arrs, tab, coef, p.nx, p.nz);
 }
