@@ -22,20 +22,25 @@ namespace khs {
 // --------------------------------------------------------------------------
 // sponge / xi_cut
 // --------------------------------------------------------------------------
+// One thread per column and CF_ROWS rows: the column's action is read once,
+// columns without one return at once (no per-element index arithmetic).
+constexpr int CF_ROWS = 16;
+
 template <typename T>
-__global__ void k_cell_filters(ArrayList<T> arrs, const double* decay, const int* action,
-                               KParams p) {
-    const size_t N = (size_t)p.nx * p.nz;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N;
-         i += (size_t)gridDim.x * blockDim.x) {
-        const int x = (int)(i % p.nx);
-        const int act = action[x];
-        if (act == 0) continue;
+__global__ __launch_bounds__(256) void k_cell_filters(ArrayList<T> arrs, const double* decay, const int* action,
+                                                      KParams p) {
+    const int x = blockIdx.x * 256 + (int)threadIdx.x;
+    if (x >= p.nx) return;
+    const int act = action[x];
+    if (act == 0) return;
+    const int z0 = blockIdx.y * CF_ROWS, z1 = min(p.nz, z0 + CF_ROWS);
+    const T d = (act == 1) ? T(decay[x]) : T(0.0);
+    for (int k = 0; k < arrs.n; ++k) {
+        T* a = arrs.a[k] + x;
         if (act == 1) {
-            const T d = T(decay[x]);
-            for (int k = 0; k < arrs.n; ++k) arrs.a[k][i] *= d;
+            for (int z = z0; z < z1; ++z) a[(size_t)z * p.nx] *= d;
         } else {
-            for (int k = 0; k < arrs.n; ++k) arrs.a[k][i] = T(0.0);
+            for (int z = z0; z < z1; ++z) a[(size_t)z * p.nx] = T(0.0);
         }
     }
 }
@@ -43,9 +48,8 @@ __global__ void k_cell_filters(ArrayList<T> arrs, const double* decay, const int
 template <typename T>
 void launch_cell_filters(const ArrayList<T>& arrs, const double* decay, const int* action,
                          const KParams& p, hipStream_t st) {
-    size_t N = (size_t)p.nx * p.nz;
-    int blocks = (int)std::min<size_t>((N + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_cell_filters<T>, dim3(blocks), dim3(256), 0, st, arrs, decay, action, p);
+    hipLaunchKernelGGL(k_cell_filters<T>, dim3((p.nx + 255) / 256, (p.nz + CF_ROWS - 1) / CF_ROWS), dim3(256), 0, st,
+                       arrs, decay, action, p);
 }
 
 // --------------------------------------------------------------------------
