@@ -28,7 +28,7 @@ def keys(nx, nz, K, mfma=1, precision="fp32", **extra):
 
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
 @pytest.mark.parametrize("mfma", [1, 0])
-@pytest.mark.parametrize("shape,K", [((128, 64), 4), ((200, 48), 8), ((256, 256), 8), ((96, 128), 1)])
+@pytest.mark.parametrize("shape,K", [((128, 64), 4), ((200, 48), 8), ((256, 256), 8), ((96, 128), 1), ((200, 512), 8)])
 def test_keep_filter_vs_oracle(shape, K, mfma, precision):
     nx, nz = shape
     tol = 1e-5 if precision == "fp32" else 1e-12
