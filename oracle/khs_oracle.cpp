@@ -26,6 +26,7 @@
 #include <omp.h>
 #include <cstdint>
 #include <cstdlib>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 #include <algorithm>
@@ -672,7 +673,13 @@ double energy(Oracle& o) {                          // YM_Energy.cu:23-32 (seria
     return total;
 }
 
-// Gauss residual, analysis/gauss_check.py:40-115 with the state's own Az1.
+// Gauss residual, analysis/gauss_check.py:40-115.  One deliberate departure:
+// Az1 is the state's own Az1, the array the step itself couples to, where
+// gauss_check.py:59-61 rebuilds it from the snapshot CSV (which omits Az1) as
+// the analytic -V0 log cosh at (i + 1/2) DX on a fixed Lx = 6 pi grid.  For the
+// frozen background the two differ by the half-cell shift (and by the grid
+// the script assumes); with Az1 evolving (freeze_override = 0) only the
+// state's value is meaningful.
 void gauss(Oracle& o, double* out) {
     const int nx = o.p.nx, nz = o.p.nz;
     const double DX = o.p.dx, DZ = o.p.dz, al = o.p.alpha;
@@ -908,6 +915,44 @@ void orc_init(void* h, int k_mode, double perturb_amp, double V0, double epsilon
             o.A(NB)[idx] = 1.0; o.A(PXB)[idx] = 0.0; o.A(PZB)[idx] = -vz_A;
             o.A(Q1B)[idx] = -1.0; o.A(Q2B)[idx] = qB_amp * cz; o.A(Q3B)[idx] = qB_amp * sz;
         }
+}
+
+// Eigenfunction seed file (main_ym.cu:178-217): int32 n_fields, int32
+// nx_file, then n_fields profiles of nx_file float32 values in the order by,
+// ex, ez, az, qA, qB (at most 6 are read).  Each profile is resampled onto
+// NX points by linear interpolation in float arithmetic: t = i (nx_file-1) /
+// (NX-1), j = min(floor t, nx_file-2), value = f[j] (1-frac) + f[j+1] frac.
+// out holds 6 x NX doubles; present[f] is 1 for the profiles the file holds.
+// Returns 0, or -1 for a missing, malformed or truncated file.
+int orc_seed_profiles(const char* path, int NX, double* out, int* present) {
+    for (int f = 0; f < 6; ++f) present[f] = 0;
+    FILE* fp = std::fopen(path, "rb");
+    if (!fp) return -1;
+    int32_t hdr[2] = {0, 0};
+    if (std::fread(hdr, sizeof(int32_t), 2, fp) != 2 || hdr[0] < 1 || hdr[1] < 2 || NX < 2) {
+        std::fclose(fp);
+        return -1;
+    }
+    const int nfile = hdr[1];
+    std::vector<float> prof(nfile);
+    for (int f = 0; f < std::min(hdr[0], 6); ++f) {
+        if (std::fread(prof.data(), sizeof(float), nfile, fp) != (size_t)nfile) {
+            std::fclose(fp);
+            return -1;
+        }
+        const float span = (float)(nfile - 1), den = (float)(NX - 1);
+        for (int i = 0; i < NX; ++i) {
+            const float t = ((float)i * span) / den;
+            const int j = std::min((int)t, nfile - 2);
+            const float fr = t - (float)j;
+            const float lo = prof[j] * (1.0f - fr);
+            const float hi = prof[j + 1] * fr;
+            out[(size_t)f * NX + i] = (double)(lo + hi);
+        }
+        present[f] = 1;
+    }
+    std::fclose(fp);
+    return 0;
 }
 
 // Counter-based white noise (extension key noise_amp/noise_seed): value for

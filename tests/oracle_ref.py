@@ -99,6 +99,8 @@ def load():
     lib.orc_stream_triad.restype = d
     lib.orc_init.argtypes = [v, i, d, d, d, i, d, i, d, dp, dp, dp, dp]
     lib.orc_noise.restype = d
+    lib.orc_seed_profiles.restype = i
+    lib.orc_seed_profiles.argtypes = [ctypes.c_char_p, i, dp, ctypes.POINTER(ctypes.c_int)]
     lib.orc_noise.argtypes = [ctypes.c_uint64, i, ctypes.c_uint64, d]
     _lib = lib
     return lib
@@ -219,6 +221,24 @@ def kz_band_column(col, lo, hi):
 
 def noise(seed, aid, cell, amp):
     return load().orc_noise(seed, aid, cell, amp)
+
+
+SEED_FIELDS = ["by", "ex", "ez", "az", "qA", "qB"]
+
+
+def seed_profiles(path, nx):
+    """Seed-file profiles resampled onto nx points (main_ym.cu:178-217), as
+    {field: array} for the fields the file holds; None if it cannot be read."""
+    out = np.zeros(6 * nx)
+    present = (ctypes.c_int * 6)()
+    if load().orc_seed_profiles(os.fsencode(path), nx, _ptr(out), present) != 0:
+        return None
+    return {f: out[k * nx:(k + 1) * nx].copy() for k, f in enumerate(SEED_FIELDS) if present[k]}
+
+
+def init_seed_args(prof):
+    """orc_init keyword arguments from seed_profiles() output."""
+    return {"seed_" + f: prof.get(f) for f in ("by", "az", "qA", "qB")}
 
 
 # --------------------------------------------------------------------------

@@ -55,6 +55,31 @@ def test_init_bitwise(mode):
         assert_bitwise(sim.get(n), want, f"mode {mode} {n}")
 
 
+@pytest.mark.parametrize("nx", [96, 77, 300])
+def test_seeded_init_resampled_bitwise(tmp_path, nx):
+    # run_mode 6 with an eigenfunction seed file of 77 points resampled onto
+    # NX (main_ym.cu:178-217): the library's reader + init against the
+    # oracle's reader + init, bit for bit
+    rng = np.random.default_rng(nx)
+    prof = rng.uniform(-1, 1, (6, 77)).astype(np.float32)
+    path = tmp_path / "seed.bin"
+    with open(path, "wb") as f:
+        np.array([6, 77], dtype=np.int32).tofile(f)
+        prof.tofile(f)
+    keys = dict(C1, run_mode=6, nx_override=nx, nz_override=64, lx_override=nx * 0.1, noise_amp=0.0,
+                perturb_amp=0.01)
+    cfg, sim = make_sim(**keys)
+    sim.init(str(path))                # again, from the seed file
+    o = orc.Oracle(orc.params_from_config(cfg))
+    f32 = np.float32
+    o.init(cfg.k_mode, float(f32(cfg.perturb_amp)), float(f32(cfg.V0)), cfg.eps, cfg.run_mode,
+           float(f32(cfg.alpha)), cfg.init_by1_eq, float(f32(cfg.vz_edge_taper)),
+           **orc.init_seed_args(orc.seed_profiles(str(path), cfg.nx)))
+    for n in orc.STATE:
+        assert_bitwise(sim.get(n), o.get(n), f"seeded nx {nx} {n}")
+    assert np.any(sim.get("Q2B") != 0)
+
+
 @pytest.mark.parametrize("periodic", [0, 1])
 @pytest.mark.parametrize("shape", SHAPES)
 def test_staged_operators_bitwise(periodic, shape):

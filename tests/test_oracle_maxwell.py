@@ -177,3 +177,99 @@ def test_gauss_closed_form():
     assert abs(g[0] - np.sqrt(np.mean(div ** 2))) < 1e-14
     assert g[1] == 0.0 and g[2] == 0.0
     assert abs(g[6] - np.sqrt(np.mean(np.abs(ex) ** 2)) / dx) < 1e-14
+
+
+def _gauss_state(nx, nz):
+    st = {n: np.zeros((nz, nx)) for n in orc.STATE}
+    st["nA"] = np.ones((nz, nx))
+    st["nB"] = np.ones((nz, nx))
+    return st
+
+
+def test_gauss_non_abelian_term_of_uniform_fields():
+    # gauss_check.py:74-84: with uniform Az^b and Ez^c (no divergence) the
+    # non-Abelian term is the constant alpha eps^{abc} Az^b Ez^c.  Az = (b1,
+    # a2, 0) and Ez = (e1, 0, e3) give nab = alpha (a2 e3, -b1 e3, -a2 e1).
+    # Without charge G^a = nab^a (rms = |nab^a|); with rho^a = nab^a planted
+    # through Q^a_A (nA = 1) G^a is exactly 0 with the reference's sign and
+    # 2 nab^a with rho = -nQ.  A transposed index or the sign of any one term
+    # fails one of these.
+    nx, nz, al = 8, 6, 1.5
+    a2, b1, e1, e3 = 0.7, -0.3, 0.4, 2.0
+    nab = np.array([al * (a2 * e3), al * (-(b1 * e3)), al * (-(a2 * e1))])
+    o = orc.Oracle(orc.params(nx, nz, 0.1, 0.2, 0.04, alpha=al, periodic_x=1))
+    one = np.ones((nz, nx))
+    st = _gauss_state(nx, nz)
+    st.update(Az1=b1 * one, Az2=a2 * one, Ez1=e1 * one, Ez3=e3 * one)
+    o.set_state(st)
+    g = o.gauss()
+    np.testing.assert_allclose(g[:3], np.abs(nab), rtol=1e-15, atol=0)
+    np.testing.assert_allclose(g[3:6], g[:3], rtol=0, atol=0)       # rho = 0: both sign conventions agree
+    assert abs(g[6] - (e1 + e3) / 0.1) < 1e-12                      # Etot_scale = rms(sum |E|) / dx
+    st.update(Q1A=nab[0] * one, Q2A=nab[1] * one, Q3A=nab[2] * one, Q1B=0 * one)
+    o.set_state(st)
+    g = o.gauss()
+    assert np.all(g[:3] == 0.0)
+    np.testing.assert_allclose(g[3:6], 2 * np.abs(nab), rtol=1e-15, atol=0)
+
+
+def test_gauss_centres_ez_from_the_face_below():
+    # Ez^c lives at z + 1/2; gauss_check.py:71-72 averages it onto the cell
+    # centre from the faces z - 1/2 and z + 1/2: 0.5 (Ez[z] + Ez[z-1]).  With
+    # Az^2 = a at one cell only, G^1 is nonzero in that cell only, so its rms
+    # fixes alpha a 0.5 (Ez3[z0] + Ez3[z0-1]) and tells z-1 from z+1.
+    nx, nz, al, a = 6, 8, 1.0, 2.0
+    rng = np.random.default_rng(3)
+    ez3_col = rng.uniform(-1.0, 1.0, nz)
+    o = orc.Oracle(orc.params(nx, nz, 0.1, 0.1, 0.04, alpha=al, periodic_x=1))
+    st = _gauss_state(nx, nz)
+    st["Ez3"] = np.tile(ez3_col[:, None], (1, nx))
+    x0, z0 = 2, 5
+    st["Az2"][z0, x0] = a
+    o.set_state(st)
+    g = o.gauss()
+    want = abs(al * a * 0.5 * (ez3_col[z0] + ez3_col[z0 - 1])) / np.sqrt(nx * nz)
+    wrong = abs(al * a * 0.5 * (ez3_col[z0] + ez3_col[(z0 + 1) % nz])) / np.sqrt(nx * nz)
+    assert abs(want - wrong) > 1e-3
+    assert abs(g[0] - want) < 1e-15
+    # colour 3 sees only the z divergence of Ez3 (a column profile): its
+    # backward difference with the periodic wrap of np.roll (gauss_check.py:65-67)
+    div = (ez3_col - np.roll(ez3_col, 1)) / 0.1
+    assert abs(g[2] - np.sqrt(np.mean(div ** 2))) < 1e-13
+
+
+def test_gauss_charge_sign_and_both_conventions():
+    # rho^a = Q^a_A nA + Q^a_B nB (gauss_check.py:87-89).  nA = 2, nB = 3,
+    # Q^1 = (+1, -1) gives rho^1 = -1; Q^2 = (0.5, 0.25) gives rho^2 = +1.75.
+    # A uniform non-Abelian term alpha Az^2 Ez^3 = +1 on colour 1 makes the
+    # sign visible: G^1 = 1 - rho^1 = 2 with the reference's sign and
+    # 1 + rho^1 = 0 with the continuity-consistent one (rho = -nQ).
+    nx, nz = 5, 4
+    o = orc.Oracle(orc.params(nx, nz, 0.1, 0.1, 0.04, alpha=1.0, periodic_x=1))
+    one = np.ones((nz, nx))
+    st = _gauss_state(nx, nz)
+    st.update(nA=2.0 * one, nB=3.0 * one, Q1A=one, Q1B=-one, Q2A=0.5 * one, Q2B=0.25 * one,
+              Az2=0.5 * one, Ez3=2.0 * one)
+    o.set_state(st)
+    g = o.gauss()
+    # colour 1: nab = +1, rho = -1;  colour 2: nab = -Az1 Ez3 = 0, rho = +1.75
+    np.testing.assert_allclose(g[:3], [2.0, 1.75, 0.0], rtol=1e-15, atol=0)
+    np.testing.assert_allclose(g[3:6], [0.0, 1.75, 0.0], rtol=1e-15, atol=0)
+
+
+def test_gauss_divergence_is_backward_in_x_with_periodic_wrap():
+    # Ex^a at x + 1/2: dEx/dx at cell x is (Ex[x] - Ex[x-1]) / DX, with x - 1
+    # wrapped (np.roll, gauss_check.py:65-66).  A single nonzero column Ex^2
+    # = v at x0 = nx - 1 gives +v/DX at x0 and, through the wrap, -v/DX at
+    # x = 0.  A charge rho^2 = -v/DX at x = 0 cancels the second exactly, so
+    # the rms is (v/DX)/sqrt(nx); a forward difference or a missing wrap
+    # leaves two or three nonzero columns instead.
+    nx, nz, dx, v = 7, 3, 0.25, 0.8
+    o = orc.Oracle(orc.params(nx, nz, dx, 0.1, 0.04, alpha=1.0, periodic_x=1))
+    st = _gauss_state(nx, nz)
+    x0 = nx - 1
+    st["Ex2"][:, x0] = v
+    st["Q2A"][:, 0] = -v / dx
+    o.set_state(st)
+    g = o.gauss()
+    assert abs(g[1] - (v / dx) / np.sqrt(nx)) < 1e-14
