@@ -212,9 +212,14 @@ void config_load(const std::string& path, khs_config& c, std::string* warnings) 
 // emhdgpu2d run (main_coupled.cu:228-260): 256x256, dx = dz = 1, dt = 0.002,
 // energy check every 500 steps, snapshot every 1000, halt at E/E0 > 5.
 // The kernels address each field array with 32-bit byte offsets.
+// Kernels address every array through 32-bit byte offsets, and halo lanes
+// that must not store use the offset 0xFFFFFFF0, which the buffer's range
+// check drops only while the array is no larger than that: NX*NZ*8 bytes
+// <= 0xFFFFFFF0, i.e. NX*NZ <= 2^29 - 2 cells.
 static void check_grid_bytes(long nx, long nz) {
-    if (nx * nz * (long)sizeof(double) >= (1L << 32))
-        throw Error(KHS_ERR_CONFIG, "grid too large: NX*NZ must stay below 2^29 cells (each field array under 4 GiB)");
+    if (nx * nz * (long)sizeof(double) > 0xFFFFFFF0L)
+        throw Error(KHS_ERR_CONFIG, "grid too large: NX*NZ must be at most 2^29 - 2 cells (FP64 arrays of at most "
+                                    "0xFFFFFFF0 bytes)");
 }
 
 static void config_derive_emhd(khs_config& c) {

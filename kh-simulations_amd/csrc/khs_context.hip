@@ -75,6 +75,15 @@ static std::mutex& tune_cache_mutex() {
     static std::mutex m;
     return m;
 }
+// Stream capture and graph instantiation are serialised over the process:
+// host threads that drive separate contexts (the concurrent batch runner,
+// callers with one context per thread) then never have two captures or
+// instantiations in flight inside the HIP runtime at once.  Launching and
+// synchronising captured graphs stays concurrent.
+static std::mutex& capture_mutex() {
+    static std::mutex m;
+    return m;
+}
 
 struct Ctx {
     khs_config cfg{};
@@ -739,9 +748,19 @@ static void capture_stride(Ctx& c, int stride) {
     HIPCHK(hipStreamSynchronize(c.stream));
     const int cur0 = c.cur;
     const long step0 = c.step;
+    std::lock_guard<std::mutex> lk(capture_mutex());
     HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeRelaxed));
-    for (int s = 0; s < stride; ++s) full_step(c);
-    c.energy_async();
+    try {
+        for (int s = 0; s < stride; ++s) full_step(c);
+        c.energy_async();
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        hipStreamEndCapture(c.stream, &g);       // leave the stream usable
+        if (g) hipGraphDestroy(g);
+        c.cur = cur0;
+        c.step = step0;
+        throw;
+    }
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamEndCapture(c.stream, &g));
     c.cur = cur0;                               // the graph does the stepping
@@ -814,6 +833,7 @@ struct Batch {
         if (set.empty()) return;
         for (int i : set) runs[i]->autotune();          // tuning launches and waits: before capture
         for (int i : set) HIPCHK(hipStreamSynchronize(runs[i]->stream));
+        std::lock_guard<std::mutex> lk(capture_mutex());
         HIPCHK(hipStreamBeginCapture(origin, hipStreamCaptureModeRelaxed));
         HIPCHK(hipEventRecord(ev[0], origin));
         for (auto l : lanes) HIPCHK(hipStreamWaitEvent(l, ev[0], 0));
@@ -1718,6 +1738,14 @@ int khs_batch_run_concurrent(khs_batch* b, int nsteps, int threads, int* steps_d
             order.push_back(i);
         }
         std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return cost[a] > cost[c]; });
+        // every run's kernel choice is made here, on this thread, before the
+        // workers start: the first context of each grid shape times its tile
+        // variants (launches, event waits), the others take the cached
+        // choice, so no timing trials run while other threads capture
+        for (int i : order) {
+            B.runs[i]->autotune();
+            HIPCHK(hipStreamSynchronize(B.runs[i]->stream));
+        }
         for (int i : order) {
             const int t = (int)(std::min_element(load.begin(), load.end()) - load.begin());
             part[t].push_back(i);

@@ -94,8 +94,8 @@ __device__ __forceinline__ void st32_nt(T* a, unsigned i, T v) {
 // write take offset kNoStore instead of a branch around the store: a store
 // under a lane-divergent branch makes the compiler's wait counting assume the
 // branch was not taken, and every later wait for an earlier load then also
-// waits for the store to complete.  Arrays stay below 4 GiB (the config
-// checks the grid), so kNoStore is always out of range.
+// waits for the store to complete.  The config admits arrays of at most
+// kNoStore bytes (NX*NZ*8 <= 0xFFFFFFF0), so kNoStore is always out of range.
 constexpr unsigned kNoStore = 0xFFFFFFF0u;
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, unsigned bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);

@@ -91,12 +91,20 @@ def test_output_dir_name_c1():
     (dict(xz_variant=12), "xz_variant"),
     (dict(nx_override=32768, nz_override=16384), "grid too large"),
     (dict(model="emhd", nx_override=32768, nz_override=16384), "grid too large"),
+    (dict(nx_override=64, nz_override=1 << 23, run_mode=6), "grid too large"),   # 2^29 cells: 2^32 bytes
     (dict(model="emhd", xz_variant=7), "xz_variant"),
 ])
 def test_constraint_errors_are_codes_not_exits(keys, msg):
     with pytest.raises(khs.KhsError) as e:
         khs.Config(**keys)
     assert e.value.code == khs.KHS_ERR_CONFIG and msg in str(e.value)
+
+
+def test_largest_grid_keeps_the_store_sentinel_out_of_range():
+    # halo lanes store at byte offset 0xFFFFFFF0, which the buffer range
+    # check must drop: the largest admitted FP64 array is exactly that size
+    c = khs.Config(nx_override=32766, nz_override=16385, run_mode=6)     # 2^29 - 2 cells
+    assert c.nx * c.nz * 8 == 0xFFFFFFF0
 
 
 def test_missing_ini_and_unknown_key():
