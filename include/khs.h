@@ -239,6 +239,23 @@ int khs_get_step(khs_ctx* ctx, int* step, double* time);
 /* ---------------- diagnostics ---------------- */
 /* Total energy (YM_Energy.cu:3-32) as a compensated FP64 device reduction. */
 int khs_energy(khs_ctx* ctx, double* E);
+/* Allocation-free energy reduction on caller-owned device arrays.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc), which compute_ym_energy (ymgpu2d/YM_Energy.cuh:10-12)
+ * cannot serve because its caller must supply an NX*NZ per-cell buffer.
+ * arrays[id] are device pointers in the order of the enum khs_array ids, layout
+ * x + z*nx, double for KHS_FP64 and float for KHS_FP32; the 15 arrays the
+ * energy reads (Ex, Ez, By of every colour, n, px, pz of both species) must
+ * be non-null.  The kernels are enqueued on `stream` (a hipStream_t, NULL =
+ * the null stream) and write the compensated FP64 sum of YM_Energy.cu:3-32
+ * to out_dev[0] (device memory); the call neither allocates nor waits, so it
+ * may be captured into a HIP graph.  The reduction's partial sums live in a
+ * fixed 16 KiB device array of the library (1024 blocks, any grid up to the
+ * library's 2^29 - 2 cells), one per GPU, shared by every call on that GPU:
+ * calls that overlap on one GPU (other streams, other threads) race. */
+int khs_energy_noalloc(const void* const arrays[KHS_N_STATE], int nx, int nz, int precision, void* stream,
+                       double* out_dev);
 /* Initial energy E0 recorded by khs_init. */
 int khs_energy0(khs_ctx* ctx, double* E0);
 /* Gauss-law residual (analysis/gauss_check.py:40-115 formula, computed on the

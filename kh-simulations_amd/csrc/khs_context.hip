@@ -887,6 +887,36 @@ Batch::~Batch() {
     runs.clear();
 }
 
+// --------------------------------------------------------------------------
+// Device scratch of the allocation-free energy entry point: the partial sums
+// of the energy reduction (one compensated pair per reduction block, at most
+// kRedBlocks blocks whatever the grid), 16 KiB per GPU, held until the
+// process ends.  Host code never passes `scratch` itself to a kernel (in host
+// code the name is a host-side stand-in, not a device address): the device
+// address is looked up once per device with hipGetSymbolAddress, with that
+// device current, and used from then on.
+// --------------------------------------------------------------------------
+constexpr size_t kScratchDoubles = (size_t)kRedBlocks * 2;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+static double* g_scratch_dev[kMaxDevices] = {};
+
+static double* scratch_of_current_device() {
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw Error(KHS_ERR_DEVICE, "device ordinal out of range");
+    static std::mutex m;
+    std::lock_guard<std::mutex> lk(m);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw Error(KHS_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_dev[dev] = static_cast<double*>(p);
+    }
+    return g_scratch_dev[dev];
+}
+
 }  // namespace khs
 
 // ==========================================================================
@@ -1237,6 +1267,43 @@ int khs_energy(khs_ctx* ctx, double* E) {
     CTX_OR_FAIL(ctx);
     if (!E) { g_last_error = "null argument"; return KHS_ERR_ARG; }
     return guard([&] { *E = ctx->impl->energy(); });
+}
+
+int khs_energy_noalloc(const void* const arrays[KHS_N_STATE], int nx, int nz, int precision, void* stream,
+                       double* out_dev) {
+    if (!arrays || !out_dev) { g_last_error = "khs_energy_noalloc: null argument"; return KHS_ERR_ARG; }
+    return guard([&] {
+        using namespace khs;
+        if (nx < 1 || nz < 1) throw Error(KHS_ERR_ARG, "khs_energy_noalloc: nx and nz must be >= 1");
+        if ((long)nx * (long)nz * 8L > 0xFFFFFFF0L)
+            throw Error(KHS_ERR_ARG, "khs_energy_noalloc: grid above the library's limit of 2^29 - 2 cells");
+        if (precision != KHS_FP64 && precision != KHS_FP32)
+            throw Error(KHS_ERR_ARG, "khs_energy_noalloc: precision must be KHS_FP64 or KHS_FP32");
+        static const int used[15] = {S_EX1, S_EX2, S_EX3, S_EZ1, S_EZ2, S_EZ3, S_BY1, S_BY2, S_BY3,
+                                     S_NA, S_PXA, S_PZA, S_NB, S_PXB, S_PZB};
+        for (int id : used)
+            if (!arrays[id]) throw Error(KHS_ERR_ARG, "khs_energy_noalloc: array " + std::to_string(id) + " is null");
+        const size_t N = (size_t)nx * nz;
+        const size_t blocks = std::min<size_t>((N + kRedThreads - 1) / kRedThreads, kRedBlocks);
+        if (blocks * 2 > kScratchDoubles)
+            throw Error(KHS_ERR_ARG, "khs_energy_noalloc: reduction needs more partial sums than the scratch holds");
+        double* partials = scratch_of_current_device();
+        KParams kp{};
+        kp.nx = nx;
+        kp.nz = nz;
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        if (precision == KHS_FP32) {
+            State<float> S{};
+            for (int i = 0; i < S_COUNT; ++i) S.a[i] = static_cast<float*>(const_cast<void*>(arrays[i]));
+            launch_energy<float>(S, kp, partials, out_dev, st);
+        } else {
+            State<double> S{};
+            for (int i = 0; i < S_COUNT; ++i) S.a[i] = static_cast<double*>(const_cast<void*>(arrays[i]));
+            launch_energy<double>(S, kp, partials, out_dev, st);
+        }
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) throw Error(KHS_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+    });
 }
 
 int khs_energy0(khs_ctx* ctx, double* E0) {

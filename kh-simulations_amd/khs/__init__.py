@@ -159,6 +159,7 @@ def _signatures():
         "khs_batch_mode_amplitude": (c_int, [c_void_p, c_int, c_int, dp]),
         "khs_batch_set_step_limit": (c_int, [c_void_p, c_int, c_int]),
         "khs_batch_run_concurrent": (c_int, [c_void_p, c_int, c_int, P(c_int), P(c_int)]),
+        "khs_energy_noalloc": (c_int, [P(c_void_p), c_int, c_int, c_int, c_void_p, c_void_p]),
     }
 
 
