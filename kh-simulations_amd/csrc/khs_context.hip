@@ -132,8 +132,7 @@ struct Ctx {
     bool fused = false;                 // khs_step runs the fused x+z sweep
     int zmvar = 0;                            // 0: split z sweep + Maxwell; 1..3: fused kernel
     // one energy stride of steps + the energy reduction, captured once
-    hipGraphExec_t sgraph = nullptr;
-    int sgraph_cur = -1;
+    hipGraphExec_t sgraph[2] = {nullptr, nullptr};   // by the ping-pong buffer a stride starts from
     bool tuned = false;
 
     // step state
@@ -501,7 +500,8 @@ struct Ctx {
 
 Ctx::~Ctx() {
     if (stream) hipStreamSynchronize(stream);
-    if (sgraph) hipGraphExecDestroy(sgraph);
+    for (auto g : sgraph)
+        if (g) hipGraphExecDestroy(g);
     for (auto& p : pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
     for (auto e : evpool) hipEventDestroy(e);
     if (dmem) hipFree(dmem);
@@ -739,11 +739,15 @@ static void commit(Ctx& c) {
 static void full_step(Ctx& c);
 
 // Capture `stride` full steps and the energy reduction (into the pinned
-// h_red[0]) as one HIP graph, starting from ping-pong buffer c.cur; stride
-// is even, so every replay starts and ends on the same buffer.
+// h_red[0]) as one HIP graph, starting from ping-pong buffer c.cur, into
+// c.sgraph[c.cur].  An even stride starts and ends on the same buffer; an odd
+// one alternates, so it is captured once from each buffer (the tile-walk
+// direction of each step, set by the step's parity, is the same on every
+// replay because buffer and parity alternate together).
 static void capture_stride(Ctx& c, int stride) {
-    if (c.sgraph) hipGraphExecDestroy(c.sgraph);
-    c.sgraph = nullptr;
+    hipGraphExec_t& slot = c.sgraph[c.cur];
+    if (slot) hipGraphExecDestroy(slot);
+    slot = nullptr;
     c.autotune();                               // tuning launches and waits: before capture
     HIPCHK(hipStreamSynchronize(c.stream));
     const int cur0 = c.cur;
@@ -765,13 +769,18 @@ static void capture_stride(Ctx& c, int stride) {
     HIPCHK(hipStreamEndCapture(c.stream, &g));
     c.cur = cur0;                               // the graph does the stepping
     c.step = step0;
-    hipError_t e = hipGraphInstantiate(&c.sgraph, g, nullptr, nullptr, 0);
+    hipError_t e = hipGraphInstantiate(&slot, g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
     if (e != hipSuccess) {
-        c.sgraph = nullptr;
+        slot = nullptr;
         throw Error(KHS_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     }
-    c.sgraph_cur = cur0;
+}
+
+// Whole energy strides run as one captured graph (no per-kernel launch cost
+// on the host); kernel timing needs the eager path.
+static bool stride_graphs(const Ctx& c) {
+    return c.cfg.step_graphs && !c.timing && c.cfg.energy_stride >= 2;
 }
 
 static void full_step(Ctx& c) {
@@ -1222,15 +1231,14 @@ int khs_step(khs_ctx* ctx, int nsteps, int* halted) {
         Ctx& c = *ctx->impl;
         khs::require_idle(c, "khs_step");
         const int stride = c.cfg.energy_stride;
-        // whole energy strides run as one captured graph (no per-kernel launch
-        // cost on the host); kernel timing needs the eager path
-        const bool graphs = c.cfg.step_graphs && !c.timing && stride >= 2 && stride % 2 == 0;
+        const bool graphs = khs::stride_graphs(c);
         for (int s = 0; s < nsteps;) {
             if (graphs && c.step % stride == 0 && nsteps - s >= stride) {
-                if (!c.sgraph || c.sgraph_cur != c.cur) khs::capture_stride(c, stride);
-                HIPCHK(hipGraphLaunch(c.sgraph, c.stream));
+                if (!c.sgraph[c.cur]) khs::capture_stride(c, stride);
+                HIPCHK(hipGraphLaunch(c.sgraph[c.cur], c.stream));
                 HIPCHK(hipStreamSynchronize(c.stream));
                 c.step += stride;
+                c.cur ^= (stride & 1);
                 s += stride;
                 const double E = c.h_red[0];
                 khs::write_energy_row(c, c.step, E);
@@ -1805,13 +1813,37 @@ int khs_batch_run_concurrent(khs_batch* b, int nsteps, int threads, int* steps_d
             order.push_back(i);
         }
         std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return cost[a] > cost[c]; });
-        // every run's kernel choice is made here, on this thread, before the
+        // Everything but replaying happens here, on this thread, before the
         // workers start: the first context of each grid shape times its tile
-        // variants (launches, event waits), the others take the cached
-        // choice, so no timing trials run while other threads capture
+        // variants (the others take the cached choice), and every run's
+        // energy-stride graph is captured and instantiated.  The workers then
+        // only launch graphs (and eager steps for a stride's remainder) on
+        // their runs' streams; no stream capture runs while another thread
+        // launches work.
         for (int i : order) {
-            B.runs[i]->autotune();
-            HIPCHK(hipStreamSynchronize(B.runs[i]->stream));
+            Ctx& c = *B.runs[i];
+            c.autotune();
+            HIPCHK(hipStreamSynchronize(c.stream));
+            const int stride = c.cfg.energy_stride;
+            const long todo = std::min<long>(nsteps, B.cap[i] - c.step);
+            if (khs::stride_graphs(c) && todo >= stride && c.step % stride == 0) {
+                if (!c.sgraph[c.cur]) khs::capture_stride(c, stride);
+                if ((stride & 1) && todo >= 2 * stride && !c.sgraph[1 - c.cur]) {
+                    // the graph of the next stride, captured from the buffer
+                    // and step parity it will start on
+                    c.cur ^= 1;
+                    c.step += stride;
+                    try {
+                        khs::capture_stride(c, stride);
+                    } catch (...) {
+                        c.cur ^= 1;
+                        c.step -= stride;
+                        throw;
+                    }
+                    c.cur ^= 1;
+                    c.step -= stride;
+                }
+            }
         }
         for (int i : order) {
             const int t = (int)(std::min_element(load.begin(), load.end()) - load.begin());

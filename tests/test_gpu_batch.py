@@ -99,3 +99,22 @@ def test_batch_concurrent_matches_single_runs():
         assert batch.energy(i)[0] == s.energy()
         s.close()
     batch.close()
+
+
+def test_batch_concurrent_odd_stride_matches_eager():
+    """An odd energy stride (the Campaign-25 geometry has 203): the concurrent
+    runner's pre-captured graphs from both ping-pong buffers give the bits of
+    eager stepping."""
+    cfgs = [dict(c, energy_stride=7) for c in small_configs()[:3]]
+    batch = khs.Batch(cfgs, lanes=2, segment=10)
+    batch.init()
+    done, halted = batch.run_concurrent(40, threads=3)
+    assert done == [40] * 3 and not any(halted)
+    for i, c in enumerate(cfgs):
+        s = khs.Simulation(khs.Config(**c, step_graphs=0))
+        s.init()
+        s.step(40)
+        for n in ("PzA", "Q2B", "By2", "Ez3", "Az2", "nB"):
+            assert_bitwise(batch.get(i, n), s.get(n), f"odd-stride run {i} {n}")
+        s.close()
+    batch.close()

@@ -1,8 +1,9 @@
 """khs_step replays whole energy strides as one captured HIP graph: the
 states, step counts, energy rows and halting must be those of the eager
 launches, over several replays, from strides that start unaligned, across a
-re-capture after the ping-pong parity changes, and with every filter kind
-(none of which may leave a memset node in the graph)."""
+re-capture after the ping-pong parity changes, with odd strides (a graph
+from each ping-pong buffer) and with every filter kind (none of which may
+leave a memset node in the graph)."""
 import numpy as np
 import pytest
 
@@ -18,6 +19,7 @@ CASES = {
     "bands_zmean_hyp": dict(BASE, run_mode=6, suppress_kz0=1, kz_suppress_max=2, kz_suppress_hi=9, hyp_diff=1e-3,
                             xi_cut=6.0),
     "fp32_keep": dict(BASE, precision="fp32", kz_keep_max=4),
+    "odd_stride": dict(BASE, kz_keep_max=4, energy_stride=7),     # one graph per starting buffer
     "emhd": dict(model="emhd", nx_override=96, nz_override=64, lx_override=48.0, lz_override=32.0,
                  courant_override=0.1, periodic=1, species=2, emhd_init="spectrum", spectrum_kmax=8,
                  energy_stride=10),
