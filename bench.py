@@ -15,12 +15,19 @@ species, Maxwell, sources, the xi_cut, and the energy check at the reference
 cadence every (int)(0.5/DT) steps).  All state is resident in HBM before the
 timed region.  Timed regions longer than SEGMENT steps are split into
 segments that restart from the initial state (untimed) so every timed step is
-physically finite.  The FP32 precision form of the same workload is timed too,
-and so is the precision-form workload (4096x2048, V0 = 0.1, a = 0.5, keep
-|kz| <= 8 on MFMA) in FP32 and FP64, with the kz = 1 growth rate of both
-(max-R^2 fit over the same window) and its relative error, and the batched
-sweep (64 uneven FP32 runs through captured graphs: runs/hour), and the
-EMHD 1024x1024 FP32 wave run with its MFMA space-time dispersion check.
+physically finite.
+
+Legs beside the headline (each reported in the same JSON line):
+  * fp32_form: the headline workload in the FP32 precision form;
+  * c1: 256x128 FP64, walls, keep |kz| <= 4, 2000 steps, with the reference's
+    production window (xi_cut = 5, suppress_kz0 = 1): throughput, the kz = 1
+    growth rate against the eigensolver restatement, energy drift;
+  * precision_form (C3): 4096x2048, V0 = 0.1, a = 0.5, keep |kz| <= 8 on MFMA,
+    same production window, FP32 and FP64 throughput, growth-rate, energy and
+    Gauss-residual errors of FP32 against FP64 over the stable part of the run;
+  * sweep (C4): 64 uneven FP32 runs of 3000 steps (runs/hour);
+  * emhd_waves (C5): EMHD 1024x1024 FP32 waves and their MFMA dispersion;
+  * cpu_baseline: the FP64 OpenMP oracle on the host cores.
 
 Prints one JSON line (one GPU: N = 1 only).
 """
@@ -40,13 +47,31 @@ WORKLOAD = dict(nx_override=2048, nz_override=1024, lx_override=204.8, lz_overri
                 V0=0.05, eps_override=1.0, courant_override=0.4, k_mode=1, run_mode=0, freeze_override=1,
                 perturb_amp=2e-5, noise_amp=1e-9, noise_seed=1, n_steps=5000, xi_cut=12.8)
 SEGMENT = 700
-# Precision-form workload: 4096x2048, V0 = 0.1, a = 0.5, keep |kz| <= 8 (the
-# column filter runs as MFMA contractions), xi_cut at |xi| = 12.8 for the same
-# reason as above; E/E0 stays below 5 for ~650 steps, so segments are 500.
+# The reference's production window for the colour-2/3 sector: xi_cut = 5 and
+# the kz = 0 removal (ymgpu2d/OUTER_REGION.md: the cut inside which the frozen
+# background's tachyonic branch stays out, suppress_kz0 "mandatory in
+# production").  Inside it, at the C1 and C3 geometries, the kz = 1 mode is
+# the fastest linear eigenmode (eigensolver restatement with the same Dirichlet
+# rows), so its growth rate is what the runs measure.
+PRODUCTION_WINDOW = dict(xi_cut=5.0, suppress_kz0=1)
+# C1: 256x128 FP64, walls, keep |kz| <= 4, 2000 steps (reference-scale run)
+WORKLOAD_C1 = dict(nx_override=256, nz_override=128, lx_override=25.6, lz_override=12.8, alpha_YM=1, V0=0.05,
+                   eps_override=1.0, courant_override=0.4, k_mode=1, run_mode=0, freeze_override=1,
+                   perturb_amp=2e-5, noise_amp=1e-9, noise_seed=0, n_steps=2000, kz_keep_max=4,
+                   **PRODUCTION_WINDOW)
+# C3 precision-form workload: 4096x2048, V0 = 0.1, a = 0.5, keep |kz| <= 8
+# (the column filter runs as MFMA contractions).  The run is stable to step
+# ~4170 (t ~ 167): at V0 >= 0.08 the saturated shear-layer mode cavitates
+# the beams and the run ends on the energy rule, the reference's documented
+# "V0 >= 0.08 hard wall" (OUTER_REGION.md), so the configured 10000 steps are
+# not reachable.  Throughput is timed over segments of SEGMENT_C3 steps from
+# the initial state; the diagnostics compare FP32 with FP64 at C3_DIAG_STEPS.
 WORKLOAD_C3 = dict(nx_override=4096, nz_override=2048, lx_override=409.6, lz_override=204.8, alpha_YM=1,
                    V0=0.1, eps_override=0.5, courant_override=0.4, k_mode=1, run_mode=0, freeze_override=1,
-                   perturb_amp=1e-5, noise_amp=1e-9, noise_seed=1, n_steps=10000, kz_keep_max=8, xi_cut=12.8)
-SEGMENT_C3 = 500
+                   perturb_amp=1e-5, noise_amp=1e-9, noise_seed=1, n_steps=10000, kz_keep_max=8,
+                   **PRODUCTION_WINDOW)
+SEGMENT_C3 = 2000
+C3_DIAG_STEPS = 3500
 
 # Compulsory (algorithmic) HBM traffic of each step kernel, in arrays per cell:
 # every input array read once and every output written once, no halo re-reads.
@@ -59,51 +84,32 @@ KERNEL_ARRAYS = {"k_fct_x": 18 + 12, "k_fct_z": 33 + 12, "k_fct_xz": 24 + 12, "k
 # The whole step's algorithmic minimum: R 24 + W 23 arrays (frozen Az1 read only).
 STEP_ARRAYS = 47
 ZM_ROWS = {1: 16, 2: 32, 3: 64}       # chunk height of the fused z sweep + Maxwell variants
-# What limits the step kernels other than HBM bytes, from rocprofv3 SQ counters
-# on this workload (profiles/sq_counters_2048x1024_fp64.md), keyed by slot.
+# What limits a kernel other than its HBM bytes, from rocprofv3 SQ counters and
+# the decomposition of the fused sweep (profiles/sq_counters_2048x1024_fp64.md,
+# profiles/fused_sweep_decomposition_2048x1024_fp64.md), keyed by (slot,
+# precision, nx, nz).
 LIMITER = {
-    "k_fct_xz": "FP64 VALU issue and latency at 2 waves per SIMD: SIMD VALU busy 0.43 of the kernel's "
-                "cycles, waves waiting on memory counters 0.32 of their life (profiles/sq_counters_2048x1024_fp64.md)",
-    "k_maxwell": "HBM: 702 MB per launch (1.20x algorithmic) at 5.6 TB/s; waves wait on memory 0.47 of their life",
-}
-# HBM bytes per launch of the FP64 step kernels on this workload, measured with
-# rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (separate runs), each scaled
-# by what the counter reads for a plain 256 MiB device copy (fetch x2.0,
-# write x1.0); summary under profiles/.  Keyed by kernel instance.
-PMC_TRAFFIC_MB = {
-    "k_fct_x<double, 128, YMModel, 0>": 507.6, "k_fct_x<double, 128, YMModel, 1>": 512.2,
-    "k_fct_x<double, 256, YMModel, 0>": 507.6, "k_fct_x<double, 256, YMModel, 1>": 512.3,
-    "k_fct_x<double, 64, YMModel, 0>": 507.4, "k_fct_x<double, 64, YMModel, 1>": 512.8,
-    "k_fct_xr<double, 4>": 507.7, "k_fct_xr<double, 8>": 528.5, "k_fct_xz2<double, 16>": 706.8,
-    "k_fct_xz2<double, 32>": 665.9, "k_fct_xz2<double, 64>": 642.0, "k_fct_xz<double, 32>": 680.7,
-    "k_fct_xzg<double, 16, 256>": 677.1, "k_fct_xzg<double, 32, 128>": 725.2, "k_fct_xzg<double, 32, 256>": 704.2,
-    "k_fct_xzg<double, 64, 256>": 699.4, "k_fct_xzg<double, 8, 128>": 657.4, "k_fct_xzr<double, 128>": 648.8,
-    "k_fct_xzr<double, 32>": 680.6, "k_fct_xzr<double, 64>": 661.0, "k_fct_z2<double, 16, YMModel>": 970.3,
-    "k_fct_z2<double, 32, YMModel>": 865.2, "k_fct_z2<double, 64, YMModel>": 811.3,
-    "k_fct_z<double, 16, 0, YMModel>": 978.6, "k_fct_z<double, 16, 1, YMModel>": 971.1,
-    "k_fct_z<double, 32, 0, YMModel>": 870.4, "k_fct_z<double, 32, 1, YMModel>": 869.1,
-    "k_fct_z<double, 64, 0, YMModel>": 806.5, "k_fct_z<double, 64, 1, YMModel>": 815.4,
-    "k_fct_zg<double, 16, 0, YMModel>": 1051.1, "k_fct_zg<double, 16, 1, YMModel>": 993.1,
-    "k_fct_zg<double, 32, 0, YMModel>": 982.8, "k_fct_zg<double, 32, 1, YMModel>": 927.8,
-    "k_fct_zg<double, 64, 0, YMModel>": 936.6, "k_fct_zg<double, 64, 1, YMModel>": 940.0,
-    "k_maxwell<double, 16>": 846.5, "k_maxwell<double, 4>": 718.5, "k_maxwell<double, 8>": 859.2,
-    "k_maxwell_cell<double>": 597.7, "k_maxwell_zm<double, 16, 0>": 646.3, "k_maxwell_zm<double, 16, 1>": 640.0,
-    "k_maxwell_zm<double, 32, 1>": 614.0, "k_maxwell_zm<double, 4, 1>": 702.4, "k_maxwell_zm<double, 8, 0>": 678.3,
-    "k_maxwell_zm<double, 8, 1>": 662.8,
+    ("k_fct_xz", "fp64", 2048, 1024):
+        "FP64 VALU issue and latency at 2 waves per SIMD (256 VGPRs): VALU busy 0.43 of the SIMD cycles, waves "
+        "wait on memory 0.32 of their life; without the steady rows' loads the kernel takes 149-172 us, without "
+        "its stores 178-183 us, and loading two rows ahead changes nothing, so the loads and stores add to the "
+        "arithmetic instead of hiding under it",
+    ("k_maxwell", "fp64", 2048, 1024): "HBM: 702 MB per launch (1.20x algorithmic) at 5.6 TB/s; waves wait on "
+                                       "memory 0.47 of their life",
 }
 
 
-def kernel_instance(slot, t):
+def kernel_instance(slot, t, T="double"):
     """Kernel instance (template arguments) that the tuned slot launches."""
     if slot == "k_fct_x":
         if t.get("x_lane_cells", 0) > 0:
-            return f"k_fct_xr<double, {t['x_lane_cells']}>"
-        return f"k_fct_x<double, {t['x_segment']}, YMModel, {t['x_serial']}>"
+            return f"k_fct_xr<{T}, {t['x_lane_cells']}>"
+        return f"k_fct_x<{T}, {t['x_segment']}, YMModel, {t['x_serial']}>"
     if slot == "k_fct_z":
         name = "k_fct_zg" if t["z_groups"] else "k_fct_z"
-        return f"{name}<double, {t['z_chunk']}, {t['z_prefetch']}, YMModel>"
+        return f"{name}<{T}, {t['z_chunk']}, {t['z_prefetch']}, YMModel>"
     if slot == "k_fct_zm":
-        return f"k_fct_zmx<double, {ZM_ROWS.get(t.get('zm_fused', 0), 0)}>"
+        return f"k_fct_zmx<{T}, {ZM_ROWS.get(t.get('zm_fused', 0), 0)}>"
     if slot == "k_fct_xz":
         v = t["xz_variant"]
         name, tz = {0: ("k_fct_xz", 32), 1: ("k_fct_xz2", 16), 2: ("k_fct_xz2", 32), 3: ("k_fct_xz2", 64),
@@ -113,15 +119,93 @@ def kernel_instance(slot, t):
         if name is None:
             return None
         if name == "k_fct_xzg":
-            return f"{name}<double, {tz}, {128 if v >= 10 else 256}>"
-        return f"{name}<double, {tz}>"
+            return f"{name}<{T}, {tz}, {128 if v >= 10 else 256}>"
+        return f"{name}<{T}, {tz}>"
     if slot == "k_maxwell" and t["maxwell"] >= 4:
         tzpf = {4: (16, 1), 5: (32, 1), 6: (16, 0), 7: (8, 1), 8: (8, 0), 9: (4, 1)}.get(t["maxwell"])
-        return f"k_maxwell_zm<double, {tzpf[0]}, {tzpf[1]}>" if tzpf else None
+        return f"k_maxwell_zm<{T}, {tzpf[0]}, {tzpf[1]}>" if tzpf else None
     if slot == "k_maxwell":
-        return {0: "k_maxwell<double, 16>", 1: "k_maxwell_cell<double>", 2: "k_maxwell<double, 4>",
-                3: "k_maxwell<double, 8>"}.get(t["maxwell"])
+        return {0: f"k_maxwell<{T}, 16>", 1: f"k_maxwell_cell<{T}>", 2: f"k_maxwell<{T}, 4>",
+                3: f"k_maxwell<{T}, 8>"}.get(t["maxwell"])
     return None
+
+
+def pmc_traffic(nx, nz, precision):
+    """HBM bytes per launch of each kernel instance, read from the committed
+    rocprofv3 summary of this grid and precision (FETCH_SIZE x 2 + WRITE_SIZE
+    x 1, the device-copy calibration): {instance: bytes}, empty when no
+    summary covers the configuration."""
+    path = os.path.join(REPO, "profiles", f"hbm_traffic_{nx}x{nz}_{precision}.md")
+    out = {}
+    if not os.path.exists(path):
+        return out
+    with open(path) as f:
+        for line in f:
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) >= 4 and cells[0].startswith("`"):
+                try:
+                    out[cells[0].strip("`")] = float(cells[3]) * 1e6
+                except ValueError:
+                    pass
+    return out
+
+
+def window_fractions(cfg):
+    """Fractions of columns that the xi_cut zeroes and that the sponge damps
+    (YM_Physics.cu:161-210: the cut wins where both apply)."""
+    import numpy as np
+    x = np.arange(cfg.nx) * cfg.dx
+    xi = np.abs((x - cfg.nx * cfg.dx / 2) / cfg.eps)
+    cut = (xi > float(cfg.xi_cut)) if cfg.xi_cut > 0 else np.zeros(cfg.nx, bool)
+    sponge = ((xi > float(cfg.xi_sponge)) & ~cut) if cfg.xi_sponge > 0 else np.zeros(cfg.nx, bool)
+    return float(np.mean(cut)), float(np.mean(sponge))
+
+
+def filter_arrays(cfg):
+    """Algorithmic arrays per cell of the filter stage (each array read once,
+    written once): the charges Q2/Q3 of both species zeroed on cut columns
+    (written) or damped on sponge columns (read and written), the z-mean of
+    the 12 colour-2/3 arrays and colour-1 zeroing (suppress_kz0), the keep
+    projection of the 21 filtered arrays (18 when colour 1 is zeroed anyway)."""
+    cut, sponge = window_fractions(cfg)
+    a = 4.0 * cut + 8.0 * sponge
+    if cfg.suppress_kz0:
+        a += 24 + 3
+    if cfg.kz_keep_max > 0:
+        a += 2 * (18 if cfg.suppress_kz0 else 21)
+    return a
+
+
+def kernel_roofline(r, cfg, precision):
+    """Per-slot algorithmic bytes and rates of a run_gpu() result, and the
+    roofline object of the slot with the longest launches."""
+    esz = 4 if precision == "fp32" else 8
+    T = "float" if precision == "fp32" else "double"
+    arrays_of = dict(KERNEL_ARRAYS, filters=filter_arrays(cfg))
+    traffic_tab = pmc_traffic(cfg.nx, cfg.nz, precision)
+    kern = {}
+    for name, arrays in arrays_of.items():
+        ms, cnt = r["times"].get(name, (0.0, 0))
+        if cnt and arrays > 0:
+            avg_s = ms / cnt * 1e-3
+            bytes_alg = arrays * esz * r["cells"]
+            inst = kernel_instance(name, r["tuning"], T)
+            kd = dict(avg_us=avg_s * 1e6, launches=cnt, alg_bytes=bytes_alg, alg_GBs=bytes_alg / avg_s / 1e9,
+                      instance=inst)
+            if inst in traffic_tab:
+                kd["pmc_traffic_bytes"] = traffic_tab[inst]
+                kd["pmc_traffic_GBs"] = traffic_tab[inst] / avg_s / 1e9
+            kern[name] = kd
+    dom = max(kern, key=lambda k: kern[k]["avg_us"])
+    d = kern[dom]
+    roof = {"bound": "hbm", "kernel": dom, "achieved": d["alg_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": d["alg_GBs"] / HBM_PEAK_GBS, "traffic": d.get("pmc_traffic_bytes"),
+            "kernel_instance": d["instance"], "alg_bytes_per_launch": d["alg_bytes"], "avg_launch_us": d["avg_us"],
+            "traffic_GBs": d.get("pmc_traffic_GBs"),
+            "traffic_source": (f"profiles/hbm_traffic_{cfg.nx}x{cfg.nz}_{precision}.md"
+                               if d.get("pmc_traffic_bytes") else None),
+            "limiter": LIMITER.get((dom, precision, cfg.nx, cfg.nz))}
+    return kern, roof
 
 
 def run_gpu(precision, steps, warmup, fused=-1, workload=WORKLOAD, segment=SEGMENT, kernel_steps=120):
@@ -177,12 +261,12 @@ def run_gpu(precision, steps, warmup, fused=-1, workload=WORKLOAD, segment=SEGME
                 segments=nseg, tuning=tuning, eager_ms_per_step=eager_s / kernel_steps * 1e3)
 
 
-def growth_c3(precision, n_steps=500, stride=10):
-    """kz = 1 growth rate of the precision-form workload (max-R^2 fit of the
-    circular Az2/Az3 amplitude, sampled every `stride` steps from init)."""
+def growth_run(workload, precision, n_steps, stride):
+    """kz = 1 circular Az2/Az3 amplitude sampled every `stride` steps from the
+    initial state, with E/E0 and the Gauss residual at the end."""
     import khs
 
-    cfg = khs.Config(**WORKLOAD_C3, precision=precision)
+    cfg = khs.Config(**workload, precision=precision)
     sim = khs.Simulation(cfg)
     sim.init()
     ts = sim.timeseries(1, n_steps, stride)
@@ -192,6 +276,63 @@ def growth_c3(precision, n_steps=500, stride=10):
     if ts["halted"]:
         raise RuntimeError("growth-rate run halted")
     return ts
+
+
+def eigen_rates(workload, n=3):
+    """The n fastest linear growth rates of the kz = 1 colour-2/3 mode at the
+    workload's geometry (khs.eigenmode, Dirichlet rows outside xi_cut)."""
+    import numpy as np
+    from khs import eigenmode
+
+    w = workload
+    kz = 2 * np.pi * w["k_mode"] / w["lz_override"]
+    vals, _ = eigenmode.solve(kz, float(w["alpha_YM"]), float(w["V0"]), float(w["eps_override"]),
+                              int(w["nx_override"]), Lx=float(w["lx_override"]), n_eigs=12, sigma=0.4,
+                              xi_cut=w.get("xi_cut") or None)
+    return [float(v.real) for v in vals[:n]]
+
+
+def growth_compare(workload, n_steps, stride, t_max=None):
+    """FP64 and FP32 runs of the workload: max-R^2 kz = 1 growth rate (FP32
+    fitted over the FP64 window), energy drift and Gauss residuals."""
+    import khs
+
+    t64, t32 = growth_run(workload, "fp64", n_steps, stride), growth_run(workload, "fp32", n_steps, stride)
+    g64, t0, t1, r2 = khs.fit_growth_max_r2(t64["t"][1:], t64["amp"][1:], t_max=t_max)
+    g32 = khs.fit_growth_max_r2(t32["t"][1:], t32["amp"][1:], t_min=t0, t_max=t1)[0]
+    eig = eigen_rates(workload)
+    return {"gamma_fp64": g64, "gamma_fp32": g32, "gamma_rel_err": abs(g32 - g64) / g64, "fit_window_t": [t0, t1],
+            "fit_r2": r2, "gamma_eigensolver": eig[0], "gamma_eigensolver_next": eig[1:],
+            "gamma_fp64_over_eigensolver": g64 / eig[0], "diag_steps": n_steps,
+            "energy_ratio_fp64": t64["energy_ratio"], "energy_ratio_fp32": t32["energy_ratio"],
+            "energy_drift_fp64": t64["energy_ratio"] - 1.0,
+            "energy_drift_err": abs(t32["energy_ratio"] - t64["energy_ratio"]),
+            "gauss_rms_fp64": t64["gauss"]["rms_ref"], "gauss_rms_fp32": t32["gauss"]["rms_ref"],
+            "gauss_rel_err": [abs(x - y) / max(y, 1e-300) for x, y in zip(t32["gauss"]["rms_ref"],
+                                                                          t64["gauss"]["rms_ref"])],
+            "gauss_rel_scale": t64["gauss"]["Etot_scale"]}
+
+
+def c1_leg(fused=-1):
+    """C1: 256x128 FP64, walls, keep |kz| <= 4, production window: the whole
+    2000-step run timed (graph-replayed strides), per-kernel times, the kz = 1
+    growth rate against the eigensolver and the energy drift."""
+    import khs
+
+    cfg = khs.Config(**WORKLOAD_C1)
+    r = run_gpu("fp64", cfg.total_steps, 20, fused, WORKLOAD_C1, cfg.total_steps, kernel_steps=200)
+    kern, roof = kernel_roofline(r, cfg, "fp64")
+    t64 = growth_run(WORKLOAD_C1, "fp64", cfg.total_steps, 10)
+    g, t0, t1, r2 = khs.fit_growth_max_r2(t64["t"][1:], t64["amp"][1:])
+    eig = eigen_rates(WORKLOAD_C1)
+    return {"workload": "256x128 FP64, x walled, dx=dz=0.1, V0=0.05, a=1.0, kz=1 seed 1e-6 + 1e-9 noise (seed 0), "
+                        "CFL 0.4, keep |kz|<=4, xi_cut=5, suppress_kz0=1, 2000 steps",
+            "cell_updates_per_s": r["cells"] * cfg.total_steps / r["seconds"],
+            "ms_per_step": r["seconds"] / cfg.total_steps * 1e3, "steps": cfg.total_steps,
+            "roofline": roof, "kernels": kern, "tiles": r["tuning"],
+            "gamma_fp64": g, "fit_window_t": [t0, t1], "fit_r2": r2, "gamma_eigensolver": eig[0],
+            "gamma_eigensolver_next": eig[1:], "gamma_over_eigensolver": g / eig[0],
+            "energy_drift": t64["energy_ratio"] - 1.0, "gauss_rms": t64["gauss"]["rms_ref"]}
 
 
 def sweep(n=64, steps=3000, lanes=8, segment=50, threads=16):
@@ -286,6 +427,11 @@ def profile_leg(leg, steps, warmup, fused=-1):
         b.close()
         return {"leg": leg, "runs": 64, "steps": steps, "seconds": el, "halted": int(sum(halted)),
                 "run_steps": int(sum(done))}
+    if leg == "c1":
+        r = run_gpu("fp64", steps, warmup, fused, WORKLOAD_C1, steps, kernel_steps=steps)
+        return {"leg": "c1", "workload": "256x128 FP64 keep |kz|<=4, production window", "steps": steps,
+                "ms_per_step": r["seconds"] / steps * 1e3, "tiles": r["tuning"],
+                "kernels_avg_us": {k: v[0] / v[1] * 1e3 for k, v in r["times"].items() if v[1]}}
     if leg == "c3":
         out = {}
         for prec in ("fp32", "fp64"):
@@ -374,13 +520,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-c3", action="store_true", help="skip the 4096x2048 precision-form leg")
-    ap.add_argument("--c3-steps", type=int, default=200)
+    ap.add_argument("--c3-steps", type=int, default=2000)
+    ap.add_argument("--no-c1", action="store_true", help="skip the 256x128 C1 leg")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched-sweep (runs/hour) leg")
     ap.add_argument("--no-c5", action="store_true", help="skip the EMHD wave-dispersion leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB")
-    ap.add_argument("--profile-leg", choices=["c3", "c5", "sweep", "sweep-graph"], default=None,
-                    help="run one short leg for a profiler and print its kernel times: c3 = the 4096x2048 "
+    ap.add_argument("--profile-leg", choices=["c1", "c3", "c5", "sweep", "sweep-graph"], default=None,
+                    help="run one short leg for a profiler and print its kernel times: c1 = the 256x128 run, c3 = the 4096x2048 "
                          "precision form (FP32 and FP64, MFMA keep filter), c5 = the EMHD 1024^2 run with "
                          "its MFMA space-time spectrum every 20 steps, sweep / sweep-graph = the 64-run "
                          "sweep in the concurrent / single-thread graph mode (--steps steps per run)")
@@ -399,36 +546,14 @@ def main():
     if a.profile_leg:
         print(json.dumps(profile_leg(a.profile_leg, a.steps, a.warmup, fused)))
         return
+    import khs
+
     r64 = run_gpu("fp64", a.steps, a.warmup, fused)
     ms_step = r64["seconds"] / a.steps * 1e3
     value = r64["cells"] * a.steps / r64["seconds"]
-
-    # per-kernel live roofline (HIP events on the library's stream); the xi_cut
-    # pass writes Q2/Q3 of both species on the cut columns only
-    import numpy as np
-    x = np.arange(r64["nx"]) * 0.1
-    cut_frac = float(np.mean(np.abs((x - r64["nx"] * 0.1 / 2) / 1.0) > 12.8))
-    arrays_of = dict(KERNEL_ARRAYS, filters=4 * cut_frac)
-    kern = {}
-    for name, arrays in arrays_of.items():
-        ms, cnt = r64["times"].get(name, (0.0, 0))
-        if cnt:
-            avg_s = ms / cnt * 1e-3
-            bytes_alg = arrays * 8 * r64["cells"]
-            kern[name] = dict(avg_us=avg_s * 1e6, launches=cnt, alg_bytes=bytes_alg,
-                              alg_GBs=bytes_alg / avg_s / 1e9)
-    dom = max(kern, key=lambda k: kern[k]["avg_us"])
-    d = kern[dom]
-    inst = kernel_instance(dom, r64["tuning"])
-    traffic = PMC_TRAFFIC_MB.get(inst)
-    traffic = traffic * 1e6 if (traffic and r64["nx"] == 2048 and r64["nz"] == 1024) else None
-    for name, kd in kern.items():
-        ki = kernel_instance(name, r64["tuning"])
-        t_mb = PMC_TRAFFIC_MB.get(ki)
-        kd["instance"] = ki
-        if t_mb:
-            kd["pmc_traffic_bytes"] = t_mb * 1e6
-            kd["pmc_traffic_GBs"] = t_mb * 1e6 / (kd["avg_us"] * 1e-6) / 1e9
+    # per-kernel live roofline (HIP events on the library's stream)
+    cfg64 = khs.Config(**WORKLOAD)
+    kern, roof = kernel_roofline(r64, cfg64, "fp64")
     step_alg_GBs = STEP_ARRAYS * 8 * r64["cells"] / (ms_step * 1e-3) / 1e9
 
     out = {
@@ -451,11 +576,7 @@ def main():
                    "tiles": r64["tuning"],
                    "nx": r64["nx"], "nz": r64["nz"], "precision": "fp64",
                    "sweep": "fused x+z" if r64["tuning"]["fused_sweep"] else "split x / z"},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": d["alg_GBs"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": d["alg_GBs"] / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel_instance": inst, "alg_bytes_per_launch": d["alg_bytes"], "avg_launch_us": d["avg_us"],
-                     "traffic_GBs": traffic / (d["avg_us"] * 1e-6) / 1e9 if traffic else None,
-                     "limiter": LIMITER.get(dom)},
+        "roofline": roof,
         "step_roofline": {"alg_bytes_per_cell": STEP_ARRAYS * 8, "achieved_GBs": step_alg_GBs,
                           "frac": step_alg_GBs / HBM_PEAK_GBS},
         "kernels": kern,
@@ -468,29 +589,28 @@ def main():
                             "speedup_vs_fp64": v32 / value,
                             "kernels_avg_us": {k: (v[0] / v[1] * 1e3 if v[1] else None)
                                                for k, v in r32["times"].items()}}
+    if not a.no_c1:
+        out["c1"] = c1_leg(fused)
     if not a.no_c3:
-        import khs
         c3 = {}
+        cfg3 = khs.Config(**WORKLOAD_C3)
         for prec in ("fp32", "fp64"):
             r = run_gpu(prec, a.c3_steps, 20, fused, WORKLOAD_C3, SEGMENT_C3)
-            c3[prec] = dict(value=r["cells"] * a.c3_steps / r["seconds"],
-                            ms_per_step=r["seconds"] / a.c3_steps * 1e3,
-                            kernels_avg_us={k: (v[0] / v[1] * 1e3 if v[1] else None) for k, v in r["times"].items()})
-        t32, t64 = growth_c3("fp32"), growth_c3("fp64")
-        g64, t0, t1, r2 = khs.fit_growth_max_r2(t64["t"], t64["amp"])
-        g32 = khs.fit_growth_max_r2(t32["t"], t32["amp"], t_min=t0, t_max=t1)[0]
-        g64w = khs.fit_growth_max_r2(t64["t"], t64["amp"], t_min=t0, t_max=t1)[0]
+            kern3, roof3 = kernel_roofline(r, cfg3, prec)
+            c3[prec] = dict(value=r["cells"] * a.c3_steps / r["seconds"], ms_per_step=r["seconds"] / a.c3_steps * 1e3,
+                            roofline=roof3, kernels=kern3, tiles=r["tuning"])
         out["precision_form"] = {
-            "workload": "4096x2048, V0=0.1, a=0.5, keep |kz|<=8 (MFMA filter), xi_cut=12.8, walled x",
+            "workload": "4096x2048, V0=0.1, a=0.5, keep |kz|<=8 (MFMA filter), xi_cut=5, suppress_kz0=1, walled x; "
+                        f"FP32 bulk + FP64 reductions vs plain FP64; {a.c3_steps} steps timed per precision "
+                        f"(segments of {SEGMENT_C3} from init); diagnostics at step {C3_DIAG_STEPS} "
+                        "(stable to ~4170 steps: the V0 >= 0.08 cavitation of the reference's OUTER_REGION.md)",
             "fp32_cell_updates_per_s": c3["fp32"]["value"], "fp64_cell_updates_per_s": c3["fp64"]["value"],
             "fp32_over_fp64": c3["fp32"]["value"] / c3["fp64"]["value"],
-            "gamma_fp64": g64w, "gamma_fp32": g32, "gamma_rel_err": abs(g32 - g64w) / g64w,
-            "energy_ratio_fp64": t64["energy_ratio"], "energy_ratio_fp32": t32["energy_ratio"],
-            "energy_drift_err": abs(t32["energy_ratio"] - t64["energy_ratio"]),
-            "gauss_rms_fp64": t64["gauss"]["rms_ref"], "gauss_rms_fp32": t32["gauss"]["rms_ref"],
-            "gauss_rel_scale": t64["gauss"]["Etot_scale"],
-            "fit_window_t": [t0, t1], "steps": a.c3_steps, "segments_of": SEGMENT_C3,
-            "kernels_avg_us": {p: c3[p]["kernels_avg_us"] for p in c3}}
+            "fp32_ms_per_step": c3["fp32"]["ms_per_step"], "fp64_ms_per_step": c3["fp64"]["ms_per_step"],
+            "roofline": c3["fp32"]["roofline"], "roofline_fp64": c3["fp64"]["roofline"],
+            "kernels": {p: c3[p]["kernels"] for p in c3}, "tiles": {p: c3[p]["tiles"] for p in c3},
+            "steps": a.c3_steps, "segments_of": SEGMENT_C3,
+            **growth_compare(WORKLOAD_C3, C3_DIAG_STEPS, 25)}
     if not a.no_sweep:
         out["sweep"] = sweep()
     if not a.no_c5:

@@ -207,7 +207,7 @@ int khs_get_array_buf(khs_ctx* ctx, int buf, int id, double* host, size_t n);
  * (they are computed from the end-of-step state, main_ym.cu:450-455). */
 int khs_fct_x_sweep(khs_ctx* ctx, int species);   /* ym_fct_x_sweep[_periodic], YM_FCT.cu:30-91 */
 int khs_fct_z_sweep(khs_ctx* ctx, int species);   /* ym_fct_z_sweep, YM_FCT.cu:93-135; + enforce_boundaries on the fluid */
-int khs_maxwell(khs_ctx* ctx);
+int khs_maxwell(khs_ctx* ctx);                   /* wall Ez^a = 0 + currents + Ampere + potential + Faraday */
 /* kernel_update_velocities (YM_Fluid.cu:3-11; EMHD update_velocities): ux =
  * px / max(n, 0.05) and uz = pz / max(n, 0.05) of species s of the committed
  * state, into host buffers of nx*nz doubles.  The sweeps evaluate the same
@@ -224,7 +224,7 @@ int khs_sources(khs_ctx* ctx, int species, double* src_px, double* src_pz, doubl
  * px = 0 and Ez^a = 0 (EMHD: Ey, Ez, Bx = 0) in the wall columns.  khs_step
  * and khs_fct_z_sweep / khs_maxwell apply the same conditions as part of
  * their outputs; this operator serves callers that write arrays directly. */
-int khs_enforce_boundaries(khs_ctx* ctx);                    /* enforce_boundaries(Ez) + currents + Ampere + potential + Faraday */
+int khs_enforce_boundaries(khs_ctx* ctx);
 int khs_filters(khs_ctx* ctx);                    /* sponge, xi_cut, z-mean, kz bands, hyperdiffusion, colour-1 zeroing */
 int khs_commit(khs_ctx* ctx);
 

@@ -442,7 +442,16 @@ struct Ctx {
         timed(K_FILTERS, [&] {
             // colour-2/3 fields were damped / cut in k_maxwell; the charges here
             if (cell_filters) launch_cell_filters<T>(list<T>(k, kQ23, 4, false), d_decay, d_action, kp, stream);
-            if (c.suppress_kz0) do_modes<T>(t_zmean, list<T>(k, kFilt15, 15, skip_c1), 0);
+            // With the MFMA keep projection (and no band filter between
+            // them) the z-mean removal of the 15 field / charge arrays is
+            // folded into it: those arrays come first in its list and are
+            // synthesised without their kz = 0 term.  The two projections
+            // commute, P_keep (1 - P_0) v = P_{1..K} v, and only rounding
+            // differs from removing the mean first.  The bands use the
+            // reference's FP32 twiddles, whose sums over z are not exactly
+            // zero, so with bands the mean is removed before them.
+            const bool fold_zmean = c.suppress_kz0 && c.kz_keep_max > 0 && d_keep_tab && !bands;
+            if (c.suppress_kz0 && !fold_zmean) do_modes<T>(t_zmean, list<T>(k, kFilt15, 15, skip_c1), 0);
             // reference band filter: 15 field/charge arrays + (pz, px, n) x 2 species
             int ids21[21];
             std::memcpy(ids21, kFilt15, sizeof(kFilt15));
@@ -450,10 +459,13 @@ struct Ctx {
             if (c.kz_suppress_max >= 1) do_modes<T>(t_lo, list<T>(k, ids21, 21, skip_c1), 0);
             if (c.kz_suppress_hi > c.kz_suppress_max + 1) do_modes<T>(t_hi, list<T>(k, ids21, 21, skip_c1), 0);
             if (c.kz_keep_max > 0) {
-                if (d_keep_tab)
-                    launch_keep_mfma<T>(list<T>(k, ids21, 21, skip_c1), (const T*)d_keep_tab, (T*)d_coef, kp, stream);
-                else
+                if (d_keep_tab) {
+                    const ArrayList<T> keep = list<T>(k, ids21, 21, skip_c1);
+                    const int nomean = fold_zmean ? keep.n - 6 : 0;     // all but the 6 fluid arrays
+                    launch_keep_mfma<T>(keep, (const T*)d_keep_tab, (T*)d_coef, kp, stream, nomean);
+                } else {
                     do_modes<T>(t_keep, list<T>(k, ids21, 21, skip_c1), 1);
+                }
             }
             if (c.hyp_diff > 0.0f) launch_hyperdiffuse<T>(list<T>(k, kCol23, 12, false), (double)c.hyp_diff, kp, stream);
             if (c.suppress_kz0) {

@@ -140,9 +140,11 @@ __global__ __launch_bounds__(256) void k_keep_coef_mfma(ArrayList<T> arrs, const
 }
 
 // Synthesis of KF_ZR rows of one 64-column strip; each wave does 16-row tiles.
+// Arrays a < nomean are synthesised without their mean (kz = 0 removed as
+// well: the z-mean removal of suppress_kz0 folded into the projection).
 template <typename T>
 __global__ __launch_bounds__(256) void k_keep_synth_mfma(ArrayList<T> arrs, const T* __restrict__ tab,
-                                                         const T* __restrict__ coef, int nx, int nz) {
+                                                         const T* __restrict__ coef, int nx, int nz, int nomean) {
     typedef typename V4<T>::type v4;
     const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int n = l & 15, kq = l >> 4;
@@ -154,6 +156,7 @@ __global__ __launch_bounds__(256) void k_keep_synth_mfma(ArrayList<T> arrs, cons
     // Lanes past the last column still take part in the MFMAs (their A values
     // feed every column); they contribute B = 0 and store nothing.
     const bool ok = col < nx;
+    const bool with_mean = a >= nomean;
     const size_t base = (size_t)a * (KF_ROWS + 1);
     if constexpr (sizeof(T) == 8) {
         // FP64: tile j holds columns x0 + 16 j + n, so that one 8-byte store
@@ -167,7 +170,7 @@ __global__ __launch_bounds__(256) void k_keep_synth_mfma(ArrayList<T> arrs, cons
         for (int j = 0; j < 4; ++j) {
             const int cj = x0 + 16 * j + n;
             okj[j] = cj < nx;
-            mj[j] = okj[j] ? coef[base * nx + cj] : T(0);
+            mj[j] = (okj[j] && with_mean) ? coef[base * nx + cj] : T(0);
 #pragma unroll
             for (int s = 0; s < 4; ++s) bs[s][j] = okj[j] ? coef[(base + 1 + 4 * kq + s) * nx + cj] : T(0);
         }
@@ -199,7 +202,7 @@ __global__ __launch_bounds__(256) void k_keep_synth_mfma(ArrayList<T> arrs, cons
 #pragma unroll
     for (int s = 0; s < 4; ++s)
         bc[s] = ok ? *reinterpret_cast<const v4*>(coef + (base + 1 + 4 * kq + s) * nx + col) : zero;
-    const v4 mean = ok ? *reinterpret_cast<const v4*>(coef + base * nx + col) : zero;
+    const v4 mean = (ok && with_mean) ? *reinterpret_cast<const v4*>(coef + base * nx + col) : zero;
     const int zend = min(zr0 + KF_ZR, nz);
     for (int zt = zr0 + 16 * wave; zt < zend; zt += 64) {
         // A[i = n][k = kq] of k-step s is T[zt + n][4 kq + s]
@@ -247,7 +250,8 @@ void fill_keep_mfma_table(int nz, int K, T* tab) {
 }
 
 template <typename T>
-void launch_keep_mfma(const ArrayList<T>& arrs, const T* tab, T* coef, const KParams& p, hipStream_t st) {
+void launch_keep_mfma(const ArrayList<T>& arrs, const T* tab, T* coef, const KParams& p, hipStream_t st,
+                      int nomean) {
     const int nxb = (p.nx + 63) / 64;
     // 128-column strips: a row group reads 512 (FP32) or 1024 (FP64) contiguous
     // bytes per row; 4096x2048, 21 arrays: FP32 pass 7% faster than 64-column
@@ -256,14 +260,15 @@ void launch_keep_mfma(const ArrayList<T>& arrs, const T* tab, T* coef, const KPa
     hipLaunchKernelGGL((k_keep_coef_mfma<T, Q>), dim3((p.nx + 64 * Q - 1) / (64 * Q), arrs.n), dim3(256), 0, st,
                        arrs, tab, coef, p.nx, p.nz);
     hipLaunchKernelGGL(k_keep_synth_mfma<T>, dim3(nxb, arrs.n, (p.nz + KF_ZR - 1) / KF_ZR), dim3(256), 0, st,
-                       arrs, tab, coef, p.nx, p.nz);
+                       arrs, tab, coef, p.nx, p.nz, nomean);
 }
 
 template void fill_keep_mfma_table<float>(int, int, float*);
 template void fill_keep_mfma_table<double>(int, int, double*);
-template void launch_keep_mfma<float>(const ArrayList<float>&, const float*, float*, const KParams&, hipStream_t);
+template void launch_keep_mfma<float>(const ArrayList<float>&, const float*, float*, const KParams&, hipStream_t,
+                                     int);
 template void launch_keep_mfma<double>(const ArrayList<double>&, const double*, double*, const KParams&,
-                                       hipStream_t);
+                                       hipStream_t, int);
 
 }  // namespace khs
 

@@ -91,7 +91,8 @@ bool keep_mfma_supported(int nx, int nz, int K);
 template <typename T>
 void fill_keep_mfma_table(int nz, int K, T* tab);
 template <typename T>
-void launch_keep_mfma(const ArrayList<T>& arrs, const T* tab, T* coef, const KParams& p, hipStream_t st);
+void launch_keep_mfma(const ArrayList<T>& arrs, const T* tab, T* coef, const KParams& p, hipStream_t st,
+                      int nomean = 0);
 // 2-D mode spectrum (kx, kz = 1..K): tables of 64 columns per grid point
 // (cos for r < K, sin for K <= r < 2K), C scratch 64 x nx, D output 64 x 64.
 bool spectrum_mfma_supported(int nx, int nz, int K);

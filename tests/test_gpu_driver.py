@@ -78,3 +78,34 @@ def test_driver_checkpoint_restart(tmp_path):
     assert filecmp.cmp(os.path.join(out_full, "ym_000020.csv"), os.path.join(out_cont, "ym_000020.csv"),
                        shallow=False)
     assert not os.path.exists(os.path.join(out_cont, "ym_000000.csv"))
+
+
+SNAP_COLUMNS = ["By1", "By2", "By3", "Az2", "Az3", "PzA", "PxA", "Q1A", "PzB", "PxB", "Q1B",
+                "Ex1", "Ex2", "Ex3", "Ez1", "Ez2", "Ez3", "nA", "nB", "Q2A", "Q3A", "Q2B", "Q3B", "Az1"]
+
+
+def test_snapshot_matches_an_independent_6_digit_formatter(tmp_path):
+    # main_ym.cu:287-306 streams doubles through std::ostream's defaults:
+    # %g with 6 significant digits ("1e-09", "-0", "0.05"), X = x*DX, Z = z*DZ,
+    # x fastest, the 26 columns of :291-292.  Python's %-formatting is an
+    # independent implementation of the same conversion.
+    ini = tmp_path / "run.ini"
+    ini.write_text(INI.format(n=20))
+    cfg = khs.Config.from_ini(str(ini))
+    sim = khs.Simulation(cfg)
+    sim.init()
+    sim.step(7)
+    arr = sim.get("PzA")
+    arr[0, 0] = -0.0                                 # the sign of zero is printed
+    sim.set("PzA", arr)
+    sim.write_snapshot(str(tmp_path), 7)
+    cols = [sim.get(n) for n in SNAP_COLUMNS]
+    lines = ["X,Z," + ",".join(SNAP_COLUMNS)]
+    for z in range(cfg.nz):
+        for x in range(cfg.nx):
+            vals = [x * cfg.dx, z * cfg.dz] + [c[z, x] for c in cols]
+            lines.append(",".join("%g" % v for v in vals))
+    sim.close()
+    got = (tmp_path / "ym_000007.csv").read_text()
+    assert got == "\n".join(lines) + "\n"
+    assert got.splitlines()[1].split(",")[7] == "-0"

@@ -194,6 +194,11 @@ FILTER_CASES = {
     "zmean_only_ragged": grid(70, 32, 0, suppress_kz0=1),
     "zmean_xicut": grid(96, 64, 0, suppress_kz0=1, xi_cut=3.0),
     "zmean_sponge_periodic": grid(128, 128, 1, suppress_kz0=1, xi_sponge=4.0, sigma_sponge=1.0),
+    # the production window with the MFMA keep projection: the z-mean removal
+    # is folded into the projection (kz = 0 dropped from the 12 colour-2/3 arrays)
+    "keep8_zmean_xicut_mfma": grid(128, 64, 0, kz_keep_max=8, suppress_kz0=1, xi_cut=5.0),
+    "keep4_zmean_bands_mfma": grid(96, 128, 1, kz_keep_max=4, suppress_kz0=1, kz_suppress_max=1,
+                                   kz_suppress_hi=6, hyp_diff=1e-3),
 }
 
 
@@ -230,6 +235,23 @@ def test_fp32_form_close():
     o.step(4)
     for n in orc.STATE:
         assert_close(sim.get(n), o.get(n), 2e-5, f"fp32 {n}")
+
+
+def test_precision_form_keep8_step_close():
+    """The precision form's whole step (FP32 sweeps, Maxwell and sources, the
+    keep |kz| <= 8 projection on FP32 MFMA with the z-mean removal folded in,
+    the xi_cut window) against the FP64 oracle over 4 steps: 5e-5 of each
+    array's scale (FP32 rounding of the state, 2e-5 without the filter, plus
+    the FP32 contraction of 64 rows)."""
+    nx, nz = 256, 64
+    keys = grid(nx, nz, 0, precision="fp32", kz_keep_max=8, suppress_kz0=1, xi_cut=5.0)
+    cfg, sim, o = make_pair(random_state(nx, nz, seed=13, smooth=True), **keys)
+    assert sim.tuning()["fused_sweep"] in (0, 1)
+    o.sources()
+    sim.step(4)
+    o.step(4)
+    for n in orc.STATE:
+        assert_close(sim.get(n), o.get(n), 5e-5, f"fp32 keep8 {n}")
 
 
 def test_full_size_step_bitwise():
