@@ -337,11 +337,13 @@ def c1_leg(fused=-1):
 
 def sweep(n=64, steps=3000, lanes=8, segment=50, threads=16):
     """Batched-sweep workload: n uneven-grid FP32 runs (khs.sweep) stepped
-    three ways: all runs' segments of `segment` steps as one captured graph
-    over `lanes` streams ("graph"); `threads` host threads each stepping its
-    share of the runs on their own streams with graph-replayed energy strides
-    ("concurrent"); the runs one after another ("eager").  A run ends at
-    `steps` or when it halts (E/E0 rule)."""
+    four ways: every step of all runs as three shared launches, segments of
+    `segment` steps captured as one graph ("packed"); all runs' segments as
+    one captured graph of per-run launches over `lanes` streams ("graph");
+    `threads` host threads each stepping its share of the runs on their own
+    streams with graph-replayed energy strides ("concurrent"); the runs one
+    after another ("eager").  A run ends at `steps` or when it halts (E/E0
+    rule)."""
     import khs
     from khs.sweep import sweep_configs
 
@@ -352,10 +354,10 @@ def sweep(n=64, steps=3000, lanes=8, segment=50, threads=16):
                        f"is counted in 'halted'), CFL 0.1, xi_cut=4",
            "lanes": lanes, "segment": segment, "threads": threads,
            "cell_steps": sum(cells) * steps}
-    for mode in ("graph", "concurrent", "eager"):
+    for mode in ("packed", "graph", "concurrent", "eager"):
         b = khs.Batch(cfgs, lanes=lanes, segment=segment)
         b.init()
-        run = {"graph": b.run, "eager": b.run_eager,
+        run = {"packed": b.run_packed, "graph": b.run, "eager": b.run_eager,
                "concurrent": lambda k: b.run_concurrent(k, threads=threads)}[mode]
         t0 = time.perf_counter()
         done, halted = run(steps)
@@ -365,8 +367,9 @@ def sweep(n=64, steps=3000, lanes=8, segment=50, threads=16):
                      "completed": int(sum(d >= steps for d in done)),
                      "cell_updates_per_s": sum(c * d for c, d in zip(cells, done)) / el}
     out["graph_over_eager"] = out["eager"]["seconds"] / out["graph"]["seconds"]
+    out["packed_over_eager"] = out["eager"]["seconds"] / out["packed"]["seconds"]
     out["concurrent_over_eager"] = out["eager"]["seconds"] / out["concurrent"]["seconds"]
-    best = min(("graph", "concurrent"), key=lambda m: out[m]["seconds"])
+    best = min(("packed", "graph", "concurrent"), key=lambda m: out[m]["seconds"])
     out["runs_per_hour"] = out[best]["runs_per_hour"]
     out["halted"] = out[best]["halted"]
     out["best_mode"] = best
@@ -416,13 +419,15 @@ def profile_leg(leg, steps, warmup, fused=-1):
     both precisions) or of the EMHD wave run with its spectrum (C5)."""
     import khs
 
-    if leg in ("sweep", "sweep-graph"):
+    if leg in ("sweep", "sweep-graph", "sweep-packed"):
         from khs.sweep import sweep_configs
 
         b = khs.Batch(sweep_configs(64, steps), lanes=8, segment=50)
         b.init()
         t0 = time.perf_counter()
-        done, halted = b.run_concurrent(steps, threads=16) if leg == "sweep" else b.run(steps)
+        run = {"sweep": lambda k: b.run_concurrent(k, threads=16), "sweep-graph": b.run,
+               "sweep-packed": b.run_packed}[leg]
+        done, halted = run(steps)
         el = time.perf_counter() - t0
         b.close()
         return {"leg": leg, "runs": 64, "steps": steps, "seconds": el, "halted": int(sum(halted)),
@@ -526,11 +531,11 @@ def main():
     ap.add_argument("--no-c5", action="store_true", help="skip the EMHD wave-dispersion leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB")
-    ap.add_argument("--profile-leg", choices=["c1", "c3", "c5", "sweep", "sweep-graph"], default=None,
+    ap.add_argument("--profile-leg", choices=["c1", "c3", "c5", "sweep", "sweep-graph", "sweep-packed"], default=None,
                     help="run one short leg for a profiler and print its kernel times: c1 = the 256x128 run, c3 = the 4096x2048 "
                          "precision form (FP32 and FP64, MFMA keep filter), c5 = the EMHD 1024^2 run with "
                          "its MFMA space-time spectrum every 20 steps, sweep / sweep-graph = the 64-run "
-                         "sweep in the concurrent / single-thread graph mode (--steps steps per run)")
+                         "sweep in the concurrent / single-thread graph / packed mode (--steps steps per run)")
     ap.add_argument("--split", action="store_true", help="force the separate x and z sweep kernels")
     ap.add_argument("--fused", action="store_true", help="force the fused x+z sweep kernel")
     a = ap.parse_args()

@@ -347,6 +347,16 @@ int khs_batch_create(const khs_config* cfgs, int n, int lanes, int segment, khs_
 int khs_batch_destroy(khs_batch* b);
 int khs_batch_init(khs_batch* b);
 int khs_batch_run(khs_batch* b, int nsteps, int* steps_done, int* halted);
+/* The same segments with the runs packed into shared launches: every step of
+ * all active runs is one fused x+z sweep launch, one Maxwell launch and one
+ * filter launch whose tiles are the runs' tiles one run after another (each
+ * workgroup steps its run with that run's own state and parameters, so the
+ * results are the bits of stepping each run alone), the segment and the
+ * runs' energy reductions captured as one HIP graph.  Small grids then share
+ * the device instead of waiting on launch latency.  All runs must use the
+ * YM model and one precision, and filter only with xi_cut / sponge /
+ * suppress_kz0 (KHS_ERR_CONFIG otherwise). */
+int khs_batch_run_packed(khs_batch* b, int nsteps, int* steps_done, int* halted);
 /* The same stepping without graphs: each run in turn through khs_step
  * (halting every energy_stride steps), for comparison. */
 int khs_batch_run_eager(khs_batch* b, int nsteps, int* steps_done, int* halted);

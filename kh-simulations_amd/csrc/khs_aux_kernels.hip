@@ -66,13 +66,13 @@ void launch_cell_filters(const ArrayList<T>& arrs, const double* decay, const in
 // --------------------------------------------------------------------------
 constexpr int ZM_G = 16;     // row groups per workgroup of k_zmean_cut
 
+// One 64-column strip of array k (the workgroup: 64 lanes x ZM_G row groups).
 template <typename T>
-__global__ __launch_bounds__(64 * ZM_G) void k_zmean_cut(ArrayList<T> arrs, int nmean, int nact, const double* decay,
-                                                         const int* action, double w, KParams p) {
+__device__ __forceinline__ void zmean_cut_strip(const ArrayList<T>& arrs, int nmean, int nact, const double* decay,
+                                                const int* action, double w, const KParams& p, int strip, int k) {
     __shared__ double red[ZM_G][64];
     const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
-    const int x = blockIdx.x * 64 + lane;
-    const int k = blockIdx.y;
+    const int x = strip * 64 + lane;
     const int nx = p.nx, nz = p.nz;
     const int rows = (nz + ZM_G - 1) / ZM_G;
     const int z0 = g * rows, z1 = min(nz, z0 + rows);
@@ -119,6 +119,65 @@ __global__ __launch_bounds__(64 * ZM_G) void k_zmean_cut(ArrayList<T> arrs, int 
             v[i] = (T)((double)val(i) - mean);
         }
 }
+
+template <typename T>
+__global__ __launch_bounds__(64 * ZM_G) void k_zmean_cut(ArrayList<T> arrs, int nmean, int nact, const double* decay,
+                                                         const int* action, double w, KParams p) {
+    zmean_cut_strip<T>(arrs, nmean, nact, decay, action, w, p, blockIdx.x, blockIdx.y);
+}
+
+// The sponge / xi_cut column action on a 64-column strip of array k (the
+// workgroup's 64 lanes x ZM_G row groups), the arithmetic of k_cell_filters.
+template <typename T>
+__device__ __forceinline__ void cell_filter_strip(const ArrayList<T>& arrs, const double* decay, const int* action,
+                                                  const KParams& p, int strip, int k) {
+    const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int x = strip * 64 + lane;
+    if (x >= p.nx) return;
+    const int act = action[x];
+    if (act == 0) return;
+    const int rows = (p.nz + ZM_G - 1) / ZM_G;
+    const int z0 = g * rows, z1 = min(p.nz, z0 + rows);
+    const T d = (act == 1) ? T(decay[x]) : T(0.0);
+    T* a = arrs.a[k] + x;
+    if (act == 1) {
+        for (int z = z0; z < z1; ++z) a[(size_t)z * p.nx] *= d;
+    } else {
+        for (int z = z0; z < z1; ++z) a[(size_t)z * p.nx] = T(0.0);
+    }
+}
+
+// Filter stage of packed runs: work item i of run r is strip i % nstrip of
+// filtered array i / nstrip.
+template <typename T>
+__global__ __launch_bounds__(64 * ZM_G) void k_filters_packed(const PackedRun<T>* __restrict__ runs,
+                                                              const int* __restrict__ begin, int n, int total) {
+    const int t = __builtin_amdgcn_readfirstlane((int)blockIdx.x);
+    if (t >= total) return;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (begin[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    const int r = __builtin_amdgcn_readfirstlane(lo);
+    const PackedRun<T>& d = runs[r];
+    const int i = t - begin[r];
+    const int strip = i % d.nstrip, k = i / d.nstrip;
+    const KParams p = d.p;
+    const ArrayList<T> arrs = d.filt;
+    if (d.fmode == 2)
+        zmean_cut_strip<T>(arrs, d.nmean, d.nact, d.decay, d.action, 1.0 / (double)p.nz, p, strip, k);
+    else if (d.fmode == 1)
+        cell_filter_strip<T>(arrs, d.decay, d.action, p, strip, k);
+}
+
+template <typename T>
+void launch_filters_packed(const PackedRun<T>* runs, const int* begin, int n, int total, hipStream_t st) {
+    if (total > 0)
+        hipLaunchKernelGGL(k_filters_packed<T>, dim3(total), dim3(64 * ZM_G), 0, st, runs, begin, n, total);
+}
+template void launch_filters_packed<float>(const PackedRun<float>*, const int*, int, int, hipStream_t);
+template void launch_filters_packed<double>(const PackedRun<double>*, const int*, int, int, hipStream_t);
 
 template <typename T>
 void launch_zmean_cut(const ArrayList<T>& arrs, int nmean, int nact, const double* decay, const int* action,

@@ -833,6 +833,10 @@ static void write_energy_row(Ctx& c, long step, double E) {
 // --------------------------------------------------------------------------
 struct Batch {
     std::vector<std::unique_ptr<Ctx>> runs;
+    // packed launches: descriptors of both step parities and tile offsets
+    char* pk_mem = nullptr;
+    size_t pk_bytes = 0;
+    int pk_shape = -1;                        // packed launch shape, timed on the first capture
     std::vector<hipStream_t> lanes;
     hipStream_t origin = nullptr;
     std::vector<hipEvent_t> ev;              // fork event + one join event per lane
@@ -842,12 +846,21 @@ struct Batch {
     hipGraphExec_t exec = nullptr;
     int exec_k = 0;
     std::vector<int> exec_set;                // runs captured in exec
+    std::vector<int> exec_cur;                // their ping-pong buffers when captured
+    bool exec_packed = false;                 // exec holds packed launches
+    std::vector<int> cur_of(const std::vector<int>& set) const {
+        std::vector<int> v;
+        for (int i : set) v.push_back(runs[i]->cur);
+        return v;
+    }
     ~Batch();
     void release_graph() {
         if (exec) hipGraphExecDestroy(exec);
         exec = nullptr;
         exec_k = 0;
         exec_set.clear();
+        exec_cur.clear();
+        exec_packed = false;
     }
     void capture(int K, const std::vector<int>& set) {
         release_graph();
@@ -896,11 +909,157 @@ struct Batch {
         }
         exec_k = K;
         exec_set = set;
+        exec_cur = cur_of(set);
+    }
+
+    // Runs that packed launches can step: the YM model, one precision, and a
+    // filter stage of at most the charge cut / sponge and the z-mean removal
+    // (the kernels of the packed filter launch).
+    bool packable(const std::vector<int>& set) const {
+        if (set.empty()) return false;
+        const bool f32 = runs[set[0]]->fp32;
+        for (int i : set) {
+            const Ctx& c = *runs[i];
+            const khs_config& k = c.cfg;
+            if (c.emhd || c.fp32 != f32) return false;
+            if (k.kz_suppress_max >= 1 || k.kz_suppress_hi > k.kz_suppress_max + 1 || k.kz_keep_max > 0 ||
+                k.hyp_diff > 0.0f)
+                return false;
+        }
+        return true;
+    }
+    template <typename T> void capture_packed_t(int K, const std::vector<int>& set) {
+        const int n = (int)set.size();
+        std::vector<PackedRun<T>> d(2 * (size_t)n);
+        std::vector<int> fb(n + 1, 0);
+        for (int j = 0; j < n; ++j) {
+            Ctx& c = *runs[set[j]];
+            const int nstrip = (c.nx + 63) / 64;
+            for (int par = 0; par < 2; ++par) {
+                PackedRun<T>& r = d[(size_t)par * n + j];
+                const int k = c.cur ^ par;                  // the buffer step parity `par` starts from
+                r.S = c.state<T>(k);
+                r.O = c.state<T>(1 - k);
+                r.p = c.kp;
+                r.p.rev = 0;
+                r.decay = c.cell_filters ? c.d_decay : nullptr;
+                r.action = c.cell_filters ? c.d_action : nullptr;
+                r.ngx = (c.nx + 57) / 58;
+                r.nwx = (c.nx + 62) / 63;
+                r.nstrip = nstrip;
+                r.fmode = 0;
+                r.nmean = r.nact = 0;
+                r.filt = ArrayList<T>{};
+                if (c.cfg.suppress_kz0) {
+                    // as the one-launch z-mean filter of do_filters
+                    int ids[15];
+                    int m = 0;
+                    for (int id : kFilt15)
+                        if (id != S_BY1 && id != S_EX1 && id != S_EZ1) ids[m++] = id;
+                    ids[m++] = S_BY1; ids[m++] = S_EX1; ids[m++] = S_EZ1;
+                    r.filt = c.list<T>(1 - k, ids, 15, false);
+                    r.fmode = 2;
+                    r.nmean = 12;
+                    r.nact = c.cell_filters ? 4 : 0;
+                } else if (c.cell_filters) {
+                    r.filt = c.list<T>(1 - k, kQ23, 4, false);
+                    r.fmode = 1;
+                }
+            }
+            fb[j + 1] = fb[j] + (d[j].fmode ? nstrip * d[j].filt.n : 0);
+        }
+        // tile offsets of the sweep and the Maxwell march for a launch shape
+        auto offsets = [&](int shape, std::vector<int>& ix) {
+            const int xc = packed_xz_chunk(shape), mc = packed_mx_chunk(shape);
+            ix.assign(3 * (size_t)(n + 1), 0);
+            for (int j = 0; j < n; ++j) {
+                const Ctx& c = *runs[set[j]];
+                ix[j + 1] = ix[j] + 2 * d[j].ngx * ((c.nz + xc - 1) / xc);
+                ix[(n + 1) + j + 1] = ix[(n + 1) + j] + d[j].nwx * ((c.nz + mc - 1) / mc);
+            }
+            std::copy(fb.begin(), fb.end(), ix.begin() + 2 * (n + 1));
+        };
+        const size_t dbytes = round_up(d.size() * sizeof(PackedRun<T>), 256);
+        const size_t ibytes = round_up(3 * (size_t)(n + 1) * sizeof(int), 256);
+        if (pk_bytes < dbytes + ibytes) {
+            if (pk_mem) hipFree(pk_mem);
+            pk_mem = nullptr;
+            HIPCHK(hipMalloc((void**)&pk_mem, dbytes + ibytes));
+            pk_bytes = dbytes + ibytes;
+        }
+        HIPCHK(hipMemcpy(pk_mem, d.data(), d.size() * sizeof(PackedRun<T>), hipMemcpyHostToDevice));
+        const PackedRun<T>* dev = reinterpret_cast<const PackedRun<T>*>(pk_mem);
+        const int* ib = reinterpret_cast<const int*>(pk_mem + dbytes);
+        std::vector<int> ix;
+        auto step = [&](int st, int shape) {
+            launch_packed_step<T>(dev + (size_t)(st & 1) * n, ib, ib + (n + 1), ib + 2 * (n + 1), n, ix[n],
+                                  ix[(n + 1) + n], ix[2 * (n + 1) + n], shape, origin);
+        };
+        if (pk_shape < 0) {
+            // time every launch shape on this batch (a step from the current
+            // state into the next-state buffers, which the first real step
+            // overwrites) and keep the fastest for the batch's lifetime
+            hipEvent_t ea, eb;
+            HIPCHK(hipEventCreate(&ea));
+            HIPCHK(hipEventCreate(&eb));
+            float best = 1e30f;
+            for (int shape = 0; shape < kPackedShapes; ++shape) {
+                offsets(shape, ix);
+                HIPCHK(hipMemcpy(pk_mem + dbytes, ix.data(), ix.size() * sizeof(int), hipMemcpyHostToDevice));
+                step(0, shape);
+                HIPCHK(hipEventRecord(ea, origin));
+                for (int r = 0; r < 3; ++r) step(0, shape);
+                HIPCHK(hipEventRecord(eb, origin));
+                HIPCHK(hipEventSynchronize(eb));
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, ea, eb));
+                if (ms < best) { best = ms; pk_shape = shape; }
+            }
+            hipEventDestroy(ea);
+            hipEventDestroy(eb);
+        }
+        offsets(pk_shape, ix);
+        HIPCHK(hipMemcpy(pk_mem + dbytes, ix.data(), ix.size() * sizeof(int), hipMemcpyHostToDevice));
+
+        std::lock_guard<std::mutex> lk(capture_mutex());
+        HIPCHK(hipStreamBeginCapture(origin, hipStreamCaptureModeRelaxed));
+        for (int st = 0; st < K; ++st) step(st, pk_shape);
+        for (int i : set) {                         // each run's energy into its pinned h_red[0]
+            Ctx& c = *runs[i];
+            hipStream_t own = c.stream;
+            c.stream = origin;
+            c.energy_async();
+            c.stream = own;
+        }
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamEndCapture(origin, &g));
+        hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            exec = nullptr;
+            throw Error(KHS_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+    }
+    void capture_packed(int K, const std::vector<int>& set) {
+        release_graph();
+        if (set.empty()) return;
+        for (int i : set) HIPCHK(hipStreamSynchronize(runs[i]->stream));
+        HIPCHK(hipStreamSynchronize(origin));
+        if (runs[set[0]]->fp32) capture_packed_t<float>(K, set);
+        else capture_packed_t<double>(K, set);
+        exec_k = K;
+        exec_set = set;
+        exec_cur = cur_of(set);
+        exec_packed = true;
     }
 };
 
 Batch::~Batch() {
     release_graph();
+    if (pk_mem) {
+        if (origin) hipStreamSynchronize(origin);
+        hipFree(pk_mem);
+    }
     if (origin) hipStreamSynchronize(origin);
     for (auto e : ev) hipEventDestroy(e);
     for (auto l : lanes) hipStreamDestroy(l);
@@ -1740,7 +1899,7 @@ int khs_batch_init(khs_batch* b) {
     });
 }
 
-int khs_batch_run(khs_batch* b, int nsteps, int* steps_done, int* halted) {
+static int batch_run_graphs(khs_batch* b, int nsteps, int* steps_done, int* halted, bool packed) {
     if (!b || !b->impl) { g_last_error = "null batch"; return KHS_ERR_ARG; }
     return guard([&] {
         khs::Batch& B = *b->impl;
@@ -1763,7 +1922,16 @@ int khs_batch_run(khs_batch* b, int nsteps, int* steps_done, int* halted) {
                 continue;
             }
             K -= K % 2;
-            if (!B.exec || B.exec_k != K || B.exec_set != set) B.capture(K, set);
+            if (packed && !B.packable(set))
+                throw Error(KHS_ERR_CONFIG, "khs_batch_run_packed: runs must share the YM model and precision and "
+                                            "filter only with xi_cut / sponge / suppress_kz0");
+            // a graph steps the ping-pong buffers it was captured from: an
+            // eager step since (an odd remainder) needs a new capture
+            if (!B.exec || B.exec_k != K || B.exec_set != set || B.exec_packed != packed ||
+                B.exec_cur != B.cur_of(set)) {
+                if (packed) B.capture_packed(K, set);
+                else B.capture(K, set);
+            }
             HIPCHK(hipGraphLaunch(B.exec, B.origin));
             HIPCHK(hipStreamSynchronize(B.origin));
             for (int i : set) {
@@ -1781,6 +1949,14 @@ int khs_batch_run(khs_batch* b, int nsteps, int* steps_done, int* halted) {
             if (halted) halted[i] = B.active[i] ? 0 : 1;
         }
     });
+}
+
+int khs_batch_run(khs_batch* b, int nsteps, int* steps_done, int* halted) {
+    return batch_run_graphs(b, nsteps, steps_done, halted, false);
+}
+
+int khs_batch_run_packed(khs_batch* b, int nsteps, int* steps_done, int* halted) {
+    return batch_run_graphs(b, nsteps, steps_done, halted, true);
 }
 
 int khs_batch_run_eager(khs_batch* b, int nsteps, int* steps_done, int* halted) {

@@ -133,4 +133,38 @@ void launch_mode_amp(const T* a2, const T* a3, int kz, const KParams& p, double*
 template <typename T>
 void launch_derived(const State<T>& S, int id, const KParams& p, double* out, hipStream_t st);
 
+// --------------------------------------------------------------------------
+// Packed runs: many independent runs of one batch advanced by shared launches
+// (one fused-sweep launch, one Maxwell launch and one filter launch per step
+// for all of them).  A launch's tiles are the runs' tiles one run after
+// another; a workgroup finds its run in `begin` (prefix offsets, n + 1
+// entries, device memory) and steps it with the run's own state pointers and
+// parameters, exactly as the run's own launch would.
+// --------------------------------------------------------------------------
+template <typename T>
+struct PackedRun {
+    State<T> S, O;                 // the step's input and output state sets
+    KParams p;
+    const double* decay;           // xi_cut / sponge column action, or null
+    const int* action;
+    int ngx;                       // fused sweep: 58-column groups (2 ngx nch tiles)
+    int nwx;                       // Maxwell march: 63-column groups (nwx nchm waves)
+    ArrayList<T> filt;             // filter stage: arrays of the run
+    int fmode;                     // 0 none, 1 charge cut / sponge on filt, 2 z-mean + cut + colour-1 zero
+    int nmean, nact;               // fmode 2: see launch_zmean_cut
+    int nstrip;                    // filter work items per array: 64-column strips
+};
+// Packed launch shapes (chunk rows of the fused sweep, of the Maxwell march):
+// 0: 16, 8; 1: 32, 4; 2: 32, 8; 3: 16, 4.
+constexpr int kPackedShapes = 4;
+inline int packed_xz_chunk(int shape) { return (shape == 1 || shape == 2) ? 32 : 16; }
+inline int packed_mx_chunk(int shape) { return (shape == 1 || shape == 3) ? 4 : 8; }
+// begin: device arrays of n + 1 tile offsets per launch kind (fused-sweep
+// tiles, Maxwell waves, filter work items) for the shape
+template <typename T>
+void launch_filters_packed(const PackedRun<T>* runs, const int* begin, int n, int total, hipStream_t st);
+template <typename T>
+void launch_packed_step(const PackedRun<T>* runs, const int* xz_begin, const int* mx_begin, const int* f_begin,
+                        int n, int xz_total, int mx_total, int f_total, int shape, hipStream_t st);
+
 }  // namespace khs

@@ -1125,14 +1125,11 @@ __device__ __forceinline__ void mrow_load(const State<T>& S, const State<T>& O, 
     }
 }
 
+// One wavefront of the march: wave wl (uniform) of a grid of nwx column
+// groups x chunks of TZ rows.
 template <typename T, int TZ, int PF>
-__global__ __launch_bounds__(256) void k_maxwell_zm(State<T> S, State<T> O, KParams p, int nwx, int nch,
-                                                   const double* decay, const int* action) {
-    const int lane = threadIdx.x & 63;
-    // the wavefront index is uniform: scalar loop control
-    int wl = __builtin_amdgcn_readfirstlane(xcd_logical(blockIdx.x, gridDim.x) * 4 + (int)(threadIdx.x >> 6));
-    if (wl >= nwx * nch) return;             // whole wavefronts leave together
-    wl = tile_order(wl, nwx * nch, p.rev);
+__device__ __forceinline__ void maxwell_zm_wave(const State<T>& S, const State<T>& O, const KParams& p, int nwx,
+                                                int wl, int lane, const double* decay, const int* action) {
     const int nx = p.nx, nz = p.nz;
     const bool per = p.periodic_x != 0;
     const int x = (wl % nwx) * MZM_OUT + lane;
@@ -1278,6 +1275,17 @@ __global__ __launch_bounds__(256) void k_maxwell_zm(State<T> S, State<T> O, KPar
             else mrow_load(S, O, rbase(e + 1) + (unsigned)xc, rbase(e + 1) + (unsigned)xm, cur);
         }
     }
+}
+
+template <typename T, int TZ, int PF>
+__global__ __launch_bounds__(256) void k_maxwell_zm(State<T> S, State<T> O, KParams p, int nwx, int nch,
+                                                   const double* decay, const int* action) {
+    const int lane = threadIdx.x & 63;
+    // the wavefront index is uniform: scalar loop control
+    int wl = __builtin_amdgcn_readfirstlane(xcd_logical(blockIdx.x, gridDim.x) * 4 + (int)(threadIdx.x >> 6));
+    if (wl >= nwx * nch) return;             // whole wavefronts leave together
+    wl = tile_order(wl, nwx * nch, p.rev);
+    maxwell_zm_wave<T, TZ, PF>(S, O, p, nwx, wl, lane, decay, action);
 }
 
 // --------------------------------------------------------------------------
@@ -2440,6 +2448,10 @@ __device__ __forceinline__ void xz_march_g(const State<T>& S, const State<T>& O,
     for (; e <= elast; ++e) row(e, std::integral_constant<int, 0>());
 }
 
+template <typename T, int TZ>
+__device__ __forceinline__ void xzg_tile(const State<T>& S, const State<T>& O, const KParams& p, int ngx, int wl,
+                                         int grp, int lane);
+
 // Workgroup = 2 column groups x 2 variable groups (waves 2i, 2i+1 share the
 // columns of tile i, so the second read of the cell's fluid hits the cache).
 template <typename T, int TZ, int BS>
@@ -2452,6 +2464,14 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void k_
     int wl = __builtin_amdgcn_readfirstlane(xcd_logical(blockIdx.x, gridDim.x) * TPB + (w >> 1));
     if (wl >= 2 * ngx * nch) return;          // whole wavefronts leave together
     wl = tile_order(wl, 2 * ngx * nch, p.rev);
+    xzg_tile<T, TZ>(S, O, p, ngx, wl, grp, lane);
+}
+
+// Tile wl (uniform) of the fused sweep: species wl % 2, column group
+// (wl / 2) % ngx, chunk wl / (2 ngx); variable group grp of the wavefront.
+template <typename T, int TZ>
+__device__ __forceinline__ void xzg_tile(const State<T>& S, const State<T>& O, const KParams& p, int ngx, int wl,
+                                         int grp, int lane) {
     const int sp = wl % 2;
     const int g = (wl / 2) % ngx;
     const int ch = wl / (2 * ngx);
@@ -2811,6 +2831,75 @@ KHS_SWEEP_INST(float, YMModel)
 KHS_SWEEP_INST(double, EMModel)
 KHS_SWEEP_INST(float, EMModel)
 #undef KHS_SWEEP_INST
+// --------------------------------------------------------------------------
+// Packed runs (see PackedRun in khs_launch.h): the fused sweep and the
+// Maxwell march of many runs in one launch each.  Tile t of a launch belongs
+// to the last run r with begin[r] <= t; the workgroup's wavefronts (uniform
+// r) take that run's state pointers and parameters into scalar registers and
+// run the single-run code on its tile.
+// --------------------------------------------------------------------------
+__device__ __forceinline__ int packed_run(const int* __restrict__ begin, int n, int t) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (begin[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    return __builtin_amdgcn_readfirstlane(lo);
+}
+
+template <typename T, int TZ, int BS>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2))) void k_fct_xzg_packed(
+    const PackedRun<T>* __restrict__ runs, const int* __restrict__ begin, int n, int total) {
+    constexpr int TPB = BS / 128;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int t = __builtin_amdgcn_readfirstlane(xcd_logical(blockIdx.x, gridDim.x) * TPB + (w >> 1));
+    if (t >= total) return;                   // whole wavefronts leave together
+    const int r = packed_run(begin, n, t);
+    // the run's descriptor is read in place (uniform loads of read-only
+    // kernel-argument memory): a local copy of the pointer tables would be
+    // indexed by the species at run time and live in scratch memory
+    const PackedRun<T>& d = runs[r];
+    xzg_tile<T, TZ>(d.S, d.O, d.p, d.ngx, t - begin[r], w & 1, lane);
+}
+
+template <typename T, int TZ, int PF>
+__global__ __launch_bounds__(256) void k_maxwell_zm_packed(const PackedRun<T>* __restrict__ runs,
+                                                          const int* __restrict__ begin, int n, int total) {
+    const int lane = threadIdx.x & 63;
+    const int t = __builtin_amdgcn_readfirstlane(xcd_logical(blockIdx.x, gridDim.x) * 4 + (int)(threadIdx.x >> 6));
+    if (t >= total) return;
+    const int r = packed_run(begin, n, t);
+    const PackedRun<T>& d = runs[r];
+    maxwell_zm_wave<T, TZ, PF>(d.S, d.O, d.p, d.nwx, t - begin[r], lane, d.decay, d.action);
+}
+
+template <typename T, int XZ, int MX>
+static void launch_packed_step_t(const PackedRun<T>* runs, const int* xz_begin, const int* mx_begin,
+                                 const int* f_begin, int n, int xz_total, int mx_total, int f_total, hipStream_t st) {
+    constexpr int BS = 256, TPB = BS / 128;
+    hipLaunchKernelGGL((k_fct_xzg_packed<T, XZ, BS>), dim3(round8((xz_total + TPB - 1) / TPB)), dim3(BS), 0, st,
+                       runs, xz_begin, n, xz_total);
+    hipLaunchKernelGGL((k_maxwell_zm_packed<T, MX, 1>), dim3(round8((mx_total + 3) / 4)), dim3(256), 0, st, runs,
+                       mx_begin, n, mx_total);
+    launch_filters_packed<T>(runs, f_begin, n, f_total, st);
+}
+
+template <typename T>
+void launch_packed_step(const PackedRun<T>* runs, const int* xz_begin, const int* mx_begin, const int* f_begin,
+                        int n, int xz_total, int mx_total, int f_total, int shape, hipStream_t st) {
+    switch (shape) {
+        case 1: launch_packed_step_t<T, 32, 4>(runs, xz_begin, mx_begin, f_begin, n, xz_total, mx_total, f_total, st); return;
+        case 2: launch_packed_step_t<T, 32, 8>(runs, xz_begin, mx_begin, f_begin, n, xz_total, mx_total, f_total, st); return;
+        case 3: launch_packed_step_t<T, 16, 4>(runs, xz_begin, mx_begin, f_begin, n, xz_total, mx_total, f_total, st); return;
+        default: launch_packed_step_t<T, 16, 8>(runs, xz_begin, mx_begin, f_begin, n, xz_total, mx_total, f_total, st); return;
+    }
+}
+template void launch_packed_step<float>(const PackedRun<float>*, const int*, const int*, const int*, int, int, int,
+                                        int, int, hipStream_t);
+template void launch_packed_step<double>(const PackedRun<double>*, const int*, const int*, const int*, int, int,
+                                         int, int, int, hipStream_t);
+
 template void launch_emhd_maxwell<double>(const State<double>&, const State<double>&, const KParams&, int,
                                           hipStream_t);
 template void launch_emhd_maxwell<float>(const State<float>&, const State<float>&, const KParams&, int, hipStream_t);

@@ -153,6 +153,7 @@ def _signatures():
         "khs_batch_destroy": (c_int, [c_void_p]), "khs_batch_init": (c_int, [c_void_p]),
         "khs_batch_run": (c_int, [c_void_p, c_int, P(c_int), P(c_int)]),
         "khs_batch_run_eager": (c_int, [c_void_p, c_int, P(c_int), P(c_int)]),
+        "khs_batch_run_packed": (c_int, [c_void_p, c_int, P(c_int), P(c_int)]),
         "khs_batch_get_array": (c_int, [c_void_p, c_int, c_int, dp, c_size_t]),
         "khs_batch_set_array": (c_int, [c_void_p, c_int, c_int, dp, c_size_t]),
         "khs_batch_energy": (c_int, [c_void_p, c_int, dp, dp]),
@@ -492,6 +493,11 @@ class Batch:
 
     def run_eager(self, nsteps):
         return self._run(_lib.khs_batch_run_eager, nsteps)
+
+    def run_packed(self, nsteps):
+        """Graph-captured segments with every step of all runs in three shared
+        launches (khs_batch_run_packed)."""
+        return self._run(_lib.khs_batch_run_packed, nsteps)
 
     def run_concurrent(self, nsteps, threads=16):
         """Runs stepped by `threads` host threads on their own streams (16

@@ -46,12 +46,18 @@ def test_batch_graph_replays_match_eager():
                 assert_bitwise(batch.get(i, n), s.get(n), f"replay {rep} run {i} {n}")
             E, E0 = batch.energy(i)
             assert E == s.energy()
-    # odd tail (one eager step per run) and a longer call spanning segments
+    # odd tail (one eager step per run) and a longer call spanning segments,
+    # then replays from the other ping-pong buffer
     done, halted = batch.run(21)
     for i, s in enumerate(sims):
         s.step(21)
         assert done[i] == 51
         assert_bitwise(batch.get(i, "PxA"), s.get("PxA"), f"tail run {i}")
+    done, halted = batch.run(20)
+    for i, s in enumerate(sims):
+        s.step(20)
+        assert done[i] == 71
+        assert_bitwise(batch.get(i, "Q1B"), s.get("Q1B"), f"after tail run {i}")
 
 
 def test_batch_halted_run_leaves_graph():
@@ -117,4 +123,74 @@ def test_batch_concurrent_odd_stride_matches_eager():
         for n in ("PzA", "Q2B", "By2", "Ez3", "Az2", "nB"):
             assert_bitwise(batch.get(i, n), s.get(n), f"odd-stride run {i} {n}")
         s.close()
+    batch.close()
+
+
+def packed_configs():
+    """Uneven grids, both precisions excluded (one per batch), walls and
+    periodic x, the charge cut, the kz = 0 removal with and without the cut,
+    a sponge, warm beams."""
+    base = dict(alpha_YM=1.0, courant_override=0.4, freeze_override=1, perturb_amp=1e-3, noise_amp=1e-9,
+                n_steps=1000, precision="fp32")
+    grids = [(128, 64, 0, dict(xi_cut=4.0)), (256, 128, 0, dict(xi_cut=4.0, suppress_kz0=1)),
+             (100, 64, 6, dict(suppress_kz0=1)), (64, 64, 0, dict()),
+             (200, 96, 6, dict(xi_sponge=3.0, sigma_sponge=2.0)), (70, 40, 0, dict(xi_cut=2.0, warm_T=0.01))]
+    out = []
+    for i, (nx, nz, mode, extra) in enumerate(grids):
+        out.append(dict(base, nx_override=nx, nz_override=nz, lx_override=nx * 0.1, lz_override=nz * 0.1,
+                        V0=0.03 + 0.02 * i, eps_override=0.6 + 0.2 * i, k_mode=1 + i % 3, run_mode=mode,
+                        noise_seed=i, **extra))
+    return out
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_batch_packed_launches_match_single_runs(precision):
+    """Runs packed into shared launches give the bits of each run stepped
+    alone, over several graph replays with eager work between them, an odd
+    tail and a run that halts and leaves the packed graph."""
+    cfgs = [dict(c, precision=precision) for c in packed_configs()]
+    batch = khs.Batch(cfgs, lanes=2, segment=6)
+    batch.init()
+    sims = []
+    for c in cfgs:
+        s = khs.Simulation(khs.Config(**c))
+        s.init()
+        sims.append(s)
+    for rep in range(2):
+        done, halted = batch.run_packed(12)
+        assert done == [12 * (rep + 1)] * len(cfgs) and not any(halted)
+        for i, s in enumerate(sims):
+            s.step(12)
+            s.sync()
+            for n in ("PzA", "Q2B", "By2", "Ez3", "Az2", "nB", "PxA", "Ex1"):
+                assert_bitwise(batch.get(i, n), s.get(n), f"packed replay {rep} run {i} {n}")
+            assert batch.energy(i)[0] == s.energy()
+    done, halted = batch.run_packed(7)                  # a segment of 6 and one eager step
+    for i, s in enumerate(sims):
+        s.step(7)
+        assert done[i] == 31
+        assert_bitwise(batch.get(i, "Q3A"), s.get("Q3A"), f"packed tail run {i}")
+    arr = batch.get(2, "nA")
+    arr[3, 3] = np.nan
+    batch.set(2, "nA", arr)
+    done, halted = batch.run_packed(12)
+    assert halted[2] and not any(h for j, h in enumerate(halted) if j != 2)
+    for i, s in enumerate(sims):
+        if i == 2:
+            continue
+        s.step(12)
+        assert_bitwise(batch.get(i, "By3"), s.get("By3"), f"packed after halt run {i}")
+    for s in sims:
+        s.close()
+    batch.close()
+
+
+def test_batch_packed_rejects_unsupported_filters():
+    cfgs = [dict(c, precision="fp32") for c in packed_configs()[:2]]
+    cfgs[1]["kz_keep_max"] = 4
+    batch = khs.Batch(cfgs, lanes=1, segment=4)
+    batch.init()
+    with pytest.raises(khs.KhsError) as e:
+        batch.run_packed(4)
+    assert e.value.code == khs.KHS_ERR_CONFIG
     batch.close()
