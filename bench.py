@@ -462,38 +462,68 @@ def profile_leg(leg, steps, warmup, fused=-1):
 
 
 def cpu_baseline(budget_s=15.0):
-    """FP64 OpenMP oracle on the host cores, same workload, bounded sample."""
+    """The FP64 OpenMP oracle behind the library's C ABI (oracle/libkhs_cpu.so)
+    on the host cores, headline workload, bounded sample.  Runs in a child
+    process so that the OpenMP placement (OMP_PROC_BIND=close,
+    OMP_PLACES=cores) is set before the OpenMP runtime starts."""
+    import subprocess
+
+    allowed = len(os.sched_getaffinity(0))
+    env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores", KHS_BASELINE_CPUS=str(allowed))
+    env.setdefault("OMP_NUM_THREADS", str(allowed))
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-child", str(budget_s)], env=env,
+                       capture_output=True, text=True, timeout=budget_s * 10 + 300)
+    if r.returncode != 0:
+        raise RuntimeError("CPU baseline failed: " + r.stderr[-2000:])
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def _host_cpu():
+    """lscpu's model name and physical cores (core, socket pairs)."""
+    import subprocess
+
+    model, cores = None, None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        model = next((l.split(":", 1)[1].strip() for l in out.splitlines() if l.startswith("Model name")), None)
+        pairs = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True).stdout
+        cores = len({l for l in pairs.splitlines() if l and not l.startswith("#")})
+    except (OSError, StopIteration):
+        pass
+    return model, cores
+
+
+def _cpu_baseline_child(budget_s):
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import khs
     import oracle_ref as orc
 
+    khs._lib = khs.load_library(os.path.join(REPO, "oracle", "libkhs_cpu.so"))
     cfg = khs.Config(**WORKLOAD)
-    o = orc.Oracle(orc.params_from_config(cfg))
-    o.init_from_config(cfg)
-    o.sources()
-    o.step(1)                          # warm-up (page-in, thread pool)
+    sim = khs.Simulation(cfg)
+    sim.init()
+    sim.step(1)                        # warm-up (page-in, thread pool)
     n, t0 = 0, time.perf_counter()
     while True:
-        o.step(1)
+        sim.step(1)
         n += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or n >= 200:
+        if el >= budget_s or n >= 400:
             break
+    sim.close()
     cells = cfg.nx * cfg.nz
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    model = None
-    try:
-        with open("/proc/cpuinfo") as f:
-            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), None)
-    except OSError:
-        pass
+    threads = int(os.environ["OMP_NUM_THREADS"])
+    model, phys = _host_cpu()
+    allowed = int(os.environ.get("KHS_BASELINE_CPUS", "0")) or None
     triad = orc.stream_triad()
     value = cells * n / el
-    return dict(value=value, unit="cell-updates/s", cores=threads, kind="port",
-                sample=f"{n} FP64 steps of the 2048x1024 workload ({el:.1f} s wall, "
-                       f"OMP_NUM_THREADS={threads}, energy check excluded)",
-                cpu_model=model, stream_triad_GBs=triad,
-                roofline_frac=(value * STEP_ARRAYS * 8 / 1e9 / triad) if triad else None)
+    print(json.dumps(dict(
+        value=value, unit="cell-updates/s", cores=threads, kind="port",
+        sample=f"{n} FP64 steps of the 2048x1024 workload through libkhs_cpu.so's khs_step (energy checks at the "
+               f"reference cadence included), {el:.1f} s wall, OMP_NUM_THREADS={threads}, OMP_PROC_BIND=close, "
+               f"OMP_PLACES=cores",
+        cpu_model=model, physical_cores=phys, cpus_allowed=allowed, compiler="g++ -O2 -fopenmp -ffp-contract=off",
+        stream_triad_GBs=triad, roofline_frac=(value * STEP_ARRAYS * 8 / 1e9 / triad) if triad else None)))
 
 
 def calibrate_copy(mb):
@@ -538,7 +568,11 @@ def main():
                          "sweep in the concurrent / single-thread graph / packed mode (--steps steps per run)")
     ap.add_argument("--split", action="store_true", help="force the separate x and z sweep kernels")
     ap.add_argument("--fused", action="store_true", help="force the fused x+z sweep kernel")
+    ap.add_argument("--cpu-baseline-child", type=float, default=0.0, help=argparse.SUPPRESS)
     a = ap.parse_args()
+    if a.cpu_baseline_child:
+        _cpu_baseline_child(a.cpu_baseline_child)
+        return
     if a.calibrate_copy:
         calibrate_copy(a.calibrate_copy)
         return

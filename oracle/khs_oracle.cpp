@@ -31,17 +31,7 @@
 #include <vector>
 #include <algorithm>
 
-extern "C" {
-
-struct OrcParams {
-    int nx, nz;
-    int periodic_x, freeze_az1;
-    int suppress_kz0, kz_suppress_max, kz_suppress_hi, kz_keep_max;
-    double dx, dz, dt, c, eps0, alpha, V0, eps;
-    double xi_sponge, sigma_sponge, xi_cut, warm_T, hyp_diff;
-};
-
-}  // extern "C"
+#include "khs_oracle.h"
 
 namespace {
 
