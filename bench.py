@@ -297,11 +297,21 @@ def growth_compare(workload, n_steps, stride, t_max=None):
     fitted over the FP64 window), energy drift and Gauss residuals."""
     import khs
 
+    import numpy as np
+
     t64, t32 = growth_run(workload, "fp64", n_steps, stride), growth_run(workload, "fp32", n_steps, stride)
-    g64, t0, t1, r2 = khs.fit_growth_max_r2(t64["t"][1:], t64["amp"][1:], t_max=t_max)
-    g32 = khs.fit_growth_max_r2(t32["t"][1:], t32["amp"][1:], t_min=t0, t_max=t1)[0]
+    g64max, t0, t1, r2 = khs.fit_growth_max_r2(t64["t"][1:], t64["amp"][1:], t_max=t_max)
+
+    def slope(ts):                   # least-squares rate of log(amp) over exactly [t0, t1]
+        m = (ts["t"] >= t0) & (ts["t"] <= t1)
+        return float(np.polyfit(ts["t"][m], np.log(ts["amp"][m]), 1)[0])
+
+    g64, g32 = slope(t64), slope(t32)
     eig = eigen_rates(workload)
     return {"gamma_fp64": g64, "gamma_fp32": g32, "gamma_rel_err": abs(g32 - g64) / g64, "fit_window_t": [t0, t1],
+            "gamma_fit": "max-R^2 window of the FP64 run (batch_analyze.py:44-69), then the least-squares rate of "
+                         "log(amp) of each run over exactly that window",
+            "gamma_fp64_max_r2": g64max,
             "fit_r2": r2, "gamma_eigensolver": eig[0], "gamma_eigensolver_next": eig[1:],
             "gamma_fp64_over_eigensolver": g64 / eig[0], "diag_steps": n_steps,
             "energy_ratio_fp64": t64["energy_ratio"], "energy_ratio_fp32": t32["energy_ratio"],
