@@ -70,6 +70,9 @@ constexpr int ZM_G = 16;     // row groups per workgroup of k_zmean_cut
 template <typename T>
 __device__ __forceinline__ void zmean_cut_strip(const ArrayList<T>& arrs, int nmean, int nact, const double* decay,
                                                 const int* action, double w, const KParams& p, int strip, int k) {
+    // rows per thread held in registers (nz <= 16 R): 64 FP32 / 32 FP64
+    // values, within the 128 VGPRs a 1024-thread workgroup allows
+    constexpr int R = sizeof(T) == 4 ? 64 : 32;
     __shared__ double red[ZM_G][64];
     const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
     const int x = strip * 64 + lane;
@@ -95,8 +98,44 @@ __device__ __forceinline__ void zmean_cut_strip(const ArrayList<T>& arrs, int nm
         else if (act == 2) a = T(0.0);
         return a;
     };
-    // four running sums (four loads in flight), added in order
+    // four running sums, added in order: rows z0 + 4j .. z0 + 4j + 3 go to
+    // s0..s3, the rows after the last whole group of four to s0
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    auto column_mean = [&](double tsum) {
+        red[g][lane] = tsum;
+        __syncthreads();
+        double tot = 0.0;
+#pragma unroll
+        for (int q = 0; q < ZM_G; ++q) tot += red[q][lane];
+        return tot * w;
+    };
+    if (z1 - z0 <= R) {
+        // one pass: the thread's rows stay in registers between the sum and
+        // the subtraction (the same sums in the same order as below)
+        const int cnt = z1 - z0;
+        T c[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+            c[q] = (x < nx && q < cnt) ? val((size_t)x + (size_t)(z0 + q) * nx) : T(0.0);
+#pragma unroll
+        for (int q = 0; q < R; q += 4) {
+            if (q + 4 <= cnt) {
+                s0 += (double)c[q]; s1 += (double)c[q + 1]; s2 += (double)c[q + 2]; s3 += (double)c[q + 3];
+            } else {
+#pragma unroll
+                for (int r = q; r < q + 4; ++r)
+                    if (r < cnt) s0 += (double)c[r];
+            }
+        }
+        const double mean = column_mean((s0 + s1) + (s2 + s3));
+        if (x < nx)
+#pragma unroll
+            for (int q = 0; q < R; ++q)
+                if (q < cnt) v[(size_t)x + (size_t)(z0 + q) * nx] = (T)((double)c[q] - mean);
+        return;
+    }
+    // taller columns: two passes (four loads in flight), the second read of
+    // the strip mostly from the cache
     if (x < nx) {
         int z = z0;
         for (; z + 4 <= z1; z += 4) {
@@ -106,12 +145,7 @@ __device__ __forceinline__ void zmean_cut_strip(const ArrayList<T>& arrs, int nm
         }
         for (; z < z1; ++z) s0 += (double)val((size_t)x + (size_t)z * nx);
     }
-    red[g][lane] = (s0 + s1) + (s2 + s3);
-    __syncthreads();
-    double tot = 0.0;
-#pragma unroll
-    for (int q = 0; q < ZM_G; ++q) tot += red[q][lane];
-    const double mean = tot * w;
+    const double mean = column_mean((s0 + s1) + (s2 + s3));
     if (x < nx)
 #pragma unroll 4
         for (int z = z0; z < z1; ++z) {
@@ -163,8 +197,10 @@ __global__ __launch_bounds__(64 * ZM_G) void k_filters_packed(const PackedRun<T>
     const PackedRun<T>& d = runs[r];
     const int i = t - begin[r];
     const int strip = i % d.nstrip, k = i / d.nstrip;
-    const KParams p = d.p;
-    const ArrayList<T> arrs = d.filt;
+    // the run's parameters and array table are read in place (a local copy
+    // of the table, indexed by k at run time, would live in scratch memory)
+    const KParams& p = d.p;
+    const ArrayList<T>& arrs = d.filt;
     if (d.fmode == 2)
         zmean_cut_strip<T>(arrs, d.nmean, d.nact, d.decay, d.action, 1.0 / (double)p.nz, p, strip, k);
     else if (d.fmode == 1)

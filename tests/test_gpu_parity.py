@@ -254,6 +254,31 @@ def test_precision_form_keep8_step_close():
         assert_close(sim.get(n), o.get(n), 5e-5, f"fp32 keep8 {n}")
 
 
+def test_precision_form_constraint_and_energy_track_fp64():
+    """The precision form's errors as the FP32 form reports them: from the same
+    seeded KH state (V0 = 0.1, a = 0.5, keep |kz| <= 8, the production window)
+    FP32 and FP64 runs of 400 steps keep E/E0 within 1e-6 of each other and
+    the Gauss-law residual rms of every colour within 1e-3 relative, or
+    within FP32's resolution of the divergence (2e-6 of the field scale, the
+    level of a residual that is round-off in both runs).  The FP64 run is the
+    oracle-exact path."""
+    keys = dict(C1, V0=0.1, eps_override=0.5, kz_keep_max=8, suppress_kz0=1, xi_cut=5.0)
+    res = {}
+    for prec in ("fp64", "fp32"):
+        sim = khs.Simulation(khs.Config(**keys, precision=prec))
+        sim.init()
+        e0 = sim.energy()
+        assert not sim.step(400)
+        g = sim.gauss()
+        res[prec] = (sim.energy() / e0, np.array(g["rms_ref"]), g["Etot_scale"])
+        sim.close()
+    (r64, g64, scale), (r32, g32, _) = res["fp64"], res["fp32"]
+    print("E/E0", r64, r32, "gauss", g64, g32, "scale", scale, flush=True)
+    assert abs(r32 - r64) <= 1e-6, (r32, r64)
+    assert np.all(g64 > 0)
+    np.testing.assert_allclose(g32, g64, rtol=1e-3, atol=2e-6 * scale)
+
+
 def test_full_size_step_bitwise():
     """One step at the 2048x1024 headline size, bit-identical to the oracle."""
     keys = dict(C1, nx_override=2048, nz_override=1024, lx_override=204.8, lz_override=102.4,
