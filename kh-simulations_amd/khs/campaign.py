@@ -11,13 +11,15 @@ batch_analyze.py).  Here:
     :119-168) and xi_sponge_for their sponge radius (:33-37);
   * a manifest (JSON: {"defaults": {...}, "runs": [{...}, ...],
     "sample_stride": S, "lanes": L, "segment": G, "mode": "concurrent" |
-    "graph"}, or a directory / glob of .ini files) names the runs;
+    "graph" | "packed"}, or a directory / glob of .ini files) names the runs;
   * lpt_assign deals them to GPUs by longest processing time first (cost =
     cells x steps), instead of round-robin;
   * run_part steps one GPU's share as one khs.Batch (each run stops at its own
     n_steps / target_tu or when it halts; the runs advance concurrently on
     host threads and their own streams, whole energy strides as captured
-    graphs, or with "mode": "graph" as one graph per segment of all runs),
+    graphs; with "mode": "graph" as one graph per segment of all runs; with
+    "mode": "packed" every step of all runs as one launch per kernel, for
+    runs of one precision whose filters are xi_cut / sponge / suppress_kz0),
     sampling the circular Az2 + i Az3
     amplitude at each run's k_mode every S steps, and fits the growth rate
     (khs.fit_growth_max_r2 and khs.fit_growth_plateau);
@@ -82,6 +84,8 @@ def load_manifest(path):
     for k in opts:
         if k in m:
             opts[k] = m[k] if k == "mode" else int(m[k])
+    if opts["mode"] not in ("concurrent", "graph", "packed"):
+        raise ValueError(f"{path}: mode must be 'concurrent', 'graph' or 'packed', not {opts['mode']!r}")
     return runs, opts
 
 
@@ -148,8 +152,13 @@ def run_part(runs, opts, out_root):
             live = [i for i in range(n) if not halted[i] and steps[i] < cfgs[i].total_steps]
             if not live:
                 break
-            done, hl = (b.run(stride) if opts.get("mode") == "graph"
-                        else b.run_concurrent(stride, threads=int(opts["lanes"])))
+            mode = opts.get("mode", "concurrent")
+            if mode == "graph":
+                done, hl = b.run(stride)
+            elif mode == "packed":
+                done, hl = b.run_packed(stride)
+            else:
+                done, hl = b.run_concurrent(stride, threads=int(opts["lanes"]))
             for i in live:
                 if done[i] != steps[i]:
                     steps[i] = done[i]

@@ -64,3 +64,13 @@ def test_load_manifest_json_and_ini(tmp_path):
     assert [r[0] for r in runs] == ["r0", "r1"]
     assert C.make_config(runs[1][1]).k_mode == 2
     assert C.run_cost(C.make_config(runs[1][1])) > 0
+
+
+def test_load_manifest_modes(tmp_path):
+    p = tmp_path / "m.json"
+    for mode in ("concurrent", "graph", "packed"):
+        p.write_text(json.dumps({"runs": [{"k_mode": 1}], "mode": mode}))
+        assert C.load_manifest(str(p))[1]["mode"] == mode
+    p.write_text(json.dumps({"runs": [{"k_mode": 1}], "mode": "threads"}))
+    with pytest.raises(ValueError, match="mode must be"):
+        C.load_manifest(str(p))

@@ -49,6 +49,35 @@ def test_campaign_part_matches_single_runs(tmp_path, mode):
         np.testing.assert_array_equal(data["amp_kz"], np.array([float(f"{a:.9e}") for a in ts["amp"]]))
 
 
+# packed launches take runs of one precision filtered only by xi_cut /
+# sponge / suppress_kz0
+PACKED_RUNS = [dict(name="p128_k1", nx_override=128, lx_override=12.8, k_mode=1, n_steps=44, noise_seed=1,
+                    suppress_kz0=1),
+               dict(name="p96_k2", nx_override=96, lx_override=9.6, k_mode=2, n_steps=30, noise_seed=2),
+               dict(name="p160_k1", nx_override=160, lx_override=16.0, k_mode=1, n_steps=52, noise_seed=3,
+                    suppress_kz0=1, xi_cut=6.0)]
+
+
+def test_campaign_packed_matches_single_runs(tmp_path):
+    base = {k: v for k, v in BASE.items() if k != "kz_keep_max"}
+    m = {"defaults": base, "runs": PACKED_RUNS, "sample_stride": 10, "lanes": 2, "segment": 10, "mode": "packed"}
+    p = tmp_path / "packed.json"
+    p.write_text(json.dumps(m))
+    runs, opts = C.load_manifest(str(p))
+    out = str(tmp_path / "out")
+    rows = C.run_part(runs, opts, out)
+    for spec, row in zip(PACKED_RUNS, rows):
+        keys = dict(base, **{k: v for k, v in spec.items() if k != "name"})
+        print("packed campaign run", spec["name"], flush=True)
+        assert row["name"] == spec["name"] and row["steps"] == spec["n_steps"] and row["halted"] == 0
+        sim = khs.Simulation(khs.Config(**keys))
+        sim.init()
+        ts = sim.timeseries(spec["k_mode"], spec["n_steps"], 10)
+        sim.close()
+        data = np.genfromtxt(os.path.join(out, spec["name"], "timeseries.csv"), delimiter=",", names=True)
+        np.testing.assert_array_equal(data["amp_kz"], np.array([float(f"{a:.9e}") for a in ts["amp"]]))
+
+
 def test_campaign_cli(tmp_path):
     out = str(tmp_path / "cli")
     env = dict(os.environ)
