@@ -273,3 +273,36 @@ def test_gauss_divergence_is_backward_in_x_with_periodic_wrap():
     o.set_state(st)
     g = o.gauss()
     assert abs(g[1] - (v / dx) / np.sqrt(nx)) < 1e-14
+
+
+@pytest.mark.parametrize("periodic", [0, 1])
+def test_enforce_boundaries_and_floored_velocities(periodic):
+    """kernel_enforce_boundaries (YM_Fluid.cu:13-30): densities below 0.05 are
+    raised to 0.05 (others untouched); with walls, px of both species and
+    Ez^1..3 are zero in the first and last columns, nothing else changes;
+    periodic x leaves the walls alone.  kernel_update_velocities (:3-11):
+    u = p / max(n, 0.05), the floor applying to the velocity only."""
+    from helpers import random_state
+    nx, nz = 12, 6
+    st = random_state(nx, nz, seed=8)
+    st["nA"][2, 3] = 0.01
+    st["nA"][4, 0] = -0.3
+    st["nB"][1, 11] = 0.0499
+    st["nB"][3, 5] = 0.05
+    o = orc.Oracle(orc.params(nx, nz, 0.1, 0.1, 0.04, periodic_x=periodic))
+    o.set_state(st)
+    o.update_velocities()
+    for sp in "AB":
+        n_safe = np.maximum(st["n" + sp], 0.05)
+        np.testing.assert_array_equal(o.get("ux" + sp), st["Px" + sp] / n_safe)
+        np.testing.assert_array_equal(o.get("uz" + sp), st["Pz" + sp] / n_safe)
+    o.enforce_boundaries()
+    walls = [] if periodic else [0, nx - 1]
+    for name in orc.STATE:
+        want = np.array(st[name], dtype=np.float64)
+        if name in ("nA", "nB"):
+            want = np.where(want < 0.05, 0.05, want)
+        if name in ("PxA", "PxB", "Ez1", "Ez2", "Ez3"):
+            want[:, walls] = 0.0
+        np.testing.assert_array_equal(o.get(name), want, err_msg=name)
+    assert o.get("nA")[2, 3] == 0.05 and o.get("nB")[3, 5] == 0.05 and o.get("nB")[1, 11] == 0.05
