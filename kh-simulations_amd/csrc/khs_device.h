@@ -156,7 +156,11 @@ __device__ __forceinline__ T fct_limit(T flx, T dm1, T d0, T dp1) {
     T term = fsgn * dm1;
     T terp = fsgn * dp1;
     T fa = fabs(flx);
-    T lim = kmax(T(0.0), kmin(term, kmin(fa, terp)));
+    // the two inner minima as hardware min (fmin: v_min_f64 / v_min_f32, the
+    // reference's CUDA min): they differ from the select form only in the
+    // sign of a zero result (and in dropping NaN, as CUDA's min does), and the
+    // outer max with +0 returns +0 for either zero
+    T lim = kmax(T(0.0), fmin(term, fmin(fa, terp)));
     return fsgn * lim;
 }
 

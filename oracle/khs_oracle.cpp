@@ -78,12 +78,14 @@ inline void nu_mu(double eps, double& nu, double& mu) {
     mu = 0.25 - 0.5 * nu;
 }
 
+// The reference's min is CUDA's fmin (the non-NaN operand of a NaN pair);
+// its max(0.0, .) returns +0 for a zero of either sign, as dmax does.
 inline double limiter(double flx, double dm1, double d0, double dp1) {
     double fsgn = std::copysign(1.0, d0);
     double term = fsgn * dm1;
     double terp = fsgn * dp1;
     double fa = std::fabs(flx);
-    double lim = dmax(0.0, dmin(term, dmin(fa, terp)));
+    double lim = dmax(0.0, std::fmin(term, std::fmin(fa, terp)));
     return fsgn * lim;
 }
 

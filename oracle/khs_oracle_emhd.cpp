@@ -57,7 +57,7 @@ inline void nu_mu(double eps, double& nu, double& mu) {
 inline double limiter(double flx, double dm1, double d0, double dp1) {
     double fsgn = std::copysign(1.0, d0);
     double term = fsgn * dm1, terp = fsgn * dp1, fa = std::fabs(flx);
-    return fsgn * dmax(0.0, dmin(term, dmin(fa, terp)));
+    return fsgn * dmax(0.0, std::fmin(term, std::fmin(fa, terp)));   // CUDA min = fmin
 }
 void fct_row(int n, const double* u, const double* v, const double* s, double* out, double dx, double dt) {
     std::vector<double> U(n + 6), V(n + 6), S(n + 6), LOR(n + 6), LNR(n + 6), FLX(n + 6);
