@@ -95,10 +95,10 @@ ZM_ROWS = {1: 16, 2: 32, 3: 64}       # chunk height of the fused z sweep + Maxw
 # precision, nx, nz).
 LIMITER = {
     ("k_fct_xz", "fp64", 2048, 1024):
-        "FP64 VALU issue and latency at 2 waves per SIMD (256 VGPRs): VALU busy 0.43 of the SIMD cycles, waves "
-        "wait on memory 0.32 of their life; without the steady rows' loads the kernel takes 149-172 us, without "
-        "its stores 178-183 us, and loading two rows ahead changes nothing, so the loads and stores add to the "
-        "arithmetic instead of hiding under it",
+        "its access pattern: the same loads, lane shifts and stores with a trivial sum in place of the FCT "
+        "arithmetic take 174-181 us (3.4 TB/s algorithmic, ~4.1 TB/s of traffic, mostly from HBM), ~80% of "
+        "the kernel's time; the rest is FP64 arithmetic that 2 waves per SIMD (256 VGPRs) do not hide "
+        "(VALU busy 0.43, a 7.5% cut in VALU instructions leaves the time unchanged)",
     ("k_maxwell", "fp64", 2048, 1024): "HBM: 702 MB per launch (1.20x algorithmic) at 5.6 TB/s; waves wait on "
                                        "memory 0.47 of their life",
 }
