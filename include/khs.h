@@ -141,6 +141,11 @@ typedef struct khs_config {
     int      snake_order;      /* 1 (default): consecutive step kernels walk the grid in opposite directions (cache reuse); 0: all forward */
     int      zm_fused;         /* YM step: -1 = autotuned (default), 0 = split z sweep and Maxwell, 1/2/3 = one fused kernel with 16/32/64-row chunks */
     int      step_graphs;      /* 1 (default): khs_step replays whole energy strides (even) as one captured HIP graph; 0: eager launches */
+    int      step_fusion;      /* YM fused-sweep steps with no column filter (only sponge / xi_cut): Maxwell of a step and
+                                  the next step's sweeps as one launch; -1 = the autotuner's choice (default), 0 = off,
+                                  1 = on */
+    int      mxz_variant;      /* that launch's shape: -1 = autotuned (default), 0..5 (16/32/64-row chunks, Maxwell
+                                  loading one row ahead; 3..5 two rows ahead) */
     /* --- derived (khs_derive) --- */
     int    nx, nz;
     int    periodic_x;         /* run_mode >= 1 */
@@ -329,9 +334,10 @@ const char* khs_kernel_name(int slot);
  * over two waves (0/1), x-sweep cells per lane of the register form (0 = LDS
  * form; the register form reports its 62*C-cell wave segment, 1 row, and 1
  * for variables one at a time), fused z sweep + Maxwell kernel (0 = split
- * kernels, 1/2/3 = fused with 16/32/64-row chunks), fused x+z sweep in use (0/1)}; chosen by the
- * autotuner on the first step unless autotune = 0. */
-int khs_get_tuning(khs_ctx* ctx, int out[11]);
+ * kernels, 1/2/3 = fused with 16/32/64-row chunks), fused x+z sweep in use (0/1), Maxwell and the next
+ * step's sweeps in one launch (0/1), that launch's variant (see mxz_variant)}; chosen by the autotuner on the
+ * first step unless autotune = 0. */
+int khs_get_tuning(khs_ctx* ctx, int out[13]);
 
 /* ---------------- batches of runs (parameter sweeps) ---------------- */
 /* No counterpart in the reference, which runs one configuration per process

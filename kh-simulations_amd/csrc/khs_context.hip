@@ -47,9 +47,9 @@ namespace khs {
             throw Error(KHS_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_));   \
     } while (0)
 
-enum Slot { K_FCT_X = 0, K_FCT_Z, K_FCT_XZ, K_MAXWELL, K_FILTERS, K_ENERGY, K_FCT_ZM, K_NSLOTS };
+enum Slot { K_FCT_X = 0, K_FCT_Z, K_FCT_XZ, K_MAXWELL, K_FILTERS, K_ENERGY, K_FCT_ZM, K_MXZ, K_NSLOTS };
 static const char* kSlotNames[K_NSLOTS] = {"k_fct_x", "k_fct_z", "k_fct_xz", "k_maxwell", "filters", "energy",
-                                           "k_fct_zm"};
+                                           "k_fct_zm", "k_mxz"};
 
 static const int kFilt15[15] = {S_BY1, S_EX1, S_EZ1, S_BY2, S_BY3, S_EX2, S_EX3, S_EZ2, S_EZ3,
                                 S_AZ2, S_AZ3, S_Q2A, S_Q3A, S_Q2B, S_Q3B};
@@ -67,8 +67,8 @@ struct ModeTable {
 };
 
 // process-wide cache of autotuned kernel choices, keyed by Ctx::tune_key()
-static std::map<std::vector<long>, std::array<int, 6>>& tune_cache() {
-    static std::map<std::vector<long>, std::array<int, 6>> m;
+static std::map<std::vector<long>, std::array<int, 8>>& tune_cache() {
+    static std::map<std::vector<long>, std::array<int, 8>> m;
     return m;
 }
 static std::mutex& tune_cache_mutex() {
@@ -131,6 +131,11 @@ struct Ctx {
     int xvar = 2, zvar = 1, mvar = 0, xzvar = 7;
     bool fused = false;                 // khs_step runs the fused x+z sweep
     int zmvar = 0;                            // 0: split z sweep + Maxwell; 1..3: fused kernel
+    // Maxwell of one step and the sweeps of the next in one launch (k_mxz):
+    // khs_step runs a segment of L steps as sweeps, L - 1 of those launches,
+    // then Maxwell and the filters
+    bool mxz = false;
+    int mxzvar = 1;
     // one energy stride of steps + the energy reduction, captured once
     hipGraphExec_t sgraph[2] = {nullptr, nullptr};   // by the ping-pong buffer a stride starts from
     bool tuned = false;
@@ -147,6 +152,20 @@ struct Ctx {
         State<T> s;
         for (int i = 0; i < S_COUNT; ++i) s.a[i] = static_cast<T*>(st[k][i]);
         return s;
+    }
+    // the 12 fields of buffer set fk with the 12 fluid arrays of set lk
+    template <typename T> State<T> mixed(int fk, int lk) const {
+        State<T> s;
+        for (int i = 0; i < S_COUNT; ++i) s.a[i] = static_cast<T*>(st[i < S_NA ? fk : lk][i]);
+        return s;
+    }
+    // k_mxz needs a step whose only filters are per column (sponge, xi_cut)
+    // and no warm-closure terms
+    bool mxz_eligible() const {
+        const khs_config& c = cfg;
+        const bool column = c.suppress_kz0 || c.kz_suppress_max >= 1 || c.kz_suppress_hi > c.kz_suppress_max + 1 ||
+                            c.kz_keep_max > 0 || c.hyp_diff > 0.0f;
+        return !emhd && fused && !column && kp.warm_T <= 0.0;
     }
     template <typename T> Stage<T> stg() const {
         Stage<T> s;
@@ -258,7 +277,7 @@ struct Ctx {
     std::vector<long> tune_key() const {
         return {nx, nz, fp32, emhd, nsp, kp.periodic_x, cell_filters, cfg.fused_sweep, cfg.x_segment, cfg.x_rows,
                 cfg.x_serial, cfg.x_lane_cells, cfg.z_chunk, cfg.z_prefetch, cfg.z_groups, cfg.maxwell_kernel,
-                cfg.zm_fused, cfg.xz_variant};
+                cfg.zm_fused, cfg.xz_variant, cfg.step_fusion, cfg.mxz_variant};
     }
     template <typename T> void do_autotune() {
         tuned = true;
@@ -295,6 +314,8 @@ struct Ctx {
         const bool try_fused = cfg.fused_sweep < 0 && !split_keys;
         fused = cfg.fused_sweep > 0 || try_fused;
         xzvar = (cfg.xz_variant >= 0) ? cfg.xz_variant : (emhd ? 5 : 7);
+        mxzvar = (cfg.mxz_variant >= 0) ? cfg.mxz_variant : 1;
+        mxz = cfg.step_fusion != 0 && mxz_eligible();
         if (!cfg.autotune) return;
         // contexts of the same shape in one process (sweeps, campaigns) reuse
         // the first one's choice: every variant computes the same bits
@@ -305,6 +326,7 @@ struct Ctx {
             if (it != tune_cache().end()) {
                 xvar = it->second[0]; zvar = it->second[1]; mvar = it->second[2];
                 xzvar = it->second[3]; zmvar = it->second[4]; fused = it->second[5] != 0;
+                mxzvar = it->second[6]; mxz = it->second[7] != 0;
                 return;
             }
         }
@@ -313,7 +335,8 @@ struct Ctx {
             std::vector<long> key;
             ~Store() {
                 std::lock_guard<std::mutex> lk(tune_cache_mutex());
-                tune_cache()[key] = {c->xvar, c->zvar, c->mvar, c->xzvar, c->zmvar, c->fused ? 1 : 0};
+                tune_cache()[key] = {c->xvar, c->zvar, c->mvar, c->xzvar, c->zmvar, c->fused ? 1 : 0, c->mxzvar,
+                                     c->mxz ? 1 : 0};
             }
         } store{this, key};
         float tx = 0.f, tz = 0.f, txz = 0.f;
@@ -328,6 +351,7 @@ struct Ctx {
                 xzvar = pick<T>(xzc, [&](int v) { fct_xz<T>(v); }, &txz);
             if (try_fused) fused = txz < tx + tz;
         }
+        mxz = cfg.step_fusion != 0 && mxz_eligible();
         zmvar = (!emhd && cfg.zm_fused > 0) ? cfg.zm_fused : 0;
         if (!emhd && cfg.zm_fused < 0 && !fused) {
             // split z sweep + (tuned) Maxwell against the fused kernel's chunk heights
@@ -350,14 +374,35 @@ struct Ctx {
             // writes the next state's fields from the current state: scratch here
             mvar = pick<T>(mc, [&](int v) { launch_emhd_maxwell<T>(state<T>(cur), state<T>(1 - cur), kp, v, stream); });
         }
-        if (!emhd && cfg.maxwell_kernel < 0) {
+        if (!emhd) {
             std::vector<int> mc;
-            for (int v = 0; v < kNumMaxwellVariants; ++v) mc.push_back(v);
+            if (cfg.maxwell_kernel < 0)
+                for (int v = 0; v < kNumMaxwellVariants; ++v) mc.push_back(v);
+            else
+                mc.push_back(mvar);
             // writes the next state's fields from the current state: scratch here
-            mvar = pick<T>(mc, [&](int v) {
-                launch_maxwell<T>(state<T>(cur), state<T>(1 - cur), kp, cell_filters ? d_decay : nullptr,
-                                  cell_filters ? d_action : nullptr, v, stream);
-            });
+            float tmx = 0.f;
+            if (mc.size() > 1 || mxz)
+                mvar = pick<T>(mc, [&](int v) {
+                    launch_maxwell<T>(state<T>(cur), state<T>(1 - cur), kp, cell_filters ? d_decay : nullptr,
+                                      cell_filters ? d_action : nullptr, v, stream);
+                }, &tmx);
+            if (mxz) {
+                // Maxwell + next sweeps in one launch against the two kernels,
+                // timed on the current state into the next-state set (scratch)
+                std::vector<int> vc;
+                for (int v = 0; v < kNumMxzVariants; ++v)
+                    if (cfg.mxz_variant < 0 || v == cfg.mxz_variant) vc.push_back(v);
+                float tmxz = 0.f;
+                mxzvar = pick<T>(vc, [&](int v) {
+                    launch_maxwell_sweep<T>(state<T>(cur), state<T>(1 - cur), kp, cell_filters ? d_decay : nullptr,
+                                            cell_filters ? d_action : nullptr, v, stream);
+                }, &tmxz);
+                if (cfg.step_fusion < 0) {
+                    if (txz <= 0.f) pick<T>({xzvar}, [&](int v) { fct_xz<T>(v); }, &txz);
+                    mxz = tmxz < txz + tmx;
+                }
+            }
         }
     }
     void autotune() {
@@ -373,6 +418,36 @@ struct Ctx {
         timed(K_FCT_XZ, [&] { fct_xz<T>(xzvar); });
         check_launch();
     }
+    // Maxwell of the step whose fields are in set X and swept fluid in set
+    // Y, then the next step's sweeps: fields -> Y, fluid -> X
+    template <typename T> void mxz_launch(int X, int Y, int v) {
+        launch_maxwell_sweep<T>(mixed<T>(X, Y), mixed<T>(Y, X), kp, cell_filters ? d_decay : nullptr,
+                                cell_filters ? d_action : nullptr, v, stream);
+    }
+    // L >= 2 steps from set cur: the sweeps of the first step, L - 1 launches
+    // of Maxwell + the next step's sweeps, then Maxwell, the filters and the
+    // commit of the last step.  Between the launches the state is one step
+    // apart (fields of step n in X, fluid after the sweeps of step n in Y);
+    // the charges' filter of each inner step is applied by the next launch as
+    // it reads them, so nothing but the last step's filters is left to do.
+    template <typename T> void do_mxz_steps(int L) {
+        set_order(0);
+        do_xz<T>();
+        int X = cur, Y = 1 - cur;
+        for (int j = 1; j < L; ++j) {
+            step += 1;
+            set_order(0);
+            timed(K_MXZ, [&] { mxz_launch<T>(X, Y, mxzvar); });
+            check_launch();
+            std::swap(X, Y);
+        }
+        cur = X;                  // fields of the last step's start; its swept fluid in 1 - cur
+        do_maxwell<T>();
+        do_filters<T>();
+        cur = 1 - cur;            // commit
+        step += 1;
+    }
+    void mxz_steps(int L) { fp32 ? do_mxz_steps<float>(L) : do_mxz_steps<double>(L); }
     // z sweep of both species then Maxwell, as the split kernels (v = 0) or
     // the fused kernel (v = 1..3); writes the next state
     template <typename T> void zm_launch(int v) {
@@ -749,6 +824,7 @@ static void commit(Ctx& c) {
 }
 
 static void full_step(Ctx& c);
+static void run_steps(Ctx& c, int L);
 
 // Capture `stride` full steps and the energy reduction (into the pinned
 // h_red[0]) as one HIP graph, starting from ping-pong buffer c.cur, into
@@ -767,7 +843,7 @@ static void capture_stride(Ctx& c, int stride) {
     std::lock_guard<std::mutex> lk(capture_mutex());
     HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeRelaxed));
     try {
-        for (int s = 0; s < stride; ++s) full_step(c);
+        run_steps(c, stride);
         c.energy_async();
     } catch (...) {
         hipGraph_t g = nullptr;
@@ -811,6 +887,16 @@ static void full_step(Ctx& c) {
     }
     c.filter_stage();
     commit(c);
+}
+
+// L consecutive full steps without an energy check between them.
+static void run_steps(Ctx& c, int L) {
+    c.autotune();
+    if (c.mxz && L >= 2) {
+        c.mxz_steps(L);
+        return;
+    }
+    for (int s = 0; s < L; ++s) full_step(c);
 }
 
 static void write_energy_row(Ctx& c, long step, double E) {
@@ -889,7 +975,7 @@ struct Batch {
             c.stream = lanes[lane_of[k]];
             const int cur0 = c.cur;
             const long step0 = c.step;
-            for (int s = 0; s < K; ++s) full_step(c);
+            run_steps(c, K);
             c.energy_async();
             c.cur = cur0;                                   // the graph does the stepping
             c.step = step0;
@@ -1419,8 +1505,11 @@ int khs_step(khs_ctx* ctx, int nsteps, int* halted) {
                 }
                 continue;
             }
-            ++s;
-            khs::full_step(c);
+            // the steps up to the next energy check (or the end) as one segment
+            long L = nsteps - s;
+            if (stride > 0) L = std::min<long>(L, stride - c.step % stride);
+            khs::run_steps(c, (int)L);
+            s += (int)L;
             if (stride > 0 && c.step % stride == 0) {
                 double E = c.energy();
                 khs::write_energy_row(c, c.step, E);
@@ -1805,7 +1894,7 @@ int khs_kernel_times(khs_ctx* ctx, double* ms, int* launches, int nslots) {
     });
 }
 
-int khs_get_tuning(khs_ctx* ctx, int out[11]) {
+int khs_get_tuning(khs_ctx* ctx, int out[13]) {
     CTX_OR_FAIL(ctx);
     if (!out) { g_last_error = "null argument"; return KHS_ERR_ARG; }
     Ctx& c = *ctx->impl;
@@ -1821,6 +1910,8 @@ int khs_get_tuning(khs_ctx* ctx, int out[11]) {
     out[8] = 0;
     out[9] = c.zmvar;
     out[10] = c.fused ? 1 : 0;
+    out[11] = c.mxz ? 1 : 0;
+    out[12] = c.mxzvar;
     if (c.xvar >= 18) {
         const int cells = c.xvar == 18 ? 4 : 8;
         out[0] = 62 * cells;

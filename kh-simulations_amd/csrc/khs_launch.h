@@ -58,6 +58,16 @@ void launch_fct_zmx(const State<T>& S, const Stage<T>& X, const State<T>& O, con
 template <typename T>
 void launch_maxwell(const State<T>& S, const State<T>& O, const KParams& p, const double* decay,
                     const int* action, int variant, hipStream_t st);
+// Maxwell of step n (with the sponge / xi_cut of its colour-2/3 fields) and
+// the fused x + z sweeps of step n + 1 (with the same filter applied to the
+// charges it reads) in one launch, YM model without column filters.  I:
+// fields of step n, fluid after the sweeps of step n; O: fields of step
+// n + 1, fluid after the sweeps of step n + 1.  Variants 0/1/2 = 16/32/64-row
+// chunks, 3/4/5 the same with Maxwell loading two rows ahead.
+constexpr int kNumMxzVariants = 6;
+template <typename T>
+void launch_maxwell_sweep(const State<T>& I, const State<T>& O, const KParams& p, const double* decay,
+                          const int* action, int variant, hipStream_t st);
 
 // filters (khs_aux_kernels.hip)
 constexpr int kMaxFilterArrays = 24;

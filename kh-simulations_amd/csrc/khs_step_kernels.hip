@@ -2823,6 +2823,456 @@ void launch_maxwell(const State<T>& S, const State<T>& O, const KParams& p, cons
         hipLaunchKernelGGL((k_maxwell<T, 16>), grid, dim3(MX * MZT), 0, st, S, O, p, ntx, ntz, decay, action);
 }
 
+// ==========================================================================
+// Maxwell of step n and the fused sweeps of step n + 1 in one launch (YM
+// model; only per-column filters -- sponge, xi_cut -- between them).  The
+// reference's order is sweeps -> Maxwell -> filters -> the next step's
+// sources and sweeps (main_ym.cu:322-455).  Maxwell and the sweeps are local
+// in x and z, so one workgroup can do both for a column tile and z chunk:
+//   * wave 4 marches Maxwell (the arithmetic of k_maxwell_zm, operation for
+//     operation) over the rows the tile's sweep reads, on the fields of step
+//     n and the fluid after the sweeps of step n; it applies the column
+//     action to the colour-2/3 fields, stores the tile's own cells and puts
+//     each new field row into an LDS ring of three rows;
+//   * waves 0..3 march the fused x + z sweep of xz_march_g (species w / 2,
+//     variable group w % 2) on that fluid, with the column action applied to
+//     the colour-2/3 charges as they are loaded (the filter stage acts on the
+//     charges after Maxwell, whose currents use them unfiltered), and take
+//     the new field rows of their sources from the ring.
+// The workgroup meets once per row at a barrier that waits for LDS traffic
+// only (the fluid loads of the next row stay in flight).  Per cell-update
+// the launch reads the fluid and the fields once and writes them once: 47
+// arrays, against 36 + 35 for the sweep and Maxwell kernels.
+// I: fields of step n, fluid after the sweeps of step n; O: fields of step
+// n + 1 (before the charge filter), fluid after the sweeps of step n + 1.
+// ==========================================================================
+constexpr int MXZ_SLOTS = 3;
+
+// Workgroup barrier ordering LDS accesses only: no wait for outstanding
+// global loads or stores.
+__device__ __forceinline__ void wg_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// The ring: slot k % 3 holds the k-th new field row of the tile (state ids
+// S_EX1 .. S_AZ3, one value per lane).  Rows are produced in the order the
+// sweep reads them: zm_per(rc0) first (k = 0), then rc0, rc0 + 1, ... (k = 1
+// + row - rc0), rc0 = the chunk's first (clamped) halo row.
+template <typename T>
+using MxzRing = T[MXZ_SLOTS][12][64];
+
+// Wave 4: the Maxwell march.  mstep(fin, k) consumes the inputs of row q
+// (cur), computes Ampere's Ex there and, with fin, finishes row q - 1 (Jz,
+// Ampere's Ez, potential, Faraday) into ring slot k % 3; then moves to row
+// q + 1.  Row q - 1 of the march is thus the k-th produced row.
+template <typename T, int TZ, int PF>
+__device__ __forceinline__ void mxz_maxwell(const State<T>& I, const State<T>& O, const KParams& p, int g, int ch,
+                                            int lane, MxzRing<T>& ring, const double* decay, const int* action) {
+    const int nx = p.nx, nz = p.nz;
+    const bool per = p.periodic_x != 0;
+    const int x = g * XZR_OUT - 3 + lane;
+    const int xc = map_x(x, p);
+    const int xm = xm_src(xc, p);
+    const bool wall = !per && (xc == 0 || xc == nx - 1);     // YM_Fluid.cu:23-26
+    const bool last_wall = !per && xc == nx - 1;              // Faraday: E(x+1) := E(x)
+    const bool store = lane >= 3 && lane < 3 + XZR_OUT && x >= 0 && x < nx;
+    const int z0 = ch * TZ;
+    const int zend = min(z0 + TZ, nz);
+    const int elast = zend + 2;
+    const int rc0 = clampi(z0 - 3, 0, nz - 1);
+    const int rf = zm_per(rc0, nz);
+    const T dt = T(p.dt), c2 = T(p.c2), aYM = T(p.alpha), eps0 = T(p.eps0);
+    const T dx = T(p.dx), dz = T(p.dz);
+    const int act = action ? action[xc] : 0;
+    const T dec = (act == 1) ? T(decay[xc]) : T(1.0);
+    const unsigned bytes = (unsigned)nx * (unsigned)nz * (unsigned)sizeof(T);
+    __amdgpu_buffer_rsrc_t rby[3], rex[3], rez[3], raz[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        rby[c] = buf_rsrc(O.a[S_BY1 + c], bytes);
+        rex[c] = buf_rsrc(O.a[S_EX1 + c], bytes);
+        rez[c] = buf_rsrc(O.a[S_EZ1 + c], bytes);
+        raz[c] = buf_rsrc(O.a[S_AZ1 + c], bytes);
+    }
+    // march rows count from rf - 1 and wrap periodically (z is periodic for Maxwell)
+    auto rbase = [&](int r) { return (unsigned)((r + nz) % nz) * (unsigned)nx; };
+
+    T byp[3], byxmp[3], azp[3], ezp[3], exn_p[3], uzp[2], nqp[2][3];
+    {
+        const unsigned im = rbase(rf - 1) + (unsigned)xc;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) byp[c] = ld32(I.a[S_BY1 + c], im);
+    }
+    MRow<T> cur, nxt;
+    int q = rf;
+    mrow_load(I, I, rbase(q) + (unsigned)xc, rbase(q) + (unsigned)xm, cur);
+    if (PF) mrow_load(I, I, rbase(q + 1) + (unsigned)xc, rbase(q + 1) + (unsigned)xm, nxt);
+
+    auto mstep = [&](bool fin, int k) {
+        MRow<T> nn;
+        if (PF) mrow_load(I, I, rbase(q + 2) + (unsigned)xc, rbase(q + 2) + (unsigned)xm, nn);
+        // cell quantities of the currents (kernel_ym_currents, YM_Physics.cu:9-60)
+        T ux[2], uz[2], nq[2][3];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const T nf = kmax(cur.n[s], T(1e-10));
+            ux[s] = cur.px[s] / nf;
+            uz[s] = cur.pz[s] / nf;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) nq[s][c] = cur.n[s] * cur.q[s][c];
+        }
+        // Jx at (x+1/2, q) with the x+1 values of lane l+1
+        T Jx[3];
+        {
+            T vxr[2], nQx[2][3];
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                vxr[s] = T(0.5) * (ux[s] + lane_from_right(ux[s]));
+#pragma unroll
+                for (int c = 0; c < 3; ++c) nQx[s][c] = T(0.5) * (nq[s][c] + lane_from_right(nq[s][c]));
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) Jx[c] = -(nQx[0][c] * vxr[0] + nQx[1][c] * vxr[1]);
+        }
+        // Ampere for Ex at row q (YM_Physics.cu:70-113)
+        T exn[3];
+        {
+            const T cr[3] = {cur.az[1] * cur.by[2] - cur.az[2] * cur.by[1],
+                             cur.az[2] * cur.by[0] - cur.az[0] * cur.by[2],
+                             cur.az[0] * cur.by[1] - cur.az[1] * cur.by[0]};
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                T dBdz = (cur.by[c] - byp[c]) / dz;
+                exn[c] = cur.ex[c] + dt * (c2 * (-dBdz - aYM * cr[c]) - Jx[c] / eps0);
+            }
+        }
+        if (fin) {
+            // row q-1: Jz at (x, q-1/2), Ampere for Ez, potential, Faraday
+            T ezn[3];
+            {
+                T vzt[2], nQz[2][3];
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    vzt[s] = T(0.5) * (uzp[s] + uz[s]);
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) nQz[s][c] = T(0.5) * (nqp[s][c] + nq[s][c]);
+                }
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    T Jz = -(nQz[0][c] * vzt[0] + nQz[1][c] * vzt[1]);
+                    T dBdx = (byp[c] - byxmp[c]) / dx;
+                    T ez = wall ? T(0.0) : ezp[c];
+                    ezn[c] = ez + dt * (c2 * dBdx - Jz / eps0);
+                }
+            }
+            T ezr[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                ezr[c] = lane_from_right(ezn[c]);
+                if (last_wall) ezr[c] = ezn[c];
+            }
+            // kernel_ym_potential (YM_Physics.cu:589-599), kernel_ym_faraday (:120-151)
+            T Az[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                Az[c] = azp[c];
+                if (c > 0 || !p.freeze_az1) Az[c] = Az[c] - dt * ezn[c];
+            }
+            const T cr[3] = {Az[1] * exn_p[2] - Az[2] * exn_p[1],
+                             Az[2] * exn_p[0] - Az[0] * exn_p[2],
+                             Az[0] * exn_p[1] - Az[1] * exn_p[0]};
+            const int r = (q - 1 + nz) % nz;
+            // the tile's own cells (each row once: k >= 1)
+            const bool own = store && k >= 1 && r >= z0 && r < zend;
+            const unsigned off = own ? ((unsigned)r * (unsigned)nx + (unsigned)xc) * (unsigned)sizeof(T) : kNoStore;
+            const int slot = k % MXZ_SLOTS;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                T dEx = (exn[c] - exn_p[c]) / dz;
+                T dEz = (ezr[c] - ezn[c]) / dx;
+                T byn = byp[c] - dt * (dEx - dEz - aYM * cr[c]);
+                T exo = exn_p[c], ezo = ezn[c], azo = Az[c];
+                // sponge / xi_cut of the colour-2/3 fields (YM_Physics.cu:161-210)
+                if (c > 0 && act == 1) { byn *= dec; exo *= dec; ezo *= dec; azo *= dec; }
+                if (c > 0 && act == 2) { byn = T(0.0); exo = T(0.0); ezo = T(0.0); azo = T(0.0); }
+                buf_st(byn, rby[c], off);
+                buf_st(exo, rex[c], off);
+                buf_st(ezo, rez[c], off);
+                if (c > 0 || !p.freeze_az1) buf_st(azo, raz[c], off);
+                ring[slot][S_EX1 + c][lane] = exo;
+                ring[slot][S_EZ1 + c][lane] = ezo;
+                ring[slot][S_BY1 + c][lane] = byn;
+                ring[slot][S_AZ1 + c][lane] = azo;
+            }
+        }
+        // carry row q
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            byp[c] = cur.by[c]; byxmp[c] = cur.byxm[c]; azp[c] = cur.az[c]; ezp[c] = cur.ez[c]; exn_p[c] = exn[c];
+        }
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            uzp[s] = uz[s];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) nqp[s][c] = nq[s][c];
+        }
+        ++q;
+        if (PF) {
+            cur = nxt;
+            nxt = nn;
+        } else {
+            mrow_load(I, I, rbase(q) + (unsigned)xc, rbase(q) + (unsigned)xm, cur);
+        }
+    };
+
+    // rows k = 0 (zm_per(rc0)) and 1 (rc0) before the sweep's first row
+    mstep(false, 0);
+    mstep(true, 0);
+    mstep(true, 1);
+    wg_sync_lds();
+    // then the row the sweep reads next, whenever its row advances (the same
+    // row sequence as xz_march_g: rc(e) = clamp(min(e, elast), 0, nz - 1))
+    int rc = rc0;
+#pragma unroll 1
+    for (int e = z0 - 3; e <= elast; ++e) {
+        const int rn = clampi(min(e + 1, elast), 0, nz - 1);
+        if (rn != rc) mstep(true, 1 + rn - rc0);
+        rc = rn;
+        wg_sync_lds();
+    }
+}
+
+// Waves 0..3: xz_march_g on the fluid of I with the new field rows from the
+// ring.  Lane l reads the fields of its column at ring lane lx and those of
+// the source stencil's x-1 column at lxm: l and l-1, except for lanes beyond
+// a wall, which take their clamped column's lane (so their sources are
+// lorentz_px / lorentz_pz / precession at that column, as the reference's
+// clamped halo loads give).
+template <typename T, int TZ, typename GR, bool EDGE>
+__device__ __forceinline__ void mxz_sweep(const State<T>& I, const State<T>& O, const KParams& p, int sp, int g,
+                                          int ch, int lane, const MxzRing<T>& ring, const double* decay,
+                                          const int* action) {
+    typedef YMModel M;
+    constexpr int NV = F_COUNT, NG = GR::NGV, VB = GR::VB;
+    const int nx = p.nx, nz = p.nz;
+    const int x = g * XZR_OUT - 3 + lane;
+    const int xc = map_x(x, p);
+    const bool inside = !EDGE || (x >= 0 && x < nx);
+    const bool store = lane >= 3 && lane < 3 + XZR_OUT && inside;
+    int lx = lane, lxm = lane - 1;
+    if (EDGE && !p.periodic_x) {
+        lx = lane + (xc - x);
+        lxm = xc > 0 ? lx - 1 : lx;
+    }
+    lxm = lxm < 0 ? 0 : lxm;                    // lane 0's x-1 values are not used
+    const bool wall = !p.periodic_x && (xc == 0 || xc == nx - 1);
+    const int b = M::base(sp);
+    const T dt = T(p.dt), dtdx = T(p.dtdx), dtdz = T(p.dtdz);
+    const int z0 = ch * TZ;
+    const int zend = min(z0 + TZ, nz);
+    const int elast = zend + 2;
+    const int rc0 = clampi(z0 - 3, 0, nz - 1);
+    // sponge / xi_cut of the charges Q2, Q3 (the filter stage of step n)
+    const bool cut = action != nullptr;
+    const int act = cut ? action[xc] : 0;
+    const T dec = (act == 1) ? T(decay[xc]) : T(1.0);
+    const unsigned bytes = (unsigned)nx * (unsigned)nz * (unsigned)sizeof(T);
+    __amdgpu_buffer_rsrc_t orsc[NG];
+#pragma unroll
+    for (int j = 0; j < NG; ++j) orsc[j] = buf_rsrc(O.a[b + VB + j], bytes);
+
+    auto load = [&](int r, T (&F)[NV]) {
+        const unsigned i = (unsigned)xc + (unsigned)r * (unsigned)nx;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) F[v] = ld32(I.a[b + v], i);
+        if (cut) {
+            // k_cell_filters: multiply by the column's decay, or zero
+            F[F_Q2] = (act == 2) ? T(0.0) : F[F_Q2] * dec;
+            F[F_Q3] = (act == 2) ? T(0.0) : F[F_Q3] * dec;
+        }
+    };
+
+    int rc = rc0;
+    T F[NV];
+    load(rc, F);
+    ZPipe<T, NG> zp;
+    zp.Um = T(0);
+    zp.MUm = T(0);
+    wg_sync_lds();                              // ring rows zm_per(rc0), rc0 are in place
+
+    auto row = [&](int e, auto mode) {
+        constexpr int MODE = decltype(mode)::value;
+        const int rn = MODE ? rc + 1 : clampi(min(e + 1, elast), 0, nz - 1);
+        T nF[NV];
+        load(rn, nF);
+        const int kc = 1 + rc - rc0;            // ring index of row rc; row rc's predecessor is kc - 1
+        const T(&cs)[12][64] = ring[kc % MXZ_SLOTS];
+        const T(&ps)[12][64] = ring[(kc + MXZ_SLOTS - 1) % MXZ_SLOTS];
+
+        // ---- x sweep of row rc at this lane's column (YM_FCT.cu:30-91) ----
+        T src = T(0.0);
+        if (GR::has(F_PX)) {                   // px source (kernel_ym_lorentz, YM_Fluid.cu:32-84)
+            const T n_safe = kmax(F[F_N], T(0.05));
+            const T vz = F[F_PZ] / n_safe;
+            T acc[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                T Exn = T(0.5) * (cs[S_EX1 + c][lx] + cs[S_EX1 + c][lxm]);
+                T Byn = T(0.25) * (cs[S_BY1 + c][lx] + cs[S_BY1 + c][lxm] + ps[S_BY1 + c][lx] + ps[S_BY1 + c][lxm]);
+                acc[c] = F[F_Q1 + c] * (Exn + vz * Byn);
+            }
+            src = -(acc[0] + acc[1] + acc[2]);
+        }
+        T V[NG];
+        {
+            const T u = F[F_PX] / kmax(F[F_N], T(0.05));          // update_velocities
+            const T ur = lane_from_right(u);
+            T vi = T(0.5) * (u + ur);
+            const T eps = vi * dtdx;
+            T nu, mu;
+            fct_nu_mu(eps, nu, mu);
+#pragma unroll
+            for (int j = 0; j < NG; ++j) {
+                const int k = VB + j;
+                const T v0 = F[k];
+                const T vp = lane_from_right(v0);
+                T fR = T(0.5) * eps * (v0 + vp);
+                T dR = nu * (vp - v0);
+                const T fL = lane_from_left(fR);
+                const T dL = lane_from_left(dR);
+                T sk = (k == F_PX) ? src : T(0.0);
+                T lor = v0 + dt * sk - (fR - fL);
+                T lnr = lor + (dR - dL);
+                const T lorr = lane_from_right(lor);
+                const T lnrl = lane_from_left(lnr);
+                const T lnrr = lane_from_right(lnr);
+                const T lnrr2 = lane_from_right(lnrr);
+                T flx = mu * (lorr - lor);
+                T fl = fct_limit(flx, lnr - lnrl, lnrr - lnr, lnrr2 - lnrr);
+                const T fll = lane_from_left(fl);
+                V[j] = lnr - (fl - fll);
+            }
+        }
+
+        // ---- z sweep of the x-swept row: stale z velocity and the z sources ----
+        T Sv[NG], out[NG];
+        const T uz = F[F_PZ] / kmax(F[F_N], T(0.05));
+        {
+            const T q[3] = {F[F_Q1], F[F_Q2], F[F_Q3]};
+            T spz = T(0), sq[3] = {T(0), T(0), T(0)};
+            if (GR::has(F_PZ)) {
+                T ez[3], ezm[3], by[3], byxm[3], byzm[3], byxmzm[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    ez[c] = cs[S_EZ1 + c][lx];
+                    ezm[c] = ps[S_EZ1 + c][lx];
+                    by[c] = cs[S_BY1 + c][lx];
+                    byxm[c] = cs[S_BY1 + c][lxm];
+                    byzm[c] = ps[S_BY1 + c][lx];
+                    byxmzm[c] = ps[S_BY1 + c][lxm];
+                }
+                spz = lorentz_pz_v(F[F_N], F[F_PX], q, ez, ezm, by, byxm, byzm, byxmzm);
+            }
+            if (GR::has(F_Q1)) {
+                T az[3], azm[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    az[c] = cs[S_AZ1 + c][lx];
+                    azm[c] = ps[S_AZ1 + c][lx];
+                }
+                precession_v(F[F_N], F[F_PZ], q, az, azm, T(p.alpha), sq);
+            }
+#pragma unroll
+            for (int j = 0; j < NG; ++j) {
+                const int k = VB + j;
+                Sv[j] = (k == F_PZ) ? spz : (k >= F_Q1) ? sq[k >= F_Q1 ? k - F_Q1 : 0] : T(0);
+            }
+        }
+        bool have = true;
+        if (MODE) zp.push_steady(V, uz, Sv, dt, dtdz, out);
+        else have = zp.push(e, z0, V, uz, Sv, dt, dtdz, out);
+        if (have) {
+            if (GR::has(F_N) && out[F_N - VB] < T(0.05)) out[F_N - VB] = T(0.05);   // kernel_enforce_boundaries
+            if (GR::has(F_PX) && wall) out[F_PX - VB] = T(0.0);
+            const unsigned off =
+                store ? ((unsigned)xc + (unsigned)(e - 3) * (unsigned)nx) * (unsigned)sizeof(T) : kNoStore;
+#pragma unroll
+            for (int j = 0; j < NG; ++j) buf_st_nt(out[j], orsc[j], off);
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) F[v] = nF[v];
+        rc = rn;
+        wg_sync_lds();                          // ring: row rn + 1 in, row rc - 1 out
+    };
+
+    int e = z0 - 3;
+    for (; e < z0 + 3; ++e) row(e, std::integral_constant<int, 0>());
+    const int esteady = min(elast - 1, nz - 2);
+    for (; e + 2 <= esteady; e += 3) {
+        row(e, std::integral_constant<int, 1>());
+        row(e + 1, std::integral_constant<int, 1>());
+        row(e + 2, std::integral_constant<int, 1>());
+    }
+    for (; e <= esteady; ++e) row(e, std::integral_constant<int, 1>());
+    for (; e <= elast; ++e) row(e, std::integral_constant<int, 0>());
+}
+
+// Workgroup: waves 0..3 sweep (species w / 2, group w % 2), wave 4 Maxwell.
+template <typename T, int TZ, int PF>
+__global__ __launch_bounds__(320) void k_mxz(State<T> I, State<T> O, KParams p, int ngx, int nch,
+                                             const double* decay, const int* action) {
+    __shared__ MxzRing<T> ring;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    int lb = xcd_logical(blockIdx.x, gridDim.x);
+    if (lb >= ngx * nch) return;                // whole workgroups leave together
+    lb = __builtin_amdgcn_readfirstlane(tile_order(lb, ngx * nch, p.rev));
+    const int g = lb % ngx, ch = lb / ngx;
+    if (w == 4) {
+        mxz_maxwell<T, TZ, PF>(I, O, p, g, ch, lane, ring, decay, action);
+        return;
+    }
+    const int sp = w >> 1;
+    const bool edge = g == 0 || g * XZR_OUT + 60 >= p.nx;   // lanes cover columns 58g-3 .. 58g+60
+    if (w & 1) {
+        if (edge) mxz_sweep<T, TZ, XzQ3, true>(I, O, p, sp, g, ch, lane, ring, decay, action);
+        else mxz_sweep<T, TZ, XzQ3, false>(I, O, p, sp, g, ch, lane, ring, decay, action);
+    } else {
+        if (edge) mxz_sweep<T, TZ, XzN3, true>(I, O, p, sp, g, ch, lane, ring, decay, action);
+        else mxz_sweep<T, TZ, XzN3, false>(I, O, p, sp, g, ch, lane, ring, decay, action);
+    }
+}
+
+template <typename T, int TZ, int PF>
+static void launch_mxz_t(const State<T>& I, const State<T>& O, const KParams& p, const double* decay,
+                         const int* action, hipStream_t st) {
+    const int ngx = (p.nx + XZR_OUT - 1) / XZR_OUT;
+    const int nch = (p.nz + TZ - 1) / TZ;
+    hipLaunchKernelGGL((k_mxz<T, TZ, PF>), dim3(round8(ngx * nch)), dim3(320), 0, st, I, O, p, ngx, nch, decay,
+                       action);
+}
+
+// Variants 0/1/2: 16/32/64-row chunks, Maxwell loads one row ahead; 3/4/5:
+// the same loading two rows ahead.
+template <typename T>
+void launch_maxwell_sweep(const State<T>& I, const State<T>& O, const KParams& p, const double* decay,
+                          const int* action, int variant, hipStream_t st) {
+    switch (variant) {
+        case 0: launch_mxz_t<T, 16, 0>(I, O, p, decay, action, st); return;
+        case 2: launch_mxz_t<T, 64, 0>(I, O, p, decay, action, st); return;
+        case 3: launch_mxz_t<T, 16, 1>(I, O, p, decay, action, st); return;
+        case 4: launch_mxz_t<T, 32, 1>(I, O, p, decay, action, st); return;
+        case 5: launch_mxz_t<T, 64, 1>(I, O, p, decay, action, st); return;
+        default: launch_mxz_t<T, 32, 0>(I, O, p, decay, action, st); return;
+    }
+}
+template void launch_maxwell_sweep<double>(const State<double>&, const State<double>&, const KParams&, const double*,
+                                           const int*, int, hipStream_t);
+template void launch_maxwell_sweep<float>(const State<float>&, const State<float>&, const KParams&, const double*,
+                                          const int*, int, hipStream_t);
+
 #define KHS_SWEEP_INST(T, M)                                                                             \
     template void launch_fct_x<T, M>(const State<T>&, const Stage<T>&, const KParams&, int, int, int, hipStream_t); \
     template void launch_fct_z<T, M>(const State<T>&, const Stage<T>&, const State<T>&, const KParams&, int, int, int, hipStream_t);

@@ -76,6 +76,7 @@ class khs_config(ctypes.Structure):
         ("smooth_init", ctypes.c_int), ("spectrum_kmax", ctypes.c_int), ("spectrum_amp", ctypes.c_double),
         ("spectrum_seed", ctypes.c_uint64), ("em_perturb_amp", ctypes.c_double),
         ("z_groups", ctypes.c_int), ("xz_variant", ctypes.c_int), ("x_serial", ctypes.c_int), ("maxwell_kernel", ctypes.c_int), ("x_rows", ctypes.c_int), ("z_prefetch", ctypes.c_int), ("mfma_filter", ctypes.c_int), ("x_lane_cells", ctypes.c_int), ("snake_order", ctypes.c_int), ("zm_fused", ctypes.c_int), ("step_graphs", ctypes.c_int),
+        ("step_fusion", ctypes.c_int), ("mxz_variant", ctypes.c_int),
         ("nx", ctypes.c_int), ("nz", ctypes.c_int), ("periodic_x", ctypes.c_int), ("freeze_az1", ctypes.c_int),
         ("lx", ctypes.c_double), ("lz", ctypes.c_double), ("dx", ctypes.c_double), ("dz", ctypes.c_double),
         ("courant", ctypes.c_double), ("dt", ctypes.c_double),
@@ -433,14 +434,14 @@ class Simulation:
 
     def tuning(self):
         """Tile shapes and kernel variants in use (khs_get_tuning)."""
-        out = (ctypes.c_int * 11)()
+        out = (ctypes.c_int * 13)()
         _check(_lib.khs_get_tuning(self._h, out))
         return dict(x_segment=out[0], x_rows=out[1], z_chunk=out[2], z_prefetch=out[3], maxwell=out[4],
                     x_serial=out[5], xz_variant=out[6], z_groups=out[7], x_lane_cells=out[8], zm_fused=out[9],
-                    fused_sweep=out[10])
+                    fused_sweep=out[10], step_fusion=out[11], mxz_variant=out[12])
 
     def kernel_times(self):
-        n = 8
+        n = 16
         ms = (ctypes.c_double * n)()
         cnt = (ctypes.c_int * n)()
         _check(_lib.khs_kernel_times(self._h, ms, cnt, n))

@@ -302,7 +302,8 @@ def test_tile_variants_bitwise(xs, xr, zc, pf, mk, xv, zg, periodic):
     cfg, sim, o = make_pair(st, **keys)
     t = sim.tuning()
     assert t == dict(x_segment=xs, x_rows=xr, z_chunk=zc, z_prefetch=pf, maxwell=mk, x_serial=xv,
-                     xz_variant=t["xz_variant"], z_groups=zg, x_lane_cells=0, zm_fused=0, fused_sweep=0)
+                     xz_variant=t["xz_variant"], z_groups=zg, x_lane_cells=0, zm_fused=0, fused_sweep=0,
+                     step_fusion=0, mxz_variant=t["mxz_variant"])
     o.sources()
     sim.step(3)
     o.step(3)
@@ -403,6 +404,57 @@ def test_fused_sweep_variants_bitwise(xzv, shape, periodic, warm):
     o.step(4)
     for n in orc.STATE:
         assert_bitwise(sim.get(n), o.get(n), f"xz{xzv} {nx}x{nz} {n}")
+
+
+MXZ_CASES = {
+    # frozen Az1, xi_cut: the headline's step on the C1 shape
+    "walled_c1_xicut": dict(C1, n_steps=0, xi_cut=5.0),
+    "periodic_sponge_unfrozen": grid(300, 136, 1, xi_sponge=4.0, sigma_sponge=5.0, freeze_override=0, alpha_YM=2.0),
+    "walled_odd_cut_unfrozen": grid(70, 36, 0, xi_cut=2.0, freeze_override=0),
+    # many wave columns, both walls, chunks taller than the grid
+    "walled_wide_short": grid(1000, 20, 0, xi_cut=30.0),
+    # several chunks, the last one ragged; columns not a multiple of 58
+    "periodic_tall_nofilter": grid(116, 300, 1),
+    "walled_58_columns": grid(116, 64, 0, xi_sponge=3.0, sigma_sponge=2.0),
+}
+
+
+@pytest.mark.parametrize("v", list(range(6)))
+@pytest.mark.parametrize("case", list(MXZ_CASES))
+def test_maxwell_sweep_launch_bitwise(case, v):
+    """Maxwell of one step and the sweeps of the next in one launch (every
+    variant, forced): 7 steps with an energy check every 5 -- a 5-step
+    segment replayed as a captured graph, then 2 eager steps -- give the
+    oracle's bits, the sponge / xi_cut of the charges applied inside the
+    launch included."""
+    keys = dict(MXZ_CASES[case], fused_sweep=1, step_fusion=1, mxz_variant=v, autotune=0,
+                energy_stride=5)
+    nx, nz = keys["nx_override"], keys["nz_override"]
+    cfg, sim, o = make_pair(random_state(nx, nz, seed=nx + 7 * v, smooth=True), **keys)
+    o.sources()
+    assert not sim.step(7), "halted on the energy check"
+    o.step(7)
+    t = sim.tuning()
+    assert t["step_fusion"] == 1 and t["mxz_variant"] == v
+    for n in orc.STATE:
+        assert_bitwise(sim.get(n), o.get(n), f"mxz{v} {case} {n}")
+
+
+def test_maxwell_sweep_launch_eager_segments():
+    """The same launch without graphs and with segments that start in the
+    middle of an energy stride (khs_step(3) twice, then 1, then 4), FP64,
+    against the oracle; and the tuner's choice (step_fusion = -1) as well."""
+    for fusion in (1, -1):
+        keys = grid(300, 136, 0, xi_cut=8.0, fused_sweep=1, step_fusion=fusion, step_graphs=0,
+                    energy_stride=4)
+        cfg, sim, o = make_pair(random_state(300, 136, seed=41, smooth=True), **keys)
+        o.sources()
+        for k in (3, 3, 1, 4):
+            assert not sim.step(k), "halted on the energy check"
+            o.step(k)
+        assert sim.tuning()["step_fusion"] in ((1,) if fusion == 1 else (0, 1))
+        for n in orc.STATE:
+            assert_bitwise(sim.get(n), o.get(n), f"fusion {fusion} {n}")
 
 
 @pytest.mark.parametrize("extra,want", [(dict(fused_sweep=1), 1), (dict(fused_sweep=0), 0),
