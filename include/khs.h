@@ -129,10 +129,10 @@ typedef struct khs_config {
     uint64_t spectrum_seed;
     double   em_perturb_amp;   /* EMHD das_kaw perturbation (argv[2] as double, default 0.05); set by perturb_amp */
     int      z_groups;         /* z sweep: -1 = autotuned (default), 0 = all variables per lane, 1 = two variable groups on two waves */
-    int      xz_variant;       /* fused sweep kernel variant: -1 = autotuned (default), YM 0..11, EMHD 0..6 */
+    int      xz_variant;       /* fused sweep kernel variant: -1 = autotuned (default), YM 0..12, EMHD 0..6 */
     int      x_serial;         /* x sweep phases per variable: -1 = autotuned (default), 0 = all at once, 1 = one at a time */
     int      maxwell_kernel;   /* YM Maxwell: -1 = autotuned (default), 0/2/3 = 64x16/64x4/64x8 LDS tiles, 1 = per cell,
-                                  4..9 = z march with 16/32/16/8/8/4-row chunks (6, 8: without next-row prefetch);
+                                  4..10 = z march with 16/32/16/8/8/4/2-row chunks (6, 8: without next-row prefetch);
                                   EMHD: 0 = 64x16 tiles, 1/2/3 = z march with 4/8/16-row chunks */
     int      x_rows;           /* x sweep rows per workgroup: 0 = autotuned (default), 1, 4 or 16 */
     int      z_prefetch;       /* z sweep: -1 = autotuned (default), 0 = load each row when used, 1 / 2 = load one / two rows ahead */
@@ -326,11 +326,11 @@ int khs_kernel_times(khs_ctx* ctx, double* ms, int* launches, int nslots);
 const char* khs_kernel_name(int slot);
 /* Tile shapes in use: out = {x-sweep segment width (cells), x-sweep rows per
  * workgroup, z-sweep chunk height (rows), z-sweep next-row prefetch (0/1),
- * YM Maxwell kernel (0/2/3 = 64x16/64x4/64x8 LDS tiles, 1 = per cell, 4..9 = z march), x-sweep variables one at a
+ * YM Maxwell kernel (0/2/3 = 64x16/64x4/64x8 LDS tiles, 1 = per cell, 4..10 = z march), x-sweep variables one at a
  * time (0/1), fused-sweep variant (0: one wave per 64 columns; 1/2/3: four
  * waves per 128 columns with 16/32/64-row chunks; 4/5/6: register x sweep in
  * the z march, 32/64/128-row chunks; 7/8/9: the same with the variables on two
- *  waves, 16/32/64-row chunks, two column tiles per workgroup; 10 / 11: one tile per workgroup, 32 / 8-row chunks; EMHD: 0..3 = 16/32/64/8-row chunks, 4..6 = two waves per species, 8/16/32-row chunks), z-sweep variables split
+ *  waves, 16/32/64-row chunks, two column tiles per workgroup; 10 / 11 / 12: one tile per workgroup, 32 / 8 / 4-row chunks; EMHD: 0..3 = 16/32/64/8-row chunks, 4..6 = two waves per species, 8/16/32-row chunks), z-sweep variables split
  * over two waves (0/1), x-sweep cells per lane of the register form (0 = LDS
  * form; the register form reports its 62*C-cell wave segment, 1 row, and 1
  * for variables one at a time), fused z sweep + Maxwell kernel (0 = split

@@ -23,19 +23,27 @@ namespace khs {
 // sponge / xi_cut
 // --------------------------------------------------------------------------
 // One thread per column and CF_ROWS rows: the column's action is read once,
-// columns without one return at once (no per-element index arithmetic).
+// columns without one return at once (no per-element index arithmetic).  The
+// last nzero arrays of the list are set to zero in every column (the
+// colour-1 EM zeroing of suppress_kz0, which no other filter of the stage
+// touches, in the same launch).
 constexpr int CF_ROWS = 16;
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_cell_filters(ArrayList<T> arrs, const double* decay, const int* action,
-                                                      KParams p) {
+__global__ __launch_bounds__(256) void k_cell_filters(ArrayList<T> arrs, int nzero, const double* decay,
+                                                      const int* action, KParams p) {
     const int x = blockIdx.x * 256 + (int)threadIdx.x;
     if (x >= p.nx) return;
-    const int act = action[x];
-    if (act == 0) return;
     const int z0 = blockIdx.y * CF_ROWS, z1 = min(p.nz, z0 + CF_ROWS);
+    const int ncut = arrs.n - nzero;
+    for (int k = ncut; k < arrs.n; ++k) {
+        T* a = arrs.a[k] + x;
+        for (int z = z0; z < z1; ++z) a[(size_t)z * p.nx] = T(0.0);
+    }
+    const int act = action ? action[x] : 0;
+    if (act == 0) return;
     const T d = (act == 1) ? T(decay[x]) : T(0.0);
-    for (int k = 0; k < arrs.n; ++k) {
+    for (int k = 0; k < ncut; ++k) {
         T* a = arrs.a[k] + x;
         if (act == 1) {
             for (int z = z0; z < z1; ++z) a[(size_t)z * p.nx] *= d;
@@ -47,9 +55,9 @@ __global__ __launch_bounds__(256) void k_cell_filters(ArrayList<T> arrs, const d
 
 template <typename T>
 void launch_cell_filters(const ArrayList<T>& arrs, const double* decay, const int* action,
-                         const KParams& p, hipStream_t st) {
+                         const KParams& p, hipStream_t st, int nzero) {
     hipLaunchKernelGGL(k_cell_filters<T>, dim3((p.nx + 255) / 256, (p.nz + CF_ROWS - 1) / CF_ROWS), dim3(256), 0, st,
-                       arrs, decay, action, p);
+                       arrs, nzero, decay, action, p);
 }
 
 // --------------------------------------------------------------------------
@@ -346,19 +354,6 @@ void launch_hyperdiffuse(const ArrayList<T>& arrs, double mu, const KParams& p, 
     if (arrs.n == 0) return;
     dim3 g((p.nx + 63) / 64, arrs.n);
     hipLaunchKernelGGL(k_hyperdiff<T>, g, dim3(64), 0, st, arrs, mu, p);
-}
-
-template <typename T>
-__global__ void k_fill_zero(ArrayList<T> arrs, size_t n) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x)
-        for (int k = 0; k < arrs.n; ++k) arrs.a[k][i] = T(0.0);
-}
-
-template <typename T>
-void launch_fill_zero(const ArrayList<T>& arrs, size_t n, hipStream_t st) {
-    int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_fill_zero<T>, dim3(blocks), dim3(256), 0, st, arrs, n);
 }
 
 // kernel_enforce_boundaries (YM_Fluid.cu:13-30; EMHD main_coupled.cu:48-63)
@@ -751,12 +746,12 @@ void launch_spectrum_f64(const double* f, const double* ztab, const double* xtab
 
 // explicit instantiations
 #define KHS_INST(T)                                                                                   \
-    template void launch_cell_filters<T>(const ArrayList<T>&, const double*, const int*, const KParams&, hipStream_t); \
+    template void launch_cell_filters<T>(const ArrayList<T>&, const double*, const int*, const KParams&, hipStream_t, \
+                                         int);                                                                     \
     template void launch_zmean_cut<T>(const ArrayList<T>&, int, int, const double*, const int*, const KParams&, hipStream_t); \
     template void launch_column_modes<T, float>(const ArrayList<T>&, const float*, const float*, const float*, int, int, double*, const KParams&, hipStream_t); \
     template void launch_column_modes<T, double>(const ArrayList<T>&, const double*, const double*, const double*, int, int, double*, const KParams&, hipStream_t); \
     template void launch_hyperdiffuse<T>(const ArrayList<T>&, double, const KParams&, hipStream_t); \
-    template void launch_fill_zero<T>(const ArrayList<T>&, size_t, hipStream_t);                    \
     template void launch_enforce_boundaries<T>(const State<T>&, const KParams&, int, hipStream_t);  \
     template void launch_energy<T>(const State<T>&, const KParams&, double*, double*, hipStream_t);  \
     template void launch_gauss<T>(const State<T>&, const KParams&, double*, double*, hipStream_t);   \

@@ -515,8 +515,15 @@ struct Ctx {
             return;
         }
         timed(K_FILTERS, [&] {
-            // colour-2/3 fields were damped / cut in k_maxwell; the charges here
-            if (cell_filters) launch_cell_filters<T>(list<T>(k, kQ23, 4, false), d_decay, d_action, kp, stream);
+            // colour-2/3 fields were damped / cut in k_maxwell; the charge cut
+            // here, and with suppress_kz0 the colour-1 EM zeroing (no later
+            // filter touches colour 1 then), in one launch
+            if (cell_filters || c.suppress_kz0) {
+                static const int cut_c1[7] = {S_Q2A, S_Q3A, S_Q2B, S_Q3B, S_BY1, S_EX1, S_EZ1};
+                const int ncut = cell_filters ? 4 : 0, nzero = c.suppress_kz0 ? 3 : 0;
+                launch_cell_filters<T>(list<T>(k, cut_c1 + 4 - ncut, ncut + nzero, false), d_decay,
+                                       cell_filters ? d_action : nullptr, kp, stream, nzero);
+            }
             // With the MFMA keep projection (and no band filter between
             // them) the z-mean removal of the 15 field / charge arrays is
             // folded into it: those arrays come first in its list and are
@@ -543,10 +550,7 @@ struct Ctx {
                 }
             }
             if (c.hyp_diff > 0.0f) launch_hyperdiffuse<T>(list<T>(k, kCol23, 12, false), (double)c.hyp_diff, kp, stream);
-            if (c.suppress_kz0) {
-                const int c1[3] = {S_BY1, S_EX1, S_EZ1};
-                launch_fill_zero<T>(list<T>(k, c1, 3, false), N, stream);
-            }
+            // (colour-1 EM zeroing of suppress_kz0: with the charge cut above)
         });
         check_launch();
     }

@@ -17,7 +17,7 @@ constexpr int kNumXVariants = 20;
 constexpr int kNumZVariants = 15;          // 12..14: 16/32/64-row chunks, loads two rows ahead
 inline int z_prefetch_of(int v) { return v >= 12 ? 2 : (v / 3) % 2; }
 inline int z_groups_of(int v) { return v >= 12 ? 0 : v / 6; }
-constexpr int kNumMaxwellVariants = 10;
+constexpr int kNumMaxwellVariants = 11;
 // M: YMModel (ymgpu2d two-beam SU(2) fluid) or EMModel (emhdgpu2d Abelian fluid)
 template <typename T, typename M>
 void launch_fct_x(const State<T>& S, const Stage<T>& out, const KParams& p, int sp0, int nsp,
@@ -36,7 +36,7 @@ template <typename T>
 void launch_derived_em(const State<T>& S, int which, const KParams& p, double* out, hipStream_t st);
 // fused x + z sweep of both species: state S -> fluid of O (variants: see
 // launch_fct_xz in khs_step_kernels.hip)
-constexpr int kNumXZVariants = 12;
+constexpr int kNumXZVariants = 13;
 // EMHD fused x+z sweep of species 0..nsp-1: variants 0/1/2/3 (16/32/64/8-row chunks), 4/5/6
 // (two wavefronts per species, 8/16/32-row chunks)
 constexpr int kNumXZEmVariants = 7;
@@ -46,9 +46,9 @@ template <typename T>
 void launch_fct_xz(const State<T>& S, const State<T>& O, const KParams& p, int variant, hipStream_t st);
 // decay/action: per-column sponge / xi_cut (nullptr when neither is active);
 // variant 0/2/3: 64x16 / 64x4 / 64x8 tiles sharing the new E through LDS,
-// 1: one thread per cell recomputing its neighbours' E, 4..9: z march, one
-// lane per column, chunks of 16/32/16/8/8/4 rows with / with / without / with
-// / without / with next-row prefetch; all bit-identical
+// 1: one thread per cell recomputing its neighbours' E, 4..10: z march, one
+// lane per column, chunks of 16/32/16/8/8/4/2 rows with / with / without /
+// with / without / with / with next-row prefetch; all bit-identical
 // fused z sweep (both species) + Maxwell of the YM model: stage X and state S
 // -> fluid and fields of O; variant 1 / 2 / 3 = 16 / 32 / 64-row chunks
 constexpr int kNumZMVariants = 4;          // 0 = split z sweep and Maxwell kernels
@@ -77,10 +77,11 @@ struct ArrayList {
     int n;
 };
 // Per-column action of the sponge / xi_cut kernels: 0 none, 1 multiply by
-// decay[x], 2 zero.
+// decay[x], 2 zero (action may be null: no action); the last nzero arrays of
+// the list are set to zero everywhere.
 template <typename T>
 void launch_cell_filters(const ArrayList<T>& arrs, const double* decay, const int* action,
-                         const KParams& p, hipStream_t st);
+                         const KParams& p, hipStream_t st, int nzero = 0);
 // z-mean removal of arrays [0, nmean) (the last nact after the per-column
 // action of launch_cell_filters, when action is given) and zeroing of arrays
 // [nmean, n), one workgroup per 64-column strip and array.
@@ -118,8 +119,6 @@ void launch_spectrum_f64(const double* f, const double* ztab, const double* xtab
                          int nz, int rows, hipStream_t st);
 template <typename T>
 void launch_hyperdiffuse(const ArrayList<T>& arrs, double mu, const KParams& p, hipStream_t st);
-template <typename T>
-void launch_fill_zero(const ArrayList<T>& arrs, size_t n, hipStream_t st);
 template <typename T>
 void launch_enforce_boundaries(const State<T>& S, const KParams& p, int emhd, hipStream_t st);
 

@@ -2743,8 +2743,8 @@ static void launch_fct_xz2_t(const State<T>& S, const State<T>& O, const KParams
 // 128 columns (variable groups split across waves), 16/32/64-row chunks;
 // 4/5/6: the x sweep in registers (DPP) inside the z march, 32/64/128-row
 // chunks; 7/8/9: the same with the variables on two waves, 16/32/64-row
-// chunks (two column tiles per workgroup), 10 / 11: 32 / 8-row chunks, one
-// tile per workgroup (short chunks: lower latency on small grids).
+// chunks (two column tiles per workgroup), 10 / 11 / 12: 32 / 8 / 4-row chunks,
+// one tile per workgroup (short chunks: lower latency on small grids).
 template <typename T>
 void launch_fct_xz(const State<T>& S, const State<T>& O, const KParams& p, int variant, hipStream_t st) {
     switch (variant) {
@@ -2759,6 +2759,7 @@ void launch_fct_xz(const State<T>& S, const State<T>& O, const KParams& p, int v
         case 9: launch_fct_xzg_t<T, 64>(S, O, p, st); return;
         case 10: launch_fct_xzg_t<T, 32, 128>(S, O, p, st); return;
         case 11: launch_fct_xzg_t<T, 8, 128>(S, O, p, st); return;
+        case 12: launch_fct_xzg_t<T, 4, 128>(S, O, p, st); return;
         default: break;
     }
     constexpr int TZ = 32;
@@ -2804,6 +2805,7 @@ void launch_maxwell(const State<T>& S, const State<T>& O, const KParams& p, cons
         case 7: launch_maxwell_zm_t<T, 8, 1>(S, O, p, decay, action, st); return;
         case 8: launch_maxwell_zm_t<T, 8, 0>(S, O, p, decay, action, st); return;
         case 9: launch_maxwell_zm_t<T, 4, 1>(S, O, p, decay, action, st); return;
+        case 10: launch_maxwell_zm_t<T, 2, 1>(S, O, p, decay, action, st); return;
         default: break;
     }
     if (variant == 1) {

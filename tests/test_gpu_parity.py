@@ -368,7 +368,7 @@ def test_fused_z_maxwell_bitwise(zm, shape, periodic, freeze, warm):
         assert_bitwise(sim.get(n), o.get(n), f"zm {zm} {shape} {n}")
 
 
-@pytest.mark.parametrize("mk", [4, 5, 6, 7, 8, 9])
+@pytest.mark.parametrize("mk", [4, 5, 6, 7, 8, 9, 10])
 @pytest.mark.parametrize("shape,periodic,freeze", [((300, 136), 1, 1), ((70, 36), 0, 1), ((256, 64), 0, 0),
                                                    ((130, 100), 1, 0)])
 def test_maxwell_march_bitwise(mk, shape, periodic, freeze):
@@ -387,7 +387,7 @@ def test_maxwell_march_bitwise(mk, shape, periodic, freeze):
         assert_bitwise(sim.get(n), o.get(n), f"maxwell {mk} {shape} {n}")
 
 
-@pytest.mark.parametrize("xzv", list(range(12)))
+@pytest.mark.parametrize("xzv", list(range(13)))
 @pytest.mark.parametrize("shape,periodic,warm", [((300, 136), 1, 0.01), ((70, 36), 0, 0.0), ((256, 64), 0, 0.0),
                                                  ((1000, 40), 0, 0.01), ((116, 300), 1, 0.0)])
 def test_fused_sweep_variants_bitwise(xzv, shape, periodic, warm):
