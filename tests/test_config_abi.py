@@ -88,7 +88,10 @@ def test_output_dir_name_c1():
     (dict(x_lane_cells=6), "x_lane_cells"),
     (dict(z_groups=2), "z_groups"),
     (dict(fused_sweep=2), "fused_sweep"),
-    (dict(xz_variant=12), "xz_variant"),
+    (dict(xz_variant=13), "xz_variant"),
+    (dict(step_fusion=2), "step_fusion"),
+    (dict(mxz_variant=6), "mxz_variant"),
+    (dict(maxwell_kernel=11), "maxwell_kernel"),
     (dict(nx_override=32768, nz_override=16384), "grid too large"),
     (dict(model="emhd", nx_override=32768, nz_override=16384), "grid too large"),
     (dict(nx_override=64, nz_override=1 << 23, run_mode=6), "grid too large"),   # 2^29 cells: 2^32 bytes
