@@ -84,8 +84,9 @@ C3_DIAG_STEPS = 3500
 #   k_fct_z : R stage 12 + step-start fluid 12 + Ez 3 + By 3 + Az 3, W fluid 12
 #   k_maxwell: R new fluid 12 + fields 12,         W E 6 + By 3 + Az2,3 2 (Az1 frozen)
 #   k_fct_xz: R fluid 12 + Ex, Ez, By, Az 12,       W fluid 12
+#   k_mxz   : R fluid 12 + fields 12,               W fields 11 + fluid 12 (Maxwell + the next sweeps)
 KERNEL_ARRAYS = {"k_fct_x": 18 + 12, "k_fct_z": 33 + 12, "k_fct_xz": 24 + 12, "k_maxwell": 24 + 11,
-                 "k_fct_zm": 36 + 23}
+                 "k_fct_zm": 36 + 23, "k_mxz": 24 + 23}
 # The whole step's algorithmic minimum: R 24 + W 23 arrays (frozen Az1 read only).
 STEP_ARRAYS = 47
 ZM_ROWS = {1: 16, 2: 32, 3: 64}       # chunk height of the fused z sweep + Maxwell variants
@@ -120,15 +121,18 @@ def kernel_instance(slot, t, T="double"):
         name, tz = {0: ("k_fct_xz", 32), 1: ("k_fct_xz2", 16), 2: ("k_fct_xz2", 32), 3: ("k_fct_xz2", 64),
                     4: ("k_fct_xzr", 32), 5: ("k_fct_xzr", 64), 6: ("k_fct_xzr", 128), 7: ("k_fct_xzg", 16),
                     8: ("k_fct_xzg", 32), 9: ("k_fct_xzg", 64), 10: ("k_fct_xzg", 32),
-                    11: ("k_fct_xzg", 8)}.get(v, (None, 0))
+                    11: ("k_fct_xzg", 8), 12: ("k_fct_xzg", 4)}.get(v, (None, 0))
         if name is None:
             return None
         if name == "k_fct_xzg":
             return f"{name}<{T}, {tz}, {128 if v >= 10 else 256}>"
         return f"{name}<{T}, {tz}>"
     if slot == "k_maxwell" and t["maxwell"] >= 4:
-        tzpf = {4: (16, 1), 5: (32, 1), 6: (16, 0), 7: (8, 1), 8: (8, 0), 9: (4, 1)}.get(t["maxwell"])
+        tzpf = {4: (16, 1), 5: (32, 1), 6: (16, 0), 7: (8, 1), 8: (8, 0), 9: (4, 1), 10: (2, 1)}.get(t["maxwell"])
         return f"k_maxwell_zm<{T}, {tzpf[0]}, {tzpf[1]}>" if tzpf else None
+    if slot == "k_mxz":
+        v = t.get("mxz_variant", 1)
+        return f"k_mxz<{T}, {(16, 32, 64)[v % 3]}, {v // 3}>"
     if slot == "k_maxwell":
         return {0: f"k_maxwell<{T}, 16>", 1: f"k_maxwell_cell<{T}>", 2: f"k_maxwell<{T}, 4>",
                 3: f"k_maxwell<{T}, 8>"}.get(t["maxwell"])
