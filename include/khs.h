@@ -146,6 +146,8 @@ typedef struct khs_config {
                                   1 = on */
     int      mxz_variant;      /* that launch's shape: -1 = autotuned (default), 0..5 (16/32/64-row chunks, Maxwell
                                   loading one row ahead; 3..5 two rows ahead) */
+    int      packed_shape;     /* khs_batch_run_packed launch shape (read from the batch's first run): -1 = timed on
+                                  the first capture (default), 0..7 (see khs_batch_run_packed) */
     /* --- derived (khs_derive) --- */
     int    nx, nz;
     int    periodic_x;         /* run_mode >= 1 */
@@ -361,7 +363,11 @@ int khs_batch_run(khs_batch* b, int nsteps, int* steps_done, int* halted);
  * runs' energy reductions captured as one HIP graph.  Small grids then share
  * the device instead of waiting on launch latency.  All runs must use the
  * YM model and one precision, and filter only with xi_cut / sponge /
- * suppress_kz0 (KHS_ERR_CONFIG otherwise). */
+ * suppress_kz0 (KHS_ERR_CONFIG otherwise).  Launch shapes (chunk rows of the
+ * sweep, of the Maxwell march; two sweep tiles per workgroup unless noted):
+ * 0: 16, 8; 1: 32, 4; 2: 32, 8; 3: 16, 4; 4: 32, 2; 5: 16, 2; 6: 32, 4 and
+ * 7: 8, 4 with one sweep tile per workgroup; timed on the first capture
+ * unless the first run's packed_shape forces one. */
 int khs_batch_run_packed(khs_batch* b, int nsteps, int* steps_done, int* halted);
 /* The same stepping without graphs: each run in turn through khs_step
  * (halting every energy_stride steps), for comparison. */

@@ -84,6 +84,7 @@ void config_defaults(khs_config& c) {
     c.step_graphs = 1;
     c.step_fusion = -1;
     c.mxz_variant = -1;
+    c.packed_shape = -1;
 }
 
 static int parse_int(const std::string& k, const std::string& v) {
@@ -178,6 +179,7 @@ bool config_set(khs_config& c, const std::string& key, const std::string& val) {
     else if (key == "step_graphs")       c.step_graphs = parse_int(key, val);
     else if (key == "step_fusion")       c.step_fusion = parse_int(key, val);
     else if (key == "mxz_variant")       c.mxz_variant = parse_int(key, val);
+    else if (key == "packed_shape")      c.packed_shape = parse_int(key, val);
     else if (key == "smooth_init")       c.smooth_init = parse_int(key, val);
     else if (key == "spectrum_kmax")     c.spectrum_kmax = parse_int(key, val);
     else if (key == "spectrum_amp")      c.spectrum_amp = parse_double(key, val);
@@ -339,6 +341,7 @@ void config_derive(khs_config& c) {
     if (c.zm_fused < -1 || c.zm_fused > 3) throw Error(KHS_ERR_CONFIG, "zm_fused must be -1..3");
     if (c.step_fusion < -1 || c.step_fusion > 1) throw Error(KHS_ERR_CONFIG, "step_fusion must be -1, 0 or 1");
     if (c.mxz_variant < -1 || c.mxz_variant > 5) throw Error(KHS_ERR_CONFIG, "mxz_variant must be -1..5");
+    if (c.packed_shape < -1 || c.packed_shape > 7) throw Error(KHS_ERR_CONFIG, "packed_shape must be -1..7");
     if (c.run_mode < 0 || c.run_mode > 6)
         throw Error(KHS_ERR_CONFIG, "run_mode must be 0..6");
 }

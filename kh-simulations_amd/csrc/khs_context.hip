@@ -1085,6 +1085,7 @@ struct Batch {
             launch_packed_step<T>(dev + (size_t)(st & 1) * n, ib, ib + (n + 1), ib + 2 * (n + 1), n, ix[n],
                                   ix[(n + 1) + n], ix[2 * (n + 1) + n], shape, origin);
         };
+        if (pk_shape < 0 && runs[set[0]]->cfg.packed_shape >= 0) pk_shape = runs[set[0]]->cfg.packed_shape;
         if (pk_shape < 0) {
             // time every launch shape on this batch (a step from the current
             // state into the next-state buffers, which the first real step

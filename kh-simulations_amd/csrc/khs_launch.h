@@ -163,11 +163,19 @@ struct PackedRun {
     int nmean, nact;               // fmode 2: see launch_zmean_cut
     int nstrip;                    // filter work items per array: 64-column strips
 };
-// Packed launch shapes (chunk rows of the fused sweep, of the Maxwell march):
-// 0: 16, 8; 1: 32, 4; 2: 32, 8; 3: 16, 4.
-constexpr int kPackedShapes = 4;
-inline int packed_xz_chunk(int shape) { return (shape == 1 || shape == 2) ? 32 : 16; }
-inline int packed_mx_chunk(int shape) { return (shape == 1 || shape == 3) ? 4 : 8; }
+// Packed launch shapes (chunk rows of the fused sweep, of the Maxwell march;
+// two sweep tiles per workgroup unless noted): 0: 16, 8; 1: 32, 4; 2: 32, 8;
+// 3: 16, 4; 4: 32, 2; 5: 16, 2; 6: 32, 4 and 7: 8, 4 with one sweep tile
+// per workgroup.
+constexpr int kPackedShapes = 8;
+inline int packed_xz_chunk(int shape) {
+    static const int rows[kPackedShapes] = {16, 32, 32, 16, 32, 16, 32, 8};
+    return rows[shape];
+}
+inline int packed_mx_chunk(int shape) {
+    static const int rows[kPackedShapes] = {8, 4, 8, 4, 2, 2, 4, 4};
+    return rows[shape];
+}
 // begin: device arrays of n + 1 tile offsets per launch kind (fused-sweep
 // tiles, Maxwell waves, filter work items) for the shape
 template <typename T>

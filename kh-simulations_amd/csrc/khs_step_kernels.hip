@@ -3326,10 +3326,10 @@ __global__ __launch_bounds__(256) void k_maxwell_zm_packed(const PackedRun<T>* _
     maxwell_zm_wave<T, TZ, PF>(d.S, d.O, d.p, d.nwx, t - begin[r], lane, d.decay, d.action);
 }
 
-template <typename T, int XZ, int MX>
+template <typename T, int XZ, int MX, int BS = 256>
 static void launch_packed_step_t(const PackedRun<T>* runs, const int* xz_begin, const int* mx_begin,
                                  const int* f_begin, int n, int xz_total, int mx_total, int f_total, hipStream_t st) {
-    constexpr int BS = 256, TPB = BS / 128;
+    constexpr int TPB = BS / 128;
     hipLaunchKernelGGL((k_fct_xzg_packed<T, XZ, BS>), dim3(round8((xz_total + TPB - 1) / TPB)), dim3(BS), 0, st,
                        runs, xz_begin, n, xz_total);
     hipLaunchKernelGGL((k_maxwell_zm_packed<T, MX, 1>), dim3(round8((mx_total + 3) / 4)), dim3(256), 0, st, runs,
@@ -3344,6 +3344,10 @@ void launch_packed_step(const PackedRun<T>* runs, const int* xz_begin, const int
         case 1: launch_packed_step_t<T, 32, 4>(runs, xz_begin, mx_begin, f_begin, n, xz_total, mx_total, f_total, st); return;
         case 2: launch_packed_step_t<T, 32, 8>(runs, xz_begin, mx_begin, f_begin, n, xz_total, mx_total, f_total, st); return;
         case 3: launch_packed_step_t<T, 16, 4>(runs, xz_begin, mx_begin, f_begin, n, xz_total, mx_total, f_total, st); return;
+        case 4: launch_packed_step_t<T, 32, 2>(runs, xz_begin, mx_begin, f_begin, n, xz_total, mx_total, f_total, st); return;
+        case 5: launch_packed_step_t<T, 16, 2>(runs, xz_begin, mx_begin, f_begin, n, xz_total, mx_total, f_total, st); return;
+        case 6: launch_packed_step_t<T, 32, 4, 128>(runs, xz_begin, mx_begin, f_begin, n, xz_total, mx_total, f_total, st); return;
+        case 7: launch_packed_step_t<T, 8, 4, 128>(runs, xz_begin, mx_begin, f_begin, n, xz_total, mx_total, f_total, st); return;
         default: launch_packed_step_t<T, 16, 8>(runs, xz_begin, mx_begin, f_begin, n, xz_total, mx_total, f_total, st); return;
     }
 }

@@ -76,7 +76,7 @@ class khs_config(ctypes.Structure):
         ("smooth_init", ctypes.c_int), ("spectrum_kmax", ctypes.c_int), ("spectrum_amp", ctypes.c_double),
         ("spectrum_seed", ctypes.c_uint64), ("em_perturb_amp", ctypes.c_double),
         ("z_groups", ctypes.c_int), ("xz_variant", ctypes.c_int), ("x_serial", ctypes.c_int), ("maxwell_kernel", ctypes.c_int), ("x_rows", ctypes.c_int), ("z_prefetch", ctypes.c_int), ("mfma_filter", ctypes.c_int), ("x_lane_cells", ctypes.c_int), ("snake_order", ctypes.c_int), ("zm_fused", ctypes.c_int), ("step_graphs", ctypes.c_int),
-        ("step_fusion", ctypes.c_int), ("mxz_variant", ctypes.c_int),
+        ("step_fusion", ctypes.c_int), ("mxz_variant", ctypes.c_int), ("packed_shape", ctypes.c_int),
         ("nx", ctypes.c_int), ("nz", ctypes.c_int), ("periodic_x", ctypes.c_int), ("freeze_az1", ctypes.c_int),
         ("lx", ctypes.c_double), ("lz", ctypes.c_double), ("dx", ctypes.c_double), ("dz", ctypes.c_double),
         ("courant", ctypes.c_double), ("dt", ctypes.c_double),

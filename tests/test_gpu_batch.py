@@ -185,6 +185,26 @@ def test_batch_packed_launches_match_single_runs(precision):
     batch.close()
 
 
+@pytest.mark.parametrize("shape", list(range(8)))
+def test_batch_packed_every_launch_shape(shape):
+    """Every packed launch shape (sweep chunks of 8 to 32 rows, one or two
+    sweep tiles per workgroup, Maxwell chunks of 2 to 8 rows) gives the bits
+    of each run stepped alone."""
+    cfgs = [dict(c, precision="fp32" if shape % 2 else "fp64", packed_shape=shape) for c in packed_configs()[:3]]
+    batch = khs.Batch(cfgs, lanes=2, segment=4)
+    batch.init()
+    done, halted = batch.run_packed(8)
+    assert done == [8] * len(cfgs) and not any(halted)
+    for i, c in enumerate(cfgs):
+        s = khs.Simulation(khs.Config(**c))
+        s.init()
+        s.step(8)
+        for n in ("PzA", "Q2B", "By2", "Ez3", "Az2", "nB", "PxA", "Ex1"):
+            assert_bitwise(batch.get(i, n), s.get(n), f"shape {shape} run {i} {n}")
+        s.close()
+    batch.close()
+
+
 def test_batch_packed_rejects_unsupported_filters():
     cfgs = [dict(c, precision="fp32") for c in packed_configs()[:2]]
     cfgs[1]["kz_keep_max"] = 4
