@@ -2294,9 +2294,11 @@ __device__ __forceinline__ void xz_march_g(const State<T>& S, const State<T>& O,
             if (GR::EZ) row.ez[c] = ld32(S.a[S_EZ1 + c], i);
             if (GR::BY) {
                 row.by[c] = ld32(S.a[S_BY1 + c], i);
-                // interior wavefronts: By(x-1) is lane l-1's By (lane 0's
-                // value is unused: it is a halo lane)
-                row.byxm[c] = EDGE ? ld32(S.a[S_BY1 + c], ixm) : lane_from_left(row.by[c]);
+                // interior wavefronts take By(x-1) from lane l-1 when the
+                // row is used (shift_in), not here: a lane shift of a value
+                // still being loaded makes the wave wait for that load in
+                // the middle of the previous row's arithmetic
+                if (EDGE) row.byxm[c] = ld32(S.a[S_BY1 + c], ixm);
             }
             if (GR::AZ) row.az[c] = ld32(S.a[S_AZ1 + c], i);
         }
@@ -2308,6 +2310,15 @@ __device__ __forceinline__ void xz_march_g(const State<T>& S, const State<T>& O,
         }
     };
 
+    // interior wavefronts: By(x-1) of a row from lane l-1 (lane 0's value is
+    // unused: it is a halo lane)
+    auto shift_in = [&](Row& row) {
+        if (!EDGE && GR::BY) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) row.byxm[c] = lane_from_left(row.by[c]);
+        }
+    };
+
     int rc = clampi(z0 - 3, 0, nz - 1);
     Row prev, cur;
     T F[NV], EX[3];
@@ -2316,6 +2327,7 @@ __device__ __forceinline__ void xz_march_g(const State<T>& S, const State<T>& O,
         load(zm_per(rc, nz), prev, Fd, Ed);
     }
     load(rc, cur, F, EX);
+    shift_in(prev);
     ZPipe<T, NG> zp;
     zp.Um = T(0);
     zp.MUm = T(0);
@@ -2331,6 +2343,7 @@ __device__ __forceinline__ void xz_march_g(const State<T>& S, const State<T>& O,
         Row nrow;
         T nF[NV], nEX[3];
         load(rn, nrow, nF, nEX);
+        shift_in(cur);
 
         // ---- x sweep of row rc at this lane's column (YM_FCT.cu:30-91) ----
         T src = T(0.0);
@@ -2540,13 +2553,21 @@ __device__ __forceinline__ void xz_march_em(const State<T>& S, const State<T>& O
             row.exm = ld32(S.a[E_EX], ixm);
             row.bzxm = ld32(S.a[E_BZ], ixm);
             row.byxm = ld32(S.a[E_BY], ixm);
-        } else {                               // lane 0 is a halo lane: its value is unused
+        }                                      // interior: shift_in when the row is used
+#pragma unroll
+        for (int v = 0; v < NV; ++v) F[v] = ld32(S.a[b + v], i);
+    };
+
+    // interior wavefronts: the x-1 fields of a row from lane l-1, taken when
+    // the row is used (a lane shift of a value still being loaded would make
+    // the wave wait for it in the middle of the previous row; lane 0 is a
+    // halo lane, its value unused)
+    auto shift_in = [&](Row& row) {
+        if (!EDGE) {
             row.exm = lane_from_left(row.ex);
             row.bzxm = lane_from_left(row.bz);
             row.byxm = lane_from_left(row.by);
         }
-#pragma unroll
-        for (int v = 0; v < NV; ++v) F[v] = ld32(S.a[b + v], i);
     };
 
     int rc = clampi(z0 - 3, 0, nz - 1);
@@ -2557,6 +2578,7 @@ __device__ __forceinline__ void xz_march_em(const State<T>& S, const State<T>& O
         load(zm_per(rc, nz), prev, Fd);
     }
     load(rc, cur, F);
+    shift_in(prev);
     ZPipe<T, NG> zp;
     zp.Um = T(0);
     zp.MUm = T(0);
@@ -2567,6 +2589,7 @@ __device__ __forceinline__ void xz_march_em(const State<T>& S, const State<T>& O
         Row nrow;
         T nF[NV];
         load(rn, nrow, nF);
+        shift_in(cur);
 
         // Lorentz force of the cell (both sweeps' source on px, py, pz)
         T f[3];
