@@ -2858,35 +2858,66 @@ void launch_maxwell(const State<T>& S, const State<T>& O, const KParams& p, cons
 //     operation) over the rows the tile's sweep reads, on the fields of step
 //     n and the fluid after the sweeps of step n; it applies the column
 //     action to the colour-2/3 fields, stores the tile's own cells and puts
-//     each new field row into an LDS ring of three rows;
+//     each new field row into an LDS ring of MXZ_SLOTS rows;
 //   * waves 0..3 march the fused x + z sweep of xz_march_g (species w / 2,
 //     variable group w % 2) on that fluid, with the column action applied to
 //     the colour-2/3 charges as they are loaded (the filter stage acts on the
 //     charges after Maxwell, whose currents use them unfiltered), and take
 //     the new field rows of their sources from the ring.
-// The workgroup meets once per row at a barrier that waits for LDS traffic
-// only (the fluid loads of the next row stay in flight).  Per cell-update
+// The waves are not in lockstep: Maxwell publishes how many ring rows it has
+// written, each sweep wave how many it no longer reads, and each waits (in
+// LDS, with a bounded spin) only for what it needs, so Maxwell runs up to
+// MXZ_SLOTS - 2 rows ahead and a wave's memory stalls overlap the others'
+// arithmetic instead of holding the workgroup.  Per cell-update
 // the launch reads the fluid and the fields once and writes them once: 47
 // arrays, against 36 + 35 for the sweep and Maxwell kernels.
 // I: fields of step n, fluid after the sweeps of step n; O: fields of step
 // n + 1 (before the charge filter), fluid after the sweeps of step n + 1.
 // ==========================================================================
-constexpr int MXZ_SLOTS = 3;
+constexpr int MXZ_SLOTS = 6;
 
-// Workgroup barrier ordering LDS accesses only: no wait for outstanding
-// global loads or stores.
-__device__ __forceinline__ void wg_sync_lds() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
-// The ring: slot k % 3 holds the k-th new field row of the tile (state ids
-// S_EX1 .. S_AZ3, one value per lane).  Rows are produced in the order the
-// sweep reads them: zm_per(rc0) first (k = 0), then rc0, rc0 + 1, ... (k = 1
-// + row - rc0), rc0 = the chunk's first (clamped) halo row.
+// The ring: slot k % MXZ_SLOTS holds the k-th new field row of the tile
+// (state ids S_EX1 .. S_AZ3, one value per lane).  Rows are produced in the
+// order the sweep reads them: zm_per(rc0) first (k = 0), then rc0, rc0 + 1,
+// ... (k = 1 + row - rc0), rc0 = the chunk's first (clamped) halo row.
+// produced: rows written so far; released[w]: sweep wave w reads no row
+// below it any more.
+template <typename T>
+struct MxzShared {
+    T ring[MXZ_SLOTS][12][64];
+    int produced;
+    int released[4];
+};
 template <typename T>
 using MxzRing = T[MXZ_SLOTS][12][64];
+
+// Progress counters in LDS: a relaxed store after a fence that orders this
+// wave's LDS accesses before it (writes of the rows it announces, reads of
+// the rows it gives up), and a bounded wait followed by the matching fence.
+// The fences order LDS only: global loads and stores stay in flight.  The
+// bound (2^16 sleeps of 64 cycles, about 2 ms, a thousand times a row's
+// work) only guards against a wait that cannot end; with the counters'
+// protocol every wait ends.
+constexpr int kMxzSpinLimit = 1 << 16;
+__device__ __forceinline__ void mxz_publish(int* c, int v) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __hip_atomic_store(c, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ int mxz_read(const int* c) {
+    return __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void mxz_wait_produced(const int* c, int v) {
+    for (int i = 0; i < kMxzSpinLimit && mxz_read(c) < v; ++i) __builtin_amdgcn_s_sleep(1);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+__device__ __forceinline__ void mxz_wait_released(const int (&r)[4], int v) {
+    for (int i = 0; i < kMxzSpinLimit; ++i) {
+        const int m = min(min(mxz_read(&r[0]), mxz_read(&r[1])), min(mxz_read(&r[2]), mxz_read(&r[3])));
+        if (m >= v) break;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 
 // Wave 4: the Maxwell march.  mstep(fin, k) consumes the inputs of row q
 // (cur), computes Ampere's Ex there and, with fin, finishes row q - 1 (Jz,
@@ -2894,7 +2925,8 @@ using MxzRing = T[MXZ_SLOTS][12][64];
 // q + 1.  Row q - 1 of the march is thus the k-th produced row.
 template <typename T, int TZ, int PF>
 __device__ __forceinline__ void mxz_maxwell(const State<T>& I, const State<T>& O, const KParams& p, int g, int ch,
-                                            int lane, MxzRing<T>& ring, const double* decay, const int* action) {
+                                            int lane, MxzShared<T>& sh, const double* decay, const int* action) {
+    MxzRing<T>& ring = sh.ring;
     const int nx = p.nx, nz = p.nz;
     const bool per = p.periodic_x != 0;
     const int x = g * XZR_OUT - 3 + lane;
@@ -3052,20 +3084,16 @@ __device__ __forceinline__ void mxz_maxwell(const State<T>& I, const State<T>& O
         }
     };
 
-    // rows k = 0 (zm_per(rc0)) and 1 (rc0) before the sweep's first row
+    // rows k = 0 (zm_per(rc0)), 1 (rc0), ..., 1 + rlast - rc0 (rlast: the
+    // sweep's last row, clamp(elast)); row k overwrites row k - MXZ_SLOTS,
+    // so it waits until every sweep wave has released that one
+    const int nrows = 2 + clampi(elast, 0, nz - 1) - rc0;
     mstep(false, 0);
-    mstep(true, 0);
-    mstep(true, 1);
-    wg_sync_lds();
-    // then the row the sweep reads next, whenever its row advances (the same
-    // row sequence as xz_march_g: rc(e) = clamp(min(e, elast), 0, nz - 1))
-    int rc = rc0;
 #pragma unroll 1
-    for (int e = z0 - 3; e <= elast; ++e) {
-        const int rn = clampi(min(e + 1, elast), 0, nz - 1);
-        if (rn != rc) mstep(true, 1 + rn - rc0);
-        rc = rn;
-        wg_sync_lds();
+    for (int k = 0; k < nrows; ++k) {
+        if (k >= MXZ_SLOTS) mxz_wait_released(sh.released, k - MXZ_SLOTS + 1);
+        mstep(true, k);
+        mxz_publish(&sh.produced, k + 1);
     }
 }
 
@@ -3077,8 +3105,9 @@ __device__ __forceinline__ void mxz_maxwell(const State<T>& I, const State<T>& O
 // clamped halo loads give).
 template <typename T, int TZ, typename GR, bool EDGE>
 __device__ __forceinline__ void mxz_sweep(const State<T>& I, const State<T>& O, const KParams& p, int sp, int g,
-                                          int ch, int lane, const MxzRing<T>& ring, const double* decay,
+                                          int ch, int lane, MxzShared<T>& sh, int w, const double* decay,
                                           const int* action) {
+    const MxzRing<T>& ring = sh.ring;
     typedef YMModel M;
     constexpr int NV = F_COUNT, NG = GR::NGV, VB = GR::VB;
     const int nx = p.nx, nz = p.nz;
@@ -3119,20 +3148,24 @@ __device__ __forceinline__ void mxz_sweep(const State<T>& I, const State<T>& O, 
         }
     };
 
+    // the fluid is loaded two rows ahead: F (row rc(e)), nF (rc(e + 1)), nnF
+    // (rc(e + 2)), rc(e) = clamp(min(e, elast), 0, nz - 1)
+    auto rowof = [&](int e) { return clampi(min(e, elast), 0, nz - 1); };
     int rc = rc0;
-    T F[NV];
+    T F[NV], nF[NV];
     load(rc, F);
+    load(rowof(z0 - 2), nF);
     ZPipe<T, NG> zp;
     zp.Um = T(0);
     zp.MUm = T(0);
-    wg_sync_lds();                              // ring rows zm_per(rc0), rc0 are in place
 
     auto row = [&](int e, auto mode) {
         constexpr int MODE = decltype(mode)::value;
-        const int rn = MODE ? rc + 1 : clampi(min(e + 1, elast), 0, nz - 1);
-        T nF[NV];
-        load(rn, nF);
+        const int rn = MODE ? rc + 1 : rowof(e + 1);
+        T nnF[NV];
+        load(rowof(e + 2), nnF);
         const int kc = 1 + rc - rc0;            // ring index of row rc; row rc's predecessor is kc - 1
+        mxz_wait_produced(&sh.produced, kc + 1);
         const T(&cs)[12][64] = ring[kc % MXZ_SLOTS];
         const T(&ps)[12][64] = ring[(kc + MXZ_SLOTS - 1) % MXZ_SLOTS];
 
@@ -3227,9 +3260,13 @@ __device__ __forceinline__ void mxz_sweep(const State<T>& I, const State<T>& O, 
             for (int j = 0; j < NG; ++j) buf_st_nt(out[j], orsc[j], off);
         }
 #pragma unroll
-        for (int v = 0; v < NV; ++v) F[v] = nF[v];
+        for (int v = 0; v < NV; ++v) {
+            F[v] = nF[v];
+            nF[v] = nnF[v];
+        }
+        // the next row reads ring rows kn - 1 and kn: the ones below are free
+        mxz_publish(&sh.released[w], kc + (rn != rc) - 1);
         rc = rn;
-        wg_sync_lds();                          // ring: row rn + 1 in, row rc - 1 out
     };
 
     int e = z0 - 3;
@@ -3245,28 +3282,35 @@ __device__ __forceinline__ void mxz_sweep(const State<T>& I, const State<T>& O, 
 }
 
 // Workgroup: waves 0..3 sweep (species w / 2, group w % 2), wave 4 Maxwell.
+// Occupancy: two waves per SIMD in FP64 (175 / 235 VGPRs; three would spill
+// 91), four in FP32.
 template <typename T, int TZ, int PF>
-__global__ __launch_bounds__(320) void k_mxz(State<T> I, State<T> O, KParams p, int ngx, int nch,
-                                             const double* decay, const int* action) {
-    __shared__ MxzRing<T> ring;
+__global__ __launch_bounds__(320) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2))) void k_mxz(
+    State<T> I, State<T> O, KParams p, int ngx, int nch, const double* decay, const int* action) {
+    __shared__ MxzShared<T> sh;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     int lb = xcd_logical(blockIdx.x, gridDim.x);
     if (lb >= ngx * nch) return;                // whole workgroups leave together
     lb = __builtin_amdgcn_readfirstlane(tile_order(lb, ngx * nch, p.rev));
     const int g = lb % ngx, ch = lb / ngx;
+    if (threadIdx.x == 0) {
+        sh.produced = 0;
+        for (int j = 0; j < 4; ++j) sh.released[j] = 0;
+    }
+    __syncthreads();                            // the only barrier: the counters are set
     if (w == 4) {
-        mxz_maxwell<T, TZ, PF>(I, O, p, g, ch, lane, ring, decay, action);
+        mxz_maxwell<T, TZ, PF>(I, O, p, g, ch, lane, sh, decay, action);
         return;
     }
     const int sp = w >> 1;
     const bool edge = g == 0 || g * XZR_OUT + 60 >= p.nx;   // lanes cover columns 58g-3 .. 58g+60
     if (w & 1) {
-        if (edge) mxz_sweep<T, TZ, XzQ3, true>(I, O, p, sp, g, ch, lane, ring, decay, action);
-        else mxz_sweep<T, TZ, XzQ3, false>(I, O, p, sp, g, ch, lane, ring, decay, action);
+        if (edge) mxz_sweep<T, TZ, XzQ3, true>(I, O, p, sp, g, ch, lane, sh, w, decay, action);
+        else mxz_sweep<T, TZ, XzQ3, false>(I, O, p, sp, g, ch, lane, sh, w, decay, action);
     } else {
-        if (edge) mxz_sweep<T, TZ, XzN3, true>(I, O, p, sp, g, ch, lane, ring, decay, action);
-        else mxz_sweep<T, TZ, XzN3, false>(I, O, p, sp, g, ch, lane, ring, decay, action);
+        if (edge) mxz_sweep<T, TZ, XzN3, true>(I, O, p, sp, g, ch, lane, sh, w, decay, action);
+        else mxz_sweep<T, TZ, XzN3, false>(I, O, p, sp, g, ch, lane, sh, w, decay, action);
     }
 }
 
