@@ -96,12 +96,21 @@ ZM_ROWS = {1: 16, 2: 32, 3: 64}       # chunk height of the fused z sweep + Maxw
 # precision, nx, nz).
 LIMITER = {
     ("k_fct_xz", "fp64", 2048, 1024):
-        "its access pattern: the same loads, lane shifts and stores with a trivial sum in place of the FCT "
-        "arithmetic take 174-181 us (3.4 TB/s algorithmic, ~4.1 TB/s of traffic, mostly from HBM), ~80% of "
-        "the kernel's time; the rest is FP64 arithmetic that 2 waves per SIMD (256 VGPRs) do not hide "
-        "(VALU busy 0.43, a 7.5% cut in VALU instructions leaves the time unchanged)",
-    ("k_maxwell", "fp64", 2048, 1024): "HBM: 702 MB per launch (1.20x algorithmic) at 5.6 TB/s; waves wait on "
-                                       "memory 0.47 of their life",
+        "memory-level parallelism: ~13,700 HBM reads (~1.5 MB) in flight at a 0.63 us read latency, a third of "
+        "what 8 TB/s needs (profiles/latency_icache_2048x1024_fp64.md), because each wave holds one row of loads "
+        "ahead in registers at 2 waves per SIMD (~190 VGPRs); its access pattern alone (the same loads, lane "
+        "shifts and stores with a trivial sum) takes 174-181 us, ~80% of the kernel's time, and the rest is FP64 "
+        "arithmetic that 2 waves per SIMD do not hide (VALU busy 0.43)",
+    ("k_maxwell", "fp64", 2048, 1024): "HBM: 620 MB per launch (1.06x algorithmic) at 5.0 TB/s, ~21,500 reads "
+                                       "in flight (profiles/latency_icache_2048x1024_fp64.md)",
+    ("k_fct_xz", "fp32", 4096, 2048):
+        "the fused sweep's memory-level parallelism and FP32 arithmetic at 3 waves per SIMD: the same instruction "
+        "count as the FP64 form on half the bytes",
+    ("k_fct_xz", "fp64", 4096, 2048):
+        "memory-level parallelism at 2 waves per SIMD, as at 2048x1024 (profiles/latency_icache_2048x1024_fp64.md)",
+    ("k_fct_xz", "fp64", 256, 128):
+        "latency: 640 waves of 10 rows each on a 256x128 grid (a quarter of the device's wave slots), each row "
+        "waiting on the loads issued one row earlier",
 }
 
 
