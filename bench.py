@@ -109,7 +109,7 @@ LIMITER = {
     ("k_fct_xz", "fp64", 4096, 2048):
         "memory-level parallelism at 2 waves per SIMD, as at 2048x1024 (profiles/latency_icache_2048x1024_fp64.md)",
     ("k_fct_xz", "fp64", 256, 128):
-        "latency: 640 waves of 10 rows each on a 256x128 grid (a quarter of the device's wave slots), each row "
+        "latency: 640 waves of 10 rows each on a 256x128 grid (under a third of the wave slots at 2 per SIMD), each row "
         "waiting on the loads issued one row earlier",
 }
 
