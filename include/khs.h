@@ -270,6 +270,9 @@ int khs_energy0(khs_ctx* ctx, double* E0);
  * out[3..5] = rms G^a with rho = -nQ (continuity-consistent sign),
  * out[6] = Etot_scale = rms(sum|E|)/dx. */
 int khs_gauss(khs_ctx* ctx, double out[7]);
+/* The same residual as gauss_check.py reports it (:95-107): rms[a] = rms G^a
+ * with rho = +nQ, rel[a] = rms[a] / (rms(sum|E|)/dx). */
+int khs_gauss_residual(khs_ctx* ctx, double rms[3], double rel[3]);
 /* Circular amplitude of the colour-2/3 potential at z mode kz
  * (analysis/remote_timeseries.py:62-77): F(x) = sum_z (Az2 + i Az3)
  * e^{-2 pi i kz z/NZ}; out[0] = mean_x |F| / NZ, out[1] = the same for

@@ -1586,6 +1586,18 @@ int khs_energy0(khs_ctx* ctx, double* E0) {
     return KHS_OK;
 }
 
+int khs_gauss_residual(khs_ctx* ctx, double rms[3], double rel[3]) {
+    if (!rms || !rel) { g_last_error = "null argument"; return KHS_ERR_ARG; }
+    double g[7];
+    const int rc = khs_gauss(ctx, g);
+    if (rc != KHS_OK) return rc;
+    for (int a = 0; a < 3; ++a) {
+        rms[a] = g[a];
+        rel[a] = g[a] / g[6];
+    }
+    return KHS_OK;
+}
+
 int khs_gauss(khs_ctx* ctx, double out[7]) {
     CTX_OR_FAIL(ctx);
     if (!out) { g_last_error = "null argument"; return KHS_ERR_ARG; }

@@ -132,6 +132,7 @@ def _signatures():
         "khs_get_step": (c_int, [c_void_p, P(c_int), dp]),
         "khs_energy": (c_int, [c_void_p, dp]), "khs_energy0": (c_int, [c_void_p, dp]),
         "khs_gauss": (c_int, [c_void_p, dp]),
+        "khs_gauss_residual": (c_int, [c_void_p, dp, dp]),
         "khs_write_snapshot": (c_int, [c_void_p, c_char_p, c_int]),
         "khs_energy_csv_open": (c_int, [c_void_p, c_char_p]),
         "khs_energy_csv_row": (c_int, [c_void_p, c_int, c_double]),
@@ -374,6 +375,12 @@ class Simulation:
         _check(_lib.khs_gauss(self._h, out))
         v = list(out)
         return {"rms_ref": v[0:3], "rms_cont": v[3:6], "Etot_scale": v[6]}
+
+    def gauss_residual(self):
+        """(rms[3], rel[3]) as analysis/gauss_check.py reports them (khs_gauss_residual)."""
+        rms, rel = (ctypes.c_double * 3)(), (ctypes.c_double * 3)()
+        _check(_lib.khs_gauss_residual(self._h, rms, rel))
+        return list(rms), list(rel)
 
     def mode_amplitude(self, kz):
         """Circular amplitude of Az2 + i Az3 at z mode kz -> (amp, amp_conj)."""

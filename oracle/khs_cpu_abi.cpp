@@ -245,6 +245,17 @@ int khs_gauss(khs_ctx* ctx, double out[7]) {
     orc_gauss(c.orc, out);
     return KHS_OK;
 }
+int khs_gauss_residual(khs_ctx* ctx, double rms[3], double rel[3]) {
+    if (!rms || !rel) { g_err = "null argument"; return KHS_ERR_ARG; }
+    double g[7];
+    const int rc = khs_gauss(ctx, g);
+    if (rc != KHS_OK) return rc;
+    for (int a = 0; a < 3; ++a) {
+        rms[a] = g[a];
+        rel[a] = g[a] / g[6];
+    }
+    return KHS_OK;
+}
 // remote_timeseries.py:62-77 on the host: mean_x |sum_z (Az2 +- i Az3) e^{-2 pi i kz z / NZ}| / NZ
 int khs_mode_amplitude(khs_ctx* ctx, int kz, double out[2]) {
     CTX(ctx);

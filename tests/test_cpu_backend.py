@@ -55,6 +55,9 @@ def test_cpu_backend_steps_are_the_oracle_steps(cpu):
     assert sim.energy() == o.energy() and sim.step_count == 7
     g = sim.gauss()
     np.testing.assert_array_equal(np.array(g["rms_ref"] + g["rms_cont"] + [g["Etot_scale"]]), o.gauss())
+    rms, rel = sim.gauss_residual()
+    np.testing.assert_array_equal(rms, o.gauss()[0:3])
+    np.testing.assert_array_equal(rel, o.gauss()[0:3] / o.gauss()[6])
     # set_array invalidates the sources: a state written from outside steps
     # like the oracle from that state
     arr = sim.get("By2") * 1.5

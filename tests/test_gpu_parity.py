@@ -223,6 +223,9 @@ def test_energy_gauss_close():
     g, go = sim.gauss(), o.gauss()
     got = np.array(g["rms_ref"] + g["rms_cont"] + [g["Etot_scale"]])
     np.testing.assert_allclose(got, go, rtol=1e-12)
+    rms, rel = sim.gauss_residual()             # gauss_check.py's rms and relative residual
+    np.testing.assert_allclose(rms, go[0:3], rtol=1e-12)
+    np.testing.assert_allclose(rel, np.asarray(go[0:3]) / go[6], rtol=1e-12)
 
 
 def test_fp32_form_close():
