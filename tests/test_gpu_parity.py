@@ -419,6 +419,9 @@ MXZ_CASES = {
     # several chunks, the last one ragged; columns not a multiple of 58
     "periodic_tall_nofilter": grid(116, 300, 1),
     "walled_58_columns": grid(116, 64, 0, xi_sponge=3.0, sigma_sponge=2.0),
+    # one wave column holds both walls / wraps around the periodic grid
+    "walled_narrow": grid(40, 24, 0, xi_cut=1.0),
+    "periodic_narrow_short": grid(50, 12, 1, xi_sponge=1.0, sigma_sponge=1.0, freeze_override=0),
 }
 
 
