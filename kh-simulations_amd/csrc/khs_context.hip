@@ -590,15 +590,15 @@ struct Ctx {
 };
 
 Ctx::~Ctx() {
-    if (stream) hipStreamSynchronize(stream);
+    if (stream) (void)hipStreamSynchronize(stream);
     for (auto g : sgraph)
-        if (g) hipGraphExecDestroy(g);
-    for (auto& p : pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
-    for (auto e : evpool) hipEventDestroy(e);
-    if (dmem) hipFree(dmem);
-    if (spec_mem) hipFree(spec_mem);
-    if (h_red) hipHostFree(h_red);
-    if (stream) hipStreamDestroy(stream);
+        if (g) (void)hipGraphExecDestroy(g);
+    for (auto& p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+    for (auto e : evpool) (void)hipEventDestroy(e);
+    if (dmem) (void)hipFree(dmem);
+    if (spec_mem) (void)hipFree(spec_mem);
+    if (h_red) (void)hipHostFree(h_red);
+    if (stream) (void)hipStreamDestroy(stream);
 }
 
 // --------------------------------------------------------------------------
@@ -838,7 +838,7 @@ static void run_steps(Ctx& c, int L);
 // replay because buffer and parity alternate together).
 static void capture_stride(Ctx& c, int stride) {
     hipGraphExec_t& slot = c.sgraph[c.cur];
-    if (slot) hipGraphExecDestroy(slot);
+    if (slot) (void)hipGraphExecDestroy(slot);
     slot = nullptr;
     c.autotune();                               // tuning launches and waits: before capture
     HIPCHK(hipStreamSynchronize(c.stream));
@@ -851,8 +851,8 @@ static void capture_stride(Ctx& c, int stride) {
         c.energy_async();
     } catch (...) {
         hipGraph_t g = nullptr;
-        hipStreamEndCapture(c.stream, &g);       // leave the stream usable
-        if (g) hipGraphDestroy(g);
+        (void)hipStreamEndCapture(c.stream, &g);       // leave the stream usable
+        if (g) (void)hipGraphDestroy(g);
         c.cur = cur0;
         c.step = step0;
         throw;
@@ -862,7 +862,7 @@ static void capture_stride(Ctx& c, int stride) {
     c.cur = cur0;                               // the graph does the stepping
     c.step = step0;
     hipError_t e = hipGraphInstantiate(&slot, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);
     if (e != hipSuccess) {
         slot = nullptr;
         throw Error(KHS_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
@@ -945,7 +945,7 @@ struct Batch {
     }
     ~Batch();
     void release_graph() {
-        if (exec) hipGraphExecDestroy(exec);
+        if (exec) (void)hipGraphExecDestroy(exec);
         exec = nullptr;
         exec_k = 0;
         exec_set.clear();
@@ -992,7 +992,7 @@ struct Batch {
         hipGraph_t g = nullptr;
         HIPCHK(hipStreamEndCapture(origin, &g));
         hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
-        hipGraphDestroy(g);
+        (void)hipGraphDestroy(g);
         if (e != hipSuccess) {
             exec = nullptr;
             throw Error(KHS_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
@@ -1072,7 +1072,7 @@ struct Batch {
         const size_t dbytes = round_up(d.size() * sizeof(PackedRun<T>), 256);
         const size_t ibytes = round_up(3 * (size_t)(n + 1) * sizeof(int), 256);
         if (pk_bytes < dbytes + ibytes) {
-            if (pk_mem) hipFree(pk_mem);
+            if (pk_mem) (void)hipFree(pk_mem);
             pk_mem = nullptr;
             HIPCHK(hipMalloc((void**)&pk_mem, dbytes + ibytes));
             pk_bytes = dbytes + ibytes;
@@ -1106,8 +1106,8 @@ struct Batch {
                 HIPCHK(hipEventElapsedTime(&ms, ea, eb));
                 if (ms < best) { best = ms; pk_shape = shape; }
             }
-            hipEventDestroy(ea);
-            hipEventDestroy(eb);
+            (void)hipEventDestroy(ea);
+            (void)hipEventDestroy(eb);
         }
         offsets(pk_shape, ix);
         HIPCHK(hipMemcpy(pk_mem + dbytes, ix.data(), ix.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -1125,7 +1125,7 @@ struct Batch {
         hipGraph_t g = nullptr;
         HIPCHK(hipStreamEndCapture(origin, &g));
         hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
-        hipGraphDestroy(g);
+        (void)hipGraphDestroy(g);
         if (e != hipSuccess) {
             exec = nullptr;
             throw Error(KHS_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
@@ -1148,13 +1148,13 @@ struct Batch {
 Batch::~Batch() {
     release_graph();
     if (pk_mem) {
-        if (origin) hipStreamSynchronize(origin);
-        hipFree(pk_mem);
+        if (origin) (void)hipStreamSynchronize(origin);
+        (void)hipFree(pk_mem);
     }
-    if (origin) hipStreamSynchronize(origin);
-    for (auto e : ev) hipEventDestroy(e);
-    for (auto l : lanes) hipStreamDestroy(l);
-    if (origin) hipStreamDestroy(origin);
+    if (origin) (void)hipStreamSynchronize(origin);
+    for (auto e : ev) (void)hipEventDestroy(e);
+    for (auto l : lanes) (void)hipStreamDestroy(l);
+    if (origin) (void)hipStreamDestroy(origin);
     runs.clear();
 }
 
@@ -1341,7 +1341,7 @@ int khs_get_array(khs_ctx* ctx, int id, double* host, size_t n) {
             khs::download(c, c.st[c.cur][slot], c.fp32, host);
         } else if (c.emhd && id >= KHS_EM_JX && id <= KHS_EM_ENERGY_DENSITY) {
             const int w = id - KHS_EM_JX;
-            if (c.nsp < 2 && (w == 5 || w == 6 || w >= 10 && w <= 12))
+            if (c.nsp < 2 && (w == 5 || w == 6 || (w >= 10 && w <= 12)))
                 throw Error(KHS_ERR_ARG, "array id " + std::to_string(id) + " needs species = 2");
             if (c.fp32) khs::launch_derived_em<float>(c.state<float>(c.cur), w, c.kp, c.d_derived, c.stream);
             else khs::launch_derived_em<double>(c.state<double>(c.cur), w, c.kp, c.d_derived, c.stream);

@@ -2423,8 +2423,12 @@ __device__ __forceinline__ void xz_march_g(const State<T>& S, const State<T>& O,
         if (MODE) zp.push_steady(V, uz, Sv, dt, dtdz, out);
         else have = zp.push(e, z0, V, uz, Sv, dt, dtdz, out);
         if (have) {
-            if (GR::has(F_N) && out[F_N - VB] < T(0.05)) out[F_N - VB] = T(0.05);   // kernel_enforce_boundaries
-            if (GR::has(F_PX) && wall) out[F_PX - VB] = T(0.0);
+            if constexpr (GR::has(F_N)) {                                        // kernel_enforce_boundaries
+                if (out[F_N - VB] < T(0.05)) out[F_N - VB] = T(0.05);
+            }
+            if constexpr (GR::has(F_PX)) {
+                if (wall) out[F_PX - VB] = T(0.0);
+            }
             const unsigned off =
                 store ? ((unsigned)xc + (unsigned)(e - 3) * (unsigned)nx) * (unsigned)sizeof(T) : kNoStore;
 #pragma unroll
@@ -2638,7 +2642,9 @@ __device__ __forceinline__ void xz_march_em(const State<T>& S, const State<T>& O
         else have = zp.push(e, z0, V, uz, Sv, dt, dtdz, out);
         if (have) {
             if (VB == EV_N && out[0] < T(0.05)) out[0] = T(0.05);          // enforce_boundaries (fluid)
-            if (VB <= EV_PX && EV_PX < VB + NG && wall) out[EV_PX - VB] = T(0.0);
+            if constexpr (VB <= EV_PX && EV_PX < VB + NG) {
+                if (wall) out[EV_PX - VB] = T(0.0);
+            }
             const unsigned off =
                 store ? ((unsigned)xc + (unsigned)(e - 3) * (unsigned)nx) * (unsigned)sizeof(T) : kNoStore;
 #pragma unroll
@@ -3252,8 +3258,12 @@ __device__ __forceinline__ void mxz_sweep(const State<T>& I, const State<T>& O, 
         if (MODE) zp.push_steady(V, uz, Sv, dt, dtdz, out);
         else have = zp.push(e, z0, V, uz, Sv, dt, dtdz, out);
         if (have) {
-            if (GR::has(F_N) && out[F_N - VB] < T(0.05)) out[F_N - VB] = T(0.05);   // kernel_enforce_boundaries
-            if (GR::has(F_PX) && wall) out[F_PX - VB] = T(0.0);
+            if constexpr (GR::has(F_N)) {                                        // kernel_enforce_boundaries
+                if (out[F_N - VB] < T(0.05)) out[F_N - VB] = T(0.05);
+            }
+            if constexpr (GR::has(F_PX)) {
+                if (wall) out[F_PX - VB] = T(0.0);
+            }
             const unsigned off =
                 store ? ((unsigned)xc + (unsigned)(e - 3) * (unsigned)nx) * (unsigned)sizeof(T) : kNoStore;
 #pragma unroll
