@@ -63,7 +63,7 @@ void launch_maxwell(const State<T>& S, const State<T>& O, const KParams& p, cons
 // charges it reads) in one launch, YM model without column filters.  I:
 // fields of step n, fluid after the sweeps of step n; O: fields of step
 // n + 1, fluid after the sweeps of step n + 1.  Variants 0/1/2 = 16/32/64-row
-// chunks, 3/4/5 the same with Maxwell loading two rows ahead.
+// chunks with two Maxwell wavefronts per workgroup, 3/4/5 with four.
 constexpr int kNumMxzVariants = 6;
 template <typename T>
 void launch_maxwell_sweep(const State<T>& I, const State<T>& O, const KParams& p, const double* decay,

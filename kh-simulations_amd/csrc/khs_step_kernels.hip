@@ -2860,38 +2860,39 @@ void launch_maxwell(const State<T>& S, const State<T>& O, const KParams& p, cons
 // reference's order is sweeps -> Maxwell -> filters -> the next step's
 // sources and sweeps (main_ym.cu:322-455).  Maxwell and the sweeps are local
 // in x and z, so one workgroup can do both for a column tile and z chunk:
-//   * wave 4 marches Maxwell (the arithmetic of k_maxwell_zm, operation for
-//     operation) over the rows the tile's sweep reads, on the fields of step
-//     n and the fluid after the sweeps of step n; it applies the column
-//     action to the colour-2/3 fields, stores the tile's own cells and puts
-//     each new field row into an LDS ring of MXZ_SLOTS rows;
+//   * waves 4 .. 4 + M - 1 compute Maxwell (the arithmetic of k_maxwell_zm,
+//     operation for operation) for the rows the tile's sweep reads, on the
+//     fields of step n and the fluid after the sweeps of step n, one row per
+//     task, the rows dealt round-robin to the M waves; each applies the
+//     column action to the colour-2/3 fields, stores the tile's own cells
+//     and puts the new field row into an LDS ring of MXZ_SLOTS rows;
 //   * waves 0..3 march the fused x + z sweep of xz_march_g (species w / 2,
 //     variable group w % 2) on that fluid, with the column action applied to
 //     the colour-2/3 charges as they are loaded (the filter stage acts on the
 //     charges after Maxwell, whose currents use them unfiltered), and take
 //     the new field rows of their sources from the ring.
-// The waves are not in lockstep: Maxwell publishes how many ring rows it has
-// written, each sweep wave how many it no longer reads, and each waits (in
-// LDS, with a bounded spin) only for what it needs, so Maxwell runs up to
-// MXZ_SLOTS - 2 rows ahead and a wave's memory stalls overlap the others'
-// arithmetic instead of holding the workgroup.  Per cell-update
+// The waves are not in lockstep: a Maxwell wave publishes each ring row it
+// has written in that row's slot, each sweep wave how many rows it no longer
+// reads, and each waits (in LDS, with a bounded spin) only for what it needs,
+// so Maxwell runs up to MXZ_SLOTS - 2 rows ahead and a wave's memory stalls
+// overlap the others' arithmetic instead of holding the workgroup.  Per cell-update
 // the launch reads the fluid and the fields once and writes them once: 47
 // arrays, against 36 + 35 for the sweep and Maxwell kernels.
 // I: fields of step n, fluid after the sweeps of step n; O: fields of step
 // n + 1 (before the charge filter), fluid after the sweeps of step n + 1.
 // ==========================================================================
-constexpr int MXZ_SLOTS = 6;
+constexpr int MXZ_SLOTS = 8;
 
 // The ring: slot k % MXZ_SLOTS holds the k-th new field row of the tile
 // (state ids S_EX1 .. S_AZ3, one value per lane).  Rows are produced in the
 // order the sweep reads them: zm_per(rc0) first (k = 0), then rc0, rc0 + 1,
 // ... (k = 1 + row - rc0), rc0 = the chunk's first (clamped) halo row.
-// produced: rows written so far; released[w]: sweep wave w reads no row
-// below it any more.
+// ready[s]: 1 + the ring row whose values slot s holds (0: none yet);
+// released[w]: sweep wave w reads no row below it any more.
 template <typename T>
 struct MxzShared {
     T ring[MXZ_SLOTS][12][64];
-    int produced;
+    int ready[MXZ_SLOTS];
     int released[4];
 };
 template <typename T>
@@ -2912,8 +2913,12 @@ __device__ __forceinline__ void mxz_publish(int* c, int v) {
 __device__ __forceinline__ int mxz_read(const int* c) {
     return __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
-__device__ __forceinline__ void mxz_wait_produced(const int* c, int v) {
-    for (int i = 0; i < kMxzSpinLimit && mxz_read(c) < v; ++i) __builtin_amdgcn_s_sleep(1);
+// wait until ring rows k and k - 1 are written (their slots' ready counts)
+__device__ __forceinline__ void mxz_wait_rows(const int (&ready)[MXZ_SLOTS], int k) {
+    for (int i = 0; i < kMxzSpinLimit; ++i) {
+        if (mxz_read(&ready[k % MXZ_SLOTS]) >= k + 1 && mxz_read(&ready[(k - 1) % MXZ_SLOTS]) >= k) break;
+        __builtin_amdgcn_s_sleep(1);
+    }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 __device__ __forceinline__ void mxz_wait_released(const int (&r)[4], int v) {
@@ -2925,14 +2930,56 @@ __device__ __forceinline__ void mxz_wait_released(const int (&r)[4], int v) {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// Wave 4: the Maxwell march.  mstep(fin, k) consumes the inputs of row q
-// (cur), computes Ampere's Ex there and, with fin, finishes row q - 1 (Jz,
-// Ampere's Ez, potential, Faraday) into ring slot k % 3; then moves to row
-// q + 1.  Row q - 1 of the march is thus the k-th produced row.
-template <typename T, int TZ, int PF>
+// Waves 4 .. 4 + M - 1: Maxwell as row tasks.  Ring row k is grid row
+// r = rf + k (periodic, rf = zm_per(rc0)); Maxwell wave m finishes rows k = m,
+// m + M, m + 2M, ..., each from its own loads (rows r - 1 .. r + 1), so the M
+// waves work on different rows at once instead of one serial march.  A task
+// evaluates Ampere's Ex at rows r and r + 1 (the task of row r + 1 evaluates
+// the latter again: the same operations on the same inputs, hence the same
+// bits) and, for row r, Jz, Ampere's Ez, the potential and Faraday, with the
+// expressions of maxwell_zm_wave in the same order.
+template <typename T>
+__device__ __forceinline__ void mxz_cell(const MRow<T>& m, T (&ux)[2], T (&uz)[2], T (&nq)[2][3]) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const T nf = kmax(m.n[s], T(1e-10));
+        ux[s] = m.px[s] / nf;
+        uz[s] = m.pz[s] / nf;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) nq[s][c] = m.n[s] * m.q[s][c];
+    }
+}
+// Ampere's Ex (YM_Physics.cu:70-113) at the row of m; byzm: By of the row
+// below, Jx at x + 1/2 from lane l + 1
+template <typename T>
+__device__ __forceinline__ void mxz_ampere_ex(const MRow<T>& m, const T (&byzm)[3], const T (&ux)[2],
+                                              const T (&nq)[2][3], const KParams& p, T (&exn)[3]) {
+    const T dt = T(p.dt), c2 = T(p.c2), aYM = T(p.alpha), eps0 = T(p.eps0), dz = T(p.dz);
+    T Jx[3];
+    {
+        T vxr[2], nQx[2][3];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            vxr[s] = T(0.5) * (ux[s] + lane_from_right(ux[s]));
+#pragma unroll
+            for (int c = 0; c < 3; ++c) nQx[s][c] = T(0.5) * (nq[s][c] + lane_from_right(nq[s][c]));
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Jx[c] = -(nQx[0][c] * vxr[0] + nQx[1][c] * vxr[1]);
+    }
+    const T cr[3] = {m.az[1] * m.by[2] - m.az[2] * m.by[1], m.az[2] * m.by[0] - m.az[0] * m.by[2],
+                     m.az[0] * m.by[1] - m.az[1] * m.by[0]};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        T dBdz = (m.by[c] - byzm[c]) / dz;
+        exn[c] = m.ex[c] + dt * (c2 * (-dBdz - aYM * cr[c]) - Jx[c] / eps0);
+    }
+}
+
+template <typename T, int TZ, int M>
 __device__ __forceinline__ void mxz_maxwell(const State<T>& I, const State<T>& O, const KParams& p, int g, int ch,
-                                            int lane, MxzShared<T>& sh, const double* decay, const int* action) {
-    MxzRing<T>& ring = sh.ring;
+                                            int lane, int m, MxzShared<T>& sh, const double* decay,
+                                            const int* action) {
     const int nx = p.nx, nz = p.nz;
     const bool per = p.periodic_x != 0;
     const int x = g * XZR_OUT - 3 + lane;
@@ -2959,147 +3006,84 @@ __device__ __forceinline__ void mxz_maxwell(const State<T>& I, const State<T>& O
         rez[c] = buf_rsrc(O.a[S_EZ1 + c], bytes);
         raz[c] = buf_rsrc(O.a[S_AZ1 + c], bytes);
     }
-    // march rows count from rf - 1 and wrap periodically (z is periodic for Maxwell)
     auto rbase = [&](int r) { return (unsigned)((r + nz) % nz) * (unsigned)nx; };
-
-    T byp[3], byxmp[3], azp[3], ezp[3], exn_p[3], uzp[2], nqp[2][3];
-    {
-        const unsigned im = rbase(rf - 1) + (unsigned)xc;
+    // rows k = 0 (zm_per(rc0)), 1 (rc0), ..., 1 + clamp(elast) - rc0
+    const int nrows = 2 + clampi(elast, 0, nz - 1) - rc0;
+#pragma unroll 1
+    for (int k = m; k < nrows; k += M) {
+        const int q = rf + k;                     // unwrapped grid row of ring row k
+        MRow<T> a, b;                             // rows q, q + 1
+        T bym[3];                                 // By of row q - 1
+        mrow_load(I, I, rbase(q) + (unsigned)xc, rbase(q) + (unsigned)xm, a);
+        mrow_load(I, I, rbase(q + 1) + (unsigned)xc, rbase(q + 1) + (unsigned)xm, b);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) byp[c] = ld32(I.a[S_BY1 + c], im);
-    }
-    MRow<T> cur, nxt;
-    int q = rf;
-    mrow_load(I, I, rbase(q) + (unsigned)xc, rbase(q) + (unsigned)xm, cur);
-    if (PF) mrow_load(I, I, rbase(q + 1) + (unsigned)xc, rbase(q + 1) + (unsigned)xm, nxt);
-
-    auto mstep = [&](bool fin, int k) {
-        MRow<T> nn;
-        if (PF) mrow_load(I, I, rbase(q + 2) + (unsigned)xc, rbase(q + 2) + (unsigned)xm, nn);
+        for (int c = 0; c < 3; ++c) bym[c] = ld32(I.a[S_BY1 + c], rbase(q - 1) + (unsigned)xc);
         // cell quantities of the currents (kernel_ym_currents, YM_Physics.cu:9-60)
-        T ux[2], uz[2], nq[2][3];
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            const T nf = kmax(cur.n[s], T(1e-10));
-            ux[s] = cur.px[s] / nf;
-            uz[s] = cur.pz[s] / nf;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) nq[s][c] = cur.n[s] * cur.q[s][c];
-        }
-        // Jx at (x+1/2, q) with the x+1 values of lane l+1
-        T Jx[3];
+        T uxa[2], uza[2], nqa[2][3], uxb[2], uzb[2], nqb[2][3];
+        mxz_cell(a, uxa, uza, nqa);
+        mxz_cell(b, uxb, uzb, nqb);
+        T exa[3], exb[3];
+        mxz_ampere_ex(a, bym, uxa, nqa, p, exa);
+        mxz_ampere_ex(b, a.by, uxb, nqb, p, exb);
+        // row q: Jz at (x, q + 1/2), Ampere for Ez, potential, Faraday
+        T ezn[3];
         {
-            T vxr[2], nQx[2][3];
+            T vzt[2], nQz[2][3];
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
-                vxr[s] = T(0.5) * (ux[s] + lane_from_right(ux[s]));
+                vzt[s] = T(0.5) * (uza[s] + uzb[s]);
 #pragma unroll
-                for (int c = 0; c < 3; ++c) nQx[s][c] = T(0.5) * (nq[s][c] + lane_from_right(nq[s][c]));
+                for (int c = 0; c < 3; ++c) nQz[s][c] = T(0.5) * (nqa[s][c] + nqb[s][c]);
             }
 #pragma unroll
-            for (int c = 0; c < 3; ++c) Jx[c] = -(nQx[0][c] * vxr[0] + nQx[1][c] * vxr[1]);
-        }
-        // Ampere for Ex at row q (YM_Physics.cu:70-113)
-        T exn[3];
-        {
-            const T cr[3] = {cur.az[1] * cur.by[2] - cur.az[2] * cur.by[1],
-                             cur.az[2] * cur.by[0] - cur.az[0] * cur.by[2],
-                             cur.az[0] * cur.by[1] - cur.az[1] * cur.by[0]};
-#pragma unroll
             for (int c = 0; c < 3; ++c) {
-                T dBdz = (cur.by[c] - byp[c]) / dz;
-                exn[c] = cur.ex[c] + dt * (c2 * (-dBdz - aYM * cr[c]) - Jx[c] / eps0);
+                T Jz = -(nQz[0][c] * vzt[0] + nQz[1][c] * vzt[1]);
+                T dBdx = (a.by[c] - a.byxm[c]) / dx;
+                T ez = wall ? T(0.0) : a.ez[c];
+                ezn[c] = ez + dt * (c2 * dBdx - Jz / eps0);
             }
         }
-        if (fin) {
-            // row q-1: Jz at (x, q-1/2), Ampere for Ez, potential, Faraday
-            T ezn[3];
-            {
-                T vzt[2], nQz[2][3];
-#pragma unroll
-                for (int s = 0; s < 2; ++s) {
-                    vzt[s] = T(0.5) * (uzp[s] + uz[s]);
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) nQz[s][c] = T(0.5) * (nqp[s][c] + nq[s][c]);
-                }
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    T Jz = -(nQz[0][c] * vzt[0] + nQz[1][c] * vzt[1]);
-                    T dBdx = (byp[c] - byxmp[c]) / dx;
-                    T ez = wall ? T(0.0) : ezp[c];
-                    ezn[c] = ez + dt * (c2 * dBdx - Jz / eps0);
-                }
-            }
-            T ezr[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                ezr[c] = lane_from_right(ezn[c]);
-                if (last_wall) ezr[c] = ezn[c];
-            }
-            // kernel_ym_potential (YM_Physics.cu:589-599), kernel_ym_faraday (:120-151)
-            T Az[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                Az[c] = azp[c];
-                if (c > 0 || !p.freeze_az1) Az[c] = Az[c] - dt * ezn[c];
-            }
-            const T cr[3] = {Az[1] * exn_p[2] - Az[2] * exn_p[1],
-                             Az[2] * exn_p[0] - Az[0] * exn_p[2],
-                             Az[0] * exn_p[1] - Az[1] * exn_p[0]};
-            const int r = (q - 1 + nz) % nz;
-            // the tile's own cells (each row once: k >= 1)
-            const bool own = store && k >= 1 && r >= z0 && r < zend;
-            const unsigned off = own ? ((unsigned)r * (unsigned)nx + (unsigned)xc) * (unsigned)sizeof(T) : kNoStore;
-            const int slot = k % MXZ_SLOTS;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                T dEx = (exn[c] - exn_p[c]) / dz;
-                T dEz = (ezr[c] - ezn[c]) / dx;
-                T byn = byp[c] - dt * (dEx - dEz - aYM * cr[c]);
-                T exo = exn_p[c], ezo = ezn[c], azo = Az[c];
-                // sponge / xi_cut of the colour-2/3 fields (YM_Physics.cu:161-210)
-                if (c > 0 && act == 1) { byn *= dec; exo *= dec; ezo *= dec; azo *= dec; }
-                if (c > 0 && act == 2) { byn = T(0.0); exo = T(0.0); ezo = T(0.0); azo = T(0.0); }
-                buf_st(byn, rby[c], off);
-                buf_st(exo, rex[c], off);
-                buf_st(ezo, rez[c], off);
-                if (c > 0 || !p.freeze_az1) buf_st(azo, raz[c], off);
-                ring[slot][S_EX1 + c][lane] = exo;
-                ring[slot][S_EZ1 + c][lane] = ezo;
-                ring[slot][S_BY1 + c][lane] = byn;
-                ring[slot][S_AZ1 + c][lane] = azo;
-            }
-        }
-        // carry row q
+        T ezr[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            byp[c] = cur.by[c]; byxmp[c] = cur.byxm[c]; azp[c] = cur.az[c]; ezp[c] = cur.ez[c]; exn_p[c] = exn[c];
+            ezr[c] = lane_from_right(ezn[c]);
+            if (last_wall) ezr[c] = ezn[c];
         }
+        // kernel_ym_potential (YM_Physics.cu:589-599), kernel_ym_faraday (:120-151)
+        T Az[3];
 #pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            uzp[s] = uz[s];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) nqp[s][c] = nq[s][c];
+        for (int c = 0; c < 3; ++c) {
+            Az[c] = a.az[c];
+            if (c > 0 || !p.freeze_az1) Az[c] = Az[c] - dt * ezn[c];
         }
-        ++q;
-        if (PF) {
-            cur = nxt;
-            nxt = nn;
-        } else {
-            mrow_load(I, I, rbase(q) + (unsigned)xc, rbase(q) + (unsigned)xm, cur);
-        }
-    };
-
-    // rows k = 0 (zm_per(rc0)), 1 (rc0), ..., 1 + rlast - rc0 (rlast: the
-    // sweep's last row, clamp(elast)); row k overwrites row k - MXZ_SLOTS,
-    // so it waits until every sweep wave has released that one
-    const int nrows = 2 + clampi(elast, 0, nz - 1) - rc0;
-    mstep(false, 0);
-#pragma unroll 1
-    for (int k = 0; k < nrows; ++k) {
+        const T cr[3] = {Az[1] * exa[2] - Az[2] * exa[1], Az[2] * exa[0] - Az[0] * exa[2],
+                         Az[0] * exa[1] - Az[1] * exa[0]};
+        const int r = (q + nz) % nz;
+        // the tile's own cells (each row once: k >= 1)
+        const bool own = store && k >= 1 && r >= z0 && r < zend;
+        const unsigned off = own ? ((unsigned)r * (unsigned)nx + (unsigned)xc) * (unsigned)sizeof(T) : kNoStore;
+        // row k overwrites row k - MXZ_SLOTS: wait until no sweep wave reads it
         if (k >= MXZ_SLOTS) mxz_wait_released(sh.released, k - MXZ_SLOTS + 1);
-        mstep(true, k);
-        mxz_publish(&sh.produced, k + 1);
+        const int slot = k % MXZ_SLOTS;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            T dEx = (exb[c] - exa[c]) / dz;
+            T dEz = (ezr[c] - ezn[c]) / dx;
+            T byn = a.by[c] - dt * (dEx - dEz - aYM * cr[c]);
+            T exo = exa[c], ezo = ezn[c], azo = Az[c];
+            // sponge / xi_cut of the colour-2/3 fields (YM_Physics.cu:161-210)
+            if (c > 0 && act == 1) { byn *= dec; exo *= dec; ezo *= dec; azo *= dec; }
+            if (c > 0 && act == 2) { byn = T(0.0); exo = T(0.0); ezo = T(0.0); azo = T(0.0); }
+            buf_st(byn, rby[c], off);
+            buf_st(exo, rex[c], off);
+            buf_st(ezo, rez[c], off);
+            if (c > 0 || !p.freeze_az1) buf_st(azo, raz[c], off);
+            sh.ring[slot][S_EX1 + c][lane] = exo;
+            sh.ring[slot][S_EZ1 + c][lane] = ezo;
+            sh.ring[slot][S_BY1 + c][lane] = byn;
+            sh.ring[slot][S_AZ1 + c][lane] = azo;
+        }
+        mxz_publish(&sh.ready[slot], k + 1);
     }
 }
 
@@ -3171,7 +3155,7 @@ __device__ __forceinline__ void mxz_sweep(const State<T>& I, const State<T>& O, 
         T nnF[NV];
         load(rowof(e + 2), nnF);
         const int kc = 1 + rc - rc0;            // ring index of row rc; row rc's predecessor is kc - 1
-        mxz_wait_produced(&sh.produced, kc + 1);
+        mxz_wait_rows(sh.ready, kc);
         const T(&cs)[12][64] = ring[kc % MXZ_SLOTS];
         const T(&ps)[12][64] = ring[(kc + MXZ_SLOTS - 1) % MXZ_SLOTS];
 
@@ -3292,10 +3276,10 @@ __device__ __forceinline__ void mxz_sweep(const State<T>& I, const State<T>& O, 
 }
 
 // Workgroup: waves 0..3 sweep (species w / 2, group w % 2), wave 4 Maxwell.
-// Occupancy: two waves per SIMD in FP64 (175 / 235 VGPRs; three would spill
-// 91), four in FP32.
-template <typename T, int TZ, int PF>
-__global__ __launch_bounds__(320) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2))) void k_mxz(
+// Occupancy: two waves per SIMD in FP64, four in FP32; 4 + M waves per
+// workgroup (M = 4: the eight wave slots of a CU in FP64).
+template <typename T, int TZ, int M>
+__global__ __launch_bounds__(64 * (4 + M)) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2))) void k_mxz(
     State<T> I, State<T> O, KParams p, int ngx, int nch, const double* decay, const int* action) {
     __shared__ MxzShared<T> sh;
     const int lane = threadIdx.x & 63;
@@ -3304,13 +3288,11 @@ __global__ __launch_bounds__(320) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
     if (lb >= ngx * nch) return;                // whole workgroups leave together
     lb = __builtin_amdgcn_readfirstlane(tile_order(lb, ngx * nch, p.rev));
     const int g = lb % ngx, ch = lb / ngx;
-    if (threadIdx.x == 0) {
-        sh.produced = 0;
-        for (int j = 0; j < 4; ++j) sh.released[j] = 0;
-    }
+    if (threadIdx.x < MXZ_SLOTS) sh.ready[threadIdx.x] = 0;
+    if (threadIdx.x < 4) sh.released[threadIdx.x] = 0;
     __syncthreads();                            // the only barrier: the counters are set
-    if (w == 4) {
-        mxz_maxwell<T, TZ, PF>(I, O, p, g, ch, lane, sh, decay, action);
+    if (w >= 4) {
+        mxz_maxwell<T, TZ, M>(I, O, p, g, ch, lane, w - 4, sh, decay, action);
         return;
     }
     const int sp = w >> 1;
@@ -3324,29 +3306,29 @@ __global__ __launch_bounds__(320) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
     }
 }
 
-template <typename T, int TZ, int PF>
+template <typename T, int TZ, int M>
 static void launch_mxz_t(const State<T>& I, const State<T>& O, const KParams& p, const double* decay,
                          const int* action, hipStream_t st) {
     const int ngx = (p.nx + XZR_OUT - 1) / XZR_OUT;
     const int nch = (p.nz + TZ - 1) / TZ;
-    hipLaunchKernelGGL((k_mxz<T, TZ, PF>), dim3(round8(ngx * nch)), dim3(320), 0, st, I, O, p, ngx, nch, decay,
-                       action);
+    hipLaunchKernelGGL((k_mxz<T, TZ, M>), dim3(round8(ngx * nch)), dim3(64 * (4 + M)), 0, st, I, O, p, ngx, nch,
+                       decay, action);
 }
 
-// Variants 0/1/2: 16/32/64-row chunks, Maxwell loads one row ahead; 3/4/5:
-// the same loading two rows ahead.
+// variants 0/1/2: 16/32/64-row chunks with two Maxwell waves, 3/4/5 with four
 template <typename T>
 void launch_maxwell_sweep(const State<T>& I, const State<T>& O, const KParams& p, const double* decay,
                           const int* action, int variant, hipStream_t st) {
     switch (variant) {
-        case 0: launch_mxz_t<T, 16, 0>(I, O, p, decay, action, st); return;
-        case 2: launch_mxz_t<T, 64, 0>(I, O, p, decay, action, st); return;
-        case 3: launch_mxz_t<T, 16, 1>(I, O, p, decay, action, st); return;
-        case 4: launch_mxz_t<T, 32, 1>(I, O, p, decay, action, st); return;
-        case 5: launch_mxz_t<T, 64, 1>(I, O, p, decay, action, st); return;
-        default: launch_mxz_t<T, 32, 0>(I, O, p, decay, action, st); return;
+        case 0: launch_mxz_t<T, 16, 2>(I, O, p, decay, action, st); return;
+        case 2: launch_mxz_t<T, 64, 2>(I, O, p, decay, action, st); return;
+        case 3: launch_mxz_t<T, 16, 4>(I, O, p, decay, action, st); return;
+        case 4: launch_mxz_t<T, 32, 4>(I, O, p, decay, action, st); return;
+        case 5: launch_mxz_t<T, 64, 4>(I, O, p, decay, action, st); return;
+        default: launch_mxz_t<T, 32, 2>(I, O, p, decay, action, st); return;
     }
 }
+
 template void launch_maxwell_sweep<double>(const State<double>&, const State<double>&, const KParams&, const double*,
                                            const int*, int, hipStream_t);
 template void launch_maxwell_sweep<float>(const State<float>&, const State<float>&, const KParams&, const double*,

@@ -1,19 +1,22 @@
-"""Model check of the progress-counter protocol that couples the five waves
-of k_mxz (Maxwell + the next step's sweeps in one launch): the Maxwell wave
-writes ring row k into slot k % S after every sweep wave has released row
-k - S, and publishes `produced = k + 1`; a sweep wave at ring row kc waits for
-produced >= kc + 1, reads rows kc and kc - 1, then releases the rows below the
-ones its next row reads.  Random interleavings of the five waves over the
-row sequences of many chunk shapes must finish (no wait without end) and
-never overwrite a row that a sweep wave may still read."""
+"""Model check of the progress-counter protocol that couples the waves of
+k_mxz (Maxwell + the next step's sweeps in one launch).  M Maxwell waves share
+the ring rows round-robin (wave m: rows m, m + M, ...); for row k a Maxwell
+wave waits until every sweep wave has released row k - S, writes slot k % S,
+then publishes ready[k % S] = k + 1.  A sweep wave at ring row kc waits for
+ready[kc % S] >= kc + 1 and ready[(kc - 1) % S] >= kc, reads rows kc and
+kc - 1, then releases the rows below the ones its next row reads.  Writing a
+slot, publishing it, reading the ring and releasing are separate steps.
+Random interleavings of all waves over the row sequences of many chunk shapes
+must finish (no wait without end) and never overwrite a row that a sweep wave
+may still read, nor let a sweep wave read a slot that holds another row."""
 import random
 
 import pytest
 
-S = 6       # MXZ_SLOTS
+S = 8       # MXZ_SLOTS
 
 
-def run(z0, tz, nz, seed):
+def run(z0, tz, nz, nmax, seed):
     rnd = random.Random(seed)
     zend = min(z0 + tz, nz)
     elast = zend + 2
@@ -23,39 +26,57 @@ def run(z0, tz, nz, seed):
         return max(0, min(min(e, elast), nz - 1))
 
     nrows = 2 + rowof(elast) - rc0
-    produced, released, held = 0, [0] * 4, {}
-    pk, ce = 0, [z0 - 3] * 4
+    ready, released, held = [0] * S, [0] * 4, {}
+    pk = list(range(nmax))          # next row of each Maxwell wave
+    pstate = [0] * nmax             # 0: before the write, 1: written, not yet published
+    ce = [z0 - 3] * 4               # next row e of each sweep wave
+    cstate = [0] * 4                # 0: before the read, 1: read, not yet released
+    reading = [None] * 4            # ring rows a sweep wave reads (between read and release)
     for _ in range(10 ** 6):
-        actors = (["P"] if pk < nrows else []) + [w for w in range(4) if ce[w] <= elast]
+        actors = [("P", m) for m in range(nmax) if pk[m] < nrows] + [("C", w) for w in range(4) if ce[w] <= elast]
         if not actors:
             return
-        a = rnd.choice(actors)
-        if a == "P":
-            k = pk
-            if k >= S and min(released) < k - S + 1:
-                continue                                   # waits
-            old = held.get(k % S)
-            for w in range(4):
-                if ce[w] <= elast and old is not None:
-                    kc = 1 + rowof(ce[w]) - rc0
-                    assert old not in (kc, kc - 1), f"row {old} overwritten while wave {w} reads it"
-            held[k % S] = k
-            produced, pk = k + 1, pk + 1
+        kind, i = rnd.choice(actors)
+        if kind == "P":
+            k = pk[i]
+            if pstate[i] == 0:
+                if k >= S and min(released) < k - S + 1:
+                    continue                               # waits
+                old = held.get(k % S)
+                for w in range(4):
+                    if old is not None and reading[w] is not None:
+                        assert old not in reading[w], f"row {old} overwritten while wave {w} reads it"
+                    if old is not None and ce[w] <= elast:
+                        kc = 1 + rowof(ce[w]) - rc0
+                        assert old not in (kc, kc - 1), f"row {old} overwritten before wave {w} reads it"
+                held[k % S] = k
+                pstate[i] = 1
+            else:
+                ready[k % S] = k + 1
+                pstate[i] = 0
+                pk[i] += nmax
         else:
-            e = ce[a]
+            e = ce[i]
             rc = rowof(e)
             kc = 1 + rc - rc0
-            if produced < kc + 1:
-                continue                                   # waits
-            assert held.get(kc % S) == kc and held.get((kc - 1) % S) == kc - 1
-            released[a] = kc + (rowof(e + 1) != rc) - 1
-            ce[a] += 1
+            if cstate[i] == 0:
+                if ready[kc % S] < kc + 1 or ready[(kc - 1) % S] < kc:
+                    continue                               # waits
+                assert held.get(kc % S) == kc and held.get((kc - 1) % S) == kc - 1
+                reading[i] = (kc, kc - 1)
+                cstate[i] = 1
+            else:
+                released[i] = kc + (rowof(e + 1) != rc) - 1
+                reading[i] = None
+                cstate[i] = 0
+                ce[i] += 1
     raise AssertionError("no progress")
 
 
 @pytest.mark.parametrize("nz", [4, 5, 7, 16, 33, 100])
-def test_progress_counters_end_and_protect_rows(nz):
+@pytest.mark.parametrize("nmax", [1, 2, 4])
+def test_progress_counters_end_and_protect_rows(nz, nmax):
     for tz in (16, 32, 64):
         for z0 in range(0, nz, tz):
             for seed in range(20):
-                run(z0, tz, nz, seed)
+                run(z0, tz, nz, nmax, seed)
