@@ -141,7 +141,7 @@ def kernel_instance(slot, t, T="double"):
         return f"k_maxwell_zm<{T}, {tzpf[0]}, {tzpf[1]}>" if tzpf else None
     if slot == "k_mxz":
         v = t.get("mxz_variant", 1)
-        return f"k_mxz<{T}, {(16, 32, 64)[v % 3]}, {(2, 4)[v // 3]}>"
+        return f"k_mxz<{T}, {(2, 2, 2, 4, 4, 4, 4, 4, 2)[v]}>"
     if slot == "k_maxwell":
         return {0: f"k_maxwell<{T}, 16>", 1: f"k_maxwell_cell<{T}>", 2: f"k_maxwell<{T}, 4>",
                 3: f"k_maxwell<{T}, 8>"}.get(t["maxwell"])

@@ -144,8 +144,10 @@ typedef struct khs_config {
     int      step_fusion;      /* YM fused-sweep steps with no column filter (only sponge / xi_cut): Maxwell of a step and
                                   the next step's sweeps as one launch; -1 = the autotuner's choice (default), 0 = off,
                                   1 = on */
-    int      mxz_variant;      /* that launch's shape: -1 = autotuned (default), 0..5 (16/32/64-row chunks with
-                                  two Maxwell wavefronts per workgroup; 3..5 with four) */
+    int      mxz_variant;      /* that launch's shape: -1 = autotuned (default), 0..8 (16/32/64-row chunks with
+                                  two Maxwell wavefronts per workgroup; 3..5 with four; 6, 7 with four and
+                                  chunks sized to fill the device's resident workgroups once / twice; 8 with
+                                  two, once) */
     int      packed_shape;     /* khs_batch_run_packed launch shape (read from the batch's first run): -1 = timed on
                                   the first capture (default), 0..7 (see khs_batch_run_packed) */
     /* --- derived (khs_derive) --- */

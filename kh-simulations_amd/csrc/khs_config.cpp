@@ -340,7 +340,7 @@ void config_derive(khs_config& c) {
     if (c.snake_order != 0 && c.snake_order != 1) throw Error(KHS_ERR_CONFIG, "snake_order must be 0 or 1");
     if (c.zm_fused < -1 || c.zm_fused > 3) throw Error(KHS_ERR_CONFIG, "zm_fused must be -1..3");
     if (c.step_fusion < -1 || c.step_fusion > 1) throw Error(KHS_ERR_CONFIG, "step_fusion must be -1, 0 or 1");
-    if (c.mxz_variant < -1 || c.mxz_variant > 5) throw Error(KHS_ERR_CONFIG, "mxz_variant must be -1..5");
+    if (c.mxz_variant < -1 || c.mxz_variant > 8) throw Error(KHS_ERR_CONFIG, "mxz_variant must be -1..8");
     if (c.packed_shape < -1 || c.packed_shape > 7) throw Error(KHS_ERR_CONFIG, "packed_shape must be -1..7");
     if (c.run_mode < 0 || c.run_mode > 6)
         throw Error(KHS_ERR_CONFIG, "run_mode must be 0..6");

@@ -634,6 +634,9 @@ static std::unique_ptr<Ctx> make_ctx(const khs_config& in) {
     kp.dtdz = c.dt / c.dz;
     kp.c2 = c.c * c.c;
     kp.eps0 = c.eps_0;
+    kp.rdx = 1.0 / c.dx;
+    kp.rdz = 1.0 / c.dz;
+    kp.reps0 = 1.0 / c.eps_0;
     kp.alpha = c.alpha;
     kp.warm_T = (double)c.warm_T;
     kp.q0 = c.charge1;

@@ -31,10 +31,47 @@ struct KParams {
     double dt, dx, dz;
     double dtdx, dtdz;        // dt/dx, dt/dz (the reference evaluates dt_step/dx per call; same value)
     double c2, eps0, alpha, warm_T;
+    double rdx, rdz, reps0;   // 1/dx, 1/dz, 1/eps0 rounded to nearest (CDiv)
     double q0, q1;            // EMHD species charges (emhdgpu2d fluid: -1)
     int nsp;                  // EMHD: 1 or 2 fluids (YM: always 2)
     int rev;                  // walk the launch's tiles in reverse logical order
 };
+
+// Division by a launch-wide constant b > 0 without a division sequence
+// (FP64): y = RN(1/b) is computed on the host, q = RN(a y), then two
+// corrections q = RN(q + r y) with the remainder r = a - b q, exact in one
+// fma.  After the first correction q is within one ulp of a / b; with y
+// within half an ulp of 1/b the second gives RN(a / b) (Markstein's theorem,
+// IBM J. Res. Dev. 34(1), 1990): the bits of IEEE division, zeros of either
+// sign included (the remainder is formed as b q - a and subtracted), while
+// the remainder stays normal (|a| above ~2^-960).  5 FP64 operations instead
+// of 11 with a quarter-rate reciprocal.  The FP32 form divides: there the
+// remainder of |a| below ~2^-100 is subnormal and the correction can miss by
+// one ulp.
+template <typename T>
+struct CDiv {
+    T b, y;
+    __device__ __forceinline__ T operator()(T a) const {
+        if constexpr (sizeof(T) == 4) {
+            return a / b;
+        } else {
+            T q = a * y;
+            T rn = __builtin_fma(b, q, -a);
+            q = __builtin_fma(-rn, y, q);
+            rn = __builtin_fma(b, q, -a);
+            return __builtin_fma(-rn, y, q);
+        }
+    }
+};
+template <typename T> __device__ __forceinline__ CDiv<T> cdiv_dx(const KParams& p);
+template <> __device__ __forceinline__ CDiv<double> cdiv_dx<double>(const KParams& p) { return {p.dx, p.rdx}; }
+template <> __device__ __forceinline__ CDiv<float> cdiv_dx<float>(const KParams& p) { return {float(p.dx), 0.0f}; }
+template <typename T> __device__ __forceinline__ CDiv<T> cdiv_dz(const KParams& p);
+template <> __device__ __forceinline__ CDiv<double> cdiv_dz<double>(const KParams& p) { return {p.dz, p.rdz}; }
+template <> __device__ __forceinline__ CDiv<float> cdiv_dz<float>(const KParams& p) { return {float(p.dz), 0.0f}; }
+template <typename T> __device__ __forceinline__ CDiv<T> cdiv_eps0(const KParams& p);
+template <> __device__ __forceinline__ CDiv<double> cdiv_eps0<double>(const KParams& p) { return {p.eps0, p.reps0}; }
+template <> __device__ __forceinline__ CDiv<float> cdiv_eps0<float>(const KParams& p) { return {float(p.eps0), 0.0f}; }
 
 // --------------------------------------------------------------------------
 // index helpers

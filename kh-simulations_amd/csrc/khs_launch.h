@@ -63,8 +63,10 @@ void launch_maxwell(const State<T>& S, const State<T>& O, const KParams& p, cons
 // charges it reads) in one launch, YM model without column filters.  I:
 // fields of step n, fluid after the sweeps of step n; O: fields of step
 // n + 1, fluid after the sweeps of step n + 1.  Variants 0/1/2 = 16/32/64-row
-// chunks with two Maxwell wavefronts per workgroup, 3/4/5 with four.
-constexpr int kNumMxzVariants = 6;
+// chunks with two Maxwell wavefronts per workgroup, 3/4/5 with four, 6/7 with
+// four and chunks sized so that the tiles fill the device's resident
+// workgroups once / twice, 8 with two and one round.
+constexpr int kNumMxzVariants = 9;
 template <typename T>
 void launch_maxwell_sweep(const State<T>& I, const State<T>& O, const KParams& p, const double* decay,
                           const int* action, int variant, hipStream_t st);

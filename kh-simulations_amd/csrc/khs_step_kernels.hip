@@ -19,6 +19,8 @@
 //
 // Reference semantics: ymgpu2d/FCT_Sweeps.cu:39-177, YM_FCT.cu:30-135,
 // YM_Fluid.cu:3-108, YM_Physics.cu:9-151,589-599, main_ym.cu:322-363.
+#include <algorithm>
+#include <atomic>
 #include <type_traits>
 
 #include "khs_device.h"
@@ -1141,8 +1143,8 @@ __device__ __forceinline__ void maxwell_zm_wave(const State<T>& S, const State<T
     const bool store = lane < MZM_OUT && x < nx;
     const int z0 = (wl / nwx) * TZ;
     const int z1 = min(z0 + TZ, nz);
-    const T dt = T(p.dt), c2 = T(p.c2), aYM = T(p.alpha), eps0 = T(p.eps0);
-    const T dx = T(p.dx), dz = T(p.dz);
+    const T dt = T(p.dt), c2 = T(p.c2), aYM = T(p.alpha);
+    const CDiv<T> ddx = cdiv_dx<T>(p), ddz = cdiv_dz<T>(p), deps0 = cdiv_eps0<T>(p);
     const int act = action ? action[xc] : 0;
     const T dec = (act == 1) ? T(decay[xc]) : T(1.0);
     // halo lane 63 and lanes past the last column store through range-checked
@@ -1203,8 +1205,8 @@ __device__ __forceinline__ void maxwell_zm_wave(const State<T>& S, const State<T
                              cur.az[0] * cur.by[1] - cur.az[1] * cur.by[0]};
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                T dBdz = (cur.by[c] - byp[c]) / dz;
-                exn[c] = cur.ex[c] + dt * (c2 * (-dBdz - aYM * cr[c]) - Jx[c] / eps0);
+                T dBdz = ddz(cur.by[c] - byp[c]);
+                exn[c] = cur.ex[c] + dt * (c2 * (-dBdz - aYM * cr[c]) - deps0(Jx[c]));
             }
         }
         if (e > z0) {
@@ -1221,9 +1223,9 @@ __device__ __forceinline__ void maxwell_zm_wave(const State<T>& S, const State<T
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     T Jz = -(nQz[0][c] * vzt[0] + nQz[1][c] * vzt[1]);
-                    T dBdx = (byp[c] - byxmp[c]) / dx;
+                    T dBdx = ddx(byp[c] - byxmp[c]);
                     T ez = wall ? T(0.0) : ezp[c];
-                    ezn[c] = ez + dt * (c2 * dBdx - Jz / eps0);
+                    ezn[c] = ez + dt * (c2 * dBdx - deps0(Jz));
                 }
             }
             T ezr[3];
@@ -1246,8 +1248,8 @@ __device__ __forceinline__ void maxwell_zm_wave(const State<T>& S, const State<T
                 const unsigned off = store ? (rbase(e - 1) + (unsigned)xc) * (unsigned)sizeof(T) : kNoStore;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    T dEx = (exn[c] - exn_p[c]) / dz;
-                    T dEz = (ezr[c] - ezn[c]) / dx;
+                    T dEx = ddz(exn[c] - exn_p[c]);
+                    T dEz = ddx(ezr[c] - ezn[c]);
                     T byn = byp[c] - dt * (dEx - dEz - aYM * cr[c]);
                     T exo = exn_p[c], ezo = ezn[c], azo = Az[c];
                     if (c > 0 && act == 1) { byn *= dec; exo *= dec; ezo *= dec; azo *= dec; }
@@ -2954,7 +2956,8 @@ __device__ __forceinline__ void mxz_cell(const MRow<T>& m, T (&ux)[2], T (&uz)[2
 template <typename T>
 __device__ __forceinline__ void mxz_ampere_ex(const MRow<T>& m, const T (&byzm)[3], const T (&ux)[2],
                                               const T (&nq)[2][3], const KParams& p, T (&exn)[3]) {
-    const T dt = T(p.dt), c2 = T(p.c2), aYM = T(p.alpha), eps0 = T(p.eps0), dz = T(p.dz);
+    const T dt = T(p.dt), c2 = T(p.c2), aYM = T(p.alpha);
+    const CDiv<T> ddz = cdiv_dz<T>(p), deps0 = cdiv_eps0<T>(p);
     T Jx[3];
     {
         T vxr[2], nQx[2][3];
@@ -2971,14 +2974,14 @@ __device__ __forceinline__ void mxz_ampere_ex(const MRow<T>& m, const T (&byzm)[
                      m.az[0] * m.by[1] - m.az[1] * m.by[0]};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        T dBdz = (m.by[c] - byzm[c]) / dz;
-        exn[c] = m.ex[c] + dt * (c2 * (-dBdz - aYM * cr[c]) - Jx[c] / eps0);
+        T dBdz = ddz(m.by[c] - byzm[c]);
+        exn[c] = m.ex[c] + dt * (c2 * (-dBdz - aYM * cr[c]) - deps0(Jx[c]));
     }
 }
 
-template <typename T, int TZ, int M>
+template <typename T, int M>
 __device__ __forceinline__ void mxz_maxwell(const State<T>& I, const State<T>& O, const KParams& p, int g, int ch,
-                                            int lane, int m, MxzShared<T>& sh, const double* decay,
+                                            int tz, int lane, int m, MxzShared<T>& sh, const double* decay,
                                             const int* action) {
     const int nx = p.nx, nz = p.nz;
     const bool per = p.periodic_x != 0;
@@ -2988,13 +2991,13 @@ __device__ __forceinline__ void mxz_maxwell(const State<T>& I, const State<T>& O
     const bool wall = !per && (xc == 0 || xc == nx - 1);     // YM_Fluid.cu:23-26
     const bool last_wall = !per && xc == nx - 1;              // Faraday: E(x+1) := E(x)
     const bool store = lane >= 3 && lane < 3 + XZR_OUT && x >= 0 && x < nx;
-    const int z0 = ch * TZ;
-    const int zend = min(z0 + TZ, nz);
+    const int z0 = ch * tz;
+    const int zend = min(z0 + tz, nz);
     const int elast = zend + 2;
     const int rc0 = clampi(z0 - 3, 0, nz - 1);
     const int rf = zm_per(rc0, nz);
-    const T dt = T(p.dt), c2 = T(p.c2), aYM = T(p.alpha), eps0 = T(p.eps0);
-    const T dx = T(p.dx), dz = T(p.dz);
+    const T dt = T(p.dt), c2 = T(p.c2), aYM = T(p.alpha);
+    const CDiv<T> ddx = cdiv_dx<T>(p), ddz = cdiv_dz<T>(p), deps0 = cdiv_eps0<T>(p);
     const int act = action ? action[xc] : 0;
     const T dec = (act == 1) ? T(decay[xc]) : T(1.0);
     const unsigned bytes = (unsigned)nx * (unsigned)nz * (unsigned)sizeof(T);
@@ -3038,9 +3041,9 @@ __device__ __forceinline__ void mxz_maxwell(const State<T>& I, const State<T>& O
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 T Jz = -(nQz[0][c] * vzt[0] + nQz[1][c] * vzt[1]);
-                T dBdx = (a.by[c] - a.byxm[c]) / dx;
+                T dBdx = ddx(a.by[c] - a.byxm[c]);
                 T ez = wall ? T(0.0) : a.ez[c];
-                ezn[c] = ez + dt * (c2 * dBdx - Jz / eps0);
+                ezn[c] = ez + dt * (c2 * dBdx - deps0(Jz));
             }
         }
         T ezr[3];
@@ -3067,8 +3070,8 @@ __device__ __forceinline__ void mxz_maxwell(const State<T>& I, const State<T>& O
         const int slot = k % MXZ_SLOTS;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            T dEx = (exb[c] - exa[c]) / dz;
-            T dEz = (ezr[c] - ezn[c]) / dx;
+            T dEx = ddz(exb[c] - exa[c]);
+            T dEz = ddx(ezr[c] - ezn[c]);
             T byn = a.by[c] - dt * (dEx - dEz - aYM * cr[c]);
             T exo = exa[c], ezo = ezn[c], azo = Az[c];
             // sponge / xi_cut of the colour-2/3 fields (YM_Physics.cu:161-210)
@@ -3093,9 +3096,9 @@ __device__ __forceinline__ void mxz_maxwell(const State<T>& I, const State<T>& O
 // a wall, which take their clamped column's lane (so their sources are
 // lorentz_px / lorentz_pz / precession at that column, as the reference's
 // clamped halo loads give).
-template <typename T, int TZ, typename GR, bool EDGE>
+template <typename T, typename GR, bool EDGE>
 __device__ __forceinline__ void mxz_sweep(const State<T>& I, const State<T>& O, const KParams& p, int sp, int g,
-                                          int ch, int lane, MxzShared<T>& sh, int w, const double* decay,
+                                          int ch, int tz, int lane, MxzShared<T>& sh, int w, const double* decay,
                                           const int* action) {
     const MxzRing<T>& ring = sh.ring;
     typedef YMModel M;
@@ -3114,8 +3117,8 @@ __device__ __forceinline__ void mxz_sweep(const State<T>& I, const State<T>& O, 
     const bool wall = !p.periodic_x && (xc == 0 || xc == nx - 1);
     const int b = M::base(sp);
     const T dt = T(p.dt), dtdx = T(p.dtdx), dtdz = T(p.dtdz);
-    const int z0 = ch * TZ;
-    const int zend = min(z0 + TZ, nz);
+    const int z0 = ch * tz;
+    const int zend = min(z0 + tz, nz);
     const int elast = zend + 2;
     const int rc0 = clampi(z0 - 3, 0, nz - 1);
     // sponge / xi_cut of the charges Q2, Q3 (the filter stage of step n)
@@ -3275,12 +3278,13 @@ __device__ __forceinline__ void mxz_sweep(const State<T>& I, const State<T>& O, 
     for (; e <= elast; ++e) row(e, std::integral_constant<int, 0>());
 }
 
-// Workgroup: waves 0..3 sweep (species w / 2, group w % 2), wave 4 Maxwell.
-// Occupancy: two waves per SIMD in FP64, four in FP32; 4 + M waves per
-// workgroup (M = 4: the eight wave slots of a CU in FP64).
-template <typename T, int TZ, int M>
+// Workgroup: waves 0..3 sweep (species w / 2, group w % 2), waves 4.. Maxwell;
+// tile = 58 columns x a chunk of tz rows.  Occupancy: two waves per SIMD in
+// FP64, four in FP32; 4 + M waves per workgroup (M = 4: the eight wave slots
+// of a CU in FP64).
+template <typename T, int M>
 __global__ __launch_bounds__(64 * (4 + M)) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2))) void k_mxz(
-    State<T> I, State<T> O, KParams p, int ngx, int nch, const double* decay, const int* action) {
+    State<T> I, State<T> O, KParams p, int ngx, int nch, int tz, const double* decay, const int* action) {
     __shared__ MxzShared<T> sh;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -3292,40 +3296,74 @@ __global__ __launch_bounds__(64 * (4 + M)) __attribute__((amdgpu_waves_per_eu(si
     if (threadIdx.x < 4) sh.released[threadIdx.x] = 0;
     __syncthreads();                            // the only barrier: the counters are set
     if (w >= 4) {
-        mxz_maxwell<T, TZ, M>(I, O, p, g, ch, lane, w - 4, sh, decay, action);
+        mxz_maxwell<T, M>(I, O, p, g, ch, tz, lane, w - 4, sh, decay, action);
         return;
     }
     const int sp = w >> 1;
     const bool edge = g == 0 || g * XZR_OUT + 60 >= p.nx;   // lanes cover columns 58g-3 .. 58g+60
     if (w & 1) {
-        if (edge) mxz_sweep<T, TZ, XzQ3, true>(I, O, p, sp, g, ch, lane, sh, w, decay, action);
-        else mxz_sweep<T, TZ, XzQ3, false>(I, O, p, sp, g, ch, lane, sh, w, decay, action);
+        if (edge) mxz_sweep<T, XzQ3, true>(I, O, p, sp, g, ch, tz, lane, sh, w, decay, action);
+        else mxz_sweep<T, XzQ3, false>(I, O, p, sp, g, ch, tz, lane, sh, w, decay, action);
     } else {
-        if (edge) mxz_sweep<T, TZ, XzN3, true>(I, O, p, sp, g, ch, lane, sh, w, decay, action);
-        else mxz_sweep<T, TZ, XzN3, false>(I, O, p, sp, g, ch, lane, sh, w, decay, action);
+        if (edge) mxz_sweep<T, XzN3, true>(I, O, p, sp, g, ch, tz, lane, sh, w, decay, action);
+        else mxz_sweep<T, XzN3, false>(I, O, p, sp, g, ch, tz, lane, sh, w, decay, action);
     }
 }
 
-template <typename T, int TZ, int M>
-static void launch_mxz_t(const State<T>& I, const State<T>& O, const KParams& p, const double* decay,
+// Workgroups of k_mxz<T, M> the current device holds at once (its compute
+// units times the workgroups per unit that the kernel's registers and LDS
+// allow), queried once per device.
+template <typename T, int M>
+static int mxz_resident() {
+    static std::atomic<int> cache[16];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) dev = 0;
+    int r = cache[dev].load(std::memory_order_relaxed);
+    if (r == 0) {
+        int per_cu = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mxz<T, M>, 64 * (4 + M), 0) != hipSuccess ||
+            per_cu < 1)
+            per_cu = 1;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;
+        r = per_cu * cus;
+        cache[dev].store(r, std::memory_order_relaxed);
+    }
+    return r;
+}
+
+// tz > 0: chunks of tz rows; tz = -R: the chunk height that makes the tiles
+// fill the device's resident workgroups R times (at least 16 rows), so that
+// the launch runs in R rounds with long chunks instead of many rounds of
+// short ones and a partial last round.
+template <typename T, int M>
+static void launch_mxz_t(const State<T>& I, const State<T>& O, const KParams& p, int tz, const double* decay,
                          const int* action, hipStream_t st) {
     const int ngx = (p.nx + XZR_OUT - 1) / XZR_OUT;
-    const int nch = (p.nz + TZ - 1) / TZ;
-    hipLaunchKernelGGL((k_mxz<T, TZ, M>), dim3(round8(ngx * nch)), dim3(64 * (4 + M)), 0, st, I, O, p, ngx, nch,
+    if (tz <= 0) {
+        const int nch = std::max(1, (-tz * mxz_resident<T, M>()) / ngx);
+        tz = std::max(16, (p.nz + nch - 1) / nch);
+    }
+    const int nch = (p.nz + tz - 1) / tz;
+    hipLaunchKernelGGL((k_mxz<T, M>), dim3(round8(ngx * nch)), dim3(64 * (4 + M)), 0, st, I, O, p, ngx, nch, tz,
                        decay, action);
 }
 
-// variants 0/1/2: 16/32/64-row chunks with two Maxwell waves, 3/4/5 with four
+// variants 0/1/2: 16/32/64-row chunks with two Maxwell waves, 3/4/5 with
+// four; 6/7: four Maxwell waves, chunks sized for one / two rounds of
+// resident workgroups; 8: two Maxwell waves, one round
 template <typename T>
 void launch_maxwell_sweep(const State<T>& I, const State<T>& O, const KParams& p, const double* decay,
                           const int* action, int variant, hipStream_t st) {
     switch (variant) {
-        case 0: launch_mxz_t<T, 16, 2>(I, O, p, decay, action, st); return;
-        case 2: launch_mxz_t<T, 64, 2>(I, O, p, decay, action, st); return;
-        case 3: launch_mxz_t<T, 16, 4>(I, O, p, decay, action, st); return;
-        case 4: launch_mxz_t<T, 32, 4>(I, O, p, decay, action, st); return;
-        case 5: launch_mxz_t<T, 64, 4>(I, O, p, decay, action, st); return;
-        default: launch_mxz_t<T, 32, 2>(I, O, p, decay, action, st); return;
+        case 0: launch_mxz_t<T, 2>(I, O, p, 16, decay, action, st); return;
+        case 2: launch_mxz_t<T, 2>(I, O, p, 64, decay, action, st); return;
+        case 3: launch_mxz_t<T, 4>(I, O, p, 16, decay, action, st); return;
+        case 4: launch_mxz_t<T, 4>(I, O, p, 32, decay, action, st); return;
+        case 5: launch_mxz_t<T, 4>(I, O, p, 64, decay, action, st); return;
+        case 6: launch_mxz_t<T, 4>(I, O, p, -1, decay, action, st); return;
+        case 7: launch_mxz_t<T, 4>(I, O, p, -2, decay, action, st); return;
+        case 8: launch_mxz_t<T, 2>(I, O, p, -1, decay, action, st); return;
+        default: launch_mxz_t<T, 2>(I, O, p, 32, decay, action, st); return;
     }
 }
 

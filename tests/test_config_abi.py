@@ -90,7 +90,7 @@ def test_output_dir_name_c1():
     (dict(fused_sweep=2), "fused_sweep"),
     (dict(xz_variant=13), "xz_variant"),
     (dict(step_fusion=2), "step_fusion"),
-    (dict(mxz_variant=6), "mxz_variant"),
+    (dict(mxz_variant=9), "mxz_variant"),
     (dict(packed_shape=8), "packed_shape"),
     (dict(maxwell_kernel=11), "maxwell_kernel"),
     (dict(nx_override=32768, nz_override=16384), "grid too large"),

@@ -425,7 +425,7 @@ MXZ_CASES = {
 }
 
 
-@pytest.mark.parametrize("v", list(range(6)))
+@pytest.mark.parametrize("v", list(range(9)))
 @pytest.mark.parametrize("case", list(MXZ_CASES))
 def test_maxwell_sweep_launch_bitwise(case, v):
     """Maxwell of one step and the sweeps of the next in one launch (every

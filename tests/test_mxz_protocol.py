@@ -76,7 +76,8 @@ def run(z0, tz, nz, nmax, seed):
 @pytest.mark.parametrize("nz", [4, 5, 7, 16, 33, 100])
 @pytest.mark.parametrize("nmax", [1, 2, 4])
 def test_progress_counters_end_and_protect_rows(nz, nmax):
-    for tz in (16, 32, 64):
+    # fixed chunk heights and heights sized at run time (any height >= 16)
+    for tz in (16, 32, 64, 37, 147):
         for z0 in range(0, nz, tz):
             for seed in range(20):
                 run(z0, tz, nz, nmax, seed)
