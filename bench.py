@@ -46,6 +46,8 @@ sys.path.insert(0, os.path.join(REPO, "kh-simulations_amd"))
 
 METRIC = "cell-updates/s/GPU FP32 vs FP64 at ≤1e-3 growth-rate err; sweep runs/hour"
 HBM_PEAK_GBS = 8000.0
+# a plain 256 MiB device-to-device copy on MI355X (bench.py --calibrate-copy 256)
+COPY_GBS = 5300.0
 
 WORKLOAD = dict(nx_override=2048, nz_override=1024, lx_override=204.8, lz_override=102.4, alpha_YM=1,
                 V0=0.05, eps_override=1.0, courant_override=0.4, k_mode=1, run_mode=0, freeze_override=1,
@@ -95,6 +97,11 @@ ZM_ROWS = {1: 16, 2: 32, 3: 64}       # chunk height of the fused z sweep + Maxw
 # profiles/fused_sweep_decomposition_2048x1024_fp64.md), keyed by (slot,
 # precision, nx, nz).
 LIMITER = {
+    ("k_mxz", "fp64", 2048, 1024):
+        "issue and dependency latency of the sweep wavefronts' row march: two FP64 waves per SIMD (one sweep, one "
+        "Maxwell wave, 176 VGPRs), SIMD VALU busy 0.46 of the launch, each wave waiting on memory counters for 0.36 "
+        "of its lifetime (profiles/sq_counters_2048x1024_fp64.md); HBM traffic 1.15x algorithmic at ~3.5 TB/s, "
+        "under the ~5.3 TB/s of a plain device copy",
     ("k_fct_xz", "fp64", 2048, 1024):
         "memory-level parallelism: ~13,700 HBM reads (~1.5 MB) in flight at a 0.63 us read latency, a third of "
         "what 8 TB/s needs (profiles/latency_icache_2048x1024_fp64.md), because each wave holds one row of loads "
@@ -112,6 +119,13 @@ LIMITER = {
         "latency: 640 waves of 10 rows each on a 256x128 grid (under a third of the wave slots at 2 per SIMD), each row "
         "waiting on the loads issued one row earlier",
 }
+
+
+# launch shapes of k_mxz (one kernel name per Maxwell wave count: the chunk
+# height is a run-time argument); the committed traffic tables label them so
+MXZ_SHAPES = {0: "v0, 16-row chunks", 1: "v1, 32-row chunks", 2: "v2, 64-row chunks", 3: "v3, 16-row chunks",
+              4: "v4, 32-row chunks", 5: "v5, 64-row chunks", 6: "v6, one round", 7: "v7, two rounds",
+              8: "v8, one round"}
 
 
 def kernel_instance(slot, t, T="double"):
@@ -141,7 +155,8 @@ def kernel_instance(slot, t, T="double"):
         return f"k_maxwell_zm<{T}, {tzpf[0]}, {tzpf[1]}>" if tzpf else None
     if slot == "k_mxz":
         v = t.get("mxz_variant", 1)
-        return f"k_mxz<{T}, {(2, 2, 2, 4, 4, 4, 4, 4, 2)[v]}>"
+        name = f"k_mxz<{T}, {(2, 2, 2, 4, 4, 4, 4, 4, 2)[v]}>"
+        return name if v == 6 else f"{name} {MXZ_SHAPES[v]}"
     if slot == "k_maxwell":
         return {0: f"k_maxwell<{T}, 16>", 1: f"k_maxwell_cell<{T}>", 2: f"k_maxwell<{T}, 4>",
                 3: f"k_maxwell<{T}, 8>"}.get(t["maxwell"])
@@ -162,7 +177,7 @@ def pmc_traffic(nx, nz, precision):
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
             if len(cells) >= 4 and cells[0].startswith("`"):
                 try:
-                    out[cells[0].strip("`")] = float(cells[3]) * 1e6
+                    out[cells[0].replace("`", "")] = float(cells[3]) * 1e6
                 except ValueError:
                     pass
     return out
@@ -225,6 +240,11 @@ def kernel_roofline(r, cfg, precision):
             "traffic_source": (f"profiles/hbm_traffic_{cfg.nx}x{cfg.nz}_{precision}.md"
                                if d.get("pmc_traffic_bytes") else None),
             "limiter": LIMITER.get((dom, precision, cfg.nx, cfg.nz))}
+    # what holds the kernel below the roofline: its measured HBM traffic rate
+    # against what a plain copy reaches ("hbm" within 15% of it), else the
+    # issue / memory latency its counters show
+    tg = d.get("pmc_traffic_GBs")
+    roof["limited_by"] = None if tg is None else ("hbm" if tg >= 0.85 * COPY_GBS else "latency")
     return kern, roof
 
 
