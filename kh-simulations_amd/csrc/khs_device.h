@@ -240,11 +240,9 @@ __device__ __forceinline__ T lorentz_px(const State<T>& S, int sp, int x, int z,
 // x momentum and charges of the cell, Ez at (x, z) and (x, z-1), By at
 // (x, z), (x-1, z), (x, z-1), (x-1, z-1).  Cold part only.
 template <typename T>
-__device__ __forceinline__ T lorentz_pz_v(T n, T px, const T (&q)[3], const T (&ez)[3], const T (&ezm)[3],
+__device__ __forceinline__ T lorentz_pz_u(T vx, const T (&q)[3], const T (&ez)[3], const T (&ezm)[3],
                                           const T (&by)[3], const T (&byxm)[3], const T (&byzm)[3],
                                           const T (&byxmzm)[3]) {
-    T n_safe = kmax(n, T(0.05));
-    T vx = px / n_safe;
     T acc[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -253,6 +251,12 @@ __device__ __forceinline__ T lorentz_pz_v(T n, T px, const T (&q)[3], const T (&
         acc[c] = q[c] * (Ezn - vx * Byn);
     }
     return -(acc[0] + acc[1] + acc[2]);
+}
+template <typename T>
+__device__ __forceinline__ T lorentz_pz_v(T n, T px, const T (&q)[3], const T (&ez)[3], const T (&ezm)[3],
+                                          const T (&by)[3], const T (&byxm)[3], const T (&byzm)[3],
+                                          const T (&byxmzm)[3]) {
+    return lorentz_pz_u(px / kmax(n, T(0.05)), q, ez, ezm, by, byxm, byzm, byxmzm);
 }
 
 // Warm-closure pressure term of the z source: -T dn/dz (centred, periodic z).
@@ -293,16 +297,19 @@ __device__ __forceinline__ T lorentz_pz(const State<T>& S, int sp, int x, int z,
 // kernel_ym_precession (YM_Fluid.cu:86-108) from values: density, z momentum
 // and charges of the cell, Az at (x, z) and (x, z-1); out[0..2] = sQ^a.
 template <typename T>
-__device__ __forceinline__ void precession_v(T n, T pz, const T (&q)[3], const T (&az)[3], const T (&azm)[3],
-                                             T alpha, T out[3]) {
-    T n_safe = kmax(n, T(0.05));
-    T vz = pz / n_safe;
+__device__ __forceinline__ void precession_u(T vz, const T (&q)[3], const T (&az)[3], const T (&azm)[3], T alpha,
+                                             T out[3]) {
     T a1 = T(0.5) * (az[0] + azm[0]);
     T a2 = T(0.5) * (az[1] + azm[1]);
     T a3 = T(0.5) * (az[2] + azm[2]);
     out[0] = alpha * vz * (q[1] * a3 - q[2] * a2);
     out[1] = alpha * vz * (q[2] * a1 - q[0] * a3);
     out[2] = alpha * vz * (q[0] * a2 - q[1] * a1);
+}
+template <typename T>
+__device__ __forceinline__ void precession_v(T n, T pz, const T (&q)[3], const T (&az)[3], const T (&azm)[3],
+                                             T alpha, T out[3]) {
+    precession_u(pz / kmax(n, T(0.05)), q, az, azm, alpha, out);
 }
 
 // kernel_ym_precession at real cell (x, z).

@@ -2894,6 +2894,9 @@ constexpr int MXZ_SLOTS = 8;
 template <typename T>
 struct MxzShared {
     T ring[MXZ_SLOTS][12][64];
+    // per species, the sweep's cell velocities p / max(n, 0.05) (x, z) and
+    // x-interface coefficients eps, nu, mu of the ring row's fluid
+    T vel[MXZ_SLOTS][2][5][64];
     int ready[MXZ_SLOTS];
     int released[4];
 };
@@ -3086,6 +3089,26 @@ __device__ __forceinline__ void mxz_maxwell(const State<T>& I, const State<T>& O
             sh.ring[slot][S_BY1 + c][lane] = byn;
             sh.ring[slot][S_AZ1 + c][lane] = azo;
         }
+        // the sweep's velocities and x-interface coefficients of row q
+        // (update_velocities, YM_Fluid.cu:3-11: p / max(n, 0.05); the
+        // currents' quotients above floor n at 1e-10, the same values for
+        // n >= 0.05, which every density after a sweep is)
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const bool low = a.n[s] < T(0.05);
+            const T u = low ? a.px[s] / T(0.05) : uxa[s];
+            const T uz = low ? a.pz[s] / T(0.05) : uza[s];
+            const T ur = lane_from_right(u);
+            const T vi = T(0.5) * (u + ur);
+            const T eps = vi * T(p.dtdx);
+            T nu, mu;
+            fct_nu_mu(eps, nu, mu);
+            sh.vel[slot][s][0][lane] = u;
+            sh.vel[slot][s][1][lane] = uz;
+            sh.vel[slot][s][2][lane] = eps;
+            sh.vel[slot][s][3][lane] = nu;
+            sh.vel[slot][s][4][lane] = mu;
+        }
         mxz_publish(&sh.ready[slot], k + 1);
     }
 }
@@ -3116,7 +3139,7 @@ __device__ __forceinline__ void mxz_sweep(const State<T>& I, const State<T>& O, 
     lxm = lxm < 0 ? 0 : lxm;                    // lane 0's x-1 values are not used
     const bool wall = !p.periodic_x && (xc == 0 || xc == nx - 1);
     const int b = M::base(sp);
-    const T dt = T(p.dt), dtdx = T(p.dtdx), dtdz = T(p.dtdz);
+    const T dt = T(p.dt), dtdz = T(p.dtdz);
     const int z0 = ch * tz;
     const int zend = min(z0 + tz, nz);
     const int elast = zend + 2;
@@ -3161,12 +3184,13 @@ __device__ __forceinline__ void mxz_sweep(const State<T>& I, const State<T>& O, 
         mxz_wait_rows(sh.ready, kc);
         const T(&cs)[12][64] = ring[kc % MXZ_SLOTS];
         const T(&ps)[12][64] = ring[(kc + MXZ_SLOTS - 1) % MXZ_SLOTS];
+        const T(&vs)[5][64] = sh.vel[kc % MXZ_SLOTS][sp];
+        const T u_x = vs[0][lane], u_z = vs[1][lane];
 
         // ---- x sweep of row rc at this lane's column (YM_FCT.cu:30-91) ----
         T src = T(0.0);
         if (GR::has(F_PX)) {                   // px source (kernel_ym_lorentz, YM_Fluid.cu:32-84)
-            const T n_safe = kmax(F[F_N], T(0.05));
-            const T vz = F[F_PZ] / n_safe;
+            const T vz = u_z;
             T acc[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
@@ -3178,12 +3202,8 @@ __device__ __forceinline__ void mxz_sweep(const State<T>& I, const State<T>& O, 
         }
         T V[NG];
         {
-            const T u = F[F_PX] / kmax(F[F_N], T(0.05));          // update_velocities
-            const T ur = lane_from_right(u);
-            T vi = T(0.5) * (u + ur);
-            const T eps = vi * dtdx;
-            T nu, mu;
-            fct_nu_mu(eps, nu, mu);
+            // velocity and interface coefficients from the Maxwell wave
+            const T eps = vs[2][lane], nu = vs[3][lane], mu = vs[4][lane];
 #pragma unroll
             for (int j = 0; j < NG; ++j) {
                 const int k = VB + j;
@@ -3209,7 +3229,7 @@ __device__ __forceinline__ void mxz_sweep(const State<T>& I, const State<T>& O, 
 
         // ---- z sweep of the x-swept row: stale z velocity and the z sources ----
         T Sv[NG], out[NG];
-        const T uz = F[F_PZ] / kmax(F[F_N], T(0.05));
+        const T uz = u_z;
         {
             const T q[3] = {F[F_Q1], F[F_Q2], F[F_Q3]};
             T spz = T(0), sq[3] = {T(0), T(0), T(0)};
@@ -3224,7 +3244,7 @@ __device__ __forceinline__ void mxz_sweep(const State<T>& I, const State<T>& O, 
                     byzm[c] = ps[S_BY1 + c][lx];
                     byxmzm[c] = ps[S_BY1 + c][lxm];
                 }
-                spz = lorentz_pz_v(F[F_N], F[F_PX], q, ez, ezm, by, byxm, byzm, byxmzm);
+                spz = lorentz_pz_u(u_x, q, ez, ezm, by, byxm, byzm, byxmzm);
             }
             if (GR::has(F_Q1)) {
                 T az[3], azm[3];
@@ -3233,7 +3253,7 @@ __device__ __forceinline__ void mxz_sweep(const State<T>& I, const State<T>& O, 
                     az[c] = cs[S_AZ1 + c][lx];
                     azm[c] = ps[S_AZ1 + c][lx];
                 }
-                precession_v(F[F_N], F[F_PZ], q, az, azm, T(p.alpha), sq);
+                precession_u(u_z, q, az, azm, T(p.alpha), sq);
             }
 #pragma unroll
             for (int j = 0; j < NG; ++j) {
