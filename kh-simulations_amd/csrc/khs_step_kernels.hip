@@ -3015,23 +3015,13 @@ __device__ __forceinline__ void mxz_maxwell(const State<T>& I, const State<T>& O
     auto rbase = [&](int r) { return (unsigned)((r + nz) % nz) * (unsigned)nx; };
     // rows k = 0 (zm_per(rc0)), 1 (rc0), ..., 1 + clamp(elast) - rc0
     const int nrows = 2 + clampi(elast, 0, nz - 1) - rc0;
-#pragma unroll 1
-    for (int k = m; k < nrows; k += M) {
-        const int q = rf + k;                     // unwrapped grid row of ring row k
-        MRow<T> a, b;                             // rows q, q + 1
-        T bym[3];                                 // By of row q - 1
-        mrow_load(I, I, rbase(q) + (unsigned)xc, rbase(q) + (unsigned)xm, a);
-        mrow_load(I, I, rbase(q + 1) + (unsigned)xc, rbase(q + 1) + (unsigned)xm, b);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) bym[c] = ld32(I.a[S_BY1 + c], rbase(q - 1) + (unsigned)xc);
-        // cell quantities of the currents (kernel_ym_currents, YM_Physics.cu:9-60)
-        T uxa[2], uza[2], nqa[2][3], uxb[2], uzb[2], nqb[2][3];
-        mxz_cell(a, uxa, uza, nqa);
-        mxz_cell(b, uxb, uzb, nqb);
-        T exa[3], exb[3];
-        mxz_ampere_ex(a, bym, uxa, nqa, p, exa);
-        mxz_ampere_ex(b, a.by, uxb, nqb, p, exb);
-        // row q: Jz at (x, q + 1/2), Ampere for Ez, potential, Faraday
+    // Finish ring row k (grid row q = rf + k) from its inputs a, the cell
+    // quantities of rows q and q + 1 and Ampere's new Ex at both: Jz at
+    // (x, q + 1/2), Ampere for Ez, potential, Faraday, the column action,
+    // stores, the ring row and the sweep's velocities.
+    auto finish = [&](int k, const MRow<T>& a, const T (&uxa)[2], const T (&uza)[2], const T (&nqa)[2][3],
+                      const T (&uzb)[2], const T (&nqb)[2][3], const T (&exa)[3], const T (&exb)[3]) {
+        const int q = rf + k;
         T ezn[3];
         {
             T vzt[2], nQz[2][3];
@@ -3110,6 +3100,41 @@ __device__ __forceinline__ void mxz_maxwell(const State<T>& I, const State<T>& O
             sh.vel[slot][s][4][lane] = mu;
         }
         mxz_publish(&sh.ready[slot], k + 1);
+    };
+
+    // Tasks of TR ring rows, dealt round-robin.  FP64: two rows k, k + 1
+    // (rows q, q + 1), from loads of rows q - 1 (By) .. q + 2, with Ampere's
+    // Ex at q, q + 1, q + 2 (the next task evaluates Ex at its first row
+    // q + 2 again: the same operations on the same inputs, the same bits);
+    // 2048x1024: 257 -> 247 us.  FP32: one row (two rows' registers spill at
+    // four waves per SIMD: 152 -> 161 us).
+    constexpr int TR = sizeof(T) == 8 ? 2 : 1;
+#pragma unroll 1
+    for (int k = TR * m; k < nrows; k += TR * M) {
+        const int q = rf + k;                     // unwrapped grid row of ring row k
+        const bool two = TR == 2 && k + 1 < nrows;
+        MRow<T> a, b, c;                          // rows q, q + 1, q + 2
+        T bym[3];                                 // By of row q - 1
+        mrow_load(I, I, rbase(q) + (unsigned)xc, rbase(q) + (unsigned)xm, a);
+        mrow_load(I, I, rbase(q + 1) + (unsigned)xc, rbase(q + 1) + (unsigned)xm, b);
+        if (two) mrow_load(I, I, rbase(q + 2) + (unsigned)xc, rbase(q + 2) + (unsigned)xm, c);
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) bym[cc] = ld32(I.a[S_BY1 + cc], rbase(q - 1) + (unsigned)xc);
+        // cell quantities of the currents (kernel_ym_currents, YM_Physics.cu:9-60)
+        T uxa[2], uza[2], nqa[2][3], uxb[2], uzb[2], nqb[2][3];
+        mxz_cell(a, uxa, uza, nqa);
+        mxz_cell(b, uxb, uzb, nqb);
+        T exa[3], exb[3];
+        mxz_ampere_ex(a, bym, uxa, nqa, p, exa);
+        mxz_ampere_ex(b, a.by, uxb, nqb, p, exb);
+        finish(k, a, uxa, uza, nqa, uzb, nqb, exa, exb);
+        if constexpr (TR == 2) {
+            if (!two) continue;
+            T uxc[2], uzc[2], nqc[2][3], exc[3];
+            mxz_cell(c, uxc, uzc, nqc);
+            mxz_ampere_ex(c, b.by, uxc, nqc, p, exc);
+            finish(k + 1, b, uxb, uzb, nqb, uzc, nqc, exb, exc);
+        }
     }
 }
 

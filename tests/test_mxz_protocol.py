@@ -1,6 +1,8 @@
 """Model check of the progress-counter protocol that couples the waves of
 k_mxz (Maxwell + the next step's sweeps in one launch).  M Maxwell waves share
-the ring rows round-robin (wave m: rows m, m + M, ...); for row k a Maxwell
+the ring rows in tasks of one (FP32) or two (FP64) rows dealt round-robin
+(wave m, two-row tasks: rows 2m, 2m + 1, 2m + 2M, 2m + 2M + 1, ...); for
+row k a Maxwell
 wave waits until every sweep wave has released row k - S, writes slot k % S,
 then publishes ready[k % S] = k + 1.  A sweep wave at ring row kc waits for
 ready[kc % S] >= kc + 1 and ready[(kc - 1) % S] >= kc, reads rows kc and
@@ -16,7 +18,7 @@ import pytest
 S = 8       # MXZ_SLOTS
 
 
-def run(z0, tz, nz, nmax, seed):
+def run(z0, tz, nz, nmax, seed, task=2):
     rnd = random.Random(seed)
     zend = min(z0 + tz, nz)
     elast = zend + 2
@@ -27,18 +29,22 @@ def run(z0, tz, nz, nmax, seed):
 
     nrows = 2 + rowof(elast) - rc0
     ready, released, held = [0] * S, [0] * 4, {}
-    pk = list(range(nmax))          # next row of each Maxwell wave
+    # the rows of each Maxwell wave, in its order, and the next one's index
+    prow = [[k for t in range(task * m, nrows, task * nmax) for k in range(t, t + task) if k < nrows]
+            for m in range(nmax)]
+    pk = [0] * nmax
     pstate = [0] * nmax             # 0: before the write, 1: written, not yet published
     ce = [z0 - 3] * 4               # next row e of each sweep wave
     cstate = [0] * 4                # 0: before the read, 1: read, not yet released
     reading = [None] * 4            # ring rows a sweep wave reads (between read and release)
     for _ in range(10 ** 6):
-        actors = [("P", m) for m in range(nmax) if pk[m] < nrows] + [("C", w) for w in range(4) if ce[w] <= elast]
+        actors = [("P", m) for m in range(nmax) if pk[m] < len(prow[m])] + \
+                 [("C", w) for w in range(4) if ce[w] <= elast]
         if not actors:
             return
         kind, i = rnd.choice(actors)
         if kind == "P":
-            k = pk[i]
+            k = prow[i][pk[i]]
             if pstate[i] == 0:
                 if k >= S and min(released) < k - S + 1:
                     continue                               # waits
@@ -54,7 +60,7 @@ def run(z0, tz, nz, nmax, seed):
             else:
                 ready[k % S] = k + 1
                 pstate[i] = 0
-                pk[i] += nmax
+                pk[i] += 1
         else:
             e = ce[i]
             rc = rowof(e)
@@ -75,9 +81,10 @@ def run(z0, tz, nz, nmax, seed):
 
 @pytest.mark.parametrize("nz", [4, 5, 7, 16, 33, 100])
 @pytest.mark.parametrize("nmax", [1, 2, 4])
-def test_progress_counters_end_and_protect_rows(nz, nmax):
+@pytest.mark.parametrize("task", [1, 2])
+def test_progress_counters_end_and_protect_rows(nz, nmax, task):
     # fixed chunk heights and heights sized at run time (any height >= 16)
     for tz in (16, 32, 64, 37, 147):
         for z0 in range(0, nz, tz):
             for seed in range(20):
-                run(z0, tz, nz, nmax, seed)
+                run(z0, tz, nz, nmax, seed, task)
