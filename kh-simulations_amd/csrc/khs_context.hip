@@ -165,7 +165,9 @@ struct Ctx {
         const khs_config& c = cfg;
         const bool column = c.suppress_kz0 || c.kz_suppress_max >= 1 || c.kz_suppress_hi > c.kz_suppress_max + 1 ||
                             c.kz_keep_max > 0 || c.hyp_diff > 0.0f;
-        return !emhd && fused && !column && kp.warm_T <= 0.0;
+        // k_mxz drops the factors c^2 and 1/eps0 (exact only for c = eps0 = 1,
+        // the reference's fixed values)
+        return !emhd && fused && !column && kp.warm_T <= 0.0 && kp.c2 == 1.0 && kp.eps0 == 1.0;
     }
     template <typename T> Stage<T> stg() const {
         Stage<T> s;

@@ -73,6 +73,30 @@ template <typename T> __device__ __forceinline__ CDiv<T> cdiv_eps0(const KParams
 template <> __device__ __forceinline__ CDiv<double> cdiv_eps0<double>(const KParams& p) { return {p.eps0, p.reps0}; }
 template <> __device__ __forceinline__ CDiv<float> cdiv_eps0<float>(const KParams& p) { return {float(p.eps0), 0.0f}; }
 
+// a1 / b and a2 / b with one reciprocal (FP64): the operations of the
+// compiler's IEEE double division on gfx950 (reciprocal estimate v_rcp_f64,
+// two Newton steps, the quotient a y and one fma correction by the exact
+// remainder) with the reciprocal shared, and without that sequence's range
+// scaling and special-case fix-up, which are identities for finite operands
+// whose magnitudes and quotients lie well inside the normal range (momenta
+// over floored densities here).  The remainder is formed as b q - a and
+// subtracted, so a zero numerator keeps its sign.  The FP32 form divides.
+__device__ __forceinline__ void div2(double a1, double a2, double b, double& q1, double& q2) {
+    double y = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    double q = a1 * y;
+    q1 = __builtin_fma(-__builtin_fma(b, q, -a1), y, q);
+    q = a2 * y;
+    q2 = __builtin_fma(-__builtin_fma(b, q, -a2), y, q);
+}
+__device__ __forceinline__ void div2(float a1, float a2, float b, float& q1, float& q2) {
+    q1 = a1 / b;
+    q2 = a2 / b;
+}
+
 // --------------------------------------------------------------------------
 // index helpers
 // --------------------------------------------------------------------------

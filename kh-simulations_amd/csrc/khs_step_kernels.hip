@@ -2947,9 +2947,7 @@ template <typename T>
 __device__ __forceinline__ void mxz_cell(const MRow<T>& m, T (&ux)[2], T (&uz)[2], T (&nq)[2][3]) {
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
-        const T nf = kmax(m.n[s], T(1e-10));
-        ux[s] = m.px[s] / nf;
-        uz[s] = m.pz[s] / nf;
+        div2(m.px[s], m.pz[s], kmax(m.n[s], T(1e-10)), ux[s], uz[s]);
 #pragma unroll
         for (int c = 0; c < 3; ++c) nq[s][c] = m.n[s] * m.q[s][c];
     }
@@ -2959,8 +2957,8 @@ __device__ __forceinline__ void mxz_cell(const MRow<T>& m, T (&ux)[2], T (&uz)[2
 template <typename T>
 __device__ __forceinline__ void mxz_ampere_ex(const MRow<T>& m, const T (&byzm)[3], const T (&ux)[2],
                                               const T (&nq)[2][3], const KParams& p, T (&exn)[3]) {
-    const T dt = T(p.dt), c2 = T(p.c2), aYM = T(p.alpha);
-    const CDiv<T> ddz = cdiv_dz<T>(p), deps0 = cdiv_eps0<T>(p);
+    const T dt = T(p.dt), aYM = T(p.alpha);
+    const CDiv<T> ddz = cdiv_dz<T>(p);
     T Jx[3];
     {
         T vxr[2], nQx[2][3];
@@ -2978,7 +2976,9 @@ __device__ __forceinline__ void mxz_ampere_ex(const MRow<T>& m, const T (&byzm)[
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         T dBdz = ddz(m.by[c] - byzm[c]);
-        exn[c] = m.ex[c] + dt * (c2 * (-dBdz - aYM * cr[c]) - deps0(Jx[c]));
+        // c^2 (-dBdz - a cr) - Jx / eps0 with c = eps0 = 1 (main_ym.cu:69; the
+        // launch is used only then): the product and quotient by 1 are exact
+        exn[c] = m.ex[c] + dt * ((-dBdz - aYM * cr[c]) - Jx[c]);
     }
 }
 
@@ -2999,8 +2999,8 @@ __device__ __forceinline__ void mxz_maxwell(const State<T>& I, const State<T>& O
     const int elast = zend + 2;
     const int rc0 = clampi(z0 - 3, 0, nz - 1);
     const int rf = zm_per(rc0, nz);
-    const T dt = T(p.dt), c2 = T(p.c2), aYM = T(p.alpha);
-    const CDiv<T> ddx = cdiv_dx<T>(p), ddz = cdiv_dz<T>(p), deps0 = cdiv_eps0<T>(p);
+    const T dt = T(p.dt), aYM = T(p.alpha);
+    const CDiv<T> ddx = cdiv_dx<T>(p), ddz = cdiv_dz<T>(p);
     const int act = action ? action[xc] : 0;
     const T dec = (act == 1) ? T(decay[xc]) : T(1.0);
     const unsigned bytes = (unsigned)nx * (unsigned)nz * (unsigned)sizeof(T);
@@ -3036,7 +3036,7 @@ __device__ __forceinline__ void mxz_maxwell(const State<T>& I, const State<T>& O
                 T Jz = -(nQz[0][c] * vzt[0] + nQz[1][c] * vzt[1]);
                 T dBdx = ddx(a.by[c] - a.byxm[c]);
                 T ez = wall ? T(0.0) : a.ez[c];
-                ezn[c] = ez + dt * (c2 * dBdx - deps0(Jz));
+                ezn[c] = ez + dt * (dBdx - Jz);     // c = eps0 = 1: exact
             }
         }
         T ezr[3];
