@@ -3182,11 +3182,6 @@ __device__ __forceinline__ void mxz_sweep(const State<T>& I, const State<T>& O, 
         const unsigned i = (unsigned)xc + (unsigned)r * (unsigned)nx;
 #pragma unroll
         for (int v = 0; v < NV; ++v) F[v] = ld32(I.a[b + v], i);
-        if (cut) {
-            // k_cell_filters: multiply by the column's decay, or zero
-            F[F_Q2] = (act == 2) ? T(0.0) : F[F_Q2] * dec;
-            F[F_Q3] = (act == 2) ? T(0.0) : F[F_Q3] * dec;
-        }
     };
 
     // the fluid is loaded two rows ahead: F (row rc(e)), nF (rc(e + 1)), nnF
@@ -3205,6 +3200,14 @@ __device__ __forceinline__ void mxz_sweep(const State<T>& I, const State<T>& O, 
         const int rn = MODE ? rc + 1 : rowof(e + 1);
         T nnF[NV];
         load(rowof(e + 2), nnF);
+        if (cut) {
+            // k_cell_filters on the charges of the row in use: multiply by
+            // the column's decay, or zero (applied here rather than at the
+            // load, which would wait for the load two rows early, and with
+            // it for every store issued before it)
+            F[F_Q2] = (act == 2) ? T(0.0) : F[F_Q2] * dec;
+            F[F_Q3] = (act == 2) ? T(0.0) : F[F_Q3] * dec;
+        }
         const int kc = 1 + rc - rc0;            // ring index of row rc; row rc's predecessor is kc - 1
         mxz_wait_rows(sh.ready, kc);
         const T(&cs)[12][64] = ring[kc % MXZ_SLOTS];
