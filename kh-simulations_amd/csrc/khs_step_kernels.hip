@@ -2897,6 +2897,11 @@ struct MxzShared {
     // per species, the sweep's cell velocities p / max(n, 0.05) (x, z) and
     // x-interface coefficients eps, nu, mu of the ring row's fluid
     T vel[MXZ_SLOTS][2][5][64];
+    // FP32: the ring row's fluid (n, px, pz, Q1..Q3 of both species, before
+    // the charge filter), the sweep waves' input, loaded once by the Maxwell
+    // wave (FP64 sweep waves load it themselves: there the LDS copy raises
+    // the kernel to 256 VGPRs with spills, 249 -> 293 us; FP32 141 -> 132 us)
+    T fl[sizeof(T) == 4 ? MXZ_SLOTS : 1][2 * F_COUNT][64];
     int ready[MXZ_SLOTS];
     int released[4];
 };
@@ -3079,6 +3084,16 @@ __device__ __forceinline__ void mxz_maxwell(const State<T>& I, const State<T>& O
             sh.ring[slot][S_BY1 + c][lane] = byn;
             sh.ring[slot][S_AZ1 + c][lane] = azo;
         }
+        if constexpr (sizeof(T) == 4) {
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                sh.fl[slot][s * F_COUNT + F_N][lane] = a.n[s];
+                sh.fl[slot][s * F_COUNT + F_PX][lane] = a.px[s];
+                sh.fl[slot][s * F_COUNT + F_PZ][lane] = a.pz[s];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) sh.fl[slot][s * F_COUNT + F_Q1 + c][lane] = a.q[s][c];
+            }
+        }
         // the sweep's velocities and x-interface coefficients of row q
         // (update_velocities, YM_Fluid.cu:3-11: p / max(n, 0.05); the
         // currents' quotients above floor n at 1e-10, the same values for
@@ -3178,19 +3193,22 @@ __device__ __forceinline__ void mxz_sweep(const State<T>& I, const State<T>& O, 
 #pragma unroll
     for (int j = 0; j < NG; ++j) orsc[j] = buf_rsrc(O.a[b + VB + j], bytes);
 
+    // FP32: the fluid rows come from the ring (the Maxwell wave's loads of
+    // the same rows); FP64: loaded two rows ahead, F (row rc(e)), nF
+    // (rc(e + 1)), nnF (rc(e + 2)); rc(e) = clamp(min(e, elast), 0, nz - 1)
+    constexpr bool FL = sizeof(T) == 4;
     auto load = [&](int r, T (&F)[NV]) {
         const unsigned i = (unsigned)xc + (unsigned)r * (unsigned)nx;
 #pragma unroll
         for (int v = 0; v < NV; ++v) F[v] = ld32(I.a[b + v], i);
     };
-
-    // the fluid is loaded two rows ahead: F (row rc(e)), nF (rc(e + 1)), nnF
-    // (rc(e + 2)), rc(e) = clamp(min(e, elast), 0, nz - 1)
     auto rowof = [&](int e) { return clampi(min(e, elast), 0, nz - 1); };
     int rc = rc0;
     T F[NV], nF[NV];
-    load(rc, F);
-    load(rowof(z0 - 2), nF);
+    if constexpr (!FL) {
+        load(rc, F);
+        load(rowof(z0 - 2), nF);
+    }
     ZPipe<T, NG> zp;
     zp.Um = T(0);
     zp.MUm = T(0);
@@ -3199,17 +3217,19 @@ __device__ __forceinline__ void mxz_sweep(const State<T>& I, const State<T>& O, 
         constexpr int MODE = decltype(mode)::value;
         const int rn = MODE ? rc + 1 : rowof(e + 1);
         T nnF[NV];
-        load(rowof(e + 2), nnF);
+        if constexpr (!FL) load(rowof(e + 2), nnF);
+        const int kc = 1 + rc - rc0;            // ring index of row rc; row rc's predecessor is kc - 1
+        mxz_wait_rows(sh.ready, kc);
+        if constexpr (FL) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) F[v] = sh.fl[kc % MXZ_SLOTS][sp * F_COUNT + v][lane];
+        }
         if (cut) {
             // k_cell_filters on the charges of the row in use: multiply by
-            // the column's decay, or zero (applied here rather than at the
-            // load, which would wait for the load two rows early, and with
-            // it for every store issued before it)
+            // the column's decay, or zero
             F[F_Q2] = (act == 2) ? T(0.0) : F[F_Q2] * dec;
             F[F_Q3] = (act == 2) ? T(0.0) : F[F_Q3] * dec;
         }
-        const int kc = 1 + rc - rc0;            // ring index of row rc; row rc's predecessor is kc - 1
-        mxz_wait_rows(sh.ready, kc);
         const T(&cs)[12][64] = ring[kc % MXZ_SLOTS];
         const T(&ps)[12][64] = ring[(kc + MXZ_SLOTS - 1) % MXZ_SLOTS];
         const T(&vs)[5][64] = sh.vel[kc % MXZ_SLOTS][sp];
@@ -3304,10 +3324,12 @@ __device__ __forceinline__ void mxz_sweep(const State<T>& I, const State<T>& O, 
 #pragma unroll
             for (int j = 0; j < NG; ++j) buf_st_nt(out[j], orsc[j], off);
         }
+        if constexpr (!FL) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            F[v] = nF[v];
-            nF[v] = nnF[v];
+            for (int v = 0; v < NV; ++v) {
+                F[v] = nF[v];
+                nF[v] = nnF[v];
+            }
         }
         // the next row reads ring rows kn - 1 and kn: the ones below are free
         mxz_publish(&sh.released[w], kc + (rn != rc) - 1);
@@ -3328,10 +3350,10 @@ __device__ __forceinline__ void mxz_sweep(const State<T>& I, const State<T>& O, 
 
 // Workgroup: waves 0..3 sweep (species w / 2, group w % 2), waves 4.. Maxwell;
 // tile = 58 columns x a chunk of tz rows.  Occupancy: two waves per SIMD in
-// FP64, four in FP32; 4 + M waves per workgroup (M = 4: the eight wave slots
-// of a CU in FP64).
+// FP64, four in FP32 (three with M = 2: two workgroups' LDS per CU); 4 + M
+// waves per workgroup (M = 4: the eight wave slots of a CU in FP64).
 template <typename T, int M>
-__global__ __launch_bounds__(64 * (4 + M)) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2))) void k_mxz(
+__global__ __launch_bounds__(64 * (4 + M)) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? (M == 2 ? 3 : 4) : 2))) void k_mxz(
     State<T> I, State<T> O, KParams p, int ngx, int nch, int tz, const double* decay, const int* action) {
     __shared__ MxzShared<T> sh;
     const int lane = threadIdx.x & 63;
