@@ -2884,6 +2884,10 @@ void launch_maxwell(const State<T>& S, const State<T>& O, const KParams& p, cons
 // n + 1 (before the charge filter), fluid after the sweeps of step n + 1.
 // ==========================================================================
 constexpr int MXZ_SLOTS = 8;
+// whether the sweep waves take their fluid rows from the ring (filled by the
+// Maxwell wave that loads them for its currents) or load them themselves
+template <typename T>
+constexpr bool kMxzFluidRing = true;
 
 // The ring: slot k % MXZ_SLOTS holds the k-th new field row of the tile
 // (state ids S_EX1 .. S_AZ3, one value per lane).  Rows are produced in the
@@ -2897,11 +2901,9 @@ struct MxzShared {
     // per species, the sweep's cell velocities p / max(n, 0.05) (x, z) and
     // x-interface coefficients eps, nu, mu of the ring row's fluid
     T vel[MXZ_SLOTS][2][5][64];
-    // FP32: the ring row's fluid (n, px, pz, Q1..Q3 of both species, before
-    // the charge filter), the sweep waves' input, loaded once by the Maxwell
-    // wave (FP64 sweep waves load it themselves: there the LDS copy raises
-    // the kernel to 256 VGPRs with spills, 249 -> 293 us; FP32 141 -> 132 us)
-    T fl[sizeof(T) == 4 ? MXZ_SLOTS : 1][2 * F_COUNT][64];
+    // the ring row's fluid (n, px, pz, Q1..Q3 of both species, before the
+    // charge filter), the sweep waves' input, loaded once by the Maxwell wave
+    T fl[kMxzFluidRing<T> ? MXZ_SLOTS : 1][2 * F_COUNT][64];
     int ready[MXZ_SLOTS];
     int released[4];
 };
@@ -3084,16 +3086,6 @@ __device__ __forceinline__ void mxz_maxwell(const State<T>& I, const State<T>& O
             sh.ring[slot][S_BY1 + c][lane] = byn;
             sh.ring[slot][S_AZ1 + c][lane] = azo;
         }
-        if constexpr (sizeof(T) == 4) {
-#pragma unroll
-            for (int s = 0; s < 2; ++s) {
-                sh.fl[slot][s * F_COUNT + F_N][lane] = a.n[s];
-                sh.fl[slot][s * F_COUNT + F_PX][lane] = a.px[s];
-                sh.fl[slot][s * F_COUNT + F_PZ][lane] = a.pz[s];
-#pragma unroll
-                for (int c = 0; c < 3; ++c) sh.fl[slot][s * F_COUNT + F_Q1 + c][lane] = a.q[s][c];
-            }
-        }
         // the sweep's velocities and x-interface coefficients of row q
         // (update_velocities, YM_Fluid.cu:3-11: p / max(n, 0.05); the
         // currents' quotients above floor n at 1e-10, the same values for
@@ -3117,6 +3109,17 @@ __device__ __forceinline__ void mxz_maxwell(const State<T>& I, const State<T>& O
         mxz_publish(&sh.ready[slot], k + 1);
     };
 
+    auto put_fl = [&](int slot, const MRow<T>& r) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            sh.fl[slot][s * F_COUNT + F_N][lane] = r.n[s];
+            sh.fl[slot][s * F_COUNT + F_PX][lane] = r.px[s];
+            sh.fl[slot][s * F_COUNT + F_PZ][lane] = r.pz[s];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) sh.fl[slot][s * F_COUNT + F_Q1 + c][lane] = r.q[s][c];
+        }
+    };
+
     // Tasks of TR ring rows, dealt round-robin.  FP64: two rows k, k + 1
     // (rows q, q + 1), from loads of rows q - 1 (By) .. q + 2, with Ampere's
     // Ex at q, q + 1, q + 2 (the next task evaluates Ex at its first row
@@ -3135,6 +3138,15 @@ __device__ __forceinline__ void mxz_maxwell(const State<T>& I, const State<T>& O
         if (two) mrow_load(I, I, rbase(q + 2) + (unsigned)xc, rbase(q + 2) + (unsigned)xm, c);
 #pragma unroll
         for (int cc = 0; cc < 3; ++cc) bym[cc] = ld32(I.a[S_BY1 + cc], rbase(q - 1) + (unsigned)xc);
+        if constexpr (kMxzFluidRing<T>) {
+            // the task's fluid rows into their ring slots for the sweep waves,
+            // as soon as they are loaded (the slots' rows k - S, k + 1 - S
+            // released first), so that they do not stay live in registers
+            const int klast = two ? k + 1 : k;
+            if (klast >= MXZ_SLOTS) mxz_wait_released(sh.released, klast - MXZ_SLOTS + 1);
+            put_fl(k % MXZ_SLOTS, a);
+            if (two) put_fl((k + 1) % MXZ_SLOTS, b);
+        }
         // cell quantities of the currents (kernel_ym_currents, YM_Physics.cu:9-60)
         T uxa[2], uza[2], nqa[2][3], uxb[2], uzb[2], nqb[2][3];
         mxz_cell(a, uxa, uza, nqa);
@@ -3196,7 +3208,7 @@ __device__ __forceinline__ void mxz_sweep(const State<T>& I, const State<T>& O, 
     // FP32: the fluid rows come from the ring (the Maxwell wave's loads of
     // the same rows); FP64: loaded two rows ahead, F (row rc(e)), nF
     // (rc(e + 1)), nnF (rc(e + 2)); rc(e) = clamp(min(e, elast), 0, nz - 1)
-    constexpr bool FL = sizeof(T) == 4;
+    constexpr bool FL = kMxzFluidRing<T>;
     auto load = [&](int r, T (&F)[NV]) {
         const unsigned i = (unsigned)xc + (unsigned)r * (unsigned)nx;
 #pragma unroll

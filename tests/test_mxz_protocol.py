@@ -2,10 +2,10 @@
 k_mxz (Maxwell + the next step's sweeps in one launch).  M Maxwell waves share
 the ring rows in tasks of one (FP32) or two (FP64) rows dealt round-robin
 (wave m, two-row tasks: rows 2m, 2m + 1, 2m + 2M, 2m + 2M + 1, ...); for
-row k a Maxwell
-wave waits until every sweep wave has released row k - S, writes slot k % S,
-then publishes ready[k % S] = k + 1.  A sweep wave at ring row kc waits for
-ready[kc % S] >= kc + 1 and ready[(kc - 1) % S] >= kc, reads rows kc and
+a task of rows k .. k' a Maxwell wave waits until every sweep wave has
+released row k' - S, writes the slots of its rows, then publishes
+ready[k % S] = k + 1 for each row in turn.  A sweep wave at ring row kc
+waits for ready[kc % S] >= kc + 1 and ready[(kc - 1) % S] >= kc, reads rows kc and
 kc - 1, then releases the rows below the ones its next row reads.  Writing a
 slot, publishing it, reading the ring and releasing are separate steps.
 Random interleavings of all waves over the row sequences of many chunk shapes
@@ -29,38 +29,45 @@ def run(z0, tz, nz, nmax, seed, task=2):
 
     nrows = 2 + rowof(elast) - rc0
     ready, released, held = [0] * S, [0] * 4, {}
-    # the rows of each Maxwell wave, in its order, and the next one's index
-    prow = [[k for t in range(task * m, nrows, task * nmax) for k in range(t, t + task) if k < nrows]
-            for m in range(nmax)]
+    # the tasks of each Maxwell wave, in its order, and the next one's index;
+    # a task waits until the slots of all its rows are released, writes them
+    # (the fluid rows go into the ring as soon as they are loaded), then
+    # publishes its rows one at a time
+    ptask = [[[k for k in range(t, t + task) if k < nrows] for t in range(task * m, nrows, task * nmax)]
+             for m in range(nmax)]
     pk = [0] * nmax
-    pstate = [0] * nmax             # 0: before the write, 1: written, not yet published
+    pstate = [0] * nmax             # 0: before the writes, j >= 1: written, j - 1 rows published
     ce = [z0 - 3] * 4               # next row e of each sweep wave
     cstate = [0] * 4                # 0: before the read, 1: read, not yet released
     reading = [None] * 4            # ring rows a sweep wave reads (between read and release)
     for _ in range(10 ** 6):
-        actors = [("P", m) for m in range(nmax) if pk[m] < len(prow[m])] + \
+        actors = [("P", m) for m in range(nmax) if pk[m] < len(ptask[m])] + \
                  [("C", w) for w in range(4) if ce[w] <= elast]
         if not actors:
             return
         kind, i = rnd.choice(actors)
         if kind == "P":
-            k = prow[i][pk[i]]
+            rows = ptask[i][pk[i]]
             if pstate[i] == 0:
-                if k >= S and min(released) < k - S + 1:
+                if rows[-1] >= S and min(released) < rows[-1] - S + 1:
                     continue                               # waits
-                old = held.get(k % S)
-                for w in range(4):
-                    if old is not None and reading[w] is not None:
-                        assert old not in reading[w], f"row {old} overwritten while wave {w} reads it"
-                    if old is not None and ce[w] <= elast:
-                        kc = 1 + rowof(ce[w]) - rc0
-                        assert old not in (kc, kc - 1), f"row {old} overwritten before wave {w} reads it"
-                held[k % S] = k
+                for k in rows:
+                    old = held.get(k % S)
+                    for w in range(4):
+                        if old is not None and reading[w] is not None:
+                            assert old not in reading[w], f"row {old} overwritten while wave {w} reads it"
+                        if old is not None and ce[w] <= elast:
+                            kc = 1 + rowof(ce[w]) - rc0
+                            assert old not in (kc, kc - 1), f"row {old} overwritten before wave {w} reads it"
+                    held[k % S] = k
                 pstate[i] = 1
             else:
+                k = rows[pstate[i] - 1]
                 ready[k % S] = k + 1
-                pstate[i] = 0
-                pk[i] += 1
+                pstate[i] += 1
+                if pstate[i] > len(rows):
+                    pstate[i] = 0
+                    pk[i] += 1
         else:
             e = ce[i]
             rc = rowof(e)
