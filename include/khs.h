@@ -248,6 +248,12 @@ int khs_get_step(khs_ctx* ctx, int* step, double* time);
 /* ---------------- diagnostics ---------------- */
 /* Total energy (YM_Energy.cu:3-32) as a compensated FP64 device reduction. */
 int khs_energy(khs_ctx* ctx, double* E);
+/* The energy that the last energy check of khs_step (main_ym.cu:457-479) or
+ * of a batch segment used, and the step it was taken at (-1: no check yet).
+ * Where the steps before a check run as Maxwell + next-sweep launches, the
+ * check's energy is summed by the last Maxwell launch itself, so this is the
+ * value the halting rule saw.  No counterpart in the reference. */
+int khs_last_energy(khs_ctx* ctx, double* E, int* step);
 /* Allocation-free energy reduction on caller-owned device arrays.  No
  * counterpart in the reference: it is for callers that cannot allocate
  * device memory (code under HIP graph capture, serving loops, harnesses that

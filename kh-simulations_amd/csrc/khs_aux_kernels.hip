@@ -392,25 +392,7 @@ void launch_enforce_boundaries(const State<T>& S, const KParams& p, int emhd, hi
 // --------------------------------------------------------------------------
 // compensated reductions
 // --------------------------------------------------------------------------
-struct CSum {
-    double s, c;
-};
-__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
-    s = a + b;
-    double bb = s - a;
-    e = (a - (s - bb)) + (b - bb);
-}
-__device__ __forceinline__ void cadd(CSum& acc, double v) {   // Neumaier
-    double s, e;
-    two_sum(acc.s, v, s, e);
-    acc.s = s;
-    acc.c += e;
-}
-__device__ __forceinline__ CSum cmerge(CSum a, CSum b) {
-    double s, e;
-    two_sum(a.s, b.s, s, e);
-    return CSum{s, a.c + b.c + e};
-}
+// (CSum, two_sum, cadd, cmerge: khs_device.h)
 
 template <int K>
 __device__ void block_reduce_store(CSum (&acc)[K], double* partials) {
@@ -491,6 +473,32 @@ __global__ __launch_bounds__(kRedThreads) void k_energy(State<T> S, KParams p, d
     for (size_t i = blockIdx.x * (size_t)kRedThreads + threadIdx.x; i < N; i += (size_t)gridDim.x * kRedThreads)
         cadd(acc[0], energy_cell(S, i));
     block_reduce_store<1>(acc, partials);
+}
+
+// First stage for long lists of partial sums: block b merges the CSum pairs
+// [4 kRedThreads b, 4 kRedThreads (b + 1)) into out[b].
+__global__ __launch_bounds__(kRedThreads) void k_reduce_stage(const double* __restrict__ in, int n, double* out) {
+    CSum acc[1] = {CSum{0.0, 0.0}};
+    const int base = blockIdx.x * 4 * kRedThreads + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int i = base + k * kRedThreads;
+        if (i < n) acc[0] = cmerge(acc[0], CSum{in[2 * (size_t)i], in[2 * (size_t)i + 1]});
+    }
+    block_reduce_store<1>(acc, out);
+}
+
+// the compensated sum of n partial sums (CSum pairs) into out[0]; lists
+// longer than 4 kRedThreads go through one first stage into scratch (room
+// for ceil(n / (4 kRedThreads)) pairs), in a fixed order
+void launch_reduce_partials(const double* partials, int n, double* scratch, double* out, hipStream_t st) {
+    if (n <= 4 * kRedThreads) {
+        hipLaunchKernelGGL(k_reduce_final<1>, dim3(1), dim3(kRedThreads), 0, st, partials, n, out);
+        return;
+    }
+    const int nb = (n + 4 * kRedThreads - 1) / (4 * kRedThreads);
+    hipLaunchKernelGGL(k_reduce_stage, dim3(nb), dim3(kRedThreads), 0, st, partials, n, scratch);
+    hipLaunchKernelGGL(k_reduce_final<1>, dim3(1), dim3(kRedThreads), 0, st, scratch, nb, out);
 }
 
 template <typename T>

@@ -136,6 +136,13 @@ struct Ctx {
     // then Maxwell and the filters
     bool mxz = false;
     int mxzvar = 1;
+    // per-wavefront energy sums of the last Maxwell of an energy stride
+    double* d_epart = nullptr;
+    int n_epart = 0;
+    bool energy_ready = false;   // d_red holds the energy of the state at step energy_step
+    long energy_step = -1;       // (summed by that step's Maxwell)
+    double last_check_E = 0.0;   // the energy of khs_step's last check, and its step
+    long last_check_step = -1;
     // one energy stride of steps + the energy reduction, captured once
     hipGraphExec_t sgraph[2] = {nullptr, nullptr};   // by the ping-pong buffer a stride starts from
     bool tuned = false;
@@ -432,7 +439,7 @@ struct Ctx {
     // apart (fields of step n in X, fluid after the sweeps of step n in Y);
     // the charges' filter of each inner step is applied by the next launch as
     // it reads them, so nothing but the last step's filters is left to do.
-    template <typename T> void do_mxz_steps(int L) {
+    template <typename T> void do_mxz_steps(int L, bool energy_next) {
         set_order(0);
         do_xz<T>();
         int X = cur, Y = 1 - cur;
@@ -444,12 +451,38 @@ struct Ctx {
             std::swap(X, Y);
         }
         cur = X;                  // fields of the last step's start; its swept fluid in 1 - cur
-        do_maxwell<T>();
+        // an energy check follows: the last Maxwell sums the new state's
+        // energy as it writes it (the charge filter after it leaves the
+        // energy unchanged), instead of a separate pass over 15 arrays
+        int nw = 0;
+        // (the reduction's first stage leaves one pair per 4 kRedThreads
+        // wavefront sums in d_partials, which holds kRedBlocks * 8 pairs)
+        if (energy_next && d_epart && (n_epart + 4 * kRedThreads - 1) / (4 * kRedThreads) <= kRedBlocks * 8) {
+            set_order(2);
+            timed(K_MAXWELL, [&] {
+                nw = launch_maxwell_energy<T>(state<T>(cur), state<T>(1 - cur), kp, cell_filters ? d_decay : nullptr,
+                                              cell_filters ? d_action : nullptr, mvar, d_epart, stream);
+            });
+            check_launch();
+        }
+        const bool summed = nw > 0;             // nw <= n_epart: sized for the shortest chunks
+        if (summed) {
+            launch_reduce_partials(d_epart, nw, d_partials, d_red, stream);
+            check_launch();
+        } else {
+            do_maxwell<T>();
+        }
         do_filters<T>();
         cur = 1 - cur;            // commit
         step += 1;
+        if (summed) {
+            energy_ready = true;
+            energy_step = step;
+        }
     }
-    void mxz_steps(int L) { fp32 ? do_mxz_steps<float>(L) : do_mxz_steps<double>(L); }
+    void mxz_steps(int L, bool energy_next) {
+        fp32 ? do_mxz_steps<float>(L, energy_next) : do_mxz_steps<double>(L, energy_next);
+    }
     // z sweep of both species then Maxwell, as the split kernels (v = 0) or
     // the fused kernel (v = 1..3); writes the next state
     template <typename T> void zm_launch(int v) {
@@ -572,6 +605,11 @@ struct Ctx {
     // Energy reduction and its copy into the pinned h_red[0], without waiting
     // (also used inside a captured graph).
     void energy_async() {
+        if (energy_ready && energy_step == step) {   // summed by the stride's last Maxwell
+            energy_ready = false;
+            HIPCHK(hipMemcpyAsync(h_red, d_red, sizeof(double), hipMemcpyDeviceToHost, stream));
+            return;
+        }
         timed(K_ENERGY, [&] {
             if (emhd) {
                 if (fp32) launch_energy_em<float>(state<float>(cur), kp, d_partials, d_red, stream);
@@ -742,6 +780,11 @@ static std::unique_ptr<Ctx> make_ctx(const khs_config& in) {
     if (coef_bytes) reserve((void**)&ctx->d_coef, coef_bytes);
     if (keep_mfma) reserve((void**)&ctx->d_keep_tab, (size_t)c.nz * 16 * ctx->esz);
     reserve((void**)&ctx->d_partials, (size_t)kRedBlocks * 8 * 2 * sizeof(double));
+    // the Maxwell march's per-wavefront energy sums: 63 columns x 2 rows at least
+    if (!ctx->emhd) {
+        ctx->n_epart = ((c.nx + 62) / 63) * ((c.nz + 1) / 2);
+        reserve((void**)&ctx->d_epart, (size_t)ctx->n_epart * 2 * sizeof(double));
+    }
     reserve((void**)&ctx->d_red, 16 * sizeof(double));
     reserve((void**)&ctx->d_derived, ctx->N * sizeof(double));
 
@@ -833,7 +876,7 @@ static void commit(Ctx& c) {
 }
 
 static void full_step(Ctx& c);
-static void run_steps(Ctx& c, int L);
+static void run_steps(Ctx& c, int L, bool energy_next = false);
 
 // Capture `stride` full steps and the energy reduction (into the pinned
 // h_red[0]) as one HIP graph, starting from ping-pong buffer c.cur, into
@@ -852,7 +895,7 @@ static void capture_stride(Ctx& c, int stride) {
     std::lock_guard<std::mutex> lk(capture_mutex());
     HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeRelaxed));
     try {
-        run_steps(c, stride);
+        run_steps(c, stride, true);
         c.energy_async();
     } catch (...) {
         hipGraph_t g = nullptr;
@@ -860,6 +903,7 @@ static void capture_stride(Ctx& c, int stride) {
         if (g) (void)hipGraphDestroy(g);
         c.cur = cur0;
         c.step = step0;
+        c.energy_ready = false;
         throw;
     }
     hipGraph_t g = nullptr;
@@ -898,11 +942,14 @@ static void full_step(Ctx& c) {
     commit(c);
 }
 
-// L consecutive full steps without an energy check between them.
-static void run_steps(Ctx& c, int L) {
+// L consecutive full steps without an energy check between them;
+// energy_next: the caller reads the energy of the resulting state next
+// (energy_async), which the last step may then sum as it writes the state.
+static void run_steps(Ctx& c, int L, bool energy_next) {
     c.autotune();
+    c.energy_ready = false;
     if (c.mxz && L >= 2) {
-        c.mxz_steps(L);
+        c.mxz_steps(L, energy_next);
         return;
     }
     for (int s = 0; s < L; ++s) full_step(c);
@@ -984,7 +1031,7 @@ struct Batch {
             c.stream = lanes[lane_of[k]];
             const int cur0 = c.cur;
             const long step0 = c.step;
-            run_steps(c, K);
+            run_steps(c, K, true);
             c.energy_async();
             c.cur = cur0;                                   // the graph does the stepping
             c.step = step0;
@@ -1508,6 +1555,8 @@ int khs_step(khs_ctx* ctx, int nsteps, int* halted) {
                 c.cur ^= (stride & 1);
                 s += stride;
                 const double E = c.h_red[0];
+                c.last_check_E = E;
+                c.last_check_step = c.step;
                 khs::write_energy_row(c, c.step, E);
                 if (std::isnan(E) || std::isinf(E) || E / c.E0 > c.cfg.energy_thresh) {
                     if (halted) *halted = 1;
@@ -1518,10 +1567,12 @@ int khs_step(khs_ctx* ctx, int nsteps, int* halted) {
             // the steps up to the next energy check (or the end) as one segment
             long L = nsteps - s;
             if (stride > 0) L = std::min<long>(L, stride - c.step % stride);
-            khs::run_steps(c, (int)L);
+            khs::run_steps(c, (int)L, stride > 0 && (c.step + L) % stride == 0);
             s += (int)L;
             if (stride > 0 && c.step % stride == 0) {
                 double E = c.energy();
+                c.last_check_E = E;
+                c.last_check_step = c.step;
                 khs::write_energy_row(c, c.step, E);
                 double ratio = E / c.E0;
                 if (std::isnan(E) || std::isinf(E) || ratio > c.cfg.energy_thresh) {
@@ -1545,6 +1596,14 @@ int khs_energy(khs_ctx* ctx, double* E) {
     CTX_OR_FAIL(ctx);
     if (!E) { g_last_error = "null argument"; return KHS_ERR_ARG; }
     return guard([&] { *E = ctx->impl->energy(); });
+}
+
+int khs_last_energy(khs_ctx* ctx, double* E, int* step) {
+    CTX_OR_FAIL(ctx);
+    if (!E || !step) { g_last_error = "null argument"; return KHS_ERR_ARG; }
+    *E = ctx->impl->last_check_E;
+    *step = (int)ctx->impl->last_check_step;
+    return KHS_OK;
 }
 
 int khs_energy_noalloc(const void* const arrays[KHS_N_STATE], int nx, int nz, int precision, void* stream,
@@ -2051,6 +2110,8 @@ static int batch_run_graphs(khs_batch* b, int nsteps, int* steps_done, int* halt
                 Ctx& c = *B.runs[i];
                 c.step += K;
                 const double E = c.h_red[0];
+                c.last_check_E = E;
+                c.last_check_step = c.step;
                 khs::write_energy_row(c, c.step, E);
                 const double ratio = E / c.E0;
                 if (std::isnan(E) || std::isinf(E) || ratio > c.cfg.energy_thresh) B.active[i] = 0;

@@ -97,6 +97,27 @@ __device__ __forceinline__ void div2(float a1, float a2, float b, float& q1, flo
     q2 = a2 / b;
 }
 
+// Compensated FP64 sums: Neumaier accumulation and TwoSum merges.
+struct CSum {
+    double s, c;
+};
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+    s = a + b;
+    double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+__device__ __forceinline__ void cadd(CSum& acc, double v) {   // Neumaier
+    double s, e;
+    two_sum(acc.s, v, s, e);
+    acc.s = s;
+    acc.c += e;
+}
+__device__ __forceinline__ CSum cmerge(CSum a, CSum b) {
+    double s, e;
+    two_sum(a.s, b.s, s, e);
+    return CSum{s, a.c + b.c + e};
+}
+
 // --------------------------------------------------------------------------
 // index helpers
 // --------------------------------------------------------------------------

@@ -58,6 +58,12 @@ void launch_fct_zmx(const State<T>& S, const Stage<T>& X, const State<T>& O, con
 template <typename T>
 void launch_maxwell(const State<T>& S, const State<T>& O, const KParams& p, const double* decay,
                     const int* action, int variant, hipStream_t st);
+// Maxwell (z-march variants 4..10) that also writes one compensated partial
+// sum of the new state's energy per wavefront into epart; returns their
+// number, 0 (nothing launched) for the other variants.
+template <typename T>
+int launch_maxwell_energy(const State<T>& S, const State<T>& O, const KParams& p, const double* decay,
+                          const int* action, int variant, double* epart, hipStream_t st);
 // Maxwell of step n (with the sponge / xi_cut of its colour-2/3 fields) and
 // the fused x + z sweeps of step n + 1 (with the same filter applied to the
 // charges it reads) in one launch, YM model without column filters.  I:
@@ -127,6 +133,9 @@ void launch_enforce_boundaries(const State<T>& S, const KParams& p, int emhd, hi
 // diagnostics
 constexpr int kRedBlocks = 1024;
 constexpr int kRedThreads = 256;
+// The compensated sum of n (sum, compensation) pairs into out[0]; scratch
+// holds ceil(n / (4 kRedThreads)) pairs for long lists.
+void launch_reduce_partials(const double* partials, int n, double* scratch, double* out, hipStream_t st);
 // Energy: out[0] = sum, out[1] = nonfinite flag (>0 if NaN/Inf seen).
 template <typename T>
 void launch_energy(const State<T>& S, const KParams& p, double* partials, double* out,

@@ -1129,9 +1129,10 @@ __device__ __forceinline__ void mrow_load(const State<T>& S, const State<T>& O, 
 
 // One wavefront of the march: wave wl (uniform) of a grid of nwx column
 // groups x chunks of TZ rows.
-template <typename T, int TZ, int PF>
+template <typename T, int TZ, int PF, bool EN = false>
 __device__ __forceinline__ void maxwell_zm_wave(const State<T>& S, const State<T>& O, const KParams& p, int nwx,
-                                                int wl, int lane, const double* decay, const int* action) {
+                                                int wl, int lane, const double* decay, const int* action,
+                                                double* epart = nullptr) {
     const int nx = p.nx, nz = p.nz;
     const bool per = p.periodic_x != 0;
     const int x = (wl % nwx) * MZM_OUT + lane;
@@ -1168,6 +1169,10 @@ __device__ __forceinline__ void maxwell_zm_wave(const State<T>& S, const State<T
         for (int c = 0; c < 3; ++c) byp[c] = ld32(S.a[S_BY1 + c], im);
     }
     MRow<T> cur, nxt;
+    // EN: the energy density of the wave's own cells after this step
+    // (YM_Energy.cu:3-21: kinetic 0.5 |p|^2 / max(n, 1e-10) of both species,
+    // 0.5 (E^2 + By^2) of the filtered fields), a compensated sum per lane
+    CSum eacc{0.0, 0.0};
     auto rbase = [&](int e) { return (unsigned)(e < nz ? e : e - nz) * (unsigned)nx; };
     mrow_load(S, O, rbase(z0) + (unsigned)xc, rbase(z0) + (unsigned)xm, cur);
 
@@ -1183,6 +1188,18 @@ __device__ __forceinline__ void maxwell_zm_wave(const State<T>& S, const State<T
             uz[s] = cur.pz[s] / nf;
 #pragma unroll
             for (int c = 0; c < 3; ++c) nq[s][c] = cur.n[s] * cur.q[s][c];
+        }
+        if constexpr (EN) {
+            if (store && e < z1) {
+                double ke = 0.0;
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    const double nn = kmax((double)cur.n[s], 1e-10);
+                    const double px = cur.px[s], pz = cur.pz[s];
+                    ke = ke + 0.5 * (px * px + pz * pz) / nn;
+                }
+                cadd(eacc, ke);
+            }
         }
         // Jx at (x+1/2, e) with the x+1 values of lane l+1
         T Jx[3];
@@ -1246,6 +1263,7 @@ __device__ __forceinline__ void maxwell_zm_wave(const State<T>& S, const State<T
                              Az[0] * exn_p[1] - Az[1] * exn_p[0]};
             {
                 const unsigned off = store ? (rbase(e - 1) + (unsigned)xc) * (unsigned)sizeof(T) : kNoStore;
+                double em = 0.0;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     T dEx = ddz(exn[c] - exn_p[c]);
@@ -1258,6 +1276,13 @@ __device__ __forceinline__ void maxwell_zm_wave(const State<T>& S, const State<T
                     buf_st(exo, rex[c], off);
                     buf_st(ezo, rez[c], off);
                     if (c > 0 || !p.freeze_az1) buf_st(azo, raz[c], off);
+                    if constexpr (EN) {
+                        const double a = exo, b = ezo, f = byn;
+                        em = em + a * a + b * b + f * f;
+                    }
+                }
+                if constexpr (EN) {
+                    if (store) cadd(eacc, 0.5 * em);
                 }
             }
         }
@@ -1277,17 +1302,29 @@ __device__ __forceinline__ void maxwell_zm_wave(const State<T>& S, const State<T
             else mrow_load(S, O, rbase(e + 1) + (unsigned)xc, rbase(e + 1) + (unsigned)xm, cur);
         }
     }
+    if constexpr (EN) {
+        // the wave's lanes merged in a fixed order; one CSum per wave
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const CSum o{__shfl_down(eacc.s, off, 64), __shfl_down(eacc.c, off, 64)};
+            eacc = cmerge(eacc, o);
+        }
+        if (lane == 0) {
+            epart[2 * (size_t)wl] = eacc.s;
+            epart[2 * (size_t)wl + 1] = eacc.c;
+        }
+    }
 }
 
-template <typename T, int TZ, int PF>
+template <typename T, int TZ, int PF, bool EN = false>
 __global__ __launch_bounds__(256) void k_maxwell_zm(State<T> S, State<T> O, KParams p, int nwx, int nch,
-                                                   const double* decay, const int* action) {
+                                                   const double* decay, const int* action, double* epart) {
     const int lane = threadIdx.x & 63;
     // the wavefront index is uniform: scalar loop control
     int wl = __builtin_amdgcn_readfirstlane(xcd_logical(blockIdx.x, gridDim.x) * 4 + (int)(threadIdx.x >> 6));
     if (wl >= nwx * nch) return;             // whole wavefronts leave together
     wl = tile_order(wl, nwx * nch, p.rev);
-    maxwell_zm_wave<T, TZ, PF>(S, O, p, nwx, wl, lane, decay, action);
+    maxwell_zm_wave<T, TZ, PF, EN>(S, O, p, nwx, wl, lane, decay, action, epart);
 }
 
 // --------------------------------------------------------------------------
@@ -2801,13 +2838,18 @@ void launch_fct_xz(const State<T>& S, const State<T>& O, const KParams& p, int v
 }
 
 template <typename T, int TZ, int PF>
-static void launch_maxwell_zm_t(const State<T>& S, const State<T>& O, const KParams& p, const double* decay,
-                                const int* action, hipStream_t st) {
+static int launch_maxwell_zm_t(const State<T>& S, const State<T>& O, const KParams& p, const double* decay,
+                               const int* action, hipStream_t st, double* epart = nullptr) {
     const int nwx = (p.nx + MZM_OUT - 1) / MZM_OUT;
     const int nch = (p.nz + TZ - 1) / TZ;
     const int waves = nwx * nch;
-    hipLaunchKernelGGL((k_maxwell_zm<T, TZ, PF>), dim3(round8((waves + 3) / 4)), dim3(256), 0, st, S, O, p, nwx, nch,
-                       decay, action);
+    if (epart)
+        hipLaunchKernelGGL((k_maxwell_zm<T, TZ, PF, true>), dim3(round8((waves + 3) / 4)), dim3(256), 0, st, S, O, p,
+                           nwx, nch, decay, action, epart);
+    else
+        hipLaunchKernelGGL((k_maxwell_zm<T, TZ, PF>), dim3(round8((waves + 3) / 4)), dim3(256), 0, st, S, O, p, nwx,
+                           nch, decay, action, nullptr);
+    return waves;
 }
 
 // Fused z sweep + Maxwell, variant 1 / 2 / 3: 16 / 32 / 64-row chunks.
@@ -2855,6 +2897,31 @@ void launch_maxwell(const State<T>& S, const State<T>& O, const KParams& p, cons
     else
         hipLaunchKernelGGL((k_maxwell<T, 16>), grid, dim3(MX * MZT), 0, st, S, O, p, ntx, ntz, decay, action);
 }
+
+// Maxwell as a z march (variants 4..10) that also leaves, in epart, one
+// compensated partial sum of the new state's energy density per wavefront
+// (the fields after Maxwell and its column action, the fluid it reads), for
+// steps whose remaining filters leave the energy unchanged (the charge
+// filter).  Returns the number of partial sums, 0 for the other variants
+// (nothing launched).
+template <typename T>
+int launch_maxwell_energy(const State<T>& S, const State<T>& O, const KParams& p, const double* decay,
+                          const int* action, int variant, double* epart, hipStream_t st) {
+    switch (variant) {
+        case 4: return launch_maxwell_zm_t<T, 16, 1>(S, O, p, decay, action, st, epart);
+        case 5: return launch_maxwell_zm_t<T, 32, 1>(S, O, p, decay, action, st, epart);
+        case 6: return launch_maxwell_zm_t<T, 16, 0>(S, O, p, decay, action, st, epart);
+        case 7: return launch_maxwell_zm_t<T, 8, 1>(S, O, p, decay, action, st, epart);
+        case 8: return launch_maxwell_zm_t<T, 8, 0>(S, O, p, decay, action, st, epart);
+        case 9: return launch_maxwell_zm_t<T, 4, 1>(S, O, p, decay, action, st, epart);
+        case 10: return launch_maxwell_zm_t<T, 2, 1>(S, O, p, decay, action, st, epart);
+        default: return 0;
+    }
+}
+template int launch_maxwell_energy<double>(const State<double>&, const State<double>&, const KParams&, const double*,
+                                           const int*, int, double*, hipStream_t);
+template int launch_maxwell_energy<float>(const State<float>&, const State<float>&, const KParams&, const double*,
+                                          const int*, int, double*, hipStream_t);
 
 // ==========================================================================
 // Maxwell of step n and the fused sweeps of step n + 1 in one launch (YM

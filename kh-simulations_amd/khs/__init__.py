@@ -131,6 +131,7 @@ def _signatures():
         "khs_step": (c_int, [c_void_p, c_int, P(c_int)]),
         "khs_get_step": (c_int, [c_void_p, P(c_int), dp]),
         "khs_energy": (c_int, [c_void_p, dp]), "khs_energy0": (c_int, [c_void_p, dp]),
+        "khs_last_energy": (c_int, [c_void_p, dp, P(c_int)]),
         "khs_gauss": (c_int, [c_void_p, dp]),
         "khs_gauss_residual": (c_int, [c_void_p, dp, dp]),
         "khs_write_snapshot": (c_int, [c_void_p, c_char_p, c_int]),
@@ -363,6 +364,12 @@ class Simulation:
         e = ctypes.c_double()
         _check(_lib.khs_energy(self._h, ctypes.byref(e)))
         return e.value
+
+    def last_energy_check(self):
+        """(E, step) of the last energy check of khs_step (step -1: none yet)."""
+        e, st = ctypes.c_double(), ctypes.c_int()
+        _check(_lib.khs_last_energy(self._h, ctypes.byref(e), ctypes.byref(st)))
+        return e.value, st.value
 
     @property
     def energy0(self):

@@ -508,3 +508,28 @@ def test_nan_halts_and_errors():
         sim.fct_x_sweep(2)
     with pytest.raises(khs.KhsError):
         sim.get(99)
+
+
+@pytest.mark.parametrize("graphs", [1, 0])
+def test_stride_energy_summed_by_last_maxwell(graphs):
+    """With Maxwell fused to the next step's sweeps, the energy of each check
+    is summed by the stride's last Maxwell launch as it writes the state: the
+    value the halting rule saw (khs_last_energy) equals the separate energy
+    reduction of the same state and the oracle's energy to 1e-13, and the
+    state itself stays the oracle's to the bit."""
+    keys = grid(300, 136, 0, xi_cut=8.0, fused_sweep=1, step_fusion=1, maxwell_kernel=10,
+                step_graphs=graphs, energy_stride=5)
+    cfg, sim, o = make_pair(random_state(300, 136, seed=17, smooth=True), **keys)
+    o.sources()
+    assert sim.last_energy_check()[1] == -1
+    for n_steps in (5, 5):
+        assert not sim.step(n_steps), "halted on the energy check"
+        o.step(n_steps)
+        E, at = sim.last_energy_check()
+        assert at == sim.step_count
+        E_sep, E_orc = sim.energy(), o.energy()
+        assert abs(E - E_sep) <= 1e-13 * abs(E_sep), (E, E_sep)
+        assert abs(E - E_orc) <= 1e-13 * abs(E_orc), (E, E_orc)
+    assert sim.tuning()["step_fusion"] == 1
+    for n in orc.STATE:
+        assert_bitwise(sim.get(n), o.get(n), f"energy-fused {n}")
