@@ -29,6 +29,8 @@ struct CpuCtx {
     void* orc = nullptr;
     long step = 0;
     double E0 = 0.0;
+    double last_E = 0.0;           // the last energy check of khs_step and its step
+    long last_step = -1;
     bool sources_stale = true;     // the oracle's sources follow the state it steps
     std::ofstream ecsv;
     ~CpuCtx() {
@@ -212,6 +214,8 @@ int khs_step(khs_ctx* ctx, int nsteps, int* halted) {
             ++c.step;
             if (stride > 0 && c.step % stride == 0) {
                 const double E = energy_of(c);
+                c.last_E = E;
+                c.last_step = c.step;
                 write_row(c, c.step, E);
                 if (std::isnan(E) || std::isinf(E) || E / c.E0 > c.cfg.energy_thresh) {
                     if (halted) *halted = 1;
@@ -231,6 +235,13 @@ int khs_energy(khs_ctx* ctx, double* E) {
     CTX(ctx);
     if (!E) { g_err = "null argument"; return KHS_ERR_ARG; }
     *E = energy_of(c);
+    return KHS_OK;
+}
+int khs_last_energy(khs_ctx* ctx, double* E, int* step) {
+    CTX(ctx);
+    if (!E || !step) { g_err = "null argument"; return KHS_ERR_ARG; }
+    *E = c.last_E;
+    *step = (int)c.last_step;
     return KHS_OK;
 }
 int khs_energy0(khs_ctx* ctx, double* E0) {
