@@ -129,7 +129,7 @@ typedef struct khs_config {
     uint64_t spectrum_seed;
     double   em_perturb_amp;   /* EMHD das_kaw perturbation (argv[2] as double, default 0.05); set by perturb_amp */
     int      z_groups;         /* z sweep: -1 = autotuned (default), 0 = all variables per lane, 1 = two variable groups on two waves */
-    int      xz_variant;       /* fused sweep kernel variant: -1 = autotuned (default), YM 0..12, EMHD 0..6 */
+    int      xz_variant;       /* fused sweep kernel variant: -1 = autotuned (default), YM 0..14 (13, 14 with warm_T = 0 only), EMHD 0..6 */
     int      x_serial;         /* x sweep phases per variable: -1 = autotuned (default), 0 = all at once, 1 = one at a time */
     int      maxwell_kernel;   /* YM Maxwell: -1 = autotuned (default), 0/2/3 = 64x16/64x4/64x8 LDS tiles, 1 = per cell,
                                   4..10 = z march with 16/32/16/8/8/4/2-row chunks (6, 8: without next-row prefetch);

@@ -264,7 +264,8 @@ static void config_derive_emhd(khs_config& c) {
         throw Error(KHS_ERR_CONFIG, "x_rows must be 0, 1, 4 or 16");
     if (c.maxwell_kernel < -1 || c.maxwell_kernel > 10) throw Error(KHS_ERR_CONFIG, "maxwell_kernel must be -1..10");
     if (c.x_serial < -1 || c.x_serial > 1) throw Error(KHS_ERR_CONFIG, "x_serial must be -1, 0 or 1");
-    if (c.xz_variant < -1 || c.xz_variant > 12) throw Error(KHS_ERR_CONFIG, "xz_variant must be -1..12");
+    if (c.xz_variant < -1 || c.xz_variant > 14) throw Error(KHS_ERR_CONFIG, "xz_variant must be -1..14");
+    if (c.xz_variant >= 13 && c.warm_T > 0.0f) throw Error(KHS_ERR_CONFIG, "xz_variant 13, 14 need warm_T = 0");
     if (c.fused_sweep < -1 || c.fused_sweep > 1) throw Error(KHS_ERR_CONFIG, "fused_sweep must be -1, 0 or 1");
     if (c.z_groups < -1 || c.z_groups > 1) throw Error(KHS_ERR_CONFIG, "z_groups must be -1, 0 or 1");
     if (c.x_lane_cells != -1 && c.x_lane_cells != 0 && c.x_lane_cells != 4 && c.x_lane_cells != 8)
@@ -332,7 +333,8 @@ void config_derive(khs_config& c) {
         throw Error(KHS_ERR_CONFIG, "x_rows must be 0, 1, 4 or 16");
     if (c.maxwell_kernel < -1 || c.maxwell_kernel > 10) throw Error(KHS_ERR_CONFIG, "maxwell_kernel must be -1..10");
     if (c.x_serial < -1 || c.x_serial > 1) throw Error(KHS_ERR_CONFIG, "x_serial must be -1, 0 or 1");
-    if (c.xz_variant < -1 || c.xz_variant > 12) throw Error(KHS_ERR_CONFIG, "xz_variant must be -1..12");
+    if (c.xz_variant < -1 || c.xz_variant > 14) throw Error(KHS_ERR_CONFIG, "xz_variant must be -1..14");
+    if (c.xz_variant >= 13 && c.warm_T > 0.0f) throw Error(KHS_ERR_CONFIG, "xz_variant 13, 14 need warm_T = 0");
     if (c.fused_sweep < -1 || c.fused_sweep > 1) throw Error(KHS_ERR_CONFIG, "fused_sweep must be -1, 0 or 1");
     if (c.z_groups < -1 || c.z_groups > 1) throw Error(KHS_ERR_CONFIG, "z_groups must be -1, 0 or 1");
     if (c.x_lane_cells != -1 && c.x_lane_cells != 0 && c.x_lane_cells != 4 && c.x_lane_cells != 8)

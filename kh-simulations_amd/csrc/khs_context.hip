@@ -355,7 +355,9 @@ struct Ctx {
         if (fused) {
             std::vector<int> xzc;
             for (int v = 0; v < (emhd ? kNumXZEmVariants : kNumXZVariants); ++v)
-                if (cfg.xz_variant < 0 || v == cfg.xz_variant) xzc.push_back(v);
+                if ((cfg.xz_variant < 0 && (emhd || v < kFirstColdXZVariant || kp.warm_T <= 0.0)) ||
+                    v == cfg.xz_variant)
+                    xzc.push_back(v);
             if (xzc.size() > 1 || try_fused)
                 xzvar = pick<T>(xzc, [&](int v) { fct_xz<T>(v); }, &txz);
             if (try_fused) fused = txz < tx + tz;

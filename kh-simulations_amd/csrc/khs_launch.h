@@ -36,7 +36,9 @@ template <typename T>
 void launch_derived_em(const State<T>& S, int which, const KParams& p, double* out, hipStream_t st);
 // fused x + z sweep of both species: state S -> fluid of O (variants: see
 // launch_fct_xz in khs_step_kernels.hip)
-constexpr int kNumXZVariants = 13;
+constexpr int kNumXZVariants = 15;
+// variants from this one on take no warm terms (warm_T = 0 only)
+constexpr int kFirstColdXZVariant = 13;
 // EMHD fused x+z sweep of species 0..nsp-1: variants 0/1/2/3 (16/32/64/8-row chunks), 4/5/6
 // (two wavefronts per species, 8/16/32-row chunks)
 constexpr int kNumXZEmVariants = 7;

@@ -2812,7 +2812,12 @@ static void launch_fct_xz2_t(const State<T>& S, const State<T>& O, const KParams
 // 4/5/6: the x sweep in registers (DPP) inside the z march, 32/64/128-row
 // chunks; 7/8/9: the same with the variables on two waves, 16/32/64-row
 // chunks (two column tiles per workgroup), 10 / 11 / 12: 32 / 8 / 4-row chunks,
-// one tile per workgroup (short chunks: lower latency on small grids).
+// one tile per workgroup (short chunks: lower latency on small grids); 13 /
+// 14: the sweep waves of the Maxwell + next-sweep launch, fed by four waves
+// that load the rows into the LDS ring instead of computing Maxwell, chunks
+// sized for one / two rounds of resident workgroups (no warm terms).
+template <typename T>
+void launch_sweep_ring(const State<T>& S, const State<T>& O, const KParams& p, int tz, hipStream_t st);
 template <typename T>
 void launch_fct_xz(const State<T>& S, const State<T>& O, const KParams& p, int variant, hipStream_t st) {
     switch (variant) {
@@ -2828,6 +2833,8 @@ void launch_fct_xz(const State<T>& S, const State<T>& O, const KParams& p, int v
         case 10: launch_fct_xzg_t<T, 32, 128>(S, O, p, st); return;
         case 11: launch_fct_xzg_t<T, 8, 128>(S, O, p, st); return;
         case 12: launch_fct_xzg_t<T, 4, 128>(S, O, p, st); return;
+        case 13: launch_sweep_ring<T>(S, O, p, -1, st); return;
+        case 14: launch_sweep_ring<T>(S, O, p, -2, st); return;
         default: break;
     }
     constexpr int TZ = 32;
@@ -3232,6 +3239,62 @@ __device__ __forceinline__ void mxz_maxwell(const State<T>& I, const State<T>& O
     }
 }
 
+// Waves 4 .. 4 + M - 1 of the sweep-only launch: ring row k (grid row
+// rf + k) from the state's own fields and fluid, no Maxwell; the sweep's
+// velocities p / max(n, 0.05) and interface coefficients with it.
+template <typename T, int M>
+__device__ __forceinline__ void mxz_rows(const State<T>& I, const KParams& p, int g, int ch, int tz, int lane, int m,
+                                         MxzShared<T>& sh) {
+    const int nx = p.nx, nz = p.nz;
+    const int x = g * XZR_OUT - 3 + lane;
+    const int xc = map_x(x, p);
+    const int xm = xm_src(xc, p);
+    const int z0 = ch * tz;
+    const int zend = min(z0 + tz, nz);
+    const int elast = zend + 2;
+    const int rc0 = clampi(z0 - 3, 0, nz - 1);
+    const int rf = zm_per(rc0, nz);
+    auto rbase = [&](int r) { return (unsigned)((r + nz) % nz) * (unsigned)nx; };
+    const int nrows = 2 + clampi(elast, 0, nz - 1) - rc0;
+#pragma unroll 1
+    for (int k = m; k < nrows; k += M) {
+        const int q = rf + k;
+        MRow<T> a;
+        mrow_load(I, I, rbase(q) + (unsigned)xc, rbase(q) + (unsigned)xm, a);
+        if (k >= MXZ_SLOTS) mxz_wait_released(sh.released, k - MXZ_SLOTS + 1);
+        const int slot = k % MXZ_SLOTS;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            sh.ring[slot][S_EX1 + c][lane] = a.ex[c];
+            sh.ring[slot][S_EZ1 + c][lane] = a.ez[c];
+            sh.ring[slot][S_BY1 + c][lane] = a.by[c];
+            sh.ring[slot][S_AZ1 + c][lane] = a.az[c];
+        }
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            sh.fl[slot][s * F_COUNT + F_N][lane] = a.n[s];
+            sh.fl[slot][s * F_COUNT + F_PX][lane] = a.px[s];
+            sh.fl[slot][s * F_COUNT + F_PZ][lane] = a.pz[s];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) sh.fl[slot][s * F_COUNT + F_Q1 + c][lane] = a.q[s][c];
+            const T nsafe = kmax(a.n[s], T(0.05));
+            const T u = a.px[s] / nsafe;
+            const T uz = a.pz[s] / nsafe;
+            const T ur = lane_from_right(u);
+            const T vi = T(0.5) * (u + ur);
+            const T eps = vi * T(p.dtdx);
+            T nu, mu;
+            fct_nu_mu(eps, nu, mu);
+            sh.vel[slot][s][0][lane] = u;
+            sh.vel[slot][s][1][lane] = uz;
+            sh.vel[slot][s][2][lane] = eps;
+            sh.vel[slot][s][3][lane] = nu;
+            sh.vel[slot][s][4][lane] = mu;
+        }
+        mxz_publish(&sh.ready[slot], k + 1);
+    }
+}
+
 // Waves 0..3: xz_march_g on the fluid of I with the new field rows from the
 // ring.  Lane l reads the fields of its column at ring lane lx and those of
 // the source stencil's x-1 column at lxm: l and l-1, except for lanes beyond
@@ -3431,7 +3494,7 @@ __device__ __forceinline__ void mxz_sweep(const State<T>& I, const State<T>& O, 
 // tile = 58 columns x a chunk of tz rows.  Occupancy: two waves per SIMD in
 // FP64, four in FP32 (three with M = 2: two workgroups' LDS per CU); 4 + M
 // waves per workgroup (M = 4: the eight wave slots of a CU in FP64).
-template <typename T, int M>
+template <typename T, int M, bool MX = true>
 __global__ __launch_bounds__(64 * (4 + M)) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? (M == 2 ? 3 : 4) : 2))) void k_mxz(
     State<T> I, State<T> O, KParams p, int ngx, int nch, int tz, const double* decay, const int* action) {
     __shared__ MxzShared<T> sh;
@@ -3445,7 +3508,8 @@ __global__ __launch_bounds__(64 * (4 + M)) __attribute__((amdgpu_waves_per_eu(si
     if (threadIdx.x < 4) sh.released[threadIdx.x] = 0;
     __syncthreads();                            // the only barrier: the counters are set
     if (w >= 4) {
-        mxz_maxwell<T, M>(I, O, p, g, ch, tz, lane, w - 4, sh, decay, action);
+        if constexpr (MX) mxz_maxwell<T, M>(I, O, p, g, ch, tz, lane, w - 4, sh, decay, action);
+        else mxz_rows<T, M>(I, p, g, ch, tz, lane, w - 4, sh);
         return;
     }
     const int sp = w >> 1;
@@ -3462,7 +3526,7 @@ __global__ __launch_bounds__(64 * (4 + M)) __attribute__((amdgpu_waves_per_eu(si
 // Workgroups of k_mxz<T, M> the current device holds at once (its compute
 // units times the workgroups per unit that the kernel's registers and LDS
 // allow), queried once per device.
-template <typename T, int M>
+template <typename T, int M, bool MX = true>
 static int mxz_resident() {
     static std::atomic<int> cache[16];
     int dev = 0;
@@ -3470,7 +3534,7 @@ static int mxz_resident() {
     int r = cache[dev].load(std::memory_order_relaxed);
     if (r == 0) {
         int per_cu = 0, cus = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mxz<T, M>, 64 * (4 + M), 0) != hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mxz<T, M, MX>, 64 * (4 + M), 0) != hipSuccess ||
             per_cu < 1)
             per_cu = 1;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;
@@ -3484,17 +3548,17 @@ static int mxz_resident() {
 // fill the device's resident workgroups R times (at least 16 rows), so that
 // the launch runs in R rounds with long chunks instead of many rounds of
 // short ones and a partial last round.
-template <typename T, int M>
+template <typename T, int M, bool MX = true>
 static void launch_mxz_t(const State<T>& I, const State<T>& O, const KParams& p, int tz, const double* decay,
                          const int* action, hipStream_t st) {
     const int ngx = (p.nx + XZR_OUT - 1) / XZR_OUT;
     if (tz <= 0) {
-        const int nch = std::max(1, (-tz * mxz_resident<T, M>()) / ngx);
+        const int nch = std::max(1, (-tz * mxz_resident<T, M, MX>()) / ngx);
         tz = std::max(16, (p.nz + nch - 1) / nch);
     }
     const int nch = (p.nz + tz - 1) / tz;
-    hipLaunchKernelGGL((k_mxz<T, M>), dim3(round8(ngx * nch)), dim3(64 * (4 + M)), 0, st, I, O, p, ngx, nch, tz,
-                       decay, action);
+    hipLaunchKernelGGL((k_mxz<T, M, MX>), dim3(round8(ngx * nch)), dim3(64 * (4 + M)), 0, st, I, O, p, ngx, nch,
+                       tz, decay, action);
 }
 
 // variants 0/1/2: 16/32/64-row chunks with two Maxwell waves, 3/4/5 with
@@ -3514,6 +3578,12 @@ void launch_maxwell_sweep(const State<T>& I, const State<T>& O, const KParams& p
         case 8: launch_mxz_t<T, 2>(I, O, p, -1, decay, action, st); return;
         default: launch_mxz_t<T, 2>(I, O, p, 32, decay, action, st); return;
     }
+}
+
+// The fused sweeps alone (xz variants 13 / 14): state S -> fluid of O.
+template <typename T>
+void launch_sweep_ring(const State<T>& S, const State<T>& O, const KParams& p, int tz, hipStream_t st) {
+    launch_mxz_t<T, 4, false>(S, O, p, tz, nullptr, nullptr, st);
 }
 
 template void launch_maxwell_sweep<double>(const State<double>&, const State<double>&, const KParams&, const double*,

@@ -88,7 +88,8 @@ def test_output_dir_name_c1():
     (dict(x_lane_cells=6), "x_lane_cells"),
     (dict(z_groups=2), "z_groups"),
     (dict(fused_sweep=2), "fused_sweep"),
-    (dict(xz_variant=13), "xz_variant"),
+    (dict(xz_variant=15), "xz_variant"),
+    (dict(xz_variant=13, warm_T=0.01), "xz_variant"),
     (dict(step_fusion=2), "step_fusion"),
     (dict(mxz_variant=9), "mxz_variant"),
     (dict(packed_shape=8), "packed_shape"),

@@ -390,7 +390,7 @@ def test_maxwell_march_bitwise(mk, shape, periodic, freeze):
         assert_bitwise(sim.get(n), o.get(n), f"maxwell {mk} {shape} {n}")
 
 
-@pytest.mark.parametrize("xzv", list(range(13)))
+@pytest.mark.parametrize("xzv", list(range(15)))
 @pytest.mark.parametrize("shape,periodic,warm", [((300, 136), 1, 0.01), ((70, 36), 0, 0.0), ((256, 64), 0, 0.0),
                                                  ((1000, 40), 0, 0.01), ((116, 300), 1, 0.0)])
 def test_fused_sweep_variants_bitwise(xzv, shape, periodic, warm):
@@ -399,6 +399,8 @@ def test_fused_sweep_variants_bitwise(xzv, shape, periodic, warm):
     columns a multiple of the 58 a wave stores, several z chunks) gives the
     oracle's bits."""
     nx, nz = shape
+    if xzv >= 13 and warm > 0:
+        pytest.skip("the ring sweep (13, 14) takes no warm terms")
     keys = grid(nx, nz, periodic, warm_T=warm, fused_sweep=1, xz_variant=xzv, autotune=0)
     cfg, sim, o = make_pair(random_state(nx, nz, seed=nx + xzv, smooth=True), **keys)
     assert sim.tuning()["xz_variant"] == xzv
