@@ -111,10 +111,12 @@ LIMITER = {
     ("k_maxwell", "fp64", 2048, 1024): "HBM: 620 MB per launch (1.06x algorithmic) at 5.0 TB/s, ~21,500 reads "
                                        "in flight (profiles/latency_icache_2048x1024_fp64.md)",
     ("k_fct_xz", "fp32", 4096, 2048):
-        "the fused sweep's memory-level parallelism and FP32 arithmetic at 3 waves per SIMD: the same instruction "
-        "count as the FP64 form on half the bytes",
+        "the ring sweep's FP32 arithmetic: the same instruction count as the FP64 form on half the bytes, its "
+        "sweep wavefronts' row march issue-bound as in FP64",
     ("k_fct_xz", "fp64", 4096, 2048):
-        "memory-level parallelism at 2 waves per SIMD, as at 2048x1024 (profiles/latency_icache_2048x1024_fp64.md)",
+        "issue latency of the ring sweep's sweep wavefronts (k_mxz<double, 4, false>: four row-loading wavefronts "
+        "fill the LDS ring, so the sweep wavefronts wait on memory for 0.14 of their lifetime against 0.48 in "
+        "k_fct_xzg; VALU busy 0.48 at 2048x1024, profiles/sq_counters_2048x1024_fp64.md)",
     ("k_fct_xz", "fp64", 256, 128):
         "latency: 640 waves of 10 rows each on a 256x128 grid (under a third of the wave slots at 2 per SIMD), each row "
         "waiting on the loads issued one row earlier",
@@ -145,6 +147,8 @@ def kernel_instance(slot, t, T="double"):
                     4: ("k_fct_xzr", 32), 5: ("k_fct_xzr", 64), 6: ("k_fct_xzr", 128), 7: ("k_fct_xzg", 16),
                     8: ("k_fct_xzg", 32), 9: ("k_fct_xzg", 64), 10: ("k_fct_xzg", 32),
                     11: ("k_fct_xzg", 8), 12: ("k_fct_xzg", 4)}.get(v, (None, 0))
+        if v in (13, 14):
+            return f"k_mxz<{T}, 4, false> ring sweep, {('one', 'two')[v - 13]} round{'s' * (v - 13)}"
         if name is None:
             return None
         if name == "k_fct_xzg":
