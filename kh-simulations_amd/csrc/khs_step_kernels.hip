@@ -2976,7 +2976,8 @@ struct MxzShared {
     // x-interface coefficients eps, nu, mu of the ring row's fluid
     T vel[MXZ_SLOTS][2][5][64];
     // the ring row's fluid (n, px, pz, Q1..Q3 of both species, before the
-    // charge filter), the sweep waves' input, loaded once by the Maxwell wave
+    // charge filter; in the sweep-only launch n and pz already swept along
+    // x), the sweep waves' input, loaded once by the wave that fills the row
     T fl[kMxzFluidRing<T> ? MXZ_SLOTS : 1][2 * F_COUNT][64];
     int ready[MXZ_SLOTS];
     int released[4];
@@ -3015,6 +3016,34 @@ __device__ __forceinline__ void mxz_wait_released(const int (&r)[4], int v) {
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
+
+// The x sweep of one variable of a row at this lane's column (YM_FCT.cu:30-91):
+// v0 its value, sk its source, eps / nu / mu the interface coefficients at
+// x + 1/2; every x neighbour from lanes l +- 1, +- 2.
+template <typename T>
+__device__ __forceinline__ T mxz_xsweep(T v0, T sk, T eps, T nu, T mu, T dt) {
+    const T vp = lane_from_right(v0);
+    T fR = T(0.5) * eps * (v0 + vp);
+    T dR = nu * (vp - v0);
+    const T fL = lane_from_left(fR);
+    const T dL = lane_from_left(dR);
+    T lor = v0 + dt * sk - (fR - fL);
+    T lnr = lor + (dR - dL);
+    const T lorr = lane_from_right(lor);
+    const T lnrl = lane_from_left(lnr);
+    const T lnrr = lane_from_right(lnr);
+    const T lnrr2 = lane_from_right(lnrr);
+    T flx = mu * (lorr - lor);
+    T fl = fct_limit(flx, lnr - lnrl, lnrr - lnr, lnrr2 - lnrr);
+    const T fll = lane_from_left(fl);
+    return lnr - (fl - fll);
+}
+// Variables whose x sweep the row-loading waves of the sweep-only launch do
+// (n and pz of both species: no source, no filter), so that the sweep waves,
+// which set the launch's pace, only read the result (2048x1024 FP64: 161-166
+// -> 159-161 us; done by the Maxwell waves of the Maxwell + sweep launch
+// instead, it makes that launch slower: 223-225 -> 232 us)
+__device__ __forceinline__ constexpr bool mxz_xpre(int v) { return v == F_N || v == F_PZ; }
 
 // Waves 4 .. 4 + M - 1: Maxwell as row tasks.  Ring row k is grid row
 // r = rf + k (periodic, rf = zm_per(rc0)); Maxwell wave m finishes rows k = m,
@@ -3290,6 +3319,8 @@ __device__ __forceinline__ void mxz_rows(const State<T>& I, const KParams& p, in
             sh.vel[slot][s][2][lane] = eps;
             sh.vel[slot][s][3][lane] = nu;
             sh.vel[slot][s][4][lane] = mu;
+            sh.fl[slot][s * F_COUNT + F_N][lane] = mxz_xsweep(a.n[s], T(0.0), eps, nu, mu, T(p.dt));
+            sh.fl[slot][s * F_COUNT + F_PZ][lane] = mxz_xsweep(a.pz[s], T(0.0), eps, nu, mu, T(p.dt));
         }
         mxz_publish(&sh.ready[slot], k + 1);
     }
@@ -3301,7 +3332,7 @@ __device__ __forceinline__ void mxz_rows(const State<T>& I, const KParams& p, in
 // a wall, which take their clamped column's lane (so their sources are
 // lorentz_px / lorentz_pz / precession at that column, as the reference's
 // clamped halo loads give).
-template <typename T, typename GR, bool EDGE>
+template <typename T, typename GR, bool EDGE, bool XP>
 __device__ __forceinline__ void mxz_sweep(const State<T>& I, const State<T>& O, const KParams& p, int sp, int g,
                                           int ch, int tz, int lane, MxzShared<T>& sh, int w, const double* decay,
                                           const int* action) {
@@ -3397,23 +3428,9 @@ __device__ __forceinline__ void mxz_sweep(const State<T>& I, const State<T>& O, 
 #pragma unroll
             for (int j = 0; j < NG; ++j) {
                 const int k = VB + j;
-                const T v0 = F[k];
-                const T vp = lane_from_right(v0);
-                T fR = T(0.5) * eps * (v0 + vp);
-                T dR = nu * (vp - v0);
-                const T fL = lane_from_left(fR);
-                const T dL = lane_from_left(dR);
-                T sk = (k == F_PX) ? src : T(0.0);
-                T lor = v0 + dt * sk - (fR - fL);
-                T lnr = lor + (dR - dL);
-                const T lorr = lane_from_right(lor);
-                const T lnrl = lane_from_left(lnr);
-                const T lnrr = lane_from_right(lnr);
-                const T lnrr2 = lane_from_right(lnrr);
-                T flx = mu * (lorr - lor);
-                T fl = fct_limit(flx, lnr - lnrl, lnrr - lnr, lnrr2 - lnrr);
-                const T fll = lane_from_left(fl);
-                V[j] = lnr - (fl - fll);
+                // n, pz: swept by the wave that filled the row (the ring holds the result)
+                if (XP && mxz_xpre(k)) V[j] = F[k];
+                else V[j] = mxz_xsweep(F[k], (k == F_PX) ? src : T(0.0), eps, nu, mu, dt);
             }
         }
 
@@ -3515,11 +3532,11 @@ __global__ __launch_bounds__(64 * (4 + M)) __attribute__((amdgpu_waves_per_eu(si
     const int sp = w >> 1;
     const bool edge = g == 0 || g * XZR_OUT + 60 >= p.nx;   // lanes cover columns 58g-3 .. 58g+60
     if (w & 1) {
-        if (edge) mxz_sweep<T, XzQ3, true>(I, O, p, sp, g, ch, tz, lane, sh, w, decay, action);
-        else mxz_sweep<T, XzQ3, false>(I, O, p, sp, g, ch, tz, lane, sh, w, decay, action);
+        if (edge) mxz_sweep<T, XzQ3, true, !MX>(I, O, p, sp, g, ch, tz, lane, sh, w, decay, action);
+        else mxz_sweep<T, XzQ3, false, !MX>(I, O, p, sp, g, ch, tz, lane, sh, w, decay, action);
     } else {
-        if (edge) mxz_sweep<T, XzN3, true>(I, O, p, sp, g, ch, tz, lane, sh, w, decay, action);
-        else mxz_sweep<T, XzN3, false>(I, O, p, sp, g, ch, tz, lane, sh, w, decay, action);
+        if (edge) mxz_sweep<T, XzN3, true, !MX>(I, O, p, sp, g, ch, tz, lane, sh, w, decay, action);
+        else mxz_sweep<T, XzN3, false, !MX>(I, O, p, sp, g, ch, tz, lane, sh, w, decay, action);
     }
 }
 
