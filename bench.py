@@ -99,9 +99,9 @@ ZM_ROWS = {1: 16, 2: 32, 3: 64}       # chunk height of the fused z sweep + Maxw
 LIMITER = {
     ("k_mxz", "fp64", 2048, 1024):
         "issue and dependency latency of the sweep wavefronts' row march: two FP64 waves per SIMD (one sweep, one "
-        "Maxwell wave, 246 VGPRs), SIMD VALU busy 0.46 of the launch, each wave waiting on memory and LDS counters "
-        "for 0.35 of its lifetime (profiles/sq_counters_2048x1024_fp64.md); HBM traffic 1.09x algorithmic at "
-        "~3.8 TB/s, under the ~5.3 TB/s of a plain device copy",
+        "Maxwell wave, 252 VGPRs), SIMD VALU busy 0.50 of the launch, each wave waiting on memory and LDS counters "
+        "for 0.34 of its lifetime (profiles/sq_counters_2048x1024_fp64.md); HBM traffic 1.07x algorithmic at "
+        "~4.1 TB/s, under the ~5.3 TB/s of a plain device copy",
     ("k_fct_xz", "fp64", 2048, 1024):
         "memory-level parallelism: ~13,700 HBM reads (~1.5 MB) in flight at a 0.63 us read latency, a third of "
         "what 8 TB/s needs (profiles/latency_icache_2048x1024_fp64.md), because each wave holds one row of loads "
